@@ -1,0 +1,213 @@
+/*
+ * spm.h -- C ABI of the MI355X Smoothed Profile Method (SPM) time step.
+ *
+ * The library replaces KAPSEL's CPU time step `Time_evolution_hydro`
+ * (sp_3d_ns.cxx:61-212) and the solver calls it is built from:
+ *   NS_solver_slavedEuler          fluid_solver.cxx:328 (decl. fluid_solver.h:80)
+ *   Make_phi_u_particle            make_phi.cxx:401     (decl. make_phi.h:70)
+ *   Calc_f_hydro_correct_precision md_force.cxx:263     (decl. md_force.h:60)
+ *   Force                          particle_solver.h:97
+ * The reference keeps all of this in globals (input.cxx:13-271); here the
+ * state lives in an opaque context that owns device memory on one GPU.
+ *
+ * Layouts at the boundary are the reference's:
+ *   - every field is NX*NY*NZ_ doubles, NZ_ = 2*(NZ/2+1), index
+ *     (i*NY + j)*NZ_ + k (e.g. operate_omega.cxx:37);
+ *   - a real-space field uses k < NZ (the two padding slots are ignored);
+ *   - a spectrum holds NZ/2+1 interleaved complex values per (i,j) row,
+ *     X(k) = sum_x a(x) exp(+i k.x) (Ooura sign, fftsg3d.c:109-120), and
+ *     the inverse scales by 1/(NX*NY*NZ) overall (fft_wrapper.h:552);
+ *   - particles are the reference `Particle` struct (variable.h:27-75),
+ *     1000 bytes each, mirrored as `spm_particle` below.
+ *
+ * Every entry point returns an int status (SPM_OK = 0) and never exits the
+ * process (the reference calls exit(1), macro.h:194-196).  Calls on one
+ * context are ordered on the context's HIP stream; results copied out by
+ * spm_get_* are complete when the call returns.  A context is not
+ * thread-safe; separate contexts are independent.
+ */
+#ifndef SPM_H
+#define SPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPM_DIM 3
+#define SPM_MAX_SPECIES 8
+#define SPM_MT_N 624
+
+/* ---- status codes ---------------------------------------------------- */
+enum spm_status {
+  SPM_OK = 0,
+  SPM_ERR_INVALID = 1,       /* bad argument or parameter                */
+  SPM_ERR_OOM = 2,           /* device or host allocation failed         */
+  SPM_ERR_HIP = 3,           /* a HIP runtime call failed                */
+  SPM_ERR_NONFINITE = 4,     /* NaN/Inf found by spm_check_finite        */
+  SPM_ERR_UNSUPPORTED = 5,   /* option outside what this build implements*/
+  SPM_ERR_NO_DEVICE = 6      /* no usable GPU                            */
+};
+
+/* ---- field identifiers for spm_get_field / spm_put_field -------------- */
+enum spm_field {
+  SPM_FIELD_ZETA0 = 0, SPM_FIELD_ZETA1 = 1,        /* reduced vorticity (k-space) */
+  SPM_FIELD_U0 = 2, SPM_FIELD_U1 = 3, SPM_FIELD_U2 = 4,    /* fluid velocity    */
+  SPM_FIELD_UP0 = 5, SPM_FIELD_UP1 = 6, SPM_FIELD_UP2 = 7, /* phi*(v+w x r)     */
+  SPM_FIELD_PHI = 8,                                       /* smooth profile    */
+  SPM_FIELD_F0 = 9, SPM_FIELD_F1 = 10, SPM_FIELD_F2 = 11,  /* SPM body force    */
+  SPM_NUM_FIELDS = 12
+};
+
+/* ---- LJ switches (input.cxx:1451-1480) -------------------------------- */
+enum spm_lj_truncate { SPM_LJ_NONE = -1, SPM_LJ_FULL = 0, SPM_LJ_WCA = 1 };
+enum spm_lj_powers { SPM_LJ_12_6 = 0, SPM_LJ_24_12 = 1, SPM_LJ_36_18 = 2 };
+
+/* ---- arithmetic of the field path ------------------------------------- */
+enum spm_precision { SPM_FP64 = 0, SPM_FP32 = 1 };
+
+/*
+ * Run parameters: the subset of the reference's UDF input
+ * (Udf/define.udf:14-225, read in input.cxx:438-1813) that the
+ * Navier_Stokes + spherical_particle time step uses.  Derived constants
+ * follow Set_global_parameters (input.cxx:274-433).
+ */
+typedef struct spm_params {
+  int nx, ny, nz;            /* grid; each a power of two, 8..1024       */
+  double dx;                 /* DX                                        */
+  double rho, eta;           /* RHO, ETA (NU = ETA/RHO)                   */
+  double kbt;                /* kBT; thermostat runs when > 0             */
+  double alpha_v, alpha_o;   /* thermostat factors (input.cxx:518-519)    */
+  double dt;                 /* DT (time_increment manual)                */
+  double a, a_xi;            /* radius and interface width, in units of DX */
+  int n_particles;           /* Particle_Number                           */
+  int n_species;             /* Component_Number, 1..SPM_MAX_SPECIES      */
+  double mass_ratio[SPM_MAX_SPECIES];  /* MASS_RATIOS                     */
+  int rotation;              /* ROTATION switch                           */
+  int lj_truncate;           /* enum spm_lj_truncate                      */
+  int lj_powers;             /* enum spm_lj_powers                        */
+  double epsilon;            /* EPSILON                                   */
+  double g;                  /* gravity G                                 */
+  int g_direction;           /* 0,1,2 for -X,-Y,-Z                        */
+  int fix_cell[SPM_DIM];     /* FIX_CELLxyz                               */
+  unsigned long seed;        /* MT19937 seed (reference: 0, sp_3d_ns.cxx:477) */
+  int precision;             /* enum spm_precision                        */
+  int device;                /* HIP device ordinal                        */
+} spm_params;
+
+/* The reference quaternion (quaternion.h:22-25). */
+typedef struct spm_quaternion {
+  double s;
+  double v[SPM_DIM];
+} spm_quaternion;
+
+/* Byte-for-byte mirror of the reference `Particle` (variable.h:27-75). */
+typedef struct spm_particle {
+  int spec;
+  double x[SPM_DIM];
+  double x_previous[SPM_DIM];
+  double x_nopbc[SPM_DIM];
+  double v[SPM_DIM];
+  double v_old[SPM_DIM];
+  double v_slip[SPM_DIM];
+  double f_hydro[SPM_DIM];
+  double f_hydro_previous[SPM_DIM];
+  double f_hydro1[SPM_DIM];
+  double f_slip[SPM_DIM];
+  double f_slip_previous[SPM_DIM];
+  double fr[SPM_DIM];
+  double fr_previous[SPM_DIM];
+  double omega[SPM_DIM];
+  double omega_old[SPM_DIM];
+  double omega_slip[SPM_DIM];
+  double torque_hydro[SPM_DIM];
+  double torque_hydro_previous[SPM_DIM];
+  double torque_hydro1[SPM_DIM];
+  double torque_slip[SPM_DIM];
+  double torque_slip_previous[SPM_DIM];
+  double momentum_depend_fr[SPM_DIM];
+  double QR[SPM_DIM][SPM_DIM];
+  double QR_old[SPM_DIM][SPM_DIM];
+  spm_quaternion q;
+  spm_quaternion q_old;
+  double mass;
+  double mass_center[SPM_DIM];
+  double inertia[SPM_DIM][SPM_DIM];
+  double surface_mass;
+  double surface_mass_center[SPM_DIM];
+  double surface_inertia[SPM_DIM][SPM_DIM];
+  double surface_dv[SPM_DIM];
+  double surface_dw[SPM_DIM];
+} spm_particle;
+
+typedef struct spm_ctx spm_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Allocates the context and its device buffers; fields start at zero,
+ * particles at rest at the origin, ts = 0, the RNG seeded with prm->seed
+ * (MT_seed, fluct.h:34-46). Replaces Gourmet_file_io + Init_fft +
+ * Mem_alloc_var (input.cxx:438, fft_wrapper.cxx:92, sp_3d_ns.cxx:392). */
+int spm_create(const spm_params *prm, spm_ctx **out);
+int spm_destroy(spm_ctx *ctx);
+const char *spm_strerror(int status);
+/* Text of the last error on this context ("" if none). */
+const char *spm_last_error(const spm_ctx *ctx);
+/* Number of points in the integration stencil (NP_domain, profile.cxx:25-74). */
+int spm_stencil_size(const spm_ctx *ctx);
+
+/* ---- state transfer (host <-> device, reference layouts) --------------- */
+int spm_put_particles(spm_ctx *ctx, const spm_particle *p, int n);
+int spm_get_particles(spm_ctx *ctx, spm_particle *p, int n);
+int spm_put_field(spm_ctx *ctx, int field, const double *host);
+int spm_get_field(spm_ctx *ctx, int field, double *host);
+int spm_put_uk_dc(spm_ctx *ctx, const double uk_dc[SPM_DIM]);
+int spm_get_uk_dc(spm_ctx *ctx, double uk_dc[SPM_DIM]);
+/* CTime jikan.ts / jikan.time (variable.h:18-25). */
+int spm_put_time(spm_ctx *ctx, int ts, double time);
+int spm_get_time(spm_ctx *ctx, int *ts, double *time);
+/* MT19937 state (mt19937ar.c:60-61): 624 words and the index mti. */
+int spm_put_rng(spm_ctx *ctx, const uint32_t mt[SPM_MT_N], int mti);
+int spm_get_rng(spm_ctx *ctx, uint32_t mt[SPM_MT_N], int *mti);
+
+/* ---- the solver call surface ------------------------------------------ */
+/* Init_zeta_k (init_fluid.cxx:10) followed by the initial projection
+ * block of main (sp_3d_ns.cxx:508-524). */
+int spm_init_fluid(spm_ctx *ctx);
+/* NS_solver_slavedEuler (fluid_solver.cxx:328-354) over the four 2/3-rule
+ * blocks of Time_evolution_noparticle (sp_3d_ns.cxx:15-20). */
+int spm_ns_solver_slaved_euler(spm_ctx *ctx);
+/* FIX_CELL, truncation and Zeta_k2u of Time_evolution_hydro
+ * (sp_3d_ns.cxx:67-79): zeta -> real-space u. */
+int spm_zeta_k2u(spm_ctx *ctx);
+/* MD_solver_position_Euler at ts == 0 else MD_solver_position_AB2
+ * (particle_solver.cxx:36-88; sp_3d_ns.cxx:81-87). */
+int spm_md_solver_position(spm_ctx *ctx);
+/* Reset_phi_u + Calc_f_hydro_correct_precision(p, u) (md_force.cxx:263-389). */
+int spm_calc_f_hydro(spm_ctx *ctx);
+/* Force(p) (particle_solver.h:97-109): LJ/WCA + gravity, added to fr. */
+int spm_force(spm_ctx *ctx);
+/* MD_solver_velocity_Euler at ts == 0 else MD_solver_velocity_AB2_hydro
+ * (particle_solver.cxx:106-147, 247-289); calls Force first. */
+int spm_md_solver_velocity(spm_ctx *ctx);
+/* Add_random_force_thermostat (fluct.cxx:10-86); no-op when kbt <= 0. */
+int spm_add_random_force_thermostat(spm_ctx *ctx);
+/* Reset_phi_u + Make_phi_u_particle (make_phi.cxx:401-409). */
+int spm_make_phi_u_particle(spm_ctx *ctx);
+/* Make_f_particle_dt_sole + Add_f_particle (f_particle.h:71-99). */
+int spm_make_f_particle_dt_sole(spm_ctx *ctx);
+/* U2zeta_k (operate_omega.h:186-189): real-space u -> zeta, uk_dc. */
+int spm_u2zeta_k(spm_ctx *ctx);
+/* nsteps x { Time_evolution_hydro (sp_3d_ns.cxx:61-212);
+ *            jikan.time += dt; jikan.ts++ }   (sp_3d_ns.cxx:593-594). */
+int spm_step(spm_ctx *ctx, int nsteps);
+/* Waits for all work queued on the context. */
+int spm_synchronize(spm_ctx *ctx);
+/* Scans u and zeta for NaN/Inf; SPM_ERR_NONFINITE if any is found. */
+int spm_check_finite(spm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPM_H */

@@ -1,0 +1,1176 @@
+/*
+ * spm_oracle.c -- CPU FP64 restatement of KAPSEL's SPM time step.
+ *
+ * TEST INFRASTRUCTURE ONLY (parity checker and CPU baseline); see
+ * spm_oracle.h for what may use it and how each function is pinned.
+ * Single-threaded, like the reference's shipped GCC build (Makefile:43-48).
+ *
+ * Deviations from the reference, all without effect on the arithmetic of a
+ * run with fixed parameters:
+ *   - state is held in a context instead of globals and function-local
+ *     statics (input.cxx:13-271; fft_wrapper.h:552; md_force.cxx:264;
+ *     fluct.cxx:11-18), so several configurations can live in one process;
+ *   - the 3-D FFT is this file's own radix-2 code (not Ooura's); it
+ *     computes the same transform (pinned against oracle/_ref);
+ *   - the padding slots k = NZ, NZ+1 of a real-space field are set to 0
+ *     where the reference leaves leftovers (fft_wrapper.h:534-631).
+ */
+#include "spm_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define DIM 3
+#define MT_N 624
+#define MT_M 397
+
+typedef struct {
+  int n;
+  double *c, *s;  /* cos, sin of 2*pi*t/n, t < n */
+  int *rev;       /* bit reversal of t < n */
+} fft_plan;
+
+struct oracle_ctx {
+  spm_params P;
+  int NX, NY, NZ, NZ_, HNZ_, TRN_X, TRN_Y, TRN_Z;
+  size_t nfield;                /* NX*NY*NZ_ */
+  double DX, RHO, ETA, NU, IRHO, DX3, kBT, RADIUS, XI, HXI, SIGMA;
+  double L[DIM], WAVE_X, WAVE_Y, WAVE_Z;
+  double MASS[SPM_MAX_SPECIES], IMASS[SPM_MAX_SPECIES];
+  double MOI[SPM_MAX_SPECIES], IMOI[SPM_MAX_SPECIES];
+  double LJ_dia, A_R_cutoff;
+  int *KX_int, *KY_int, *KZ_int;
+  double *K2, *IK2;
+  double *zeta[2], *u[3], *up[3], *phi, *f[3], *f_ns0[2], *adv_tmp[2];
+  double uk_dc[3];
+  int np_domain;
+  int *sekibun;                 /* np_domain x 3 */
+  spm_particle *p;
+  int Np;
+  int ts;
+  double time, dt, hdt;
+  uint32_t mt[MT_N];
+  int mti;
+  fft_plan px, py, pz;          /* pz has n = NZ/2 */
+  double *rowbuf;
+  double *ztw;                  /* cos, sin of 2*pi*k/NZ, k <= NZ/2 */
+};
+
+/* ------------------------------------------------------------------ */
+/* small helpers (macro.h:38-45, 129-150)                              */
+/* ------------------------------------------------------------------ */
+static inline double SQ(double x) { return x * x; }
+static inline double POW3(double x) { return x * x * x; }
+static inline int Nint(double a) { return a >= 0.0 ? (int)(a + .5) : (int)(a - .5); }
+static inline double DMIN(double a, double b) { return a < b ? a : b; }
+
+static size_t IDX(const oracle_ctx *c, int i, int j, int k) {
+  return ((size_t)i * c->NY + j) * c->NZ_ + k;
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT: own radix-2 code computing Ooura's transform                  */
+/* X(k) = sum a(x) exp(+2 pi i k.x/N), half spectrum in [NX][NY][NZ_]  */
+/* (fftsg3d.c:109-120, fft_wrapper.h:432-631)                          */
+/* ------------------------------------------------------------------ */
+static int plan_init(fft_plan *pl, int n) {
+  pl->n = n;
+  pl->c = (double *)malloc(sizeof(double) * n);
+  pl->s = (double *)malloc(sizeof(double) * n);
+  pl->rev = (int *)malloc(sizeof(int) * n);
+  if (!pl->c || !pl->s || !pl->rev) return -1;
+  int lg = 0;
+  while ((1 << lg) < n) lg++;
+  for (int t = 0; t < n; t++) {
+    long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+    pl->c[t] = (double)cosl(ang);
+    pl->s[t] = (double)sinl(ang);
+    int r = 0;
+    for (int b = 0; b < lg; b++)
+      if (t & (1 << b)) r |= 1 << (lg - 1 - b);
+    pl->rev[t] = r;
+  }
+  return 0;
+}
+
+static void plan_free(fft_plan *pl) {
+  free(pl->c);
+  free(pl->s);
+  free(pl->rev);
+}
+
+/* In-place complex FFT of n points on `batch` interleaved columns:
+ * element j of column b is the complex number at a[2*(j*stride + b)].
+ * sign = +1 computes sum x_j exp(+2 pi i jk/n), -1 the conjugate kernel. */
+static void fft_batch(const fft_plan *pl, double *a, long stride, int batch, int sign) {
+  const int n = pl->n;
+  for (int j = 0; j < n; j++) {
+    int r = pl->rev[j];
+    if (r > j) {
+      double *x = a + 2 * (long)j * stride, *y = a + 2 * (long)r * stride;
+      for (int b = 0; b < 2 * batch; b++) {
+        double t = x[b];
+        x[b] = y[b];
+        y[b] = t;
+      }
+    }
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    const int half = len >> 1, step = n / len;
+    for (int k = 0; k < half; k++) {
+      const double wr = pl->c[k * step], wi = sign * pl->s[k * step];
+      for (int start = 0; start < n; start += len) {
+        double *x = a + 2 * (long)(start + k) * stride;
+        double *y = a + 2 * (long)(start + k + half) * stride;
+        for (int b = 0; b < batch; b++) {
+          double yr = y[2 * b] * wr - y[2 * b + 1] * wi;
+          double yi = y[2 * b] * wi + y[2 * b + 1] * wr;
+          double xr = x[2 * b], xi = x[2 * b + 1];
+          x[2 * b] = xr + yr;
+          x[2 * b + 1] = xi + yi;
+          y[2 * b] = xr - yr;
+          y[2 * b + 1] = xi - yi;
+        }
+      }
+    }
+  }
+}
+
+/* A2a_k, fft_wrapper.h:432-527: real field (k < NZ) -> half spectrum. */
+void oracle_A2a_k(oracle_ctx *c, double *a) {
+  const int NX = c->NX, NY = c->NY, NZ = c->NZ, M = NZ / 2, H = c->HNZ_;
+  double *z = c->rowbuf;
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++) {
+      double *row = a + IDX(c, i, j, 0);
+      memcpy(z, row, sizeof(double) * NZ);
+      fft_batch(&c->pz, z, 1, 1, +1);
+      /* X[k] = E[k] + w^k O[k], w = exp(+2 pi i/NZ) */
+      for (int k = 0; k <= M; k++) {
+        int k1 = k % M, k2 = (M - k) % M;
+        double zr = z[2 * k1], zi = z[2 * k1 + 1];
+        double cr = z[2 * k2], ci = -z[2 * k2 + 1];
+        double er = 0.5 * (zr + cr), ei = 0.5 * (zi + ci);
+        /* O = (Z - conj Z')/(2i) */
+        double dr = zr - cr, di = zi - ci;
+        double orr = 0.5 * di, oi = -0.5 * dr;
+        double wr = c->ztw[2 * k], wi = c->ztw[2 * k + 1];
+        row[2 * k] = er + (wr * orr - wi * oi);
+        row[2 * k + 1] = ei + (wr * oi + wi * orr);
+      }
+    }
+  for (int i = 0; i < NX; i++) fft_batch(&c->py, a + IDX(c, i, 0, 0), H, H, +1);
+  for (int j = 0; j < NY; j++) fft_batch(&c->px, a + IDX(c, 0, j, 0), (long)NY * H, H, +1);
+}
+
+/* A_k2a, fft_wrapper.h:534-631: half spectrum -> real field, scale 1/N. */
+void oracle_A_k2a(oracle_ctx *c, double *a) {
+  const int NX = c->NX, NY = c->NY, NZ = c->NZ, M = NZ / 2, H = c->HNZ_;
+  const double scale = 1.0 / ((double)NX * NY * NZ);
+  double *z = c->rowbuf;
+  for (int j = 0; j < NY; j++) fft_batch(&c->px, a + IDX(c, 0, j, 0), (long)NY * H, H, -1);
+  for (int i = 0; i < NX; i++) fft_batch(&c->py, a + IDX(c, i, 0, 0), H, H, -1);
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++) {
+      double *row = a + IDX(c, i, j, 0);
+      for (int k = 0; k < M; k++) {
+        double xr = row[2 * k], xi = row[2 * k + 1];
+        double cr = row[2 * (M - k)], ci = -row[2 * (M - k) + 1];
+        double er = xr + cr, ei = xi + ci;
+        double dr = xr - cr, di = xi - ci;
+        double wr = c->ztw[2 * k], wi = -c->ztw[2 * k + 1];
+        double orr = dr * wr - di * wi, oi = dr * wi + di * wr;
+        z[2 * k] = er - oi;      /* Z = E + i O */
+        z[2 * k + 1] = ei + orr;
+      }
+      fft_batch(&c->pz, z, 1, 1, -1);
+      for (int k = 0; k < NZ; k++) row[k] = z[k] * scale;
+      row[NZ] = 0.0;
+      row[NZ + 1] = 0.0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* k tables, Init_K fft_wrapper.cxx:29-62, Calc_K*_Ooura h:687-699      */
+/* ------------------------------------------------------------------ */
+static void init_k(oracle_ctx *c) {
+  const int NX = c->NX, NY = c->NY, NZ_ = c->NZ_;
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++)
+      for (int k = 0; k < NZ_; k++) {
+        size_t im = IDX(c, i, j, k);
+        int kx = (i > NX / 2) ? i - NX : i;
+        int ky = (j > NY / 2) ? j - NY : j;
+        int kz = k / 2;
+        c->KX_int[im] = kx;
+        c->KY_int[im] = ky;
+        c->KZ_int[im] = kz;
+        c->K2[im] = SQ(c->WAVE_X * (double)kx) + SQ(c->WAVE_Y * (double)ky) + SQ(c->WAVE_Z * (double)kz);
+        c->IK2[im] = c->K2[im] > 0.0 ? 1. / c->K2[im] : 0.0;
+      }
+}
+
+/* Truncate_two_third_rule_ooura, fft_wrapper.h:723-766: zero every mode
+ * outside |kx| < TRN_X, |ky| < TRN_Y, kz < TRN_Z (4 regions). */
+void oracle_truncate(oracle_ctx *c, double *a) {
+  const int NX = c->NX, NY = c->NY, NZ_ = c->NZ_;
+  const int TX = c->TRN_X, TY = c->TRN_Y, tz2 = 2 * c->TRN_Z;
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++)
+      for (int k = tz2; k < NZ_; k++) a[IDX(c, i, j, k)] = 0.0;
+  for (int i = 0; i < NX; i++)
+    for (int j = TY; j <= NY - TY; j++)
+      for (int k = 0; k < tz2; k++) a[IDX(c, i, j, k)] = 0.0;
+  for (int i = TX; i <= NX - TX; i++)
+    for (int j = 0; j <= TY - 1; j++)
+      for (int k = 0; k < tz2; k++) a[IDX(c, i, j, k)] = 0.0;
+  for (int i = TX; i <= NX - TX; i++)
+    for (int j = NY - TY + 1; j <= NY - 1; j++)
+      for (int k = 0; k < tz2; k++) a[IDX(c, i, j, k)] = 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* spectral operators                                                  */
+/* ------------------------------------------------------------------ */
+/* Zeta_k2u_k, fft_wrapper.cxx:408-511. */
+void oracle_zeta_k2u_k(oracle_ctx *c, double *const zeta[2], const double uk_dc[3], double *const u[3]) {
+  const int NX = c->NX, NY = c->NY, H = c->HNZ_;
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++)
+      for (int k = 0; k < H; k++) {
+        size_t im = IDX(c, i, j, 2 * k);
+        double kx = c->WAVE_X * c->KX_int[im];
+        double ky = c->WAVE_Y * c->KY_int[im];
+        double kz = c->WAVE_Z * c->KZ_int[im];
+        double ik2 = c->IK2[im];
+        double d1r = zeta[0][im], d1i = zeta[0][im + 1];
+        double d2r = zeta[1][im], d2i = zeta[1][im + 1];
+        double wr[3], wi[3];
+        if (c->KX_int[im] != 0) {
+          wr[0] = -(1. / kx) * (ky * d1r + kz * d2r);
+          wi[0] = -(1. / kx) * (ky * d1i + kz * d2i);
+          wr[1] = d1r; wi[1] = d1i;
+          wr[2] = d2r; wi[2] = d2i;
+        } else if (c->KY_int[im] != 0) {
+          double dmy = -(kz / ky);
+          wr[0] = d2r; wi[0] = d2i;
+          wr[1] = dmy * d1r; wi[1] = dmy * d1i;
+          wr[2] = d1r; wi[2] = d1i;
+        } else {
+          wr[0] = d1r; wi[0] = d1i;
+          wr[1] = d2r; wi[1] = d2i;
+          wr[2] = 0.0; wi[2] = 0.0;
+        }
+        kx *= ik2;
+        ky *= ik2;
+        kz *= ik2;
+        u[0][im] = (ky * wi[2] - kz * wi[1]);
+        u[0][im + 1] = -(ky * wr[2] - kz * wr[1]);
+        u[1][im] = (kz * wi[0] - kx * wi[2]);
+        u[1][im + 1] = -(kz * wr[0] - kx * wr[2]);
+        u[2][im] = (kx * wi[1] - ky * wi[0]);
+        u[2][im + 1] = -(kx * wr[1] - ky * wr[0]);
+      }
+  u[0][0] = uk_dc[0];
+  u[1][0] = uk_dc[1];
+  u[2][0] = uk_dc[2];
+}
+
+/* U_k2zeta_k, fft_wrapper.cxx:258-339. */
+void oracle_u_k2zeta_k(oracle_ctx *c, double *const u[3], double *const zeta[2], double uk_dc[3]) {
+  const int NX = c->NX, NY = c->NY, H = c->HNZ_;
+  uk_dc[0] = u[0][0];
+  uk_dc[1] = u[1][0];
+  uk_dc[2] = u[2][0];
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++)
+      for (int k = 0; k < H; k++) {
+        size_t im = IDX(c, i, j, 2 * k);
+        double ks0 = c->KX_int[im] * c->WAVE_X;
+        double ks1 = c->KY_int[im] * c->WAVE_Y;
+        double ks2 = c->KZ_int[im] * c->WAVE_Z;
+        double wr[3], wi[3];
+        wr[0] = (ks1 * u[2][im + 1] - ks2 * u[1][im + 1]);
+        wi[0] = -(ks1 * u[2][im] - ks2 * u[1][im]);
+        wr[1] = (ks2 * u[0][im + 1] - ks0 * u[2][im + 1]);
+        wi[1] = -(ks2 * u[0][im] - ks0 * u[2][im]);
+        wr[2] = (ks0 * u[1][im + 1] - ks1 * u[0][im + 1]);
+        wi[2] = -(ks0 * u[1][im] - ks1 * u[0][im]);
+        int a0, a1;
+        if (c->KX_int[im] != 0) {
+          a0 = 1; a1 = 2;
+        } else if (c->KY_int[im] != 0) {
+          a0 = 2; a1 = 0;
+        } else {
+          a0 = 0; a1 = 1;
+        }
+        zeta[0][im] = wr[a0];
+        zeta[0][im + 1] = wi[a0];
+        zeta[1][im] = wr[a1];
+        zeta[1][im + 1] = wi[a1];
+      }
+}
+
+/* Omega_k2zeta_k, fft_wrapper.cxx:137-173. */
+static void omega_k2zeta_k(oracle_ctx *c, double *const omega[3], double *const zetak[2]) {
+  for (size_t im = 0; im < c->nfield; im++) {
+    if (c->KX_int[im] != 0) {
+      zetak[0][im] = omega[1][im];
+      zetak[1][im] = omega[2][im];
+    } else if (c->KY_int[im] != 0) {
+      zetak[0][im] = omega[2][im];
+      zetak[1][im] = omega[0][im];
+    } else {
+      zetak[0][im] = omega[0][im];
+      zetak[1][im] = omega[1][im];
+    }
+  }
+}
+
+/* U2advection_k, operate_omega.cxx:14-100 (u is overwritten). */
+void oracle_u2advection_k(oracle_ctx *c, double *const u[3], double *const adv[2]) {
+  const int NX = c->NX, NY = c->NY, NZ = c->NZ;
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++)
+      for (int k = 0; k < NZ; k++) {
+        size_t im = IDX(c, i, j, k);
+        double u1 = u[0][im], u2 = u[1][im], u3 = u[2][im];
+        u[0][im] = u2 * u3;
+        u[1][im] = u3 * u1;
+        u[2][im] = u1 * u2;
+        adv[0][im] = (u2 - u3) * (u2 + u3);
+        adv[1][im] = (u3 - u1) * (u3 + u1);
+      }
+  for (int d = 0; d < DIM; d++) oracle_A2a_k(c, u[d]);
+  oracle_A2a_k(c, adv[0]);
+  oracle_A2a_k(c, adv[1]);
+  for (size_t im = 0; im < c->nfield; im++) {
+    double k1 = c->KX_int[im] * c->WAVE_X;
+    double k2 = c->KY_int[im] * c->WAVE_Y;
+    double k3 = c->KZ_int[im] * c->WAVE_Z;
+    double u2u3 = u[0][im], u3u1 = u[1][im], u1u2 = u[2][im];
+    double u22_u32 = adv[0][im], u32_u12 = adv[1][im];
+    double u12_u22 = -u32_u12 - u22_u32;
+    double k1k2 = k1 * k2, k2k3 = k2 * k3, k3k1 = k3 * k1;
+    u[0][im] = -(k2k3 * u22_u32 + (k3 - k2) * (k3 + k2) * u2u3 + k3k1 * u1u2 - k1k2 * u3u1);
+    u[1][im] = -(k3k1 * u32_u12 + (k1 - k3) * (k1 + k3) * u3u1 + k1k2 * u2u3 - k2k3 * u1u2);
+    u[2][im] = -(k1k2 * u12_u22 + (k2 - k1) * (k2 + k1) * u1u2 + k2k3 * u3u1 - k3k1 * u2u3);
+  }
+  omega_k2zeta_k(c, u, adv);
+}
+
+/* Solenoidal_uk, operate_omega.cxx:186-219. */
+void oracle_solenoidal_uk(oracle_ctx *c, double *const u[3]) {
+  for (size_t im = 0; im < c->nfield; im++) {
+    double kx = c->KX_int[im] * c->WAVE_X;
+    double ky = c->KY_int[im] * c->WAVE_Y;
+    double kz = c->KZ_int[im] * c->WAVE_Z;
+    double dmy = c->IK2[im] * (u[0][im] * kx + u[1][im] * ky + u[2][im] * kz);
+    u[0][im] -= dmy * kx;
+    u[1][im] -= dmy * ky;
+    u[2][im] -= dmy * kz;
+  }
+}
+
+static void u2u_k(oracle_ctx *c, double *const u[3]) {
+  for (int d = 0; d < DIM; d++) oracle_A2a_k(c, u[d]);
+}
+static void u_k2u(oracle_ctx *c, double *const u[3]) {
+  for (int d = 0; d < DIM; d++) oracle_A_k2a(c, u[d]);
+}
+/* Zeta_k2u, operate_omega.h:125-128. */
+static void zeta_k2u(oracle_ctx *c) {
+  oracle_zeta_k2u_k(c, c->zeta, c->uk_dc, c->u);
+  u_k2u(c, c->u);
+}
+/* Solenoidal_u, operate_omega.h:96-103. */
+static void solenoidal_u(oracle_ctx *c, double *const u[3]) {
+  u2u_k(c, u);
+  for (int d = 0; d < DIM; d++) oracle_truncate(c, u[d]);
+  oracle_solenoidal_uk(c, u);
+  u_k2u(c, u);
+}
+
+/* NS_solver_slavedEuler, fluid_solver.cxx:328-354, with Zeta_k2advection_k
+ * (operate_omega.cxx:102-113) and the ijk_range blocks of
+ * sp_3d_ns.cxx:15-20. */
+int oracle_ns_solver_slaved_euler(oracle_ctx *c) {
+  for (int d = 0; d < DIM - 1; d++) oracle_truncate(c, c->zeta[d]);
+  oracle_zeta_k2u_k(c, c->zeta, c->uk_dc, c->u);
+  u_k2u(c, c->u);
+  oracle_u2advection_k(c, c->u, c->f_ns0);
+  const double dmy0 = -c->NU * c->dt;
+  const double dmy1 = 1. / c->NU;
+  const int NX = c->NX, NY = c->NY, TX = c->TRN_X, TY = c->TRN_Y, TZ = c->TRN_Z;
+  const int rng[4][6] = {{0, TX - 1, 0, TY - 1, 0, 2 * TZ - 1},
+                         {0, TX - 1, NY - TY + 1, NY - 1, 0, 2 * TZ - 1},
+                         {NX - TX + 1, NX - 1, 0, TY - 1, 0, 2 * TZ - 1},
+                         {NX - TX + 1, NX - 1, NY - TY + 1, NY - 1, 0, 2 * TZ - 1}};
+  for (int n = 0; n < 4; n++)
+    for (int i = rng[n][0]; i <= rng[n][1]; i++)
+      for (int j = rng[n][2]; j <= rng[n][3]; j++)
+        for (int k = rng[n][4]; k <= rng[n][5]; k++) {
+          size_t im = IDX(c, i, j, k);
+          double dmy = exp(dmy0 * c->K2[im]) - 1.;
+          c->zeta[0][im] += dmy * (c->zeta[0][im] - dmy1 * c->IK2[im] * c->f_ns0[0][im]);
+          c->zeta[1][im] += dmy * (c->zeta[1][im] - dmy1 * c->IK2[im] * c->f_ns0[1][im]);
+        }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* particles on the grid                                               */
+/* ------------------------------------------------------------------ */
+/* H and Phi, profile.h:29-49. */
+static inline double Hfun(const oracle_ctx *c, double x) { return x > 0 ? exp(-SQ(c->DX / x)) : 0; }
+static inline double Phi(const oracle_ctx *c, double x, double radius) {
+  double dmy = Hfun(c, radius + c->HXI - x);
+  return dmy / (dmy + Hfun(c, x - radius + c->HXI));
+}
+double oracle_phi(const oracle_ctx *c, double x) { return Phi(c, x, c->RADIUS); }
+
+/* Particle_domain, profile.cxx:25-74. */
+static int particle_domain(oracle_ctx *c) {
+  const int nh = (int)c->P.a + (int)c->P.a_xi + 2;
+  const double MAX_RADIUS = c->RADIUS + c->HXI + 1.7321 * c->DX;
+  int cap = (2 * nh + 1) * (2 * nh + 1) * (2 * nh + 1);
+  c->sekibun = (int *)malloc(sizeof(int) * 3 * cap);
+  if (!c->sekibun) return -1;
+  int mesh = 0;
+  for (int i = -nh; i <= nh; i++)
+    for (int j = -nh; j <= nh; j++)
+      for (int k = -nh; k <= nh; k++) {
+        double dmy = sqrt(SQ((double)i * c->DX) + SQ((double)j * c->DX) + SQ((double)k * c->DX));
+        if (dmy < MAX_RADIUS) {
+          c->sekibun[3 * mesh] = i;
+          c->sekibun[3 * mesh + 1] = j;
+          c->sekibun[3 * mesh + 2] = k;
+          mesh++;
+        }
+      }
+  c->np_domain = mesh;
+  return 0;
+}
+
+/* Particle_cell, make_phi.h:102-130. */
+static void particle_cell(const oracle_ctx *c, const double *xp, int *x_int, double *residue) {
+  const double idx = 1. / c->DX;
+  for (int d = 0; d < DIM; d++) {
+    double dmy = xp[d] * idx;
+    x_int[d] = (int)dmy;
+    residue[d] = (dmy - (double)x_int[d]) * c->DX;
+  }
+}
+
+/* Relative_coord, make_phi.h:143-160. */
+static void relative_coord(const oracle_ctx *c, const int *cell, const int *x_int, const double *residue,
+                           int *r_mesh, double *r) {
+  const int Ns[3] = {c->NX, c->NY, c->NZ};
+  for (int d = 0; d < DIM; d++) {
+    r_mesh[d] = (x_int[d] + cell[d] + Ns[d]) % Ns[d];
+    r[d] = (double)cell[d] * c->DX - residue[d];
+  }
+}
+
+/* Distance0 / Distance, interaction.h:110-135. */
+static double distance0(const oracle_ctx *c, const double *x1, const double *x2, double *x12) {
+  double dmy = 0.0;
+  for (int d = 0; d < DIM; d++) {
+    x12[d] = x2[d] - x1[d];
+    x12[d] -= (double)Nint(x12[d] / c->L[d]) * c->L[d];
+    dmy += SQ(x12[d]);
+  }
+  return sqrt(dmy);
+}
+
+/* Angular2v_rot_on / _rot_off, make_phi.h:355-374. */
+static void angular2v(const oracle_ctx *c, const double *w, const double *r, double *v) {
+  if (c->P.rotation) {
+    v[0] = w[1] * r[2] - w[2] * r[1];
+    v[1] = w[2] * r[0] - w[0] * r[2];
+    v[2] = w[0] * r[1] - w[1] * r[0];
+  } else {
+    v[0] = v[1] = v[2] = 0.0;
+  }
+}
+
+/* Reset_phi_u, make_phi.h:426-468. */
+static void reset_phi_u(oracle_ctx *c) {
+  memset(c->phi, 0, sizeof(double) * c->nfield);
+  for (int d = 0; d < DIM; d++) memset(c->up[d], 0, sizeof(double) * c->nfield);
+}
+
+/* Make_phi_u_primitive, make_phi.cxx:221-298. */
+static void make_phi_u_primitive(oracle_ctx *c, int sw_up) {
+  for (int n = 0; n < c->Np; n++) {
+    const spm_particle *pn = &c->p[n];
+    int x_int[3], r_mesh[3];
+    double residue[3], r[3], x[3], x12[3], v_rot[3];
+    particle_cell(c, pn->x, x_int, residue);
+    for (int mesh = 0; mesh < c->np_domain; mesh++) {
+      relative_coord(c, &c->sekibun[3 * mesh], x_int, residue, r_mesh, r);
+      for (int d = 0; d < DIM; d++) x[d] = r_mesh[d] * c->DX;
+      double dmy = distance0(c, x, pn->x, x12);
+      double dmy_phi = Phi(c, dmy, c->RADIUS);
+      size_t im = IDX(c, r_mesh[0], r_mesh[1], r_mesh[2]);
+      c->phi[im] += dmy_phi;
+      if (sw_up) {
+        angular2v(c, pn->omega, r, v_rot);
+        for (int d = 0; d < DIM; d++) c->up[d][im] += ((pn->v[d] + v_rot[d]) * dmy_phi);
+      }
+    }
+  }
+  if (sw_up) {
+    for (int i = 0; i < c->NX; i++)
+      for (int j = 0; j < c->NY; j++)
+        for (int k = 0; k < c->NZ; k++) {
+          size_t im = IDX(c, i, j, k);
+          if (c->phi[im] > 1.) {
+            double idmy_phi = 1. / c->phi[im];
+            c->up[0][im] *= idmy_phi;
+            c->up[1][im] *= idmy_phi;
+            c->up[2][im] *= idmy_phi;
+            c->phi[im] = 1.;
+          }
+        }
+  }
+}
+
+int oracle_make_phi_u_particle(oracle_ctx *c) {
+  reset_phi_u(c);
+  make_phi_u_primitive(c, 1);
+  return 0;
+}
+int oracle_make_phi_particle(oracle_ctx *c) {
+  memset(c->phi, 0, sizeof(double) * c->nfield);
+  make_phi_u_primitive(c, 0);
+  return 0;
+}
+
+/* Make_f_particle_dt_sole + Add_f_particle, f_particle.h:41-99. */
+int oracle_make_f_particle_dt_sole(oracle_ctx *c) {
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) {
+        size_t im = IDX(c, i, j, k);
+        for (int d = 0; d < DIM; d++) c->f[d][im] = c->up[d][im] - c->phi[im] * c->u[d][im];
+      }
+  solenoidal_u(c, c->f);
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) {
+        size_t im = IDX(c, i, j, k);
+        for (int d = 0; d < DIM; d++) c->u[d][im] += c->f[d][im];
+      }
+  return 0;
+}
+
+/* Calc_f_hydro_correct_precision, md_force.cxx:263-389 (DBG_MASS_GRID off,
+ * no rigid bodies), preceded by Reset_phi_u as at sp_3d_ns.cxx:127. */
+int oracle_calc_f_hydro(oracle_ctx *c) {
+  reset_phi_u(c);
+  const double dmy0 = -c->DX3 * c->RHO;
+  const double dmy = dmy0 / c->dt;
+  for (int n = 0; n < c->Np; n++) {
+    spm_particle *pn = &c->p[n];
+    int x_int[3], r_mesh[3];
+    double residue[3], r[3], x[3], x12[3], v_rot[3], dmy_fp[3];
+    double force[3] = {0, 0, 0}, torque[3] = {0, 0, 0};
+    particle_cell(c, pn->x, x_int, residue);
+    for (int mesh = 0; mesh < c->np_domain; mesh++) {
+      relative_coord(c, &c->sekibun[3 * mesh], x_int, residue, r_mesh, r);
+      for (int d = 0; d < DIM; d++) x[d] = r_mesh[d] * c->DX;
+      double dmyR = distance0(c, x, pn->x, x12);
+      double dmy_phi = Phi(c, dmyR, c->RADIUS);
+      angular2v(c, pn->omega, r, v_rot);
+      size_t im = IDX(c, r_mesh[0], r_mesh[1], r_mesh[2]);
+      for (int d = 0; d < DIM; d++) {
+        dmy_fp[d] = ((pn->v[d] + v_rot[d]) - c->u[d][im]) * dmy_phi;
+        force[d] += dmy_fp[d];
+      }
+      torque[0] += (r[1] * dmy_fp[2] - r[2] * dmy_fp[1]);
+      torque[1] += (r[2] * dmy_fp[0] - r[0] * dmy_fp[2]);
+      torque[2] += (r[0] * dmy_fp[1] - r[1] * dmy_fp[0]);
+    }
+    for (int d = 0; d < DIM; d++) {
+      pn->f_hydro[d] = (dmy * force[d]) * 1.0;
+      pn->f_slip[d] = 0.0;
+    }
+    if (c->P.rotation)
+      for (int d = 0; d < DIM; d++) {
+        pn->torque_hydro[d] = (dmy * torque[d]) * 1.0;
+        pn->torque_slip[d] = 0.0;
+      }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* forces                                                              */
+/* ------------------------------------------------------------------ */
+/* Lennard_Jones_f, interaction.h:156-181. */
+static double lennard_jones_f(const oracle_ctx *c, double x, double sigma) {
+  double answer = 0.0;
+  double dmy = sigma / x;
+  if (c->P.lj_powers == 0) {
+    dmy = SQ(dmy) * SQ(dmy) * SQ(dmy);
+    answer = (24. * c->P.epsilon) / SQ(x) * (2.0 * SQ(dmy) - dmy);
+  } else if (c->P.lj_powers == 1) {
+    dmy = SQ(dmy) * SQ(dmy) * SQ(dmy) * SQ(dmy) * SQ(dmy) * SQ(dmy);
+    answer = (48. * c->P.epsilon) / SQ(x) * (2.0 * SQ(dmy) - dmy);
+  } else if (c->P.lj_powers == 2) {
+    dmy = SQ(dmy) * SQ(dmy) * SQ(dmy) * SQ(dmy) * SQ(dmy) * SQ(dmy) * SQ(dmy) * SQ(dmy) * SQ(dmy);
+    answer = (72. * c->P.epsilon) / SQ(x) * (2.0 * SQ(dmy) - dmy);
+  }
+  return answer;
+}
+
+static void lj_pair(oracle_ctx *c, const double *xi, const double *xj, double *fi, double *fj) {
+  double r_ij_vec[3];
+  double r_ij = distance0(c, xi, xj, r_ij_vec);
+  if (r_ij < c->A_R_cutoff * c->LJ_dia) {
+    double dmy = DMIN(DBL_MAX / r_ij, lennard_jones_f(c, r_ij, c->LJ_dia));
+    for (int d = 0; d < DIM; d++) {
+      double dmy1 = dmy * -r_ij_vec[d];
+      fi[d] += dmy1;
+      fj[d] -= dmy1;
+    }
+  }
+}
+
+/* Calc_f_Lennard_Jones_shear_cap_primitive_lnk, md_force.cxx:23-134:
+ * cells of Cell_length = 16 grid spacings, 27 neighbours with wrap, pairs
+ * taken with i > j.  Reproduces the reference's double visit of a
+ * neighbour cell when fewer than 3 cells span an axis. */
+void oracle_lj_lnk(oracle_ctx *c, const double *x, int n, double *fr) {
+  const int Cell_length = 16;
+  int lc[3] = {c->NX / Cell_length, c->NY / Cell_length, c->NZ / Cell_length};
+  double lc_r = Cell_length * c->DX;
+  int lcyz = lc[1] * lc[2], lcxyz = lc[0] * lcyz;
+  if (lcxyz <= 0) return;
+  int *head = (int *)malloc(sizeof(int) * lcxyz);
+  int *lscl = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  for (int d = 0; d < lcxyz; d++) head[d] = -1;
+  for (int i = 0; i < n; i++) {
+    int mc[3];
+    for (int d = 0; d < DIM; d++) mc[d] = (int)(x[3 * i + d] / lc_r);
+    int cn = mc[0] * lcyz + mc[1] * lc[2] + mc[2];
+    lscl[i] = head[cn];
+    head[cn] = i;
+  }
+  int ic[3], icl[3], icls[3];
+  for (ic[0] = 0; ic[0] < lc[0]; ic[0]++)
+    for (ic[1] = 0; ic[1] < lc[1]; ic[1]++)
+      for (ic[2] = 0; ic[2] < lc[2]; ic[2]++) {
+        int cn = ic[0] * lcyz + ic[1] * lc[2] + ic[2];
+        for (icl[0] = ic[0] - 1; icl[0] <= ic[0] + 1; icl[0]++)
+          for (icl[1] = ic[1] - 1; icl[1] <= ic[1] + 1; icl[1]++)
+            for (icl[2] = ic[2] - 1; icl[2] <= ic[2] + 1; icl[2]++) {
+              for (int d = 0; d < DIM; d++) {
+                icls[d] = icl[d];
+                if (icls[d] < 0) icls[d] = lc[d] - 1;
+                if (icls[d] > lc[d] - 1) icls[d] = 0;
+              }
+              int cl = ((icls[0] + lc[0]) % lc[0]) * lcyz + ((icls[1] + lc[1]) % lc[1]) * lc[2] +
+                       ((icls[2] + lc[2]) % lc[2]);
+              for (int i = head[cn]; i != -1; i = lscl[i])
+                for (int j = head[cl]; j != -1; j = lscl[j])
+                  if (i > j) lj_pair(c, &x[3 * i], &x[3 * j], &fr[3 * i], &fr[3 * j]);
+            }
+      }
+  free(head);
+  free(lscl);
+}
+
+/* Calc_f_Lennard_Jones_shear_cap_primitive, md_force.cxx:136-179. */
+void oracle_lj_allpairs(oracle_ctx *c, const double *x, int n, double *fr) {
+  for (int i = 0; i < n; i++)
+    for (int m = i + 1; m < n; m++) lj_pair(c, &x[3 * i], &x[3 * m], &fr[3 * i], &fr[3 * m]);
+}
+
+/* Force, particle_solver.h:97-109, with Add_f_gravity md_force.cxx:181-191. */
+int oracle_force(oracle_ctx *c) {
+  if (c->P.lj_truncate >= 0 && c->Np > 0) {
+    double *x = (double *)malloc(sizeof(double) * 3 * c->Np);
+    double *fr = (double *)malloc(sizeof(double) * 3 * c->Np);
+    for (int n = 0; n < c->Np; n++)
+      for (int d = 0; d < DIM; d++) {
+        x[3 * n + d] = c->p[n].x[d];
+        fr[3 * n + d] = c->p[n].fr[d];
+      }
+    oracle_lj_lnk(c, x, c->Np, fr);
+    for (int n = 0; n < c->Np; n++)
+      for (int d = 0; d < DIM; d++) c->p[n].fr[d] = fr[3 * n + d];
+    free(x);
+    free(fr);
+  }
+  if (c->P.g != 0.0) {
+    const double Gravity_on_fluid = c->P.g * c->RHO * 4. / 3. * M_PI * SQ(c->RADIUS) * c->RADIUS;
+    for (int n = 0; n < c->Np; n++)
+      c->p[n].fr[c->P.g_direction] -= Gravity_on_fluid * (c->P.mass_ratio[c->p[n].spec] - 1.0);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* quaternions, quaternion.h:55-250 and rigid_body.cxx:10-64           */
+/* ------------------------------------------------------------------ */
+static void qprod(spm_quaternion *q, const spm_quaternion *qa, const spm_quaternion *qb, double alpha) {
+  spm_quaternion r;
+  r.s = alpha * (qa->s * qb->s - (qa->v[0] * qb->v[0] + qa->v[1] * qb->v[1] + qa->v[2] * qb->v[2]));
+  r.v[0] = alpha * (qa->s * qb->v[0] + qb->s * qa->v[0] + qa->v[1] * qb->v[2] - qa->v[2] * qb->v[1]);
+  r.v[1] = alpha * (qa->s * qb->v[1] + qb->s * qa->v[1] + qa->v[2] * qb->v[0] - qa->v[0] * qb->v[2]);
+  r.v[2] = alpha * (qa->s * qb->v[2] + qb->s * qa->v[2] + qa->v[0] * qb->v[1] - qa->v[1] * qb->v[0]);
+  *q = r;
+}
+static void qnormalize(spm_quaternion *q) {
+  double qnorm = sqrt(q->s * q->s + q->v[0] * q->v[0] + q->v[1] * q->v[1] + q->v[2] * q->v[2]);
+  qnorm = 1.0 / qnorm;
+  q->s *= qnorm;
+  q->v[0] *= qnorm;
+  q->v[1] *= qnorm;
+  q->v[2] *= qnorm;
+}
+static void qadd(spm_quaternion *q, const spm_quaternion *qb, double alpha) {
+  q->s += qb->s * alpha;
+  q->v[0] += qb->v[0] * alpha;
+  q->v[1] += qb->v[1] * alpha;
+  q->v[2] += qb->v[2] * alpha;
+}
+/* rigid_body_rotation SPACE2BODY: conj(q) (0,v) q. */
+static void space2body(double *rot, const double *orig, const spm_quaternion *q) {
+  spm_quaternion qx = {0.0, {orig[0], orig[1], orig[2]}};
+  spm_quaternion qc = {q->s, {-q->v[0], -q->v[1], -q->v[2]}};
+  qprod(&qx, &qx, q, 1.0);
+  qprod(&qc, &qc, &qx, 1.0);
+  rot[0] = qc.v[0];
+  rot[1] = qc.v[1];
+  rot[2] = qc.v[2];
+}
+
+/* MD_solver_orientation_Euler / _AB2, particle_rotation_solver.h:15-39. */
+static void orientation_euler(spm_particle *p, double dt) {
+  spm_quaternion dqdt, qw = {0.0, {p->omega[0], p->omega[1], p->omega[2]}};
+  p->q_old = p->q;
+  qprod(&dqdt, &qw, &p->q, 0.5);
+  qadd(&p->q, &dqdt, dt);
+  qnormalize(&p->q);
+}
+static void orientation_ab2(spm_particle *p, double dt) {
+  double wb[3], wb_old[3];
+  space2body(wb, p->omega, &p->q);
+  space2body(wb_old, p->omega_old, &p->q_old);
+  for (int d = 0; d < DIM; d++) wb[d] = 3.0 * wb[d] - wb_old[d];
+  spm_quaternion dqdt, qw = {0.0, {wb[0], wb[1], wb[2]}};
+  p->q_old = p->q;
+  qprod(&dqdt, &p->q, &qw, 0.5);
+  qadd(&p->q, &dqdt, dt);
+  qnormalize(&p->q);
+}
+
+/* PBC, periodic_boundary.h:9-15. */
+static void pbc(const oracle_ctx *c, double *x) {
+  for (int d = 0; d < DIM; d++) x[d] = fmod(x[d] + c->L[d], c->L[d]);
+}
+
+/* MD_solver_position_Euler / _AB2, particle_solver.cxx:36-88. */
+int oracle_md_solver_position(oracle_ctx *c) {
+  for (int n = 0; n < c->Np; n++) {
+    spm_particle *p = &c->p[n];
+    for (int d = 0; d < DIM; d++) {
+      p->x_previous[d] = p->x[d];
+      double delta_x = (c->ts == 0) ? c->dt * p->v[d] : c->hdt * (3.0 * p->v[d] - p->v_old[d]);
+      p->x_nopbc[d] += delta_x;
+      p->x[d] += delta_x;
+    }
+    pbc(c, p->x);
+    if (c->P.rotation) {
+      if (c->ts == 0)
+        orientation_euler(p, c->dt);
+      else
+        orientation_ab2(p, c->hdt);
+    }
+  }
+  return 0;
+}
+
+/* reset_Forces, particle_solver.cxx:11-34. */
+static void reset_forces(oracle_ctx *c) {
+  for (int n = 0; n < c->Np; n++) {
+    spm_particle *p = &c->p[n];
+    for (int d = 0; d < DIM; d++) {
+      p->fr_previous[d] = p->fr[d];
+      p->fr[d] = 0.0;
+      p->f_hydro_previous[d] = p->f_hydro[d];
+      p->f_hydro[d] = 0.0;
+      p->f_slip_previous[d] = p->f_slip[d];
+      p->f_slip[d] = 0.0;
+      p->torque_hydro_previous[d] = p->torque_hydro[d];
+      p->torque_hydro[d] = 0.0;
+      p->torque_slip_previous[d] = p->torque_slip[d];
+      p->torque_slip[d] = 0.0;
+    }
+  }
+}
+
+/* MD_solver_velocity_Euler / _AB2_hydro, particle_solver.cxx:106-147,
+ * 247-289 (no self propulsion: self force and torque are 0). */
+int oracle_md_solver_velocity(oracle_ctx *c) {
+  oracle_force(c);
+  for (int n = 0; n < c->Np; n++) {
+    spm_particle *p = &c->p[n];
+    int s = p->spec;
+    if (c->ts == 0) {
+      double dmy = c->dt * c->IMASS[s], dmy_rot = c->dt * c->IMOI[s];
+      for (int d = 0; d < DIM; d++) {
+        p->v_old[d] = p->v[d];
+        p->omega_old[d] = p->omega[d];
+        p->v[d] += (dmy * (p->f_hydro[d] + p->fr[d] + 0.0));
+        p->omega[d] += (dmy_rot * (p->torque_hydro[d] + 0.0));
+      }
+    } else {
+      double dmy = c->hdt * c->IMASS[s], dmy_rot = c->hdt * c->IMOI[s];
+      for (int d = 0; d < DIM; d++) {
+        p->v_old[d] = p->v[d];
+        p->omega_old[d] = p->omega[d];
+        p->v[d] += dmy * (2.0 * (p->f_hydro[d] + 0.0) + p->fr[d] + p->fr_previous[d]);
+        p->omega[d] += dmy_rot * 2.0 * (p->torque_hydro[d] + 0.0);
+      }
+    }
+  }
+  reset_forces(c);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* MT19937 (mt19937ar.c:65-171) and thermostat (fluct.cxx:10-36)       */
+/* ------------------------------------------------------------------ */
+static void init_genrand(oracle_ctx *c, unsigned long s) {
+  c->mt[0] = (uint32_t)(s & 0xffffffffUL);
+  for (c->mti = 1; c->mti < MT_N; c->mti++)
+    c->mt[c->mti] = (uint32_t)(1812433253UL * (c->mt[c->mti - 1] ^ (c->mt[c->mti - 1] >> 30)) + c->mti);
+}
+
+uint32_t oracle_genrand_int32(oracle_ctx *c) {
+  static const uint32_t mag01[2] = {0x0U, 0x9908b0dfU};
+  uint32_t y;
+  if (c->mti >= MT_N) {
+    int kk;
+    for (kk = 0; kk < MT_N - MT_M; kk++) {
+      y = (c->mt[kk] & 0x80000000U) | (c->mt[kk + 1] & 0x7fffffffU);
+      c->mt[kk] = c->mt[kk + MT_M] ^ (y >> 1) ^ mag01[y & 1U];
+    }
+    for (; kk < MT_N - 1; kk++) {
+      y = (c->mt[kk] & 0x80000000U) | (c->mt[kk + 1] & 0x7fffffffU);
+      c->mt[kk] = c->mt[kk + (MT_M - MT_N)] ^ (y >> 1) ^ mag01[y & 1U];
+    }
+    y = (c->mt[MT_N - 1] & 0x80000000U) | (c->mt[0] & 0x7fffffffU);
+    c->mt[MT_N - 1] = c->mt[MT_M - 1] ^ (y >> 1) ^ mag01[y & 1U];
+    c->mti = 0;
+  }
+  y = c->mt[c->mti++];
+  y ^= (y >> 11);
+  y ^= (y << 7) & 0x9d2c5680U;
+  y ^= (y << 15) & 0xefc60000U;
+  y ^= (y >> 18);
+  return y;
+}
+
+double oracle_genrand_real3(oracle_ctx *c) {
+  return (((double)oracle_genrand_int32(c)) + 0.5) * (1.0 / 4294967296.0);
+}
+
+/* Gauss2, fluct.h:48-60 (Marsaglia polar method). */
+void oracle_gauss2(oracle_ctx *c, double out[2]) {
+  double x1, x2, rsq;
+  do {
+    x1 = 2. * oracle_genrand_real3(c) - 1.;
+    x2 = 2. * oracle_genrand_real3(c) - 1.;
+    rsq = x1 * x1 + x2 * x2;
+  } while (rsq >= 1.0 || rsq == 0.0);
+  double factor = sqrt(-2. * log(rsq) / rsq);
+  out[0] = factor * x1;
+  out[1] = factor * x2;
+}
+
+/* Add_random_force_thermostat, fluct.cxx:10-36 (non-rigid branch). */
+int oracle_add_random_force_thermostat(oracle_ctx *c) {
+  if (!(c->kBT > 0.)) return 0;
+  const double Zeta_drag = 6. * M_PI * c->ETA * c->RADIUS;
+  const double Zeta_drag_rot = 8. * M_PI * c->ETA * POW3(c->RADIUS);
+  const double sdv_v = sqrt(Zeta_drag * c->dt * c->kBT * c->P.alpha_v);
+  const double sdv_omega = sqrt(Zeta_drag_rot * c->dt * c->kBT * c->P.alpha_o);
+  const double noise_intensity_v = 1.439 * sdv_v;
+  const double noise_intensity_o = 1.22 * sdv_omega;
+  for (int n = 0; n < c->Np; n++) {
+    double dmy[6];
+    oracle_gauss2(c, dmy);
+    oracle_gauss2(c, dmy + 2);
+    oracle_gauss2(c, dmy + 4);
+    spm_particle *p = &c->p[n];
+    double imass = c->IMASS[p->spec], imoi = c->IMOI[p->spec];
+    for (int d = 0; d < DIM; d++) {
+      p->v[d] += dmy[d] * noise_intensity_v * imass;
+      if (c->P.rotation) p->omega[d] += dmy[d + 3] * noise_intensity_o * imoi;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* the step                                                            */
+/* ------------------------------------------------------------------ */
+/* sp_3d_ns.cxx:67-79: FIX_CELL, truncation, Zeta_k2u. */
+int oracle_zeta_k2u(oracle_ctx *c) {
+  for (int d = 0; d < DIM; d++)
+    if (c->P.fix_cell[d]) c->uk_dc[d] = 0.0;
+  for (int d = 0; d < DIM - 1; d++) oracle_truncate(c, c->zeta[d]);
+  zeta_k2u(c);
+  return 0;
+}
+
+/* U2zeta_k, operate_omega.h:186-189. */
+int oracle_u2zeta_k(oracle_ctx *c) {
+  u2u_k(c, c->u);
+  oracle_u_k2zeta_k(c, c->u, c->zeta, c->uk_dc);
+  return 0;
+}
+
+/* Init_zeta_k (init_fluid.cxx:10-23) and sp_3d_ns.cxx:508-524. */
+int oracle_init_fluid(oracle_ctx *c) {
+  for (int d = 0; d < DIM; d++) memset(c->u[d], 0, sizeof(double) * c->nfield);
+  u2u_k(c, c->u);
+  oracle_solenoidal_uk(c, c->u);
+  oracle_u_k2zeta_k(c, c->u, c->zeta, c->uk_dc);
+  oracle_make_phi_u_particle(c);
+  zeta_k2u(c);
+  oracle_make_f_particle_dt_sole(c);
+  oracle_u2zeta_k(c);
+  for (int d = 0; d < DIM; d++) c->uk_dc[d] = 0.0;
+  return 0;
+}
+
+/* Time_evolution_hydro (sp_3d_ns.cxx:61-212) for Navier_Stokes with free
+ * spherical particles, then jikan.time += dt and ts++ (:593-594). */
+int oracle_step(oracle_ctx *c, int nsteps) {
+  for (int s = 0; s < nsteps; s++) {
+    oracle_ns_solver_slaved_euler(c);
+    oracle_zeta_k2u(c);
+    oracle_md_solver_position(c);
+    oracle_calc_f_hydro(c);
+    oracle_md_solver_velocity(c);
+    if (c->kBT > 0.) oracle_add_random_force_thermostat(c);
+    oracle_make_phi_u_particle(c);
+    oracle_make_f_particle_dt_sole(c);
+    oracle_u2zeta_k(c);
+    c->time += c->dt;
+    c->ts++;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* lifetime and transfers                                              */
+/* ------------------------------------------------------------------ */
+static int is_pow2(int n) { return n >= 8 && (n & (n - 1)) == 0; }
+
+oracle_ctx *oracle_create(const spm_params *prm) {
+  if (!prm || !is_pow2(prm->nx) || !is_pow2(prm->ny) || !is_pow2(prm->nz) || prm->n_particles < 0 ||
+      prm->n_species < 1 || prm->n_species > SPM_MAX_SPECIES || !(prm->dt > 0))
+    return NULL;
+  oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
+  if (!c) return NULL;
+  c->P = *prm;
+  c->NX = prm->nx;
+  c->NY = prm->ny;
+  c->NZ = prm->nz;
+  c->HNZ_ = c->NZ / 2 + 1;
+  c->NZ_ = 2 * c->HNZ_;
+  c->TRN_X = (c->NX + 2) / 3;
+  c->TRN_Y = (c->NY + 2) / 3;
+  c->TRN_Z = (c->NZ + 2) / 3;
+  c->nfield = (size_t)c->NX * c->NY * c->NZ_;
+  /* Set_global_parameters, input.cxx:274-433 */
+  c->DX = prm->dx;
+  c->RHO = prm->rho;
+  c->ETA = prm->eta;
+  c->kBT = prm->kbt;
+  c->IRHO = 1. / c->RHO;
+  c->NU = c->ETA * c->IRHO;
+  c->DX3 = c->DX * c->DX * c->DX;
+  c->RADIUS = prm->a * c->DX;
+  c->XI = prm->a_xi * c->DX;
+  c->HXI = c->XI * 0.5;
+  c->SIGMA = 2.0 * c->RADIUS;
+  const int Ns[3] = {c->NX, c->NY, c->NZ};
+  for (int d = 0; d < DIM; d++) c->L[d] = Ns[d] * c->DX;
+  const double PI2 = 2.0 * M_PI;
+  c->WAVE_X = PI2 / c->L[0];
+  c->WAVE_Y = PI2 / c->L[1];
+  c->WAVE_Z = PI2 / c->L[2];
+  for (int s = 0; s < prm->n_species; s++) {
+    double rho_p = prm->mass_ratio[s] * c->RHO;
+    c->MASS[s] = rho_p * 4. / 3. * M_PI * POW3(c->RADIUS);
+    c->IMASS[s] = 1. / c->MASS[s];
+    c->MOI[s] = 2. / 5. * c->MASS[s] * SQ(c->RADIUS);
+    c->IMOI[s] = 1. / c->MOI[s];
+  }
+  c->LJ_dia = c->SIGMA;
+  if (prm->lj_truncate > 0) {
+    c->A_R_cutoff = prm->lj_powers == 0 ? pow(2., 1. / 6.) : prm->lj_powers == 1 ? pow(2., 1. / 12.) : pow(2., 1. / 18.);
+  } else if (prm->lj_truncate == 0) {
+    int nmax = c->NX > c->NY ? c->NX : c->NY;
+    nmax = nmax > c->NZ ? nmax : c->NZ;
+    c->A_R_cutoff = DMIN(nmax * c->DX * .5 / c->SIGMA, 2.5);
+  } else {
+    c->A_R_cutoff = 0.;
+  }
+  c->dt = prm->dt;
+  c->hdt = prm->dt * 0.5;
+  size_t nb = sizeof(double) * c->nfield;
+  int fail = 0;
+  c->KX_int = (int *)malloc(sizeof(int) * c->nfield);
+  c->KY_int = (int *)malloc(sizeof(int) * c->nfield);
+  c->KZ_int = (int *)malloc(sizeof(int) * c->nfield);
+  c->K2 = (double *)malloc(nb);
+  c->IK2 = (double *)malloc(nb);
+  fail |= !c->KX_int || !c->KY_int || !c->KZ_int || !c->K2 || !c->IK2;
+  for (int d = 0; d < 2; d++) {
+    c->zeta[d] = (double *)calloc(c->nfield, sizeof(double));
+    c->f_ns0[d] = (double *)calloc(c->nfield, sizeof(double));
+    fail |= !c->zeta[d] || !c->f_ns0[d];
+  }
+  for (int d = 0; d < 3; d++) {
+    c->u[d] = (double *)calloc(c->nfield, sizeof(double));
+    c->up[d] = (double *)calloc(c->nfield, sizeof(double));
+    c->f[d] = (double *)calloc(c->nfield, sizeof(double));
+    fail |= !c->u[d] || !c->up[d] || !c->f[d];
+  }
+  c->phi = (double *)calloc(c->nfield, sizeof(double));
+  c->rowbuf = (double *)malloc(sizeof(double) * (c->NZ + 2));
+  c->ztw = (double *)malloc(sizeof(double) * (c->NZ + 2));
+  fail |= !c->phi || !c->rowbuf || !c->ztw;
+  if (c->ztw)
+    for (int k = 0; k <= c->NZ / 2; k++) {
+      long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)c->NZ;
+      c->ztw[2 * k] = (double)cosl(ang);
+      c->ztw[2 * k + 1] = (double)sinl(ang);
+    }
+  fail |= plan_init(&c->px, c->NX) | plan_init(&c->py, c->NY) | plan_init(&c->pz, c->NZ / 2);
+  fail |= particle_domain(c);
+  c->Np = prm->n_particles;
+  c->p = (spm_particle *)calloc(c->Np > 0 ? c->Np : 1, sizeof(spm_particle));
+  fail |= !c->p;
+  if (fail) {
+    oracle_destroy(c);
+    return NULL;
+  }
+  for (int n = 0; n < c->Np; n++) c->p[n].q.s = 1.0;
+  init_k(c);
+  init_genrand(c, prm->seed);
+  return c;
+}
+
+void oracle_destroy(oracle_ctx *c) {
+  if (!c) return;
+  free(c->KX_int);
+  free(c->KY_int);
+  free(c->KZ_int);
+  free(c->K2);
+  free(c->IK2);
+  for (int d = 0; d < 2; d++) {
+    free(c->zeta[d]);
+    free(c->f_ns0[d]);
+  }
+  for (int d = 0; d < 3; d++) {
+    free(c->u[d]);
+    free(c->up[d]);
+    free(c->f[d]);
+  }
+  free(c->phi);
+  free(c->rowbuf);
+  free(c->ztw);
+  plan_free(&c->px);
+  plan_free(&c->py);
+  plan_free(&c->pz);
+  free(c->sekibun);
+  free(c->p);
+  free(c);
+}
+
+int oracle_stencil_size(const oracle_ctx *c) { return c->np_domain; }
+int oracle_stencil(const oracle_ctx *c, int *out) {
+  memcpy(out, c->sekibun, sizeof(int) * 3 * c->np_domain);
+  return c->np_domain;
+}
+
+static double *field_ptr(oracle_ctx *c, int f) {
+  switch (f) {
+    case SPM_FIELD_ZETA0: return c->zeta[0];
+    case SPM_FIELD_ZETA1: return c->zeta[1];
+    case SPM_FIELD_U0: return c->u[0];
+    case SPM_FIELD_U1: return c->u[1];
+    case SPM_FIELD_U2: return c->u[2];
+    case SPM_FIELD_UP0: return c->up[0];
+    case SPM_FIELD_UP1: return c->up[1];
+    case SPM_FIELD_UP2: return c->up[2];
+    case SPM_FIELD_PHI: return c->phi;
+    case SPM_FIELD_F0: return c->f[0];
+    case SPM_FIELD_F1: return c->f[1];
+    case SPM_FIELD_F2: return c->f[2];
+    default: return NULL;
+  }
+}
+
+int oracle_put_field(oracle_ctx *c, int field, const double *h) {
+  double *p = field_ptr(c, field);
+  if (!p || !h) return SPM_ERR_INVALID;
+  memcpy(p, h, sizeof(double) * c->nfield);
+  return 0;
+}
+int oracle_get_field(oracle_ctx *c, int field, double *h) {
+  double *p = field_ptr(c, field);
+  if (!p || !h) return SPM_ERR_INVALID;
+  memcpy(h, p, sizeof(double) * c->nfield);
+  return 0;
+}
+int oracle_put_particles(oracle_ctx *c, const spm_particle *p, int n) {
+  if (n != c->Np) return SPM_ERR_INVALID;
+  memcpy(c->p, p, sizeof(spm_particle) * n);
+  return 0;
+}
+int oracle_get_particles(oracle_ctx *c, spm_particle *p, int n) {
+  if (n != c->Np) return SPM_ERR_INVALID;
+  memcpy(p, c->p, sizeof(spm_particle) * n);
+  return 0;
+}
+int oracle_put_uk_dc(oracle_ctx *c, const double u[3]) {
+  memcpy(c->uk_dc, u, sizeof(double) * 3);
+  return 0;
+}
+int oracle_get_uk_dc(oracle_ctx *c, double u[3]) {
+  memcpy(u, c->uk_dc, sizeof(double) * 3);
+  return 0;
+}
+int oracle_put_time(oracle_ctx *c, int ts, double time) {
+  c->ts = ts;
+  c->time = time;
+  return 0;
+}
+int oracle_get_time(oracle_ctx *c, int *ts, double *time) {
+  *ts = c->ts;
+  *time = c->time;
+  return 0;
+}
+int oracle_put_rng(oracle_ctx *c, const uint32_t mt[624], int mti) {
+  memcpy(c->mt, mt, sizeof(uint32_t) * MT_N);
+  c->mti = mti;
+  return 0;
+}
+int oracle_get_rng(oracle_ctx *c, uint32_t mt[624], int *mti) {
+  memcpy(mt, c->mt, sizeof(uint32_t) * MT_N);
+  *mti = c->mti;
+  return 0;
+}
