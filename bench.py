@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the SPM time step (KAPSEL Time_evolution_hydro) on one MI355X.
+
+Workload: 256^3 periodic grid, dx = 1, rho = eta = 1, dt = 0.05, 4000 spheres
+of radius a = 5 with interface width xi = 2 (volume fraction 0.128), placed
+uniformly at random (seed 12345) without overlap closer than 1.05 * 2a, WCA
+repulsion (sigma = 2a, epsilon = 1, cutoff 2^(1/6) sigma), thermal noise
+kBT = 0.1 from MT19937 seeded 12345, rotation on, FIX_CELL on.  Inputs are
+resident in HBM before the timed region; one step is one full
+Time_evolution_hydro.
+
+Prints one JSON line: time-steps/s (whole job), the roofline of the dominant
+kernel family measured with HIP events on the library's stream, and the CPU
+oracle timed on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from kapsel_old_amd import abi  # noqa: E402
+
+FAMILIES = ["fft_z_r2c", "fft_z_c2r", "fft_y", "fft_x", "spectral", "realspace", "make_phi", "f_hydro",
+            "particle"]
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_params(n, np_, precision=abi.FP64):
+    return abi.make_params(n, a=5.0, a_xi=2.0, dt=0.05, kbt=0.1, n_particles=np_, lj_truncate=abi.LJ_WCA,
+                           lj_powers=abi.LJ_12_6, epsilon=1.0, rotation=1, fix_cell=(1, 1, 1), seed=12345,
+                           precision=precision)
+
+
+def workload_particles(n, np_):
+    p = abi.new_particles(np_)
+    pos = abi.random_positions(np_, (n, n, n), 1.05 * 2 * 5.0, seed=12345)
+    p["x"] = pos
+    p["x_nopbc"] = pos
+    return p
+
+
+def algorithmic_step_bytes(n, np_, stencil, w=8, n_fft=20):
+    """B_step = N_FFT (8N + 16 NX NY (NZ/2+1)) (w/8) + Np S 11 w: each 3-D FFT
+    reads its input and writes its output once, the gather reads 3 values and
+    the scatter read-modify-writes 4 per stencil point."""
+    nn = n ** 3
+    return n_fft * (8 * nn + 16 * n * n * (n // 2 + 1)) * (w / 8) + np_ * stencil * 11 * w
+
+
+def timing_read(lib, h):
+    import ctypes as C
+    ms = (C.c_double * 9)()
+    by = (C.c_double * 9)()
+    cnt = (C.c_long * 9)()
+    rc = lib.spm_timing_read(h, ms, by, cnt)
+    if rc:
+        raise RuntimeError("spm_timing_read failed (%d)" % rc)
+    return list(ms), list(by), list(cnt)
+
+
+def gpu_run(n, np_, steps, warmup, profile_steps):
+    import ctypes as C
+
+    from kapsel_old_amd.spm import SPM, lib
+    L = lib()
+    L.spm_timing_enable.argtypes = [C.c_void_p, C.c_int]
+    L.spm_timing_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_long)]
+    g = SPM(workload_params(n, np_))
+    g.put_particles(workload_particles(n, np_))
+    g.init_fluid()
+    g.step(warmup)
+    g.synchronize()
+    print("bench: warmup done (%d steps)" % warmup, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    print("bench: timed %d steps in %.3f s" % (steps, elapsed), file=sys.stderr, flush=True)
+    g.check_finite()
+    # kernel durations: HIP events around every launch of the same step loop
+    L.spm_timing_enable(g._h, 1)
+    timing_read(L, g._h)
+    t2 = time.perf_counter()
+    g.step(profile_steps)
+    g.synchronize()
+    t3 = time.perf_counter()
+    ms, by, cnt = timing_read(L, g._h)
+    L.spm_timing_enable(g._h, 0)
+    stencil = g.stencil_size()
+    g.close()
+    return elapsed, (t3 - t2), ms, by, cnt, stencil
+
+
+def cpu_baseline(n, np_, max_steps=2, budget_s=40.0):
+    """The CPU oracle (single-threaded FP64 restatement of the reference
+    step) on the same workload: init untimed, then up to max_steps steps."""
+    from oracle.oracle import Oracle
+    o = Oracle(workload_params(n, np_))
+    o.put_particles(workload_particles(n, np_))
+    o.init_fluid()
+    done = 0
+    t0 = time.perf_counter()
+    while done < max_steps:
+        o.step(1)
+        done += 1
+        if time.perf_counter() - t0 > budget_s / max_steps * done and done >= 1:
+            if time.perf_counter() - t0 > budget_s:
+                break
+    t1 = time.perf_counter()
+    o.close()
+    return done / (t1 - t0), done, t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--particles", type=int, default=4000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (one box per GPU); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    n, np_ = args.grid, args.particles
+    elapsed, prof_elapsed, ms, by, cnt, stencil = gpu_run(n, np_, args.steps, args.warmup, args.profile_steps)
+    value = args.steps / elapsed
+    dom = int(np.argmax(ms))
+    per_launch_ms = ms[dom] / max(cnt[dom], 1)
+    per_launch_bytes = by[dom] / max(cnt[dom], 1)
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    b_step = algorithmic_step_bytes(n, np_, stencil)
+    families = {FAMILIES[f]: {"ms_per_step": ms[f] / args.profile_steps, "launches_per_step":
+                              cnt[f] / args.profile_steps,
+                              "GBps": (by[f] / (ms[f] * 1e-3) / 1e9) if ms[f] > 0 else None}
+                for f in range(len(FAMILIES))}
+    out = {
+        "metric": "time-steps/s at 256^3 w/ 4000 spheres",
+        "value": value,
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "SPM time step (Time_evolution_hydro), %d^3 grid, %d spheres a=5 xi=2, "
+                               "WCA, kBT=0.1, dt=0.05, FP64 fields" % (n, np_),
+                   "grid": [n, n, n], "n_particles": np_, "stencil_points": stencil, "precision": "fp64"},
+        "hbm_TBps_algorithmic": b_step * value / 1e12,
+        "step_floor_ms_at_8TBps": b_step / 8e12 * 1e3,
+        "roofline": {"bound": "hbm", "kernel": FAMILIES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": per_launch_bytes, "us_per_launch": per_launch_ms * 1e3},
+        "kernel_families": families,
+    }
+    if not args.no_cpu_baseline:
+        v, done, secs = cpu_baseline(n, np_)
+        out["cpu_baseline"] = {"value": v, "unit": "time-steps/s", "cores": 1, "kind": "port",
+                               "sample": "%d step(s) of the same %d^3/%d workload after an untimed init, "
+                                         "%.1f s, oracle/liboracle.so single-threaded" % (done, n, np_, secs)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -206,6 +206,28 @@ int spm_synchronize(spm_ctx *ctx);
 /* Scans u and zeta for NaN/Inf; SPM_ERR_NONFINITE if any is found. */
 int spm_check_finite(spm_ctx *ctx);
 
+/* ---- measurement (no counterpart in the reference) --------------------
+ * With timing on, every kernel launch is bracketed by HIP events on the
+ * context's stream and attributed to a kernel family together with the
+ * algorithmic HBM bytes it moves (each element it must read or write, once).
+ * spm_timing_read waits for the stream and returns the totals since the
+ * last read (and clears them). */
+enum spm_kernel_family {
+  SPM_K_FFT_Z_R2C = 0,   /* z pass, real -> half spectrum               */
+  SPM_K_FFT_Z_C2R = 1,   /* z pass, half spectrum -> real               */
+  SPM_K_FFT_Y = 2,       /* y pass, complex lines                       */
+  SPM_K_FFT_X = 3,       /* x pass, complex lines                       */
+  SPM_K_SPECTRAL = 4,    /* pointwise k-space kernels                   */
+  SPM_K_REALSPACE = 5,   /* pointwise real-space kernels                */
+  SPM_K_MAKE_PHI = 6,    /* phi / up scatter (owner-computes gather)    */
+  SPM_K_F_HYDRO = 7,     /* hydrodynamic force / torque reduction       */
+  SPM_K_PARTICLE = 8,    /* lists, LJ, integrators, thermostat          */
+  SPM_NUM_KERNEL_FAMILIES = 9
+};
+int spm_timing_enable(spm_ctx *ctx, int on);
+int spm_timing_read(spm_ctx *ctx, double ms[SPM_NUM_KERNEL_FAMILIES], double bytes[SPM_NUM_KERNEL_FAMILIES],
+                    long launches[SPM_NUM_KERNEL_FAMILIES]);
+
 #ifdef __cplusplus
 }
 #endif

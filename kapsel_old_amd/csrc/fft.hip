@@ -1,0 +1,388 @@
+// fft.hip -- 3-D real FFTs in KAPSEL's layout, hand-written for gfx950.
+//
+// Replaces A2a_k / A_k2a (fft_wrapper.h:432-631), which call Ooura's
+// rdft3d + rdft3dsort (fftsg3d.c:560-723).  Same transform:
+//   forward  X(k) = sum_x a(x) exp(+2 pi i k.x / N)   (fftsg3d.c:109-120)
+//   inverse  a(x) = (1/N) sum_k X(k) exp(-2 pi i k.x / N)
+// on a [NX][NY][NZ_] array, real data in k < NZ, spectrum as NZ/2+1
+// interleaved complex values per row.
+//
+// A 3-D transform is three passes, each one read and one write of the
+// field, staged through LDS:
+//   z pass  : rows are contiguous; NZ reals are packed as NZ/2 complex
+//             values, transformed, and split into the half spectrum.
+//   y, x    : complex lines with stride NZ_/2 (y) or NY*NZ_/2 (x); a block
+//             loads C adjacent columns (16 B each) of N points, so global
+//             accesses are C*16-byte contiguous segments.
+// Inside a block the transform is a radix-8 Stockham FFT: each thread
+// keeps 8 points in registers per stage, stages exchange through LDS laid
+// out [point][column] so a wavefront touches consecutive LDS words.
+#include "spm_internal.h"
+
+namespace spm {
+
+template <typename C2, int SIGN>
+__device__ __forceinline__ C2 mul_si(C2 a) {  // a * (SIGN i)
+  return SIGN > 0 ? C2{-a.y, a.x} : C2{a.y, -a.x};
+}
+
+template <typename C2, int SIGN>
+__device__ __forceinline__ void dft2(C2 *a) {
+  C2 t = a[0];
+  a[0] = cadd(t, a[1]);
+  a[1] = csub(t, a[1]);
+}
+
+template <typename C2, int SIGN>
+__device__ __forceinline__ void dft4v(C2 &a0, C2 &a1, C2 &a2, C2 &a3) {
+  C2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3);
+  C2 t3 = mul_si<C2, SIGN>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a1 = cadd(t1, t3);
+  a2 = csub(t0, t2);
+  a3 = csub(t1, t3);
+}
+
+template <typename C2, int SIGN>
+__device__ __forceinline__ void dft4(C2 *a) {
+  dft4v<C2, SIGN>(a[0], a[1], a[2], a[3]);
+}
+
+template <typename C2, int SIGN>
+__device__ __forceinline__ void dft8(C2 *a) {
+  using R = decltype(a[0].x);
+  C2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  C2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4v<C2, SIGN>(e0, e1, e2, e3);
+  dft4v<C2, SIGN>(o0, o1, o2, o3);
+  const R r = (R)0.70710678118654752440084436210485;
+  const R s = (R)SIGN;
+  o1 = C2{r * (o1.x - s * o1.y), r * (o1.y + s * o1.x)};  // * exp(SIGN i pi/4)
+  o2 = mul_si<C2, SIGN>(o2);                               // * exp(SIGN i pi/2)
+  o3 = C2{-r * (o3.x + s * o3.y), r * (s * o3.x - o3.y)};  // * exp(SIGN 3 i pi/4)
+  a[0] = cadd(e0, o0);
+  a[4] = csub(e0, o0);
+  a[1] = cadd(e1, o1);
+  a[5] = csub(e1, o1);
+  a[2] = cadd(e2, o2);
+  a[6] = csub(e2, o2);
+  a[3] = cadd(e3, o3);
+  a[7] = csub(e3, o3);
+}
+
+template <typename C2, int RS, int SIGN>
+__device__ __forceinline__ void dft(C2 *a) {
+  if constexpr (RS == 2) dft2<C2, SIGN>(a);
+  else if constexpr (RS == 4) dft4<C2, SIGN>(a);
+  else dft8<C2, SIGN>(a);
+}
+
+// One radix-RS Stockham stage of span NS on C columns of N points held in
+// LDS at lds[point * CP + column].  The block has C * N / R threads; each
+// performs R / RS butterflies.  Twiddles come from tw[t] = exp(+2 pi i t/N).
+template <typename T, int N, int R, int RS, int NS, int C, int CP, int SIGN>
+__device__ __forceinline__ void stockham_stage(cplx<T> *lds, const double2 *__restrict__ tw, int tid) {
+  using C2 = cplx<T>;
+  constexpr int NB = N / RS;  // butterflies per column
+  constexpr int PER = R / RS;
+  const int c = tid % C;
+  const int j0 = tid / C;
+  C2 a[R];
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int j = j0 + s * (N / R);
+#pragma unroll
+    for (int r = 0; r < RS; ++r) a[s * RS + r] = lds[(j + r * NB) * CP + c];
+    if constexpr (NS > 1) {
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 1; r < RS; ++r) {
+        const double2 w = tw[k * r * (N / (NS * RS))];
+        a[s * RS + r] = cmul(a[s * RS + r], C2{(T)w.x, (T)(SIGN * w.y)});
+      }
+    }
+    dft<C2, RS, SIGN>(&a[s * RS]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int j = j0 + s * (N / R);
+    const int k = j % NS;
+    const int base = (j / NS) * NS * RS + k;
+#pragma unroll
+    for (int r = 0; r < RS; ++r) lds[(base + r * NS) * CP + c] = a[s * RS + r];
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int R, int C, int CP, int SIGN, int NS>
+__device__ __forceinline__ void fft_stages(cplx<T> *lds, const double2 *__restrict__ tw, int tid) {
+  if constexpr (NS < N) {
+    constexpr int RS = (N / NS >= R) ? R : N / NS;
+    stockham_stage<T, N, R, RS, NS, C, CP, SIGN>(lds, tw, tid);
+    fft_stages<T, N, R, C, CP, SIGN, NS * RS>(lds, tw, tid);
+  }
+}
+
+template <typename T>
+struct FieldPtrs {
+  T *p[5];
+};
+
+// ---------------------------------------------------------------------
+// y / x passes: complex lines of N points, element stride `estride`
+// (complex units), `ncols` columns per outer index, outer stride `ostride`.
+// blockIdx.x = column tile, blockIdx.y = outer index, blockIdx.z = field.
+// ---------------------------------------------------------------------
+template <typename T, int N, int R, int C, int SIGN>
+__global__ __launch_bounds__(C *N / R) void fft_lines_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw,
+                                                             int ncols, long estride, long ostride) {
+  using C2 = cplx<T>;
+  constexpr int NT = C * N / R;
+  __shared__ C2 lds[N * C];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * C;
+  C2 *g = reinterpret_cast<C2 *>(fp.p[blockIdx.z]) + (long)blockIdx.y * ostride + c0;
+#pragma unroll 4
+  for (int f = tid; f < N * C; f += NT) {
+    const int c = f % C, j = f / C;
+    lds[j * C + c] = (c0 + c < ncols) ? g[j * estride + c] : C2{0, 0};
+  }
+  __syncthreads();
+  fft_stages<T, N, R, C, C, SIGN, 1>(lds, tw, tid);
+#pragma unroll 4
+  for (int f = tid; f < N * C; f += NT) {
+    const int c = f % C, j = f / C;
+    if (c0 + c < ncols) g[j * estride + c] = lds[j * C + c];
+  }
+}
+
+// ---------------------------------------------------------------------
+// z pass, forward: ROWS rows per block, each NZ = 2M reals -> M+1 complex.
+// ---------------------------------------------------------------------
+template <typename T, int M, int R, int ROWS>
+__global__ __launch_bounds__(ROWS *M / R) void fft_z_r2c_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw,
+                                                               const double2 *__restrict__ twr, int nrows,
+                                                               int nz_) {
+  using C2 = cplx<T>;
+  constexpr int NT = ROWS * M / R;
+  constexpr int CP = ROWS + 1;
+  constexpr int H = M + 1;
+  __shared__ C2 lds[M * CP];
+  T *a = fp.p[blockIdx.z];
+  const int tid = threadIdx.x;
+  const long row0 = (long)blockIdx.x * ROWS;
+#pragma unroll 4
+  for (int f = tid; f < ROWS * M; f += NT) {
+    const int m = f % M, r = f / M;
+    const long row = row0 + r;
+    lds[m * CP + r] = (row < nrows) ? *reinterpret_cast<const C2 *>(a + row * nz_ + 2 * m) : C2{0, 0};
+  }
+  __syncthreads();
+  fft_stages<T, M, R, ROWS, CP, +1, 1>(lds, tw, tid);
+  for (int f = tid; f < ROWS * H; f += NT) {
+    const int k = f % H, r = f / H;
+    const long row = row0 + r;
+    const C2 z = lds[(k % M) * CP + r];
+    C2 zc = lds[((M - k) % M) * CP + r];
+    zc.y = -zc.y;
+    const C2 e{(T)0.5 * (z.x + zc.x), (T)0.5 * (z.y + zc.y)};
+    const C2 o{(T)0.5 * (z.y - zc.y), (T)-0.5 * (z.x - zc.x)};  // (z - zc) / (2i)
+    const double2 w = twr[k];
+    const C2 x = cadd(e, cmul(o, C2{(T)w.x, (T)w.y}));
+    if (row < nrows) *reinterpret_cast<C2 *>(a + row * nz_ + 2 * k) = x;
+  }
+}
+
+// ---------------------------------------------------------------------
+// z pass, inverse: M+1 complex -> NZ = 2M reals per row, times `scale`.
+// ---------------------------------------------------------------------
+template <typename T, int M, int R, int ROWS>
+__global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw,
+                                                               const double2 *__restrict__ twr, int nrows, int nz_,
+                                                               T scale) {
+  using C2 = cplx<T>;
+  constexpr int NT = ROWS * M / R;
+  constexpr int CP = ROWS + 1;
+  constexpr int H = M + 1;
+  constexpr int HP = M / 2 + 1;  // pairs (k, M-k)
+  __shared__ C2 lds[H * CP];
+  T *a = fp.p[blockIdx.z];
+  const int tid = threadIdx.x;
+  const long row0 = (long)blockIdx.x * ROWS;
+  for (int f = tid; f < ROWS * H; f += NT) {
+    const int k = f % H, r = f / H;
+    const long row = row0 + r;
+    lds[k * CP + r] = (row < nrows) ? *reinterpret_cast<const C2 *>(a + row * nz_ + 2 * k) : C2{0, 0};
+  }
+  __syncthreads();
+  // Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) exp(-2 pi i k/NZ)
+  for (int f = tid; f < ROWS * HP; f += NT) {
+    const int p = f % HP, r = f / HP;
+    const int k = p, kk = M - p;
+    const C2 xk = lds[k * CP + r], xkk = lds[kk * CP + r];
+    {
+      const C2 cj{xkk.x, -xkk.y};
+      const C2 e = cadd(xk, cj), d = csub(xk, cj);
+      const double2 w = twr[k];
+      const C2 o = cmul(d, C2{(T)w.x, (T)-w.y});
+      lds[k * CP + r] = C2{e.x - o.y, e.y + o.x};
+    }
+    if (p > 0 && kk != k) {
+      const C2 cj{xk.x, -xk.y};
+      const C2 e = cadd(xkk, cj), d = csub(xkk, cj);
+      const double2 w = twr[kk];
+      const C2 o = cmul(d, C2{(T)w.x, (T)-w.y});
+      lds[kk * CP + r] = C2{e.x - o.y, e.y + o.x};
+    }
+  }
+  __syncthreads();
+  fft_stages<T, M, R, ROWS, CP, -1, 1>(lds, tw, tid);
+  for (int f = tid; f < ROWS * H; f += NT) {
+    const int m = f % H, r = f / H;
+    const long row = row0 + r;
+    if (row < nrows) {
+      C2 v{0, 0};
+      if (m < M) {
+        v = lds[m * CP + r];
+        v.x *= scale;
+        v.y *= scale;
+      }
+      *reinterpret_cast<C2 *>(a + row * nz_ + 2 * m) = v;  // m == M writes the zero padding
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// host side: configuration per size and launch
+// ---------------------------------------------------------------------
+namespace {
+
+constexpr int kLinesC(int n) { return n >= 4096 ? 1 : (4096 / n > 32 ? 32 : 4096 / n); }
+constexpr int kRows(int m) { return 2048 / m > 32 ? 32 : (2048 / m < 1 ? 1 : 2048 / m); }
+constexpr int kRadix(int n) { return n >= 8 ? 8 : n; }
+
+template <typename T, int N, int SIGN>
+void launch_lines_n(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
+                    int nouter, hipStream_t st) {
+  constexpr int R = kRadix(N), C = kLinesC(N);
+  dim3 grid((ncols + C - 1) / C, nouter, nf);
+  hipLaunchKernelGGL((fft_lines_kernel<T, N, R, C, SIGN>), grid, dim3(C * N / R), 0, st, fp, tw, ncols, estride,
+                     ostride);
+}
+
+template <typename T, int SIGN>
+int launch_lines(int n, const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
+                 int nouter, hipStream_t st) {
+  switch (n) {
+    case 8: launch_lines_n<T, 8, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 16: launch_lines_n<T, 16, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 32: launch_lines_n<T, 32, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 64: launch_lines_n<T, 64, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 128: launch_lines_n<T, 128, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 256: launch_lines_n<T, 256, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 512: launch_lines_n<T, 512, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 1024: launch_lines_n<T, 1024, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+  }
+  return -1;
+}
+
+template <typename T, int M>
+void launch_z_n(bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr, int nrows, int nz_,
+                T scale, hipStream_t st) {
+  constexpr int R = kRadix(M), ROWS = kRows(M);
+  dim3 grid((nrows + ROWS - 1) / ROWS, 1, nf);
+  if (fwd)
+    hipLaunchKernelGGL((fft_z_r2c_kernel<T, M, R, ROWS>), grid, dim3(ROWS * M / R), 0, st, fp, tw, twr, nrows, nz_);
+  else
+    hipLaunchKernelGGL((fft_z_c2r_kernel<T, M, R, ROWS>), grid, dim3(ROWS * M / R), 0, st, fp, tw, twr, nrows, nz_,
+                       scale);
+}
+
+template <typename T>
+int launch_z(int m, bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr, int nrows,
+             int nz_, T scale, hipStream_t st) {
+  switch (m) {
+    case 4: launch_z_n<T, 4>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 8: launch_z_n<T, 8>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 16: launch_z_n<T, 16>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 32: launch_z_n<T, 32>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 64: launch_z_n<T, 64>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 128: launch_z_n<T, 128>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 256: launch_z_n<T, 256>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 512: launch_z_n<T, 512>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+  }
+  return -1;
+}
+
+int upload_roots(spm_ctx *c, double2 **dst, int n, int count) {
+  std::vector<double2> h(count);
+  for (int t = 0; t < count; ++t) {
+    long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+    h[t] = double2{(double)cosl(ang), (double)sinl(ang)};
+  }
+  SPM_HIP(c, hipMalloc(dst, sizeof(double2) * count));
+  SPM_HIP(c, hipMemcpy(*dst, h.data(), sizeof(double2) * count, hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+int fft_init_tables(spm_ctx *c) {
+  const Grid &g = c->grid;
+  int rc;
+  if ((rc = upload_roots(c, &c->tw_x, g.nx, g.nx))) return rc;
+  if ((rc = upload_roots(c, &c->tw_y, g.ny, g.ny))) return rc;
+  if ((rc = upload_roots(c, &c->tw_z, g.nz / 2, g.nz / 2))) return rc;
+  if ((rc = upload_roots(c, &c->tw_rz, g.nz, g.nz / 2 + 1))) return rc;
+  return 0;
+}
+
+static int fft_many(spm_ctx *c, double *const *a, int n, bool fwd) {
+  const Grid &g = c->grid;
+  for (int b = 0; b < n; b += 5) {
+    FieldPtrs<double> fp{};
+    const int nf = (n - b) < 5 ? (n - b) : 5;
+    for (int q = 0; q < nf; ++q) fp.p[q] = a[b + q];
+    const int nrows = g.nx * g.ny;
+    const long h = g.hnz;
+    // every pass reads and writes each field once: 2 * NX*NY*NZ_ doubles
+    const double bytes = 2.0 * nf * (double)nrows * g.nz_ * sizeof(double);
+    int rc = 0;
+    if (fwd) {
+      tbeg(c);
+      rc |= launch_z<double>(g.nz / 2, true, fp, nf, c->tw_z, c->tw_rz, nrows, g.nz_, 1.0, c->stream);
+      tend(c, SPM_K_FFT_Z_R2C, bytes);
+      tbeg(c);
+      rc |= launch_lines<double, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
+      tend(c, SPM_K_FFT_Y, bytes);
+      tbeg(c);
+      rc |= launch_lines<double, +1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream);
+      tend(c, SPM_K_FFT_X, bytes);
+    } else {
+      const double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+      tbeg(c);
+      rc |= launch_lines<double, -1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream);
+      tend(c, SPM_K_FFT_X, bytes);
+      tbeg(c);
+      rc |= launch_lines<double, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
+      tend(c, SPM_K_FFT_Y, bytes);
+      tbeg(c);
+      rc |= launch_z<double>(g.nz / 2, false, fp, nf, c->tw_z, c->tw_rz, nrows, g.nz_, scale, c->stream);
+      tend(c, SPM_K_FFT_Z_C2R, bytes);
+    }
+    if (rc) {
+      c->err = "unsupported FFT size";
+      return SPM_ERR_UNSUPPORTED;
+    }
+    SPM_LAUNCH_CHECK(c);
+  }
+  return 0;
+}
+
+int fft_r2c_many(spm_ctx *c, double *const *a, int n) { return fft_many(c, a, n, true); }
+int fft_c2r_many(spm_ctx *c, double *const *a, int n) { return fft_many(c, a, n, false); }
+int fft_r2c(spm_ctx *c, double *a) { return fft_many(c, &a, 1, true); }
+int fft_c2r(spm_ctx *c, double *a) { return fft_many(c, &a, 1, false); }
+
+}  // namespace spm

@@ -1,0 +1,647 @@
+// particles.hip -- particle <-> grid coupling and particle dynamics.
+//
+//   make_phi   Make_phi_u_primitive (make_phi.cxx:221-298) as an
+//              owner-computes gather: the grid is cut into 8x8x8 tiles,
+//              each tile holds the sorted list of particles whose stencil
+//              cube reaches it, and each grid point sums its particles in
+//              ascending index -- the order in which the reference's
+//              particle loop adds into the same point.  No atomics on
+//              floating point, so results are run-to-run identical.
+//   f_hydro    Calc_f_hydro_correct_precision (md_force.cxx:263-389): one
+//              wavefront per particle strides the stencil table and
+//              reduces force and torque with cross-lane shuffles in FP64.
+//   force      Force (particle_solver.h:97-109): LJ/WCA on a linked-cell
+//              list whose cells are at least one cutoff wide (a correct
+//              list also for boxes under 3 cells per axis, where the
+//              reference's md_force.cxx:78-92 visits a cell twice), plus
+//              gravity (md_force.cxx:181-191).
+//   integrators MD_solver_position_* / MD_solver_velocity_* and the
+//              quaternion updates, one thread per particle, FP64.
+#include "spm_internal.h"
+
+namespace spm {
+
+namespace {
+
+constexpr int kTile = 8;
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+__device__ __forceinline__ int nint(double a) { return a >= 0.0 ? (int)(a + .5) : (int)(a - .5); }
+
+// H and Phi, profile.h:29-49.
+__device__ __forceinline__ double hfun(double dx, double x) { return x > 0 ? exp(-sq(dx / x)) : 0.0; }
+__device__ __forceinline__ double phi_of(const PModel &pm, double x) {
+  const double dmy = hfun(pm.dx, pm.radius + pm.hxi - x);
+  return dmy / (dmy + hfun(pm.dx, x - pm.radius + pm.hxi));
+}
+
+// Distance0 (interaction.h:110-123) from grid point x1 to particle x2.
+__device__ __forceinline__ double distance0(const Grid &g, const double *x1, const double *x2, double *x12) {
+  const double L[3] = {g.lx, g.ly, g.lz};
+  double dmy = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    x12[d] = x2[d] - x1[d];
+    x12[d] -= (double)nint(x12[d] / L[d]) * L[d];
+    dmy += sq(x12[d]);
+  }
+  return sqrt(dmy);
+}
+
+__device__ __forceinline__ double P(const double *pb, int np, int slot, int n) { return pb[(long)slot * np + n]; }
+__device__ __forceinline__ double &PR(double *pb, int np, int slot, int n) { return pb[(long)slot * np + n]; }
+
+// ---------------------------------------------------------------------
+// anchors and per-tile lists
+// ---------------------------------------------------------------------
+// Particle_cell (make_phi.h:102-130) and the number of tiles each
+// particle's stencil cube [x_int - nh, x_int + nh] reaches.
+__device__ __forceinline__ void tile_span(int xint, int nh, int ntile, int &t0, int &cnt) {
+  const int lo = xint - nh, hi = xint + nh;
+  const int a = (lo >= 0) ? lo / kTile : -((-lo + kTile - 1) / kTile);
+  const int b = (hi >= 0) ? hi / kTile : -((-hi + kTile - 1) / kTile);
+  cnt = b - a + 1;
+  if (cnt > ntile) cnt = ntile;
+  t0 = ((a % ntile) + ntile) % ntile;
+}
+
+__global__ void anchor_count_kernel(Grid g, PModel pm, const double *__restrict__ pb, int *pxint, double *pres,
+                                    int tx, int ty, int tz, int *tile_count) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= pm.np) return;
+  const double idx = 1. / pm.dx;
+  const int N[3] = {g.nx, g.ny, g.nz};
+  int xi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double dmy = P(pb, pm.np, P_X + d, n) * idx;
+    // positions are kept in [0, L) by PBC; the clamp only keeps indices in
+    // range if a non-finite or escaped position ever arrives
+    xi[d] = (dmy >= 0.0 && dmy < (double)N[d]) ? (int)dmy : 0;
+    pxint[3 * n + d] = xi[d];
+    pres[3 * n + d] = (dmy - (double)xi[d]) * pm.dx;
+  }
+  int t0[3], cn[3];
+  const int nt[3] = {tx, ty, tz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) tile_span(xi[d], pm.nh, nt[d], t0[d], cn[d]);
+  for (int a = 0; a < cn[0]; ++a)
+    for (int b = 0; b < cn[1]; ++b)
+      for (int cc = 0; cc < cn[2]; ++cc) {
+        const int ti = ((t0[0] + a) % tx * ty + (t0[1] + b) % ty) * tz + (t0[2] + cc) % tz;
+        atomicAdd(&tile_count[ti], 1);
+      }
+}
+
+__global__ void tile_fill_kernel(PModel pm, const int *__restrict__ pxint, int tx, int ty, int tz, int *tile_cursor,
+                                 int *tile_list) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= pm.np) return;
+  int t0[3], cn[3];
+  const int nt[3] = {tx, ty, tz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) tile_span(pxint[3 * n + d], pm.nh, nt[d], t0[d], cn[d]);
+  for (int a = 0; a < cn[0]; ++a)
+    for (int b = 0; b < cn[1]; ++b)
+      for (int cc = 0; cc < cn[2]; ++cc) {
+        const int ti = ((t0[0] + a) % tx * ty + (t0[1] + b) % ty) * tz + (t0[2] + cc) % tz;
+        const int slot = atomicAdd(&tile_cursor[ti], 1);
+        tile_list[slot] = n;
+      }
+}
+
+// Single-block exclusive scan of count[0..n) into start[] and cursor[].
+__global__ __launch_bounds__(1024) void scan_kernel(const int *__restrict__ count, int *start, int *cursor, int n) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int b = t * chunk, e = min(n, b + chunk);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += count[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int i = b; i < e; ++i) {
+    start[i] = run;
+    cursor[i] = run;
+    run += count[i];
+  }
+}
+
+// Sort each list segment ascending (segments are short: insertion sort).
+__global__ void sort_segments_kernel(const int *__restrict__ start, const int *__restrict__ count, int *list, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int *a = list + start[s];
+  const int m = count[s];
+  for (int i = 1; i < m; ++i) {
+    const int v = a[i];
+    int j = i - 1;
+    while (j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      --j;
+    }
+    a[j + 1] = v;
+  }
+}
+
+__global__ void izero_kernel(int *a, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0;
+}
+
+// ---------------------------------------------------------------------
+// make_phi: owner-computes over 8x8x8 tiles (one point per thread)
+// ---------------------------------------------------------------------
+// The stencil point of particle n that lands on grid index gi along an axis
+// of N points has offset cell = gi - x_int (mod N) taken in [-N/2, N/2);
+// it belongs to the stencil only if |cell| <= nh (the stencil cube).  Any
+// point with distance >= a + xi/2 has phi = 0 exactly and adds nothing, and
+// every point closer than that lies inside the stencil sphere
+// (a + xi/2 + |residue| < a + xi/2 + 1.7321 dx), so the distance test alone
+// reproduces the reference's sum over Sekibun_cell.
+__global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+                                                       const int *__restrict__ pxint, const double *__restrict__ pres,
+                                                       const int *__restrict__ tile_start,
+                                                       const int *__restrict__ tile_count,
+                                                       const int *__restrict__ tile_list, int tx, int ty, int tz,
+                                                       double *phi, double *up0, double *up1, double *up2, int sw_up) {
+  const int tile = blockIdx.x;
+  const int tk = tile % tz, tj = (tile / tz) % ty, ti = tile / (tz * ty);
+  const int lk = threadIdx.x % kTile, lj = (threadIdx.x / kTile) % kTile, li = threadIdx.x / (kTile * kTile);
+  const int gi[3] = {ti * kTile + li, tj * kTile + lj, tk * kTile + lk};
+  const int N[3] = {g.nx, g.ny, g.nz};
+  const double x[3] = {gi[0] * pm.dx, gi[1] * pm.dx, gi[2] * pm.dx};
+  const int s0 = tile_start[tile], cnt = tile_count[tile];
+  const double rcut = pm.radius + pm.hxi;
+  double ph = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  for (int q = 0; q < cnt; ++q) {
+    const int n = tile_list[s0 + q];
+    int cell[3];
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int o = (gi[d] - pxint[3 * n + d]) % N[d];
+      if (o < 0) o += N[d];
+      if (o >= N[d] / 2) o -= N[d];
+      cell[d] = o;
+      in = in && (o <= pm.nh && o >= -pm.nh);
+    }
+    if (!in) continue;
+    const double xp[3] = {P(pb, pm.np, P_X, n), P(pb, pm.np, P_X + 1, n), P(pb, pm.np, P_X + 2, n)};
+    double x12[3];
+    const double dist = distance0(g, x, xp, x12);
+    if (!(dist < rcut)) continue;
+    const double dphi = phi_of(pm, dist);
+    ph += dphi;
+    if (sw_up) {
+      double vr[3] = {0.0, 0.0, 0.0};
+      if (pm.rotation) {
+        double r[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) r[d] = (double)cell[d] * pm.dx - pres[3 * n + d];
+        const double w0 = P(pb, pm.np, P_OMEGA, n), w1 = P(pb, pm.np, P_OMEGA + 1, n), w2 = P(pb, pm.np, P_OMEGA + 2, n);
+        vr[0] = w1 * r[2] - w2 * r[1];
+        vr[1] = w2 * r[0] - w0 * r[2];
+        vr[2] = w0 * r[1] - w1 * r[0];
+      }
+      u0 += (P(pb, pm.np, P_V, n) + vr[0]) * dphi;
+      u1 += (P(pb, pm.np, P_V + 1, n) + vr[1]) * dphi;
+      u2 += (P(pb, pm.np, P_V + 2, n) + vr[2]) * dphi;
+    }
+  }
+  if (sw_up && ph > 1.) {  // make_phi.cxx:280-297
+    const double inv = 1. / ph;
+    u0 *= inv;
+    u1 *= inv;
+    u2 *= inv;
+    ph = 1.;
+  }
+  const long im = ((long)gi[0] * g.ny + gi[1]) * g.nz_ + gi[2];
+  phi[im] = ph;
+  if (sw_up) {
+    up0[im] = u0;
+    up1[im] = u1;
+    up2[im] = u2;
+  }
+  if (gi[2] == g.nz - 1) {  // padding slots, zero as after Reset_phi_u
+    phi[im + 1] = 0.0;
+    phi[im + 2] = 0.0;
+    if (sw_up) {
+      up0[im + 1] = up0[im + 2] = 0.0;
+      up1[im + 1] = up1[im + 2] = 0.0;
+      up2[im + 1] = up2[im + 2] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Calc_f_hydro_correct_precision: one wavefront per particle
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void f_hydro_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ pxint,
+                                                      const double *__restrict__ pres,
+                                                      const int *__restrict__ stencil, int nst,
+                                                      const double *__restrict__ u0, const double *__restrict__ u1,
+                                                      const double *__restrict__ u2, double dmy) {
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (n >= pm.np) return;
+  const int N[3] = {g.nx, g.ny, g.nz};
+  const int xi[3] = {pxint[3 * n], pxint[3 * n + 1], pxint[3 * n + 2]};
+  const double res[3] = {pres[3 * n], pres[3 * n + 1], pres[3 * n + 2]};
+  const double xp[3] = {P(pb, pm.np, P_X, n), P(pb, pm.np, P_X + 1, n), P(pb, pm.np, P_X + 2, n)};
+  const double vp[3] = {P(pb, pm.np, P_V, n), P(pb, pm.np, P_V + 1, n), P(pb, pm.np, P_V + 2, n)};
+  const double wp[3] = {P(pb, pm.np, P_OMEGA, n), P(pb, pm.np, P_OMEGA + 1, n), P(pb, pm.np, P_OMEGA + 2, n)};
+  const double rcut = pm.radius + pm.hxi;
+  double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0;
+  for (int s = lane; s < nst; s += 64) {
+    int rm[3];
+    double r[3], x[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int cd = stencil[3 * s + d];
+      rm[d] = (xi[d] + cd + N[d]) % N[d];
+      r[d] = (double)cd * pm.dx - res[d];
+      x[d] = rm[d] * pm.dx;
+    }
+    double x12[3];
+    const double dist = distance0(g, x, xp, x12);
+    if (!(dist < rcut)) continue;  // phi = 0: adds nothing
+    const double dphi = phi_of(pm, dist);
+    double vr[3] = {0.0, 0.0, 0.0};
+    if (pm.rotation) {
+      vr[0] = wp[1] * r[2] - wp[2] * r[1];
+      vr[1] = wp[2] * r[0] - wp[0] * r[2];
+      vr[2] = wp[0] * r[1] - wp[1] * r[0];
+    }
+    const long im = ((long)rm[0] * g.ny + rm[1]) * g.nz_ + rm[2];
+    const double f0 = ((vp[0] + vr[0]) - u0[im]) * dphi;
+    const double f1 = ((vp[1] + vr[1]) - u1[im]) * dphi;
+    const double f2 = ((vp[2] + vr[2]) - u2[im]) * dphi;
+    fx += f0;
+    fy += f1;
+    fz += f2;
+    tx += (r[1] * f2 - r[2] * f1);
+    ty += (r[2] * f0 - r[0] * f2);
+    tz += (r[0] * f1 - r[1] * f0);
+  }
+  fx = wave_sum(fx);
+  fy = wave_sum(fy);
+  fz = wave_sum(fz);
+  tx = wave_sum(tx);
+  ty = wave_sum(ty);
+  tz = wave_sum(tz);
+  if (lane == 0) {
+    PR(pb, pm.np, P_F_HYDRO, n) = (dmy * fx) * 1.0;
+    PR(pb, pm.np, P_F_HYDRO + 1, n) = (dmy * fy) * 1.0;
+    PR(pb, pm.np, P_F_HYDRO + 2, n) = (dmy * fz) * 1.0;
+    for (int d = 0; d < 3; ++d) PR(pb, pm.np, P_F_SLIP + d, n) = 0.0;
+    if (pm.rotation) {
+      PR(pb, pm.np, P_TORQUE_HYDRO, n) = (dmy * tx) * 1.0;
+      PR(pb, pm.np, P_TORQUE_HYDRO + 1, n) = (dmy * ty) * 1.0;
+      PR(pb, pm.np, P_TORQUE_HYDRO + 2, n) = (dmy * tz) * 1.0;
+      for (int d = 0; d < 3; ++d) PR(pb, pm.np, P_TORQUE_SLIP + d, n) = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// LJ / WCA on linked cells, gravity
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int cell_coord(double x, double L, int nc) {
+  int c = (int)(x / (L / nc));
+  return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__global__ void cell_count_kernel(Grid g, PModel pm, const double *__restrict__ pb, int cx, int cy, int cz,
+                                  int *cell_count, int *pcell) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= pm.np) return;
+  const int c = (cell_coord(P(pb, pm.np, P_X, n), g.lx, cx) * cy + cell_coord(P(pb, pm.np, P_X + 1, n), g.ly, cy)) * cz +
+                cell_coord(P(pb, pm.np, P_X + 2, n), g.lz, cz);
+  pcell[n] = c;
+  atomicAdd(&cell_count[c], 1);
+}
+
+__global__ void cell_fill_kernel(PModel pm, const int *__restrict__ pcell, int *cell_cursor, int *cell_list) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= pm.np) return;
+  const int slot = atomicAdd(&cell_cursor[pcell[n]], 1);
+  cell_list[slot] = n;
+}
+
+// Lennard_Jones_f, interaction.h:156-181.
+__device__ __forceinline__ double lj_f(const PModel &pm, double x) {
+  double dmy = pm.lj_dia / x;
+  double coeff;
+  if (pm.lj_powers == 0) {
+    dmy = sq(dmy) * sq(dmy) * sq(dmy);
+    coeff = 24. * pm.epsilon;
+  } else if (pm.lj_powers == 1) {
+    dmy = sq(dmy) * sq(dmy) * sq(dmy) * sq(dmy) * sq(dmy) * sq(dmy);
+    coeff = 48. * pm.epsilon;
+  } else {
+    dmy = sq(dmy) * sq(dmy) * sq(dmy) * sq(dmy) * sq(dmy) * sq(dmy) * sq(dmy) * sq(dmy) * sq(dmy);
+    coeff = 72. * pm.epsilon;
+  }
+  return coeff / sq(x) * (2.0 * sq(dmy) - dmy);
+}
+
+__global__ void force_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ spec,
+                             const int *__restrict__ pcell, const int *__restrict__ cell_start,
+                             const int *__restrict__ cell_count, const int *__restrict__ cell_list, int cx, int cy,
+                             int cz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= pm.np) return;
+  double F[3] = {0.0, 0.0, 0.0};
+  if (pm.lj_on) {
+    const double xi[3] = {P(pb, pm.np, P_X, i), P(pb, pm.np, P_X + 1, i), P(pb, pm.np, P_X + 2, i)};
+    const int ci = pcell[i];
+    const int c[3] = {ci / (cy * cz), (ci / cz) % cy, ci % cz};
+    const int nc[3] = {cx, cy, cz};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = nc[d] >= 3 ? -1 : 0;
+      hi[d] = nc[d] >= 2 ? 1 : 0;
+    }
+    for (int a = lo[0]; a <= hi[0]; ++a)
+      for (int b = lo[1]; b <= hi[1]; ++b)
+        for (int e = lo[2]; e <= hi[2]; ++e) {
+          const int nb = (((c[0] + a + cx) % cx) * cy + (c[1] + b + cy) % cy) * cz + (c[2] + e + cz) % cz;
+          const int s0 = cell_start[nb], cnt = cell_count[nb];
+          for (int q = 0; q < cnt; ++q) {
+            const int j = cell_list[s0 + q];
+            if (j == i) continue;
+            const double xj[3] = {P(pb, pm.np, P_X, j), P(pb, pm.np, P_X + 1, j), P(pb, pm.np, P_X + 2, j)};
+            double rv[3];
+            const double r = distance0(g, xi, xj, rv);
+            if (r < pm.lj_cutoff) {
+              const double ljf = lj_f(pm, r);
+              const double cap = 1.79769313486231570815e+308 / r;
+              const double dmy = cap < ljf ? cap : ljf;
+#pragma unroll
+              for (int d = 0; d < 3; ++d) F[d] += dmy * -rv[d];
+            }
+          }
+        }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) PR(pb, pm.np, P_FR + d, i) += F[d];
+  if (pm.g_on_fluid != 0.0) PR(pb, pm.np, P_FR + pm.g_direction, i) -= pm.g_on_fluid * (pm.mass_ratio[spec[i]] - 1.0);
+}
+
+// ---------------------------------------------------------------------
+// integrators (particle_solver.cxx, particle_rotation_solver.h)
+// ---------------------------------------------------------------------
+struct Q {
+  double s, v0, v1, v2;
+};
+
+__device__ __forceinline__ Q qprod(const Q &a, const Q &b, double alpha) {
+  Q r;
+  r.s = alpha * (a.s * b.s - (a.v0 * b.v0 + a.v1 * b.v1 + a.v2 * b.v2));
+  r.v0 = alpha * (a.s * b.v0 + b.s * a.v0 + a.v1 * b.v2 - a.v2 * b.v1);
+  r.v1 = alpha * (a.s * b.v1 + b.s * a.v1 + a.v2 * b.v0 - a.v0 * b.v2);
+  r.v2 = alpha * (a.s * b.v2 + b.s * a.v2 + a.v0 * b.v1 - a.v1 * b.v0);
+  return r;
+}
+__device__ __forceinline__ Q qnorm(Q q) {
+  double nn = sqrt(q.s * q.s + q.v0 * q.v0 + q.v1 * q.v1 + q.v2 * q.v2);
+  nn = 1.0 / nn;
+  q.s *= nn;
+  q.v0 *= nn;
+  q.v1 *= nn;
+  q.v2 *= nn;
+  return q;
+}
+__device__ __forceinline__ Q qload(const double *pb, int np, int slot, int n) {
+  return Q{P(pb, np, slot, n), P(pb, np, slot + 1, n), P(pb, np, slot + 2, n), P(pb, np, slot + 3, n)};
+}
+__device__ __forceinline__ void qstore(double *pb, int np, int slot, int n, const Q &q) {
+  PR(pb, np, slot, n) = q.s;
+  PR(pb, np, slot + 1, n) = q.v0;
+  PR(pb, np, slot + 2, n) = q.v1;
+  PR(pb, np, slot + 3, n) = q.v2;
+}
+// rigid_body_rotation SPACE2BODY (rigid_body.cxx:10-34): conj(q) (0,v) q.
+__device__ __forceinline__ void space2body(double *out, const double *v, const Q &q) {
+  Q qx{0.0, v[0], v[1], v[2]};
+  Q qc{q.s, -q.v0, -q.v1, -q.v2};
+  qx = qprod(qx, q, 1.0);
+  qc = qprod(qc, qx, 1.0);
+  out[0] = qc.v0;
+  out[1] = qc.v1;
+  out[2] = qc.v2;
+}
+
+__global__ void position_kernel(Grid g, PModel pm, double *pb, double dt, double hdt, int euler) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= pm.np) return;
+  const double L[3] = {g.lx, g.ly, g.lz};
+  for (int d = 0; d < 3; ++d) {
+    const double x = P(pb, pm.np, P_X + d, n);
+    PR(pb, pm.np, P_X_PREV + d, n) = x;
+    const double v = P(pb, pm.np, P_V + d, n);
+    const double delta = euler ? dt * v : hdt * (3.0 * v - P(pb, pm.np, P_V_OLD + d, n));
+    PR(pb, pm.np, P_X_NOPBC + d, n) += delta;
+    PR(pb, pm.np, P_X + d, n) = fmod((x + delta) + L[d], L[d]);  // PBC, periodic_boundary.h:9-15
+  }
+  if (pm.rotation) {
+    const Q q = qload(pb, pm.np, P_Q, n);
+    Q dq;
+    if (euler) {  // MD_solver_orientation_Euler
+      const Q qw{0.0, P(pb, pm.np, P_OMEGA, n), P(pb, pm.np, P_OMEGA + 1, n), P(pb, pm.np, P_OMEGA + 2, n)};
+      dq = qprod(qw, q, 0.5);
+    } else {  // MD_solver_orientation_AB2 (Simo & Wong), step hdt
+      const double w[3] = {P(pb, pm.np, P_OMEGA, n), P(pb, pm.np, P_OMEGA + 1, n), P(pb, pm.np, P_OMEGA + 2, n)};
+      const double wo[3] = {P(pb, pm.np, P_OMEGA_OLD, n), P(pb, pm.np, P_OMEGA_OLD + 1, n),
+                            P(pb, pm.np, P_OMEGA_OLD + 2, n)};
+      double wb[3], wbo[3];
+      space2body(wb, w, q);
+      space2body(wbo, wo, qload(pb, pm.np, P_Q_OLD, n));
+      const Q qw{0.0, 3.0 * wb[0] - wbo[0], 3.0 * wb[1] - wbo[1], 3.0 * wb[2] - wbo[2]};
+      dq = qprod(q, qw, 0.5);
+    }
+    const double h = euler ? dt : hdt;
+    Q qn{q.s + dq.s * h, q.v0 + dq.v0 * h, q.v1 + dq.v1 * h, q.v2 + dq.v2 * h};
+    qstore(pb, pm.np, P_Q_OLD, n, q);
+    qstore(pb, pm.np, P_Q, n, qnorm(qn));
+  }
+}
+
+// MD_solver_velocity_Euler / _AB2_hydro + reset_Forces.
+__global__ void velocity_kernel(PModel pm, double *pb, const int *__restrict__ spec, double dt, double hdt,
+                                int euler) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= pm.np) return;
+  const int s = spec[n];
+  for (int d = 0; d < 3; ++d) {
+    const double v = P(pb, pm.np, P_V + d, n), w = P(pb, pm.np, P_OMEGA + d, n);
+    const double fh = P(pb, pm.np, P_F_HYDRO + d, n), fr = P(pb, pm.np, P_FR + d, n);
+    const double th = P(pb, pm.np, P_TORQUE_HYDRO + d, n);
+    PR(pb, pm.np, P_V_OLD + d, n) = v;
+    PR(pb, pm.np, P_OMEGA_OLD + d, n) = w;
+    if (euler) {
+      const double dmy = dt * pm.imass[s], dmy_rot = dt * pm.imoi[s];
+      PR(pb, pm.np, P_V + d, n) = v + (dmy * (fh + fr + 0.0));
+      PR(pb, pm.np, P_OMEGA + d, n) = w + (dmy_rot * (th + 0.0));
+    } else {
+      const double dmy = hdt * pm.imass[s], dmy_rot = hdt * pm.imoi[s];
+      PR(pb, pm.np, P_V + d, n) = v + dmy * (2.0 * (fh + 0.0) + fr + P(pb, pm.np, P_FR_PREV + d, n));
+      PR(pb, pm.np, P_OMEGA + d, n) = w + dmy_rot * 2.0 * (th + 0.0);
+    }
+    // reset_Forces, particle_solver.cxx:11-34
+    PR(pb, pm.np, P_FR_PREV + d, n) = fr;
+    PR(pb, pm.np, P_FR + d, n) = 0.0;
+    PR(pb, pm.np, P_F_HYDRO_PREV + d, n) = fh;
+    PR(pb, pm.np, P_F_HYDRO + d, n) = 0.0;
+    PR(pb, pm.np, P_F_SLIP_PREV + d, n) = P(pb, pm.np, P_F_SLIP + d, n);
+    PR(pb, pm.np, P_F_SLIP + d, n) = 0.0;
+    PR(pb, pm.np, P_TORQUE_HYDRO_PREV + d, n) = th;
+    PR(pb, pm.np, P_TORQUE_HYDRO + d, n) = 0.0;
+    PR(pb, pm.np, P_TORQUE_SLIP_PREV + d, n) = P(pb, pm.np, P_TORQUE_SLIP + d, n);
+    PR(pb, pm.np, P_TORQUE_SLIP + d, n) = 0.0;
+  }
+}
+
+// Add_random_force_thermostat (fluct.cxx:10-36): noise[6n..6n+5] holds the
+// particle's three Gauss2 pairs, drawn on the host in the reference order.
+__global__ void thermostat_kernel(PModel pm, double *pb, const int *__restrict__ spec, const double *__restrict__ noise,
+                                  double nv, double no) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= pm.np) return;
+  const int s = spec[n];
+  const double imass = pm.imass[s], imoi = pm.imoi[s];
+  for (int d = 0; d < 3; ++d) {
+    PR(pb, pm.np, P_V + d, n) += noise[6 * n + d] * nv * imass;
+    if (pm.rotation) PR(pb, pm.np, P_OMEGA + d, n) += noise[6 * n + d + 3] * no * imoi;
+  }
+}
+
+inline unsigned blocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------
+int p_position(spm_ctx *c, int euler) {
+  if (c->np == 0) return 0;
+  tbeg(c);
+  hipLaunchKernelGGL(position_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->prm.dt, 0.5 * c->prm.dt, euler);
+  tend(c, SPM_K_PARTICLE, 8.0 * 40 * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int p_build_tiles(spm_ctx *c) {
+  if (c->np == 0) return 0;
+  tbeg(c);
+  hipLaunchKernelGGL(izero_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_count, c->ntiles);
+  hipLaunchKernelGGL(anchor_count_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->pxint, c->pres, c->tiles[0], c->tiles[1], c->tiles[2], c->tile_count);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->tile_count, c->tile_start, c->tile_cursor,
+                     c->ntiles);
+  hipLaunchKernelGGL(tile_fill_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pxint,
+                     c->tiles[0], c->tiles[1], c->tiles[2], c->tile_cursor, c->tile_list);
+  hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_start,
+                     c->tile_count, c->tile_list, c->ntiles);
+  tend(c, SPM_K_PARTICLE, 16.0 * c->ntiles);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int p_make_phi(spm_ctx *c, int sw_up) {
+  if (c->np == 0) {
+    int rc = k_zero(c, c->phi, c->nfield);
+    if (rc) return rc;
+    if (sw_up)
+      for (int d = 0; d < 3; ++d)
+        if ((rc = k_zero(c, c->up[d], c->nfield))) return rc;
+    return 0;
+  }
+  int rc = p_build_tiles(c);
+  if (rc) return rc;
+  tbeg(c);
+  hipLaunchKernelGGL(make_phi_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf, c->pxint,
+                     c->pres, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2],
+                     c->phi, c->up[0], c->up[1], c->up[2], sw_up);
+  // writes phi (and up) once at every grid point
+  tend(c, SPM_K_MAKE_PHI, (sw_up ? 4.0 : 1.0) * 8.0 * (double)c->nfield);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int p_calc_f_hydro(spm_ctx *c) {
+  if (c->np == 0) return 0;
+  // anchors (x_int, residue) for the current positions
+  int rc = p_build_tiles(c);
+  if (rc) return rc;
+  const double dmy = (-c->pm.dx3 * c->pm.rho) / c->prm.dt;
+  tbeg(c);
+  hipLaunchKernelGGL(f_hydro_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->pxint, c->pres, c->stencil, c->n_stencil, c->u[0], c->u[1], c->u[2], dmy);
+  // gathers u (3 doubles) at every stencil point of every particle
+  tend(c, SPM_K_F_HYDRO, 24.0 * (double)c->n_stencil * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int p_force(spm_ctx *c) {
+  if (c->np == 0) return 0;
+  tbeg(c);
+  if (c->pm.lj_on) {
+    hipLaunchKernelGGL(izero_kernel, dim3(blocks(c->ncells, 256)), dim3(256), 0, c->stream, c->cell_count, c->ncells);
+    hipLaunchKernelGGL(cell_count_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf,
+                       c->cells[0], c->cells[1], c->cells[2], c->cell_count, c->pcell);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->cell_count, c->cell_start, c->cell_cursor,
+                       c->ncells);
+    hipLaunchKernelGGL(cell_fill_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pcell,
+                       c->cell_cursor, c->cell_list);
+    hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ncells, 256)), dim3(256), 0, c->stream, c->cell_start,
+                       c->cell_count, c->cell_list, c->ncells);
+  }
+  hipLaunchKernelGGL(force_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
+                     c->pcell, c->cell_start, c->cell_count, c->cell_list, c->cells[0], c->cells[1], c->cells[2]);
+  tend(c, SPM_K_PARTICLE, 8.0 * 6 * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int p_velocity(spm_ctx *c, int euler) {
+  if (c->np == 0) return 0;
+  int rc = p_force(c);
+  if (rc) return rc;
+  tbeg(c);
+  hipLaunchKernelGGL(velocity_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pbuf, c->spec,
+                     c->prm.dt, 0.5 * c->prm.dt, euler);
+  tend(c, SPM_K_PARTICLE, 8.0 * 60 * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int p_thermostat_apply(spm_ctx *c, const double *noise_dev) {
+  if (c->np == 0) return 0;
+  tbeg(c);
+  hipLaunchKernelGGL(thermostat_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pbuf, c->spec,
+                     noise_dev, c->noise_v, c->noise_o);
+  tend(c, SPM_K_PARTICLE, 8.0 * 18 * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+}  // namespace spm

@@ -1,0 +1,723 @@
+// spm_api.cpp -- the C ABI of include/spm.h: context lifetime, state
+// transfer in the reference layouts, and the solver call surface that
+// sequences the HIP kernels exactly as Time_evolution_hydro does
+// (sp_3d_ns.cxx:61-212).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "spm_internal.h"
+
+using namespace spm;
+
+namespace spm {
+
+static size_t next_event(spm_ctx *c) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return (size_t)-1;
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_used++;
+}
+
+void tbeg(spm_ctx *c) {
+  if (!c->timing) return;
+  const size_t e = next_event(c);
+  if (e == (size_t)-1) return;
+  (void)hipEventRecord(c->ev_pool[e], c->stream);
+  c->launches.push_back({-1, e, e, 0.0});
+}
+
+void tend(spm_ctx *c, int family, double bytes) {
+  if (!c->timing || c->launches.empty() || c->launches.back().family != -1) return;
+  const size_t e = next_event(c);
+  if (e == (size_t)-1) return;
+  (void)hipEventRecord(c->ev_pool[e], c->stream);
+  c->launches.back().family = family;
+  c->launches.back().e1 = e;
+  c->launches.back().bytes = bytes;
+}
+
+// Folds the recorded launches into per-family totals and frees the events
+// for reuse (the stream must be idle).
+static int fold_timing(spm_ctx *c) {
+  for (const auto &l : c->launches) {
+    if (l.family < 0) continue;
+    float ms = 0.f;
+    SPM_HIP(c, hipEventElapsedTime(&ms, c->ev_pool[l.e0], c->ev_pool[l.e1]));
+    c->fam_ms[l.family] += ms;
+    c->fam_bytes[l.family] += l.bytes;
+    c->fam_count[l.family] += 1;
+  }
+  c->launches.clear();
+  c->ev_used = 0;
+  return 0;
+}
+
+}  // namespace spm
+
+namespace {
+
+// ---------------------------------------------------------------------
+// MT19937 (mt19937ar.c:65-171) and the polar Gaussian pair (fluct.h:48-60),
+// on the host: the thermal-noise stream is sequential with rejection and
+// uses the host libm's log/sqrt, so drawing it here keeps it identical to
+// the reference's stream.
+// ---------------------------------------------------------------------
+void mt_seed(spm_ctx *c, unsigned long s) {
+  c->mt[0] = (uint32_t)(s & 0xffffffffUL);
+  for (c->mti = 1; c->mti < SPM_MT_N; c->mti++)
+    c->mt[c->mti] = (uint32_t)(1812433253UL * (c->mt[c->mti - 1] ^ (c->mt[c->mti - 1] >> 30)) + c->mti);
+}
+
+uint32_t mt_next(spm_ctx *c) {
+  constexpr int N = SPM_MT_N, M = 397;
+  static const uint32_t mag01[2] = {0x0U, 0x9908b0dfU};
+  uint32_t y;
+  if (c->mti >= N) {
+    if (c->mti == N + 1) mt_seed(c, 5489UL);
+    int kk;
+    for (kk = 0; kk < N - M; kk++) {
+      y = (c->mt[kk] & 0x80000000U) | (c->mt[kk + 1] & 0x7fffffffU);
+      c->mt[kk] = c->mt[kk + M] ^ (y >> 1) ^ mag01[y & 1U];
+    }
+    for (; kk < N - 1; kk++) {
+      y = (c->mt[kk] & 0x80000000U) | (c->mt[kk + 1] & 0x7fffffffU);
+      c->mt[kk] = c->mt[kk + (M - N)] ^ (y >> 1) ^ mag01[y & 1U];
+    }
+    y = (c->mt[N - 1] & 0x80000000U) | (c->mt[0] & 0x7fffffffU);
+    c->mt[N - 1] = c->mt[M - 1] ^ (y >> 1) ^ mag01[y & 1U];
+    c->mti = 0;
+  }
+  y = c->mt[c->mti++];
+  y ^= (y >> 11);
+  y ^= (y << 7) & 0x9d2c5680U;
+  y ^= (y << 15) & 0xefc60000U;
+  y ^= (y >> 18);
+  return y;
+}
+
+inline double mt_real3(spm_ctx *c) { return (((double)mt_next(c)) + 0.5) * (1.0 / 4294967296.0); }
+
+void gauss2(spm_ctx *c, double *out) {
+  double x1, x2, rsq;
+  do {
+    x1 = 2. * mt_real3(c) - 1.;
+    x2 = 2. * mt_real3(c) - 1.;
+    rsq = x1 * x1 + x2 * x2;
+  } while (rsq >= 1.0 || rsq == 0.0);
+  const double factor = std::sqrt(-2. * std::log(rsq) / rsq);
+  out[0] = factor * x1;
+  out[1] = factor * x2;
+}
+
+bool is_pow2(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+double *field_ptr(spm_ctx *c, int f) {
+  switch (f) {
+    case SPM_FIELD_ZETA0: return c->zeta[0];
+    case SPM_FIELD_ZETA1: return c->zeta[1];
+    case SPM_FIELD_U0: return c->u[0];
+    case SPM_FIELD_U1: return c->u[1];
+    case SPM_FIELD_U2: return c->u[2];
+    case SPM_FIELD_UP0: return c->up[0];
+    case SPM_FIELD_UP1: return c->up[1];
+    case SPM_FIELD_UP2: return c->up[2];
+    case SPM_FIELD_PHI: return c->phi;
+    case SPM_FIELD_F0: return c->f[0];
+    case SPM_FIELD_F1: return c->f[1];
+    case SPM_FIELD_F2: return c->f[2];
+    default: return nullptr;
+  }
+}
+
+int fix_mask(const spm_ctx *c) {
+  return (c->prm.fix_cell[0] ? 1 : 0) | (c->prm.fix_cell[1] ? 2 : 0) | (c->prm.fix_cell[2] ? 4 : 0);
+}
+
+#define RC(x)                  \
+  do {                         \
+    int rc_ = (x);             \
+    if (rc_) return rc_;       \
+  } while (0)
+
+// ---------------------------------------------------------------------
+// the step pieces
+// ---------------------------------------------------------------------
+// NS_solver_slavedEuler (fluid_solver.cxx:328-354) with
+// Zeta_k2advection_k (operate_omega.cxx:102-113).
+int ns_solver(spm_ctx *c) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  double *const p5[5] = {c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]};
+  RC(k_truncate(c, c->zeta[0]));
+  RC(k_truncate(c, c->zeta[1]));
+  RC(k_zeta_k2u_k(c, c->zeta[0], c->zeta[1], c->u[0], c->u[1], c->u[2], 0));
+  RC(fft_c2r_many(c, u3, 3));
+  RC(k_products(c, c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]));
+  RC(fft_r2c_many(c, p5, 5));
+  RC(k_advection_contract(c, c->u[0], c->u[1], c->u[2], c->work[0], c->work[1], c->work[0], c->work[1]));
+  const double nu = c->prm.eta * (1. / c->prm.rho);
+  RC(k_slaved_euler(c, c->zeta[0], c->zeta[1], c->work[0], c->work[1], nu, c->prm.dt));
+  return 0;
+}
+
+// sp_3d_ns.cxx:67-79.
+int zeta_k2u(spm_ctx *c, int mask) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  RC(k_truncate(c, c->zeta[0]));
+  RC(k_truncate(c, c->zeta[1]));
+  RC(k_zeta_k2u_k(c, c->zeta[0], c->zeta[1], c->u[0], c->u[1], c->u[2], mask));
+  RC(fft_c2r_many(c, u3, 3));
+  return 0;
+}
+
+// Make_f_particle_dt_sole + Add_f_particle (f_particle.h:71-99).
+int make_f_sole_add(spm_ctx *c) {
+  double *const f3[3] = {c->f[0], c->f[1], c->f[2]};
+  RC(k_make_f(c));
+  RC(fft_r2c_many(c, f3, 3));
+  RC(k_solenoidal_trunc(c, c->f[0], c->f[1], c->f[2]));
+  RC(fft_c2r_many(c, f3, 3));
+  RC(k_add_f(c));
+  return 0;
+}
+
+// U2zeta_k (operate_omega.h:186-189).
+int u2zeta_k(spm_ctx *c) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  RC(fft_r2c_many(c, u3, 3));
+  RC(k_u_k2zeta_k(c, c->u[0], c->u[1], c->u[2], c->zeta[0], c->zeta[1]));
+  return 0;
+}
+
+int reset_phi_u(spm_ctx *c) {
+  RC(k_zero(c, c->phi, c->nfield));
+  for (int d = 0; d < 3; ++d) RC(k_zero(c, c->up[d], c->nfield));
+  return 0;
+}
+
+// Add_random_force_thermostat (fluct.cxx:10-36): draw 6 normals per
+// particle on the host, ship them, apply on the device.  Two pinned
+// staging slots let the host draw step n+1 while step n is in flight.
+int thermostat(spm_ctx *c) {
+  if (!(c->prm.kbt > 0.) || c->np == 0) return 0;
+  const int s = c->noise_slot;
+  SPM_HIP(c, hipEventSynchronize(c->noise_done[s]));
+  double *h = c->noise_host[s];
+  for (int n = 0; n < c->np; ++n) {
+    gauss2(c, h + 6 * n);
+    gauss2(c, h + 6 * n + 2);
+    gauss2(c, h + 6 * n + 4);
+  }
+  SPM_HIP(c, hipMemcpyAsync(c->noise_dev, h, sizeof(double) * 6 * c->np, hipMemcpyHostToDevice, c->stream));
+  SPM_HIP(c, hipEventRecord(c->noise_done[s], c->stream));
+  c->noise_slot ^= 1;
+  return p_thermostat_apply(c, c->noise_dev);
+}
+
+// Time_evolution_hydro (sp_3d_ns.cxx:61-212), Navier_Stokes with free
+// spherical particles, no slip.
+int time_evolution_hydro(spm_ctx *c) {
+  RC(ns_solver(c));
+  RC(zeta_k2u(c, fix_mask(c)));
+  RC(p_position(c, c->ts == 0));
+  RC(p_calc_f_hydro(c));  // Reset_phi_u is skipped: make_phi below rewrites phi/up
+  RC(p_velocity(c, c->ts == 0));
+  RC(thermostat(c));
+  RC(p_make_phi(c, 1));
+  RC(make_f_sole_add(c));
+  RC(u2zeta_k(c));
+  return 0;
+}
+
+void free_ctx(spm_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (double *p : {c->zeta[0], c->zeta[1], c->u[0], c->u[1], c->u[2], c->up[0], c->up[1], c->up[2], c->phi, c->f[0],
+                    c->f[1], c->f[2], c->work[0], c->work[1], c->work[2], c->work[3], c->work[4], c->uk_dc, c->pbuf,
+                    c->pres, c->noise_dev})
+    if (p) (void)hipFree(p);
+  for (int *p : {c->spec, c->stencil, c->pxint, c->tile_count, c->tile_start, c->tile_cursor, c->tile_list,
+                 c->cell_count, c->cell_start, c->cell_cursor, c->cell_list, c->pcell, c->flag})
+    if (p) (void)hipFree(p);
+  for (double2 *p : {c->tw_x, c->tw_y, c->tw_z, c->tw_rz})
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (int s = 0; s < 2; ++s) {
+    if (c->noise_host[s]) (void)hipHostFree(c->noise_host[s]);
+    if (c->noise_done[s]) (void)hipEventDestroy(c->noise_done[s]);
+  }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+template <typename P>
+int dalloc(spm_ctx *c, P **p, size_t count) {
+  SPM_HIP(c, hipMalloc((void **)p, sizeof(P) * (count > 0 ? count : 1)));
+  SPM_HIP(c, hipMemsetAsync(*p, 0, sizeof(P) * (count > 0 ? count : 1), c->stream));
+  return 0;
+}
+
+int setup(spm_ctx *c) {
+  const spm_params &p = c->prm;
+  Grid &g = c->grid;
+  g.nx = p.nx;
+  g.ny = p.ny;
+  g.nz = p.nz;
+  g.hnz = p.nz / 2 + 1;
+  g.nz_ = 2 * g.hnz;
+  g.trn_x = (p.nx + 2) / 3;
+  g.trn_y = (p.ny + 2) / 3;
+  g.trn_z = (p.nz + 2) / 3;
+  g.dx = p.dx;
+  g.lx = p.nx * p.dx;
+  g.ly = p.ny * p.dx;
+  g.lz = p.nz * p.dx;
+  const double PI2 = 2.0 * M_PI;
+  g.wave_x = PI2 / g.lx;
+  g.wave_y = PI2 / g.ly;
+  g.wave_z = PI2 / g.lz;
+  c->nfield = (size_t)g.nx * g.ny * g.nz_;
+
+  PModel &m = c->pm;
+  m.np = p.n_particles;
+  m.rotation = p.rotation ? 1 : 0;
+  m.dx = p.dx;
+  m.radius = p.a * p.dx;
+  m.hxi = (p.a_xi * p.dx) * 0.5;
+  m.max_radius = m.radius + m.hxi + 1.7321 * p.dx;
+  m.nh = (int)p.a + (int)p.a_xi + 2;
+  m.rho = p.rho;
+  m.dx3 = p.dx * p.dx * p.dx;
+  const double R = m.radius;
+  for (int s = 0; s < SPM_MAX_SPECIES; ++s) {
+    const double mr = s < p.n_species ? p.mass_ratio[s] : 1.0;
+    const double mass = (mr * p.rho) * 4. / 3. * M_PI * (R * R * R);
+    const double moi = 2. / 5. * mass * (R * R);
+    m.mass_ratio[s] = mr;
+    m.imass[s] = 1. / mass;
+    m.imoi[s] = 1. / moi;
+  }
+  m.lj_dia = 2.0 * R;  // SIGMA (input.cxx:1340, 400-403)
+  double a_r_cutoff = 0.0;
+  if (p.lj_truncate > 0)
+    a_r_cutoff = p.lj_powers == 0 ? std::pow(2., 1. / 6.) : p.lj_powers == 1 ? std::pow(2., 1. / 12.)
+                                                                           : std::pow(2., 1. / 18.);
+  else if (p.lj_truncate == 0) {
+    const int nmax = std::max(p.nx, std::max(p.ny, p.nz));  // Nmin as set at input.cxx:1406
+    a_r_cutoff = std::min(nmax * p.dx * .5 / m.lj_dia, 2.5);
+  }
+  m.lj_cutoff = a_r_cutoff * m.lj_dia;
+  m.lj_on = (p.lj_truncate >= 0 && m.lj_cutoff > 0.0) ? 1 : 0;
+  m.epsilon = p.epsilon;
+  m.lj_powers = p.lj_powers;
+  m.g_on_fluid = p.g * p.rho * 4. / 3. * M_PI * (R * R) * R;
+  m.g_direction = p.g_direction;
+  c->np = m.np;
+
+  // thermostat amplitudes (fluct.cxx:11-18)
+  const double zeta_drag = 6. * M_PI * p.eta * R;
+  const double zeta_drag_rot = 8. * M_PI * p.eta * (R * R * R);
+  c->noise_v = 1.439 * std::sqrt(zeta_drag * p.dt * p.kbt * p.alpha_v);
+  c->noise_o = 1.22 * std::sqrt(zeta_drag_rot * p.dt * p.kbt * p.alpha_o);
+
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipStreamCreate(&c->stream));
+
+  for (int d = 0; d < 2; ++d) RC(dalloc(c, &c->zeta[d], c->nfield));
+  for (int d = 0; d < 3; ++d) {
+    RC(dalloc(c, &c->u[d], c->nfield));
+    RC(dalloc(c, &c->up[d], c->nfield));
+    RC(dalloc(c, &c->f[d], c->nfield));
+  }
+  RC(dalloc(c, &c->phi, c->nfield));
+  for (int d = 0; d < 2; ++d) RC(dalloc(c, &c->work[d], c->nfield));
+  RC(dalloc(c, &c->uk_dc, 3));
+  RC(dalloc(c, &c->flag, 1));
+  RC(fft_init_tables(c));
+
+  // Particle_domain (profile.cxx:25-74)
+  {
+    std::vector<int> st;
+    const int nh = m.nh;
+    for (int i = -nh; i <= nh; i++)
+      for (int j = -nh; j <= nh; j++)
+        for (int k = -nh; k <= nh; k++) {
+          const double dmy = std::sqrt(((double)i * p.dx) * ((double)i * p.dx) + ((double)j * p.dx) * ((double)j * p.dx) +
+                                       ((double)k * p.dx) * ((double)k * p.dx));
+          if (dmy < m.max_radius) {
+            st.push_back(i);
+            st.push_back(j);
+            st.push_back(k);
+          }
+        }
+    c->n_stencil = (int)st.size() / 3;
+    RC(dalloc(c, &c->stencil, st.size()));
+    SPM_HIP(c, hipMemcpy(c->stencil, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
+  }
+
+  // particles
+  const int np = c->np;
+  RC(dalloc(c, &c->pbuf, (size_t)P_NSLOTS * np));
+  RC(dalloc(c, &c->spec, np));
+  RC(dalloc(c, &c->pxint, 3 * (size_t)np));
+  RC(dalloc(c, &c->pres, 3 * (size_t)np));
+  RC(dalloc(c, &c->pcell, np));
+  c->pstage.assign((size_t)P_NSLOTS * (np > 0 ? np : 1), 0.0);
+  {
+    std::vector<double> init((size_t)P_NSLOTS * np, 0.0);
+    for (int n = 0; n < np; ++n) {
+      init[(size_t)P_Q * np + n] = 1.0;
+      init[(size_t)P_Q_OLD * np + n] = 1.0;
+    }
+    if (np) SPM_HIP(c, hipMemcpy(c->pbuf, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice));
+  }
+
+  // tiles for make_phi
+  c->tiles[0] = g.nx / 8;
+  c->tiles[1] = g.ny / 8;
+  c->tiles[2] = g.nz / 8;
+  c->ntiles = c->tiles[0] * c->tiles[1] * c->tiles[2];
+  RC(dalloc(c, &c->tile_count, c->ntiles));
+  RC(dalloc(c, &c->tile_start, c->ntiles));
+  RC(dalloc(c, &c->tile_cursor, c->ntiles));
+  {
+    long span = 1;
+    for (int d = 0; d < 3; ++d) span *= std::min(c->tiles[d], (2 * m.nh + 1 + 7) / 8 + 1);
+    c->tile_list_cap = (int)std::max<long>(1, span * np);
+    RC(dalloc(c, &c->tile_list, c->tile_list_cap));
+  }
+
+  // LJ cells: at least one cutoff wide
+  if (m.lj_on) {
+    const double L[3] = {g.lx, g.ly, g.lz};
+    for (int d = 0; d < 3; ++d) {
+      int nc = (int)std::floor(L[d] / m.lj_cutoff);
+      c->cells[d] = std::max(1, std::min(nc, 128));
+    }
+  } else {
+    c->cells[0] = c->cells[1] = c->cells[2] = 1;
+  }
+  c->ncells = c->cells[0] * c->cells[1] * c->cells[2];
+  RC(dalloc(c, &c->cell_count, c->ncells));
+  RC(dalloc(c, &c->cell_start, c->ncells));
+  RC(dalloc(c, &c->cell_cursor, c->ncells));
+  RC(dalloc(c, &c->cell_list, np));
+
+  // thermostat staging
+  for (int s = 0; s < 2; ++s) {
+    SPM_HIP(c, hipHostMalloc((void **)&c->noise_host[s], sizeof(double) * 6 * (np > 0 ? np : 1), hipHostMallocDefault));
+    SPM_HIP(c, hipEventCreateWithFlags(&c->noise_done[s], hipEventDisableTiming));
+    SPM_HIP(c, hipEventRecord(c->noise_done[s], c->stream));
+  }
+  RC(dalloc(c, &c->noise_dev, 6 * (size_t)np));
+  mt_seed(c, p.seed);
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+thread_local std::string g_create_err;
+
+}  // namespace
+
+// =====================================================================
+// C ABI
+// =====================================================================
+extern "C" {
+
+const char *spm_strerror(int status) {
+  switch (status) {
+    case SPM_OK: return "ok";
+    case SPM_ERR_INVALID: return "invalid argument";
+    case SPM_ERR_OOM: return "out of memory";
+    case SPM_ERR_HIP: return "HIP runtime error";
+    case SPM_ERR_NONFINITE: return "non-finite value in fields";
+    case SPM_ERR_UNSUPPORTED: return "unsupported option";
+    case SPM_ERR_NO_DEVICE: return "no GPU device";
+    default: return "unknown status";
+  }
+}
+
+const char *spm_last_error(const spm_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int spm_create(const spm_params *prm, spm_ctx **out) {
+  if (!prm || !out) return SPM_ERR_INVALID;
+  *out = nullptr;
+  g_create_err.clear();
+  const spm_params &p = *prm;
+  if (!is_pow2(p.nx, 8, 1024) || !is_pow2(p.ny, 8, 1024) || !is_pow2(p.nz, 8, 1024)) {
+    g_create_err = "grid sizes must be powers of two in [8, 1024]";
+    return SPM_ERR_UNSUPPORTED;
+  }
+  if (p.n_particles < 0 || p.n_species < 1 || p.n_species > SPM_MAX_SPECIES || !(p.dt > 0) || !(p.dx > 0) ||
+      !(p.rho > 0) || !(p.eta > 0) || !(p.a > 0) || !(p.a_xi > 0) || p.g_direction < 0 || p.g_direction > 2) {
+    g_create_err = "invalid parameter";
+    return SPM_ERR_INVALID;
+  }
+  for (int s = 0; s < p.n_species; ++s)
+    if (!(p.mass_ratio[s] > 0)) {
+      g_create_err = "mass ratios must be positive";
+      return SPM_ERR_INVALID;
+    }
+  if (p.precision != SPM_FP64) {
+    g_create_err = "only the FP64 field path is built";
+    return SPM_ERR_UNSUPPORTED;
+  }
+  const int nh = (int)p.a + (int)p.a_xi + 2;
+  if (p.n_particles > 0 && (p.nx < 2 * nh + 1 || p.ny < 2 * nh + 1 || p.nz < 2 * nh + 1)) {
+    g_create_err = "box narrower than a particle stencil (2*(a+xi+2)+1 points)";
+    return SPM_ERR_UNSUPPORTED;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    g_create_err = "no HIP device";
+    return SPM_ERR_NO_DEVICE;
+  }
+  if (p.device < 0 || p.device >= ndev) {
+    g_create_err = "device ordinal out of range";
+    return SPM_ERR_INVALID;
+  }
+  spm_ctx *c = new (std::nothrow) spm_ctx();
+  if (!c) return SPM_ERR_OOM;
+  c->prm = p;
+  c->device = p.device;
+  int rc = setup(c);
+  if (rc) {
+    g_create_err = c->err;
+    free_ctx(c);
+    return rc == SPM_ERR_HIP && g_create_err.find("out of memory") != std::string::npos ? SPM_ERR_OOM : rc;
+  }
+  *out = c;
+  return SPM_OK;
+}
+
+int spm_destroy(spm_ctx *ctx) {
+  free_ctx(ctx);
+  return SPM_OK;
+}
+
+int spm_stencil_size(const spm_ctx *ctx) { return ctx ? ctx->n_stencil : -1; }
+
+int spm_put_field(spm_ctx *c, int field, const double *host) {
+  if (!c || !host) return SPM_ERR_INVALID;
+  double *p = field_ptr(c, field);
+  if (!p) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipMemcpyAsync(p, host, sizeof(double) * c->nfield, hipMemcpyHostToDevice, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  return SPM_OK;
+}
+
+int spm_get_field(spm_ctx *c, int field, double *host) {
+  if (!c || !host) return SPM_ERR_INVALID;
+  double *p = field_ptr(c, field);
+  if (!p) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipMemcpyAsync(host, p, sizeof(double) * c->nfield, hipMemcpyDeviceToHost, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  return SPM_OK;
+}
+
+int spm_put_particles(spm_ctx *c, const spm_particle *p, int n) {
+  if (!c || (!p && n > 0) || n != c->np) return SPM_ERR_INVALID;
+  const double L[3] = {c->grid.lx, c->grid.ly, c->grid.lz};
+  std::vector<int> spec(n > 0 ? n : 1);
+  for (int i = 0; i < n; ++i) {
+    for (int d = 0; d < 3; ++d)
+      if (!(p[i].x[d] >= 0.0 && p[i].x[d] < L[d])) {
+        c->err = "particle position outside [0, L)";
+        return SPM_ERR_INVALID;
+      }
+    if (p[i].spec < 0 || p[i].spec >= c->prm.n_species) {
+      c->err = "particle species out of range";
+      return SPM_ERR_INVALID;
+    }
+    spec[i] = p[i].spec;
+    const double *src = reinterpret_cast<const double *>(reinterpret_cast<const char *>(&p[i]) + 8);
+    for (int s = 0; s < P_NSLOTS; ++s) c->pstage[(size_t)s * n + i] = src[s];
+  }
+  if (n == 0) return SPM_OK;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipMemcpyAsync(c->pbuf, c->pstage.data(), sizeof(double) * P_NSLOTS * n, hipMemcpyHostToDevice, c->stream));
+  SPM_HIP(c, hipMemcpyAsync(c->spec, spec.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  return SPM_OK;
+}
+
+int spm_get_particles(spm_ctx *c, spm_particle *p, int n) {
+  if (!c || (!p && n > 0) || n != c->np) return SPM_ERR_INVALID;
+  if (n == 0) return SPM_OK;
+  std::vector<int> spec(n);
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipMemcpyAsync(c->pstage.data(), c->pbuf, sizeof(double) * P_NSLOTS * n, hipMemcpyDeviceToHost, c->stream));
+  SPM_HIP(c, hipMemcpyAsync(spec.data(), c->spec, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; ++i) {
+    std::memset(&p[i], 0, sizeof(spm_particle));
+    p[i].spec = spec[i];
+    double *dst = reinterpret_cast<double *>(reinterpret_cast<char *>(&p[i]) + 8);
+    for (int s = 0; s < P_NSLOTS; ++s) dst[s] = c->pstage[(size_t)s * n + i];
+  }
+  return SPM_OK;
+}
+
+int spm_put_uk_dc(spm_ctx *c, const double uk_dc[3]) {
+  if (!c || !uk_dc) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipMemcpyAsync(c->uk_dc, uk_dc, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  return SPM_OK;
+}
+
+int spm_get_uk_dc(spm_ctx *c, double uk_dc[3]) {
+  if (!c || !uk_dc) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipMemcpyAsync(uk_dc, c->uk_dc, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  return SPM_OK;
+}
+
+int spm_put_time(spm_ctx *c, int ts, double time) {
+  if (!c || ts < 0) return SPM_ERR_INVALID;
+  c->ts = ts;
+  c->time = time;
+  return SPM_OK;
+}
+
+int spm_get_time(spm_ctx *c, int *ts, double *time) {
+  if (!c || !ts || !time) return SPM_ERR_INVALID;
+  *ts = c->ts;
+  *time = c->time;
+  return SPM_OK;
+}
+
+int spm_put_rng(spm_ctx *c, const uint32_t mt[SPM_MT_N], int mti) {
+  if (!c || !mt || mti < 0 || mti > SPM_MT_N + 1) return SPM_ERR_INVALID;
+  std::memcpy(c->mt, mt, sizeof(uint32_t) * SPM_MT_N);
+  c->mti = mti;
+  return SPM_OK;
+}
+
+int spm_get_rng(spm_ctx *c, uint32_t mt[SPM_MT_N], int *mti) {
+  if (!c || !mt || !mti) return SPM_ERR_INVALID;
+  // the host stream is current as soon as the draw has been made
+  std::memcpy(mt, c->mt, sizeof(uint32_t) * SPM_MT_N);
+  *mti = c->mti;
+  return SPM_OK;
+}
+
+#define ENTRY(body)                                \
+  do {                                             \
+    if (!c) return SPM_ERR_INVALID;                \
+    c->err.clear();                                \
+    SPM_HIP(c, hipSetDevice(c->device));           \
+    int rc_ = (body);                              \
+    return rc_;                                    \
+  } while (0)
+
+static int init_fluid(spm_ctx *c) {
+    // Init_zeta_k (init_fluid.cxx:10-23) from u = 0
+    for (int d = 0; d < 3; ++d) RC(k_zero(c, c->u[d], c->nfield));
+    double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+    RC(fft_r2c_many(c, u3, 3));
+    RC(k_u_k2zeta_k(c, c->u[0], c->u[1], c->u[2], c->zeta[0], c->zeta[1]));
+    // sp_3d_ns.cxx:508-524
+    RC(p_make_phi(c, 1));
+    RC(k_zeta_k2u_k(c, c->zeta[0], c->zeta[1], c->u[0], c->u[1], c->u[2], 0));
+    RC(fft_c2r_many(c, u3, 3));
+    RC(make_f_sole_add(c));
+    RC(u2zeta_k(c));
+    SPM_HIP(c, hipMemsetAsync(c->uk_dc, 0, sizeof(double) * 3, c->stream));
+    SPM_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int calc_f_hydro_api(spm_ctx *c) {
+  RC(reset_phi_u(c));
+  return p_calc_f_hydro(c);
+}
+
+static int step_n(spm_ctx *c, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) {
+    RC(time_evolution_hydro(c));
+    c->time += c->prm.dt;
+    c->ts++;
+    // keep the event pool bounded when timing long runs
+    if (c->timing && c->launches.size() > 20000) {
+      SPM_HIP(c, hipStreamSynchronize(c->stream));
+      RC(fold_timing(c));
+    }
+  }
+  return 0;
+}
+
+static int check_finite(spm_ctx *c) {
+  SPM_HIP(c, hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));
+  for (int d = 0; d < 3; ++d) RC(k_check_finite(c, c->u[d], false));
+  for (int d = 0; d < 2; ++d) RC(k_check_finite(c, c->zeta[d], true));
+  int h = 0;
+  SPM_HIP(c, hipMemcpyAsync(&h, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  if (h) {
+    c->err = "NaN or Inf in u or zeta";
+    return SPM_ERR_NONFINITE;
+  }
+  return 0;
+}
+
+int spm_init_fluid(spm_ctx *c) { ENTRY(init_fluid(c)); }
+
+int spm_ns_solver_slaved_euler(spm_ctx *c) { ENTRY(ns_solver(c)); }
+int spm_zeta_k2u(spm_ctx *c) { ENTRY(zeta_k2u(c, fix_mask(c))); }
+int spm_md_solver_position(spm_ctx *c) { ENTRY(p_position(c, c->ts == 0)); }
+int spm_calc_f_hydro(spm_ctx *c) { ENTRY(calc_f_hydro_api(c)); }
+int spm_force(spm_ctx *c) { ENTRY(p_force(c)); }
+int spm_md_solver_velocity(spm_ctx *c) { ENTRY(p_velocity(c, c->ts == 0)); }
+int spm_add_random_force_thermostat(spm_ctx *c) { ENTRY(thermostat(c)); }
+int spm_make_phi_u_particle(spm_ctx *c) { ENTRY(p_make_phi(c, 1)); }
+int spm_make_f_particle_dt_sole(spm_ctx *c) { ENTRY(make_f_sole_add(c)); }
+int spm_u2zeta_k(spm_ctx *c) { ENTRY(u2zeta_k(c)); }
+
+int spm_step(spm_ctx *c, int nsteps) {
+  if (nsteps < 0) return SPM_ERR_INVALID;
+  ENTRY(step_n(c, nsteps));
+}
+
+int spm_synchronize(spm_ctx *c) {
+  if (!c) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  return SPM_OK;
+}
+
+int spm_check_finite(spm_ctx *c) { ENTRY(check_finite(c)); }
+
+int spm_timing_enable(spm_ctx *c, int on) {
+  if (!c) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  RC(fold_timing(c));
+  c->timing = on != 0;
+  return SPM_OK;
+}
+
+int spm_timing_read(spm_ctx *c, double ms[SPM_NUM_KERNEL_FAMILIES], double bytes[SPM_NUM_KERNEL_FAMILIES],
+                    long launches[SPM_NUM_KERNEL_FAMILIES]) {
+  if (!c || !ms || !bytes || !launches) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  RC(fold_timing(c));
+  for (int f = 0; f < SPM_NUM_KERNEL_FAMILIES; ++f) {
+    ms[f] = c->fam_ms[f];
+    bytes[f] = c->fam_bytes[f];
+    launches[f] = c->fam_count[f];
+    c->fam_ms[f] = c->fam_bytes[f] = 0.0;
+    c->fam_count[f] = 0;
+  }
+  return SPM_OK;
+}
+
+}  // extern "C"

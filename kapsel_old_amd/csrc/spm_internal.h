@@ -1,0 +1,231 @@
+// spm_internal.h -- shared definitions of the MI355X SPM library.
+//
+// Data layout in HBM (one context = one box on one GPU):
+//   fields    NX*NY*NZ_ elements of T (double on the FP64 path), row-major
+//             [i][j][k], NZ_ = 2*(NZ/2+1); real space uses k < NZ, a
+//             spectrum holds NZ/2+1 interleaved complex values per row.
+//   particles SoA copy of the reference Particle struct: slot s of
+//             particle n at pbuf[s*Np + n] (s indexes the 124 doubles of
+//             the struct after `spec`), spec in a separate int array.
+//   lists     per-tile particle lists (make_phi) and per-cell particle
+//             lists (LJ), both as CSR (start, count, sorted members).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/spm.h"
+
+namespace spm {
+
+// ---------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------
+template <typename T>
+struct cplx_t;
+template <>
+struct cplx_t<double> {
+  using type = double2;
+};
+template <>
+struct cplx_t<float> {
+  using type = float2;
+};
+template <typename T>
+using cplx = typename cplx_t<T>::type;
+
+template <typename C2>
+__device__ __forceinline__ C2 cadd(C2 a, C2 b) {
+  return C2{a.x + b.x, a.y + b.y};
+}
+template <typename C2>
+__device__ __forceinline__ C2 csub(C2 a, C2 b) {
+  return C2{a.x - b.x, a.y - b.y};
+}
+template <typename C2>
+__device__ __forceinline__ C2 cmul(C2 a, C2 b) {
+  return C2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// ---------------------------------------------------------------------
+// particle SoA slots: offsets (in doubles) of the fields of spm_particle
+// after `spec`, so slot = (offsetof(field) - 8) / 8.
+// ---------------------------------------------------------------------
+enum PSlot : int {
+  P_X = 0, P_X_PREV = 3, P_X_NOPBC = 6, P_V = 9, P_V_OLD = 12, P_V_SLIP = 15,
+  P_F_HYDRO = 18, P_F_HYDRO_PREV = 21, P_F_HYDRO1 = 24, P_F_SLIP = 27,
+  P_F_SLIP_PREV = 30, P_FR = 33, P_FR_PREV = 36, P_OMEGA = 39,
+  P_OMEGA_OLD = 42, P_OMEGA_SLIP = 45, P_TORQUE_HYDRO = 48,
+  P_TORQUE_HYDRO_PREV = 51, P_TORQUE_HYDRO1 = 54, P_TORQUE_SLIP = 57,
+  P_TORQUE_SLIP_PREV = 60, P_MOMENTUM_DEPEND_FR = 63, P_QR = 66,
+  P_QR_OLD = 75, P_Q = 84, P_Q_OLD = 88, P_NSLOTS = 124
+};
+
+// Grid-wide constants handed to kernels by value.
+struct Grid {
+  int nx, ny, nz, nz_, hnz;  // hnz = NZ/2+1
+  int trn_x, trn_y, trn_z;   // (N+2)/3, input.cxx:297
+  double wave_x, wave_y, wave_z;
+  double lx, ly, lz;
+  double dx;
+};
+
+// Particle-model constants (Set_global_parameters, input.cxx:274-433).
+struct PModel {
+  int np;
+  int rotation;
+  double dx, radius, hxi, max_radius;  // max_radius: stencil bound
+  int nh;                              // half-width of the stencil cube
+  double rho, dx3;
+  double imass[SPM_MAX_SPECIES], imoi[SPM_MAX_SPECIES];
+  double mass_ratio[SPM_MAX_SPECIES];
+  double lj_dia, lj_cutoff, epsilon;
+  int lj_powers, lj_on;
+  double g_on_fluid;
+  int g_direction;
+};
+
+}  // namespace spm
+
+// ---------------------------------------------------------------------
+// the context (opaque to callers of include/spm.h)
+// ---------------------------------------------------------------------
+struct spm_ctx {
+  spm_params prm;
+  spm::Grid grid;
+  spm::PModel pm;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  size_t nfield = 0;  // elements per field
+  int ts = 0;
+  double time = 0.0;
+
+  // fields (double path)
+  double *zeta[2] = {nullptr, nullptr};
+  double *u[3] = {nullptr, nullptr, nullptr};
+  double *up[3] = {nullptr, nullptr, nullptr};
+  double *phi = nullptr;
+  double *f[3] = {nullptr, nullptr, nullptr};
+  double *work[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double *uk_dc = nullptr;  // device, 3 doubles
+
+  // FFT tables: roots exp(+2 pi i t/N), t < N, for N = NX, NY, NZ/2, and
+  // the real-to-complex post twiddle exp(+2 pi i k/NZ), k <= NZ/2.
+  double2 *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr, *tw_rz = nullptr;
+
+  // particles
+  int np = 0;
+  double *pbuf = nullptr;  // P_NSLOTS * np
+  int *spec = nullptr;
+  int *stencil = nullptr;  // 3 * n_stencil ints (Sekibun_cell)
+  int n_stencil = 0;
+  int *pxint = nullptr;     // 3 * np: anchor cell of each particle (Particle_cell)
+  double *pres = nullptr;   // 3 * np: residue x - anchor*DX
+
+  // per-tile particle lists for make_phi (tiles of 8x8x8 points)
+  int tiles[3] = {0, 0, 0};
+  int ntiles = 0;
+  int *tile_count = nullptr, *tile_start = nullptr, *tile_cursor = nullptr;
+  int *tile_list = nullptr;
+  int tile_list_cap = 0;
+
+  // LJ cell list
+  int cells[3] = {0, 0, 0};
+  int ncells = 0;
+  int *cell_count = nullptr, *cell_start = nullptr, *cell_cursor = nullptr;
+  int *cell_list = nullptr;
+  int *pcell = nullptr;     // np: LJ cell of each particle
+  int *flag = nullptr;      // device int: finiteness check result
+
+  // thermostat noise (host MT19937, device apply)
+  uint32_t mt[SPM_MT_N];
+  int mti = SPM_MT_N + 1;
+  double *noise_host[2] = {nullptr, nullptr};  // pinned
+  double *noise_dev = nullptr;
+  hipEvent_t noise_done[2] = {nullptr, nullptr};
+  int noise_slot = 0;
+  double noise_v = 0.0, noise_o = 0.0;
+
+  // host staging
+  std::vector<double> pstage;
+
+  // optional per-kernel-family timing (HIP events on `stream`)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct TimedLaunch {
+    int family;
+    size_t e0, e1;
+    double bytes;
+  };
+  std::vector<TimedLaunch> launches;
+  double fam_ms[SPM_NUM_KERNEL_FAMILIES] = {};
+  double fam_bytes[SPM_NUM_KERNEL_FAMILIES] = {};
+  long fam_count[SPM_NUM_KERNEL_FAMILIES] = {};
+};
+
+// ---------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------
+#define SPM_HIP(ctx, call)                                                      \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);           \
+      return SPM_ERR_HIP;                                                       \
+    }                                                                           \
+  } while (0)
+
+#define SPM_LAUNCH_CHECK(ctx)                                                   \
+  do {                                                                          \
+    hipError_t e_ = hipGetLastError();                                          \
+    if (e_ != hipSuccess) {                                                     \
+      (ctx)->err = std::string("kernel launch: ") + hipGetErrorString(e_);      \
+      return SPM_ERR_HIP;                                                       \
+    }                                                                           \
+  } while (0)
+
+// ---------------------------------------------------------------------
+// launchers implemented in the .hip files
+// ---------------------------------------------------------------------
+namespace spm {
+// timing brackets around a launch; no-ops unless timing is enabled
+void tbeg(spm_ctx *c);
+void tend(spm_ctx *c, int family, double bytes);
+
+// fft.hip: 3-D real transforms in the reference layout (in place).
+int fft_r2c(spm_ctx *c, double *a);
+int fft_c2r(spm_ctx *c, double *a);
+// batched versions over several fields (same grid)
+int fft_r2c_many(spm_ctx *c, double *const *a, int n);
+int fft_c2r_many(spm_ctx *c, double *const *a, int n);
+int fft_init_tables(spm_ctx *c);
+
+// spectral.hip
+int k_truncate(spm_ctx *c, double *a);
+int k_zeta_k2u_k(spm_ctx *c, const double *z0, const double *z1, double *u0, double *u1, double *u2, int fix_cell_mask);
+int k_u_k2zeta_k(spm_ctx *c, const double *u0, const double *u1, const double *u2, double *z0, double *z1);
+int k_products(spm_ctx *c, double *u0, double *u1, double *u2, double *a0, double *a1);
+int k_advection_contract(spm_ctx *c, double *u0, double *u1, double *u2, const double *a0, const double *a1, double *o0, double *o1);
+int k_slaved_euler(spm_ctx *c, double *z0, double *z1, const double *f0, const double *f1, double nu, double dt);
+int k_solenoidal_trunc(spm_ctx *c, double *u0, double *u1, double *u2);
+int k_make_f(spm_ctx *c);
+int k_add_f(spm_ctx *c);
+int k_zero(spm_ctx *c, double *a, size_t n);
+int k_check_finite(spm_ctx *c, const double *a, bool spectrum);
+
+// particles.hip
+int p_position(spm_ctx *c, int euler);
+int p_calc_f_hydro(spm_ctx *c);
+int p_force(spm_ctx *c);
+int p_velocity(spm_ctx *c, int euler);
+int p_thermostat_apply(spm_ctx *c, const double *noise_dev);
+int p_build_tiles(spm_ctx *c);
+int p_make_phi(spm_ctx *c, int sw_up);
+int p_init_lists(spm_ctx *c);
+}  // namespace spm

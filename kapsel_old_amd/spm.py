@@ -1,0 +1,189 @@
+"""Python front end of libspm.so (the C ABI declared in include/spm.h).
+
+`SPM` mirrors the reference's solver call surface by name --
+NS_solver_slavedEuler, Make_phi_u_particle, Calc_f_hydro_correct_precision,
+Force, ... -- over one GPU context.  Everything runs in the HIP library;
+this module only moves arguments.  It raises if libspm.so is missing or
+fails: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspm.so")
+
+SYMBOLS = [
+    "spm_create", "spm_destroy", "spm_strerror", "spm_last_error", "spm_stencil_size",
+    "spm_put_particles", "spm_get_particles", "spm_put_field", "spm_get_field",
+    "spm_put_uk_dc", "spm_get_uk_dc", "spm_put_time", "spm_get_time",
+    "spm_put_rng", "spm_get_rng",
+    "spm_init_fluid", "spm_ns_solver_slaved_euler", "spm_zeta_k2u", "spm_md_solver_position",
+    "spm_calc_f_hydro", "spm_force", "spm_md_solver_velocity", "spm_add_random_force_thermostat",
+    "spm_make_phi_u_particle", "spm_make_f_particle_dt_sole", "spm_u2zeta_k", "spm_step",
+    "spm_synchronize", "spm_check_finite",
+]
+
+_lib = None
+
+
+class SpmError(RuntimeError):
+    def __init__(self, status, where, detail=""):
+        self.status = status
+        super().__init__("%s: %s (%d)%s" % (where, _strerror(status), status,
+                                             (": " + detail) if detail else ""))
+
+
+def _strerror(status):
+    return lib().spm_strerror(status).decode()
+
+
+def lib():
+    """Load libspm.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("kapsel_old_amd/libspm.so is not built; run python -m kapsel_old_amd.build")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.c_int, C.POINTER(C.c_double)
+        L.spm_create.argtypes = [C.POINTER(abi.SpmParams), C.POINTER(vp)]
+        L.spm_destroy.argtypes = [vp]
+        L.spm_strerror.argtypes = [ip]
+        L.spm_strerror.restype = C.c_char_p
+        L.spm_last_error.argtypes = [vp]
+        L.spm_last_error.restype = C.c_char_p
+        L.spm_stencil_size.argtypes = [vp]
+        L.spm_put_particles.argtypes = [vp, vp, ip]
+        L.spm_get_particles.argtypes = [vp, vp, ip]
+        L.spm_put_field.argtypes = [vp, ip, dp]
+        L.spm_get_field.argtypes = [vp, ip, dp]
+        L.spm_put_uk_dc.argtypes = [vp, dp]
+        L.spm_get_uk_dc.argtypes = [vp, dp]
+        L.spm_put_time.argtypes = [vp, ip, C.c_double]
+        L.spm_get_time.argtypes = [vp, C.POINTER(ip), dp]
+        L.spm_put_rng.argtypes = [vp, C.POINTER(C.c_uint32), ip]
+        L.spm_get_rng.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(ip)]
+        for name in ["spm_init_fluid", "spm_ns_solver_slaved_euler", "spm_zeta_k2u",
+                     "spm_md_solver_position", "spm_calc_f_hydro", "spm_force",
+                     "spm_md_solver_velocity", "spm_add_random_force_thermostat",
+                     "spm_make_phi_u_particle", "spm_make_f_particle_dt_sole",
+                     "spm_u2zeta_k", "spm_synchronize", "spm_check_finite"]:
+            getattr(L, name).argtypes = [vp]
+        L.spm_step.argtypes = [vp, ip]
+        for name in SYMBOLS:
+            f = getattr(L, name)
+            if f.restype is C.c_int or f.restype is None:
+                pass
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class SPM:
+    """One SPM box on one GPU (an spm_ctx)."""
+
+    def __init__(self, params):
+        self.params = params
+        h = C.c_void_p()
+        rc = lib().spm_create(C.byref(params), C.byref(h))
+        if rc:
+            raise SpmError(rc, "spm_create", lib().spm_last_error(None).decode())
+        self._h = h
+        p = params
+        self.shape = (p.nx, p.ny, 2 * (p.nz // 2 + 1))
+        self.n_particles = p.n_particles
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().spm_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _call(self, name, *args):
+        rc = getattr(lib(), "spm_" + name)(self._h, *args)
+        if rc:
+            raise SpmError(rc, "spm_" + name, lib().spm_last_error(self._h).decode())
+
+    # ---- the solver call surface (names follow the reference) ----
+    def init_fluid(self): self._call("init_fluid")
+    def NS_solver_slavedEuler(self): self._call("ns_solver_slaved_euler")
+    def Zeta_k2u(self): self._call("zeta_k2u")
+    def MD_solver_position(self): self._call("md_solver_position")
+    def Calc_f_hydro_correct_precision(self): self._call("calc_f_hydro")
+    def Force(self): self._call("force")
+    def MD_solver_velocity(self): self._call("md_solver_velocity")
+    def Add_random_force_thermostat(self): self._call("add_random_force_thermostat")
+    def Make_phi_u_particle(self): self._call("make_phi_u_particle")
+    def Make_f_particle_dt_sole(self): self._call("make_f_particle_dt_sole")
+    def U2zeta_k(self): self._call("u2zeta_k")
+
+    def step(self, nsteps=1):
+        self._call("step", nsteps)
+
+    def synchronize(self):
+        self._call("synchronize")
+
+    def check_finite(self):
+        self._call("check_finite")
+
+    # ---- state ----
+    def stencil_size(self):
+        return lib().spm_stencil_size(self._h)
+
+    def get_field(self, name):
+        out = np.empty(self.shape)
+        self._call("get_field", abi.FIELD_IDS[name], _dp(out))
+        return out
+
+    def put_field(self, name, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape)
+        self._call("put_field", abi.FIELD_IDS[name], _dp(arr))
+
+    def get_particles(self):
+        p = abi.new_particles(self.n_particles)
+        self._call("get_particles", p.ctypes.data_as(C.c_void_p), self.n_particles)
+        return p
+
+    def put_particles(self, p):
+        p = np.ascontiguousarray(p, dtype=abi.PARTICLE_DTYPE)
+        self._call("put_particles", p.ctypes.data_as(C.c_void_p), len(p))
+
+    def get_uk_dc(self):
+        out = np.empty(3)
+        self._call("get_uk_dc", _dp(out))
+        return out
+
+    def put_uk_dc(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._call("put_uk_dc", _dp(v))
+
+    def get_time(self):
+        ts, t = C.c_int(), C.c_double()
+        self._call("get_time", C.byref(ts), C.byref(t))
+        return ts.value, t.value
+
+    def put_time(self, ts, t):
+        self._call("put_time", ts, C.c_double(t))
+
+    def get_rng(self):
+        mt = (C.c_uint32 * abi.MT_N)()
+        mti = C.c_int()
+        self._call("get_rng", mt, C.byref(mti))
+        return np.array(mt, dtype=np.uint32), mti.value
+
+    def put_rng(self, mt, mti):
+        arr = (C.c_uint32 * abi.MT_N)(*[int(x) for x in mt])
+        self._call("put_rng", arr, mti)

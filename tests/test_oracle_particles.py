@@ -1,0 +1,218 @@
+"""Pins the oracle's particle side.
+
+  Particle_domain (profile.cxx:25-74)      stencil sizes measured on the
+                                           reference build: 751 / 1309 / 1935
+                                           / 2801 / 5185 points for a = 3..8
+  Phi (profile.h:29-49)                    phi(a) = 1/2, phi(a-d) + phi(a+d) = 1,
+                                           1 inside, 0 outside, volume
+  Make_phi_u_primitive (make_phi.cxx:221)  up / phi = v + omega x r; clamp
+  Calc_f_hydro_correct_precision           uniform-flow drag and rigid-rotation
+    (md_force.cxx:263-389)                 torque in closed form
+  LJ linked cells (md_force.cxx:23-134)    = all-pairs brute force; Newton's
+                                           third law; the reference's double
+                                           count when 2 cells span an axis
+  Add_f_gravity (md_force.cxx:181-191)     buoyancy-corrected weight
+  MD solvers (particle_solver.cxx)         AB2 exact for linear velocity;
+                                           quaternion rotation about an axis
+"""
+import numpy as np
+import pytest
+
+from kapsel_old_amd import abi
+from oracle.oracle import Oracle
+from tests.helpers import particles
+
+
+@pytest.mark.parametrize("a,count", [(3.0, 751), (4.0, 1309), (5.0, 1935), (6.0, 2801), (8.0, 5185)])
+def test_stencil_sizes_match_reference(a, count):
+    o = Oracle(abi.make_params(64, a=a, a_xi=2.0))
+    assert o.stencil().shape[0] == count
+
+
+def test_profile_identities():
+    o = Oracle(abi.make_params(32, a=5.0, a_xi=2.0))
+    assert o.phi_profile(5.0) == 0.5
+    assert o.phi_profile(0.0) == 1.0 and o.phi_profile(3.99) == 1.0
+    assert o.phi_profile(6.0) == 0.0 and o.phi_profile(9.0) == 0.0
+    for d in [0.1, 0.37, 0.8, 0.99]:
+        assert abs(o.phi_profile(5.0 - d) + o.phi_profile(5.0 + d) - 1.0) < 1e-15
+    xs = np.linspace(4.0, 6.0, 101)
+    ph = [o.phi_profile(x) for x in xs]
+    assert all(ph[i] >= ph[i + 1] for i in range(100))
+
+
+def test_phi_volume_and_rigid_velocity():
+    n, a = 32, 5.0
+    prm = abi.make_params(n, a=a, n_particles=1, rotation=1)
+    o = Oracle(prm)
+    p = particles(1, (n, n, n), a, positions=np.array([[16.3, 15.8, 16.1]]))
+    p["v"][0] = [0.01, -0.02, 0.005]
+    p["omega"][0] = [0.003, 0.001, -0.002]
+    o.put_particles(p)
+    o.Make_phi_u_particle()
+    phi = o.get_field("phi")[:, :, :n]
+    assert abs(phi.sum() / (4 / 3 * np.pi * a ** 3) - 1.0) < 0.02
+    ups = [o.get_field("up%d" % d)[:, :, :n] for d in range(3)]
+    X = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).astype(float)
+    r = X - p["x"][0]
+    rigid = p["v"][0] + np.cross(p["omega"][0], r)
+    m = phi > 0
+    for d in range(3):
+        assert np.abs(ups[d][m] / phi[m] - rigid[..., d][m]).max() < 1e-14
+    assert np.all(ups[0][~m] == 0.0)
+
+
+def test_phi_clamp_on_overlap():
+    n, a = 32, 4.0
+    o = Oracle(abi.make_params(n, a=a, n_particles=2, rotation=0))
+    p = particles(2, (n, n, n), a, positions=np.array([[12.0, 16.0, 16.0], [18.0, 16.0, 16.0]]))
+    p["v"][0] = [1.0, 0.0, 0.0]
+    p["v"][1] = [0.0, 1.0, 0.0]
+    o.put_particles(p)
+    o.Make_phi_u_particle()
+    phi = o.get_field("phi")
+    assert phi.max() == 1.0
+    # unclamped profiles of each particle alone
+    single = []
+    for k in range(2):
+        q = Oracle(abi.make_params(n, a=a, n_particles=1, rotation=0))
+        q.put_particles(p[k:k + 1])
+        q.Make_phi_particle()
+        single.append(q.get_field("phi"))
+    tot = single[0] + single[1]
+    over = tot > 1.0
+    assert over.sum() > 10
+    up0, up1 = o.get_field("up0"), o.get_field("up1")
+    assert np.allclose(up0[over], single[0][over] / tot[over], rtol=1e-14, atol=0)
+    assert np.allclose(up1[over], single[1][over] / tot[over], rtol=1e-14, atol=0)
+
+
+def test_f_hydro_uniform_flow_closed_form():
+    n, a, dt, rho = 32, 4.0, 0.05, 1.3
+    o = Oracle(abi.make_params(n, a=a, n_particles=1, rotation=1, dt=dt, rho=rho))
+    p = particles(1, (n, n, n), a, positions=np.array([[16.0, 16.0, 16.0]]))
+    V, U = np.array([0.02, -0.01, 0.03]), np.array([0.005, 0.007, -0.004])
+    p["v"][0] = V
+    o.put_particles(p)
+    o.Make_phi_particle()
+    sphi = o.get_field("phi").sum()
+    for d in range(3):
+        f = np.zeros(o.shape)
+        f[:, :, :n] = U[d]
+        o.put_field("u%d" % d, f)
+    o.Calc_f_hydro_correct_precision()
+    q = o.get_particles()[0]
+    assert np.allclose(q["f_hydro"], -(rho / dt) * (V - U) * sphi, rtol=1e-12, atol=0)
+    assert np.abs(q["torque_hydro"]).max() < 1e-12 * np.abs(q["f_hydro"]).max()
+
+
+def test_f_hydro_rigid_rotation_torque():
+    n, a, dt = 32, 4.0, 0.05
+    o = Oracle(abi.make_params(n, a=a, n_particles=1, rotation=1, dt=dt))
+    p = particles(1, (n, n, n), a, positions=np.array([[16.0, 16.0, 16.0]]))
+    W = np.array([0.01, -0.02, 0.015])
+    p["omega"][0] = W
+    o.put_particles(p)
+    o.Make_phi_particle()
+    phi = o.get_field("phi")[:, :, :n]
+    X = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).astype(float) - 16.0
+    i2 = (phi * (X ** 2).sum(-1)).sum() * 2.0 / 3.0   # cubic symmetry: sum phi (r^2 - r r) = (2/3) sum phi r^2
+    for d in range(3):
+        o.put_field("u%d" % d, np.zeros(o.shape))
+    o.Calc_f_hydro_correct_precision()
+    q = o.get_particles()[0]
+    assert np.allclose(q["torque_hydro"], -(1.0 / dt) * i2 * W, rtol=1e-12, atol=0)
+    assert np.abs(q["f_hydro"]).max() < 1e-12 * np.abs(q["torque_hydro"]).max()
+
+
+def lj_brute(x, L, sigma, eps, cut):
+    n = len(x)
+    f = np.zeros_like(x)
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            d = x[j] - x[i]
+            d -= np.round(d / L) * L
+            r = np.sqrt((d * d).sum())
+            if r < cut:
+                s6 = (sigma / r) ** 6
+                f[i] += 24 * eps * (2 * s6 * s6 - s6) / r ** 2 * (-d)
+    return f
+
+
+def test_lj_linked_cells_equal_brute_force():
+    n, a = 64, 4.0
+    o = Oracle(abi.make_params(n, a=a, n_particles=40, lj_truncate=abi.LJ_WCA))
+    rng = np.random.default_rng(2)
+    base = abi.random_positions(20, (n, n, n), 4 * a, seed=2)
+    off = rng.standard_normal((20, 3))
+    off *= 2 * a * rng.uniform(0.95, 1.1, (20, 1)) / np.linalg.norm(off, axis=1, keepdims=True)
+    x = np.concatenate([base, np.mod(base + off, n)])
+    f = o.lj(x, lnk=True)
+    ref = lj_brute(x, n, 2 * a, 1.0, 2 ** (1 / 6) * 2 * a)
+    assert np.abs(ref).max() > 1e-3
+    assert np.abs(f - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.abs(f.sum(0)).max() < 1e-12 * np.abs(ref).max()
+    assert np.abs(o.lj(x, lnk=False) - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_lj_reference_double_count_with_two_cells():
+    """With NX = 32 the reference's 16-spacing cells give 2 per axis and the
+    -1/+1 neighbour wrap visits the other cell twice: a pair in different
+    cells gets twice the all-pairs force (quirk documented for the build)."""
+    o = Oracle(abi.make_params(32, a=4.0, n_particles=2, lj_truncate=abi.LJ_WCA))
+    x = np.array([[15.0, 8.0, 8.0], [15.0 + 7.6, 8.0, 8.0]])
+    lnk, allp = o.lj(x, lnk=True), o.lj(x, lnk=False)
+    assert np.abs(allp).max() > 1e-3
+    assert np.allclose(lnk, 2 * allp, rtol=1e-14, atol=0)
+
+
+def test_gravity_and_ab2_exact_for_constant_force():
+    n, a, dt = 32, 4.0, 0.05
+    mr, G = 2.5, 0.003
+    o = Oracle(abi.make_params(n, a=a, n_particles=1, mass_ratio=(mr,), g=G, g_direction=1, dt=dt, rotation=0))
+    p = particles(1, (n, n, n), a, positions=np.array([[10.0, 20.0, 5.0]]))
+    v0 = np.array([0.01, 0.02, -0.01])
+    p["v"][0] = v0
+    o.put_particles(p)
+    vol = 4.0 / 3.0 * np.pi * a ** 3
+    F = np.array([0.0, -G * vol * (mr - 1.0), 0.0])
+    M = mr * vol
+    x0 = p["x"][0].copy()
+    for step in range(12):
+        o.put_time(step, step * dt)
+        o.MD_solver_position()
+        o.MD_solver_velocity()
+        q = o.get_particles()[0]
+        t = (step + 1) * dt
+        assert np.allclose(q["v"], v0 + F / M * t, rtol=1e-13, atol=1e-17)
+        # positions move before velocities: x_{S} after S = step+1 position
+        # updates, Euler first then AB2 (exact on the linear v_s = v0 + s a dt):
+        # x_S = x0 + v0 t + a (t^2 - dt^2) / 2, t = S dt
+        xe = x0 + v0 * t + F / M * (t * t - dt * dt) / 2
+        assert np.allclose(q["x_nopbc"], xe, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("axis", [0, 2])
+def test_quaternion_rotation_second_order(axis):
+    errs = []
+    for dt in [0.02, 0.01]:
+        o = Oracle(abi.make_params(32, a=4.0, n_particles=1, rotation=1, dt=dt))
+        p = particles(1, (32, 32, 32), 4.0, positions=np.array([[16.0, 16.0, 16.0]]))
+        w = 0.7
+        p["omega"][0][axis] = w
+        p["omega_old"][0][axis] = w
+        o.put_particles(p)
+        steps = int(round(1.0 / dt))
+        for s in range(steps):
+            o.put_time(s, s * dt)
+            o.MD_solver_position()
+        q = o.get_particles()[0]["q"]
+        exact = np.zeros(4)
+        exact[0] = np.cos(w * 1.0 / 2)
+        exact[1 + axis] = np.sin(w * 1.0 / 2)
+        got = np.array([q["s"], *q["v"]])
+        errs.append(np.abs(got - exact).max())
+    assert errs[0] < 1e-3
+    assert 3.0 < errs[0] / errs[1] < 5.0  # second order
