@@ -203,6 +203,15 @@ int spm_u2zeta_k(spm_ctx *ctx);
 int spm_step(spm_ctx *ctx, int nsteps);
 /* Waits for all work queued on the context. */
 int spm_synchronize(spm_ctx *ctx);
+
+/* How spm_step runs a step (no counterpart in the reference):
+ *   SPM_SCHEDULE_FUSED (default) -- pointwise work fused into the FFT passes;
+ *     on return zeta, uk_dc, u, phi, up and the particles hold the reference's
+ *     end-of-step state, the f field is scratch;
+ *   SPM_SCHEDULE_LITERAL -- one kernel sequence per reference routine, every
+ *     field (f included) as the reference leaves it. */
+enum spm_schedule { SPM_SCHEDULE_FUSED = 0, SPM_SCHEDULE_LITERAL = 1 };
+int spm_set_schedule(spm_ctx *ctx, int schedule);
 /* Scans u and zeta for NaN/Inf; SPM_ERR_NONFINITE if any is found. */
 int spm_check_finite(spm_ctx *ctx);
 

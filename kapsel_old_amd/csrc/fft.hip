@@ -17,117 +17,9 @@
 // Inside a block the transform is a radix-8 Stockham FFT: each thread
 // keeps 8 points in registers per stage, stages exchange through LDS laid
 // out [point][column] so a wavefront touches consecutive LDS words.
-#include "spm_internal.h"
+#include "fft_core.h"
 
 namespace spm {
-
-template <typename C2, int SIGN>
-__device__ __forceinline__ C2 mul_si(C2 a) {  // a * (SIGN i)
-  return SIGN > 0 ? C2{-a.y, a.x} : C2{a.y, -a.x};
-}
-
-template <typename C2, int SIGN>
-__device__ __forceinline__ void dft2(C2 *a) {
-  C2 t = a[0];
-  a[0] = cadd(t, a[1]);
-  a[1] = csub(t, a[1]);
-}
-
-template <typename C2, int SIGN>
-__device__ __forceinline__ void dft4v(C2 &a0, C2 &a1, C2 &a2, C2 &a3) {
-  C2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3);
-  C2 t3 = mul_si<C2, SIGN>(csub(a1, a3));
-  a0 = cadd(t0, t2);
-  a1 = cadd(t1, t3);
-  a2 = csub(t0, t2);
-  a3 = csub(t1, t3);
-}
-
-template <typename C2, int SIGN>
-__device__ __forceinline__ void dft4(C2 *a) {
-  dft4v<C2, SIGN>(a[0], a[1], a[2], a[3]);
-}
-
-template <typename C2, int SIGN>
-__device__ __forceinline__ void dft8(C2 *a) {
-  using R = decltype(a[0].x);
-  C2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
-  C2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
-  dft4v<C2, SIGN>(e0, e1, e2, e3);
-  dft4v<C2, SIGN>(o0, o1, o2, o3);
-  const R r = (R)0.70710678118654752440084436210485;
-  const R s = (R)SIGN;
-  o1 = C2{r * (o1.x - s * o1.y), r * (o1.y + s * o1.x)};  // * exp(SIGN i pi/4)
-  o2 = mul_si<C2, SIGN>(o2);                               // * exp(SIGN i pi/2)
-  o3 = C2{-r * (o3.x + s * o3.y), r * (s * o3.x - o3.y)};  // * exp(SIGN 3 i pi/4)
-  a[0] = cadd(e0, o0);
-  a[4] = csub(e0, o0);
-  a[1] = cadd(e1, o1);
-  a[5] = csub(e1, o1);
-  a[2] = cadd(e2, o2);
-  a[6] = csub(e2, o2);
-  a[3] = cadd(e3, o3);
-  a[7] = csub(e3, o3);
-}
-
-template <typename C2, int RS, int SIGN>
-__device__ __forceinline__ void dft(C2 *a) {
-  if constexpr (RS == 2) dft2<C2, SIGN>(a);
-  else if constexpr (RS == 4) dft4<C2, SIGN>(a);
-  else dft8<C2, SIGN>(a);
-}
-
-// One radix-RS Stockham stage of span NS on C columns of N points held in
-// LDS at lds[point * CP + column].  The block has C * N / R threads; each
-// performs R / RS butterflies.  Twiddles come from tw[t] = exp(+2 pi i t/N).
-template <typename T, int N, int R, int RS, int NS, int C, int CP, int SIGN>
-__device__ __forceinline__ void stockham_stage(cplx<T> *lds, const double2 *__restrict__ tw, int tid) {
-  using C2 = cplx<T>;
-  constexpr int NB = N / RS;  // butterflies per column
-  constexpr int PER = R / RS;
-  const int c = tid % C;
-  const int j0 = tid / C;
-  C2 a[R];
-#pragma unroll
-  for (int s = 0; s < PER; ++s) {
-    const int j = j0 + s * (N / R);
-#pragma unroll
-    for (int r = 0; r < RS; ++r) a[s * RS + r] = lds[(j + r * NB) * CP + c];
-    if constexpr (NS > 1) {
-      const int k = j % NS;
-#pragma unroll
-      for (int r = 1; r < RS; ++r) {
-        const double2 w = tw[k * r * (N / (NS * RS))];
-        a[s * RS + r] = cmul(a[s * RS + r], C2{(T)w.x, (T)(SIGN * w.y)});
-      }
-    }
-    dft<C2, RS, SIGN>(&a[s * RS]);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < PER; ++s) {
-    const int j = j0 + s * (N / R);
-    const int k = j % NS;
-    const int base = (j / NS) * NS * RS + k;
-#pragma unroll
-    for (int r = 0; r < RS; ++r) lds[(base + r * NS) * CP + c] = a[s * RS + r];
-  }
-  __syncthreads();
-}
-
-template <typename T, int N, int R, int C, int CP, int SIGN, int NS>
-__device__ __forceinline__ void fft_stages(cplx<T> *lds, const double2 *__restrict__ tw, int tid) {
-  if constexpr (NS < N) {
-    constexpr int RS = (N / NS >= R) ? R : N / NS;
-    stockham_stage<T, N, R, RS, NS, C, CP, SIGN>(lds, tw, tid);
-    fft_stages<T, N, R, C, CP, SIGN, NS * RS>(lds, tw, tid);
-  }
-}
-
-template <typename T>
-struct FieldPtrs {
-  T *p[5];
-};
 
 // ---------------------------------------------------------------------
 // y / x passes: complex lines of N points, element stride `estride`
@@ -384,5 +276,52 @@ int fft_r2c_many(spm_ctx *c, double *const *a, int n) { return fft_many(c, a, n,
 int fft_c2r_many(spm_ctx *c, double *const *a, int n) { return fft_many(c, a, n, false); }
 int fft_r2c(spm_ctx *c, double *a) { return fft_many(c, &a, 1, true); }
 int fft_c2r(spm_ctx *c, double *a) { return fft_many(c, &a, 1, false); }
+
+}  // namespace spm
+
+namespace spm {
+
+// Single passes, used by the fused schedule (fused.hip supplies the x and
+// z passes that carry pointwise work).
+int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign) {
+  const Grid &g = c->grid;
+  for (int b = 0; b < n; b += 5) {
+    FieldPtrs<double> fp{};
+    const int nf = (n - b) < 5 ? (n - b) : 5;
+    for (int q = 0; q < nf; ++q) fp.p[q] = a[b + q];
+    const long h = g.hnz;
+    const double bytes = 2.0 * nf * (double)c->nfield * sizeof(double);
+    tbeg(c);
+    const int rc = sign > 0 ? launch_lines<double, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream)
+                            : launch_lines<double, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
+    tend(c, SPM_K_FFT_Y, bytes);
+    if (rc) {
+      c->err = "unsupported FFT size";
+      return SPM_ERR_UNSUPPORTED;
+    }
+    SPM_LAUNCH_CHECK(c);
+  }
+  return 0;
+}
+
+int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n) {
+  const Grid &g = c->grid;
+  const double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+  for (int b = 0; b < n; b += 5) {
+    FieldPtrs<double> fp{};
+    const int nf = (n - b) < 5 ? (n - b) : 5;
+    for (int q = 0; q < nf; ++q) fp.p[q] = a[b + q];
+    const double bytes = 2.0 * nf * (double)c->nfield * sizeof(double);
+    tbeg(c);
+    const int rc = launch_z<double>(g.nz / 2, false, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, scale, c->stream);
+    tend(c, SPM_K_FFT_Z_C2R, bytes);
+    if (rc) {
+      c->err = "unsupported FFT size";
+      return SPM_ERR_UNSUPPORTED;
+    }
+    SPM_LAUNCH_CHECK(c);
+  }
+  return 0;
+}
 
 }  // namespace spm

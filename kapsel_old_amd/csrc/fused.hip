@@ -1,0 +1,642 @@
+// fused.hip -- the SPM step as fused FFT passes (FP64).
+//
+// The literal schedule (spm_api.cpp: one kernel per reference routine)
+// moves about 187 field-sized arrays through HBM per step.  Here every
+// pointwise operation rides on the FFT pass that already holds the data:
+//
+//   NS_solver_slavedEuler (fluid_solver.cxx:328-354)
+//     x_zeta2u   : x pass (inverse) with Zeta_k2u_k + 2/3 mask as prologue
+//     y pass x3  : inverse
+//     z_advect   : z pass (inverse) x3 -> products of U2advection_k
+//                  (operate_omega.cxx:33-49) -> z pass (forward) x5
+//     y pass x5  : forward
+//     x_update   : x pass (forward) x5 with the contraction + Omega_k2zeta_k
+//                  (operate_omega.cxx:60-98) and the slaved-Euler update
+//                  (fluid_solver.cxx:337-353) as epilogue
+//   Zeta_k2u (sp_3d_ns.cxx:67-79): x_zeta2u (+FIX_CELL), y x3, z x3
+//   Make_phi_u_particle + Make_f_particle_dt_nonsole (make_phi.cxx:221-298,
+//   f_particle.h:41-64)
+//     z_makef    : phi and up computed per grid point from the tile lists,
+//                  f = up - phi u, z pass (forward) x3
+//     y pass x3  : forward
+//   Solenoidal_u + Add_f_particle + U2zeta_k (f_particle.h:71-99,
+//   operate_omega.h:186-189)
+//     x_project  : x pass (forward) x3 with 2/3 mask + projection of f, plus
+//                  u_k = Zeta_k2u_k(zeta) and zeta = U_k2zeta_k(u_k + P f).
+//
+// The last kernel uses R2C(C2R(u_k)) = u_k for the truncated, Hermitian
+// spectrum u_k: the reference transforms u + f back to k space, which is
+// the same field up to rounding.  About 74 field-passes per step remain.
+#include "fft_core.h"
+
+namespace spm {
+
+namespace {
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+__device__ __forceinline__ int nint(double a) { return a >= 0.0 ? (int)(a + .5) : (int)(a - .5); }
+
+struct ModeK {
+  int kx, ky, kz;
+  bool keep;
+};
+
+__device__ __forceinline__ ModeK mode_k(const Grid &g, int i, int j, int kz) {
+  ModeK m;
+  m.kx = i > g.nx / 2 ? i - g.nx : i;
+  m.ky = j > g.ny / 2 ? j - g.ny : j;
+  m.kz = kz;
+  m.keep = abs(m.kx) < g.trn_x && abs(m.ky) < g.trn_y && m.kz < g.trn_z;
+  return m;
+}
+
+// Zeta_k2u_k (fft_wrapper.cxx:408-511) on one mode; component d.
+__device__ __forceinline__ double2 zeta2u_mode(const Grid &g, const ModeK &m, double2 d1, double2 d2, int d) {
+  double kx = g.wave_x * m.kx, ky = g.wave_y * m.ky, kz = g.wave_z * m.kz;
+  const double k2 = sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz);
+  const double ik2 = k2 > 0.0 ? 1. / k2 : 0.0;
+  double wr0, wi0, wr1, wi1, wr2, wi2;
+  if (m.kx != 0) {
+    wr0 = -(1. / kx) * (ky * d1.x + kz * d2.x);
+    wi0 = -(1. / kx) * (ky * d1.y + kz * d2.y);
+    wr1 = d1.x; wi1 = d1.y;
+    wr2 = d2.x; wi2 = d2.y;
+  } else if (m.ky != 0) {
+    const double dmy = -(kz / ky);
+    wr0 = d2.x; wi0 = d2.y;
+    wr1 = dmy * d1.x; wi1 = dmy * d1.y;
+    wr2 = d1.x; wi2 = d1.y;
+  } else {
+    wr0 = d1.x; wi0 = d1.y;
+    wr1 = d2.x; wi1 = d2.y;
+    wr2 = 0.0; wi2 = 0.0;
+  }
+  kx *= ik2;
+  ky *= ik2;
+  kz *= ik2;
+  if (d == 0) return double2{(ky * wi2 - kz * wi1), -(ky * wr2 - kz * wr1)};
+  if (d == 1) return double2{(kz * wi0 - kx * wi2), -(kz * wr0 - kx * wr2)};
+  return double2{(kx * wi1 - ky * wi0), -(kx * wr1 - ky * wr0)};
+}
+
+// ---------------------------------------------------------------------
+// x pass (inverse) with Zeta_k2u_k prologue: zeta (2 spectra) -> 3 spectra
+// transformed along x.  Modes outside the 2/3 box read as zero (the in-place
+// truncation of sp_3d_ns.cxx:75-77 / operate_omega.cxx:106-108).  The thread
+// holding mode 0 sets u(0) = uk_dc, zeroing the FIX_CELL components first.
+// ---------------------------------------------------------------------
+template <int N, int R, int C>
+__global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const double *__restrict__ z0,
+                                                           const double *__restrict__ z1, double *u0, double *u1,
+                                                           double *u2, double *uk_dc, int fix_mask,
+                                                           const double2 *__restrict__ tw) {
+  constexpr int NT = C * N / R, E = N * C / NT;
+  __shared__ double2 lds[N * C];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * C, j = blockIdx.y;
+  const long H = g.hnz;
+  const long base = (long)j * H + c0;  // complex index of (i = 0, j, c0)
+  const long es = (long)g.ny * H;
+  double2 za[E], zb[E];
+#pragma unroll
+  for (int t = 0; t < E; ++t) {
+    const int f = tid + t * NT, c = f % C, i = f / C;
+    za[t] = zb[t] = double2{0.0, 0.0};
+    if (c0 + c < H) {
+      const ModeK m = mode_k(g, i, j, c0 + c);
+      if (m.keep) {
+        const long q = 2 * (base + i * es + c);
+        za[t] = *reinterpret_cast<const double2 *>(z0 + q);
+        zb[t] = *reinterpret_cast<const double2 *>(z1 + q);
+      }
+    }
+  }
+  double dc[3] = {0.0, 0.0, 0.0};
+  const bool has0 = (j == 0 && c0 == 0);
+  if (has0) {
+    for (int d = 0; d < 3; ++d) dc[d] = (fix_mask & (1 << d)) ? 0.0 : uk_dc[d];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double *out = d == 0 ? u0 : (d == 1 ? u1 : u2);
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, c = f % C, i = f / C;
+      const ModeK m = mode_k(g, i, j, c0 + c);
+      double2 v = zeta2u_mode(g, m, za[t], zb[t], d);
+      if (has0 && f == 0) v.x = dc[d];
+      lds[i * C + c] = v;
+    }
+    __syncthreads();
+    fft_stages<double, N, R, C, C, -1, 1>(lds, tw, tid);
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, c = f % C, i = f / C;
+      if (c0 + c < H) *reinterpret_cast<double2 *>(out + 2 * (base + i * es + c)) = lds[i * C + c];
+    }
+    __syncthreads();
+  }
+  if (has0 && tid == 0)
+    for (int d = 0; d < 3; ++d)
+      if (fix_mask & (1 << d)) uk_dc[d] = 0.0;
+}
+
+// ---------------------------------------------------------------------
+// z pass: C2R of (u0, u1, u2) -> real products of U2advection_k -> R2C of
+// the five products into (p0 .. p4).  p0..p2 may alias u0..u2.
+// ---------------------------------------------------------------------
+template <int M, int R, int ROWS>
+__device__ __forceinline__ void z_c2r_row_to_lds(double2 *lds, const double *a, long row0, int nrows, int nz_,
+                                                 const double2 *__restrict__ twr, int tid) {
+  constexpr int NT = ROWS * M / R, CP = ROWS + 1, H = M + 1, HP = M / 2 + 1;
+  for (int f = tid; f < ROWS * H; f += NT) {
+    const int k = f % H, r = f / H;
+    const long row = row0 + r;
+    lds[k * CP + r] = (row < nrows) ? *reinterpret_cast<const double2 *>(a + row * nz_ + 2 * k) : double2{0, 0};
+  }
+  __syncthreads();
+  for (int f = tid; f < ROWS * HP; f += NT) {
+    const int p = f % HP, r = f / HP;
+    const int k = p, kk = M - p;
+    const double2 xk = lds[k * CP + r], xkk = lds[kk * CP + r];
+    {
+      const double2 cj{xkk.x, -xkk.y};
+      const double2 e = cadd(xk, cj), d = csub(xk, cj);
+      const double2 w = twr[k];
+      const double2 o = cmul(d, double2{w.x, -w.y});
+      lds[k * CP + r] = double2{e.x - o.y, e.y + o.x};
+    }
+    if (p > 0 && kk != k) {
+      const double2 cj{xk.x, -xk.y};
+      const double2 e = cadd(xkk, cj), d = csub(xkk, cj);
+      const double2 w = twr[kk];
+      const double2 o = cmul(d, double2{w.x, -w.y});
+      lds[kk * CP + r] = double2{e.x - o.y, e.y + o.x};
+    }
+  }
+  __syncthreads();
+}
+
+// writes the forward z transform of the LDS rows (packed real pairs) to `a`
+template <int M, int R, int ROWS>
+__device__ __forceinline__ void z_r2c_lds_to_row(double2 *lds, double *a, long row0, int nrows, int nz_,
+                                                 const double2 *__restrict__ tw, const double2 *__restrict__ twr,
+                                                 int tid) {
+  constexpr int NT = ROWS * M / R, CP = ROWS + 1, H = M + 1;
+  fft_stages<double, M, R, ROWS, CP, +1, 1>(lds, tw, tid);
+  for (int f = tid; f < ROWS * H; f += NT) {
+    const int k = f % H, r = f / H;
+    const long row = row0 + r;
+    const double2 z = lds[(k % M) * CP + r];
+    double2 zc = lds[((M - k) % M) * CP + r];
+    zc.y = -zc.y;
+    const double2 e{0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y)};
+    const double2 o{0.5 * (z.y - zc.y), -0.5 * (z.x - zc.x)};
+    const double2 w = twr[k];
+    const double2 x = cadd(e, cmul(o, w));
+    if (row < nrows) *reinterpret_cast<double2 *>(a + row * nz_ + 2 * k) = x;
+  }
+  __syncthreads();
+}
+
+template <int M, int R, int ROWS>
+__global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, double *u0, double *u1, double *u2, double *p3,
+                                                              double *p4, const double2 *__restrict__ tw,
+                                                              const double2 *__restrict__ twr, double scale) {
+  constexpr int NT = ROWS * M / R, CP = ROWS + 1, E = ROWS * M / NT;
+  __shared__ double2 lds[(M + 1) * CP];
+  const int tid = threadIdx.x;
+  const long row0 = (long)blockIdx.x * ROWS;
+  const int nrows = g.nx * g.ny;
+  double2 v[3][E];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    z_c2r_row_to_lds<M, R, ROWS>(lds, d == 0 ? u0 : (d == 1 ? u1 : u2), row0, nrows, g.nz_, twr, tid);
+    fft_stages<double, M, R, ROWS, CP, -1, 1>(lds, tw, tid);
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, m = f % M, r = f / M;
+      const double2 x = lds[m * CP + r];
+      v[d][t] = double2{x.x * scale, x.y * scale};
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    double *out = q == 0 ? u0 : q == 1 ? u1 : q == 2 ? u2 : q == 3 ? p3 : p4;
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, m = f % M, r = f / M;
+      double2 pr;
+      const double a1 = v[0][t].x, a2 = v[1][t].x, a3 = v[2][t].x;
+      const double b1 = v[0][t].y, b2 = v[1][t].y, b3 = v[2][t].y;
+      if (q == 0) pr = double2{a2 * a3, b2 * b3};
+      else if (q == 1) pr = double2{a3 * a1, b3 * b1};
+      else if (q == 2) pr = double2{a1 * a2, b1 * b2};
+      else if (q == 3) pr = double2{(a2 - a3) * (a2 + a3), (b2 - b3) * (b2 + b3)};
+      else pr = double2{(a3 - a1) * (a3 + a1), (b3 - b1) * (b3 + b1)};
+      lds[m * CP + r] = pr;
+    }
+    __syncthreads();
+    z_r2c_lds_to_row<M, R, ROWS>(lds, out, row0, nrows, g.nz_, tw, twr, tid);
+  }
+}
+
+// ---------------------------------------------------------------------
+// x pass (forward) of the five product spectra with the advection
+// contraction, Omega_k2zeta_k and the slaved-Euler update as epilogue.
+// The transformed products are not stored.
+// ---------------------------------------------------------------------
+template <int N, int R, int C>
+__global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const double *p0, const double *p1,
+                                                           const double *p2, const double *p3, const double *p4,
+                                                           double *z0, double *z1, double nu, double dt,
+                                                           const double2 *__restrict__ tw) {
+  constexpr int NT = C * N / R, E = N * C / NT;
+  __shared__ double2 lds[N * C];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * C, j = blockIdx.y;
+  const long H = g.hnz;
+  const long base = (long)j * H + c0;
+  const long es = (long)g.ny * H;
+  // The contraction (operate_omega.cxx:60-97) is linear in the five product
+  // spectra P0..P4 = u2u3, u3u1, u1u2, u2^2-u3^2, u3^2-u1^2: accumulate the
+  // two components Omega_k2zeta_k keeps as each product is transformed.
+  double2 acc0[E], acc1[E];
+#pragma unroll
+  for (int t = 0; t < E; ++t) acc0[t] = acc1[t] = double2{0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double *in = q == 0 ? p0 : q == 1 ? p1 : q == 2 ? p2 : q == 3 ? p3 : p4;
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, c = f % C, i = f / C;
+      lds[i * C + c] = (c0 + c < H) ? *reinterpret_cast<const double2 *>(in + 2 * (base + i * es + c))
+                                    : double2{0.0, 0.0};
+    }
+    __syncthreads();
+    fft_stages<double, N, R, C, C, +1, 1>(lds, tw, tid);
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, c = f % C, i = f / C;
+      const double2 P = lds[i * C + c];
+      const ModeK m = mode_k(g, i, j, c0 + c);
+      const double k1 = m.kx * g.wave_x, k2 = m.ky * g.wave_y, k3 = m.kz * g.wave_z;
+      // coefficient of P_q in -Omega_s, s = 0, 1, 2
+      double cf[3];
+      if (q == 0) {
+        cf[0] = (k3 - k2) * (k3 + k2); cf[1] = k1 * k2; cf[2] = -(k3 * k1);
+      } else if (q == 1) {
+        cf[0] = -(k1 * k2); cf[1] = (k1 - k3) * (k1 + k3); cf[2] = k2 * k3;
+      } else if (q == 2) {
+        cf[0] = k3 * k1; cf[1] = -(k2 * k3); cf[2] = (k2 - k1) * (k2 + k1);
+      } else if (q == 3) {
+        cf[0] = k2 * k3; cf[1] = 0.0; cf[2] = -(k1 * k2);
+      } else {
+        cf[0] = 0.0; cf[1] = k3 * k1; cf[2] = -(k1 * k2);
+      }
+      const int s0 = m.kx != 0 ? 1 : (m.ky != 0 ? 2 : 0);
+      const int s1 = m.kx != 0 ? 2 : (m.ky != 0 ? 0 : 1);
+      const double a = cf[s0], b = cf[s1];
+      acc0[t].x -= a * P.x;
+      acc0[t].y -= a * P.y;
+      acc1[t].x -= b * P.x;
+      acc1[t].y -= b * P.y;
+    }
+    __syncthreads();
+  }
+  const double dmy0 = -nu * dt, dmy1 = 1. / nu;
+#pragma unroll
+  for (int t = 0; t < E; ++t) {
+    const int f = tid + t * NT, c = f % C, i = f / C;
+    if (c0 + c >= H) continue;
+    const ModeK m = mode_k(g, i, j, c0 + c);
+    const long q = 2 * (base + i * es + c);
+    double2 za{0.0, 0.0}, zb{0.0, 0.0};
+    if (m.keep) {
+      const double kk2 = sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz);
+      const double ik2 = kk2 > 0.0 ? 1. / kk2 : 0.0;
+      const double dmy = exp(dmy0 * kk2) - 1.;
+      za = *reinterpret_cast<const double2 *>(z0 + q);
+      zb = *reinterpret_cast<const double2 *>(z1 + q);
+      za.x += dmy * (za.x - dmy1 * ik2 * acc0[t].x);
+      za.y += dmy * (za.y - dmy1 * ik2 * acc0[t].y);
+      zb.x += dmy * (zb.x - dmy1 * ik2 * acc1[t].x);
+      zb.y += dmy * (zb.y - dmy1 * ik2 * acc1[t].y);
+    }
+    *reinterpret_cast<double2 *>(z0 + q) = za;
+    *reinterpret_cast<double2 *>(z1 + q) = zb;
+  }
+}
+
+// ---------------------------------------------------------------------
+// z pass (forward) of f = up - phi u, with phi and up evaluated per grid
+// point from the tile lists (Make_phi_u_primitive with the clamp,
+// make_phi.cxx:221-298) -- the phi/up fields are never stored.
+// ---------------------------------------------------------------------
+struct PhiUp {
+  double phi, u0, u1, u2;
+};
+
+__device__ __forceinline__ PhiUp phi_up_at(const Grid &g, const PModel &pm, const double *__restrict__ pb,
+                                           const int *__restrict__ pxint, const double *__restrict__ pres,
+                                           const int *__restrict__ tile_start, const int *__restrict__ tile_count,
+                                           const int *__restrict__ tile_list, int tile, int gi, int gj, int gk) {
+  const int N[3] = {g.nx, g.ny, g.nz};
+  const int gg[3] = {gi, gj, gk};
+  const double x[3] = {gi * pm.dx, gj * pm.dx, gk * pm.dx};
+  const double L[3] = {g.lx, g.ly, g.lz};
+  const double rcut = pm.radius + pm.hxi;
+  const int s0 = tile_start[tile], cnt = tile_count[tile];
+  double ph = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  const long np = pm.np;
+  for (int q = 0; q < cnt; ++q) {
+    const int n = tile_list[s0 + q];
+    int cell[3];
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int o = (gg[d] - pxint[3 * n + d]) % N[d];
+      if (o < 0) o += N[d];
+      if (o >= N[d] / 2) o -= N[d];
+      cell[d] = o;
+      in = in && (o <= pm.nh && o >= -pm.nh);
+    }
+    if (!in) continue;
+    double dd = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double x12 = pb[(P_X + d) * np + n] - x[d];
+      x12 -= (double)nint(x12 / L[d]) * L[d];
+      dd += sq(x12);
+    }
+    const double dist = sqrt(dd);
+    if (!(dist < rcut)) continue;
+    const double h1 = (pm.radius + pm.hxi - dist) > 0 ? exp(-sq(pm.dx / (pm.radius + pm.hxi - dist))) : 0.0;
+    const double h2 = (dist - pm.radius + pm.hxi) > 0 ? exp(-sq(pm.dx / (dist - pm.radius + pm.hxi))) : 0.0;
+    const double dphi = h1 / (h1 + h2);
+    ph += dphi;
+    double vr0 = 0.0, vr1 = 0.0, vr2 = 0.0;
+    if (pm.rotation) {
+      const double r0 = (double)cell[0] * pm.dx - pres[3 * n];
+      const double r1 = (double)cell[1] * pm.dx - pres[3 * n + 1];
+      const double r2 = (double)cell[2] * pm.dx - pres[3 * n + 2];
+      const double w0 = pb[P_OMEGA * np + n], w1 = pb[(P_OMEGA + 1) * np + n], w2 = pb[(P_OMEGA + 2) * np + n];
+      vr0 = w1 * r2 - w2 * r1;
+      vr1 = w2 * r0 - w0 * r2;
+      vr2 = w0 * r1 - w1 * r0;
+    }
+    a0 += (pb[P_V * np + n] + vr0) * dphi;
+    a1 += (pb[(P_V + 1) * np + n] + vr1) * dphi;
+    a2 += (pb[(P_V + 2) * np + n] + vr2) * dphi;
+  }
+  if (ph > 1.) {
+    const double inv = 1. / ph;
+    a0 *= inv;
+    a1 *= inv;
+    a2 *= inv;
+    ph = 1.;
+  }
+  return PhiUp{ph, a0, a1, a2};
+}
+
+template <int M, int R, int ROWS>
+__global__ __launch_bounds__(ROWS *M / R) void z_makef_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+                                                             const int *__restrict__ pxint,
+                                                             const double *__restrict__ pres,
+                                                             const int *__restrict__ tile_start,
+                                                             const int *__restrict__ tile_count,
+                                                             const int *__restrict__ tile_list, int tx, int ty, int tz,
+                                                             const double *__restrict__ u0,
+                                                             const double *__restrict__ u1,
+                                                             const double *__restrict__ u2, double *f0, double *f1,
+                                                             double *f2, const double2 *__restrict__ tw,
+                                                             const double2 *__restrict__ twr) {
+  constexpr int NT = ROWS * M / R, CP = ROWS + 1, E = ROWS * M / NT;
+  __shared__ double2 lds[(M + 1) * CP];
+  const int tid = threadIdx.x;
+  const long row0 = (long)blockIdx.x * ROWS;
+  const int nrows = g.nx * g.ny;
+  double2 fv[3][E];
+#pragma unroll
+  for (int t = 0; t < E; ++t) {
+    const int f = tid + t * NT, m = f % M, r = f / M;
+    const long row = row0 + r;
+    fv[0][t] = fv[1][t] = fv[2][t] = double2{0.0, 0.0};
+    if (row >= nrows) continue;
+    const int gi = (int)(row / g.ny), gj = (int)(row % g.ny);
+    double res[3][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int gk = 2 * m + h;
+      const int tile = ((gi / 8) * ty + gj / 8) * tz + gk / 8;
+      const PhiUp pu = phi_up_at(g, pm, pb, pxint, pres, tile_start, tile_count, tile_list, tile, gi, gj, gk);
+      const long im = row * g.nz_ + gk;
+      res[0][h] = pu.u0 - pu.phi * u0[im];
+      res[1][h] = pu.u1 - pu.phi * u1[im];
+      res[2][h] = pu.u2 - pu.phi * u2[im];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) fv[d][t] = double2{res[d][0], res[d][1]};
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, m = f % M, r = f / M;
+      lds[m * CP + r] = fv[d][t];
+    }
+    __syncthreads();
+    z_r2c_lds_to_row<M, R, ROWS>(lds, d == 0 ? f0 : (d == 1 ? f1 : f2), row0, nrows, g.nz_, tw, twr, tid);
+  }
+}
+
+// ---------------------------------------------------------------------
+// x pass (forward) of f, then zeta += U_k2zeta_k(mask(f)) and
+// uk_dc += Re f(0).  The reference projects f (Solenoidal_u), adds it to u
+// in real space and takes U2zeta_k of the sum; U_k2zeta_k is linear, maps
+// the spectrum of u back to zeta exactly (u = Zeta_k2u(zeta)) and sends the
+// gradient part removed by the projection to zero, while the projection
+// keeps the k = 0 mode (IK2(0) = 0).  So the result is the same field.
+// ---------------------------------------------------------------------
+template <int N, int R, int C>
+__global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const double *f0, const double *f1,
+                                                            const double *f2, double *z0, double *z1, double *uk_dc,
+                                                            const double2 *__restrict__ tw) {
+  constexpr int NT = C * N / R, E = N * C / NT;
+  __shared__ double2 lds[N * C];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * C, j = blockIdx.y;
+  const long H = g.hnz;
+  const long base = (long)j * H + c0;
+  const long es = (long)g.ny * H;
+  double2 acc0[E], acc1[E];
+  double f00[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < E; ++t) acc0[t] = acc1[t] = double2{0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double *in = q == 0 ? f0 : (q == 1 ? f1 : f2);
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, c = f % C, i = f / C;
+      lds[i * C + c] = (c0 + c < H) ? *reinterpret_cast<const double2 *>(in + 2 * (base + i * es + c))
+                                    : double2{0.0, 0.0};
+    }
+    __syncthreads();
+    fft_stages<double, N, R, C, C, +1, 1>(lds, tw, tid);
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, c = f % C, i = f / C;
+      const double2 F = lds[i * C + c];
+      if (f == 0) f00[q] = F.x;
+      const ModeK m = mode_k(g, i, j, c0 + c);
+      const double kx = m.kx * g.wave_x, ky = m.ky * g.wave_y, kz = m.kz * g.wave_z;
+      // omega = U_k2zeta_k's curl: omega_s gets (ks[a] * F_b - ks[b] * F_a)
+      // with (s, a, b) cyclic; omega_re = .. * Im, omega_im = -(..) * Re
+      const int s0 = m.kx != 0 ? 1 : (m.ky != 0 ? 2 : 0);
+      const int s1 = m.kx != 0 ? 2 : (m.ky != 0 ? 0 : 1);
+      // coefficient of F_q in omega_s: +ks[s-1] for q = s+1, -ks[s+1] for q = s-1
+      // (s, q) -> coefficient: (0,2) ky, (0,1) -kz, (1,0) kz, (1,2) -kx, (2,1) kx, (2,0) -ky
+      auto coef = [&](int s) -> double {
+        if (s == 0) return q == 2 ? ky : (q == 1 ? -kz : 0.0);
+        if (s == 1) return q == 0 ? kz : (q == 2 ? -kx : 0.0);
+        return q == 1 ? kx : (q == 0 ? -ky : 0.0);
+      };
+      const double a = coef(s0), b = coef(s1);
+      if (m.keep) {
+        acc0[t].x += a * F.y;
+        acc0[t].y -= a * F.x;
+        acc1[t].x += b * F.y;
+        acc1[t].y -= b * F.x;
+      }
+    }
+    __syncthreads();
+  }
+  const bool has0 = (j == 0 && c0 == 0);
+#pragma unroll
+  for (int t = 0; t < E; ++t) {
+    const int f = tid + t * NT, c = f % C, i = f / C;
+    if (c0 + c >= H) continue;
+    const ModeK m = mode_k(g, i, j, c0 + c);
+    const long q = 2 * (base + i * es + c);
+    double2 za{0.0, 0.0}, zb{0.0, 0.0};
+    if (m.keep) {
+      za = *reinterpret_cast<const double2 *>(z0 + q);
+      zb = *reinterpret_cast<const double2 *>(z1 + q);
+      za = cadd(za, acc0[t]);
+      zb = cadd(zb, acc1[t]);
+    }
+    *reinterpret_cast<double2 *>(z0 + q) = za;
+    *reinterpret_cast<double2 *>(z1 + q) = zb;
+    if (has0 && f == 0)
+      for (int d = 0; d < 3; ++d) uk_dc[d] += f00[d];
+  }
+}
+
+constexpr int kLinesC(int n) { return n >= 4096 ? 1 : (4096 / n > 32 ? 32 : 4096 / n); }
+constexpr int kRows(int m) { return 2048 / m > 32 ? 32 : (2048 / m < 1 ? 1 : 2048 / m); }
+constexpr int kR4(int n) { return n >= 4 ? 4 : n; }
+// fused x passes: half the columns of a plain pass, 512-thread blocks, so
+// each thread may hold up to 256 VGPRs
+constexpr int kFusedC(int n) { return kLinesC(n) / 2 > 0 ? kLinesC(n) / 2 : 1; }
+
+#define SPM_SIZE_SWITCH(n, ...)                \
+  switch (n) {                                  \
+    case 8: { constexpr int S = 8; __VA_ARGS__; } break;     \
+    case 16: { constexpr int S = 16; __VA_ARGS__; } break;   \
+    case 32: { constexpr int S = 32; __VA_ARGS__; } break;   \
+    case 64: { constexpr int S = 64; __VA_ARGS__; } break;   \
+    case 128: { constexpr int S = 128; __VA_ARGS__; } break; \
+    case 256: { constexpr int S = 256; __VA_ARGS__; } break; \
+    case 512: { constexpr int S = 512; __VA_ARGS__; } break; \
+    case 1024: { constexpr int S = 1024; __VA_ARGS__; } break; \
+    default: return SPM_ERR_UNSUPPORTED;        \
+  }
+
+#define SPM_ZSIZE_SWITCH(n, ...)               \
+  switch (n) {                                  \
+    case 4: { constexpr int S = 4; __VA_ARGS__; } break;     \
+    case 8: { constexpr int S = 8; __VA_ARGS__; } break;     \
+    case 16: { constexpr int S = 16; __VA_ARGS__; } break;   \
+    case 32: { constexpr int S = 32; __VA_ARGS__; } break;   \
+    case 64: { constexpr int S = 64; __VA_ARGS__; } break;   \
+    case 128: { constexpr int S = 128; __VA_ARGS__; } break; \
+    case 256: { constexpr int S = 256; __VA_ARGS__; } break; \
+    case 512: { constexpr int S = 512; __VA_ARGS__; } break; \
+    default: return SPM_ERR_UNSUPPORTED;        \
+  }
+
+inline double field_bytes(const spm_ctx *c) { return (double)c->nfield * sizeof(double); }
+
+}  // namespace
+
+int fx_zeta2u(spm_ctx *c, int fix_mask) {
+  const Grid &g = c->grid;
+  tbeg(c);
+  SPM_SIZE_SWITCH(g.nx, {
+    constexpr int R = kR4(S), C = kFusedC(S);
+    hipLaunchKernelGGL((x_zeta2u_kernel<S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream, g,
+                       c->zeta[0], c->zeta[1], c->u[0], c->u[1], c->u[2], c->uk_dc, fix_mask, c->tw_x);
+  });
+  tend(c, SPM_K_FFT_X, 5.0 * field_bytes(c));
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int fz_advect(spm_ctx *c) {
+  const Grid &g = c->grid;
+  const double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+  tbeg(c);
+  SPM_ZSIZE_SWITCH(g.nz / 2, {
+    constexpr int R = kR4(S), ROWS = kRows(S);
+    hipLaunchKernelGGL((z_advect_kernel<S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R), 0,
+                       c->stream, g, c->u[0], c->u[1], c->u[2], c->work[0], c->work[1], c->tw_z, c->tw_rz, scale);
+  });
+  tend(c, SPM_K_FFT_Z_C2R, 8.0 * field_bytes(c));
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int fx_update(spm_ctx *c, double nu, double dt) {
+  const Grid &g = c->grid;
+  tbeg(c);
+  SPM_SIZE_SWITCH(g.nx, {
+    constexpr int R = kR4(S), C = kFusedC(S);
+    hipLaunchKernelGGL((x_update_kernel<S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream, g,
+                       c->u[0], c->u[1], c->u[2], c->work[0], c->work[1], c->zeta[0], c->zeta[1], nu, dt, c->tw_x);
+  });
+  tend(c, SPM_K_FFT_X, 9.0 * field_bytes(c));
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int fz_makef(spm_ctx *c) {
+  const Grid &g = c->grid;
+  tbeg(c);
+  SPM_ZSIZE_SWITCH(g.nz / 2, {
+    constexpr int R = kR4(S), ROWS = kRows(S);
+    hipLaunchKernelGGL((z_makef_kernel<S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R), 0,
+                       c->stream, g, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count, c->tile_list,
+                       c->tiles[0], c->tiles[1], c->tiles[2], c->u[0], c->u[1], c->u[2], c->f[0], c->f[1], c->f[2],
+                       c->tw_z, c->tw_rz);
+  });
+  tend(c, SPM_K_FFT_Z_R2C, 6.0 * field_bytes(c));
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int fx_project(spm_ctx *c) {
+  const Grid &g = c->grid;
+  tbeg(c);
+  SPM_SIZE_SWITCH(g.nx, {
+    constexpr int R = kR4(S), C = kFusedC(S);
+    hipLaunchKernelGGL((x_project_kernel<S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream,
+                       g, c->f[0], c->f[1], c->f[2], c->zeta[0], c->zeta[1], c->uk_dc, c->tw_x);
+  });
+  tend(c, SPM_K_FFT_X, 7.0 * field_bytes(c));
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+}  // namespace spm

@@ -232,6 +232,46 @@ int time_evolution_hydro(spm_ctx *c) {
   return 0;
 }
 
+// The same step as fused FFT passes (fused.hip).  Leaves zeta, uk_dc and the
+// particles as the literal schedule does; u, phi and up are rebuilt by
+// materialize() when a spm_step call returns, f is scratch.
+int fused_step(spm_ctx *c) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  double *const p5[5] = {c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]};
+  double *const f3[3] = {c->f[0], c->f[1], c->f[2]};
+  const double nu = c->prm.eta * (1. / c->prm.rho);
+  // NS_solver_slavedEuler
+  RC(fx_zeta2u(c, 0));
+  RC(fft_pass_y(c, u3, 3, -1));
+  RC(fz_advect(c));
+  RC(fft_pass_y(c, p5, 5, +1));
+  RC(fx_update(c, nu, c->prm.dt));
+  // FIX_CELL, truncation, Zeta_k2u
+  RC(fx_zeta2u(c, fix_mask(c)));
+  RC(fft_pass_y(c, u3, 3, -1));
+  RC(fft_pass_z_c2r(c, u3, 3));
+  // particles
+  RC(p_position(c, c->ts == 0));
+  RC(p_calc_f_hydro(c));  // also builds the tile lists at the new positions
+  RC(p_velocity(c, c->ts == 0));
+  RC(thermostat(c));
+  // Make_phi_u_particle + Make_f_particle_dt_sole + Add_f_particle + U2zeta_k
+  // (with no particles the tile lists stay empty and f = 0)
+  RC(fz_makef(c));
+  RC(fft_pass_y(c, f3, 3, +1));
+  RC(fx_project(c));
+  return 0;
+}
+
+// u (real space), phi and up as the literal schedule leaves them.
+int materialize(spm_ctx *c) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  RC(k_zeta_k2u_k(c, c->zeta[0], c->zeta[1], c->u[0], c->u[1], c->u[2], 0));
+  RC(fft_c2r_many(c, u3, 3));
+  RC(p_make_phi(c, 1));
+  return 0;
+}
+
 void free_ctx(spm_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
@@ -641,8 +681,9 @@ static int calc_f_hydro_api(spm_ctx *c) {
 }
 
 static int step_n(spm_ctx *c, int nsteps) {
+  const bool fused = c->schedule == SPM_SCHEDULE_FUSED;
   for (int s = 0; s < nsteps; ++s) {
-    RC(time_evolution_hydro(c));
+    RC(fused ? fused_step(c) : time_evolution_hydro(c));
     c->time += c->prm.dt;
     c->ts++;
     // keep the event pool bounded when timing long runs
@@ -651,6 +692,7 @@ static int step_n(spm_ctx *c, int nsteps) {
       RC(fold_timing(c));
     }
   }
+  if (fused && nsteps > 0) RC(materialize(c));
   return 0;
 }
 
@@ -694,6 +736,12 @@ int spm_synchronize(spm_ctx *c) {
 }
 
 int spm_check_finite(spm_ctx *c) { ENTRY(check_finite(c)); }
+
+int spm_set_schedule(spm_ctx *c, int schedule) {
+  if (!c || (schedule != SPM_SCHEDULE_FUSED && schedule != SPM_SCHEDULE_LITERAL)) return SPM_ERR_INVALID;
+  c->schedule = schedule;
+  return SPM_OK;
+}
 
 int spm_timing_enable(spm_ctx *c, int on) {
   if (!c) return SPM_ERR_INVALID;

@@ -104,6 +104,7 @@ struct spm_ctx {
   size_t nfield = 0;  // elements per field
   int ts = 0;
   double time = 0.0;
+  int schedule = SPM_SCHEDULE_FUSED;
 
   // fields (double path)
   double *zeta[2] = {nullptr, nullptr};
@@ -205,6 +206,15 @@ int fft_c2r(spm_ctx *c, double *a);
 int fft_r2c_many(spm_ctx *c, double *const *a, int n);
 int fft_c2r_many(spm_ctx *c, double *const *a, int n);
 int fft_init_tables(spm_ctx *c);
+int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign);
+int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n);
+
+// fused.hip: FFT passes carrying the pointwise work of the step
+int fx_zeta2u(spm_ctx *c, int fix_mask);
+int fz_advect(spm_ctx *c);
+int fx_update(spm_ctx *c, double nu, double dt);
+int fz_makef(spm_ctx *c);
+int fx_project(spm_ctx *c);
 
 // spectral.hip
 int k_truncate(spm_ctx *c, double *a);
