@@ -53,26 +53,10 @@ def algorithmic_step_bytes(n, np_, stencil, w=8, n_fft=20):
     return n_fft * (8 * nn + 16 * n * n * (n // 2 + 1)) * (w / 8) + np_ * stencil * 11 * w
 
 
-def timing_read(lib, h):
-    import ctypes as C
-    ms = (C.c_double * 9)()
-    by = (C.c_double * 9)()
-    cnt = (C.c_long * 9)()
-    rc = lib.spm_timing_read(h, ms, by, cnt)
-    if rc:
-        raise RuntimeError("spm_timing_read failed (%d)" % rc)
-    return list(ms), list(by), list(cnt)
-
-
-def gpu_run(n, np_, steps, warmup, profile_steps):
-    import ctypes as C
-
-    from kapsel_old_amd.spm import SPM, lib
-    L = lib()
-    L.spm_timing_enable.argtypes = [C.c_void_p, C.c_int]
-    L.spm_timing_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                  C.POINTER(C.c_long)]
+def gpu_run(n, np_, steps, warmup, profile_steps, schedule):
+    from kapsel_old_amd.spm import SPM
     g = SPM(workload_params(n, np_))
+    g.set_schedule(schedule)
     g.put_particles(workload_particles(n, np_))
     g.init_fluid()
     g.step(warmup)
@@ -86,14 +70,14 @@ def gpu_run(n, np_, steps, warmup, profile_steps):
     print("bench: timed %d steps in %.3f s" % (steps, elapsed), file=sys.stderr, flush=True)
     g.check_finite()
     # kernel durations: HIP events around every launch of the same step loop
-    L.spm_timing_enable(g._h, 1)
-    timing_read(L, g._h)
+    g.timing_enable(True)
+    g.timing_read()
     t2 = time.perf_counter()
     g.step(profile_steps)
     g.synchronize()
     t3 = time.perf_counter()
-    ms, by, cnt = timing_read(L, g._h)
-    L.spm_timing_enable(g._h, 0)
+    ms, by, cnt = g.timing_read()
+    g.timing_enable(False)
     stencil = g.stencil_size()
     g.close()
     return elapsed, (t3 - t2), ms, by, cnt, stencil
@@ -128,13 +112,16 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--particles", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--schedule", choices=["fused", "literal"], default="fused")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (one box per GPU); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
     n, np_ = args.grid, args.particles
-    elapsed, prof_elapsed, ms, by, cnt, stencil = gpu_run(n, np_, args.steps, args.warmup, args.profile_steps)
+    schedule = 0 if args.schedule == "fused" else 1
+    elapsed, prof_elapsed, ms, by, cnt, stencil = gpu_run(n, np_, args.steps, args.warmup, args.profile_steps,
+                                                          schedule)
     value = args.steps / elapsed
     dom = int(np.argmax(ms))
     per_launch_ms = ms[dom] / max(cnt[dom], 1)
@@ -160,13 +147,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "SPM time step (Time_evolution_hydro), %d^3 grid, %d spheres a=5 xi=2, "
                                "WCA, kBT=0.1, dt=0.05, FP64 fields" % (n, np_),
-                   "grid": [n, n, n], "n_particles": np_, "stencil_points": stencil, "precision": "fp64"},
+                   "grid": [n, n, n], "n_particles": np_, "stencil_points": stencil, "precision": "fp64",
+                   "schedule": args.schedule},
         "hbm_TBps_algorithmic": b_step * value / 1e12,
         "step_floor_ms_at_8TBps": b_step / 8e12 * 1e3,
         "roofline": {"bound": "hbm", "kernel": FAMILIES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": per_launch_bytes, "us_per_launch": per_launch_ms * 1e3},
         "kernel_families": families,
+        "profiled_ms_per_step": 1e3 * prof_elapsed / args.profile_steps,
     }
     if not args.no_cpu_baseline:
         v, done, secs = cpu_baseline(n, np_)

@@ -212,6 +212,15 @@ int spm_synchronize(spm_ctx *ctx);
  *     field (f included) as the reference leaves it. */
 enum spm_schedule { SPM_SCHEDULE_FUSED = 0, SPM_SCHEDULE_LITERAL = 1 };
 int spm_set_schedule(spm_ctx *ctx, int schedule);
+/* Test hook (no counterpart in the reference): runs one stage of the fused
+ * schedule so tests can compare it with the literal routines --
+ *   1  NS_solver_slavedEuler
+ *   2  FIX_CELL + truncation + Zeta_k2u (sp_3d_ns.cxx:67-79)
+ *   3  Make_phi_u_particle .. U2zeta_k on the current u and particles
+ *      (leaves u and the f field as scratch)
+ *   4  rebuild u (the spectrum U2zeta_k leaves, operate_omega.h:186-189),
+ *      phi and up from zeta and the particles. */
+int spm_fused_stage(spm_ctx *ctx, int stage);
 /* Scans u and zeta for NaN/Inf; SPM_ERR_NONFINITE if any is found. */
 int spm_check_finite(spm_ctx *ctx);
 

@@ -304,6 +304,25 @@ int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign) {
   return 0;
 }
 
+int fft_pass_z_r2c(spm_ctx *c, double *const *a, int n) {
+  const Grid &g = c->grid;
+  for (int b = 0; b < n; b += 5) {
+    FieldPtrs<double> fp{};
+    const int nf = (n - b) < 5 ? (n - b) : 5;
+    for (int q = 0; q < nf; ++q) fp.p[q] = a[b + q];
+    const double bytes = 2.0 * nf * (double)c->nfield * sizeof(double);
+    tbeg(c);
+    const int rc = launch_z<double>(g.nz / 2, true, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, 1.0, c->stream);
+    tend(c, SPM_K_FFT_Z_R2C, bytes);
+    if (rc) {
+      c->err = "unsupported FFT size";
+      return SPM_ERR_UNSUPPORTED;
+    }
+    SPM_LAUNCH_CHECK(c);
+  }
+  return 0;
+}
+
 int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n) {
   const Grid &g = c->grid;
   const double scale = 1.0 / ((double)g.nx * g.ny * g.nz);

@@ -79,12 +79,16 @@ __device__ __forceinline__ void stockham_stage(cplx<T> *lds, const double2 *__re
 #pragma unroll
     for (int r = 0; r < RS; ++r) a[s * RS + r] = lds[(j + r * NB) * CP + c];
     if constexpr (NS > 1) {
+      // w^r for r = 1..RS-1 from one table load: w^2 and w^4 by squaring,
+      // odd powers by one more product (at most three roundings deep)
       const int k = j % NS;
+      const double2 t1 = tw[k * (N / (NS * RS))];
+      C2 w[RS];
+      w[1] = C2{(T)t1.x, (T)(SIGN * t1.y)};
 #pragma unroll
-      for (int r = 1; r < RS; ++r) {
-        const double2 w = tw[k * r * (N / (NS * RS))];
-        a[s * RS + r] = cmul(a[s * RS + r], C2{(T)w.x, (T)(SIGN * w.y)});
-      }
+      for (int r = 2; r < RS; ++r) w[r] = (r % 2 == 0) ? cmul(w[r / 2], w[r / 2]) : cmul(w[r - 1], w[1]);
+#pragma unroll
+      for (int r = 1; r < RS; ++r) a[s * RS + r] = cmul(a[s * RS + r], w[r]);
     }
     dft<C2, RS, SIGN>(&a[s * RS]);
   }

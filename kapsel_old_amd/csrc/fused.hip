@@ -16,9 +16,9 @@
 //   Zeta_k2u (sp_3d_ns.cxx:67-79): x_zeta2u (+FIX_CELL), y x3, z x3
 //   Make_phi_u_particle + Make_f_particle_dt_nonsole (make_phi.cxx:221-298,
 //   f_particle.h:41-64)
-//     z_makef    : phi and up computed per grid point from the tile lists,
-//                  f = up - phi u, z pass (forward) x3
-//     y pass x3  : forward
+//     make_phi (mode f, particles.hip): phi and up evaluated per grid point
+//                  over 8x8x8 tiles, f = up - phi u written directly
+//     z pass x3, y pass x3: forward
 //   Solenoidal_u + Add_f_particle + U2zeta_k (f_particle.h:71-99,
 //   operate_omega.h:186-189)
 //     x_project  : x pass (forward) x3 with 2/3 mask + projection of f, plus
@@ -26,7 +26,7 @@
 //
 // The last kernel uses R2C(C2R(u_k)) = u_k for the truncated, Hermitian
 // spectrum u_k: the reference transforms u + f back to k space, which is
-// the same field up to rounding.  About 74 field-passes per step remain.
+// the same field up to rounding.  About 80 field-passes per step remain.
 #include "fft_core.h"
 
 namespace spm {
@@ -330,128 +330,6 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const double
 }
 
 // ---------------------------------------------------------------------
-// z pass (forward) of f = up - phi u, with phi and up evaluated per grid
-// point from the tile lists (Make_phi_u_primitive with the clamp,
-// make_phi.cxx:221-298) -- the phi/up fields are never stored.
-// ---------------------------------------------------------------------
-struct PhiUp {
-  double phi, u0, u1, u2;
-};
-
-__device__ __forceinline__ PhiUp phi_up_at(const Grid &g, const PModel &pm, const double *__restrict__ pb,
-                                           const int *__restrict__ pxint, const double *__restrict__ pres,
-                                           const int *__restrict__ tile_start, const int *__restrict__ tile_count,
-                                           const int *__restrict__ tile_list, int tile, int gi, int gj, int gk) {
-  const int N[3] = {g.nx, g.ny, g.nz};
-  const int gg[3] = {gi, gj, gk};
-  const double x[3] = {gi * pm.dx, gj * pm.dx, gk * pm.dx};
-  const double L[3] = {g.lx, g.ly, g.lz};
-  const double rcut = pm.radius + pm.hxi;
-  const int s0 = tile_start[tile], cnt = tile_count[tile];
-  double ph = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  const long np = pm.np;
-  for (int q = 0; q < cnt; ++q) {
-    const int n = tile_list[s0 + q];
-    int cell[3];
-    bool in = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      int o = (gg[d] - pxint[3 * n + d]) % N[d];
-      if (o < 0) o += N[d];
-      if (o >= N[d] / 2) o -= N[d];
-      cell[d] = o;
-      in = in && (o <= pm.nh && o >= -pm.nh);
-    }
-    if (!in) continue;
-    double dd = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      double x12 = pb[(P_X + d) * np + n] - x[d];
-      x12 -= (double)nint(x12 / L[d]) * L[d];
-      dd += sq(x12);
-    }
-    const double dist = sqrt(dd);
-    if (!(dist < rcut)) continue;
-    const double h1 = (pm.radius + pm.hxi - dist) > 0 ? exp(-sq(pm.dx / (pm.radius + pm.hxi - dist))) : 0.0;
-    const double h2 = (dist - pm.radius + pm.hxi) > 0 ? exp(-sq(pm.dx / (dist - pm.radius + pm.hxi))) : 0.0;
-    const double dphi = h1 / (h1 + h2);
-    ph += dphi;
-    double vr0 = 0.0, vr1 = 0.0, vr2 = 0.0;
-    if (pm.rotation) {
-      const double r0 = (double)cell[0] * pm.dx - pres[3 * n];
-      const double r1 = (double)cell[1] * pm.dx - pres[3 * n + 1];
-      const double r2 = (double)cell[2] * pm.dx - pres[3 * n + 2];
-      const double w0 = pb[P_OMEGA * np + n], w1 = pb[(P_OMEGA + 1) * np + n], w2 = pb[(P_OMEGA + 2) * np + n];
-      vr0 = w1 * r2 - w2 * r1;
-      vr1 = w2 * r0 - w0 * r2;
-      vr2 = w0 * r1 - w1 * r0;
-    }
-    a0 += (pb[P_V * np + n] + vr0) * dphi;
-    a1 += (pb[(P_V + 1) * np + n] + vr1) * dphi;
-    a2 += (pb[(P_V + 2) * np + n] + vr2) * dphi;
-  }
-  if (ph > 1.) {
-    const double inv = 1. / ph;
-    a0 *= inv;
-    a1 *= inv;
-    a2 *= inv;
-    ph = 1.;
-  }
-  return PhiUp{ph, a0, a1, a2};
-}
-
-template <int M, int R, int ROWS>
-__global__ __launch_bounds__(ROWS *M / R) void z_makef_kernel(Grid g, PModel pm, const double *__restrict__ pb,
-                                                             const int *__restrict__ pxint,
-                                                             const double *__restrict__ pres,
-                                                             const int *__restrict__ tile_start,
-                                                             const int *__restrict__ tile_count,
-                                                             const int *__restrict__ tile_list, int tx, int ty, int tz,
-                                                             const double *__restrict__ u0,
-                                                             const double *__restrict__ u1,
-                                                             const double *__restrict__ u2, double *f0, double *f1,
-                                                             double *f2, const double2 *__restrict__ tw,
-                                                             const double2 *__restrict__ twr) {
-  constexpr int NT = ROWS * M / R, CP = ROWS + 1, E = ROWS * M / NT;
-  __shared__ double2 lds[(M + 1) * CP];
-  const int tid = threadIdx.x;
-  const long row0 = (long)blockIdx.x * ROWS;
-  const int nrows = g.nx * g.ny;
-  double2 fv[3][E];
-#pragma unroll
-  for (int t = 0; t < E; ++t) {
-    const int f = tid + t * NT, m = f % M, r = f / M;
-    const long row = row0 + r;
-    fv[0][t] = fv[1][t] = fv[2][t] = double2{0.0, 0.0};
-    if (row >= nrows) continue;
-    const int gi = (int)(row / g.ny), gj = (int)(row % g.ny);
-    double res[3][2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int gk = 2 * m + h;
-      const int tile = ((gi / 8) * ty + gj / 8) * tz + gk / 8;
-      const PhiUp pu = phi_up_at(g, pm, pb, pxint, pres, tile_start, tile_count, tile_list, tile, gi, gj, gk);
-      const long im = row * g.nz_ + gk;
-      res[0][h] = pu.u0 - pu.phi * u0[im];
-      res[1][h] = pu.u1 - pu.phi * u1[im];
-      res[2][h] = pu.u2 - pu.phi * u2[im];
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) fv[d][t] = double2{res[d][0], res[d][1]};
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-#pragma unroll
-    for (int t = 0; t < E; ++t) {
-      const int f = tid + t * NT, m = f % M, r = f / M;
-      lds[m * CP + r] = fv[d][t];
-    }
-    __syncthreads();
-    z_r2c_lds_to_row<M, R, ROWS>(lds, d == 0 ? f0 : (d == 1 ? f1 : f2), row0, nrows, g.nz_, tw, twr, tid);
-  }
-}
-
-// ---------------------------------------------------------------------
 // x pass (forward) of f, then zeta += U_k2zeta_k(mask(f)) and
 // uk_dc += Re f(0).  The reference projects f (Solenoidal_u), adds it to u
 // in real space and takes U2zeta_k of the sum; U_k2zeta_k is linear, maps
@@ -607,21 +485,6 @@ int fx_update(spm_ctx *c, double nu, double dt) {
                        c->u[0], c->u[1], c->u[2], c->work[0], c->work[1], c->zeta[0], c->zeta[1], nu, dt, c->tw_x);
   });
   tend(c, SPM_K_FFT_X, 9.0 * field_bytes(c));
-  SPM_LAUNCH_CHECK(c);
-  return 0;
-}
-
-int fz_makef(spm_ctx *c) {
-  const Grid &g = c->grid;
-  tbeg(c);
-  SPM_ZSIZE_SWITCH(g.nz / 2, {
-    constexpr int R = kR4(S), ROWS = kRows(S);
-    hipLaunchKernelGGL((z_makef_kernel<S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R), 0,
-                       c->stream, g, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count, c->tile_list,
-                       c->tiles[0], c->tiles[1], c->tiles[2], c->u[0], c->u[1], c->u[2], c->f[0], c->f[1], c->f[2],
-                       c->tw_z, c->tw_rz);
-  });
-  tend(c, SPM_K_FFT_Z_R2C, 6.0 * field_bytes(c));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

@@ -36,13 +36,16 @@ __device__ __forceinline__ double phi_of(const PModel &pm, double x) {
 }
 
 // Distance0 (interaction.h:110-123) from grid point x1 to particle x2.
+// x12/L is taken as x12 * (1/L): the nearest-image count differs only when
+// x12/L is within one rounding of a half-integer (a pair half a box apart).
 __device__ __forceinline__ double distance0(const Grid &g, const double *x1, const double *x2, double *x12) {
   const double L[3] = {g.lx, g.ly, g.lz};
+  const double iL[3] = {1.0 / g.lx, 1.0 / g.ly, 1.0 / g.lz};
   double dmy = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     x12[d] = x2[d] - x1[d];
-    x12[d] -= (double)nint(x12[d] / L[d]) * L[d];
+    x12[d] -= (double)nint(x12[d] * iL[d]) * L[d];
     dmy += sq(x12[d]);
   }
   return sqrt(dmy);
@@ -160,18 +163,24 @@ __global__ void izero_kernel(int *a, int n) {
 // make_phi: owner-computes over 8x8x8 tiles (one point per thread)
 // ---------------------------------------------------------------------
 // The stencil point of particle n that lands on grid index gi along an axis
-// of N points has offset cell = gi - x_int (mod N) taken in [-N/2, N/2);
-// it belongs to the stencil only if |cell| <= nh (the stencil cube).  Any
+// of N points has offset cell = gi - x_int taken into [-N/2, N/2); it
+// belongs to the stencil only if |cell| <= nh (the stencil cube).  Any
 // point with distance >= a + xi/2 has phi = 0 exactly and adds nothing, and
 // every point closer than that lies inside the stencil sphere
 // (a + xi/2 + |residue| < a + xi/2 + 1.7321 dx), so the distance test alone
 // reproduces the reference's sum over Sekibun_cell.
+//   mode 0: phi and up (Make_phi_u_particle, with the clamp)
+//   mode 1: phi only (Make_phi_particle, no clamp)
+//   mode 2: f = up - phi u (Make_f_particle_dt_nonsole on the fly; phi and
+//           up are not stored)
+template <typename T>
 __global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const double *__restrict__ pb,
                                                        const int *__restrict__ pxint, const double *__restrict__ pres,
                                                        const int *__restrict__ tile_start,
                                                        const int *__restrict__ tile_count,
                                                        const int *__restrict__ tile_list, int tx, int ty, int tz,
-                                                       double *phi, double *up0, double *up1, double *up2, int sw_up) {
+                                                       T *phi, T *up0, T *up1, T *up2, const T *__restrict__ u0,
+                                                       const T *__restrict__ u1, const T *__restrict__ u2, int mode) {
   const int tile = blockIdx.x;
   const int tk = tile % tz, tj = (tile / tz) % ty, ti = tile / (tz * ty);
   const int lk = threadIdx.x % kTile, lj = (threadIdx.x / kTile) % kTile, li = threadIdx.x / (kTile * kTile);
@@ -180,16 +189,17 @@ __global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const 
   const double x[3] = {gi[0] * pm.dx, gi[1] * pm.dx, gi[2] * pm.dx};
   const int s0 = tile_start[tile], cnt = tile_count[tile];
   const double rcut = pm.radius + pm.hxi;
-  double ph = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  const bool sw_up = mode != 1;
+  double ph = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
   for (int q = 0; q < cnt; ++q) {
     const int n = tile_list[s0 + q];
     int cell[3];
     bool in = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      int o = (gi[d] - pxint[3 * n + d]) % N[d];
-      if (o < 0) o += N[d];
+      int o = gi[d] - pxint[3 * n + d];  // |o| < N
       if (o >= N[d] / 2) o -= N[d];
+      if (o < -(N[d] / 2)) o += N[d];
       cell[d] = o;
       in = in && (o <= pm.nh && o >= -pm.nh);
     }
@@ -211,33 +221,36 @@ __global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const 
         vr[1] = w2 * r[0] - w0 * r[2];
         vr[2] = w0 * r[1] - w1 * r[0];
       }
-      u0 += (P(pb, pm.np, P_V, n) + vr[0]) * dphi;
-      u1 += (P(pb, pm.np, P_V + 1, n) + vr[1]) * dphi;
-      u2 += (P(pb, pm.np, P_V + 2, n) + vr[2]) * dphi;
+      a0 += (P(pb, pm.np, P_V, n) + vr[0]) * dphi;
+      a1 += (P(pb, pm.np, P_V + 1, n) + vr[1]) * dphi;
+      a2 += (P(pb, pm.np, P_V + 2, n) + vr[2]) * dphi;
     }
   }
   if (sw_up && ph > 1.) {  // make_phi.cxx:280-297
     const double inv = 1. / ph;
-    u0 *= inv;
-    u1 *= inv;
-    u2 *= inv;
+    a0 *= inv;
+    a1 *= inv;
+    a2 *= inv;
     ph = 1.;
   }
   const long im = ((long)gi[0] * g.ny + gi[1]) * g.nz_ + gi[2];
-  phi[im] = ph;
-  if (sw_up) {
-    up0[im] = u0;
-    up1[im] = u1;
-    up2[im] = u2;
+  const bool pad = gi[2] == g.nz - 1;  // also clear the two padding slots
+  if (mode == 2) {
+    // f_particle.h:55-57 with this point's phi and up
+    const T p = (T)ph;
+    up0[im] = (T)a0 - p * u0[im];
+    up1[im] = (T)a1 - p * u1[im];
+    up2[im] = (T)a2 - p * u2[im];
+    if (pad) up0[im + 1] = up0[im + 2] = up1[im + 1] = up1[im + 2] = up2[im + 1] = up2[im + 2] = (T)0;
+    return;
   }
-  if (gi[2] == g.nz - 1) {  // padding slots, zero as after Reset_phi_u
-    phi[im + 1] = 0.0;
-    phi[im + 2] = 0.0;
-    if (sw_up) {
-      up0[im + 1] = up0[im + 2] = 0.0;
-      up1[im + 1] = up1[im + 2] = 0.0;
-      up2[im + 1] = up2[im + 2] = 0.0;
-    }
+  phi[im] = (T)ph;
+  if (pad) phi[im + 1] = phi[im + 2] = (T)0;
+  if (sw_up) {
+    up0[im] = (T)a0;
+    up1[im] = (T)a1;
+    up2[im] = (T)a2;
+    if (pad) up0[im + 1] = up0[im + 2] = up1[im + 1] = up1[im + 2] = up2[im + 1] = up2[im + 2] = (T)0;
   }
 }
 
@@ -272,7 +285,10 @@ __global__ __launch_bounds__(256) void f_hydro_kernel(Grid g, PModel pm, double 
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const int cd = stencil[3 * s + d];
-      rm[d] = (xi[d] + cd + N[d]) % N[d];
+      int q = xi[d] + cd;  // in (-N, 2N)
+      if (q >= N[d]) q -= N[d];
+      if (q < 0) q += N[d];
+      rm[d] = q;
       r[d] = (double)cd * pm.dx - res[d];
       x[d] = rm[d] * pm.dx;
     }
@@ -565,6 +581,28 @@ int p_build_tiles(spm_ctx *c) {
   return 0;
 }
 
+static int launch_make_phi(spm_ctx *c, int mode, double *o0, double *o1, double *o2, double *o3) {
+  tbeg(c);
+  if (c->prm.precision == SPM_FP32) {
+    using T = float;
+    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf,
+                       c->pxint, c->pres, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1],
+                       c->tiles[2], (T *)o0, (T *)o1, (T *)o2, (T *)o3, (const T *)c->u[0], (const T *)c->u[1],
+                       (const T *)c->u[2], mode);
+  } else {
+    using T = double;
+    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf,
+                       c->pxint, c->pres, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1],
+                       c->tiles[2], (T *)o0, (T *)o1, (T *)o2, (T *)o3, (const T *)c->u[0], (const T *)c->u[1],
+                       (const T *)c->u[2], mode);
+  }
+  const double es = c->prm.precision == SPM_FP32 ? 4.0 : 8.0;
+  // mode 0 writes 4 fields, mode 1 one, mode 2 reads 3 and writes 3
+  tend(c, SPM_K_MAKE_PHI, (mode == 0 ? 4.0 : mode == 1 ? 1.0 : 6.0) * es * (double)c->nfield);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
 int p_make_phi(spm_ctx *c, int sw_up) {
   if (c->np == 0) {
     int rc = k_zero(c, c->phi, c->nfield);
@@ -576,15 +614,12 @@ int p_make_phi(spm_ctx *c, int sw_up) {
   }
   int rc = p_build_tiles(c);
   if (rc) return rc;
-  tbeg(c);
-  hipLaunchKernelGGL(make_phi_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf, c->pxint,
-                     c->pres, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2],
-                     c->phi, c->up[0], c->up[1], c->up[2], sw_up);
-  // writes phi (and up) once at every grid point
-  tend(c, SPM_K_MAKE_PHI, (sw_up ? 4.0 : 1.0) * 8.0 * (double)c->nfield);
-  SPM_LAUNCH_CHECK(c);
-  return 0;
+  return launch_make_phi(c, sw_up ? 0 : 1, c->phi, c->up[0], c->up[1], c->up[2]);
 }
+
+// f = up - phi u with phi, up evaluated on the fly (tile lists must be
+// current); with no particles f = -0 * u.
+int p_make_f(spm_ctx *c) { return launch_make_phi(c, 2, nullptr, c->f[0], c->f[1], c->f[2]); }
 
 int p_calc_f_hydro(spm_ctx *c) {
   if (c->np == 0) return 0;

@@ -233,8 +233,9 @@ int time_evolution_hydro(spm_ctx *c) {
 }
 
 // The same step as fused FFT passes (fused.hip).  Leaves zeta, uk_dc and the
-// particles as the literal schedule does; u, phi and up are rebuilt by
-// materialize() when a spm_step call returns, f is scratch.
+// particles as the literal schedule does; u (a spectrum at the end of a
+// step), phi and up are rebuilt by materialize() when a spm_step call
+// returns; f is scratch.
 int fused_step(spm_ctx *c) {
   double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
   double *const p5[5] = {c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]};
@@ -257,19 +258,56 @@ int fused_step(spm_ctx *c) {
   RC(thermostat(c));
   // Make_phi_u_particle + Make_f_particle_dt_sole + Add_f_particle + U2zeta_k
   // (with no particles the tile lists stay empty and f = 0)
-  RC(fz_makef(c));
+  RC(p_make_f(c));
+  RC(fft_pass_z_r2c(c, f3, 3));
   RC(fft_pass_y(c, f3, 3, +1));
   RC(fx_project(c));
   return 0;
 }
 
-// u (real space), phi and up as the literal schedule leaves them.
+// u, phi and up as the literal schedule leaves them.  The reference's step
+// ends with U2zeta_k, which leaves u transformed in place: u holds the
+// spectrum of u + f, i.e. Zeta_k2u_k of the new zeta (the field is
+// solenoidal and band-limited, so the two agree to rounding).
 int materialize(spm_ctx *c) {
-  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
   RC(k_zeta_k2u_k(c, c->zeta[0], c->zeta[1], c->u[0], c->u[1], c->u[2], 0));
-  RC(fft_c2r_many(c, u3, 3));
   RC(p_make_phi(c, 1));
   return 0;
+}
+
+// One stage of fused_step, for stage-by-stage tests against the literal
+// routines: 1 NS_solver_slavedEuler, 2 FIX_CELL + Zeta_k2u, 3 the particle
+// coupling tail (Make_phi_u_particle .. U2zeta_k, u and particles given),
+// 4 materialize.
+int fused_stage(spm_ctx *c, int stage) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  double *const p5[5] = {c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]};
+  double *const f3[3] = {c->f[0], c->f[1], c->f[2]};
+  const double nu = c->prm.eta * (1. / c->prm.rho);
+  switch (stage) {
+    case 1:
+      RC(fx_zeta2u(c, 0));
+      RC(fft_pass_y(c, u3, 3, -1));
+      RC(fz_advect(c));
+      RC(fft_pass_y(c, p5, 5, +1));
+      RC(fx_update(c, nu, c->prm.dt));
+      return 0;
+    case 2:
+      RC(fx_zeta2u(c, fix_mask(c)));
+      RC(fft_pass_y(c, u3, 3, -1));
+      RC(fft_pass_z_c2r(c, u3, 3));
+      return 0;
+    case 3:
+      RC(p_build_tiles(c));
+      RC(p_make_f(c));
+      RC(fft_pass_z_r2c(c, f3, 3));
+      RC(fft_pass_y(c, f3, 3, +1));
+      RC(fx_project(c));
+      return 0;
+    case 4:
+      return materialize(c);
+  }
+  return SPM_ERR_INVALID;
 }
 
 void free_ctx(spm_ctx *c) {
@@ -736,6 +774,8 @@ int spm_synchronize(spm_ctx *c) {
 }
 
 int spm_check_finite(spm_ctx *c) { ENTRY(check_finite(c)); }
+
+int spm_fused_stage(spm_ctx *c, int stage) { ENTRY(fused_stage(c, stage)); }
 
 int spm_set_schedule(spm_ctx *c, int schedule) {
   if (!c || (schedule != SPM_SCHEDULE_FUSED && schedule != SPM_SCHEDULE_LITERAL)) return SPM_ERR_INVALID;

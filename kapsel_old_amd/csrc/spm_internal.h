@@ -208,12 +208,12 @@ int fft_c2r_many(spm_ctx *c, double *const *a, int n);
 int fft_init_tables(spm_ctx *c);
 int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign);
 int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n);
+int fft_pass_z_r2c(spm_ctx *c, double *const *a, int n);
 
 // fused.hip: FFT passes carrying the pointwise work of the step
 int fx_zeta2u(spm_ctx *c, int fix_mask);
 int fz_advect(spm_ctx *c);
 int fx_update(spm_ctx *c, double nu, double dt);
-int fz_makef(spm_ctx *c);
 int fx_project(spm_ctx *c);
 
 // spectral.hip
@@ -237,5 +237,6 @@ int p_velocity(spm_ctx *c, int euler);
 int p_thermostat_apply(spm_ctx *c, const double *noise_dev);
 int p_build_tiles(spm_ctx *c);
 int p_make_phi(spm_ctx *c, int sw_up);
+int p_make_f(spm_ctx *c);
 int p_init_lists(spm_ctx *c);
 }  // namespace spm

@@ -24,8 +24,10 @@ SYMBOLS = [
     "spm_init_fluid", "spm_ns_solver_slaved_euler", "spm_zeta_k2u", "spm_md_solver_position",
     "spm_calc_f_hydro", "spm_force", "spm_md_solver_velocity", "spm_add_random_force_thermostat",
     "spm_make_phi_u_particle", "spm_make_f_particle_dt_sole", "spm_u2zeta_k", "spm_step",
-    "spm_synchronize", "spm_check_finite",
+    "spm_synchronize", "spm_check_finite", "spm_set_schedule", "spm_timing_enable", "spm_timing_read", "spm_fused_stage",
 ]
+
+SCHEDULE_FUSED, SCHEDULE_LITERAL = 0, 1
 
 _lib = None
 
@@ -73,6 +75,10 @@ def lib():
                      "spm_u2zeta_k", "spm_synchronize", "spm_check_finite"]:
             getattr(L, name).argtypes = [vp]
         L.spm_step.argtypes = [vp, ip]
+        L.spm_set_schedule.argtypes = [vp, ip]
+        L.spm_fused_stage.argtypes = [vp, ip]
+        L.spm_timing_enable.argtypes = [vp, ip]
+        L.spm_timing_read.argtypes = [vp, dp, dp, C.POINTER(C.c_long)]
         for name in SYMBOLS:
             f = getattr(L, name)
             if f.restype is C.c_int or f.restype is None:
@@ -135,6 +141,25 @@ class SPM:
 
     def synchronize(self):
         self._call("synchronize")
+
+    def set_schedule(self, schedule):
+        """SCHEDULE_FUSED (default) or SCHEDULE_LITERAL for spm_step."""
+        self._call("set_schedule", schedule)
+
+    def fused_stage(self, stage):
+        """Test hook: one stage of the fused schedule (see include/spm.h)."""
+        self._call("fused_stage", stage)
+
+    def timing_enable(self, on=True):
+        self._call("timing_enable", 1 if on else 0)
+
+    def timing_read(self):
+        """Per kernel family: (ms, algorithmic bytes, launches) since the last read."""
+        ms = (C.c_double * 9)()
+        by = (C.c_double * 9)()
+        cnt = (C.c_long * 9)()
+        self._call("timing_read", ms, by, cnt)
+        return list(ms), list(by), list(cnt)
 
     def check_finite(self):
         self._call("check_finite")

@@ -212,16 +212,19 @@ def test_make_f_particle_dt_sole(SPM):
     fields_close(o, g, ["u0", "u1", "u2", "f0", "f1", "f2"], 32)
 
 
+@pytest.mark.parametrize("schedule", [0, 1], ids=["fused", "literal"])
 @pytest.mark.parametrize("grid,n,a,kbt,steps", [
     ((32, 32, 32), 6, 4.0, 0.0, 4),
     ((64, 64, 64), 24, 5.0, 0.1, 3),
     ((32, 64, 32), 8, 4.0, 0.05, 3),
+    ((64, 32, 16), 3, 3.0, 0.1, 3),
 ])
-def test_full_step(SPM, grid, n, a, kbt, steps):
+def test_full_step(SPM, grid, n, a, kbt, steps, schedule):
     nx, ny, nz = grid
     prm = abi.make_params(nx, ny, nz, a=a, n_particles=n, kbt=kbt, lj_truncate=abi.LJ_WCA, seed=12345,
                           rotation=1)
     o, g = pair(SPM, prm)
+    g.set_schedule(schedule)
     p = particles(n, grid, a, seed=23, min_factor=2.05)
     both((o, g), "put_particles", p)
     both((o, g), "init_fluid")
@@ -229,19 +232,21 @@ def test_full_step(SPM, grid, n, a, kbt, steps):
     for s in range(steps):
         both((o, g), "step", 1)
         print("step", s, flush=True)
-        fields_close(o, g, ["u0", "u1", "u2"], nz)
-        fields_close(o, g, ["zeta0", "zeta1"], nz, spectrum=True)
+        # the step ends with U2zeta_k: u holds a spectrum (operate_omega.h:186-189)
+        fields_close(o, g, ["u0", "u1", "u2", "zeta0", "zeta1", "phi", "up0"], nz, spectrum=True)
         particles_close(o, g, ["x", "v", "omega", "f_hydro_previous", "torque_hydro_previous", "fr_previous"])
     assert g.get_time() == o.get_time()
 
 
-def test_config1_trajectory(SPM):
+@pytest.mark.parametrize("schedule", [0, 1], ids=["fused", "literal"])
+def test_config1_trajectory(SPM, schedule):
     """64^3, one sphere a=5 at the centre, constant +z force 0.01 (gravity
     with mass ratio 2), 60 steps: the velocity trace matches step by step."""
     a, mr = 5.0, 2.0
     G = -0.01 / ((4.0 / 3.0) * np.pi * a ** 3 * (mr - 1.0))
     prm = abi.make_params(64, a=a, n_particles=1, mass_ratio=(mr,), g=G, g_direction=2, rotation=1)
     o, g = pair(SPM, prm)
+    g.set_schedule(schedule)
     p = particles(1, (64, 64, 64), a, positions=np.array([[32.0, 32.0, 32.0]]))
     both((o, g), "put_particles", p)
     both((o, g), "init_fluid")
@@ -252,15 +257,17 @@ def test_config1_trajectory(SPM):
         worst = max(worst, abs(vo[2] - vg[2]) / abs(vo[2]))
     print("worst relative vz error over 60 steps %.3e" % worst)
     assert worst < TOL
-    fields_close(o, g, ["u0", "u1", "u2"], 64)
+    fields_close(o, g, ["u0", "u1", "u2"], 64, spectrum=True)
 
 
-def test_determinism(SPM):
+@pytest.mark.parametrize("schedule", [0, 1], ids=["fused", "literal"])
+def test_determinism(SPM, schedule):
     grid = (32, 32, 32)
     outs = []
     for _ in range(2):
         prm = abi.make_params(32, a=4.0, n_particles=10, kbt=0.05, lj_truncate=abi.LJ_WCA, seed=7)
         g = SPM(prm)
+        g.set_schedule(schedule)
         g.put_particles(particles(10, grid, 4.0, seed=29))
         g.init_fluid()
         g.step(3)
@@ -269,9 +276,11 @@ def test_determinism(SPM):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
-def test_zero_particles(SPM):
+@pytest.mark.parametrize("schedule", [0, 1], ids=["fused", "literal"])
+def test_zero_particles(SPM, schedule):
     prm = abi.make_params(32, n_particles=0)
     o, g = pair(SPM, prm)
+    g.set_schedule(schedule)
     for d in range(3):
         o.put_field("u%d" % d, smooth_field(o.shape, 32, seed=60 + d, scale=0.1))
     o.U2zeta_k()
@@ -281,7 +290,7 @@ def test_zero_particles(SPM):
         obj.put_field("zeta1", z1)
         obj.put_uk_dc(dc)
     both((o, g), "step", 2)
-    fields_close(o, g, ["u0", "u1", "u2"], 32)
+    fields_close(o, g, ["u0", "u1", "u2", "zeta0", "zeta1"], 32, spectrum=True)
 
 
 def test_nonfinite_detected(SPM):
@@ -305,3 +314,70 @@ def test_invalid_inputs_rejected(SPM):
     bad = particles(1, (32, 32, 32), 4.0, positions=np.array([[33.0, 1.0, 1.0]]))
     with pytest.raises(SpmError):
         g.put_particles(bad)
+
+
+def test_multi_step_call_matches_single_steps(SPM):
+    """spm_step(n) with the fused schedule equals n calls of spm_step(1)."""
+    prm = abi.make_params(32, a=4.0, n_particles=6, kbt=0.05, lj_truncate=abi.LJ_WCA, seed=5)
+    p = particles(6, (32, 32, 32), 4.0, seed=31)
+    a, b = SPM(prm), SPM(prm)
+    for g in (a, b):
+        g.put_particles(p)
+        g.init_fluid()
+    a.step(5)
+    for _ in range(5):
+        b.step(1)
+    assert rel(a.get_field("u0"), b.get_field("u0")) < 1e-13
+    assert rel(a.get_particles()["v"], b.get_particles()["v"]) < 1e-13
+
+
+@pytest.mark.parametrize("stage", [1, 2, 3, 4])
+def test_fused_stage_matches_literal(SPM, stage):
+    """Each stage of the fused schedule against the literal routines it
+    replaces, from identical state (64^3, 16 particles, rotation on)."""
+    n, a = 64, 4.0
+    prm = abi.make_params(n, a=a, n_particles=16, rotation=1, dt=0.05, fix_cell=(1, 0, 1))
+    o = Oracle(prm)
+    for d in range(3):
+        o.put_field("u%d" % d, smooth_field(o.shape, n, seed=70 + d, kmax=5, scale=0.05))
+    o.U2zeta_k()
+    z0, z1, dc = o.get_field("zeta0"), o.get_field("zeta1"), o.get_uk_dc()
+    p = particles(16, (n, n, n), a, seed=71, vel=0.02, omega=0.005)
+
+    def fresh():
+        g = SPM(prm)
+        g.put_field("zeta0", z0)
+        g.put_field("zeta1", z1)
+        g.put_uk_dc(dc)
+        g.put_particles(p)
+        if stage == 3:
+            g.Zeta_k2u()  # real-space u as the tail of the step receives it
+        return g
+
+    lit, fus = fresh(), fresh()
+    fus.fused_stage(stage)
+    if stage == 1:
+        lit.NS_solver_slavedEuler()
+        checks = [("zeta0", True), ("zeta1", True)]
+    elif stage == 2:
+        lit.Zeta_k2u()
+        checks = [("u0", False), ("u1", False), ("u2", False)]
+    elif stage == 3:
+        lit.Make_phi_u_particle()
+        lit.Make_f_particle_dt_sole()
+        lit.U2zeta_k()
+        checks = [("zeta0", True), ("zeta1", True)]
+    else:
+        lit.Make_phi_u_particle()
+        checks = [("phi", True), ("up0", True), ("up2", True)]
+        uk = o.zeta_k2u_k([z0, z1], dc)
+        for d in range(3):
+            assert rel(fus.get_field("u%d" % d), uk[d]) < TOL
+    for nm, spectrum in checks:
+        x, y = fus.get_field(nm), lit.get_field(nm)
+        if not spectrum:
+            x, y = x[:, :, :n], y[:, :, :n]
+        e = rel(x, y)
+        print("stage %d %s rel err %.3e" % (stage, nm, e))
+        assert e < TOL
+    assert np.abs(fus.get_uk_dc() - lit.get_uk_dc()).max() <= 1e-12 * max(1.0, np.abs(lit.get_uk_dc()).max())
