@@ -230,37 +230,39 @@ int fft_init_tables(spm_ctx *c) {
   return 0;
 }
 
-static int fft_many(spm_ctx *c, double *const *a, int n, bool fwd) {
+// Field buffers are double* handles; an FP32 context stores floats in them.
+template <typename T>
+static int fft_many_t(spm_ctx *c, double *const *a, int n, bool fwd) {
   const Grid &g = c->grid;
   for (int b = 0; b < n; b += 5) {
-    FieldPtrs<double> fp{};
+    FieldPtrs<T> fp{};
     const int nf = (n - b) < 5 ? (n - b) : 5;
-    for (int q = 0; q < nf; ++q) fp.p[q] = a[b + q];
+    for (int q = 0; q < nf; ++q) fp.p[q] = reinterpret_cast<T *>(a[b + q]);
     const int nrows = g.nx * g.ny;
     const long h = g.hnz;
-    // every pass reads and writes each field once: 2 * NX*NY*NZ_ doubles
-    const double bytes = 2.0 * nf * (double)nrows * g.nz_ * sizeof(double);
+    // every pass reads and writes each field once: 2 * NX*NY*NZ_ elements
+    const double bytes = 2.0 * nf * (double)nrows * g.nz_ * sizeof(T);
     int rc = 0;
     if (fwd) {
       tbeg(c);
-      rc |= launch_z<double>(g.nz / 2, true, fp, nf, c->tw_z, c->tw_rz, nrows, g.nz_, 1.0, c->stream);
+      rc |= launch_z<T>(g.nz / 2, true, fp, nf, c->tw_z, c->tw_rz, nrows, g.nz_, (T)1, c->stream);
       tend(c, SPM_K_FFT_Z_R2C, bytes);
       tbeg(c);
-      rc |= launch_lines<double, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
+      rc |= launch_lines<T, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
       tend(c, SPM_K_FFT_Y, bytes);
       tbeg(c);
-      rc |= launch_lines<double, +1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream);
+      rc |= launch_lines<T, +1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream);
       tend(c, SPM_K_FFT_X, bytes);
     } else {
-      const double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+      const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
       tbeg(c);
-      rc |= launch_lines<double, -1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream);
+      rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream);
       tend(c, SPM_K_FFT_X, bytes);
       tbeg(c);
-      rc |= launch_lines<double, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
+      rc |= launch_lines<T, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
       tend(c, SPM_K_FFT_Y, bytes);
       tbeg(c);
-      rc |= launch_z<double>(g.nz / 2, false, fp, nf, c->tw_z, c->tw_rz, nrows, g.nz_, scale, c->stream);
+      rc |= launch_z<T>(g.nz / 2, false, fp, nf, c->tw_z, c->tw_rz, nrows, g.nz_, scale, c->stream);
       tend(c, SPM_K_FFT_Z_C2R, bytes);
     }
     if (rc) {
@@ -270,6 +272,10 @@ static int fft_many(spm_ctx *c, double *const *a, int n, bool fwd) {
     SPM_LAUNCH_CHECK(c);
   }
   return 0;
+}
+
+static int fft_many(spm_ctx *c, double *const *a, int n, bool fwd) {
+  return c->prm.precision == SPM_FP32 ? fft_many_t<float>(c, a, n, fwd) : fft_many_t<double>(c, a, n, fwd);
 }
 
 int fft_r2c_many(spm_ctx *c, double *const *a, int n) { return fft_many(c, a, n, true); }
@@ -282,19 +288,31 @@ int fft_c2r(spm_ctx *c, double *a) { return fft_many(c, &a, 1, false); }
 namespace spm {
 
 // Single passes, used by the fused schedule (fused.hip supplies the x and
-// z passes that carry pointwise work).
-int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign) {
+// z passes that carry pointwise work).  kind: 0 = y pass, 1 = z forward,
+// 2 = z inverse.
+template <typename T>
+static int fft_pass_t(spm_ctx *c, double *const *a, int n, int kind, int sign) {
   const Grid &g = c->grid;
   for (int b = 0; b < n; b += 5) {
-    FieldPtrs<double> fp{};
+    FieldPtrs<T> fp{};
     const int nf = (n - b) < 5 ? (n - b) : 5;
-    for (int q = 0; q < nf; ++q) fp.p[q] = a[b + q];
+    for (int q = 0; q < nf; ++q) fp.p[q] = reinterpret_cast<T *>(a[b + q]);
     const long h = g.hnz;
-    const double bytes = 2.0 * nf * (double)c->nfield * sizeof(double);
+    const double bytes = 2.0 * nf * (double)c->nfield * sizeof(T);
+    int rc;
     tbeg(c);
-    const int rc = sign > 0 ? launch_lines<double, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream)
-                            : launch_lines<double, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
-    tend(c, SPM_K_FFT_Y, bytes);
+    if (kind == 0) {
+      rc = sign > 0 ? launch_lines<T, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream)
+                    : launch_lines<T, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
+      tend(c, SPM_K_FFT_Y, bytes);
+    } else if (kind == 1) {
+      rc = launch_z<T>(g.nz / 2, true, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, (T)1, c->stream);
+      tend(c, SPM_K_FFT_Z_R2C, bytes);
+    } else {
+      const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+      rc = launch_z<T>(g.nz / 2, false, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, scale, c->stream);
+      tend(c, SPM_K_FFT_Z_C2R, bytes);
+    }
     if (rc) {
       c->err = "unsupported FFT size";
       return SPM_ERR_UNSUPPORTED;
@@ -304,43 +322,13 @@ int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign) {
   return 0;
 }
 
-int fft_pass_z_r2c(spm_ctx *c, double *const *a, int n) {
-  const Grid &g = c->grid;
-  for (int b = 0; b < n; b += 5) {
-    FieldPtrs<double> fp{};
-    const int nf = (n - b) < 5 ? (n - b) : 5;
-    for (int q = 0; q < nf; ++q) fp.p[q] = a[b + q];
-    const double bytes = 2.0 * nf * (double)c->nfield * sizeof(double);
-    tbeg(c);
-    const int rc = launch_z<double>(g.nz / 2, true, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, 1.0, c->stream);
-    tend(c, SPM_K_FFT_Z_R2C, bytes);
-    if (rc) {
-      c->err = "unsupported FFT size";
-      return SPM_ERR_UNSUPPORTED;
-    }
-    SPM_LAUNCH_CHECK(c);
-  }
-  return 0;
+static int fft_pass(spm_ctx *c, double *const *a, int n, int kind, int sign) {
+  return c->prm.precision == SPM_FP32 ? fft_pass_t<float>(c, a, n, kind, sign)
+                                      : fft_pass_t<double>(c, a, n, kind, sign);
 }
 
-int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n) {
-  const Grid &g = c->grid;
-  const double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
-  for (int b = 0; b < n; b += 5) {
-    FieldPtrs<double> fp{};
-    const int nf = (n - b) < 5 ? (n - b) : 5;
-    for (int q = 0; q < nf; ++q) fp.p[q] = a[b + q];
-    const double bytes = 2.0 * nf * (double)c->nfield * sizeof(double);
-    tbeg(c);
-    const int rc = launch_z<double>(g.nz / 2, false, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, scale, c->stream);
-    tend(c, SPM_K_FFT_Z_C2R, bytes);
-    if (rc) {
-      c->err = "unsupported FFT size";
-      return SPM_ERR_UNSUPPORTED;
-    }
-    SPM_LAUNCH_CHECK(c);
-  }
-  return 0;
-}
+int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign) { return fft_pass(c, a, n, 0, sign); }
+int fft_pass_z_r2c(spm_ctx *c, double *const *a, int n) { return fft_pass(c, a, n, 1, +1); }
+int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n) { return fft_pass(c, a, n, 2, -1); }
 
 }  // namespace spm

@@ -33,7 +33,8 @@ namespace spm {
 
 namespace {
 
-__device__ __forceinline__ double sq(double x) { return x * x; }
+template <typename T>
+__device__ __forceinline__ T sq(T x) { return x * x; }
 __device__ __forceinline__ int nint(double a) { return a >= 0.0 ? (int)(a + .5) : (int)(a - .5); }
 
 struct ModeK {
@@ -51,18 +52,20 @@ __device__ __forceinline__ ModeK mode_k(const Grid &g, int i, int j, int kz) {
 }
 
 // Zeta_k2u_k (fft_wrapper.cxx:408-511) on one mode; component d.
-__device__ __forceinline__ double2 zeta2u_mode(const Grid &g, const ModeK &m, double2 d1, double2 d2, int d) {
-  double kx = g.wave_x * m.kx, ky = g.wave_y * m.ky, kz = g.wave_z * m.kz;
-  const double k2 = sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz);
-  const double ik2 = k2 > 0.0 ? 1. / k2 : 0.0;
-  double wr0, wi0, wr1, wi1, wr2, wi2;
+template <typename T>
+__device__ __forceinline__ cplx<T> zeta2u_mode(const Grid &g, const ModeK &m, cplx<T> d1, cplx<T> d2, int d) {
+  using C2 = cplx<T>;
+  T kx = (T)(g.wave_x * m.kx), ky = (T)(g.wave_y * m.ky), kz = (T)(g.wave_z * m.kz);
+  const T k2 = (T)(sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz));
+  const T ik2 = k2 > (T)0 ? (T)1 / k2 : (T)0;
+  T wr0, wi0, wr1, wi1, wr2, wi2;
   if (m.kx != 0) {
-    wr0 = -(1. / kx) * (ky * d1.x + kz * d2.x);
-    wi0 = -(1. / kx) * (ky * d1.y + kz * d2.y);
+    wr0 = -((T)1 / kx) * (ky * d1.x + kz * d2.x);
+    wi0 = -((T)1 / kx) * (ky * d1.y + kz * d2.y);
     wr1 = d1.x; wi1 = d1.y;
     wr2 = d2.x; wi2 = d2.y;
   } else if (m.ky != 0) {
-    const double dmy = -(kz / ky);
+    const T dmy = -(kz / ky);
     wr0 = d2.x; wi0 = d2.y;
     wr1 = dmy * d1.x; wi1 = dmy * d1.y;
     wr2 = d1.x; wi2 = d1.y;
@@ -74,9 +77,9 @@ __device__ __forceinline__ double2 zeta2u_mode(const Grid &g, const ModeK &m, do
   kx *= ik2;
   ky *= ik2;
   kz *= ik2;
-  if (d == 0) return double2{(ky * wi2 - kz * wi1), -(ky * wr2 - kz * wr1)};
-  if (d == 1) return double2{(kz * wi0 - kx * wi2), -(kz * wr0 - kx * wr2)};
-  return double2{(kx * wi1 - ky * wi0), -(kx * wr1 - ky * wr0)};
+  if (d == 0) return C2{(ky * wi2 - kz * wi1), -(ky * wr2 - kz * wr1)};
+  if (d == 1) return C2{(kz * wi0 - kx * wi2), -(kz * wr0 - kx * wr2)};
+  return C2{(kx * wi1 - ky * wi0), -(kx * wr1 - ky * wr0)};
 }
 
 // ---------------------------------------------------------------------
@@ -85,29 +88,30 @@ __device__ __forceinline__ double2 zeta2u_mode(const Grid &g, const ModeK &m, do
 // truncation of sp_3d_ns.cxx:75-77 / operate_omega.cxx:106-108).  The thread
 // holding mode 0 sets u(0) = uk_dc, zeroing the FIX_CELL components first.
 // ---------------------------------------------------------------------
-template <int N, int R, int C>
-__global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const double *__restrict__ z0,
-                                                           const double *__restrict__ z1, double *u0, double *u1,
-                                                           double *u2, double *uk_dc, int fix_mask,
+template <typename T, int N, int R, int C>
+__global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__restrict__ z0,
+                                                           const T *__restrict__ z1, T *u0, T *u1,
+                                                           T *u2, double *uk_dc, int fix_mask,
                                                            const double2 *__restrict__ tw) {
+  using C2 = cplx<T>;
   constexpr int NT = C * N / R, E = N * C / NT;
-  __shared__ double2 lds[N * C];
+  __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * C, j = blockIdx.y;
   const long H = g.hnz;
   const long base = (long)j * H + c0;  // complex index of (i = 0, j, c0)
   const long es = (long)g.ny * H;
-  double2 za[E], zb[E];
+  C2 za[E], zb[E];
 #pragma unroll
   for (int t = 0; t < E; ++t) {
     const int f = tid + t * NT, c = f % C, i = f / C;
-    za[t] = zb[t] = double2{0.0, 0.0};
+    za[t] = zb[t] = C2{0, 0};
     if (c0 + c < H) {
       const ModeK m = mode_k(g, i, j, c0 + c);
       if (m.keep) {
         const long q = 2 * (base + i * es + c);
-        za[t] = *reinterpret_cast<const double2 *>(z0 + q);
-        zb[t] = *reinterpret_cast<const double2 *>(z1 + q);
+        za[t] = *reinterpret_cast<const C2 *>(z0 + q);
+        zb[t] = *reinterpret_cast<const C2 *>(z1 + q);
       }
     }
   }
@@ -118,21 +122,21 @@ __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const double
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double *out = d == 0 ? u0 : (d == 1 ? u1 : u2);
+    T *out = d == 0 ? u0 : (d == 1 ? u1 : u2);
 #pragma unroll
     for (int t = 0; t < E; ++t) {
       const int f = tid + t * NT, c = f % C, i = f / C;
       const ModeK m = mode_k(g, i, j, c0 + c);
-      double2 v = zeta2u_mode(g, m, za[t], zb[t], d);
-      if (has0 && f == 0) v.x = dc[d];
+      C2 v = zeta2u_mode<T>(g, m, za[t], zb[t], d);
+      if (has0 && f == 0) v.x = (T)dc[d];
       lds[i * C + c] = v;
     }
     __syncthreads();
-    fft_stages<double, N, R, C, C, -1, 1>(lds, tw, tid);
+    fft_stages<T, N, R, C, C, -1, 1>(lds, tw, tid);
 #pragma unroll
     for (int t = 0; t < E; ++t) {
       const int f = tid + t * NT, c = f % C, i = f / C;
-      if (c0 + c < H) *reinterpret_cast<double2 *>(out + 2 * (base + i * es + c)) = lds[i * C + c];
+      if (c0 + c < H) *reinterpret_cast<C2 *>(out + 2 * (base + i * es + c)) = lds[i * C + c];
     }
     __syncthreads();
   }
@@ -145,100 +149,103 @@ __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const double
 // z pass: C2R of (u0, u1, u2) -> real products of U2advection_k -> R2C of
 // the five products into (p0 .. p4).  p0..p2 may alias u0..u2.
 // ---------------------------------------------------------------------
-template <int M, int R, int ROWS>
-__device__ __forceinline__ void z_c2r_row_to_lds(double2 *lds, const double *a, long row0, int nrows, int nz_,
+template <typename T, int M, int R, int ROWS>
+__device__ __forceinline__ void z_c2r_row_to_lds(cplx<T> *lds, const T *a, long row0, int nrows, int nz_,
                                                  const double2 *__restrict__ twr, int tid) {
+  using C2 = cplx<T>;
   constexpr int NT = ROWS * M / R, CP = ROWS + 1, H = M + 1, HP = M / 2 + 1;
   for (int f = tid; f < ROWS * H; f += NT) {
     const int k = f % H, r = f / H;
     const long row = row0 + r;
-    lds[k * CP + r] = (row < nrows) ? *reinterpret_cast<const double2 *>(a + row * nz_ + 2 * k) : double2{0, 0};
+    lds[k * CP + r] = (row < nrows) ? *reinterpret_cast<const C2 *>(a + row * nz_ + 2 * k) : C2{0, 0};
   }
   __syncthreads();
   for (int f = tid; f < ROWS * HP; f += NT) {
     const int p = f % HP, r = f / HP;
     const int k = p, kk = M - p;
-    const double2 xk = lds[k * CP + r], xkk = lds[kk * CP + r];
+    const C2 xk = lds[k * CP + r], xkk = lds[kk * CP + r];
     {
-      const double2 cj{xkk.x, -xkk.y};
-      const double2 e = cadd(xk, cj), d = csub(xk, cj);
-      const double2 w = twr[k];
-      const double2 o = cmul(d, double2{w.x, -w.y});
-      lds[k * CP + r] = double2{e.x - o.y, e.y + o.x};
+      const C2 cj{xkk.x, -xkk.y};
+      const C2 e = cadd(xk, cj), d = csub(xk, cj);
+      const C2 w = C2{(T)twr[k].x, (T)twr[k].y};
+      const C2 o = cmul(d, C2{w.x, -w.y});
+      lds[k * CP + r] = C2{e.x - o.y, e.y + o.x};
     }
     if (p > 0 && kk != k) {
-      const double2 cj{xk.x, -xk.y};
-      const double2 e = cadd(xkk, cj), d = csub(xkk, cj);
-      const double2 w = twr[kk];
-      const double2 o = cmul(d, double2{w.x, -w.y});
-      lds[kk * CP + r] = double2{e.x - o.y, e.y + o.x};
+      const C2 cj{xk.x, -xk.y};
+      const C2 e = cadd(xkk, cj), d = csub(xkk, cj);
+      const C2 w = C2{(T)twr[kk].x, (T)twr[kk].y};
+      const C2 o = cmul(d, C2{w.x, -w.y});
+      lds[kk * CP + r] = C2{e.x - o.y, e.y + o.x};
     }
   }
   __syncthreads();
 }
 
 // writes the forward z transform of the LDS rows (packed real pairs) to `a`
-template <int M, int R, int ROWS>
-__device__ __forceinline__ void z_r2c_lds_to_row(double2 *lds, double *a, long row0, int nrows, int nz_,
+template <typename T, int M, int R, int ROWS>
+__device__ __forceinline__ void z_r2c_lds_to_row(cplx<T> *lds, T *a, long row0, int nrows, int nz_,
                                                  const double2 *__restrict__ tw, const double2 *__restrict__ twr,
                                                  int tid) {
+  using C2 = cplx<T>;
   constexpr int NT = ROWS * M / R, CP = ROWS + 1, H = M + 1;
-  fft_stages<double, M, R, ROWS, CP, +1, 1>(lds, tw, tid);
+  fft_stages<T, M, R, ROWS, CP, +1, 1>(lds, tw, tid);
   for (int f = tid; f < ROWS * H; f += NT) {
     const int k = f % H, r = f / H;
     const long row = row0 + r;
-    const double2 z = lds[(k % M) * CP + r];
-    double2 zc = lds[((M - k) % M) * CP + r];
+    const C2 z = lds[(k % M) * CP + r];
+    C2 zc = lds[((M - k) % M) * CP + r];
     zc.y = -zc.y;
-    const double2 e{0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y)};
-    const double2 o{0.5 * (z.y - zc.y), -0.5 * (z.x - zc.x)};
-    const double2 w = twr[k];
-    const double2 x = cadd(e, cmul(o, w));
-    if (row < nrows) *reinterpret_cast<double2 *>(a + row * nz_ + 2 * k) = x;
+    const C2 e{(T)0.5 * (z.x + zc.x), (T)0.5 * (z.y + zc.y)};
+    const C2 o{(T)0.5 * (z.y - zc.y), (T)-0.5 * (z.x - zc.x)};
+    const C2 w = C2{(T)twr[k].x, (T)twr[k].y};
+    const C2 x = cadd(e, cmul(o, w));
+    if (row < nrows) *reinterpret_cast<C2 *>(a + row * nz_ + 2 * k) = x;
   }
   __syncthreads();
 }
 
-template <int M, int R, int ROWS>
-__global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, double *u0, double *u1, double *u2, double *p3,
-                                                              double *p4, const double2 *__restrict__ tw,
-                                                              const double2 *__restrict__ twr, double scale) {
+template <typename T, int M, int R, int ROWS>
+__global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, T *u0, T *u1, T *u2, T *p3,
+                                                              T *p4, const double2 *__restrict__ tw,
+                                                              const double2 *__restrict__ twr, T scale) {
+  using C2 = cplx<T>;
   constexpr int NT = ROWS * M / R, CP = ROWS + 1, E = ROWS * M / NT;
-  __shared__ double2 lds[(M + 1) * CP];
+  __shared__ C2 lds[(M + 1) * CP];
   const int tid = threadIdx.x;
   const long row0 = (long)blockIdx.x * ROWS;
   const int nrows = g.nx * g.ny;
-  double2 v[3][E];
+  C2 v[3][E];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    z_c2r_row_to_lds<M, R, ROWS>(lds, d == 0 ? u0 : (d == 1 ? u1 : u2), row0, nrows, g.nz_, twr, tid);
-    fft_stages<double, M, R, ROWS, CP, -1, 1>(lds, tw, tid);
+    z_c2r_row_to_lds<T, M, R, ROWS>(lds, d == 0 ? u0 : (d == 1 ? u1 : u2), row0, nrows, g.nz_, twr, tid);
+    fft_stages<T, M, R, ROWS, CP, -1, 1>(lds, tw, tid);
 #pragma unroll
     for (int t = 0; t < E; ++t) {
       const int f = tid + t * NT, m = f % M, r = f / M;
-      const double2 x = lds[m * CP + r];
-      v[d][t] = double2{x.x * scale, x.y * scale};
+      const C2 x = lds[m * CP + r];
+      v[d][t] = C2{x.x * scale, x.y * scale};
     }
     __syncthreads();
   }
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
-    double *out = q == 0 ? u0 : q == 1 ? u1 : q == 2 ? u2 : q == 3 ? p3 : p4;
+    T *out = q == 0 ? u0 : q == 1 ? u1 : q == 2 ? u2 : q == 3 ? p3 : p4;
 #pragma unroll
     for (int t = 0; t < E; ++t) {
       const int f = tid + t * NT, m = f % M, r = f / M;
-      double2 pr;
-      const double a1 = v[0][t].x, a2 = v[1][t].x, a3 = v[2][t].x;
-      const double b1 = v[0][t].y, b2 = v[1][t].y, b3 = v[2][t].y;
-      if (q == 0) pr = double2{a2 * a3, b2 * b3};
-      else if (q == 1) pr = double2{a3 * a1, b3 * b1};
-      else if (q == 2) pr = double2{a1 * a2, b1 * b2};
-      else if (q == 3) pr = double2{(a2 - a3) * (a2 + a3), (b2 - b3) * (b2 + b3)};
-      else pr = double2{(a3 - a1) * (a3 + a1), (b3 - b1) * (b3 + b1)};
+      C2 pr;
+      const T a1 = v[0][t].x, a2 = v[1][t].x, a3 = v[2][t].x;
+      const T b1 = v[0][t].y, b2 = v[1][t].y, b3 = v[2][t].y;
+      if (q == 0) pr = C2{a2 * a3, b2 * b3};
+      else if (q == 1) pr = C2{a3 * a1, b3 * b1};
+      else if (q == 2) pr = C2{a1 * a2, b1 * b2};
+      else if (q == 3) pr = C2{(a2 - a3) * (a2 + a3), (b2 - b3) * (b2 + b3)};
+      else pr = C2{(a3 - a1) * (a3 + a1), (b3 - b1) * (b3 + b1)};
       lds[m * CP + r] = pr;
     }
     __syncthreads();
-    z_r2c_lds_to_row<M, R, ROWS>(lds, out, row0, nrows, g.nz_, tw, twr, tid);
+    z_r2c_lds_to_row<T, M, R, ROWS>(lds, out, row0, nrows, g.nz_, tw, twr, tid);
   }
 }
 
@@ -247,13 +254,14 @@ __global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, double *u
 // contraction, Omega_k2zeta_k and the slaved-Euler update as epilogue.
 // The transformed products are not stored.
 // ---------------------------------------------------------------------
-template <int N, int R, int C>
-__global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const double *p0, const double *p1,
-                                                           const double *p2, const double *p3, const double *p4,
-                                                           double *z0, double *z1, double nu, double dt,
+template <typename T, int N, int R, int C>
+__global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0, const T *p1,
+                                                           const T *p2, const T *p3, const T *p4,
+                                                           T *z0, T *z1, double nu, double dt,
                                                            const double2 *__restrict__ tw) {
+  using C2 = cplx<T>;
   constexpr int NT = C * N / R, E = N * C / NT;
-  __shared__ double2 lds[N * C];
+  __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * C, j = blockIdx.y;
   const long H = g.hnz;
@@ -262,28 +270,28 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const double
   // The contraction (operate_omega.cxx:60-97) is linear in the five product
   // spectra P0..P4 = u2u3, u3u1, u1u2, u2^2-u3^2, u3^2-u1^2: accumulate the
   // two components Omega_k2zeta_k keeps as each product is transformed.
-  double2 acc0[E], acc1[E];
+  C2 acc0[E], acc1[E];
 #pragma unroll
-  for (int t = 0; t < E; ++t) acc0[t] = acc1[t] = double2{0.0, 0.0};
+  for (int t = 0; t < E; ++t) acc0[t] = acc1[t] = C2{0, 0};
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
-    const double *in = q == 0 ? p0 : q == 1 ? p1 : q == 2 ? p2 : q == 3 ? p3 : p4;
+    const T *in = q == 0 ? p0 : q == 1 ? p1 : q == 2 ? p2 : q == 3 ? p3 : p4;
 #pragma unroll
     for (int t = 0; t < E; ++t) {
       const int f = tid + t * NT, c = f % C, i = f / C;
-      lds[i * C + c] = (c0 + c < H) ? *reinterpret_cast<const double2 *>(in + 2 * (base + i * es + c))
-                                    : double2{0.0, 0.0};
+      lds[i * C + c] = (c0 + c < H) ? *reinterpret_cast<const C2 *>(in + 2 * (base + i * es + c))
+                                    : C2{0, 0};
     }
     __syncthreads();
-    fft_stages<double, N, R, C, C, +1, 1>(lds, tw, tid);
+    fft_stages<T, N, R, C, C, +1, 1>(lds, tw, tid);
 #pragma unroll
     for (int t = 0; t < E; ++t) {
       const int f = tid + t * NT, c = f % C, i = f / C;
-      const double2 P = lds[i * C + c];
+      const C2 P = lds[i * C + c];
       const ModeK m = mode_k(g, i, j, c0 + c);
-      const double k1 = m.kx * g.wave_x, k2 = m.ky * g.wave_y, k3 = m.kz * g.wave_z;
+      const T k1 = (T)(m.kx * g.wave_x), k2 = (T)(m.ky * g.wave_y), k3 = (T)(m.kz * g.wave_z);
       // coefficient of P_q in -Omega_s, s = 0, 1, 2
-      double cf[3];
+      T cf[3];
       if (q == 0) {
         cf[0] = (k3 - k2) * (k3 + k2); cf[1] = k1 * k2; cf[2] = -(k3 * k1);
       } else if (q == 1) {
@@ -297,7 +305,7 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const double
       }
       const int s0 = m.kx != 0 ? 1 : (m.ky != 0 ? 2 : 0);
       const int s1 = m.kx != 0 ? 2 : (m.ky != 0 ? 0 : 1);
-      const double a = cf[s0], b = cf[s1];
+      const T a = cf[s0], b = cf[s1];
       acc0[t].x -= a * P.x;
       acc0[t].y -= a * P.y;
       acc1[t].x -= b * P.x;
@@ -305,27 +313,27 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const double
     }
     __syncthreads();
   }
-  const double dmy0 = -nu * dt, dmy1 = 1. / nu;
+  const T dmy0 = (T)(-nu * dt), dmy1 = (T)(1. / nu);
 #pragma unroll
   for (int t = 0; t < E; ++t) {
     const int f = tid + t * NT, c = f % C, i = f / C;
     if (c0 + c >= H) continue;
     const ModeK m = mode_k(g, i, j, c0 + c);
     const long q = 2 * (base + i * es + c);
-    double2 za{0.0, 0.0}, zb{0.0, 0.0};
+    C2 za{0.0, 0.0}, zb{0.0, 0.0};
     if (m.keep) {
-      const double kk2 = sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz);
-      const double ik2 = kk2 > 0.0 ? 1. / kk2 : 0.0;
-      const double dmy = exp(dmy0 * kk2) - 1.;
-      za = *reinterpret_cast<const double2 *>(z0 + q);
-      zb = *reinterpret_cast<const double2 *>(z1 + q);
+      const T kk2 = (T)(sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz));
+      const T ik2 = kk2 > (T)0 ? (T)1 / kk2 : (T)0;
+      const T dmy = exp(dmy0 * kk2) - (T)1;
+      za = *reinterpret_cast<const C2 *>(z0 + q);
+      zb = *reinterpret_cast<const C2 *>(z1 + q);
       za.x += dmy * (za.x - dmy1 * ik2 * acc0[t].x);
       za.y += dmy * (za.y - dmy1 * ik2 * acc0[t].y);
       zb.x += dmy * (zb.x - dmy1 * ik2 * acc1[t].x);
       zb.y += dmy * (zb.y - dmy1 * ik2 * acc1[t].y);
     }
-    *reinterpret_cast<double2 *>(z0 + q) = za;
-    *reinterpret_cast<double2 *>(z1 + q) = zb;
+    *reinterpret_cast<C2 *>(z0 + q) = za;
+    *reinterpret_cast<C2 *>(z1 + q) = zb;
   }
 }
 
@@ -337,51 +345,52 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const double
 // gradient part removed by the projection to zero, while the projection
 // keeps the k = 0 mode (IK2(0) = 0).  So the result is the same field.
 // ---------------------------------------------------------------------
-template <int N, int R, int C>
-__global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const double *f0, const double *f1,
-                                                            const double *f2, double *z0, double *z1, double *uk_dc,
+template <typename T, int N, int R, int C>
+__global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0, const T *f1,
+                                                            const T *f2, T *z0, T *z1, double *uk_dc,
                                                             const double2 *__restrict__ tw) {
+  using C2 = cplx<T>;
   constexpr int NT = C * N / R, E = N * C / NT;
-  __shared__ double2 lds[N * C];
+  __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * C, j = blockIdx.y;
   const long H = g.hnz;
   const long base = (long)j * H + c0;
   const long es = (long)g.ny * H;
-  double2 acc0[E], acc1[E];
+  C2 acc0[E], acc1[E];
   double f00[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int t = 0; t < E; ++t) acc0[t] = acc1[t] = double2{0.0, 0.0};
+  for (int t = 0; t < E; ++t) acc0[t] = acc1[t] = C2{0, 0};
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const double *in = q == 0 ? f0 : (q == 1 ? f1 : f2);
+    const T *in = q == 0 ? f0 : (q == 1 ? f1 : f2);
 #pragma unroll
     for (int t = 0; t < E; ++t) {
       const int f = tid + t * NT, c = f % C, i = f / C;
-      lds[i * C + c] = (c0 + c < H) ? *reinterpret_cast<const double2 *>(in + 2 * (base + i * es + c))
-                                    : double2{0.0, 0.0};
+      lds[i * C + c] = (c0 + c < H) ? *reinterpret_cast<const C2 *>(in + 2 * (base + i * es + c))
+                                    : C2{0, 0};
     }
     __syncthreads();
-    fft_stages<double, N, R, C, C, +1, 1>(lds, tw, tid);
+    fft_stages<T, N, R, C, C, +1, 1>(lds, tw, tid);
 #pragma unroll
     for (int t = 0; t < E; ++t) {
       const int f = tid + t * NT, c = f % C, i = f / C;
-      const double2 F = lds[i * C + c];
+      const C2 F = lds[i * C + c];
       if (f == 0) f00[q] = F.x;
       const ModeK m = mode_k(g, i, j, c0 + c);
-      const double kx = m.kx * g.wave_x, ky = m.ky * g.wave_y, kz = m.kz * g.wave_z;
+      const T kx = (T)(m.kx * g.wave_x), ky = (T)(m.ky * g.wave_y), kz = (T)(m.kz * g.wave_z);
       // omega = U_k2zeta_k's curl: omega_s gets (ks[a] * F_b - ks[b] * F_a)
       // with (s, a, b) cyclic; omega_re = .. * Im, omega_im = -(..) * Re
       const int s0 = m.kx != 0 ? 1 : (m.ky != 0 ? 2 : 0);
       const int s1 = m.kx != 0 ? 2 : (m.ky != 0 ? 0 : 1);
       // coefficient of F_q in omega_s: +ks[s-1] for q = s+1, -ks[s+1] for q = s-1
       // (s, q) -> coefficient: (0,2) ky, (0,1) -kz, (1,0) kz, (1,2) -kx, (2,1) kx, (2,0) -ky
-      auto coef = [&](int s) -> double {
+      auto coef = [&](int s) -> T {
         if (s == 0) return q == 2 ? ky : (q == 1 ? -kz : 0.0);
         if (s == 1) return q == 0 ? kz : (q == 2 ? -kx : 0.0);
         return q == 1 ? kx : (q == 0 ? -ky : 0.0);
       };
-      const double a = coef(s0), b = coef(s1);
+      const T a = coef(s0), b = coef(s1);
       if (m.keep) {
         acc0[t].x += a * F.y;
         acc0[t].y -= a * F.x;
@@ -398,15 +407,15 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const doubl
     if (c0 + c >= H) continue;
     const ModeK m = mode_k(g, i, j, c0 + c);
     const long q = 2 * (base + i * es + c);
-    double2 za{0.0, 0.0}, zb{0.0, 0.0};
+    C2 za{0.0, 0.0}, zb{0.0, 0.0};
     if (m.keep) {
-      za = *reinterpret_cast<const double2 *>(z0 + q);
-      zb = *reinterpret_cast<const double2 *>(z1 + q);
+      za = *reinterpret_cast<const C2 *>(z0 + q);
+      zb = *reinterpret_cast<const C2 *>(z1 + q);
       za = cadd(za, acc0[t]);
       zb = cadd(zb, acc1[t]);
     }
-    *reinterpret_cast<double2 *>(z0 + q) = za;
-    *reinterpret_cast<double2 *>(z1 + q) = zb;
+    *reinterpret_cast<C2 *>(z0 + q) = za;
+    *reinterpret_cast<C2 *>(z1 + q) = zb;
     if (has0 && f == 0)
       for (int d = 0; d < 3; ++d) uk_dc[d] += f00[d];
   }
@@ -445,59 +454,101 @@ constexpr int kFusedC(int n) { return kLinesC(n) / 2 > 0 ? kLinesC(n) / 2 : 1; }
     default: return SPM_ERR_UNSUPPORTED;        \
   }
 
-inline double field_bytes(const spm_ctx *c) { return (double)c->nfield * sizeof(double); }
+inline double field_bytes(const spm_ctx *c) {
+  return (double)c->nfield * (c->prm.precision == SPM_FP32 ? sizeof(float) : sizeof(double));
+}
+
+template <typename T>
+inline T *F(double *p) {
+  return reinterpret_cast<T *>(p);
+}
+
+template <typename T>
+int fx_zeta2u_t(spm_ctx *c, int fix_mask) {
+  const Grid &g = c->grid;
+  SPM_SIZE_SWITCH(g.nx, {
+    constexpr int R = kR4(S), C = kFusedC(S);
+    hipLaunchKernelGGL((x_zeta2u_kernel<T, S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream,
+                       g, F<T>(c->zeta[0]), F<T>(c->zeta[1]), F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), c->uk_dc,
+                       fix_mask, c->tw_x);
+  });
+  return 0;
+}
+
+template <typename T>
+int fz_advect_t(spm_ctx *c) {
+  const Grid &g = c->grid;
+  const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+  SPM_ZSIZE_SWITCH(g.nz / 2, {
+    constexpr int R = kR4(S), ROWS = kRows(S);
+    hipLaunchKernelGGL((z_advect_kernel<T, S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R), 0,
+                       c->stream, g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
+                       c->tw_z, c->tw_rz, scale);
+  });
+  return 0;
+}
+
+template <typename T>
+int fx_update_t(spm_ctx *c, double nu, double dt) {
+  const Grid &g = c->grid;
+  SPM_SIZE_SWITCH(g.nx, {
+    constexpr int R = kR4(S), C = kFusedC(S);
+    hipLaunchKernelGGL((x_update_kernel<T, S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream,
+                       g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
+                       F<T>(c->zeta[0]), F<T>(c->zeta[1]), nu, dt, c->tw_x);
+  });
+  return 0;
+}
+
+template <typename T>
+int fx_project_t(spm_ctx *c) {
+  const Grid &g = c->grid;
+  SPM_SIZE_SWITCH(g.nx, {
+    constexpr int R = kR4(S), C = kFusedC(S);
+    hipLaunchKernelGGL((x_project_kernel<T, S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream,
+                       g, F<T>(c->f[0]), F<T>(c->f[1]), F<T>(c->f[2]), F<T>(c->zeta[0]), F<T>(c->zeta[1]), c->uk_dc,
+                       c->tw_x);
+  });
+  return 0;
+}
+
+#define SPM_PREC_CALL(c, fn, ...) \
+  ((c)->prm.precision == SPM_FP32 ? fn<float>(c, ##__VA_ARGS__) : fn<double>(c, ##__VA_ARGS__))
 
 }  // namespace
 
 int fx_zeta2u(spm_ctx *c, int fix_mask) {
-  const Grid &g = c->grid;
   tbeg(c);
-  SPM_SIZE_SWITCH(g.nx, {
-    constexpr int R = kR4(S), C = kFusedC(S);
-    hipLaunchKernelGGL((x_zeta2u_kernel<S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream, g,
-                       c->zeta[0], c->zeta[1], c->u[0], c->u[1], c->u[2], c->uk_dc, fix_mask, c->tw_x);
-  });
+  const int rc = SPM_PREC_CALL(c, fx_zeta2u_t, fix_mask);
   tend(c, SPM_K_FFT_X, 5.0 * field_bytes(c));
+  if (rc) return rc;
   SPM_LAUNCH_CHECK(c);
   return 0;
 }
 
 int fz_advect(spm_ctx *c) {
-  const Grid &g = c->grid;
-  const double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
   tbeg(c);
-  SPM_ZSIZE_SWITCH(g.nz / 2, {
-    constexpr int R = kR4(S), ROWS = kRows(S);
-    hipLaunchKernelGGL((z_advect_kernel<S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R), 0,
-                       c->stream, g, c->u[0], c->u[1], c->u[2], c->work[0], c->work[1], c->tw_z, c->tw_rz, scale);
-  });
+  const int rc = SPM_PREC_CALL(c, fz_advect_t);
   tend(c, SPM_K_FFT_Z_C2R, 8.0 * field_bytes(c));
+  if (rc) return rc;
   SPM_LAUNCH_CHECK(c);
   return 0;
 }
 
 int fx_update(spm_ctx *c, double nu, double dt) {
-  const Grid &g = c->grid;
   tbeg(c);
-  SPM_SIZE_SWITCH(g.nx, {
-    constexpr int R = kR4(S), C = kFusedC(S);
-    hipLaunchKernelGGL((x_update_kernel<S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream, g,
-                       c->u[0], c->u[1], c->u[2], c->work[0], c->work[1], c->zeta[0], c->zeta[1], nu, dt, c->tw_x);
-  });
+  const int rc = SPM_PREC_CALL(c, fx_update_t, nu, dt);
   tend(c, SPM_K_FFT_X, 9.0 * field_bytes(c));
+  if (rc) return rc;
   SPM_LAUNCH_CHECK(c);
   return 0;
 }
 
 int fx_project(spm_ctx *c) {
-  const Grid &g = c->grid;
   tbeg(c);
-  SPM_SIZE_SWITCH(g.nx, {
-    constexpr int R = kR4(S), C = kFusedC(S);
-    hipLaunchKernelGGL((x_project_kernel<S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream,
-                       g, c->f[0], c->f[1], c->f[2], c->zeta[0], c->zeta[1], c->uk_dc, c->tw_x);
-  });
+  const int rc = SPM_PREC_CALL(c, fx_project_t);
   tend(c, SPM_K_FFT_X, 7.0 * field_bytes(c));
+  if (rc) return rc;
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

@@ -263,11 +263,13 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// u is gathered in the field precision T; sums run in FP64.
+template <typename T>
 __global__ __launch_bounds__(256) void f_hydro_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ pxint,
                                                       const double *__restrict__ pres,
                                                       const int *__restrict__ stencil, int nst,
-                                                      const double *__restrict__ u0, const double *__restrict__ u1,
-                                                      const double *__restrict__ u2, double dmy) {
+                                                      const T *__restrict__ u0, const T *__restrict__ u1,
+                                                      const T *__restrict__ u2, double dmy) {
   const int lane = threadIdx.x & 63;
   const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (n >= pm.np) return;
@@ -303,9 +305,9 @@ __global__ __launch_bounds__(256) void f_hydro_kernel(Grid g, PModel pm, double 
       vr[2] = wp[0] * r[1] - wp[1] * r[0];
     }
     const long im = ((long)rm[0] * g.ny + rm[1]) * g.nz_ + rm[2];
-    const double f0 = ((vp[0] + vr[0]) - u0[im]) * dphi;
-    const double f1 = ((vp[1] + vr[1]) - u1[im]) * dphi;
-    const double f2 = ((vp[2] + vr[2]) - u2[im]) * dphi;
+    const double f0 = ((vp[0] + vr[0]) - (double)u0[im]) * dphi;
+    const double f1 = ((vp[1] + vr[1]) - (double)u1[im]) * dphi;
+    const double f2 = ((vp[2] + vr[2]) - (double)u2[im]) * dphi;
     fx += f0;
     fy += f1;
     fz += f2;
@@ -628,10 +630,16 @@ int p_calc_f_hydro(spm_ctx *c) {
   if (rc) return rc;
   const double dmy = (-c->pm.dx3 * c->pm.rho) / c->prm.dt;
   tbeg(c);
-  hipLaunchKernelGGL(f_hydro_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
-                     c->pxint, c->pres, c->stencil, c->n_stencil, c->u[0], c->u[1], c->u[2], dmy);
+  if (c->prm.precision == SPM_FP32)
+    hipLaunchKernelGGL(f_hydro_kernel<float>, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm,
+                       c->pbuf, c->pxint, c->pres, c->stencil, c->n_stencil, (const float *)c->u[0],
+                       (const float *)c->u[1], (const float *)c->u[2], dmy);
+  else
+    hipLaunchKernelGGL(f_hydro_kernel<double>, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm,
+                       c->pbuf, c->pxint, c->pres, c->stencil, c->n_stencil, (const double *)c->u[0],
+                       (const double *)c->u[1], (const double *)c->u[2], dmy);
   // gathers u (3 doubles) at every stencil point of every particle
-  tend(c, SPM_K_F_HYDRO, 24.0 * (double)c->n_stencil * c->np);
+  tend(c, SPM_K_F_HYDRO, 3.0 * (c->prm.precision == SPM_FP32 ? 4.0 : 8.0) * (double)c->n_stencil * c->np);
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

@@ -540,9 +540,9 @@ int spm_create(const spm_params *prm, spm_ctx **out) {
       g_create_err = "mass ratios must be positive";
       return SPM_ERR_INVALID;
     }
-  if (p.precision != SPM_FP64) {
-    g_create_err = "only the FP64 field path is built";
-    return SPM_ERR_UNSUPPORTED;
+  if (p.precision != SPM_FP64 && p.precision != SPM_FP32) {
+    g_create_err = "precision must be SPM_FP64 or SPM_FP32";
+    return SPM_ERR_INVALID;
   }
   const int nh = (int)p.a + (int)p.a_xi + 2;
   if (p.n_particles > 0 && (p.nx < 2 * nh + 1 || p.ny < 2 * nh + 1 || p.nz < 2 * nh + 1)) {
@@ -579,11 +579,18 @@ int spm_destroy(spm_ctx *ctx) {
 
 int spm_stencil_size(const spm_ctx *ctx) { return ctx ? ctx->n_stencil : -1; }
 
+// Fields cross the boundary as doubles; an FP32 context converts.
 int spm_put_field(spm_ctx *c, int field, const double *host) {
   if (!c || !host) return SPM_ERR_INVALID;
   double *p = field_ptr(c, field);
   if (!p) return SPM_ERR_INVALID;
   SPM_HIP(c, hipSetDevice(c->device));
+  if (c->prm.precision == SPM_FP32) {
+    std::vector<float> tmp(host, host + c->nfield);
+    SPM_HIP(c, hipMemcpyAsync(p, tmp.data(), sizeof(float) * c->nfield, hipMemcpyHostToDevice, c->stream));
+    SPM_HIP(c, hipStreamSynchronize(c->stream));
+    return SPM_OK;
+  }
   SPM_HIP(c, hipMemcpyAsync(p, host, sizeof(double) * c->nfield, hipMemcpyHostToDevice, c->stream));
   SPM_HIP(c, hipStreamSynchronize(c->stream));
   return SPM_OK;
@@ -594,6 +601,13 @@ int spm_get_field(spm_ctx *c, int field, double *host) {
   double *p = field_ptr(c, field);
   if (!p) return SPM_ERR_INVALID;
   SPM_HIP(c, hipSetDevice(c->device));
+  if (c->prm.precision == SPM_FP32) {
+    std::vector<float> tmp(c->nfield);
+    SPM_HIP(c, hipMemcpyAsync(tmp.data(), p, sizeof(float) * c->nfield, hipMemcpyDeviceToHost, c->stream));
+    SPM_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->nfield; ++i) host[i] = tmp[i];
+    return SPM_OK;
+  }
   SPM_HIP(c, hipMemcpyAsync(host, p, sizeof(double) * c->nfield, hipMemcpyDeviceToHost, c->stream));
   SPM_HIP(c, hipStreamSynchronize(c->stream));
   return SPM_OK;
