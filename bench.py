@@ -53,9 +53,9 @@ def algorithmic_step_bytes(n, np_, stencil, w=8, n_fft=20):
     return n_fft * (8 * nn + 16 * n * n * (n // 2 + 1)) * (w / 8) + np_ * stencil * 11 * w
 
 
-def gpu_run(n, np_, steps, warmup, profile_steps, schedule):
+def gpu_run(n, np_, steps, warmup, profile_steps, schedule, precision):
     from kapsel_old_amd.spm import SPM
-    g = SPM(workload_params(n, np_))
+    g = SPM(workload_params(n, np_, precision))
     g.set_schedule(schedule)
     g.put_particles(workload_particles(n, np_))
     g.init_fluid()
@@ -81,6 +81,26 @@ def gpu_run(n, np_, steps, warmup, profile_steps, schedule):
     stencil = g.stencil_size()
     g.close()
     return elapsed, (t3 - t2), ms, by, cnt, stencil
+
+
+def copy_probe(mbytes, reps=5):
+    """Device-to-device copies of a known size through the HIP runtime (a
+    plain copy kernel), used to scale rocprof's FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(mbytes * (1 << 20))
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    t1 = time.perf_counter()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": n, "reps": reps, "GBps_read_plus_write": 2 * n * reps / (t1 - t0) / 1e9}),
+          flush=True)
 
 
 def cpu_baseline(n, np_, max_steps=2, budget_s=40.0):
@@ -113,21 +133,28 @@ def main():
     ap.add_argument("--particles", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", choices=["fused", "literal"], default="fused")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (one box per GPU); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
     n, np_ = args.grid, args.particles
     schedule = 0 if args.schedule == "fused" else 1
+    precision = abi.FP64 if args.precision == "fp64" else abi.FP32
     elapsed, prof_elapsed, ms, by, cnt, stencil = gpu_run(n, np_, args.steps, args.warmup, args.profile_steps,
-                                                          schedule)
+                                                          schedule, precision)
+    wbytes = 8 if precision == abi.FP64 else 4
     value = args.steps / elapsed
     dom = int(np.argmax(ms))
     per_launch_ms = ms[dom] / max(cnt[dom], 1)
     per_launch_bytes = by[dom] / max(cnt[dom], 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-    b_step = algorithmic_step_bytes(n, np_, stencil)
+    b_step = algorithmic_step_bytes(n, np_, stencil, w=wbytes)
     families = {FAMILIES[f]: {"ms_per_step": ms[f] / args.profile_steps, "launches_per_step":
                               cnt[f] / args.profile_steps,
                               "GBps": (by[f] / (ms[f] * 1e-3) / 1e9) if ms[f] > 0 else None}
@@ -143,11 +170,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if precision == abi.FP64 else "f32 fields, f64 particles",
         "data": "synthetic",
         "config": {"workload": "SPM time step (Time_evolution_hydro), %d^3 grid, %d spheres a=5 xi=2, "
-                               "WCA, kBT=0.1, dt=0.05, FP64 fields" % (n, np_),
-                   "grid": [n, n, n], "n_particles": np_, "stencil_points": stencil, "precision": "fp64",
+                               "WCA, kBT=0.1, dt=0.05, %s fields" % (n, np_, args.precision.upper()),
+                   "grid": [n, n, n], "n_particles": np_, "stencil_points": stencil, "precision": args.precision,
                    "schedule": args.schedule},
         "hbm_TBps_algorithmic": b_step * value / 1e12,
         "step_floor_ms_at_8TBps": b_step / 8e12 * 1e3,
@@ -157,7 +184,7 @@ def main():
         "kernel_families": families,
         "profiled_ms_per_step": 1e3 * prof_elapsed / args.profile_steps,
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and precision == abi.FP64:
         v, done, secs = cpu_baseline(n, np_)
         out["cpu_baseline"] = {"value": v, "unit": "time-steps/s", "cores": 1, "kind": "port",
                                "sample": "%d step(s) of the same %d^3/%d workload after an untimed init, "
