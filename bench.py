@@ -103,6 +103,23 @@ def copy_probe(mbytes, reps=5):
           flush=True)
 
 
+def load_traffic(schedule, precision, n, np_):
+    """HBM bytes per launch of each kernel family from the committed
+    rocprofv3 --pmc summary of this workload (FETCH_SIZE and WRITE_SIZE per
+    dispatch, scaled by what the counters read for a plain copy of known
+    size).  None when no summary for this configuration is committed."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s_%s_%dcube_%d.txt" % (schedule, precision, n, np_))
+    if not os.path.exists(path):
+        return None, None
+    out = {}
+    with open(path) as fh:
+        for line in fh:
+            f = line.split()
+            if len(f) == 4 and f[0] == "family":
+                out[f[1]] = (float(f[2]), float(f[3]))
+    return out, os.path.relpath(path, ROOT)
+
+
 def cpu_baseline(n, np_, max_steps=2, budget_s=40.0):
     """The CPU oracle (single-threaded FP64 restatement of the reference
     step) on the same workload: init untimed, then up to max_steps steps."""
@@ -155,6 +172,8 @@ def main():
     per_launch_bytes = by[dom] / max(cnt[dom], 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
     b_step = algorithmic_step_bytes(n, np_, stencil, w=wbytes)
+    traffic, traffic_src = load_traffic(args.schedule, args.precision, n, np_)
+    fam_traffic = traffic.get(FAMILIES[dom]) if traffic else None
     families = {FAMILIES[f]: {"ms_per_step": ms[f] / args.profile_steps, "launches_per_step":
                               cnt[f] / args.profile_steps,
                               "GBps": (by[f] / (ms[f] * 1e-3) / 1e9) if ms[f] > 0 else None}
@@ -179,7 +198,11 @@ def main():
         "hbm_TBps_algorithmic": b_step * value / 1e12,
         "step_floor_ms_at_8TBps": b_step / 8e12 * 1e3,
         "roofline": {"bound": "hbm", "kernel": FAMILIES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (fam_traffic[0] + fam_traffic[1]) if fam_traffic else None,
+                     "traffic_fetch": fam_traffic[0] if fam_traffic else None,
+                     "traffic_write": fam_traffic[1] if fam_traffic else None,
+                     "traffic_source": traffic_src if fam_traffic else None,
                      "bytes_per_launch": per_launch_bytes, "us_per_launch": per_launch_ms * 1e3},
         "kernel_families": families,
         "profiled_ms_per_step": 1e3 * prof_elapsed / args.profile_steps,

@@ -12,8 +12,11 @@
 //   z pass  : rows are contiguous; NZ reals are packed as NZ/2 complex
 //             values, transformed, and split into the half spectrum.
 //   y, x    : complex lines with stride NZ_/2 (y) or NY*NZ_/2 (x); a block
-//             loads C adjacent columns (16 B each) of N points, so global
-//             accesses are C*16-byte contiguous segments.
+//             loads C adjacent columns of N points, so global accesses are
+//             C-element contiguous segments.  The x pass numbers its
+//             NY*(NZ/2+1) columns as one run, so a tile's segment starts on
+//             a 128-byte boundary; the y pass keeps the kz = NZ/2 column of
+//             each x plane for tiles of its own (LineTiles).
 // Inside a block the transform is a radix-8 Stockham FFT: each thread
 // keeps 8 points in registers per stage, stages exchange through LDS laid
 // out [point][column] so a wavefront touches consecutive LDS words.
@@ -22,31 +25,24 @@
 namespace spm {
 
 // ---------------------------------------------------------------------
-// y / x passes: complex lines of N points, element stride `estride`
-// (complex units), `ncols` columns per outer index, outer stride `ostride`.
-// blockIdx.x = column tile, blockIdx.y = outer index, blockIdx.z = field.
+// y / x passes: complex lines of N points at element stride `estride`
+// (complex units).  A block transforms C columns; LineTiles says which.
+// blockIdx.x = tile (XCD-remapped), blockIdx.z = field.
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C, int SIGN>
 __global__ __launch_bounds__(C *N / R) void fft_lines_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw,
-                                                             int ncols, long estride, long ostride) {
+                                                             LineTiles lt, long estride) {
   using C2 = cplx<T>;
-  constexpr int NT = C * N / R;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * C;
-  C2 *g = reinterpret_cast<C2 *>(fp.p[blockIdx.z]) + (long)blockIdx.y * ostride + c0;
-#pragma unroll 4
-  for (int f = tid; f < N * C; f += NT) {
-    const int c = f % C, j = f / C;
-    lds[j * C + c] = (c0 + c < ncols) ? g[j * estride + c] : C2{0, 0};
-  }
-  __syncthreads();
-  fft_stages<T, N, R, C, C, SIGN, 1>(lds, tw, tid);
-#pragma unroll 4
-  for (int f = tid; f < N * C; f += NT) {
-    const int c = f % C, j = f / C;
-    if (c0 + c < ncols) g[j * estride + c] = lds[j * C + c];
-  }
+  const long col = line_column(lt, xcd_tile(blockIdx.x, gridDim.x), tid % C, C);
+  const bool col_ok = col >= 0;  // the thread's column in every stage
+  C2 *g = reinterpret_cast<C2 *>(fp.p[blockIdx.z]) + (col_ok ? col : 0);
+  auto ld = [&](int row, int, int) -> C2 { return col_ok ? g[row * estride] : C2{0, 0}; };
+  auto st = [&](int row, int, int, C2 v) {
+    if (col_ok) g[row * estride] = v;
+  };
+  fft_stages_io<T, N, R, C, C, SIGN, 1, true, true>(lds, tw, tid, ld, st);
 }
 
 // ---------------------------------------------------------------------
@@ -158,11 +154,12 @@ template <typename T, int N, int SIGN>
 void launch_lines_n(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
                     int nouter, hipStream_t st) {
   constexpr int R = kRadix(N), C = kLinesC(N);
-  dim3 grid((ncols + C - 1) / C, nouter, nf);
-  hipLaunchKernelGGL((fft_lines_kernel<T, N, R, C, SIGN>), grid, dim3(C * N / R), 0, st, fp, tw, ncols, estride,
-                     ostride);
+  const LineTiles lt = make_line_tiles(ncols, nouter, ostride, C);
+  hipLaunchKernelGGL((fft_lines_kernel<T, N, R, C, SIGN>), dim3(lt.tiles, 1, nf), dim3(C * N / R), 0, st, fp, tw, lt,
+                     estride);
 }
 
+// one switch arm per supported line length
 template <typename T, int SIGN>
 int launch_lines(int n, const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
                  int nouter, hipStream_t st) {
@@ -251,12 +248,12 @@ static int fft_many_t(spm_ctx *c, double *const *a, int n, bool fwd) {
       rc |= launch_lines<T, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
       tend(c, SPM_K_FFT_Y, bytes);
       tbeg(c);
-      rc |= launch_lines<T, +1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream);
+      rc |= launch_lines<T, +1>(g.nx, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1, c->stream);
       tend(c, SPM_K_FFT_X, bytes);
     } else {
       const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
       tbeg(c);
-      rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream);
+      rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1, c->stream);
       tend(c, SPM_K_FFT_X, bytes);
       tbeg(c);
       rc |= launch_lines<T, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);

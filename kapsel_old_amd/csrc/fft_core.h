@@ -113,6 +113,128 @@ __device__ __forceinline__ void fft_stages(cplx<T> *lds, const double2 *__restri
   }
 }
 
+// The same stages with the first one reading its inputs through a load
+// functor and the last one handing its outputs to a store functor instead
+// of going through LDS: for a line pass that saves the initial copy into LDS
+// and the final copy out of it, and each thread issues its first-stage loads
+// back to back.  Functors see (point, column, q) where q = s*RS + r is the
+// compile-time slot of the value in the thread's registers.
+//   ld(point, column, q) -> cplx<T>
+//   st(point, column, q, value)
+// The first stage synchronises before writing LDS, so a caller may reuse
+// the LDS of a previous transform as soon as its own reads are issued.
+template <typename T, int N, int R, int RS, int NS, int C, int CP, int SIGN, bool IN_G, bool OUT_G, class LD,
+          class ST>
+__device__ __forceinline__ void stockham_stage_io(cplx<T> *lds, const double2 *__restrict__ tw, int tid, LD &ld,
+                                                  ST &st) {
+  using C2 = cplx<T>;
+  constexpr int NB = N / RS;
+  constexpr int PER = R / RS;
+  const int c = tid % C;
+  const int j0 = tid / C;
+  C2 a[R];
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int j = j0 + s * (N / R);
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      if constexpr (IN_G)
+        a[s * RS + r] = ld(j + r * NB, c, s * RS + r);
+      else
+        a[s * RS + r] = lds[(j + r * NB) * CP + c];
+    }
+    if constexpr (NS > 1) {
+      const int k = j % NS;
+      const double2 t1 = tw[k * (N / (NS * RS))];
+      C2 w[RS];
+      w[1] = C2{(T)t1.x, (T)(SIGN * t1.y)};
+#pragma unroll
+      for (int r = 2; r < RS; ++r) w[r] = (r % 2 == 0) ? cmul(w[r / 2], w[r / 2]) : cmul(w[r - 1], w[1]);
+#pragma unroll
+      for (int r = 1; r < RS; ++r) a[s * RS + r] = cmul(a[s * RS + r], w[r]);
+    }
+    dft<C2, RS, SIGN>(&a[s * RS]);
+  }
+  if constexpr (OUT_G) {
+    if constexpr (IN_G) __syncthreads();  // single stage in place: all reads before any write
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      const int j = j0 + s * (N / R);
+      const int base = (j / NS) * NS * RS + j % NS;
+#pragma unroll
+      for (int r = 0; r < RS; ++r) st(base + r * NS, c, s * RS + r, a[s * RS + r]);
+    }
+  } else {
+    __syncthreads();  // every read of this LDS (this stage's or a previous transform's) is done
+#pragma unroll
+    for (int s = 0; s < PER; ++s) {
+      const int j = j0 + s * (N / R);
+      const int base = (j / NS) * NS * RS + j % NS;
+#pragma unroll
+      for (int r = 0; r < RS; ++r) lds[(base + r * NS) * CP + c] = a[s * RS + r];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int N, int R, int C, int CP, int SIGN, int NS, bool FIRST_G, bool LAST_G, class LD, class ST>
+__device__ __forceinline__ void fft_stages_io(cplx<T> *lds, const double2 *__restrict__ tw, int tid, LD &ld, ST &st) {
+  if constexpr (NS < N) {
+    constexpr int RS = (N / NS >= R) ? R : N / NS;
+    constexpr bool in_g = FIRST_G && NS == 1;
+    constexpr bool out_g = LAST_G && NS * RS == N;
+    stockham_stage_io<T, N, R, RS, NS, C, CP, SIGN, in_g, out_g>(lds, tw, tid, ld, st);
+    fft_stages_io<T, N, R, C, CP, SIGN, NS * RS, FIRST_G, LAST_G>(lds, tw, tid, ld, st);
+  }
+}
+
+// Tiles of a line pass.  Column (o, col) starts at complex element
+// o * ostride + col.  Regular tiles take C adjacent columns of one outer
+// index o.  When an outer index has 1 + a multiple of C columns (the
+// NZ/2+1 columns of a y pass), its last column goes to edge tiles that take
+// that column of C consecutive outer indices, so no regular tile runs with
+// one column out of C.
+struct LineTiles {
+  long ostride;   // complex elements between outer indices
+  int ncols;      // columns per outer index in regular tiles
+  int per_outer;  // regular tiles per outer index
+  int nouter;     // outer indices
+  int main;       // regular tiles
+  int tiles;      // regular + edge tiles
+};
+
+inline LineTiles make_line_tiles(int ncols, int nouter, long ostride, int C) {
+  LineTiles t;
+  const bool edge = nouter > 1 && ncols > C && (ncols - 1) % C == 0;
+  t.ostride = ostride;
+  t.ncols = edge ? ncols - 1 : ncols;
+  t.per_outer = (t.ncols + C - 1) / C;
+  t.nouter = nouter;
+  t.main = nouter * t.per_outer;
+  t.tiles = t.main + (edge ? (nouter + C - 1) / C : 0);
+  return t;
+}
+
+// element offset of column c of a tile, or -1 where the tile has no column c
+__device__ __forceinline__ long line_column(const LineTiles &t, int tile, int c, int C) {
+  if (tile < t.main) {
+    const int o = tile / t.per_outer;
+    const int col = (tile - o * t.per_outer) * C + c;
+    return col < t.ncols ? o * t.ostride + col : -1;
+  }
+  const int o = (tile - t.main) * C + c;
+  return o < t.nouter ? o * t.ostride + t.ncols : -1;
+}
+
+// Workgroups are dispatched to the 8 XCDs round robin (id mod 8), and each
+// XCD has its own L2.  Give XCD k the contiguous run of tiles
+// [k*n/8, (k+1)*n/8) so that tiles sharing a cache line at their edges meet
+// in one L2.  The tail beyond a multiple of 8 keeps its own numbering.
+__device__ __forceinline__ int xcd_tile(int w, int n) {
+  const int per = n >> 3;
+  return w < (per << 3) ? (w & 7) * per + (w >> 3) : w;
+}
+
 template <typename T>
 struct FieldPtrs {
   T *p[5];

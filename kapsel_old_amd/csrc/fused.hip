@@ -90,56 +90,55 @@ __device__ __forceinline__ cplx<T> zeta2u_mode(const Grid &g, const ModeK &m, cp
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C>
 __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__restrict__ z0,
-                                                           const T *__restrict__ z1, T *u0, T *u1,
-                                                           T *u2, double *uk_dc, int fix_mask,
+                                                           const T *__restrict__ z1, T *u0, T *u1, T *u2,
+                                                           double *uk_dc, int fix_mask,
                                                            const double2 *__restrict__ tw) {
   using C2 = cplx<T>;
-  constexpr int NT = C * N / R, E = N * C / NT;
+  static_assert(N >= R, "one first-stage butterfly per thread");
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * C, j = blockIdx.y;
+  const int c = tid % C, j0 = tid / C;
   const long H = g.hnz;
-  const long base = (long)j * H + c0;  // complex index of (i = 0, j, c0)
   const long es = (long)g.ny * H;
-  C2 za[E], zb[E];
+  // columns (j, kz) numbered as one run j*(NZ/2+1) + kz: a tile starts on
+  // a 128-byte boundary
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * C;
+  const int col = c0 + c;
+  const bool col_ok = col < es;
+  const int j = col / (int)H, kz = col - j * (int)H;
+  // zeta at the thread's first-stage points i = j0 + q N/R (the 2/3 mask
+  // applied on load)
+  C2 za[R], zb[R];
 #pragma unroll
-  for (int t = 0; t < E; ++t) {
-    const int f = tid + t * NT, c = f % C, i = f / C;
-    za[t] = zb[t] = C2{0, 0};
-    if (c0 + c < H) {
-      const ModeK m = mode_k(g, i, j, c0 + c);
-      if (m.keep) {
-        const long q = 2 * (base + i * es + c);
-        za[t] = *reinterpret_cast<const C2 *>(z0 + q);
-        zb[t] = *reinterpret_cast<const C2 *>(z1 + q);
-      }
+  for (int q = 0; q < R; ++q) {
+    const int i = j0 + q * (N / R);
+    za[q] = zb[q] = C2{0, 0};
+    const ModeK m = mode_k(g, i, j, kz);
+    if (col_ok && m.keep) {
+      const long e = 2 * (col + i * es);
+      za[q] = *reinterpret_cast<const C2 *>(z0 + e);
+      zb[q] = *reinterpret_cast<const C2 *>(z1 + e);
     }
   }
+  const bool has0 = c0 == 0;  // the block holding column (j, kz) = (0, 0)
   double dc[3] = {0.0, 0.0, 0.0};
-  const bool has0 = (j == 0 && c0 == 0);
-  if (has0) {
+  if (has0)
     for (int d = 0; d < 3; ++d) dc[d] = (fix_mask & (1 << d)) ? 0.0 : uk_dc[d];
-  }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     T *out = d == 0 ? u0 : (d == 1 ? u1 : u2);
-#pragma unroll
-    for (int t = 0; t < E; ++t) {
-      const int f = tid + t * NT, c = f % C, i = f / C;
-      const ModeK m = mode_k(g, i, j, c0 + c);
-      C2 v = zeta2u_mode<T>(g, m, za[t], zb[t], d);
-      if (has0 && f == 0) v.x = (T)dc[d];
-      lds[i * C + c] = v;
-    }
-    __syncthreads();
-    fft_stages<T, N, R, C, C, -1, 1>(lds, tw, tid);
-#pragma unroll
-    for (int t = 0; t < E; ++t) {
-      const int f = tid + t * NT, c = f % C, i = f / C;
-      if (c0 + c < H) *reinterpret_cast<C2 *>(out + 2 * (base + i * es + c)) = lds[i * C + c];
-    }
-    __syncthreads();
+    auto ld = [&](int i, int, int q) -> C2 {
+      const ModeK m = mode_k(g, i, j, kz);
+      C2 v = zeta2u_mode<T>(g, m, za[q], zb[q], d);
+      if (has0 && i == 0 && c == 0) v.x = (T)dc[d];
+      return v;
+    };
+    auto st = [&](int i, int, int, C2 v) {
+      if (col_ok) *reinterpret_cast<C2 *>(out + 2 * (col + i * es)) = v;
+    };
+    fft_stages_io<T, N, R, C, C, -1, 1, true, true>(lds, tw, tid, ld, st);
   }
+  __syncthreads();  // every thread of block 0 has read uk_dc
   if (has0 && tid == 0)
     for (int d = 0; d < 3; ++d)
       if (fix_mask & (1 << d)) uk_dc[d] = 0.0;
@@ -255,85 +254,82 @@ __global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, T *u0, T 
 // The transformed products are not stored.
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C>
-__global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0, const T *p1,
-                                                           const T *p2, const T *p3, const T *p4,
-                                                           T *z0, T *z1, double nu, double dt,
-                                                           const double2 *__restrict__ tw) {
+__global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0, const T *p1, const T *p2,
+                                                           const T *p3, const T *p4, T *z0, T *z1, double nu,
+                                                           double dt, const double2 *__restrict__ tw) {
   using C2 = cplx<T>;
-  constexpr int NT = C * N / R, E = N * C / NT;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * C, j = blockIdx.y;
+  const int c = tid % C;
   const long H = g.hnz;
-  const long base = (long)j * H + c0;
   const long es = (long)g.ny * H;
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * C;
+  const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
+  const bool col_ok = col < es;
+  const int j = col / (int)H, kz = col - j * (int)H;
   // The contraction (operate_omega.cxx:60-97) is linear in the five product
   // spectra P0..P4 = u2u3, u3u1, u1u2, u2^2-u3^2, u3^2-u1^2: accumulate the
   // two components Omega_k2zeta_k keeps as each product is transformed.
-  C2 acc0[E], acc1[E];
+  C2 acc0[R], acc1[R];
+  int rows[R];
 #pragma unroll
-  for (int t = 0; t < E; ++t) acc0[t] = acc1[t] = C2{0, 0};
-#pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    const T *in = q == 0 ? p0 : q == 1 ? p1 : q == 2 ? p2 : q == 3 ? p3 : p4;
-#pragma unroll
-    for (int t = 0; t < E; ++t) {
-      const int f = tid + t * NT, c = f % C, i = f / C;
-      lds[i * C + c] = (c0 + c < H) ? *reinterpret_cast<const C2 *>(in + 2 * (base + i * es + c))
-                                    : C2{0, 0};
-    }
-    __syncthreads();
-    fft_stages<T, N, R, C, C, +1, 1>(lds, tw, tid);
-#pragma unroll
-    for (int t = 0; t < E; ++t) {
-      const int f = tid + t * NT, c = f % C, i = f / C;
-      const C2 P = lds[i * C + c];
-      const ModeK m = mode_k(g, i, j, c0 + c);
-      const T k1 = (T)(m.kx * g.wave_x), k2 = (T)(m.ky * g.wave_y), k3 = (T)(m.kz * g.wave_z);
-      // coefficient of P_q in -Omega_s, s = 0, 1, 2
-      T cf[3];
-      if (q == 0) {
-        cf[0] = (k3 - k2) * (k3 + k2); cf[1] = k1 * k2; cf[2] = -(k3 * k1);
-      } else if (q == 1) {
-        cf[0] = -(k1 * k2); cf[1] = (k1 - k3) * (k1 + k3); cf[2] = k2 * k3;
-      } else if (q == 2) {
-        cf[0] = k3 * k1; cf[1] = -(k2 * k3); cf[2] = (k2 - k1) * (k2 + k1);
-      } else if (q == 3) {
-        cf[0] = k2 * k3; cf[1] = 0.0; cf[2] = -(k1 * k2);
-      } else {
-        cf[0] = 0.0; cf[1] = k3 * k1; cf[2] = -(k1 * k2);
-      }
-      const int s0 = m.kx != 0 ? 1 : (m.ky != 0 ? 2 : 0);
-      const int s1 = m.kx != 0 ? 2 : (m.ky != 0 ? 0 : 1);
-      const T a = cf[s0], b = cf[s1];
-      acc0[t].x -= a * P.x;
-      acc0[t].y -= a * P.y;
-      acc1[t].x -= b * P.x;
-      acc1[t].y -= b * P.y;
-    }
-    __syncthreads();
+  for (int q = 0; q < R; ++q) {
+    acc0[q] = acc1[q] = C2{0, 0};
+    rows[q] = 0;
   }
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    const T *in = f == 0 ? p0 : f == 1 ? p1 : f == 2 ? p2 : f == 3 ? p3 : p4;
+    auto ld = [&](int i, int, int) -> C2 {
+      return col_ok ? *reinterpret_cast<const C2 *>(in + 2 * (col + i * es)) : C2{0, 0};
+    };
+    auto st = [&](int i, int, int q, C2 P) {
+      rows[q] = i;
+      const ModeK m = mode_k(g, i, j, kz);
+      const T k1 = (T)(m.kx * g.wave_x), k2 = (T)(m.ky * g.wave_y), k3 = (T)(m.kz * g.wave_z);
+      // coefficient of P_f in -Omega_s, s = 0, 1, 2
+      T cf[3];
+      if (f == 0) {
+        cf[0] = (k3 - k2) * (k3 + k2); cf[1] = k1 * k2; cf[2] = -(k3 * k1);
+      } else if (f == 1) {
+        cf[0] = -(k1 * k2); cf[1] = (k1 - k3) * (k1 + k3); cf[2] = k2 * k3;
+      } else if (f == 2) {
+        cf[0] = k3 * k1; cf[1] = -(k2 * k3); cf[2] = (k2 - k1) * (k2 + k1);
+      } else if (f == 3) {
+        cf[0] = k2 * k3; cf[1] = 0; cf[2] = -(k1 * k2);
+      } else {
+        cf[0] = 0; cf[1] = k3 * k1; cf[2] = -(k1 * k2);
+      }
+      const T a = m.kx != 0 ? cf[1] : (m.ky != 0 ? cf[2] : cf[0]);
+      const T b = m.kx != 0 ? cf[2] : (m.ky != 0 ? cf[0] : cf[1]);
+      acc0[q].x -= a * P.x;
+      acc0[q].y -= a * P.y;
+      acc1[q].x -= b * P.x;
+      acc1[q].y -= b * P.y;
+    };
+    fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
+  }
+  if (!col_ok) return;
   const T dmy0 = (T)(-nu * dt), dmy1 = (T)(1. / nu);
 #pragma unroll
-  for (int t = 0; t < E; ++t) {
-    const int f = tid + t * NT, c = f % C, i = f / C;
-    if (c0 + c >= H) continue;
-    const ModeK m = mode_k(g, i, j, c0 + c);
-    const long q = 2 * (base + i * es + c);
-    C2 za{0.0, 0.0}, zb{0.0, 0.0};
+  for (int q = 0; q < R; ++q) {
+    const int i = rows[q];
+    const ModeK m = mode_k(g, i, j, kz);
+    const long e = 2 * (col + i * es);
+    C2 za{0, 0}, zb{0, 0};
     if (m.keep) {
       const T kk2 = (T)(sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz));
       const T ik2 = kk2 > (T)0 ? (T)1 / kk2 : (T)0;
       const T dmy = exp(dmy0 * kk2) - (T)1;
-      za = *reinterpret_cast<const C2 *>(z0 + q);
-      zb = *reinterpret_cast<const C2 *>(z1 + q);
-      za.x += dmy * (za.x - dmy1 * ik2 * acc0[t].x);
-      za.y += dmy * (za.y - dmy1 * ik2 * acc0[t].y);
-      zb.x += dmy * (zb.x - dmy1 * ik2 * acc1[t].x);
-      zb.y += dmy * (zb.y - dmy1 * ik2 * acc1[t].y);
+      za = *reinterpret_cast<const C2 *>(z0 + e);
+      zb = *reinterpret_cast<const C2 *>(z1 + e);
+      za.x += dmy * (za.x - dmy1 * ik2 * acc0[q].x);
+      za.y += dmy * (za.y - dmy1 * ik2 * acc0[q].y);
+      zb.x += dmy * (zb.x - dmy1 * ik2 * acc1[q].x);
+      zb.y += dmy * (zb.y - dmy1 * ik2 * acc1[q].y);
     }
-    *reinterpret_cast<C2 *>(z0 + q) = za;
-    *reinterpret_cast<C2 *>(z1 + q) = zb;
+    *reinterpret_cast<C2 *>(z0 + e) = za;
+    *reinterpret_cast<C2 *>(z1 + e) = zb;
   }
 }
 
@@ -346,77 +342,71 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
 // keeps the k = 0 mode (IK2(0) = 0).  So the result is the same field.
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C>
-__global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0, const T *f1,
-                                                            const T *f2, T *z0, T *z1, double *uk_dc,
-                                                            const double2 *__restrict__ tw) {
+__global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0, const T *f1, const T *f2, T *z0,
+                                                            T *z1, double *uk_dc, const double2 *__restrict__ tw) {
   using C2 = cplx<T>;
-  constexpr int NT = C * N / R, E = N * C / NT;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * C, j = blockIdx.y;
+  const int c = tid % C;
   const long H = g.hnz;
-  const long base = (long)j * H + c0;
   const long es = (long)g.ny * H;
-  C2 acc0[E], acc1[E];
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * C;
+  const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
+  const bool col_ok = col < es;
+  const int j = col / (int)H, kz = col - j * (int)H;
+  C2 acc0[R], acc1[R];
+  int rows[R];
   double f00[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int t = 0; t < E; ++t) acc0[t] = acc1[t] = C2{0, 0};
+  for (int q = 0; q < R; ++q) {
+    acc0[q] = acc1[q] = C2{0, 0};
+    rows[q] = 0;
+  }
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const T *in = q == 0 ? f0 : (q == 1 ? f1 : f2);
-#pragma unroll
-    for (int t = 0; t < E; ++t) {
-      const int f = tid + t * NT, c = f % C, i = f / C;
-      lds[i * C + c] = (c0 + c < H) ? *reinterpret_cast<const C2 *>(in + 2 * (base + i * es + c))
-                                    : C2{0, 0};
-    }
-    __syncthreads();
-    fft_stages<T, N, R, C, C, +1, 1>(lds, tw, tid);
-#pragma unroll
-    for (int t = 0; t < E; ++t) {
-      const int f = tid + t * NT, c = f % C, i = f / C;
-      const C2 F = lds[i * C + c];
-      if (f == 0) f00[q] = F.x;
-      const ModeK m = mode_k(g, i, j, c0 + c);
+  for (int f = 0; f < 3; ++f) {
+    const T *in = f == 0 ? f0 : (f == 1 ? f1 : f2);
+    auto ld = [&](int i, int, int) -> C2 {
+      return col_ok ? *reinterpret_cast<const C2 *>(in + 2 * (col + i * es)) : C2{0, 0};
+    };
+    auto st = [&](int i, int, int q, C2 F) {
+      rows[q] = i;
+      if (i == 0 && c == 0) f00[f] = F.x;
+      const ModeK m = mode_k(g, i, j, kz);
+      if (!m.keep) return;
       const T kx = (T)(m.kx * g.wave_x), ky = (T)(m.ky * g.wave_y), kz = (T)(m.kz * g.wave_z);
-      // omega = U_k2zeta_k's curl: omega_s gets (ks[a] * F_b - ks[b] * F_a)
-      // with (s, a, b) cyclic; omega_re = .. * Im, omega_im = -(..) * Re
+      // omega_s of U_k2zeta_k's curl gets (ks[a] F_b - ks[b] F_a), (s, a, b)
+      // cyclic; omega_re from Im F, omega_im from -Re F.  Coefficient of F_f in
+      // omega_s: (0,2) ky, (0,1) -kz, (1,0) kz, (1,2) -kx, (2,1) kx, (2,0) -ky.
+      auto coef = [&](int sidx) -> T {
+        if (sidx == 0) return f == 2 ? ky : (f == 1 ? -kz : (T)0);
+        if (sidx == 1) return f == 0 ? kz : (f == 2 ? -kx : (T)0);
+        return f == 1 ? kx : (f == 0 ? -ky : (T)0);
+      };
       const int s0 = m.kx != 0 ? 1 : (m.ky != 0 ? 2 : 0);
       const int s1 = m.kx != 0 ? 2 : (m.ky != 0 ? 0 : 1);
-      // coefficient of F_q in omega_s: +ks[s-1] for q = s+1, -ks[s+1] for q = s-1
-      // (s, q) -> coefficient: (0,2) ky, (0,1) -kz, (1,0) kz, (1,2) -kx, (2,1) kx, (2,0) -ky
-      auto coef = [&](int s) -> T {
-        if (s == 0) return q == 2 ? ky : (q == 1 ? -kz : 0.0);
-        if (s == 1) return q == 0 ? kz : (q == 2 ? -kx : 0.0);
-        return q == 1 ? kx : (q == 0 ? -ky : 0.0);
-      };
       const T a = coef(s0), b = coef(s1);
-      if (m.keep) {
-        acc0[t].x += a * F.y;
-        acc0[t].y -= a * F.x;
-        acc1[t].x += b * F.y;
-        acc1[t].y -= b * F.x;
-      }
-    }
-    __syncthreads();
+      acc0[q].x += a * F.y;
+      acc0[q].y -= a * F.x;
+      acc1[q].x += b * F.y;
+      acc1[q].y -= b * F.x;
+    };
+    fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
   }
-  const bool has0 = (j == 0 && c0 == 0);
+  const bool has0 = c0 == 0;
+  if (!col_ok) return;
 #pragma unroll
-  for (int t = 0; t < E; ++t) {
-    const int f = tid + t * NT, c = f % C, i = f / C;
-    if (c0 + c >= H) continue;
-    const ModeK m = mode_k(g, i, j, c0 + c);
-    const long q = 2 * (base + i * es + c);
-    C2 za{0.0, 0.0}, zb{0.0, 0.0};
+  for (int q = 0; q < R; ++q) {
+    const int i = rows[q];
+    const ModeK m = mode_k(g, i, j, kz);
+    const long e = 2 * (col + i * es);
+    C2 za{0, 0}, zb{0, 0};
     if (m.keep) {
-      za = *reinterpret_cast<const C2 *>(z0 + q);
-      zb = *reinterpret_cast<const C2 *>(z1 + q);
-      za = cadd(za, acc0[t]);
-      zb = cadd(zb, acc1[t]);
+      za = cadd(*reinterpret_cast<const C2 *>(z0 + e), acc0[q]);
+      zb = cadd(*reinterpret_cast<const C2 *>(z1 + e), acc1[q]);
     }
-    *reinterpret_cast<C2 *>(z0 + q) = za;
-    *reinterpret_cast<C2 *>(z1 + q) = zb;
-    if (has0 && f == 0)
+    *reinterpret_cast<C2 *>(z0 + e) = za;
+    *reinterpret_cast<C2 *>(z1 + e) = zb;
+    if (has0 && i == 0 && c == 0)
       for (int d = 0; d < 3; ++d) uk_dc[d] += f00[d];
   }
 }
@@ -468,7 +458,7 @@ int fx_zeta2u_t(spm_ctx *c, int fix_mask) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
     constexpr int R = kR4(S), C = kFusedC(S);
-    hipLaunchKernelGGL((x_zeta2u_kernel<T, S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream,
+    hipLaunchKernelGGL((x_zeta2u_kernel<T, S, R, C>), dim3((g.ny * g.hnz + C - 1) / C), dim3(C * S / R), 0, c->stream,
                        g, F<T>(c->zeta[0]), F<T>(c->zeta[1]), F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), c->uk_dc,
                        fix_mask, c->tw_x);
   });
@@ -493,7 +483,7 @@ int fx_update_t(spm_ctx *c, double nu, double dt) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
     constexpr int R = kR4(S), C = kFusedC(S);
-    hipLaunchKernelGGL((x_update_kernel<T, S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream,
+    hipLaunchKernelGGL((x_update_kernel<T, S, R, C>), dim3((g.ny * g.hnz + C - 1) / C), dim3(C * S / R), 0, c->stream,
                        g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
                        F<T>(c->zeta[0]), F<T>(c->zeta[1]), nu, dt, c->tw_x);
   });
@@ -505,7 +495,7 @@ int fx_project_t(spm_ctx *c) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
     constexpr int R = kR4(S), C = kFusedC(S);
-    hipLaunchKernelGGL((x_project_kernel<T, S, R, C>), dim3((g.hnz + C - 1) / C, g.ny), dim3(C * S / R), 0, c->stream,
+    hipLaunchKernelGGL((x_project_kernel<T, S, R, C>), dim3((g.ny * g.hnz + C - 1) / C), dim3(C * S / R), 0, c->stream,
                        g, F<T>(c->f[0]), F<T>(c->f[1]), F<T>(c->f[2]), F<T>(c->zeta[0]), F<T>(c->zeta[1]), c->uk_dc,
                        c->tw_x);
   });

@@ -68,9 +68,21 @@ __device__ __forceinline__ void tile_span(int xint, int nh, int ntile, int &t0, 
   t0 = ((a % ntile) + ntile) % ntile;
 }
 
+// Thread (n, s) handles slot s = (a, b, c) of the k0 x k1 x k2 box of tiles
+// particle n's stencil cube can reach: one atomic per thread rather than a
+// loop of up to k0*k1*k2 atomics per particle.  Slot (0, 0, 0) also stores
+// the anchor of Particle_cell (make_phi.h:102-130).
+__device__ __forceinline__ int tile_slot(int s, const int k[3], const int t0[3], const int cn[3], const int nt[3]) {
+  const int a = s / (k[1] * k[2]), b = (s / k[2]) % k[1], cc = s % k[2];
+  if (a >= cn[0] || b >= cn[1] || cc >= cn[2]) return -1;
+  return ((t0[0] + a) % nt[0] * nt[1] + (t0[1] + b) % nt[1]) * nt[2] + (t0[2] + cc) % nt[2];
+}
+
 __global__ void anchor_count_kernel(Grid g, PModel pm, const double *__restrict__ pb, int *pxint, double *pres,
-                                    int tx, int ty, int tz, int *tile_count) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+                                    int tx, int ty, int tz, int k0, int k1, int k2, int *tile_count) {
+  const int per = k0 * k1 * k2;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = (int)(t / per), s = (int)(t % per);
   if (n >= pm.np) return;
   const double idx = 1. / pm.dx;
   const int N[3] = {g.nx, g.ny, g.nz};
@@ -81,59 +93,76 @@ __global__ void anchor_count_kernel(Grid g, PModel pm, const double *__restrict_
     // positions are kept in [0, L) by PBC; the clamp only keeps indices in
     // range if a non-finite or escaped position ever arrives
     xi[d] = (dmy >= 0.0 && dmy < (double)N[d]) ? (int)dmy : 0;
-    pxint[3 * n + d] = xi[d];
-    pres[3 * n + d] = (dmy - (double)xi[d]) * pm.dx;
+    if (s == 0) {
+      pxint[3 * n + d] = xi[d];
+      pres[3 * n + d] = (dmy - (double)xi[d]) * pm.dx;
+    }
   }
   int t0[3], cn[3];
-  const int nt[3] = {tx, ty, tz};
+  const int nt[3] = {tx, ty, tz}, k[3] = {k0, k1, k2};
 #pragma unroll
   for (int d = 0; d < 3; ++d) tile_span(xi[d], pm.nh, nt[d], t0[d], cn[d]);
-  for (int a = 0; a < cn[0]; ++a)
-    for (int b = 0; b < cn[1]; ++b)
-      for (int cc = 0; cc < cn[2]; ++cc) {
-        const int ti = ((t0[0] + a) % tx * ty + (t0[1] + b) % ty) * tz + (t0[2] + cc) % tz;
-        atomicAdd(&tile_count[ti], 1);
-      }
+  const int ti = tile_slot(s, k, t0, cn, nt);
+  if (ti >= 0) atomicAdd(&tile_count[ti], 1);
 }
 
-__global__ void tile_fill_kernel(PModel pm, const int *__restrict__ pxint, int tx, int ty, int tz, int *tile_cursor,
-                                 int *tile_list) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void tile_fill_kernel(PModel pm, const int *__restrict__ pxint, int tx, int ty, int tz, int k0, int k1,
+                                 int k2, int *tile_cursor, int *tile_list) {
+  const int per = k0 * k1 * k2;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = (int)(t / per), s = (int)(t % per);
   if (n >= pm.np) return;
   int t0[3], cn[3];
-  const int nt[3] = {tx, ty, tz};
+  const int nt[3] = {tx, ty, tz}, k[3] = {k0, k1, k2};
 #pragma unroll
   for (int d = 0; d < 3; ++d) tile_span(pxint[3 * n + d], pm.nh, nt[d], t0[d], cn[d]);
-  for (int a = 0; a < cn[0]; ++a)
-    for (int b = 0; b < cn[1]; ++b)
-      for (int cc = 0; cc < cn[2]; ++cc) {
-        const int ti = ((t0[0] + a) % tx * ty + (t0[1] + b) % ty) * tz + (t0[2] + cc) % tz;
-        const int slot = atomicAdd(&tile_cursor[ti], 1);
-        tile_list[slot] = n;
-      }
+  const int ti = tile_slot(s, k, t0, cn, nt);
+  if (ti >= 0) tile_list[atomicAdd(&tile_cursor[ti], 1)] = n;
 }
 
-// Single-block exclusive scan of count[0..n) into start[] and cursor[].
+// Single-block exclusive scan of count[0..n) into start[] and cursor[],
+// 4096 elements per round: each thread takes 4 consecutive counts, the
+// wavefronts scan their 64 partial sums with shuffles, and one LDS exchange
+// combines the 16 wavefronts.
 __global__ __launch_bounds__(1024) void scan_kernel(const int *__restrict__ count, int *start, int *cursor, int n) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int chunk = (n + 1023) / 1024;
-  const int b = t * chunk, e = min(n, b + chunk);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += count[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = (t >= off) ? part[t - off] : 0;
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int carry = 0;
+  for (int base = 0; base < n; base += 4096) {
+    const int i0 = base + 4 * t;
+    int v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = (i0 + q < n) ? count[i0 + q] : 0;
+    const int s = v[0] + v[1] + v[2] + v[3];
+    int x = s;  // inclusive scan over the wavefront
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
     __syncthreads();
-    part[t] += v;
+    if (w == 0) {
+      int y = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        const int z = __shfl_up(y, off, 64);
+        if (lane >= off) y += z;
+      }
+      if (lane < 16) wsum[lane] = y;  // inclusive over wavefronts
+    }
     __syncthreads();
-  }
-  int run = part[t] - s;
-  for (int i = b; i < e; ++i) {
-    start[i] = run;
-    cursor[i] = run;
-    run += count[i];
+    int run = carry + (w > 0 ? wsum[w - 1] : 0) + x - s;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (i0 + q < n) {
+        start[i0 + q] = run;
+        cursor[i0 + q] = run;
+      }
+      run += v[q];
+    }
+    carry += wsum[15];
+    __syncthreads();  // wsum is rewritten next round
   }
 }
 
@@ -570,12 +599,17 @@ int p_build_tiles(spm_ctx *c) {
   if (c->np == 0) return 0;
   tbeg(c);
   hipLaunchKernelGGL(izero_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_count, c->ntiles);
-  hipLaunchKernelGGL(anchor_count_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf,
-                     c->pxint, c->pres, c->tiles[0], c->tiles[1], c->tiles[2], c->tile_count);
+  // tiles a stencil cube of 2 nh + 1 points can reach per axis (the same
+  // bound sizes tile_list)
+  int k[3];
+  for (int d = 0; d < 3; ++d) k[d] = std::min(c->tiles[d], (2 * c->pm.nh + 1 + 7) / 8 + 1);
+  const long slots = (long)c->np * k[0] * k[1] * k[2];
+  hipLaunchKernelGGL(anchor_count_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->pxint, c->pres, c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_count);
   hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->tile_count, c->tile_start, c->tile_cursor,
                      c->ntiles);
-  hipLaunchKernelGGL(tile_fill_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pxint,
-                     c->tiles[0], c->tiles[1], c->tiles[2], c->tile_cursor, c->tile_list);
+  hipLaunchKernelGGL(tile_fill_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->pm, c->pxint,
+                     c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_cursor, c->tile_list);
   hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_start,
                      c->tile_count, c->tile_list, c->ntiles);
   tend(c, SPM_K_PARTICLE, 16.0 * c->ntiles);

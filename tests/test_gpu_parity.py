@@ -381,3 +381,32 @@ def test_fused_stage_matches_literal(SPM, stage):
         print("stage %d %s rel err %.3e" % (stage, nm, e))
         assert e < TOL
     assert np.abs(fus.get_uk_dc() - lit.get_uk_dc()).max() <= 1e-12 * max(1.0, np.abs(lit.get_uk_dc()).max())
+
+
+def test_fused_matches_literal_bench_grid(SPM):
+    """The fused and literal schedules agree at the benchmark's 256^3 grid
+    (256-point lines: y-pass edge tiles for kz = NZ/2, x-pass columns as one
+    run) over two steps with 200 spheres, WCA and rotation."""
+    n, a, np_ = 256, 5.0, 200
+    prm = abi.make_params(n, a=a, a_xi=2.0, n_particles=np_, rotation=1, dt=0.05, lj_truncate=abi.LJ_WCA,
+                          lj_powers=abi.LJ_12_6, epsilon=1.0, fix_cell=(1, 1, 1))
+    p = particles(np_, (n, n, n), a, seed=91, vel=0.01, omega=0.002)
+    runs = []
+    for schedule in (0, 1):
+        g = SPM(prm)
+        g.set_schedule(schedule)
+        g.put_particles(p)
+        g.init_fluid()
+        g.step(2)
+        runs.append(g)
+    fus, lit = runs
+    for nm in ("zeta0", "zeta1", "u0", "u2", "phi"):
+        e = rel(fus.get_field(nm), lit.get_field(nm))
+        print("256^3 %s rel err %.3e" % (nm, e), flush=True)
+        assert e < TOL, (nm, e)
+    for k in ("x", "v", "omega", "f_hydro_previous"):
+        e = rel(fus.get_particles()[k], lit.get_particles()[k])
+        print("256^3 particle %s rel err %.3e" % (k, e), flush=True)
+        assert e < TOL, (k, e)
+    for g in runs:
+        g.close()
