@@ -1,10 +1,14 @@
 """Builds kapsel_old_amd/libspm.so for gfx950 with hipcc, in tree.
 
-Usage: python -m kapsel_old_amd.build [--jobs N]
+Usage: python -m kapsel_old_amd.build [--jobs N] [--define NAME=VALUE ...] [--out PATH]
+
+--define overrides a launch-shape constant of csrc/fft_core.h (SPM_LINE_POINTS,
+SPM_LINE_RADIX, SPM_Z_POINTS, SPM_FUSED_RADIX, SPM_FUSED_X_DIV, ...) and
+--out writes that build somewhere other than the in-tree library.
 """
+import hashlib
 import os
 import subprocess
-import sys
 from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -21,9 +25,9 @@ FLAGS = [
 ]
 
 
-def _compile(src):
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(src, objdir=OBJ, defines=()):
+    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -32,23 +36,28 @@ def _compile(src):
     return obj
 
 
-def build(jobs=4, verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+def build(jobs=4, verbose=False, defines=(), out=LIB):
+    defines = tuple(defines)
+    objdir = OBJ if not defines else os.path.join(OBJ, hashlib.sha1(" ".join(defines).encode()).hexdigest()[:10])
+    os.makedirs(objdir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda src: _compile(src, objdir, defines), SOURCES))
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr[-4000:])
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":
-    jobs = 4
-    if "--jobs" in sys.argv:
-        jobs = int(sys.argv[sys.argv.index("--jobs") + 1])
-    build(jobs=jobs, verbose=True)
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--define", action="append", default=[])
+    ap.add_argument("--out", default=LIB)
+    a = ap.parse_args()
+    build(jobs=a.jobs, verbose=True, defines=a.define, out=os.path.abspath(a.out))

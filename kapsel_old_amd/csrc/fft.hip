@@ -30,11 +30,15 @@ namespace spm {
 // blockIdx.x = tile (XCD-remapped), blockIdx.z = field.
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C, int SIGN>
-__global__ __launch_bounds__(C *N / R) void fft_lines_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw,
+__global__ __launch_bounds__(C *N / R) void fft_lines_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw_g,
                                                              LineTiles lt, long estride) {
   using C2 = cplx<T>;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
+  // twiddle tables in LDS: the transform's barriers then wait on LDS
+  // only, and global loads issued ahead stay in flight across them
+  __shared__ double2 tw[N];
+  for (int t = tid; t < N; t += blockDim.x) tw[t] = tw_g[t];
   const long col = line_column(lt, xcd_tile(blockIdx.x, gridDim.x), tid % C, C);
   const bool col_ok = col >= 0;  // the thread's column in every stage
   C2 *g = reinterpret_cast<C2 *>(fp.p[blockIdx.z]) + (col_ok ? col : 0);
@@ -49,8 +53,8 @@ __global__ __launch_bounds__(C *N / R) void fft_lines_kernel(FieldPtrs<T> fp, co
 // z pass, forward: ROWS rows per block, each NZ = 2M reals -> M+1 complex.
 // ---------------------------------------------------------------------
 template <typename T, int M, int R, int ROWS>
-__global__ __launch_bounds__(ROWS *M / R) void fft_z_r2c_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw,
-                                                               const double2 *__restrict__ twr, int nrows,
+__global__ __launch_bounds__(ROWS *M / R) void fft_z_r2c_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw_g,
+                                                               const double2 *__restrict__ twr_g, int nrows,
                                                                int nz_) {
   using C2 = cplx<T>;
   constexpr int NT = ROWS * M / R;
@@ -59,6 +63,12 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_r2c_kernel(FieldPtrs<T> fp,
   __shared__ C2 lds[M * CP];
   T *a = fp.p[blockIdx.z];
   const int tid = threadIdx.x;
+  // twiddle tables in LDS: the transform's barriers then wait on LDS
+  // only, and global loads issued ahead stay in flight across them
+  __shared__ double2 tw[M];
+  for (int t = tid; t < M; t += blockDim.x) tw[t] = tw_g[t];
+  __shared__ double2 twr[M + 1];
+  for (int t = tid; t < M + 1; t += blockDim.x) twr[t] = twr_g[t];
   const long row0 = (long)blockIdx.x * ROWS;
 #pragma unroll 4
   for (int f = tid; f < ROWS * M; f += NT) {
@@ -86,8 +96,8 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_r2c_kernel(FieldPtrs<T> fp,
 // z pass, inverse: M+1 complex -> NZ = 2M reals per row, times `scale`.
 // ---------------------------------------------------------------------
 template <typename T, int M, int R, int ROWS>
-__global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw,
-                                                               const double2 *__restrict__ twr, int nrows, int nz_,
+__global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw_g,
+                                                               const double2 *__restrict__ twr_g, int nrows, int nz_,
                                                                T scale) {
   using C2 = cplx<T>;
   constexpr int NT = ROWS * M / R;
@@ -97,6 +107,12 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp,
   __shared__ C2 lds[H * CP];
   T *a = fp.p[blockIdx.z];
   const int tid = threadIdx.x;
+  // twiddle tables in LDS: the transform's barriers then wait on LDS
+  // only, and global loads issued ahead stay in flight across them
+  __shared__ double2 tw[M];
+  for (int t = tid; t < M; t += blockDim.x) tw[t] = tw_g[t];
+  __shared__ double2 twr[M + 1];
+  for (int t = tid; t < M + 1; t += blockDim.x) twr[t] = twr_g[t];
   const long row0 = (long)blockIdx.x * ROWS;
   for (int f = tid; f < ROWS * H; f += NT) {
     const int k = f % H, r = f / H;
@@ -146,9 +162,6 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp,
 // ---------------------------------------------------------------------
 namespace {
 
-constexpr int kLinesC(int n) { return n >= 4096 ? 1 : (4096 / n > 32 ? 32 : 4096 / n); }
-constexpr int kRows(int m) { return 2048 / m > 32 ? 32 : (2048 / m < 1 ? 1 : 2048 / m); }
-constexpr int kRadix(int n) { return n >= 8 ? 8 : n; }
 
 template <typename T, int N, int SIGN>
 void launch_lines_n(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
@@ -179,7 +192,7 @@ int launch_lines(int n, const FieldPtrs<T> &fp, int nf, const double2 *tw, int n
 template <typename T, int M>
 void launch_z_n(bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr, int nrows, int nz_,
                 T scale, hipStream_t st) {
-  constexpr int R = kRadix(M), ROWS = kRows(M);
+  constexpr int R = kZRadix<T>(M), ROWS = kRows<T>(M);
   dim3 grid((nrows + ROWS - 1) / ROWS, 1, nf);
   if (fwd)
     hipLaunchKernelGGL((fft_z_r2c_kernel<T, M, R, ROWS>), grid, dim3(ROWS * M / R), 0, st, fp, tw, twr, nrows, nz_);

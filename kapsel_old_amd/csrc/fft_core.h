@@ -6,6 +6,72 @@
 
 namespace spm {
 
+// Launch shapes.  Each may be overridden at compile time (-D...) to
+// measure another shape; the defaults are the measured choices for gfx950
+// at 256^3 (FP64 fields move twice the bytes per point of FP32 ones, so
+// some choices differ by precision).
+#ifndef SPM_LINE_POINTS
+#define SPM_LINE_POINTS 4096  // points (line length x columns) per line-pass block
+#endif
+#ifndef SPM_LINE_RADIX
+#define SPM_LINE_RADIX 8  // Stockham radix of the plain y/x passes
+#endif
+#ifndef SPM_Z_RADIX_F64
+#define SPM_Z_RADIX_F64 4  // Stockham radix of the plain z passes, FP64 fields
+#endif
+#ifndef SPM_Z_RADIX_F32
+#define SPM_Z_RADIX_F32 8
+#endif
+#ifndef SPM_Z_POINTS_F64
+#define SPM_Z_POINTS_F64 2048  // complex points (NZ/2 x rows) per z-pass block
+#endif
+#ifndef SPM_Z_POINTS_F32
+#define SPM_Z_POINTS_F32 2048
+#endif
+#ifndef SPM_FUSED_RADIX
+#define SPM_FUSED_RADIX 4  // radix of the fused x passes (more live registers)
+#endif
+#ifndef SPM_ZADV_RADIX
+#define SPM_ZADV_RADIX 4  // radix of the fused z pass (advection products)
+#endif
+#ifndef SPM_ZADV_POINTS
+#define SPM_ZADV_POINTS 2048
+#endif
+#ifndef SPM_FUSED_X_DIV
+#define SPM_FUSED_X_DIV 2  // fused x passes take 1/DIV of a plain block's columns
+#endif
+#ifndef SPM_XREMAP_F64
+#define SPM_XREMAP_F64 0  // XCD-grouped tiles in the fused x passes, FP64 (tiles are whole lines)
+#endif
+#ifndef SPM_XREMAP_F32
+#define SPM_XREMAP_F32 1  // FP32 (two tiles share each 128-byte line)
+#endif
+
+constexpr int clamp_rows(int r) { return r > 32 ? 32 : (r < 1 ? 1 : r); }
+// columns per line-pass block (at most 32)
+constexpr int kLinesC(int n) {
+  return n >= SPM_LINE_POINTS ? 1 : (SPM_LINE_POINTS / n > 32 ? 32 : SPM_LINE_POINTS / n);
+}
+constexpr int kRadix(int n) { return n >= SPM_LINE_RADIX ? SPM_LINE_RADIX : n; }
+// rows and radix of a plain z-pass block of m complex points
+template <typename T>
+constexpr int kRows(int m) {
+  return clamp_rows((sizeof(T) == 8 ? SPM_Z_POINTS_F64 : SPM_Z_POINTS_F32) / m);
+}
+template <typename T>
+constexpr int kZRadix(int m) {
+  return m >= (sizeof(T) == 8 ? SPM_Z_RADIX_F64 : SPM_Z_RADIX_F32) ? (sizeof(T) == 8 ? SPM_Z_RADIX_F64 : SPM_Z_RADIX_F32)
+                                                                   : m;
+}
+constexpr int kZadvR(int n) { return n >= SPM_ZADV_RADIX ? SPM_ZADV_RADIX : n; }
+constexpr int kZadvRows(int m) { return clamp_rows(SPM_ZADV_POINTS / m); }
+constexpr int kFusedR(int n) { return n >= SPM_FUSED_RADIX ? SPM_FUSED_RADIX : n; }
+constexpr int kFusedC(int n) { return kLinesC(n) / SPM_FUSED_X_DIV > 0 ? kLinesC(n) / SPM_FUSED_X_DIV : 1; }
+template <typename T>
+__device__ __forceinline__ int fused_x_tile() {
+  return (sizeof(T) == 8 ? SPM_XREMAP_F64 : SPM_XREMAP_F32) ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+}
+
 template <typename C2, int SIGN>
 __device__ __forceinline__ C2 mul_si(C2 a) {  // a * (SIGN i)
   return SIGN > 0 ? C2{-a.y, a.x} : C2{a.y, -a.x};
@@ -226,14 +292,6 @@ __device__ __forceinline__ long line_column(const LineTiles &t, int tile, int c,
   return o < t.nouter ? o * t.ostride + t.ncols : -1;
 }
 
-// Workgroups are dispatched to the 8 XCDs round robin (id mod 8), and each
-// XCD has its own L2.  Give XCD k the contiguous run of tiles
-// [k*n/8, (k+1)*n/8) so that tiles sharing a cache line at their edges meet
-// in one L2.  The tail beyond a multiple of 8 keeps its own numbering.
-__device__ __forceinline__ int xcd_tile(int w, int n) {
-  const int per = n >> 3;
-  return w < (per << 3) ? (w & 7) * per + (w >> 3) : w;
-}
 
 template <typename T>
 struct FieldPtrs {

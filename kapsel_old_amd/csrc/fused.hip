@@ -92,17 +92,21 @@ template <typename T, int N, int R, int C>
 __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__restrict__ z0,
                                                            const T *__restrict__ z1, T *u0, T *u1, T *u2,
                                                            double *uk_dc, int fix_mask,
-                                                           const double2 *__restrict__ tw) {
+                                                           const double2 *__restrict__ tw_g) {
   using C2 = cplx<T>;
   static_assert(N >= R, "one first-stage butterfly per thread");
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
+  // twiddle tables in LDS: the transform's barriers then wait on LDS
+  // only, and global loads issued ahead stay in flight across them
+  __shared__ double2 tw[N];
+  for (int t = tid; t < N; t += blockDim.x) tw[t] = tw_g[t];
   const int c = tid % C, j0 = tid / C;
   const long H = g.hnz;
   const long es = (long)g.ny * H;
   // columns (j, kz) numbered as one run j*(NZ/2+1) + kz: a tile starts on
   // a 128-byte boundary
-  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * C;
+  const int c0 = fused_x_tile<T>() * C;
   const int col = c0 + c;
   const bool col_ok = col < es;
   const int j = col / (int)H, kz = col - j * (int)H;
@@ -148,15 +152,37 @@ __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__r
 // z pass: C2R of (u0, u1, u2) -> real products of U2advection_k -> R2C of
 // the five products into (p0 .. p4).  p0..p2 may alias u0..u2.
 // ---------------------------------------------------------------------
+// The ROWS x (M+1) complex values of a block's rows, KF per thread
+// (element f = tid + t NT is (k = f % (M+1), row = f / (M+1))).
+template <int M, int R, int ROWS>
+struct ZRows {
+  static constexpr int NT = ROWS * M / R, H = M + 1, CP = ROWS + 1, KF = (ROWS * H + NT - 1) / NT;
+};
+
 template <typename T, int M, int R, int ROWS>
-__device__ __forceinline__ void z_c2r_row_to_lds(cplx<T> *lds, const T *a, long row0, int nrows, int nz_,
-                                                 const double2 *__restrict__ twr, int tid) {
-  using C2 = cplx<T>;
-  constexpr int NT = ROWS * M / R, CP = ROWS + 1, H = M + 1, HP = M / 2 + 1;
-  for (int f = tid; f < ROWS * H; f += NT) {
-    const int k = f % H, r = f / H;
+__device__ __forceinline__ void z_rows_fetch(const T *a, long row0, int nrows, int nz_, int tid, cplx<T> *dst) {
+  using Z = ZRows<M, R, ROWS>;
+#pragma unroll
+  for (int t = 0; t < Z::KF; ++t) {
+    const int f = tid + t * Z::NT, k = f % Z::H, r = f / Z::H;
     const long row = row0 + r;
-    lds[k * CP + r] = (row < nrows) ? *reinterpret_cast<const C2 *>(a + row * nz_ + 2 * k) : C2{0, 0};
+    dst[t] = (f < ROWS * Z::H && row < nrows) ? *reinterpret_cast<const cplx<T> *>(a + row * nz_ + 2 * k)
+                                              : cplx<T>{0, 0};
+  }
+}
+
+// fetched rows -> LDS, then the C2R pre-twiddle that packs each row's half
+// spectrum into M complex values for the inverse complex transform
+template <typename T, int M, int R, int ROWS>
+__device__ __forceinline__ void z_c2r_regs_to_lds(cplx<T> *lds, const cplx<T> *src,
+                                                  const double2 *__restrict__ twr, int tid) {
+  using C2 = cplx<T>;
+  using Z = ZRows<M, R, ROWS>;
+  constexpr int NT = Z::NT, CP = Z::CP, H = Z::H, HP = M / 2 + 1;
+#pragma unroll
+  for (int t = 0; t < Z::KF; ++t) {
+    const int f = tid + t * NT;
+    if (f < ROWS * H) lds[(f % H) * CP + f / H] = src[t];
   }
   __syncthreads();
   for (int f = tid; f < ROWS * HP; f += NT) {
@@ -206,18 +232,32 @@ __device__ __forceinline__ void z_r2c_lds_to_row(cplx<T> *lds, T *a, long row0, 
 
 template <typename T, int M, int R, int ROWS>
 __global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, T *u0, T *u1, T *u2, T *p3,
-                                                              T *p4, const double2 *__restrict__ tw,
-                                                              const double2 *__restrict__ twr, T scale) {
+                                                              T *p4, const double2 *__restrict__ tw_g,
+                                                              const double2 *__restrict__ twr_g, T scale) {
   using C2 = cplx<T>;
   constexpr int NT = ROWS * M / R, CP = ROWS + 1, E = ROWS * M / NT;
   __shared__ C2 lds[(M + 1) * CP];
   const int tid = threadIdx.x;
+  // twiddle tables in LDS: the transform's barriers then wait on LDS
+  // only, and global loads issued ahead stay in flight across them
+  __shared__ double2 tw[M];
+  for (int t = tid; t < M; t += blockDim.x) tw[t] = tw_g[t];
+  __shared__ double2 twr[M + 1];
+  for (int t = tid; t < M + 1; t += blockDim.x) twr[t] = twr_g[t];
   const long row0 = (long)blockIdx.x * ROWS;
   const int nrows = g.nx * g.ny;
   C2 v[3][E];
+  // component d+1 is fetched while component d is transformed
+  constexpr int KF = ZRows<M, R, ROWS>::KF;
+  C2 cur[KF], nxt[KF];
+  z_rows_fetch<T, M, R, ROWS>(u0, row0, nrows, g.nz_, tid, cur);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    z_c2r_row_to_lds<T, M, R, ROWS>(lds, d == 0 ? u0 : (d == 1 ? u1 : u2), row0, nrows, g.nz_, twr, tid);
+    if (d < 2) z_rows_fetch<T, M, R, ROWS>(d == 0 ? u1 : u2, row0, nrows, g.nz_, tid, nxt);
+    z_c2r_regs_to_lds<T, M, R, ROWS>(lds, cur, twr, tid);
+    if (d < 2)
+#pragma unroll
+      for (int t = 0; t < KF; ++t) cur[t] = nxt[t];
     fft_stages<T, M, R, ROWS, CP, -1, 1>(lds, tw, tid);
 #pragma unroll
     for (int t = 0; t < E; ++t) {
@@ -256,14 +296,18 @@ __global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, T *u0, T 
 template <typename T, int N, int R, int C>
 __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0, const T *p1, const T *p2,
                                                            const T *p3, const T *p4, T *z0, T *z1, double nu,
-                                                           double dt, const double2 *__restrict__ tw) {
+                                                           double dt, const double2 *__restrict__ tw_g) {
   using C2 = cplx<T>;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
+  // twiddle tables in LDS: the transform's barriers then wait on LDS
+  // only, and global loads issued ahead stay in flight across them
+  __shared__ double2 tw[N];
+  for (int t = tid; t < N; t += blockDim.x) tw[t] = tw_g[t];
   const int c = tid % C;
   const long H = g.hnz;
   const long es = (long)g.ny * H;
-  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * C;
+  const int c0 = fused_x_tile<T>() * C;
   const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
   const bool col_ok = col < es;
   const int j = col / (int)H, kz = col - j * (int)H;
@@ -277,12 +321,21 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
     acc0[q] = acc1[q] = C2{0, 0};
     rows[q] = 0;
   }
+  // Software pipeline over the five spectra: the first-stage points of
+  // spectrum f+1 (i = j0 + q N/R) are loaded while spectrum f is
+  // transformed in LDS; the barriers of the transform wait on LDS only.
+  const int j0 = tid / C;
+  auto fetch = [&](const T *in, C2 *dst) {
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+      dst[q] = col_ok ? *reinterpret_cast<const C2 *>(in + 2 * (col + (long)(j0 + q * (N / R)) * es)) : C2{0, 0};
+  };
+  C2 cur[R], nxt[R];
+  fetch(p0, cur);
 #pragma unroll
   for (int f = 0; f < 5; ++f) {
-    const T *in = f == 0 ? p0 : f == 1 ? p1 : f == 2 ? p2 : f == 3 ? p3 : p4;
-    auto ld = [&](int i, int, int) -> C2 {
-      return col_ok ? *reinterpret_cast<const C2 *>(in + 2 * (col + i * es)) : C2{0, 0};
-    };
+    if (f < 4) fetch(f == 0 ? p1 : f == 1 ? p2 : f == 2 ? p3 : p4, nxt);
+    auto ld = [&](int, int, int q) -> C2 { return cur[q]; };
     auto st = [&](int i, int, int q, C2 P) {
       rows[q] = i;
       const ModeK m = mode_k(g, i, j, kz);
@@ -308,6 +361,9 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
       acc1[q].y -= b * P.y;
     };
     fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
+    if (f < 4)
+#pragma unroll
+      for (int q = 0; q < R; ++q) cur[q] = nxt[q];
   }
   if (!col_ok) return;
   const T dmy0 = (T)(-nu * dt), dmy1 = (T)(1. / nu);
@@ -343,14 +399,18 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C>
 __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0, const T *f1, const T *f2, T *z0,
-                                                            T *z1, double *uk_dc, const double2 *__restrict__ tw) {
+                                                            T *z1, double *uk_dc, const double2 *__restrict__ tw_g) {
   using C2 = cplx<T>;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
+  // twiddle tables in LDS: the transform's barriers then wait on LDS
+  // only, and global loads issued ahead stay in flight across them
+  __shared__ double2 tw[N];
+  for (int t = tid; t < N; t += blockDim.x) tw[t] = tw_g[t];
   const int c = tid % C;
   const long H = g.hnz;
   const long es = (long)g.ny * H;
-  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * C;
+  const int c0 = fused_x_tile<T>() * C;
   const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
   const bool col_ok = col < es;
   const int j = col / (int)H, kz = col - j * (int)H;
@@ -362,12 +422,19 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0
     acc0[q] = acc1[q] = C2{0, 0};
     rows[q] = 0;
   }
+  // software pipeline over the three components, as in x_update_kernel
+  const int j0 = tid / C;
+  auto fetch = [&](const T *in, C2 *dst) {
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+      dst[q] = col_ok ? *reinterpret_cast<const C2 *>(in + 2 * (col + (long)(j0 + q * (N / R)) * es)) : C2{0, 0};
+  };
+  C2 cur[R], nxt[R];
+  fetch(f0, cur);
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    const T *in = f == 0 ? f0 : (f == 1 ? f1 : f2);
-    auto ld = [&](int i, int, int) -> C2 {
-      return col_ok ? *reinterpret_cast<const C2 *>(in + 2 * (col + i * es)) : C2{0, 0};
-    };
+    if (f < 2) fetch(f == 0 ? f1 : f2, nxt);
+    auto ld = [&](int, int, int q) -> C2 { return cur[q]; };
     auto st = [&](int i, int, int q, C2 F) {
       rows[q] = i;
       if (i == 0 && c == 0) f00[f] = F.x;
@@ -391,6 +458,9 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0
       acc1[q].y -= b * F.x;
     };
     fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
+    if (f < 2)
+#pragma unroll
+      for (int q = 0; q < R; ++q) cur[q] = nxt[q];
   }
   const bool has0 = c0 == 0;
   if (!col_ok) return;
@@ -410,13 +480,6 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0
       for (int d = 0; d < 3; ++d) uk_dc[d] += f00[d];
   }
 }
-
-constexpr int kLinesC(int n) { return n >= 4096 ? 1 : (4096 / n > 32 ? 32 : 4096 / n); }
-constexpr int kRows(int m) { return 2048 / m > 32 ? 32 : (2048 / m < 1 ? 1 : 2048 / m); }
-constexpr int kR4(int n) { return n >= 4 ? 4 : n; }
-// fused x passes: half the columns of a plain pass, 512-thread blocks, so
-// each thread may hold up to 256 VGPRs
-constexpr int kFusedC(int n) { return kLinesC(n) / 2 > 0 ? kLinesC(n) / 2 : 1; }
 
 #define SPM_SIZE_SWITCH(n, ...)                \
   switch (n) {                                  \
@@ -457,7 +520,7 @@ template <typename T>
 int fx_zeta2u_t(spm_ctx *c, int fix_mask) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
-    constexpr int R = kR4(S), C = kFusedC(S);
+    constexpr int R = kFusedR(S), C = kFusedC(S);
     hipLaunchKernelGGL((x_zeta2u_kernel<T, S, R, C>), dim3((g.ny * g.hnz + C - 1) / C), dim3(C * S / R), 0, c->stream,
                        g, F<T>(c->zeta[0]), F<T>(c->zeta[1]), F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), c->uk_dc,
                        fix_mask, c->tw_x);
@@ -470,7 +533,7 @@ int fz_advect_t(spm_ctx *c) {
   const Grid &g = c->grid;
   const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
   SPM_ZSIZE_SWITCH(g.nz / 2, {
-    constexpr int R = kR4(S), ROWS = kRows(S);
+    constexpr int R = kZadvR(S), ROWS = kZadvRows(S);
     hipLaunchKernelGGL((z_advect_kernel<T, S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R), 0,
                        c->stream, g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
                        c->tw_z, c->tw_rz, scale);
@@ -482,7 +545,7 @@ template <typename T>
 int fx_update_t(spm_ctx *c, double nu, double dt) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
-    constexpr int R = kR4(S), C = kFusedC(S);
+    constexpr int R = kFusedR(S), C = kFusedC(S);
     hipLaunchKernelGGL((x_update_kernel<T, S, R, C>), dim3((g.ny * g.hnz + C - 1) / C), dim3(C * S / R), 0, c->stream,
                        g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
                        F<T>(c->zeta[0]), F<T>(c->zeta[1]), nu, dt, c->tw_x);
@@ -494,7 +557,7 @@ template <typename T>
 int fx_project_t(spm_ctx *c) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
-    constexpr int R = kR4(S), C = kFusedC(S);
+    constexpr int R = kFusedR(S), C = kFusedC(S);
     hipLaunchKernelGGL((x_project_kernel<T, S, R, C>), dim3((g.ny * g.hnz + C - 1) / C), dim3(C * S / R), 0, c->stream,
                        g, F<T>(c->f[0]), F<T>(c->f[1]), F<T>(c->f[2]), F<T>(c->zeta[0]), F<T>(c->zeta[1]), c->uk_dc,
                        c->tw_x);

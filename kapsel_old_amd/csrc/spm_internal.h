@@ -38,6 +38,19 @@ struct cplx_t<float> {
 template <typename T>
 using cplx = typename cplx_t<T>::type;
 
+// Workgroups are dispatched to the 8 XCDs round robin (id mod 8), and each
+// XCD has its own L2.  Give XCD k the contiguous run of tiles
+// [k*n/8, (k+1)*n/8) so that tiles sharing a cache line at their edges meet
+// in one L2.  The tail beyond a multiple of 8 keeps its own numbering.
+#ifndef SPM_XCD_REMAP
+#define SPM_XCD_REMAP 1
+#endif
+__device__ __forceinline__ int xcd_tile(int w, int n) {
+  if (!SPM_XCD_REMAP) return w;
+  const int per = n >> 3;
+  return w < (per << 3) ? (w & 7) * per + (w >> 3) : w;
+}
+
 template <typename C2>
 __device__ __forceinline__ C2 cadd(C2 a, C2 b) {
   return C2{a.x + b.x, a.y + b.y};

@@ -14,7 +14,9 @@ import numpy as np
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libspm.so")
+# SPM_LIBRARY names another build of the library (python -m
+# kapsel_old_amd.build --out PATH); the default is the in-tree one.
+LIB_PATH = os.environ.get("SPM_LIBRARY") or os.path.join(HERE, "libspm.so")
 
 SYMBOLS = [
     "spm_create", "spm_destroy", "spm_strerror", "spm_last_error", "spm_stencil_size",
@@ -48,7 +50,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError("kapsel_old_amd/libspm.so is not built; run python -m kapsel_old_amd.build")
+            raise RuntimeError("%s is not built; run python -m kapsel_old_amd.build" % LIB_PATH)
         L = C.CDLL(LIB_PATH)
         vp, ip, dp = C.c_void_p, C.c_int, C.POINTER(C.c_double)
         L.spm_create.argtypes = [C.POINTER(abi.SpmParams), C.POINTER(vp)]
