@@ -24,7 +24,6 @@ namespace spm {
 namespace {
 
 constexpr int kTile = 8;
-constexpr int kStage = 64;  // particles per LDS round of make_phi
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 __device__ __forceinline__ int nint(double a) { return a >= 0.0 ? (int)(a + .5) : (int)(a - .5); }
@@ -220,60 +219,42 @@ __global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const 
   const int s0 = tile_start[tile], cnt = tile_count[tile];
   const double rcut = pm.radius + pm.hxi;
   const bool sw_up = mode != 1;
-  // the tile's particles, staged through LDS kStage at a time (anchor,
-  // position, residue, v, omega), so the per-point loop reads broadcast LDS
-  // words instead of chains of global loads
-  __shared__ int s_xi[3 * kStage];
-  __shared__ double s_d[12 * kStage];
+  // n and the particle's members are the same for every thread of the
+  // block, so the compiler reads them with scalar loads
   double ph = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  for (int b0 = 0; b0 < cnt; b0 += kStage) {  // cnt is uniform over the block
-    const int m = min(kStage, cnt - b0);
-    __syncthreads();  // the previous round's reads are done
-    if ((int)threadIdx.x < m) {
-      const int n = tile_list[s0 + b0 + threadIdx.x];
-      double *d = s_d + 12 * threadIdx.x;
+  for (int q = 0; q < cnt; ++q) {  // ascending particle index, as the reference adds
+    const int n = tile_list[s0 + q];
+    int cell[3];
+    bool in = true;
 #pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        s_xi[3 * threadIdx.x + e] = pxint[3 * n + e];
-        d[e] = P(pb, pm.np, P_X + e, n);
-        d[3 + e] = pres[3 * n + e];
-        d[6 + e] = P(pb, pm.np, P_V + e, n);
-        d[9 + e] = P(pb, pm.np, P_OMEGA + e, n);
-      }
+    for (int d = 0; d < 3; ++d) {
+      int o = gi[d] - pxint[3 * n + d];  // |o| < N
+      if (o >= N[d] / 2) o -= N[d];
+      if (o < -(N[d] / 2)) o += N[d];
+      cell[d] = o;
+      in = in && (o <= pm.nh && o >= -pm.nh);
     }
-    __syncthreads();
-    for (int q = 0; q < m; ++q) {  // ascending particle index, as the reference adds
-      const double *d = s_d + 12 * q;
-      int cell[3];
-      bool in = true;
+    if (!in) continue;
+    const double xp[3] = {P(pb, pm.np, P_X, n), P(pb, pm.np, P_X + 1, n), P(pb, pm.np, P_X + 2, n)};
+    double x12[3];
+    const double dist = distance0(g, x, xp, x12);
+    if (!(dist < rcut)) continue;
+    const double dphi = phi_of(pm, dist);
+    ph += dphi;
+    if (sw_up) {
+      double vr[3] = {0.0, 0.0, 0.0};
+      if (pm.rotation) {
+        double r[3];
 #pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        int o = gi[e] - s_xi[3 * q + e];  // |o| < N
-        if (o >= N[e] / 2) o -= N[e];
-        if (o < -(N[e] / 2)) o += N[e];
-        cell[e] = o;
-        in = in && (o <= pm.nh && o >= -pm.nh);
+        for (int d = 0; d < 3; ++d) r[d] = (double)cell[d] * pm.dx - pres[3 * n + d];
+        const double w0 = P(pb, pm.np, P_OMEGA, n), w1 = P(pb, pm.np, P_OMEGA + 1, n), w2 = P(pb, pm.np, P_OMEGA + 2, n);
+        vr[0] = w1 * r[2] - w2 * r[1];
+        vr[1] = w2 * r[0] - w0 * r[2];
+        vr[2] = w0 * r[1] - w1 * r[0];
       }
-      if (!in) continue;
-      double x12[3];
-      const double dist = distance0(g, x, d, x12);
-      if (!(dist < rcut)) continue;
-      const double dphi = phi_of(pm, dist);
-      ph += dphi;
-      if (sw_up) {
-        double vr[3] = {0.0, 0.0, 0.0};
-        if (pm.rotation) {
-          double r[3];
-#pragma unroll
-          for (int e = 0; e < 3; ++e) r[e] = (double)cell[e] * pm.dx - d[3 + e];
-          vr[0] = d[10] * r[2] - d[11] * r[1];
-          vr[1] = d[11] * r[0] - d[9] * r[2];
-          vr[2] = d[9] * r[1] - d[10] * r[0];
-        }
-        a0 += (d[6] + vr[0]) * dphi;
-        a1 += (d[7] + vr[1]) * dphi;
-        a2 += (d[8] + vr[2]) * dphi;
-      }
+      a0 += (P(pb, pm.np, P_V, n) + vr[0]) * dphi;
+      a1 += (P(pb, pm.np, P_V + 1, n) + vr[1]) * dphi;
+      a2 += (P(pb, pm.np, P_V + 2, n) + vr[2]) * dphi;
     }
   }
   if (sw_up && ph > 1.) {  // make_phi.cxx:280-297
