@@ -246,6 +246,51 @@ int spm_timing_enable(spm_ctx *ctx, int on);
 int spm_timing_read(spm_ctx *ctx, double ms[SPM_NUM_KERNEL_FAMILIES], double bytes[SPM_NUM_KERNEL_FAMILIES],
                     long launches[SPM_NUM_KERNEL_FAMILIES]);
 
+/* ---- charged colloids in an electrolyte (SW_EQ == Electrolyte) ---------
+ * The reference reads these from constitutive_eq.Electrolyte of the UDF
+ * (input.cxx:660-790) into globals (input.cxx:238-258).  Valencies and
+ * Onsager coefficients are indexed by ion species: with n_spec == 1,
+ * [0] is the counterion (Valency_counterion, Onsager_coeff_counterion);
+ * with n_spec == 2 (Add_salt), [0] the positive and [1] the negative ion. */
+typedef struct spm_electrolyte {
+  int n_spec;                                /* N_spec, 1 or 2                 */
+  double valency[2];                         /* Valency_*                      */
+  double onsager[2];                         /* Onsager_coeff_*                */
+  double dielectric;                         /* Dielectric_cst                 */
+  double elementary_charge;                  /* Elementary_charge (1 in KAPSEL) */
+  double debye_length;                       /* Debye_length (n_spec == 2)     */
+  double surface_charge[SPM_MAX_SPECIES];    /* Surface_charge per particle species */
+  int external_field;                        /* External_field                 */
+  int ac;                                    /* AC                             */
+  double e_ext[3];                           /* E_ext                          */
+  double angular_frequency;                  /* Angular_Frequency = 2 pi Frequency */
+} spm_electrolyte;
+
+/* Switches the context to the electrolyte equations (Time_evolution_hydro's
+ * SW_EQ == Electrolyte branches, sp_3d_ns.cxx:46-57,89-121,183) and
+ * allocates the ion fields.  Call once, after spm_create.  Electrolyte
+ * steps run the literal schedule (FP64). */
+int spm_set_electrolyte(spm_ctx *ctx, const spm_electrolyte *el);
+/* Init_rho_ion (operate_electrolyte.cxx:695-815) with the uniform initial
+ * distribution (Poisson_Boltzmann off): Set_uniform_ion_charge_density_salt
+ * / _nosalt, then Count_solute_each.  Needs the particles. */
+int spm_init_rho_ion(spm_ctx *ctx);
+/* Concentration[species] spectrum (reference layout) in and out. */
+int spm_put_concentration(spm_ctx *ctx, int species, const double *host);
+int spm_get_concentration(spm_ctx *ctx, int species, double *host);
+/* Total_solute[0 .. n_spec) in and out. */
+int spm_put_total_solute(spm_ctx *ctx, const double *total);
+int spm_get_total_solute(spm_ctx *ctx, double *total);
+/* NSsolute_solver_Euler (fluid_solver.cxx:306-326): explicit Euler step of
+ * zeta and the ion spectra with Rhs_NS_Nernst_Planck (:243-280). */
+int spm_nssolute_solver_euler(spm_ctx *ctx);
+/* Rescale_solute (solute_rhs.cxx:97-126): ion spectra rescaled to
+ * Total_solute over the fluid volume. */
+int spm_rescale_solute(spm_ctx *ctx);
+/* Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177): the
+ * Coulomb body force on the fluid into the f field (real space). */
+int spm_make_coulomb_force(spm_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,42 @@ class SpmParams(C.Structure):
     ]
 
 
+class SpmElectrolyte(C.Structure):
+    """spm_electrolyte of include/spm.h (constitutive_eq.Electrolyte)."""
+    _fields_ = [
+        ("n_spec", C.c_int),
+        ("valency", C.c_double * 2),
+        ("onsager", C.c_double * 2),
+        ("dielectric", C.c_double),
+        ("elementary_charge", C.c_double),
+        ("debye_length", C.c_double),
+        ("surface_charge", C.c_double * MAX_SPECIES),
+        ("external_field", C.c_int),
+        ("ac", C.c_int),
+        ("e_ext", C.c_double * DIM),
+        ("angular_frequency", C.c_double),
+    ]
+
+
+def make_electrolyte(*, n_spec=2, valency=(1.0, -1.0), onsager=(1.0, 1.0), dielectric=1.0,
+                     elementary_charge=1.0, debye_length=3.0, surface_charge=(50.0,), e_ext=(0.0, 0.0, 0.0),
+                     external_field=None, ac=0, angular_frequency=0.0):
+    """Build an SpmElectrolyte; external_field defaults to on when e_ext != 0."""
+    e = SpmElectrolyte()
+    e.n_spec = n_spec
+    for i in range(2):
+        e.valency[i] = valency[i] if i < len(valency) else 0.0
+        e.onsager[i] = onsager[i] if i < len(onsager) else 0.0
+    e.dielectric, e.elementary_charge, e.debye_length = dielectric, elementary_charge, debye_length
+    for i, q in enumerate(surface_charge):
+        e.surface_charge[i] = q
+    for d in range(DIM):
+        e.e_ext[d] = e_ext[d]
+    e.external_field = int(any(x != 0.0 for x in e_ext)) if external_field is None else external_field
+    e.ac, e.angular_frequency = ac, angular_frequency
+    return e
+
+
 class SpmQuaternion(C.Structure):
     _fields_ = [("s", C.c_double), ("v", D3)]
 

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspm.so")
-SOURCES = ["fft.hip", "fused.hip", "spectral.hip", "particles.hip", "spm_api.cpp"]
+SOURCES = ["fft.hip", "fused.hip", "spectral.hip", "particles.hip", "electrolyte.hip", "spm_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

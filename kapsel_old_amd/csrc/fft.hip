@@ -29,47 +29,24 @@ namespace spm {
 // (complex units).  A block transforms C columns; LineTiles says which.
 // blockIdx.x = tile (XCD-remapped), blockIdx.z = field.
 // ---------------------------------------------------------------------
-// SPM_LINES_WAVES caps VGPRs so that that many waves per SIMD fit (4: two
-// 512-thread FP64 blocks per CU, which is what their LDS allows).
-#ifndef SPM_LINES_WAVES
-#define SPM_LINES_WAVES 4
-#endif
 template <typename T, int N, int R, int C, int SIGN>
-__global__ __launch_bounds__(C *N / R) __attribute__((amdgpu_waves_per_eu(SPM_LINES_WAVES))) void fft_lines_kernel(FieldPtrs<T> fp, int nf, const double2 *__restrict__ tw_g,
+__global__ __launch_bounds__(C *N / R) void fft_lines_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw_g,
                                                              LineTiles lt, long estride) {
   using C2 = cplx<T>;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
-  // twiddle table in LDS: the transform's barriers then wait on LDS only,
-  // and global loads issued ahead stay in flight across them
+  // twiddle tables in LDS: the transform's barriers then wait on LDS
+  // only, and global loads issued ahead stay in flight across them
   __shared__ double2 tw[N];
   for (int t = tid; t < N; t += blockDim.x) tw[t] = tw_g[t];
   const long col = line_column(lt, xcd_tile(blockIdx.x, gridDim.x), tid % C, C);
   const bool col_ok = col >= 0;  // the thread's column in every stage
-  const long off = col_ok ? col : 0;
-  // The block transforms the same columns of all nf fields, fetching the
-  // first-stage points (j0 + q N/R) of field f+1 while field f is in LDS.
-  const int j0 = tid / C;
-  auto fetch = [&](const T *a, C2 *dst) {
-#pragma unroll
-    for (int q = 0; q < R; ++q)
-      dst[q] = col_ok ? reinterpret_cast<const C2 *>(a)[off + (long)(j0 + q * (N / R)) * estride] : C2{0, 0};
+  C2 *g = reinterpret_cast<C2 *>(fp.p[blockIdx.z]) + (col_ok ? col : 0);
+  auto ld = [&](int row, int, int) -> C2 { return col_ok ? g[row * estride] : C2{0, 0}; };
+  auto st = [&](int row, int, int, C2 v) {
+    if (col_ok) g[row * estride] = v;
   };
-  C2 cur[R], nxt[R];
-  fetch(fp.p[0], cur);
-#pragma unroll
-  for (int f = 0; f < 5; ++f) {
-    if (f >= nf) break;
-    if (f + 1 < nf) fetch(fp.p[f + 1 < 5 ? f + 1 : 4], nxt);
-    C2 *g = reinterpret_cast<C2 *>(fp.p[f]) + off;
-    auto ld = [&](int, int, int q) -> C2 { return cur[q]; };
-    auto st = [&](int row, int, int, C2 v) {
-      if (col_ok) g[row * estride] = v;
-    };
-    fft_stages_io<T, N, R, C, C, SIGN, 1, true, true>(lds, tw, tid, ld, st);
-#pragma unroll
-    for (int q = 0; q < R; ++q) cur[q] = nxt[q];
-  }
+  fft_stages_io<T, N, R, C, C, SIGN, 1, true, true>(lds, tw, tid, ld, st);
 }
 
 // ---------------------------------------------------------------------
@@ -191,7 +168,7 @@ void launch_lines_n(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols
                     int nouter, hipStream_t st) {
   constexpr int R = kRadix(N), C = kLinesC(N);
   const LineTiles lt = make_line_tiles(ncols, nouter, ostride, C);
-  hipLaunchKernelGGL((fft_lines_kernel<T, N, R, C, SIGN>), dim3(lt.tiles), dim3(C * N / R), 0, st, fp, nf, tw, lt,
+  hipLaunchKernelGGL((fft_lines_kernel<T, N, R, C, SIGN>), dim3(lt.tiles, 1, nf), dim3(C * N / R), 0, st, fp, tw, lt,
                      estride);
 }
 

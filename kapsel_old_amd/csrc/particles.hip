@@ -288,6 +288,115 @@ __global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const 
 }
 
 // ---------------------------------------------------------------------
+// electrolyte: charge profile and surface normals, owner-computes over the
+// same tiles as make_phi
+// ---------------------------------------------------------------------
+// DPhi_compact_sin (profile.h:168-181): the surface-shell profile.
+__device__ __forceinline__ double dphi_compact_sin(const PModel &pm, double x) {
+  const double xi = 2.0 * pm.hxi, hxi = xi * .5, ixi = 1. / xi;
+  const double dmy_x = pm.radius - x;
+  return fabs(dmy_x) < hxi ? M_PI * .5 * ixi * cos(M_PI * dmy_x * ixi) : 0.;
+}
+
+// The tile and offset arithmetic of make_phi_kernel: cell offset of grid
+// index gi from particle n's anchor, in [-N/2, N/2), and whether it lies in
+// the stencil cube.
+__device__ __forceinline__ bool stencil_offset(const int gi[3], const int *xi, const int N[3], int nh, int cell[3]) {
+  bool in = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int o = gi[d] - xi[d];
+    if (o >= N[d] / 2) o -= N[d];
+    if (o < -(N[d] / 2)) o += N[d];
+    cell[d] = o;
+    in = in && (o <= nh && o >= -nh);
+  }
+  return in;
+}
+
+// Make_phi_qq_particle (operate_electrolyte.cxx:179-227) before its
+// normalisation: phi (no clamp) and sum_n q_n DPhi(d_n), particles in
+// ascending order at every point as the reference's particle loop adds.
+__global__ __launch_bounds__(512) void make_phi_qq_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+                                                          const int *__restrict__ spec, const int *__restrict__ pxint,
+                                                          const int *__restrict__ tile_start,
+                                                          const int *__restrict__ tile_count,
+                                                          const int *__restrict__ tile_list, int tx, int ty, int tz,
+                                                          QTable qt, double *phi, double *surface) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tk = tile % tz, tj = (tile / tz) % ty, ti = tile / (tz * ty);
+  const int lk = threadIdx.x % kTile, lj = (threadIdx.x / kTile) % kTile, li = threadIdx.x / (kTile * kTile);
+  const int gi[3] = {ti * kTile + li, tj * kTile + lj, tk * kTile + lk};
+  const int N[3] = {g.nx, g.ny, g.nz};
+  const double x[3] = {gi[0] * pm.dx, gi[1] * pm.dx, gi[2] * pm.dx};
+  const int s0 = tile_start[tile], cnt = tile_count[tile];
+  const double rcut = pm.radius + pm.hxi;
+  double ph = 0.0, sf = 0.0;
+  for (int q = 0; q < cnt; ++q) {
+    const int n = tile_list[s0 + q];
+    int cell[3];
+    if (!stencil_offset(gi, pxint + 3 * n, N, pm.nh, cell)) continue;
+    const double xp[3] = {P(pb, pm.np, P_X, n), P(pb, pm.np, P_X + 1, n), P(pb, pm.np, P_X + 2, n)};
+    double x12[3];
+    const double dist = distance0(g, x, xp, x12);
+    if (!(dist < rcut)) continue;  // Phi and DPhi vanish
+    ph += phi_of(pm, dist);
+    sf += qt.q[spec[n]] * dphi_compact_sin(pm, dist);
+  }
+  const long im = ((long)gi[0] * g.ny + gi[1]) * g.nz_ + gi[2];
+  phi[im] = ph;
+  surface[im] = sf;
+  if (gi[2] == g.nz - 1) phi[im + 1] = phi[im + 2] = surface[im + 1] = surface[im + 2] = 0.0;
+}
+
+// Make_surface_normal (make_phi.cxx:17-58): sum of the unit radial vectors
+// r/|r| of the particles whose shell |r| - a| < xi/2 holds the point, with
+// r = offset * dx - residue (Relative_coord).  Any such point lies in the
+// particle's stencil, so the shell test alone selects the reference's points.
+__global__ __launch_bounds__(512) void surface_normal_kernel(Grid g, PModel pm, const int *__restrict__ pxint,
+                                                             const double *__restrict__ pres,
+                                                             const int *__restrict__ tile_start,
+                                                             const int *__restrict__ tile_count,
+                                                             const int *__restrict__ tile_list, int tx, int ty, int tz,
+                                                             double *n0, double *n1, double *n2) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tk = tile % tz, tj = (tile / tz) % ty, ti = tile / (tz * ty);
+  const int lk = threadIdx.x % kTile, lj = (threadIdx.x / kTile) % kTile, li = threadIdx.x / (kTile * kTile);
+  const int gi[3] = {ti * kTile + li, tj * kTile + lj, tk * kTile + lk};
+  const int N[3] = {g.nx, g.ny, g.nz};
+  const int s0 = tile_start[tile], cnt = tile_count[tile];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int q = 0; q < cnt; ++q) {
+    const int n = tile_list[s0 + q];
+    int cell[3];
+    if (!stencil_offset(gi, pxint + 3 * n, N, pm.nh, cell)) continue;
+    double r[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r[d] = (double)cell[d] * pm.dx - pres[3 * n + d];
+    const double dmy_r = sqrt(sq(r[0]) + sq(r[1]) + sq(r[2]));
+    if (fabs(dmy_r - pm.radius) < pm.hxi) {
+      const double ir = 1. / dmy_r;
+      a0 += r[0] * ir;
+      a1 += r[1] * ir;
+      a2 += r[2] * ir;
+    }
+  }
+  const long im = ((long)gi[0] * g.ny + gi[1]) * g.nz_ + gi[2];
+  n0[im] = a0;
+  n1[im] = a1;
+  n2[im] = a2;
+  if (gi[2] == g.nz - 1) n0[im + 1] = n0[im + 2] = n1[im + 1] = n1[im + 2] = n2[im + 1] = n2[im + 2] = 0.0;
+}
+
+// copies three consecutive particle slots (f_hydro1 = f_hydro, ...)
+__global__ void slot_copy_kernel(double *pb, int np, int dst, int src) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * np) return;
+  const int d = t / np, n = t % np;
+  PR(pb, np, dst + d, n) = P(pb, np, src + d, n);
+}
+
+// ---------------------------------------------------------------------
 // Calc_f_hydro_correct_precision: one wavefront per particle
 // ---------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
@@ -584,6 +693,12 @@ __global__ void thermostat_kernel(PModel pm, double *pb, const int *__restrict__
 
 inline unsigned blocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+#define RCP(x)              \
+  do {                      \
+    int rc_ = (x);          \
+    if (rc_) return rc_;    \
+  } while (0)
+
 }  // namespace
 
 // ---------------------------------------------------------------------
@@ -661,7 +776,11 @@ int p_make_phi(spm_ctx *c, int sw_up) {
 // current); with no particles f = -0 * u.
 int p_make_f(spm_ctx *c) { return launch_make_phi(c, 2, nullptr, c->f[0], c->f[1], c->f[2]); }
 
-int p_calc_f_hydro(spm_ctx *c) {
+int p_calc_f_hydro(spm_ctx *c) { return p_calc_f_hydro_field(c, c->u); }
+
+// Calc_f_hydro_correct_precision(p, fld, jikan) on any real-space field
+// (the electrolyte step passes the f field, sp_3d_ns.cxx:97-106).
+int p_calc_f_hydro_field(spm_ctx *c, double *const fld[3]) {
   if (c->np == 0) return 0;
   // anchors (x_int, residue) for the current positions
   int rc = p_build_tiles(c);
@@ -670,14 +789,59 @@ int p_calc_f_hydro(spm_ctx *c) {
   tbeg(c);
   if (c->prm.precision == SPM_FP32)
     hipLaunchKernelGGL(f_hydro_kernel<float>, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm,
-                       c->pbuf, c->pxint, c->pres, c->stencil, c->n_stencil, (const float *)c->u[0],
-                       (const float *)c->u[1], (const float *)c->u[2], dmy);
+                       c->pbuf, c->pxint, c->pres, c->stencil, c->n_stencil, (const float *)fld[0],
+                       (const float *)fld[1], (const float *)fld[2], dmy);
   else
     hipLaunchKernelGGL(f_hydro_kernel<double>, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm,
-                       c->pbuf, c->pxint, c->pres, c->stencil, c->n_stencil, (const double *)c->u[0],
-                       (const double *)c->u[1], (const double *)c->u[2], dmy);
+                       c->pbuf, c->pxint, c->pres, c->stencil, c->n_stencil, (const double *)fld[0],
+                       (const double *)fld[1], (const double *)fld[2], dmy);
   // gathers u (3 doubles) at every stencil point of every particle
   tend(c, SPM_K_F_HYDRO, 3.0 * (c->prm.precision == SPM_FP32 ? 4.0 : 8.0) * (double)c->n_stencil * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+// f_hydro1 = f_hydro, torque_hydro1 = torque_hydro (sp_3d_ns.cxx:101-106)
+int p_copy_hydro1(spm_ctx *c) {
+  if (c->np == 0) return 0;
+  tbeg(c);
+  hipLaunchKernelGGL(slot_copy_kernel, dim3(blocks(3L * c->np, 256)), dim3(256), 0, c->stream, c->pbuf, c->np,
+                     (int)P_F_HYDRO1, (int)P_F_HYDRO);
+  hipLaunchKernelGGL(slot_copy_kernel, dim3(blocks(3L * c->np, 256)), dim3(256), 0, c->stream, c->pbuf, c->np,
+                     (int)P_TORQUE_HYDRO1, (int)P_TORQUE_HYDRO);
+  tend(c, SPM_K_PARTICLE, 96.0 * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+// phi (no clamp) and the unnormalised surface charge profile; builds the
+// tile lists at the current positions first.
+int p_make_phi_qq(spm_ctx *c, double *phi, double *surface) {
+  if (c->np == 0) {
+    RCP(k_zero(c, phi, c->nfield));
+    return k_zero(c, surface, c->nfield);
+  }
+  RCP(p_build_tiles(c));
+  tbeg(c);
+  hipLaunchKernelGGL(make_phi_qq_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
+                     c->pxint, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2],
+                     c->qtab, phi, surface);
+  tend(c, SPM_K_MAKE_PHI, 16.0 * c->nfield);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+// Make_surface_normal into c->surface_normal (tile lists must be current).
+int p_surface_normal(spm_ctx *c) {
+  if (c->np == 0) {
+    for (int d = 0; d < 3; ++d) RCP(k_zero(c, c->surface_normal[d], c->nfield));
+    return 0;
+  }
+  tbeg(c);
+  hipLaunchKernelGGL(surface_normal_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pxint,
+                     c->pres, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2],
+                     c->surface_normal[0], c->surface_normal[1], c->surface_normal[2]);
+  tend(c, SPM_K_MAKE_PHI, 24.0 * c->nfield);
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

@@ -318,6 +318,9 @@ void free_ctx(spm_ctx *c) {
                     c->f[1], c->f[2], c->work[0], c->work[1], c->work[2], c->work[3], c->work[4], c->uk_dc, c->pbuf,
                     c->pres, c->noise_dev})
     if (p) (void)hipFree(p);
+  for (double *p : {c->conc[0], c->conc[1], c->conc_rhs[0], c->conc_rhs[1], c->surface_normal[0],
+                    c->surface_normal[1], c->surface_normal[2], c->f_ns1[0], c->f_ns1[1], c->red})
+    if (p) (void)hipFree(p);
   for (int *p : {c->spec, c->stencil, c->pxint, c->tile_count, c->tile_start, c->tile_cursor, c->tile_list,
                  c->cell_count, c->cell_start, c->cell_cursor, c->cell_list, c->pcell, c->flag})
     if (p) (void)hipFree(p);
@@ -497,9 +500,243 @@ int setup(spm_ctx *c) {
   return 0;
 }
 
+// ---------------------------------------------------------------------
+// electrolyte (SW_EQ == Electrolyte), literal schedule, FP64
+// ---------------------------------------------------------------------
+// result slots of c->red (after the partial sums)
+enum { SLOT_SURF = 0, SLOT_SCALE0 = 1, SLOT_VOLUME = 3, SLOT_COUNT = 4 };
+
+// Make_phi_qq_particle (operate_electrolyte.cxx:179-242) into phi, surface.
+int make_phi_qq(spm_ctx *c, double *phi, double *surface) {
+  RC(p_make_phi_qq(c, phi, surface));
+  if (c->np == 0) return 0;
+  // sum |q| over the particles, in particle order (host copy of the species)
+  double abs_total = 0.;
+  for (int n = 0; n < c->np; ++n) abs_total += std::fabs(c->qtab.q[c->spec_host[n]]);
+  RC(e_reduce(c, 0, surface, nullptr, SLOT_SURF, 1, abs_total));
+  RC(e_scale_by(c, surface, (long)c->nfield, SLOT_SURF));
+  return 0;
+}
+
+// A_k2a_out (fft_wrapper.h:639-651).
+int k2a_out(spm_ctx *c, const double *a_k, double *a_x) {
+  SPM_HIP(c, hipMemcpyAsync(a_x, a_k, sizeof(double) * c->nfield, hipMemcpyDeviceToDevice, c->stream));
+  return fft_c2r(c, a_x);
+}
+
+// Conc_k2charge_field (operate_electrolyte.cxx:45-74).
+int conc_k2charge(spm_ctx *c, double *charge, double *phi_p, double *dmy) {
+  RC(make_phi_qq(c, phi_p, charge));
+  for (int n = 0; n < c->el.n_spec; ++n) {
+    RC(k2a_out(c, c->conc[n], dmy));
+    RC(e_charge_add(c, charge, dmy, phi_p, c->valency_e[n]));
+  }
+  return 0;
+}
+
+// Rhs_NS_Nernst_Planck + Rhs_NS_solute (fluid_solver.cxx:181-280) with the
+// working memory NSsolute_solver_Euler passes: u, solute_flux = up,
+// grad_potential = f, rhs_ns = f_ns1, rhs_solute = conc_rhs.
+int rhs_nernst_planck(spm_ctx *c) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  double *const fl[3] = {c->up[0], c->up[1], c->up[2]};
+  double *const gp[3] = {c->f[0], c->f[1], c->f[2]};
+  double *const p5[5] = {c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]};
+  // potential gradient in real space (:263-274)
+  RC(conc_k2charge(c, c->u[0], c->u[1], c->u[2]));
+  RC(fft_r2c(c, c->u[0]));
+  double cst[3];
+  {
+    double amp = (double)((long)c->grid.nx * c->grid.ny * c->grid.nz);
+    if (c->el.ac) amp = std::sin(c->el.angular_frequency * c->time);
+    for (int d = 0; d < 3; ++d) cst[d] = c->el.e_ext[d] * amp;
+  }
+  RC(e_poisson_grad(c, c->u[0], gp, 1, cst));
+  RC(fft_c2r_many(c, gp, 3));
+  // Rhs_NS_solute (:181-227)
+  RC(zeta_k2u(c, 0));
+  RC(p_surface_normal(c));  // tile lists are current from make_phi_qq
+  for (int n = 0; n < c->el.n_spec; ++n) {
+    RC(k_truncate(c, c->conc[n]));
+    RC(e_grad_scaled(c, c->conc[n], c->prm.kbt * c->el.onsager[n], fl));
+    RC(fft_c2r_many(c, fl, 3));
+    RC(k2a_out(c, c->conc[n], c->conc_rhs[n]));
+    RC(e_np_flux(c, fl, c->conc_rhs[n], gp, c->surface_normal, u3, c->el.onsager[n], c->valency_e[n]));
+    RC(fft_r2c_many(c, fl, 3));
+    RC(e_divergence(c, fl, c->conc_rhs[n]));
+  }
+  // Rhs_solvent: U2advection_k (operate_omega.cxx:14-100), u as scratch
+  RC(k_products(c, c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]));
+  RC(fft_r2c_many(c, p5, 5));
+  RC(k_advection_contract(c, c->u[0], c->u[1], c->u[2], c->work[0], c->work[1], c->f_ns1[0], c->f_ns1[1]));
+  return 0;
+}
+
+// NSsolute_solver_Euler (fluid_solver.cxx:306-326); uk_dc gains dt * 0.
+int nssolute_solver(spm_ctx *c) {
+  RC(rhs_nernst_planck(c));
+  const double nu = c->prm.eta * (1. / c->prm.rho);
+  return e_nssolute_update(c, nu, c->prm.dt);
+}
+
+// Rescale_solute (solute_rhs.cxx:97-126) with phi and up[0] as working memory.
+int rescale_solute(spm_ctx *c) {
+  RC(p_make_phi(c, 0));
+  for (int n = 0; n < c->el.n_spec; ++n) {
+    RC(k2a_out(c, c->conc[n], c->up[0]));
+    RC(e_reduce(c, 1, c->up[0], c->phi, SLOT_SCALE0 + n, 1, c->total_solute[n]));
+    RC(e_scale_by(c, c->conc[n], (long)c->nfield, SLOT_SCALE0 + n));
+  }
+  return 0;
+}
+
+// Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177), free
+// particles, into f; charge in up[0], potential in up[1].
+int coulomb_force(spm_ctx *c) {
+  double *const f3[3] = {c->f[0], c->f[1], c->f[2]};
+  RC(conc_k2charge(c, c->up[0], c->f[0], c->f[1]));
+  SPM_HIP(c, hipMemcpyAsync(c->up[1], c->up[0], sizeof(double) * c->nfield, hipMemcpyDeviceToDevice, c->stream));
+  RC(fft_r2c(c, c->up[1]));
+  const double zero[3] = {0.0, 0.0, 0.0};
+  RC(e_poisson_grad(c, c->up[1], f3, 0, zero));
+  RC(fft_c2r_many(c, f3, 3));
+  double ext[3];
+  for (int d = 0; d < 3; ++d) {
+    ext[d] = c->el.e_ext[d];
+    if (c->el.ac) ext[d] *= std::sin(c->el.angular_frequency * c->time);
+  }
+  return e_coulomb(c, c->up[0], f3, ext, c->el.external_field);
+}
+
+// Solenoidal_u (operate_omega.h:96-103) on u.
+int solenoidal_u(spm_ctx *c) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  RC(fft_r2c_many(c, u3, 3));
+  RC(k_solenoidal_trunc(c, c->u[0], c->u[1], c->u[2]));
+  RC(fft_c2r_many(c, u3, 3));
+  return 0;
+}
+
+// Time_evolution_hydro, SW_EQ == Electrolyte (sp_3d_ns.cxx:46-57,61-212).
+int electrolyte_step_impl(spm_ctx *c) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  double *const f3[3] = {c->f[0], c->f[1], c->f[2]};
+  RC(nssolute_solver(c));
+  RC(zeta_k2u(c, fix_mask(c)));
+  RC(p_position(c, c->ts == 0));
+  RC(rescale_solute(c));
+  // Calc_f_hydro on the f field (the potential gradient left by the
+  // solver) into f_hydro1 / torque_hydro1 (:97-106)
+  RC(p_calc_f_hydro_field(c, f3));
+  RC(p_copy_hydro1(c));
+  RC(coulomb_force(c));
+  RC(e_add_scaled(c, u3, f3, c->prm.dt * (1. / c->prm.rho)));
+  RC(solenoidal_u(c));
+  RC(p_calc_f_hydro(c));
+  RC(p_velocity(c, c->ts == 0));
+  // no thermostat in the electrolyte path (:183)
+  RC(p_make_phi(c, 1));
+  RC(make_f_sole_add(c));
+  RC(u2zeta_k(c));
+  return 0;
+}
+
+// Init_rho_ion (operate_electrolyte.cxx:695-815), uniform initial
+// distribution.  Reads the fluid volume and ion counts back to the host.
+int init_rho_ion(spm_ctx *c) {
+  const spm_electrolyte &E = c->el;
+  double surface_ion_number = 0.;
+  for (int s = 0; s < c->prm.n_species; ++s) {
+    if (E.surface_charge[s] == 0.) {
+      c->err = "Surface_charge must be nonzero for every particle species";
+      return SPM_ERR_INVALID;
+    }
+    surface_ion_number -= E.surface_charge[s] * c->spec_count[s];
+  }
+  RC(p_make_phi(c, 0));
+  RC(e_reduce(c, 2, nullptr, c->phi, SLOT_VOLUME, 0, 0.0));
+  double volume = 0.;  // sum (1 - phi) dx^3
+  RC(e_read_slot(c, SLOT_VOLUME, &volume));
+  double dens[2] = {0.0, 0.0};
+  if (E.n_spec == 1) {
+    if (c->prm.n_species == 1 && E.valency[0] * E.surface_charge[0] >= 0.) {
+      c->err = "Valency_counterion must have the opposite sign of Surface_charge";
+      return SPM_ERR_INVALID;
+    }
+    dens[0] = (surface_ion_number / E.valency[0]) / volume;
+  } else {
+    if (E.valency[0] <= 0. || E.valency[1] >= 0.) {
+      c->err = "Valency_positive_ion must be > 0 and Valency_negative_ion < 0";
+      return SPM_ERR_INVALID;
+    }
+    double pos = 0., neg = 0.;
+    for (int s = 0; s < c->prm.n_species; ++s) {
+      const double q = E.surface_charge[s], npart = (double)c->spec_count[s];
+      if (q < 0.)
+        pos += std::fabs(q * npart / E.valency[0]);
+      else if (q > 0.)
+        neg += std::fabs(q * npart / E.valency[1]);
+    }
+    const double rho_inf = c->prm.kbt * E.dielectric / ((E.elementary_charge * E.debye_length) *
+                                                        (E.elementary_charge * E.debye_length));
+    const double rho_pos = rho_inf / (E.valency[0] * (E.valency[0] - E.valency[1]));
+    const double rho_neg = -rho_inf / (E.valency[1] * (E.valency[0] - E.valency[1]));
+    dens[0] = pos / volume + rho_pos;
+    dens[1] = neg / volume + rho_neg;
+  }
+  for (int n = 0; n < E.n_spec; ++n) {
+    RC(e_fill_real(c, c->conc[n], dens[n]));
+    RC(fft_r2c(c, c->conc[n]));
+  }
+  // Count_solute_each (solute_rhs.h:143-156), phi and up[0] as working memory
+  RC(p_make_phi(c, 0));
+  for (int n = 0; n < E.n_spec; ++n) {
+    RC(k2a_out(c, c->conc[n], c->up[0]));
+    RC(e_reduce(c, 1, c->up[0], c->phi, SLOT_COUNT + n, 0, 0.0));
+    RC(e_read_slot(c, SLOT_COUNT + n, &c->total_solute[n]));
+  }
+  return 0;
+}
+
+int set_electrolyte(spm_ctx *c, const spm_electrolyte *el) {
+  if (!el || el->n_spec < 1 || el->n_spec > 2 || !(el->dielectric > 0.) || !(el->elementary_charge > 0.)) {
+    c->err = "invalid electrolyte parameters";
+    return SPM_ERR_INVALID;
+  }
+  if (el->n_spec == 2 && !(el->debye_length > 0.)) {
+    c->err = "Debye_length must be > 0 with salt";
+    return SPM_ERR_INVALID;
+  }
+  if (c->electro) {
+    c->err = "the context already runs the electrolyte equations";
+    return SPM_ERR_INVALID;
+  }
+  if (c->prm.precision != SPM_FP64) {
+    c->err = "the electrolyte path runs in FP64 only";
+    return SPM_ERR_UNSUPPORTED;
+  }
+  c->el = *el;
+  for (int n = 0; n < 2; ++n) {
+    RC(dalloc(c, &c->conc[n], c->nfield));
+    RC(dalloc(c, &c->conc_rhs[n], c->nfield));
+    RC(dalloc(c, &c->f_ns1[n], c->nfield));
+  }
+  for (int d = 0; d < 3; ++d) RC(dalloc(c, &c->surface_normal[d], c->nfield));
+  RC(dalloc(c, &c->red, (size_t)e_red_slots()));
+  for (int s = 0; s < SPM_MAX_SPECIES; ++s) c->qtab.q[s] = el->surface_charge[s] * el->elementary_charge;
+  for (int n = 0; n < 2; ++n) c->valency_e[n] = el->valency[n] * el->elementary_charge;
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  c->electro = true;
+  return 0;
+}
+
 thread_local std::string g_create_err;
 
 }  // namespace
+
+namespace spm {
+int electrolyte_step(spm_ctx *c) { return electrolyte_step_impl(c); }
+}  // namespace spm
 
 // =====================================================================
 // C ABI
@@ -631,6 +868,9 @@ int spm_put_particles(spm_ctx *c, const spm_particle *p, int n) {
     const double *src = reinterpret_cast<const double *>(reinterpret_cast<const char *>(&p[i]) + 8);
     for (int s = 0; s < P_NSLOTS; ++s) c->pstage[(size_t)s * n + i] = src[s];
   }
+  c->spec_host.assign(spec.begin(), spec.begin() + n);
+  for (int s = 0; s < SPM_MAX_SPECIES; ++s) c->spec_count[s] = 0;
+  for (int i = 0; i < n; ++i) c->spec_count[spec[i]]++;
   if (n == 0) return SPM_OK;
   SPM_HIP(c, hipSetDevice(c->device));
   SPM_HIP(c, hipMemcpyAsync(c->pbuf, c->pstage.data(), sizeof(double) * P_NSLOTS * n, hipMemcpyHostToDevice, c->stream));
@@ -735,7 +975,10 @@ static int calc_f_hydro_api(spm_ctx *c) {
 static int step_n(spm_ctx *c, int nsteps) {
   const bool fused = c->schedule == SPM_SCHEDULE_FUSED;
   for (int s = 0; s < nsteps; ++s) {
-    RC(fused ? fused_step(c) : time_evolution_hydro(c));
+    if (c->electro)
+      RC(electrolyte_step(c));
+    else
+      RC(fused ? fused_step(c) : time_evolution_hydro(c));
     c->time += c->prm.dt;
     c->ts++;
     // keep the event pool bounded when timing long runs
@@ -744,7 +987,7 @@ static int step_n(spm_ctx *c, int nsteps) {
       RC(fold_timing(c));
     }
   }
-  if (fused && nsteps > 0) RC(materialize(c));
+  if (fused && !c->electro && nsteps > 0) RC(materialize(c));
   return 0;
 }
 
@@ -819,6 +1062,52 @@ int spm_timing_read(spm_ctx *c, double ms[SPM_NUM_KERNEL_FAMILIES], double bytes
     c->fam_ms[f] = c->fam_bytes[f] = 0.0;
     c->fam_count[f] = 0;
   }
+  return SPM_OK;
+}
+
+
+// ---- electrolyte ----
+#define ELECTRO_ENTRY(body)                                  \
+  do {                                                       \
+    if (!c) return SPM_ERR_INVALID;                          \
+    if (!c->electro) {                                       \
+      c->err = "spm_set_electrolyte has not been called";    \
+      return SPM_ERR_INVALID;                                \
+    }                                                        \
+    ENTRY(body);                                             \
+  } while (0)
+
+int spm_set_electrolyte(spm_ctx *c, const spm_electrolyte *el) { ENTRY(set_electrolyte(c, el)); }
+int spm_init_rho_ion(spm_ctx *c) { ELECTRO_ENTRY(init_rho_ion(c)); }
+int spm_nssolute_solver_euler(spm_ctx *c) { ELECTRO_ENTRY(nssolute_solver(c)); }
+int spm_rescale_solute(spm_ctx *c) { ELECTRO_ENTRY(rescale_solute(c)); }
+int spm_make_coulomb_force(spm_ctx *c) { ELECTRO_ENTRY(coulomb_force(c)); }
+
+int spm_put_concentration(spm_ctx *c, int n, const double *host) {
+  if (!c || !host || !c->electro || n < 0 || n >= c->el.n_spec) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipMemcpyAsync(c->conc[n], host, sizeof(double) * c->nfield, hipMemcpyHostToDevice, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  return SPM_OK;
+}
+
+int spm_get_concentration(spm_ctx *c, int n, double *host) {
+  if (!c || !host || !c->electro || n < 0 || n >= c->el.n_spec) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  SPM_HIP(c, hipMemcpyAsync(host, c->conc[n], sizeof(double) * c->nfield, hipMemcpyDeviceToHost, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  return SPM_OK;
+}
+
+int spm_put_total_solute(spm_ctx *c, const double *total) {
+  if (!c || !total || !c->electro) return SPM_ERR_INVALID;
+  for (int n = 0; n < c->el.n_spec; ++n) c->total_solute[n] = total[n];
+  return SPM_OK;
+}
+
+int spm_get_total_solute(spm_ctx *c, double *total) {
+  if (!c || !total || !c->electro) return SPM_ERR_INVALID;
+  for (int n = 0; n < c->el.n_spec; ++n) total[n] = c->total_solute[n];
   return SPM_OK;
 }
 

@@ -102,6 +102,11 @@ struct PModel {
   int g_direction;
 };
 
+// surface charge (times the elementary charge) of each particle species
+struct QTable {
+  double q[SPM_MAX_SPECIES];
+};
+
 }  // namespace spm
 
 // ---------------------------------------------------------------------
@@ -167,6 +172,24 @@ struct spm_ctx {
 
   // host staging
   std::vector<double> pstage;
+
+  // particle species counts and sum |surface charge| over particles, in
+  // particle order (kept on the host at spm_put_particles)
+  int spec_count[SPM_MAX_SPECIES] = {};
+  std::vector<int> spec_host;
+
+  // electrolyte (SW_EQ == Electrolyte): ion spectra, their right-hand sides,
+  // surface normals, the solvent right-hand side, reduction scratch
+  bool electro = false;
+  spm_electrolyte el{};
+  double *conc[2] = {nullptr, nullptr};
+  double *conc_rhs[2] = {nullptr, nullptr};
+  double *surface_normal[3] = {nullptr, nullptr, nullptr};
+  double *f_ns1[2] = {nullptr, nullptr};
+  double *red = nullptr;  // partial sums, then result scalars
+  double total_solute[2] = {0.0, 0.0};
+  spm::QTable qtab{};
+  double valency_e[2] = {0.0, 0.0};
 
   // optional per-kernel-family timing (HIP events on `stream`)
   bool timing = false;
@@ -252,4 +275,25 @@ int p_build_tiles(spm_ctx *c);
 int p_make_phi(spm_ctx *c, int sw_up);
 int p_make_f(spm_ctx *c);
 int p_init_lists(spm_ctx *c);
+int p_calc_f_hydro_field(spm_ctx *c, double *const fld[3]);
+int electrolyte_step(spm_ctx *c);
+
+// electrolyte kernels (electrolyte.hip)
+int e_red_slots();
+int e_poisson_grad(spm_ctx *c, const double *pot, double *const grad[3], int trunc, const double cst[3]);
+int e_grad_scaled(spm_ctx *c, const double *a, double scale, double *const out[3]);
+int e_divergence(spm_ctx *c, double *const flux[3], double *div);
+int e_nssolute_update(spm_ctx *c, double nu, double dt);
+int e_charge_add(spm_ctx *c, double *charge, const double *conc, const double *phi, double ze);
+int e_np_flux(spm_ctx *c, double *const flux[3], const double *cx, double *const grad[3], double *const normal[3],
+              double *const u[3], double onsager, double ze);
+int e_coulomb(spm_ctx *c, const double *charge, double *const f[3], const double ext[3], int external);
+int e_add_scaled(spm_ctx *c, double *const u[3], double *const f[3], double dmy);
+int e_fill_real(spm_ctx *c, double *a, double v);
+int e_reduce(spm_ctx *c, int mode, const double *a, const double *phi, int slot, int ratio, double numer);
+int e_scale_by(spm_ctx *c, double *a, long n, int slot);
+int e_read_slot(spm_ctx *c, int slot, double *out);
+int p_copy_hydro1(spm_ctx *c);
+int p_make_phi_qq(spm_ctx *c, double *phi, double *surface);
+int p_surface_normal(spm_ctx *c);
 }  // namespace spm

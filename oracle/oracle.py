@@ -76,6 +76,19 @@ def lib():
         L.oracle_genrand_real3.restype = C.c_double
         L.oracle_gauss2.argtypes = [vp, dp]
         L.oracle_gauss2.restype = None
+        L.oracle_set_electrolyte.argtypes = [vp, C.POINTER(abi.SpmElectrolyte)]
+        for name in ["init_rho_ion", "nssolute_solver_euler", "rescale_solute", "make_coulomb_force"]:
+            f = getattr(L, "oracle_" + name)
+            f.argtypes = [vp]
+            f.restype = ip
+        for name in ["put_concentration", "get_concentration"]:
+            getattr(L, "oracle_" + name).argtypes = [vp, ip, dp]
+        for name in ["put_total_solute", "get_total_solute"]:
+            getattr(L, "oracle_" + name).argtypes = [vp, dp]
+        L.oracle_make_phi_qq.argtypes = [vp, dp, dp]
+        L.oracle_make_phi_qq.restype = None
+        L.oracle_make_surface_normal.argtypes = [vp, PP]
+        L.oracle_make_surface_normal.restype = None
         _lib = L
     return _lib
 
@@ -129,6 +142,46 @@ class Oracle:
     def Make_phi_particle(self): self._call("make_phi_particle")
     def Make_f_particle_dt_sole(self): self._call("make_f_particle_dt_sole")
     def U2zeta_k(self): self._call("u2zeta_k")
+
+    # electrolyte (SW_EQ == Electrolyte)
+    def set_electrolyte(self, el):
+        self.n_spec = el.n_spec
+        self._call("set_electrolyte", C.byref(el))
+
+    def Init_rho_ion(self): self._call("init_rho_ion")
+    def NSsolute_solver_Euler(self): self._call("nssolute_solver_euler")
+    def Rescale_solute(self): self._call("rescale_solute")
+    def Make_Coulomb_force_x_on_fluid(self): self._call("make_coulomb_force")
+
+    def get_concentration(self, n):
+        a = np.zeros(self.shape)
+        self._call("get_concentration", n, _dp(a))
+        return a
+
+    def put_concentration(self, n, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._call("put_concentration", n, _dp(a))
+
+    def get_total_solute(self):
+        t = np.zeros(2)
+        self._call("get_total_solute", _dp(t))
+        return t[:self.n_spec]
+
+    def put_total_solute(self, t):
+        tt = np.zeros(2)
+        tt[:len(t)] = t
+        self._call("put_total_solute", _dp(tt))
+
+    def make_phi_qq(self):
+        """(phi, surface charge density) of Make_phi_qq_particle."""
+        phi, surf = np.zeros(self.shape), np.zeros(self.shape)
+        lib().oracle_make_phi_qq(self._h, _dp(phi), _dp(surf))
+        return phi, surf
+
+    def make_surface_normal(self):
+        n, ptrs = self._vec([np.zeros(self.shape) for _ in range(3)])
+        lib().oracle_make_surface_normal(self._h, ptrs)
+        return n
 
     def step(self, nsteps=1):
         self._call("step", nsteps)

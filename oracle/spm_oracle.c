@@ -56,6 +56,13 @@ struct oracle_ctx {
   fft_plan px, py, pz;          /* pz has n = NZ/2 */
   double *rowbuf;
   double *ztw;                  /* cos, sin of 2*pi*k/NZ, k <= NZ/2 */
+  /* electrolyte (SW_EQ == Electrolyte) */
+  int electro;
+  spm_electrolyte E;
+  double *conc[2], *conc_rhs[2], *surface_normal[3], *f_ns1[2];
+  double total_solute[2];
+  double surface_charge_e[SPM_MAX_SPECIES];
+  double valency_e[2];
 };
 
 /* ------------------------------------------------------------------ */
@@ -569,7 +576,12 @@ int oracle_make_f_particle_dt_sole(oracle_ctx *c) {
 
 /* Calc_f_hydro_correct_precision, md_force.cxx:263-389 (DBG_MASS_GRID off,
  * no rigid bodies), preceded by Reset_phi_u as at sp_3d_ns.cxx:127. */
+static void calc_f_hydro_field(oracle_ctx *c, double *const fld[3]);
 int oracle_calc_f_hydro(oracle_ctx *c) {
+  calc_f_hydro_field(c, c->u);
+  return 0;
+}
+static void calc_f_hydro_field(oracle_ctx *c, double *const fld[3]) {
   reset_phi_u(c);
   const double dmy0 = -c->DX3 * c->RHO;
   const double dmy = dmy0 / c->dt;
@@ -587,7 +599,7 @@ int oracle_calc_f_hydro(oracle_ctx *c) {
       angular2v(c, pn->omega, r, v_rot);
       size_t im = IDX(c, r_mesh[0], r_mesh[1], r_mesh[2]);
       for (int d = 0; d < DIM; d++) {
-        dmy_fp[d] = ((pn->v[d] + v_rot[d]) - c->u[d][im]) * dmy_phi;
+        dmy_fp[d] = ((pn->v[d] + v_rot[d]) - fld[d][im]) * dmy_phi;
         force[d] += dmy_fp[d];
       }
       torque[0] += (r[1] * dmy_fp[2] - r[2] * dmy_fp[1]);
@@ -604,7 +616,6 @@ int oracle_calc_f_hydro(oracle_ctx *c) {
         pn->torque_slip[d] = 0.0;
       }
   }
-  return 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -952,10 +963,437 @@ int oracle_init_fluid(oracle_ctx *c) {
   return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* electrolyte: Nernst-Planck ions + Poisson (SW_EQ == Electrolyte)    */
+/* ------------------------------------------------------------------ */
+static const int (*ijk_blocks(const oracle_ctx *c, int rng[4][6]))[6] {
+  const int NX = c->NX, NY = c->NY, TX = c->TRN_X, TY = c->TRN_Y, TZ = c->TRN_Z;
+  const int r[4][6] = {{0, TX - 1, 0, TY - 1, 0, 2 * TZ - 1},
+                       {0, TX - 1, NY - TY + 1, NY - 1, 0, 2 * TZ - 1},
+                       {NX - TX + 1, NX - 1, 0, TY - 1, 0, 2 * TZ - 1},
+                       {NX - TX + 1, NX - 1, NY - TY + 1, NY - 1, 0, 2 * TZ - 1}};
+  memcpy(rng, r, sizeof(r));
+  return (const int (*)[6])rng;
+}
+
+/* DPhi_compact_sin, profile.h:168-181. */
+static double dphi_compact_sin(const oracle_ctx *c, double x) {
+  const double hxi = c->XI * .5, ixi = 1. / c->XI;
+  const double dmy_x = c->RADIUS - x;
+  return fabs(dmy_x) < hxi ? M_PI * .5 * ixi * cos(M_PI * dmy_x * ixi) : 0.;
+}
+
+/* Make_phi_qq_particle, operate_electrolyte.cxx:179-242: phi (no clamp)
+ * and the surface charge profile, normalised so that its volume integral
+ * of |.| is the total |surface charge|. */
+void oracle_make_phi_qq(oracle_ctx *c, double *phi, double *surface) {
+  double abs_total = 0.;
+  for (int n = 0; n < c->Np; n++) {
+    const spm_particle *pn = &c->p[n];
+    const double q = c->surface_charge_e[pn->spec];
+    int x_int[3], r_mesh[3];
+    double residue[3], r[3], x[3], x12[3];
+    particle_cell(c, pn->x, x_int, residue);
+    for (int mesh = 0; mesh < c->np_domain; mesh++) {
+      relative_coord(c, &c->sekibun[3 * mesh], x_int, residue, r_mesh, r);
+      for (int d = 0; d < DIM; d++) x[d] = r_mesh[d] * c->DX;
+      const double dmy = distance0(c, x, pn->x, x12);
+      const size_t im = IDX(c, r_mesh[0], r_mesh[1], r_mesh[2]);
+      phi[im] += Phi(c, dmy, c->RADIUS);
+      surface[im] += q * dphi_compact_sin(c, dmy);
+    }
+    abs_total += fabs(q);
+  }
+  double dmy = 0.0;
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) dmy += fabs(surface[IDX(c, i, j, k)]);
+  const double rescale = abs_total / (dmy * c->DX3);
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) surface[IDX(c, i, j, k)] *= rescale;
+}
+
+/* Make_surface_normal, make_phi.cxx:17-58. */
+void oracle_make_surface_normal(oracle_ctx *c, double *const normal[3]) {
+  for (int d = 0; d < DIM; d++) memset(normal[d], 0, sizeof(double) * c->nfield);
+  for (int n = 0; n < c->Np; n++) {
+    int x_int[3], r_mesh[3];
+    double residue[3], r[3];
+    particle_cell(c, c->p[n].x, x_int, residue);
+    for (int mesh = 0; mesh < c->np_domain; mesh++) {
+      relative_coord(c, &c->sekibun[3 * mesh], x_int, residue, r_mesh, r);
+      const double dmy_r = sqrt(SQ(r[0]) + SQ(r[1]) + SQ(r[2]));
+      if (fabs(dmy_r - c->RADIUS) < c->HXI) {
+        const double ir = 1. / dmy_r;
+        const size_t im = IDX(c, r_mesh[0], r_mesh[1], r_mesh[2]);
+        for (int d = 0; d < DIM; d++) normal[d][im] += r[d] * ir;
+      }
+    }
+  }
+}
+
+/* A_k2a_out, fft_wrapper.h:639-651. */
+static void a_k2a_out(oracle_ctx *c, const double *a_k, double *a_x) {
+  memcpy(a_x, a_k, sizeof(double) * c->nfield);
+  oracle_A_k2a(c, a_x);
+}
+
+/* Conc_k2charge_field, operate_electrolyte.cxx:45-74. */
+static void conc_k2charge_field(oracle_ctx *c, double *charge, double *phi_p, double *dmy_value) {
+  memset(phi_p, 0, sizeof(double) * c->nfield);
+  memset(charge, 0, sizeof(double) * c->nfield);
+  oracle_make_phi_qq(c, phi_p, charge);
+  for (int n = 0; n < c->E.n_spec; n++) {
+    a_k2a_out(c, c->conc[n], dmy_value);
+    for (int i = 0; i < c->NX; i++)
+      for (int j = 0; j < c->NY; j++)
+        for (int k = 0; k < c->NZ; k++) {
+          const size_t im = IDX(c, i, j, k);
+          charge[im] += c->valency_e[n] * dmy_value[im] * (1. - phi_p[im]);
+        }
+  }
+}
+
+/* Charge_field_k2Coulomb_potential_k_PBC, operate_electrolyte.cxx:76-88. */
+static void coulomb_potential_k(oracle_ctx *c, double *pot) {
+  const double ieps = 1. / c->E.dielectric;
+  for (size_t im = 0; im < c->nfield; im++) pot[im] *= (c->IK2[im] * ieps);
+}
+
+/* A_k2da_k, fft_wrapper.h:681-685 over A_k2dja_k_primitive,
+ * fft_wrapper.cxx:103-126: (re, im) -> (k im, -k re). */
+static void a_k2da_k(oracle_ctx *c, const double *a, double *const da[3]) {
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->HNZ_; k++) {
+        const size_t im = IDX(c, i, j, 2 * k);
+        const double ks[3] = {c->KX_int[im] * c->WAVE_X, c->KY_int[im] * c->WAVE_Y, c->KZ_int[im] * c->WAVE_Z};
+        for (int d = 0; d < DIM; d++) {
+          const double re = a[im], imv = a[im + 1];
+          da[d][im] = ks[d] * imv;
+          da[d][im + 1] = -ks[d] * re;
+        }
+      }
+}
+
+/* U_k2divergence_k, fft_wrapper.cxx:649-677. */
+static void u_k2divergence_k(oracle_ctx *c, double *const u[3], double *div) {
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->HNZ_; k++) {
+        const size_t im0 = IDX(c, i, j, 2 * k), im1 = im0 + 1;
+        const double ks0 = c->KX_int[im0] * c->WAVE_X, ks1 = c->KY_int[im0] * c->WAVE_Y,
+                     ks2 = c->KZ_int[im0] * c->WAVE_Z;
+        const double d0 = ks0 * u[0][im1] + ks1 * u[1][im1] + ks2 * u[2][im1];
+        const double d1 = -ks0 * u[0][im0] - ks1 * u[1][im0] - ks2 * u[2][im0];
+        div[im0] = d0;
+        div[im1] = d1;
+      }
+}
+
+/* the applied field's amplitude: E_ext, times sin(w t) for AC
+ * (operate_electrolyte.cxx:116-123) */
+static void external_field_now(const oracle_ctx *c, double ext[3]) {
+  for (int d = 0; d < DIM; d++) {
+    ext[d] = c->E.e_ext[d];
+    if (c->E.ac) ext[d] *= sin(c->E.angular_frequency * c->time);
+  }
+}
+
+/* Count_single_solute, solute_rhs.cxx:128-147. */
+static double count_single_solute(const oracle_ctx *c, const double *conc_x, const double *phi_p) {
+  double dmy = 0.;
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) {
+        const size_t im = IDX(c, i, j, k);
+        dmy += (1. - phi_p[im]) * conc_x[im];
+      }
+  return dmy * c->DX3;
+}
+
+/* Rhs_NS_Nernst_Planck + Rhs_NS_solute, fluid_solver.cxx:181-280 and
+ * solute_rhs.cxx:42-235.  Working memory as NSsolute_solver_Euler passes
+ * it (:306-326): u, solute_flux = up, grad_potential = f (f_particle),
+ * rhs_ns = f_ns1, rhs_solute = Concentration_rhs0. */
+static void rhs_ns_nernst_planck(oracle_ctx *c) {
+  double *const *u = c->u, *const *flux = c->up, *const *grad = c->f;
+  /* potential gradient in real space (fluid_solver.cxx:263-274) */
+  conc_k2charge_field(c, u[0], u[1], u[2]);
+  oracle_A2a_k(c, u[0]);
+  coulomb_potential_k(c, u[0]);
+  oracle_truncate(c, u[0]);
+  a_k2da_k(c, u[0], grad);
+  { /* Add_constant_field_k, fluid_solver.cxx:228-241 */
+    double amp = (double)(c->NX * c->NY * c->NZ);
+    if (c->E.ac) amp = sin(c->E.angular_frequency * c->time);
+    for (int d = 0; d < DIM; d++) grad[d][0] -= c->E.e_ext[d] * amp;
+  }
+  u_k2u(c, grad);
+  /* Rhs_NS_solute, fluid_solver.cxx:181-227 */
+  for (int d = 0; d < DIM - 1; d++) oracle_truncate(c, c->zeta[d]);
+  zeta_k2u(c);
+  oracle_make_surface_normal(c, c->surface_normal);
+  for (int n = 0; n < c->E.n_spec; n++) {
+    oracle_truncate(c, c->conc[n]);
+    { /* Diffusion_flux_single, solute_rhs.cxx:216-235 (rhs_ns[0] as working memory) */
+      double *dmy = c->f_ns1[0];
+      for (size_t im = 0; im < c->nfield; im++) dmy[im] = c->kBT * c->E.onsager[n] * c->conc[n][im];
+      a_k2da_k(c, dmy, flux);
+      u_k2u(c, flux);
+    }
+    double *cx = c->conc_rhs[n];
+    a_k2a_out(c, c->conc[n], cx);
+    /* Solute_solver_rhs_nonlinear_x_single, solute_rhs.cxx:149-187 */
+    for (int i = 0; i < c->NX; i++)
+      for (int j = 0; j < c->NY; j++)
+        for (int k = 0; k < c->NZ; k++) {
+          const size_t im = IDX(c, i, j, k);
+          for (int d = 0; d < DIM; d++) {
+            const double dmy_interaction = grad[d][im] * c->valency_e[n];
+            flux[d][im] += cx[im] * (c->E.onsager[n] * dmy_interaction);
+          }
+        }
+    /* Solute_impermeability, solute_rhs.cxx:42-95 */
+    for (int p = 0; p < c->Np; p++) {
+      int x_int[3], r_mesh[3];
+      double residue[3], r[3];
+      particle_cell(c, c->p[p].x, x_int, residue);
+      for (int mesh = 0; mesh < c->np_domain; mesh++) {
+        relative_coord(c, &c->sekibun[3 * mesh], x_int, residue, r_mesh, r);
+        const size_t im = IDX(c, r_mesh[0], r_mesh[1], r_mesh[2]);
+        double norm2 = 0.;
+        for (int d = 0; d < DIM; d++) norm2 += SQ(c->surface_normal[d][im]);
+        if (norm2 > 0.) {
+          double dmy = 0.;
+          for (int d = 0; d < DIM; d++) dmy += flux[d][im] * c->surface_normal[d][im];
+          dmy /= norm2;
+          for (int d = 0; d < DIM; d++) flux[d][im] -= c->surface_normal[d][im] * dmy;
+        }
+      }
+    }
+    /* Add_advection_flux, solute_rhs.cxx:188-207 */
+    for (int i = 0; i < c->NX; i++)
+      for (int j = 0; j < c->NY; j++)
+        for (int k = 0; k < c->NZ; k++) {
+          const size_t im = IDX(c, i, j, k);
+          for (int d = 0; d < DIM; d++) flux[d][im] += cx[im] * -u[d][im];
+        }
+    u2u_k(c, flux);
+    u_k2divergence_k(c, flux, cx);
+  }
+  /* Rhs_solvent (fluid_solver.cxx:168-179) and Add_zeta_viscous_term
+   * (operate_omega.cxx:172-184) */
+  oracle_u2advection_k(c, u, c->f_ns1);
+  int rng_buf[4][6];
+  const int (*rng)[6] = ijk_blocks(c, rng_buf);
+  for (int b = 0; b < 4; b++)
+    for (int i = rng[b][0]; i <= rng[b][1]; i++)
+      for (int j = rng[b][2]; j <= rng[b][3]; j++)
+        for (int k = rng[b][4]; k <= rng[b][5]; k++) {
+          const size_t im = IDX(c, i, j, k);
+          c->f_ns1[0][im] += -(c->NU * c->K2[im] * c->zeta[0][im]);
+          c->f_ns1[1][im] += -(c->NU * c->K2[im] * c->zeta[1][im]);
+        }
+}
+
+/* NSsolute_solver_Euler, fluid_solver.cxx:306-326, with Field_solver_Euler
+ * (:122-135) over the ijk_range blocks. */
+int oracle_nssolute_solver_euler(oracle_ctx *c) {
+  if (!c->electro) return -1;
+  rhs_ns_nernst_planck(c);
+  int rng_buf[4][6];
+  const int (*rng)[6] = ijk_blocks(c, rng_buf);
+  for (int b = 0; b < 4; b++)
+    for (int i = rng[b][0]; i <= rng[b][1]; i++)
+      for (int j = rng[b][2]; j <= rng[b][3]; j++)
+        for (int k = rng[b][4]; k <= rng[b][5]; k++) {
+          const size_t im = IDX(c, i, j, k);
+          for (int d = 0; d < DIM - 1; d++) c->zeta[d][im] += (c->dt * c->f_ns1[d][im]);
+          for (int n = 0; n < c->E.n_spec; n++) c->conc[n][im] += (c->dt * c->conc_rhs[n][im]);
+        }
+  /* uk_dc += dt * rhs_uk_dc with rhs_uk_dc = 0 (Rhs_solvent) */
+  return 0;
+}
+
+/* Rescale_solute, solute_rhs.cxx:97-126 (phi and up[0] as working memory,
+ * as Time_evolution_hydro passes them, sp_3d_ns.cxx:90-93). */
+int oracle_rescale_solute(oracle_ctx *c) {
+  if (!c->electro) return -1;
+  oracle_make_phi_particle(c);
+  for (int n = 0; n < c->E.n_spec; n++) {
+    a_k2a_out(c, c->conc[n], c->up[0]);
+    const double dmy = count_single_solute(c, c->up[0], c->phi);
+    const double factor = c->total_solute[n] / dmy;
+    for (size_t im = 0; im < c->nfield; im++) c->conc[n][im] *= factor;
+  }
+  return 0;
+}
+
+/* Make_Coulomb_force_x_on_fluid, operate_electrolyte.cxx:90-177 (free
+ * particles): force = rho_e (-grad Phi + E_ext), into f. */
+int oracle_make_coulomb_force(oracle_ctx *c) {
+  if (!c->electro) return -1;
+  double *charge = c->up[0], *pot = c->up[1];
+  conc_k2charge_field(c, charge, c->f[0], c->f[1]);
+  memcpy(pot, charge, sizeof(double) * c->nfield); /* A2a_k_out */
+  oracle_A2a_k(c, pot);
+  coulomb_potential_k(c, pot);
+  a_k2da_k(c, pot, c->f);
+  u_k2u(c, c->f);
+  double ext[3];
+  external_field_now(c, ext);
+  for (int d = 0; d < DIM; d++)
+    for (int i = 0; i < c->NX; i++)
+      for (int j = 0; j < c->NY; j++)
+        for (int k = 0; k < c->NZ; k++) {
+          const size_t im = IDX(c, i, j, k);
+          double electric_field = -c->f[d][im];
+          if (c->E.external_field) electric_field += ext[d];
+          c->f[d][im] = charge[im] * electric_field;
+        }
+  return 0;
+}
+
+int oracle_set_electrolyte(oracle_ctx *c, const spm_electrolyte *el) {
+  if (!el || c->electro || el->n_spec < 1 || el->n_spec > 2 || !(el->dielectric > 0)) return -1;
+  c->E = *el;
+  int fail = 0;
+  for (int n = 0; n < 2; n++) {
+    c->conc[n] = (double *)calloc(c->nfield, sizeof(double));
+    c->conc_rhs[n] = (double *)calloc(c->nfield, sizeof(double));
+    c->f_ns1[n] = (double *)calloc(c->nfield, sizeof(double));
+    fail |= !c->conc[n] || !c->conc_rhs[n] || !c->f_ns1[n];
+  }
+  for (int d = 0; d < DIM; d++) {
+    c->surface_normal[d] = (double *)calloc(c->nfield, sizeof(double));
+    fail |= !c->surface_normal[d];
+  }
+  if (fail) return -2;
+  for (int s = 0; s < SPM_MAX_SPECIES; s++) c->surface_charge_e[s] = el->surface_charge[s] * el->elementary_charge;
+  for (int n = 0; n < el->n_spec; n++) c->valency_e[n] = el->valency[n] * el->elementary_charge;
+  c->electro = 1;
+  return 0;
+}
+
+/* Init_rho_ion, operate_electrolyte.cxx:695-815, uniform initial
+ * distribution (Poisson_Boltzmann off): Set_uniform_ion_charge_density_
+ * nosalt (:370-403) or _salt (:522-575), then Count_solute_each
+ * (solute_rhs.h:143-156). */
+int oracle_init_rho_ion(oracle_ctx *c) {
+  if (!c->electro) return -1;
+  const spm_electrolyte *E = &c->E;
+  int count[SPM_MAX_SPECIES] = {0};
+  for (int n = 0; n < c->Np; n++) count[c->p[n].spec]++;
+  double surface_ion_number = 0.;
+  for (int s = 0; s < c->P.n_species; s++) {
+    if (E->surface_charge[s] == 0.) return -3;
+    surface_ion_number -= E->surface_charge[s] * count[s];
+  }
+  oracle_make_phi_particle(c);
+  double volume_phi = 0.;
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) volume_phi += (1. - c->phi[IDX(c, i, j, k)]);
+  double dens[2];
+  if (E->n_spec == 1) {
+    if (c->P.n_species == 1 && E->valency[0] * E->surface_charge[0] >= 0.) return -3;
+    const double counterion_number = surface_ion_number / E->valency[0];
+    dens[0] = counterion_number / (volume_phi * c->DX3);
+  } else {
+    if (E->valency[0] <= 0. || E->valency[1] >= 0.) return -3;
+    double pos = 0., neg = 0.;
+    for (int s = 0; s < c->P.n_species; s++) {
+      const double q = E->surface_charge[s], np = (double)count[s];
+      if (q < 0.)
+        pos += fabs(q * np / E->valency[0]);
+      else if (q > 0.)
+        neg += fabs(q * np / E->valency[1]);
+    }
+    const double rho_inf = c->kBT * E->dielectric / SQ(E->elementary_charge * E->debye_length);
+    const double rho_pos = rho_inf / (E->valency[0] * (E->valency[0] - E->valency[1]));
+    const double rho_neg = -rho_inf / (E->valency[1] * (E->valency[0] - E->valency[1]));
+    dens[0] = pos / (volume_phi * c->DX3) + rho_pos;
+    dens[1] = neg / (volume_phi * c->DX3) + rho_neg;
+  }
+  for (int n = 0; n < E->n_spec; n++) {
+    for (int i = 0; i < c->NX; i++)
+      for (int j = 0; j < c->NY; j++)
+        for (int k = 0; k < c->NZ; k++) c->conc[n][IDX(c, i, j, k)] = dens[n];
+    c->total_solute[n] = count_single_solute(c, c->conc[n], c->phi);
+    oracle_A2a_k(c, c->conc[n]);
+  }
+  /* Count_solute_each (phi and up[0] as working memory) */
+  oracle_make_phi_particle(c);
+  for (int n = 0; n < E->n_spec; n++) {
+    a_k2a_out(c, c->conc[n], c->up[0]);
+    c->total_solute[n] = count_single_solute(c, c->up[0], c->phi);
+  }
+  return 0;
+}
+
+int oracle_put_concentration(oracle_ctx *c, int n, const double *host) {
+  if (!c->electro || n < 0 || n >= c->E.n_spec) return -1;
+  memcpy(c->conc[n], host, sizeof(double) * c->nfield);
+  return 0;
+}
+int oracle_get_concentration(oracle_ctx *c, int n, double *host) {
+  if (!c->electro || n < 0 || n >= c->E.n_spec) return -1;
+  memcpy(host, c->conc[n], sizeof(double) * c->nfield);
+  return 0;
+}
+int oracle_put_total_solute(oracle_ctx *c, const double *t) {
+  if (!c->electro) return -1;
+  for (int n = 0; n < c->E.n_spec; n++) c->total_solute[n] = t[n];
+  return 0;
+}
+int oracle_get_total_solute(oracle_ctx *c, double *t) {
+  if (!c->electro) return -1;
+  for (int n = 0; n < c->E.n_spec; n++) t[n] = c->total_solute[n];
+  return 0;
+}
+
+/* Time_evolution_hydro with SW_EQ == Electrolyte (sp_3d_ns.cxx:46-57,
+ * 61-212): no thermostat (:183); the first Calc_f_hydro reads the f field
+ * (the potential gradient left by the solver) into f_hydro1 (:97-106). */
+static void electrolyte_step(oracle_ctx *c) {
+  oracle_nssolute_solver_euler(c);
+  oracle_zeta_k2u(c);
+  oracle_md_solver_position(c);
+  oracle_rescale_solute(c);
+  calc_f_hydro_field(c, c->f);
+  for (int n = 0; n < c->Np; n++)
+    for (int d = 0; d < DIM; d++) {
+      c->p[n].f_hydro1[d] = c->p[n].f_hydro[d];
+      c->p[n].torque_hydro1[d] = c->p[n].torque_hydro[d];
+    }
+  oracle_make_coulomb_force(c);
+  const double dmy = c->dt * c->IRHO;
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) {
+        const size_t im = IDX(c, i, j, k);
+        for (int d = 0; d < DIM; d++) c->u[d][im] += (c->f[d][im] * dmy);
+      }
+  solenoidal_u(c, c->u);
+  oracle_calc_f_hydro(c);
+  oracle_md_solver_velocity(c);
+  oracle_make_phi_u_particle(c);
+  oracle_make_f_particle_dt_sole(c);
+  oracle_u2zeta_k(c);
+}
+
 /* Time_evolution_hydro (sp_3d_ns.cxx:61-212) for Navier_Stokes with free
  * spherical particles, then jikan.time += dt and ts++ (:593-594). */
 int oracle_step(oracle_ctx *c, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
+    if (c->electro) {
+      electrolyte_step(c);
+      c->time += c->dt;
+      c->ts++;
+      continue;
+    }
     oracle_ns_solver_slaved_euler(c);
     oracle_zeta_k2u(c);
     oracle_md_solver_position(c);
@@ -1090,6 +1528,12 @@ void oracle_destroy(oracle_ctx *c) {
     free(c->f[d]);
   }
   free(c->phi);
+  for (int n = 0; n < 2; n++) {
+    free(c->conc[n]);
+    free(c->conc_rhs[n]);
+    free(c->f_ns1[n]);
+  }
+  for (int d = 0; d < 3; d++) free(c->surface_normal[d]);
   free(c->rowbuf);
   free(c->ztw);
   plan_free(&c->px);

@@ -73,6 +73,19 @@ int oracle_make_f_particle_dt_sole(oracle_ctx *c);
 int oracle_u2zeta_k(oracle_ctx *c);
 int oracle_step(oracle_ctx *c, int nsteps);
 
+/* Electrolyte (SW_EQ == Electrolyte), mirroring spm_set_electrolyte .. */
+int oracle_set_electrolyte(oracle_ctx *c, const spm_electrolyte *el);
+int oracle_init_rho_ion(oracle_ctx *c);
+int oracle_put_concentration(oracle_ctx *c, int species, const double *host);
+int oracle_get_concentration(oracle_ctx *c, int species, double *host);
+int oracle_put_total_solute(oracle_ctx *c, const double *total);
+int oracle_get_total_solute(oracle_ctx *c, double *total);
+int oracle_nssolute_solver_euler(oracle_ctx *c);
+int oracle_rescale_solute(oracle_ctx *c);
+int oracle_make_coulomb_force(oracle_ctx *c);
+void oracle_make_phi_qq(oracle_ctx *c, double *phi, double *surface);
+void oracle_make_surface_normal(oracle_ctx *c, double *const normal[3]);
+
 /* Building blocks, on caller buffers of NX*NY*NZ_ doubles. */
 void oracle_A2a_k(oracle_ctx *c, double *a);
 void oracle_A_k2a(oracle_ctx *c, double *a);
