@@ -158,6 +158,39 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp,
 }
 
 // ---------------------------------------------------------------------
+// The inverse reads the kz = 0 and kz = NZ/2 planes as Ooura's rdft3d does
+// (rdft3dsort(-1), fftsg3d.c:645-668): of each pair (k, -k) in a plane one
+// member is kept and the other taken as its conjugate, and the four
+// self-conjugate points keep their real part.  Kept on kz = 0: 1 <= j < NY/2,
+// or j in {0, NY/2} with i <= NX/2; on kz = NZ/2: j > NY/2, or j in
+// {0, NY/2} with i > NX/2.  For a Hermitian spectrum this changes nothing;
+// for one that is not (the Nyquist modes of a derivative) it makes the
+// result the reference's.  A kept point is never written, so in place is
+// safe.  One thread per (i, j) and plane.
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void ooura_planes_kernel(FieldPtrs<T> fp, int nx, int ny, int nz_, int m) {
+  using C2 = cplx<T>;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2L * nx * ny) return;
+  const int plane = (int)(t / ((long)nx * ny));
+  const int ij = (int)(t % ((long)nx * ny));
+  const int i = ij / ny, j = ij % ny, n1h = nx / 2, n2h = ny / 2;
+  const int kz = plane == 0 ? 0 : m;
+  const bool edge = (j == 0 || j == n2h);
+  const bool selfc = edge && (i == 0 || i == n1h);
+  const bool keep = plane == 0 ? ((j >= 1 && j < n2h) || (edge && i <= n1h)) : ((j > n2h) || (edge && i > n1h) || selfc);
+  T *a = fp.p[blockIdx.y];
+  C2 *x = reinterpret_cast<C2 *>(a + ((long)i * ny + j) * nz_ + 2 * kz);
+  if (selfc) {
+    x->y = (T)0;
+  } else if (!keep) {
+    const C2 mv = *reinterpret_cast<const C2 *>(a + ((long)((nx - i) % nx) * ny + (ny - j) % ny) * nz_ + 2 * kz);
+    *x = C2{mv.x, -mv.y};
+  }
+}
+
+// ---------------------------------------------------------------------
 // host side: configuration per size and launch
 // ---------------------------------------------------------------------
 namespace {
@@ -265,6 +298,10 @@ static int fft_many_t(spm_ctx *c, double *const *a, int n, bool fwd) {
       tend(c, SPM_K_FFT_X, bytes);
     } else {
       const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+      tbeg(c);
+      hipLaunchKernelGGL((ooura_planes_kernel<T>), dim3((unsigned)((2L * g.nx * g.ny + 255) / 256), nf), dim3(256), 0,
+                         c->stream, fp, g.nx, g.ny, g.nz_, g.nz / 2);
+      tend(c, SPM_K_SPECTRAL, 2.0 * nf * 2.0 * g.nx * g.ny * 2 * sizeof(T));
       tbeg(c);
       rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1, c->stream);
       tend(c, SPM_K_FFT_X, bytes);

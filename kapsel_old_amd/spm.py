@@ -27,6 +27,9 @@ SYMBOLS = [
     "spm_calc_f_hydro", "spm_force", "spm_md_solver_velocity", "spm_add_random_force_thermostat",
     "spm_make_phi_u_particle", "spm_make_f_particle_dt_sole", "spm_u2zeta_k", "spm_step",
     "spm_synchronize", "spm_check_finite", "spm_set_schedule", "spm_timing_enable", "spm_timing_read", "spm_fused_stage",
+    "spm_set_electrolyte", "spm_init_rho_ion", "spm_put_concentration", "spm_get_concentration",
+    "spm_put_total_solute", "spm_get_total_solute", "spm_nssolute_solver_euler", "spm_rescale_solute",
+    "spm_make_coulomb_force",
 ]
 
 SCHEDULE_FUSED, SCHEDULE_LITERAL = 0, 1
@@ -81,6 +84,14 @@ def lib():
         L.spm_fused_stage.argtypes = [vp, ip]
         L.spm_timing_enable.argtypes = [vp, ip]
         L.spm_timing_read.argtypes = [vp, dp, dp, C.POINTER(C.c_long)]
+        L.spm_set_electrolyte.argtypes = [vp, C.POINTER(abi.SpmElectrolyte)]
+        for name in ["spm_init_rho_ion", "spm_nssolute_solver_euler", "spm_rescale_solute",
+                     "spm_make_coulomb_force"]:
+            getattr(L, name).argtypes = [vp]
+        L.spm_put_concentration.argtypes = [vp, ip, dp]
+        L.spm_get_concentration.argtypes = [vp, ip, dp]
+        L.spm_put_total_solute.argtypes = [vp, dp]
+        L.spm_get_total_solute.argtypes = [vp, dp]
         for name in SYMBOLS:
             f = getattr(L, name)
             if f.restype is C.c_int or f.restype is None:
@@ -147,6 +158,35 @@ class SPM:
     def set_schedule(self, schedule):
         """SCHEDULE_FUSED (default) or SCHEDULE_LITERAL for spm_step."""
         self._call("set_schedule", schedule)
+
+    # ---- electrolyte (SW_EQ == Electrolyte) ----
+    def set_electrolyte(self, el):
+        self.n_spec = el.n_spec
+        self._call("set_electrolyte", C.byref(el))
+
+    def Init_rho_ion(self): self._call("init_rho_ion")
+    def NSsolute_solver_Euler(self): self._call("nssolute_solver_euler")
+    def Rescale_solute(self): self._call("rescale_solute")
+    def Make_Coulomb_force_x_on_fluid(self): self._call("make_coulomb_force")
+
+    def get_concentration(self, n):
+        a = np.zeros(self.shape)
+        self._call("get_concentration", n, _dp(a))
+        return a
+
+    def put_concentration(self, n, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._call("put_concentration", n, _dp(a))
+
+    def get_total_solute(self):
+        t = np.zeros(2)
+        self._call("get_total_solute", _dp(t))
+        return t[:self.n_spec]
+
+    def put_total_solute(self, t):
+        tt = np.zeros(2)
+        tt[:len(t)] = t
+        self._call("put_total_solute", _dp(tt))
 
     def fused_stage(self, stage):
         """Test hook: one stage of the fused schedule (see include/spm.h)."""

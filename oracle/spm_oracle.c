@@ -173,8 +173,42 @@ void oracle_A2a_k(oracle_ctx *c, double *a) {
 }
 
 /* A_k2a, fft_wrapper.h:534-631: half spectrum -> real field, scale 1/N. */
+/* The kz = 0 and kz = NZ/2 planes of a half spectrum are Hermitian in
+ * (kx, ky) for a real field.  Ooura's inverse reads only one member of each
+ * pair (k, -k) of those planes (rdft3dsort(-1), fftsg3d.c:645-668) and takes
+ * the other as its conjugate; the four self-conjugate points keep only their
+ * real part.  Kept on kz = 0: 1 <= j < NY/2, or j in {0, NY/2} with
+ * i <= NX/2; on kz = NZ/2: j > NY/2, or j in {0, NY/2} with i > NX/2.  This
+ * makes a spectrum that is not Hermitian (a derivative's Nyquist modes)
+ * transform as the reference transforms it. */
+static void ooura_hermitian_planes(const oracle_ctx *c, double *a) {
+  const int NX = c->NX, NY = c->NY, n1h = NX / 2, n2h = NY / 2, M = c->NZ / 2;
+  for (int plane = 0; plane < 2; plane++) {
+    const int kz = plane == 0 ? 0 : M;
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++) {
+        const int edge = (j == 0 || j == n2h);
+        const int selfc = edge && (i == 0 || i == n1h);
+        int keep;
+        if (plane == 0)
+          keep = (j >= 1 && j < n2h) || (edge && i <= n1h);
+        else
+          keep = (j > n2h) || (edge && i > n1h) || selfc;
+        double *x = a + IDX(c, i, j, 2 * kz);
+        if (selfc) {
+          x[1] = 0.0;
+        } else if (!keep) {
+          const double *m = a + IDX(c, (NX - i) % NX, (NY - j) % NY, 2 * kz);
+          x[0] = m[0];
+          x[1] = -m[1];
+        }
+      }
+  }
+}
+
 void oracle_A_k2a(oracle_ctx *c, double *a) {
   const int NX = c->NX, NY = c->NY, NZ = c->NZ, M = NZ / 2, H = c->HNZ_;
+  ooura_hermitian_planes(c, a);
   const double scale = 1.0 / ((double)NX * NY * NZ);
   double *z = c->rowbuf;
   for (int j = 0; j < NY; j++) fft_batch(&c->px, a + IDX(c, 0, j, 0), (long)NY * H, H, -1);

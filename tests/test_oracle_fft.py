@@ -91,3 +91,22 @@ def test_single_mode_sign_convention():
     k = o.A2a_k(a)
     # sum sin(2 pi 3 z/n) exp(+2 pi i 3 z/n) = i n/2 per (x, y) line
     assert abs(k[0, 0, 7] - n ** 3 / 2) < 1e-9 and abs(k[0, 0, 6]) < 1e-9
+
+
+@pytest.mark.skipif(ref_lib() is None, reason="oracle/_ref (reference Ooura build) absent")
+@pytest.mark.parametrize("grid", GRIDS)
+def test_inverse_of_a_non_hermitian_spectrum_matches_reference_ooura(grid):
+    """A derivative's Nyquist modes make a spectrum that is not Hermitian on
+    the kz = 0 and kz = NZ/2 planes; the reference's inverse reads only one
+    member of each (k, -k) pair there.  The oracle must read it the same
+    way, and a plain conjugate-symmetric average (numpy's irfftn) must not
+    pass."""
+    nx, ny, nz = grid
+    o = Oracle(abi.make_params(nx, ny, nz))
+    rng = np.random.default_rng(17)
+    X = rng.standard_normal(o.shape)
+    want = ref_A_k2a(X)[:, :, :nz]
+    got = o.A_k2a(X)[:, :, :nz]
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    avg = np.fft.irfftn(X[:, :, 0::2] - 1j * X[:, :, 1::2], s=(nx, ny, nz), axes=(0, 1, 2))
+    assert np.abs(avg - want).max() > 1e-3 * np.abs(want).max()
