@@ -20,6 +20,8 @@
 // Inside a block the transform is a radix-8 Stockham FFT: each thread
 // keeps 8 points in registers per stage, stages exchange through LDS laid
 // out [point][column] so a wavefront touches consecutive LDS words.
+#include <climits>
+
 #include "fft_core.h"
 
 namespace spm {
@@ -28,23 +30,35 @@ namespace spm {
 // y / x passes: complex lines of N points at element stride `estride`
 // (complex units).  A block transforms C columns; LineTiles says which.
 // blockIdx.x = tile (XCD-remapped), blockIdx.z = field.
+//
+// The fused schedule only carries modes the 2/3 rule keeps (|kx| < TRN_X,
+// |ky| < TRN_Y, kz < TRN_Z).  For its y passes: columns whose index within
+// the x plane (their kz) is >= col_cut are left alone; of each line, the
+// points j with min(j, N - j) >= row_keep (|ky| >= TRN_Y) are taken as zero
+// on the way in (inverse) or not stored (forward).  col_cut = INT_MAX and
+// row_keep = N give the plain transform.
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C, int SIGN>
 __global__ __launch_bounds__(C *N / R) void fft_lines_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw_g,
-                                                             LineTiles lt, long estride) {
+                                                             LineTiles lt, long estride, int col_cut, int row_keep) {
   using C2 = cplx<T>;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (line_first_col(lt, tile, C) >= col_cut) return;  // uniform over the block
   // twiddle tables in LDS: the transform's barriers then wait on LDS
   // only, and global loads issued ahead stay in flight across them
   __shared__ double2 tw[N];
   for (int t = tid; t < N; t += blockDim.x) tw[t] = tw_g[t];
-  const long col = line_column(lt, xcd_tile(blockIdx.x, gridDim.x), tid % C, C);
-  const bool col_ok = col >= 0;  // the thread's column in every stage
+  const long col = line_column(lt, tile, tid % C, C);
+  const bool col_ok = col >= 0 && line_first_col(lt, tile, C) + (tile < lt.main ? tid % C : 0) < col_cut;
   C2 *g = reinterpret_cast<C2 *>(fp.p[blockIdx.z]) + (col_ok ? col : 0);
-  auto ld = [&](int row, int, int) -> C2 { return col_ok ? g[row * estride] : C2{0, 0}; };
+  auto row_kept = [&](int row) { return (row < N - row ? row : N - row) < row_keep; };
+  auto ld = [&](int row, int, int) -> C2 {
+    return (col_ok && (SIGN > 0 || row_kept(row))) ? g[row * estride] : C2{0, 0};
+  };
   auto st = [&](int row, int, int, C2 v) {
-    if (col_ok) g[row * estride] = v;
+    if (col_ok && (SIGN < 0 || row_kept(row))) g[row * estride] = v;
   };
   fft_stages_io<T, N, R, C, C, SIGN, 1, true, true>(lds, tw, tid, ld, st);
 }
@@ -55,7 +69,7 @@ __global__ __launch_bounds__(C *N / R) void fft_lines_kernel(FieldPtrs<T> fp, co
 template <typename T, int M, int R, int ROWS>
 __global__ __launch_bounds__(ROWS *M / R) void fft_z_r2c_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw_g,
                                                                const double2 *__restrict__ twr_g, int nrows,
-                                                               int nz_) {
+                                                               int nz_, int kz_cut) {
   using C2 = cplx<T>;
   constexpr int NT = ROWS * M / R;
   constexpr int CP = ROWS + 1;
@@ -88,7 +102,7 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_r2c_kernel(FieldPtrs<T> fp,
     const C2 o{(T)0.5 * (z.y - zc.y), (T)-0.5 * (z.x - zc.x)};  // (z - zc) / (2i)
     const double2 w = twr[k];
     const C2 x = cadd(e, cmul(o, C2{(T)w.x, (T)w.y}));
-    if (row < nrows) *reinterpret_cast<C2 *>(a + row * nz_ + 2 * k) = x;
+    if (row < nrows && k < kz_cut) *reinterpret_cast<C2 *>(a + row * nz_ + 2 * k) = x;
   }
 }
 
@@ -98,7 +112,7 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_r2c_kernel(FieldPtrs<T> fp,
 template <typename T, int M, int R, int ROWS>
 __global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw_g,
                                                                const double2 *__restrict__ twr_g, int nrows, int nz_,
-                                                               T scale) {
+                                                               T scale, int kz_cut) {
   using C2 = cplx<T>;
   constexpr int NT = ROWS * M / R;
   constexpr int CP = ROWS + 1;
@@ -117,7 +131,7 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp,
   for (int f = tid; f < ROWS * H; f += NT) {
     const int k = f % H, r = f / H;
     const long row = row0 + r;
-    lds[k * CP + r] = (row < nrows) ? *reinterpret_cast<const C2 *>(a + row * nz_ + 2 * k) : C2{0, 0};
+    lds[k * CP + r] = (row < nrows && k < kz_cut) ? *reinterpret_cast<const C2 *>(a + row * nz_ + 2 * k) : C2{0, 0};
   }
   __syncthreads();
   // Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) exp(-2 pi i k/NZ)
@@ -198,54 +212,56 @@ namespace {
 
 template <typename T, int N, int SIGN>
 void launch_lines_n(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
-                    int nouter, hipStream_t st) {
+                    int nouter, hipStream_t st, int col_cut, int row_keep) {
   constexpr int R = kRadix(N), C = kLinesC(N);
   const LineTiles lt = make_line_tiles(ncols, nouter, ostride, C);
   hipLaunchKernelGGL((fft_lines_kernel<T, N, R, C, SIGN>), dim3(lt.tiles, 1, nf), dim3(C * N / R), 0, st, fp, tw, lt,
-                     estride);
+                     estride, col_cut, row_keep);
 }
 
 // one switch arm per supported line length
 template <typename T, int SIGN>
 int launch_lines(int n, const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
-                 int nouter, hipStream_t st) {
+                 int nouter, hipStream_t st, int col_cut = INT_MAX, int row_keep = -1) {
+  if (row_keep < 0) row_keep = n;
   switch (n) {
-    case 8: launch_lines_n<T, 8, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 16: launch_lines_n<T, 16, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 32: launch_lines_n<T, 32, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 64: launch_lines_n<T, 64, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 128: launch_lines_n<T, 128, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 256: launch_lines_n<T, 256, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 512: launch_lines_n<T, 512, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 1024: launch_lines_n<T, 1024, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 8: launch_lines_n<T, 8, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
+    case 16: launch_lines_n<T, 16, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
+    case 32: launch_lines_n<T, 32, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
+    case 64: launch_lines_n<T, 64, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
+    case 128: launch_lines_n<T, 128, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
+    case 256: launch_lines_n<T, 256, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
+    case 512: launch_lines_n<T, 512, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
+    case 1024: launch_lines_n<T, 1024, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
   }
   return -1;
 }
 
 template <typename T, int M>
 void launch_z_n(bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr, int nrows, int nz_,
-                T scale, hipStream_t st) {
+                T scale, hipStream_t st, int kz_cut) {
   constexpr int R = kZRadix<T>(M), ROWS = kRows<T>(M);
   dim3 grid((nrows + ROWS - 1) / ROWS, 1, nf);
   if (fwd)
-    hipLaunchKernelGGL((fft_z_r2c_kernel<T, M, R, ROWS>), grid, dim3(ROWS * M / R), 0, st, fp, tw, twr, nrows, nz_);
+    hipLaunchKernelGGL((fft_z_r2c_kernel<T, M, R, ROWS>), grid, dim3(ROWS * M / R), 0, st, fp, tw, twr, nrows, nz_,
+                       kz_cut);
   else
     hipLaunchKernelGGL((fft_z_c2r_kernel<T, M, R, ROWS>), grid, dim3(ROWS * M / R), 0, st, fp, tw, twr, nrows, nz_,
-                       scale);
+                       scale, kz_cut);
 }
 
 template <typename T>
 int launch_z(int m, bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr, int nrows,
-             int nz_, T scale, hipStream_t st) {
+             int nz_, T scale, hipStream_t st, int kz_cut = INT_MAX) {
   switch (m) {
-    case 4: launch_z_n<T, 4>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 8: launch_z_n<T, 8>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 16: launch_z_n<T, 16>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 32: launch_z_n<T, 32>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 64: launch_z_n<T, 64>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 128: launch_z_n<T, 128>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 256: launch_z_n<T, 256>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 512: launch_z_n<T, 512>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 4: launch_z_n<T, 4>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
+    case 8: launch_z_n<T, 8>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
+    case 16: launch_z_n<T, 16>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
+    case 32: launch_z_n<T, 32>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
+    case 64: launch_z_n<T, 64>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
+    case 128: launch_z_n<T, 128>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
+    case 256: launch_z_n<T, 256>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
+    case 512: launch_z_n<T, 512>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
   }
   return -1;
 }
@@ -336,7 +352,9 @@ namespace spm {
 
 // Single passes, used by the fused schedule (fused.hip supplies the x and
 // z passes that carry pointwise work).  kind: 0 = y pass, 1 = z forward,
-// 2 = z inverse.
+// 2 = z inverse.  They carry only the modes the 2/3 rule keeps: kz <
+// TRN_Z columns and entries, and in the y direction |ky| < TRN_Y (taken as
+// zero on the way into an inverse, not stored by a forward pass).
 template <typename T>
 static int fft_pass_t(spm_ctx *c, double *const *a, int n, int kind, int sign) {
   const Grid &g = c->grid;
@@ -345,20 +363,24 @@ static int fft_pass_t(spm_ctx *c, double *const *a, int n, int kind, int sign) {
     const int nf = (n - b) < 5 ? (n - b) : 5;
     for (int q = 0; q < nf; ++q) fp.p[q] = reinterpret_cast<T *>(a[b + q]);
     const long h = g.hnz;
-    const double bytes = 2.0 * nf * (double)c->nfield * sizeof(T);
+    // bytes moved with the band limits: field fractions kept in kz and ky
+    const double fz = (double)g.trn_z / g.hnz, fy = (double)(2 * g.trn_y - 1) / g.ny;
+    const double fb = nf * (double)c->nfield * sizeof(T);
     int rc;
     tbeg(c);
     if (kind == 0) {
-      rc = sign > 0 ? launch_lines<T, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream)
-                    : launch_lines<T, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
-      tend(c, SPM_K_FFT_Y, bytes);
+      rc = sign > 0 ? launch_lines<T, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream, g.trn_z,
+                                          g.trn_y)
+                    : launch_lines<T, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream, g.trn_z,
+                                          g.trn_y);
+      tend(c, SPM_K_FFT_Y, fb * fz * (1.0 + fy));
     } else if (kind == 1) {
-      rc = launch_z<T>(g.nz / 2, true, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, (T)1, c->stream);
-      tend(c, SPM_K_FFT_Z_R2C, bytes);
+      rc = launch_z<T>(g.nz / 2, true, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, (T)1, c->stream, g.trn_z);
+      tend(c, SPM_K_FFT_Z_R2C, fb * (1.0 + fz));
     } else {
       const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
-      rc = launch_z<T>(g.nz / 2, false, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, scale, c->stream);
-      tend(c, SPM_K_FFT_Z_C2R, bytes);
+      rc = launch_z<T>(g.nz / 2, false, fp, nf, c->tw_z, c->tw_rz, g.nx * g.ny, g.nz_, scale, c->stream, g.trn_z);
+      tend(c, SPM_K_FFT_Z_C2R, fb * (1.0 + fz));
     }
     if (rc) {
       c->err = "unsupported FFT size";

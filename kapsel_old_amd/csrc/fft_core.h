@@ -281,6 +281,11 @@ inline LineTiles make_line_tiles(int ncols, int nouter, long ostride, int C) {
   return t;
 }
 
+// index within its outer index (the kz of a y pass) of a tile's first column
+__device__ __forceinline__ int line_first_col(const LineTiles &t, int tile, int C) {
+  return tile < t.main ? (tile % t.per_outer) * C : t.ncols;
+}
+
 // element offset of column c of a tile, or -1 where the tile has no column c
 __device__ __forceinline__ long line_column(const LineTiles &t, int tile, int c, int C) {
   if (tile < t.main) {

@@ -51,6 +51,25 @@ __device__ __forceinline__ ModeK mode_k(const Grid &g, int i, int j, int kz) {
   return m;
 }
 
+// Column (j, kz) of an x pass holds kept modes (2/3 rule in ky and kz).
+__device__ __forceinline__ bool col_kept(const Grid &g, int j, int kz) {
+  return (j < g.ny - j ? j : g.ny - j) < g.trn_y && kz < g.trn_z;
+}
+
+// Whether any of the C flat columns [c0, c0 + C) is kept; the same answer
+// in every thread, so a block may return on it before its first barrier.
+template <int C>
+__device__ __forceinline__ bool tile_kept(const Grid &g, int c0) {
+  const int H = g.hnz, ncol = g.ny * H;
+  bool any = false;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int col = c0 + c;
+    if (col < ncol) any = any || col_kept(g, col / H, col - (col / H) * H);
+  }
+  return any;
+}
+
 // Zeta_k2u_k (fft_wrapper.cxx:408-511) on one mode; component d.
 template <typename T>
 __device__ __forceinline__ cplx<T> zeta2u_mode(const Grid &g, const ModeK &m, cplx<T> d1, cplx<T> d2, int d) {
@@ -107,9 +126,11 @@ __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__r
   // columns (j, kz) numbered as one run j*(NZ/2+1) + kz: a tile starts on
   // a 128-byte boundary
   const int c0 = fused_x_tile<T>() * C;
+  // columns outside the 2/3 box are zero and the y pass never reads them
+  if (!tile_kept<C>(g, c0)) return;
   const int col = c0 + c;
-  const bool col_ok = col < es;
   const int j = col / (int)H, kz = col - j * (int)H;
+  const bool col_ok = col < es && col_kept(g, j, kz);
   // zeta at the thread's first-stage points i = j0 + q N/R (the 2/3 mask
   // applied on load)
   C2 za[R], zb[R];
@@ -160,14 +181,16 @@ struct ZRows {
 };
 
 template <typename T, int M, int R, int ROWS>
-__device__ __forceinline__ void z_rows_fetch(const T *a, long row0, int nrows, int nz_, int tid, cplx<T> *dst) {
+__device__ __forceinline__ void z_rows_fetch(const T *a, long row0, int nrows, int nz_, int kz_cut, int tid,
+                                             cplx<T> *dst) {
   using Z = ZRows<M, R, ROWS>;
 #pragma unroll
   for (int t = 0; t < Z::KF; ++t) {
     const int f = tid + t * Z::NT, k = f % Z::H, r = f / Z::H;
     const long row = row0 + r;
-    dst[t] = (f < ROWS * Z::H && row < nrows) ? *reinterpret_cast<const cplx<T> *>(a + row * nz_ + 2 * k)
-                                              : cplx<T>{0, 0};
+    dst[t] = (f < ROWS * Z::H && row < nrows && k < kz_cut)
+                 ? *reinterpret_cast<const cplx<T> *>(a + row * nz_ + 2 * k)
+                 : cplx<T>{0, 0};
   }
 }
 
@@ -209,7 +232,7 @@ __device__ __forceinline__ void z_c2r_regs_to_lds(cplx<T> *lds, const cplx<T> *s
 
 // writes the forward z transform of the LDS rows (packed real pairs) to `a`
 template <typename T, int M, int R, int ROWS>
-__device__ __forceinline__ void z_r2c_lds_to_row(cplx<T> *lds, T *a, long row0, int nrows, int nz_,
+__device__ __forceinline__ void z_r2c_lds_to_row(cplx<T> *lds, T *a, long row0, int nrows, int nz_, int kz_cut,
                                                  const double2 *__restrict__ tw, const double2 *__restrict__ twr,
                                                  int tid) {
   using C2 = cplx<T>;
@@ -225,7 +248,7 @@ __device__ __forceinline__ void z_r2c_lds_to_row(cplx<T> *lds, T *a, long row0, 
     const C2 o{(T)0.5 * (z.y - zc.y), (T)-0.5 * (z.x - zc.x)};
     const C2 w = C2{(T)twr[k].x, (T)twr[k].y};
     const C2 x = cadd(e, cmul(o, w));
-    if (row < nrows) *reinterpret_cast<C2 *>(a + row * nz_ + 2 * k) = x;
+    if (row < nrows && k < kz_cut) *reinterpret_cast<C2 *>(a + row * nz_ + 2 * k) = x;
   }
   __syncthreads();
 }
@@ -250,10 +273,13 @@ __global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, T *u0, T 
   // component d+1 is fetched while component d is transformed
   constexpr int KF = ZRows<M, R, ROWS>::KF;
   C2 cur[KF], nxt[KF];
-  z_rows_fetch<T, M, R, ROWS>(u0, row0, nrows, g.nz_, tid, cur);
+  // the inputs hold kept modes only below kz = TRN_Z, and only those
+  // outputs are read on
+  const int kz_cut = g.trn_z;
+  z_rows_fetch<T, M, R, ROWS>(u0, row0, nrows, g.nz_, kz_cut, tid, cur);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (d < 2) z_rows_fetch<T, M, R, ROWS>(d == 0 ? u1 : u2, row0, nrows, g.nz_, tid, nxt);
+    if (d < 2) z_rows_fetch<T, M, R, ROWS>(d == 0 ? u1 : u2, row0, nrows, g.nz_, kz_cut, tid, nxt);
     z_c2r_regs_to_lds<T, M, R, ROWS>(lds, cur, twr, tid);
     if (d < 2)
 #pragma unroll
@@ -284,7 +310,7 @@ __global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, T *u0, T 
       lds[m * CP + r] = pr;
     }
     __syncthreads();
-    z_r2c_lds_to_row<T, M, R, ROWS>(lds, out, row0, nrows, g.nz_, tw, twr, tid);
+    z_r2c_lds_to_row<T, M, R, ROWS>(lds, out, row0, nrows, g.nz_, kz_cut, tw, twr, tid);
   }
 }
 
@@ -308,9 +334,12 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
   const long H = g.hnz;
   const long es = (long)g.ny * H;
   const int c0 = fused_x_tile<T>() * C;
+  // a column outside the 2/3 box gives only discarded modes, and zeta is
+  // zero there already
+  if (!tile_kept<C>(g, c0)) return;
   const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
-  const bool col_ok = col < es;
   const int j = col / (int)H, kz = col - j * (int)H;
+  const bool col_ok = col < es && col_kept(g, j, kz);
   // The contraction (operate_omega.cxx:60-97) is linear in the five product
   // spectra P0..P4 = u2u3, u3u1, u1u2, u2^2-u3^2, u3^2-u1^2: accumulate the
   // two components Omega_k2zeta_k keeps as each product is transformed.
@@ -372,18 +401,16 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
     const int i = rows[q];
     const ModeK m = mode_k(g, i, j, kz);
     const long e = 2 * (col + i * es);
-    C2 za{0, 0}, zb{0, 0};
-    if (m.keep) {
-      const T kk2 = (T)(sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz));
-      const T ik2 = kk2 > (T)0 ? (T)1 / kk2 : (T)0;
-      const T dmy = exp(dmy0 * kk2) - (T)1;
-      za = *reinterpret_cast<const C2 *>(z0 + e);
-      zb = *reinterpret_cast<const C2 *>(z1 + e);
-      za.x += dmy * (za.x - dmy1 * ik2 * acc0[q].x);
-      za.y += dmy * (za.y - dmy1 * ik2 * acc0[q].y);
-      zb.x += dmy * (zb.x - dmy1 * ik2 * acc1[q].x);
-      zb.y += dmy * (zb.y - dmy1 * ik2 * acc1[q].y);
-    }
+    if (!m.keep) continue;  // zeta stays zero outside the 2/3 box
+    const T kk2 = (T)(sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz));
+    const T ik2 = kk2 > (T)0 ? (T)1 / kk2 : (T)0;
+    const T dmy = exp(dmy0 * kk2) - (T)1;
+    C2 za = *reinterpret_cast<const C2 *>(z0 + e);
+    C2 zb = *reinterpret_cast<const C2 *>(z1 + e);
+    za.x += dmy * (za.x - dmy1 * ik2 * acc0[q].x);
+    za.y += dmy * (za.y - dmy1 * ik2 * acc0[q].y);
+    zb.x += dmy * (zb.x - dmy1 * ik2 * acc1[q].x);
+    zb.y += dmy * (zb.y - dmy1 * ik2 * acc1[q].y);
     *reinterpret_cast<C2 *>(z0 + e) = za;
     *reinterpret_cast<C2 *>(z1 + e) = zb;
   }
@@ -411,9 +438,12 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0
   const long H = g.hnz;
   const long es = (long)g.ny * H;
   const int c0 = fused_x_tile<T>() * C;
+  // a column outside the 2/3 box gives only discarded modes, and zeta is
+  // zero there already
+  if (!tile_kept<C>(g, c0)) return;
   const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
-  const bool col_ok = col < es;
   const int j = col / (int)H, kz = col - j * (int)H;
+  const bool col_ok = col < es && col_kept(g, j, kz);
   C2 acc0[R], acc1[R];
   int rows[R];
   double f00[3] = {0.0, 0.0, 0.0};
@@ -469,15 +499,11 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0
     const int i = rows[q];
     const ModeK m = mode_k(g, i, j, kz);
     const long e = 2 * (col + i * es);
-    C2 za{0, 0}, zb{0, 0};
-    if (m.keep) {
-      za = cadd(*reinterpret_cast<const C2 *>(z0 + e), acc0[q]);
-      zb = cadd(*reinterpret_cast<const C2 *>(z1 + e), acc1[q]);
-    }
-    *reinterpret_cast<C2 *>(z0 + e) = za;
-    *reinterpret_cast<C2 *>(z1 + e) = zb;
     if (has0 && i == 0 && c == 0)
       for (int d = 0; d < 3; ++d) uk_dc[d] += f00[d];
+    if (!m.keep) continue;  // zeta stays zero outside the 2/3 box
+    *reinterpret_cast<C2 *>(z0 + e) = cadd(*reinterpret_cast<const C2 *>(z0 + e), acc0[q]);
+    *reinterpret_cast<C2 *>(z1 + e) = cadd(*reinterpret_cast<const C2 *>(z1 + e), acc1[q]);
   }
 }
 
@@ -568,12 +594,26 @@ int fx_project_t(spm_ctx *c) {
 #define SPM_PREC_CALL(c, fn, ...) \
   ((c)->prm.precision == SPM_FP32 ? fn<float>(c, ##__VA_ARGS__) : fn<double>(c, ##__VA_ARGS__))
 
+// Fractions of a field the band-limited passes touch: kept (ky, kz)
+// columns, kept kx within them, kept kz entries of a row.
+struct Band {
+  double col, kx, kz;
+};
+inline Band band(const spm_ctx *c) {
+  const Grid &g = c->grid;
+  return Band{(double)(2 * g.trn_y - 1) / g.ny * g.trn_z / g.hnz, (double)(2 * g.trn_x - 1) / g.nx,
+              (double)g.trn_z / g.hnz};
+}
+
 }  // namespace
 
+// Algorithmic bytes below count what each pass must move: kept columns and
+// modes only.
 int fx_zeta2u(spm_ctx *c, int fix_mask) {
   tbeg(c);
   const int rc = SPM_PREC_CALL(c, fx_zeta2u_t, fix_mask);
-  tend(c, SPM_K_FFT_X, 5.0 * field_bytes(c));
+  const Band b = band(c);
+  tend(c, SPM_K_FFT_X, (2.0 * b.col * b.kx + 3.0 * b.col) * field_bytes(c));
   if (rc) return rc;
   SPM_LAUNCH_CHECK(c);
   return 0;
@@ -582,7 +622,7 @@ int fx_zeta2u(spm_ctx *c, int fix_mask) {
 int fz_advect(spm_ctx *c) {
   tbeg(c);
   const int rc = SPM_PREC_CALL(c, fz_advect_t);
-  tend(c, SPM_K_FFT_Z_C2R, 8.0 * field_bytes(c));
+  tend(c, SPM_K_FFT_Z_C2R, 8.0 * band(c).kz * field_bytes(c));
   if (rc) return rc;
   SPM_LAUNCH_CHECK(c);
   return 0;
@@ -591,7 +631,8 @@ int fz_advect(spm_ctx *c) {
 int fx_update(spm_ctx *c, double nu, double dt) {
   tbeg(c);
   const int rc = SPM_PREC_CALL(c, fx_update_t, nu, dt);
-  tend(c, SPM_K_FFT_X, 9.0 * field_bytes(c));
+  const Band b = band(c);
+  tend(c, SPM_K_FFT_X, (5.0 * b.col + 4.0 * b.col * b.kx) * field_bytes(c));
   if (rc) return rc;
   SPM_LAUNCH_CHECK(c);
   return 0;
@@ -600,7 +641,8 @@ int fx_update(spm_ctx *c, double nu, double dt) {
 int fx_project(spm_ctx *c) {
   tbeg(c);
   const int rc = SPM_PREC_CALL(c, fx_project_t);
-  tend(c, SPM_K_FFT_X, 7.0 * field_bytes(c));
+  const Band b = band(c);
+  tend(c, SPM_K_FFT_X, (3.0 * b.col + 4.0 * b.col * b.kx) * field_bytes(c));
   if (rc) return rc;
   SPM_LAUNCH_CHECK(c);
   return 0;

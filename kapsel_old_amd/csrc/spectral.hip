@@ -156,7 +156,7 @@ __global__ void products_kernel(Grid g, T *u0, T *u1, T *u2, T *a0, T *a1) {
 // 80 B read, 32 B written per mode.
 template <typename T>
 __global__ void advection_contract_kernel(Grid g, const T *__restrict__ u0, const T *__restrict__ u1,
-                                          const T *__restrict__ u2, T *a0, T *a1) {
+                                          const T *__restrict__ u2, const T *a0, const T *a1, T *o0, T *o1) {
   using C2 = cplx<T>;
   Mode m;
   if (!mode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m)) return;
@@ -185,8 +185,9 @@ __global__ void advection_contract_kernel(Grid g, const T *__restrict__ u0, cons
   } else {
     s0 = 0; s1 = 1;
   }
-  *reinterpret_cast<C2 *>(a0 + m.im) = C2{om[s0][0], om[s0][1]};
-  *reinterpret_cast<C2 *>(a1 + m.im) = C2{om[s1][0], om[s1][1]};
+  // o0, o1 may alias a0, a1: each thread reads its mode before writing it
+  *reinterpret_cast<C2 *>(o0 + m.im) = C2{om[s0][0], om[s0][1]};
+  *reinterpret_cast<C2 *>(o1 + m.im) = C2{om[s1][0], om[s1][1]};
 }
 
 // Slaved-Euler update of NS_solver_slavedEuler (fluid_solver.cxx:337-353)
@@ -361,13 +362,12 @@ int k_products(spm_ctx *c, double *u0, double *u1, double *u2, double *a0, doubl
 
 int k_advection_contract(spm_ctx *c, double *u0, double *u1, double *u2, const double *a0, const double *a1,
                          double *o0, double *o1) {
-  // the contraction writes its zeta-form result over the two difference spectra
-  (void)o0;
-  (void)o1;
+  // the zeta-form result goes to o0, o1 (which may be a0, a1)
   tbeg(c);
   SPM_BY_PRECISION(c, hipLaunchKernelGGL(advection_contract_kernel<T>, dim3(nblk(nmodes(c->grid))), dim3(kBlock), 0,
                                          c->stream, c->grid, F<T>(u0), F<T>(u1), F<T>(u2),
-                                         const_cast<T *>(F<T>(a0)), const_cast<T *>(F<T>(a1))));
+                                         reinterpret_cast<const T *>(a0), reinterpret_cast<const T *>(a1), F<T>(o0),
+                                         F<T>(o1)));
   tend(c, SPM_K_SPECTRAL, 14.0 * esz(c) * (double)nmodes(c->grid));
   SPM_LAUNCH_CHECK(c);
   return 0;

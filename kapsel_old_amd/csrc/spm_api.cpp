@@ -284,6 +284,10 @@ int fused_stage(spm_ctx *c, int stage) {
   double *const p5[5] = {c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]};
   double *const f3[3] = {c->f[0], c->f[1], c->f[2]};
   const double nu = c->prm.eta * (1. / c->prm.rho);
+  if (stage == 1 || stage == 3) {  // as at the start of a fused step_n call
+    RC(k_truncate(c, c->zeta[0]));
+    RC(k_truncate(c, c->zeta[1]));
+  }
   switch (stage) {
     case 1:
       RC(fx_zeta2u(c, 0));
@@ -974,6 +978,13 @@ static int calc_f_hydro_api(spm_ctx *c) {
 
 static int step_n(spm_ctx *c, int nsteps) {
   const bool fused = c->schedule == SPM_SCHEDULE_FUSED;
+  // The fused passes carry only the modes the 2/3 rule keeps and leave zeta
+  // untouched outside them; the reference's first act in a step is the same
+  // truncation (Zeta_k2advection_k, operate_omega.cxx:102-108).
+  if (fused && !c->electro && nsteps > 0) {
+    RC(k_truncate(c, c->zeta[0]));
+    RC(k_truncate(c, c->zeta[1]));
+  }
   for (int s = 0; s < nsteps; ++s) {
     if (c->electro)
       RC(electrolyte_step(c));
