@@ -35,6 +35,30 @@ __device__ __forceinline__ double phi_of(const PModel &pm, double x) {
   return dmy / (dmy + hfun(pm.dx, x - pm.radius + pm.hxi));
 }
 
+// The same Phi with one exponential: H1 / (H1 + H2) = 1 / (1 + H2 / H1),
+// H2 / H1 = exp(dx^2/x1^2 - dx^2/x2^2) with x1 = a + xi/2 - d,
+// x2 = d - a + xi/2; H1 = 0 (x1 <= 0) gives 0 and H2 = 0 gives 1 as in the
+// two-exponential form.  Equal to phi_of up to rounding; A is the
+// arithmetic type (FP64, or FP32 for the FP32 field path).
+template <typename A>
+__device__ __forceinline__ A phi_fast(const PModel &pm, A d) {
+  const A x1 = (A)(pm.radius + pm.hxi) - d, x2 = d - (A)(pm.radius - pm.hxi);
+  if (!(x1 > (A)0)) return (A)0;
+  if (!(x2 > (A)0)) return (A)1;
+  const A dx2 = (A)(pm.dx * pm.dx);
+  return (A)1 / ((A)1 + exp(dx2 / (x1 * x1) - dx2 / (x2 * x2)));
+}
+
+// accumulation type of make_phi: FP64, or FP32 for FP32 fields
+template <typename T>
+struct PhiAcc {
+  using type = double;
+};
+template <>
+struct PhiAcc<float> {
+  using type = float;
+};
+
 // Distance0 (interaction.h:110-123) from grid point x1 to particle x2.
 // x12/L is taken as x12 * (1/L): the nearest-image count differs only when
 // x12/L is within one rounding of a half-integer (a pair half a box apart).
@@ -220,8 +244,10 @@ __global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const 
   const double rcut = pm.radius + pm.hxi;
   const bool sw_up = mode != 1;
   // n and the particle's members are the same for every thread of the
-  // block, so the compiler reads them with scalar loads
-  double ph = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  // block, so the compiler reads them with scalar loads.  FP32 fields take
+  // phi and its sums in FP32 (the distance itself in FP64).
+  using A = typename PhiAcc<T>::type;
+  A ph = 0, a0 = 0, a1 = 0, a2 = 0;
   for (int q = 0; q < cnt; ++q) {  // ascending particle index, as the reference adds
     const int n = tile_list[s0 + q];
     int cell[3];
@@ -235,34 +261,39 @@ __global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const 
       in = in && (o <= pm.nh && o >= -pm.nh);
     }
     if (!in) continue;
-    const double xp[3] = {P(pb, pm.np, P_X, n), P(pb, pm.np, P_X + 1, n), P(pb, pm.np, P_X + 2, n)};
-    double x12[3];
-    const double dist = distance0(g, x, xp, x12);
+    double r[3];  // Relative_coord: offset * dx - residue
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r[d] = (double)cell[d] * pm.dx - pres[3 * n + d];
+    double dist;
+    if constexpr (sizeof(T) == 8) {  // Distance0 as the reference takes it
+      const double xp[3] = {P(pb, pm.np, P_X, n), P(pb, pm.np, P_X + 1, n), P(pb, pm.np, P_X + 2, n)};
+      double x12[3];
+      dist = distance0(g, x, xp, x12);
+    } else {
+      dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    }
     if (!(dist < rcut)) continue;
-    const double dphi = phi_of(pm, dist);
+    const A dphi = phi_fast<A>(pm, (A)dist);
     ph += dphi;
     if (sw_up) {
       double vr[3] = {0.0, 0.0, 0.0};
       if (pm.rotation) {
-        double r[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) r[d] = (double)cell[d] * pm.dx - pres[3 * n + d];
         const double w0 = P(pb, pm.np, P_OMEGA, n), w1 = P(pb, pm.np, P_OMEGA + 1, n), w2 = P(pb, pm.np, P_OMEGA + 2, n);
         vr[0] = w1 * r[2] - w2 * r[1];
         vr[1] = w2 * r[0] - w0 * r[2];
         vr[2] = w0 * r[1] - w1 * r[0];
       }
-      a0 += (P(pb, pm.np, P_V, n) + vr[0]) * dphi;
-      a1 += (P(pb, pm.np, P_V + 1, n) + vr[1]) * dphi;
-      a2 += (P(pb, pm.np, P_V + 2, n) + vr[2]) * dphi;
+      a0 += (A)(P(pb, pm.np, P_V, n) + vr[0]) * dphi;
+      a1 += (A)(P(pb, pm.np, P_V + 1, n) + vr[1]) * dphi;
+      a2 += (A)(P(pb, pm.np, P_V + 2, n) + vr[2]) * dphi;
     }
   }
-  if (sw_up && ph > 1.) {  // make_phi.cxx:280-297
-    const double inv = 1. / ph;
+  if (sw_up && ph > (A)1) {  // make_phi.cxx:280-297
+    const A inv = (A)1 / ph;
     a0 *= inv;
     a1 *= inv;
     a2 *= inv;
-    ph = 1.;
+    ph = (A)1;
   }
   const long im = ((long)gi[0] * g.ny + gi[1]) * g.nz_ + gi[2];
   const bool pad = gi[2] == g.nz - 1;  // also clear the two padding slots
@@ -270,7 +301,7 @@ __global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const 
     // f_particle.h:55-57 with this point's phi and up; where phi = 0 (and so
     // up = 0) f = 0 - 0 u = 0 exactly and u is not read
     const T p = (T)ph;
-    const bool touch = ph != 0.0;
+    const bool touch = ph != (A)0;
     up0[im] = touch ? (T)a0 - p * u0[im] : (T)0;
     up1[im] = touch ? (T)a1 - p * u1[im] : (T)0;
     up2[im] = touch ? (T)a2 - p * u2[im] : (T)0;
@@ -439,7 +470,7 @@ __global__ __launch_bounds__(256) void f_hydro_kernel(Grid g, PModel pm, double 
     double x12[3];
     const double dist = distance0(g, x, xp, x12);
     if (!(dist < rcut)) continue;  // phi = 0: adds nothing
-    const double dphi = phi_of(pm, dist);
+    const double dphi = phi_fast<double>(pm, dist);
     double vr[3] = {0.0, 0.0, 0.0};
     if (pm.rotation) {
       vr[0] = wp[1] * r[2] - wp[2] * r[1];
@@ -519,45 +550,53 @@ __device__ __forceinline__ double lj_f(const PModel &pm, double x) {
   return coeff / sq(x) * (2.0 * sq(dmy) - dmy);
 }
 
-__global__ void force_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ spec,
-                             const int *__restrict__ pcell, const int *__restrict__ cell_start,
-                             const int *__restrict__ cell_count, const int *__restrict__ cell_list, int cx, int cy,
-                             int cz) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= pm.np) return;
+// One wavefront per particle: lane l takes the l-th neighbour cell (the
+// cells around the particle's own, each visited once even when fewer than
+// three span an axis) and sums its pairs in list order; the 64 lane sums are
+// combined by a fixed shuffle tree, so the result is run-to-run identical.
+__global__ __launch_bounds__(256) void force_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ spec,
+                                                    const int *__restrict__ pcell, const int *__restrict__ cell_start,
+                                                    const int *__restrict__ cell_count,
+                                                    const int *__restrict__ cell_list, int cx, int cy, int cz) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= pm.np) return;  // whole wavefronts leave together
   double F[3] = {0.0, 0.0, 0.0};
   if (pm.lj_on) {
-    const double xi[3] = {P(pb, pm.np, P_X, i), P(pb, pm.np, P_X + 1, i), P(pb, pm.np, P_X + 2, i)};
     const int ci = pcell[i];
     const int c[3] = {ci / (cy * cz), (ci / cz) % cy, ci % cz};
     const int nc[3] = {cx, cy, cz};
-    int lo[3], hi[3];
+    int lo[3], span[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       lo[d] = nc[d] >= 3 ? -1 : 0;
-      hi[d] = nc[d] >= 2 ? 1 : 0;
+      span[d] = (nc[d] >= 2 ? 1 : 0) - lo[d] + 1;
     }
-    for (int a = lo[0]; a <= hi[0]; ++a)
-      for (int b = lo[1]; b <= hi[1]; ++b)
-        for (int e = lo[2]; e <= hi[2]; ++e) {
-          const int nb = (((c[0] + a + cx) % cx) * cy + (c[1] + b + cy) % cy) * cz + (c[2] + e + cz) % cz;
-          const int s0 = cell_start[nb], cnt = cell_count[nb];
-          for (int q = 0; q < cnt; ++q) {
-            const int j = cell_list[s0 + q];
-            if (j == i) continue;
-            const double xj[3] = {P(pb, pm.np, P_X, j), P(pb, pm.np, P_X + 1, j), P(pb, pm.np, P_X + 2, j)};
-            double rv[3];
-            const double r = distance0(g, xi, xj, rv);
-            if (r < pm.lj_cutoff) {
-              const double ljf = lj_f(pm, r);
-              const double cap = 1.79769313486231570815e+308 / r;
-              const double dmy = cap < ljf ? cap : ljf;
+    if (lane < span[0] * span[1] * span[2]) {
+      const int a = lo[0] + lane / (span[1] * span[2]), b = lo[1] + (lane / span[2]) % span[1],
+                e = lo[2] + lane % span[2];
+      const double xi[3] = {P(pb, pm.np, P_X, i), P(pb, pm.np, P_X + 1, i), P(pb, pm.np, P_X + 2, i)};
+      const int nb = (((c[0] + a + cx) % cx) * cy + (c[1] + b + cy) % cy) * cz + (c[2] + e + cz) % cz;
+      const int s0 = cell_start[nb], cnt = cell_count[nb];
+      for (int q = 0; q < cnt; ++q) {
+        const int j = cell_list[s0 + q];
+        if (j == i) continue;
+        const double xj[3] = {P(pb, pm.np, P_X, j), P(pb, pm.np, P_X + 1, j), P(pb, pm.np, P_X + 2, j)};
+        double rv[3];
+        const double r = distance0(g, xi, xj, rv);
+        if (r < pm.lj_cutoff) {
+          const double ljf = lj_f(pm, r);
+          const double cap = 1.79769313486231570815e+308 / r;
+          const double dmy = cap < ljf ? cap : ljf;
 #pragma unroll
-              for (int d = 0; d < 3; ++d) F[d] += dmy * -rv[d];
-            }
-          }
+          for (int d = 0; d < 3; ++d) F[d] += dmy * -rv[d];
         }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[d] = wave_sum(F[d]);
   }
+  if (lane != 0) return;
 #pragma unroll
   for (int d = 0; d < 3; ++d) PR(pb, pm.np, P_FR + d, i) += F[d];
   if (pm.g_on_fluid != 0.0) PR(pb, pm.np, P_FR + pm.g_direction, i) -= pm.g_on_fluid * (pm.mass_ratio[spec[i]] - 1.0);
@@ -860,7 +899,7 @@ int p_force(spm_ctx *c) {
     hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ncells, 256)), dim3(256), 0, c->stream, c->cell_start,
                        c->cell_count, c->cell_list, c->ncells);
   }
-  hipLaunchKernelGGL(force_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
+  hipLaunchKernelGGL(force_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
                      c->pcell, c->cell_start, c->cell_count, c->cell_list, c->cells[0], c->cells[1], c->cells[2]);
   tend(c, SPM_K_PARTICLE, 8.0 * 6 * c->np);
   SPM_LAUNCH_CHECK(c);
