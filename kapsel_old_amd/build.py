@@ -15,7 +15,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspm.so")
-SOURCES = ["fft.hip", "fused.hip", "spectral.hip", "particles.hip", "electrolyte.hip", "spm_api.cpp"]
+SOURCES = ["fft.hip", "fused.hip", "spectral.hip", "particles.hip", "electrolyte.hip", "spm_api.cpp", "udf.cpp"]
+# plain host C++ (no device code), built with the host compiler
+HOST_SOURCES = {"udf.cpp"}
+CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -27,12 +30,15 @@ FLAGS = [
 
 def _compile(src, objdir=OBJ, defines=()):
     obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
-    if src.endswith(".cpp"):
-        cmd[1:1] = ["-x", "hip"]
+    if src in HOST_SOURCES:
+        cmd = [CXX, "-O2", "-fPIC", "-std=c++17", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed on %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))
+        raise RuntimeError("compile failed on %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))
     return obj
 
 

@@ -1,0 +1,943 @@
+// udf.cpp -- reader for the UDF subset KAPSEL's input uses (include/spm_udf.h).
+//
+// A UDF file holds a schema and data.  The schema sections (global_def and
+// def) declare classes (`class Vector3d:{x: float, ...}`) and entries
+// `name: type "description"`, with types int, double, float, string,
+// `select {'a','b'}`, a braced record of entries, or a class name; `name[]`
+// declares an array (one [] per dimension).  The data section gives
+// `name: value` for top-level entries, each value laid out as its schema
+// says: a record's member values in braces (commas optional), an array's
+// elements in brackets, a select as the chosen quoted string.  A record
+// member that is itself a record (the bodies of a select's alternatives, as
+// in constitutive_eq) appears in braces in schema order whether chosen or
+// not.  Comments run from // to the end of the line.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/spm_udf.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------------
+// tokens
+// ---------------------------------------------------------------------
+enum Tok { T_END, T_LBRACE, T_RBRACE, T_LBRACK, T_RBRACK, T_COMMA, T_COLON, T_STRING, T_WORD, T_DIRECTIVE };
+
+struct Token {
+  Tok t;
+  std::string s;  // string contents, word, or directive "begin:data" / "end:data" / "include:path"
+  int line;
+};
+
+bool lex(const std::string &src, std::vector<Token> &out, std::string &err) {
+  size_t i = 0;
+  int line = 1;
+  const size_t n = src.size();
+  while (i < n) {
+    const char ch = src[i];
+    if (ch == '\n') {
+      ++line;
+      ++i;
+    } else if (isspace((unsigned char)ch)) {
+      ++i;
+    } else if (ch == '/' && i + 1 < n && src[i + 1] == '/') {
+      while (i < n && src[i] != '\n') ++i;
+    } else if (ch == '{' || ch == '}' || ch == '[' || ch == ']' || ch == ',' || ch == ':') {
+      const Tok t = ch == '{' ? T_LBRACE : ch == '}' ? T_RBRACE : ch == '[' ? T_LBRACK : ch == ']' ? T_RBRACK
+                                                                                                    : ch == ',' ? T_COMMA
+                                                                                                                : T_COLON;
+      out.push_back({t, std::string(1, ch), line});
+      ++i;
+    } else if (ch == '"' || ch == '\'') {
+      size_t j = i + 1;
+      while (j < n && src[j] != ch) {
+        if (src[j] == '\n') ++line;
+        ++j;
+      }
+      if (j >= n) {
+        err = "unterminated string at line " + std::to_string(line);
+        return false;
+      }
+      out.push_back({T_STRING, src.substr(i + 1, j - i - 1), line});
+      i = j + 1;
+    } else if (ch == '\\') {
+      // \begin{x}, \end{x}, \include{"file"}
+      size_t j = i + 1;
+      while (j < n && isalpha((unsigned char)src[j])) ++j;
+      const std::string cmd = src.substr(i + 1, j - i - 1);
+      if (j >= n || src[j] != '{') {
+        err = "malformed directive at line " + std::to_string(line);
+        return false;
+      }
+      const size_t k = src.find('}', j);
+      if (k == std::string::npos) {
+        err = "malformed directive at line " + std::to_string(line);
+        return false;
+      }
+      std::string arg = src.substr(j + 1, k - j - 1);
+      if (!arg.empty() && (arg.front() == '"' || arg.front() == '\'')) arg = arg.substr(1, arg.size() - 2);
+      out.push_back({T_DIRECTIVE, cmd + ":" + arg, line});
+      i = k + 1;
+    } else {
+      size_t j = i;
+      while (j < n && !isspace((unsigned char)src[j]) && strchr("{}[],:\"'", src[j]) == nullptr &&
+             !(src[j] == '/' && j + 1 < n && src[j + 1] == '/'))
+        ++j;
+      out.push_back({T_WORD, src.substr(i, j - i), line});
+      i = j;
+    }
+  }
+  out.push_back({T_END, "", line});
+  return true;
+}
+
+// ---------------------------------------------------------------------
+// schema and values
+// ---------------------------------------------------------------------
+struct Type;
+using TypeP = std::shared_ptr<Type>;
+enum Kind { K_INT, K_REAL, K_STRING, K_SELECT, K_RECORD, K_ARRAY, K_REF };
+
+struct Type {
+  Kind kind = K_REAL;
+  std::vector<std::pair<std::string, TypeP>> fields;  // K_RECORD
+  std::vector<std::string> options;                   // K_SELECT
+  TypeP elem;                                         // K_ARRAY
+  std::string ref;                                    // K_REF: class name
+};
+
+struct Value {
+  Kind kind = K_REAL;
+  double num = 0.0;
+  std::string str;
+  std::vector<std::pair<std::string, Value>> fields;  // K_RECORD (names from the schema)
+  std::vector<Value> items;                           // K_ARRAY
+};
+
+struct Doc {
+  std::map<std::string, TypeP> classes;
+  std::vector<std::pair<std::string, TypeP>> entries;  // top-level schema entries
+  std::map<std::string, Value> data;
+  std::vector<std::map<std::string, Value>> records;  // \begin{record} blocks in file order
+};
+
+class Parser {
+ public:
+  Parser(Doc &d, std::vector<Token> &t) : doc(d), tok(t) {}
+
+  bool run(std::string &err) {
+    try {
+      while (peek().t != T_END) {
+        const Token &d = peek();
+        if (d.t != T_DIRECTIVE) throw std::runtime_error("expected a section at line " + std::to_string(d.line));
+        const std::string s = next().s;
+        if (s == "begin:global_def" || s == "begin:def")
+          schema_section();
+        else if (s == "begin:data")
+          data_section(doc.data, "end:data");
+        else if (s == "begin:record") {
+          doc.records.emplace_back();
+          data_section(doc.records.back(), "end:record");
+        }
+        else if (s == "begin:header")
+          skip_to("end:header");
+        else if (s.rfind("begin:", 0) == 0)
+          skip_to("end:" + s.substr(6));
+        else
+          throw std::runtime_error("unexpected " + s + " at line " + std::to_string(d.line));
+      }
+    } catch (const std::exception &e) {
+      err = e.what();
+      return false;
+    }
+    return true;
+  }
+
+ private:
+  Doc &doc;
+  std::vector<Token> &tok;
+  size_t pos = 0;
+
+  const Token &peek() const { return tok[pos]; }
+  const Token &next() { return tok[pos++]; }
+  void expect(Tok t, const char *what) {
+    if (peek().t != t) throw std::runtime_error(std::string("expected ") + what + " at line " + std::to_string(peek().line));
+    ++pos;
+  }
+  void skip_to(const std::string &dir) {
+    while (peek().t != T_END && !(peek().t == T_DIRECTIVE && peek().s == dir)) ++pos;
+    if (peek().t == T_END) throw std::runtime_error("missing \\" + dir);
+    ++pos;
+  }
+  void skip_commas() {
+    while (peek().t == T_COMMA) ++pos;
+  }
+
+  // name with optional [] dimensions; returns the dimension count
+  std::string entry_name(int &dims) {
+    const Token &t = next();
+    if (t.t != T_WORD) throw std::runtime_error("expected a name at line " + std::to_string(t.line));
+    std::string name = t.s;
+    dims = 0;
+    while (peek().t == T_LBRACK) {
+      ++pos;
+      expect(T_RBRACK, "]");
+      ++dims;
+    }
+    return name;
+  }
+
+  TypeP type_expr() {
+    auto ty = std::make_shared<Type>();
+    const Token &t = next();
+    if (t.t == T_LBRACE) {
+      ty->kind = K_RECORD;
+      skip_commas();
+      while (peek().t != T_RBRACE) {
+        int dims = 0;
+        const std::string nm = entry_name(dims);
+        expect(T_COLON, ":");
+        TypeP ft = type_expr();
+        for (int k = 0; k < dims; ++k) {
+          auto a = std::make_shared<Type>();
+          a->kind = K_ARRAY;
+          a->elem = ft;
+          ft = a;
+        }
+        ty->fields.push_back({nm, ft});
+        skip_commas();
+      }
+      ++pos;
+    } else if (t.t == T_WORD) {
+      if (t.s == "int") {
+        ty->kind = K_INT;
+      } else if (t.s == "double" || t.s == "float") {
+        ty->kind = K_REAL;
+      } else if (t.s == "string") {
+        ty->kind = K_STRING;
+      } else if (t.s == "select") {
+        ty->kind = K_SELECT;
+        expect(T_LBRACE, "{");
+        while (peek().t != T_RBRACE) {
+          if (peek().t == T_STRING || peek().t == T_WORD) ty->options.push_back(next().s);
+          else if (peek().t == T_COMMA) ++pos;
+          else throw std::runtime_error("bad select at line " + std::to_string(peek().line));
+        }
+        ++pos;
+      } else {
+        ty->kind = K_REF;
+        ty->ref = t.s;
+      }
+    } else {
+      throw std::runtime_error("expected a type at line " + std::to_string(t.line));
+    }
+    if (peek().t == T_STRING) ++pos;  // description
+    return ty;
+  }
+
+  void schema_section() {
+    while (!(peek().t == T_DIRECTIVE && peek().s.rfind("end:", 0) == 0)) {
+      if (peek().t == T_END) throw std::runtime_error("unterminated schema section");
+      if (peek().t == T_WORD && peek().s == "class") {
+        ++pos;
+        const std::string nm = next().s;
+        expect(T_COLON, ":");
+        doc.classes[nm] = type_expr();
+        continue;
+      }
+      int dims = 0;
+      const std::string nm = entry_name(dims);
+      expect(T_COLON, ":");
+      TypeP ty = type_expr();
+      for (int k = 0; k < dims; ++k) {
+        auto a = std::make_shared<Type>();
+        a->kind = K_ARRAY;
+        a->elem = ty;
+        ty = a;
+      }
+      doc.entries.push_back({nm, ty});
+      skip_commas();
+    }
+    ++pos;
+  }
+
+  TypeP resolve(TypeP t) {
+    int guard = 0;
+    while (t->kind == K_REF) {
+      auto it = doc.classes.find(t->ref);
+      if (it == doc.classes.end() || ++guard > 64) throw std::runtime_error("unknown class " + t->ref);
+      t = it->second;
+    }
+    return t;
+  }
+
+  Value value(TypeP ty) {
+    ty = resolve(ty);
+    Value v;
+    v.kind = ty->kind;
+    switch (ty->kind) {
+      case K_INT:
+      case K_REAL: {
+        const Token &t = next();
+        if (t.t != T_WORD) throw std::runtime_error("expected a number at line " + std::to_string(t.line));
+        char *end = nullptr;
+        v.num = std::strtod(t.s.c_str(), &end);
+        if (end == t.s.c_str() || *end != '\0')
+          throw std::runtime_error("bad number '" + t.s + "' at line " + std::to_string(t.line));
+        break;
+      }
+      case K_STRING:
+      case K_SELECT: {
+        const Token &t = next();
+        if (t.t != T_STRING && t.t != T_WORD)
+          throw std::runtime_error("expected a string at line " + std::to_string(t.line));
+        v.str = t.s;
+        break;
+      }
+      case K_RECORD: {
+        expect(T_LBRACE, "{");
+        for (auto &f : ty->fields) {
+          skip_commas();
+          v.fields.push_back({f.first, value(f.second)});
+        }
+        skip_commas();
+        expect(T_RBRACE, "}");
+        break;
+      }
+      case K_ARRAY: {
+        expect(T_LBRACK, "[");
+        skip_commas();
+        while (peek().t != T_RBRACK) {
+          v.items.push_back(value(ty->elem));
+          skip_commas();
+        }
+        ++pos;
+        break;
+      }
+      default:
+        throw std::runtime_error("unresolved type");
+    }
+    return v;
+  }
+
+  void data_section(std::map<std::string, Value> &out, const std::string &end) {
+    while (!(peek().t == T_DIRECTIVE && peek().s == end)) {
+      if (peek().t == T_END) throw std::runtime_error("missing \\" + end);
+      int dims = 0;
+      const int line = peek().line;
+      const std::string nm = entry_name(dims);
+      expect(T_COLON, ":");
+      TypeP ty;
+      for (auto &e : doc.entries)
+        if (e.first == nm) ty = e.second;
+      if (!ty) throw std::runtime_error("no schema for '" + nm + "' at line " + std::to_string(line));
+      out[nm] = value(ty);
+      skip_commas();
+    }
+    ++pos;
+  }
+};
+
+bool read_file(const std::string &path, std::string &out) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return false;
+  std::ostringstream ss;
+  ss << f.rdbuf();
+  out = ss.str();
+  return true;
+}
+
+// tokens of `path` with \include{...} replaced by the included file's tokens
+bool lex_file(const std::string &path, std::vector<Token> &out, std::string &err, int depth = 0) {
+  if (depth > 8) {
+    err = "include nesting too deep";
+    return false;
+  }
+  std::string src;
+  if (!read_file(path, src)) {
+    err = "cannot read " + path;
+    return false;
+  }
+  std::vector<Token> t;
+  if (!lex(src, t, err)) {
+    err = path + ": " + err;
+    return false;
+  }
+  t.pop_back();  // T_END
+  const std::string dir = path.find('/') == std::string::npos ? "" : path.substr(0, path.rfind('/') + 1);
+  for (auto &x : t) {
+    if (x.t == T_DIRECTIVE && x.s.rfind("include:", 0) == 0) {
+      std::string inc = x.s.substr(8);
+      if (!inc.empty() && inc[0] != '/') inc = dir + inc;
+      if (!lex_file(inc, out, err, depth + 1)) return false;
+    } else {
+      out.push_back(x);
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------
+// locations
+// ---------------------------------------------------------------------
+// A location is a top-level name followed by .member and [i] steps; an
+// empty [] stands for every element of that array.
+struct Step {
+  enum { MEMBER, INDEX, ALL } kind;
+  std::string name;
+  long index;
+};
+
+bool split_location(const std::string &loc, std::string &head, std::vector<Step> &steps) {
+  size_t i = 0;
+  while (i < loc.size() && loc[i] != '.' && loc[i] != '[') ++i;
+  head = loc.substr(0, i);
+  if (head.empty()) return false;
+  while (i < loc.size()) {
+    if (loc[i] == '.') {
+      size_t j = i + 1;
+      while (j < loc.size() && loc[j] != '.' && loc[j] != '[') ++j;
+      if (j == i + 1) return false;
+      steps.push_back({Step::MEMBER, loc.substr(i + 1, j - i - 1), 0});
+      i = j;
+    } else {  // '['
+      const size_t k = loc.find(']', i);
+      if (k == std::string::npos) return false;
+      const std::string idx = loc.substr(i + 1, k - i - 1);
+      if (idx.empty()) {
+        steps.push_back({Step::ALL, "", 0});
+      } else {
+        if (idx.find_first_not_of("0123456789") != std::string::npos || idx.size() > 9) return false;
+        steps.push_back({Step::INDEX, "", std::strtol(idx.c_str(), nullptr, 10)});
+      }
+      i = k + 1;
+    }
+  }
+  return true;
+}
+
+// Appends the values `steps[s..]` reaches from v; false if a step fails.
+bool walk(const Value *v, const std::vector<Step> &steps, size_t s, std::vector<const Value *> &out) {
+  for (; s < steps.size(); ++s) {
+    const Step &st = steps[s];
+    if (st.kind == Step::MEMBER) {
+      if (v->kind != K_RECORD) return false;
+      const Value *nxt = nullptr;
+      for (auto &f : v->fields)
+        if (f.first == st.name) nxt = &f.second;
+      if (!nxt) return false;
+      v = nxt;
+    } else if (st.kind == Step::INDEX) {
+      if (v->kind != K_ARRAY || st.index >= (long)v->items.size()) return false;
+      v = &v->items[st.index];
+    } else {
+      if (v->kind != K_ARRAY) return false;
+      if (s + 1 == steps.size()) break;  // trailing []: the array itself
+      for (auto &it : v->items)
+        if (!walk(&it, steps, s + 1, out)) return false;
+      return true;
+    }
+  }
+  out.push_back(v);
+  return true;
+}
+
+// Values at `loc`: the current record's entries first, then the data section.
+bool lookup(const Doc &d, int record, const std::string &loc, std::vector<const Value *> &out) {
+  std::string head;
+  std::vector<Step> steps;
+  if (!split_location(loc, head, steps)) return false;
+  const Value *root = nullptr;
+  if (record >= 0) {
+    auto it = d.records[record].find(head);
+    if (it != d.records[record].end()) root = &it->second;
+  }
+  if (!root) {
+    auto it = d.data.find(head);
+    if (it == d.data.end()) return false;
+    root = &it->second;
+  }
+  return walk(root, steps, 0, out);
+}
+
+// The single value at `loc` (no [] inside it); a trailing [] gives the array.
+const Value *find(const Doc &d, int record, const std::string &loc) {
+  std::vector<const Value *> v;
+  const size_t inner = loc.find("[]");
+  if (inner != std::string::npos && inner + 2 != loc.size()) return nullptr;
+  if (!lookup(d, record, loc, v) || v.size() != 1) return nullptr;
+  return v[0];
+}
+
+}  // namespace
+
+struct spm_udf {
+  Doc doc;
+  int record = -1;  // record the getters read first (spm_udf_jump), -1 for none
+};
+
+extern "C" {
+
+const char *spm_udf_last_error(void) { return g_err.c_str(); }
+
+int spm_udf_open(const char *path, spm_udf **out) {
+  if (!path || !out) return fail(SPM_ERR_INVALID, "null argument");
+  *out = nullptr;
+  std::vector<Token> tok;
+  std::string err;
+  if (!lex_file(path, tok, err)) return fail(SPM_ERR_INVALID, err);
+  tok.push_back({T_END, "", 0});
+  auto u = std::make_unique<spm_udf>();
+  Parser p(u->doc, tok);
+  if (!p.run(err)) return fail(SPM_ERR_INVALID, std::string(path) + ": " + err);
+  *out = u.release();
+  return SPM_OK;
+}
+
+void spm_udf_close(spm_udf *udf) { delete udf; }
+
+int spm_udf_get_double(const spm_udf *u, const char *loc, double *v) {
+  if (!u || !loc || !v) return fail(SPM_ERR_INVALID, "null argument");
+  const Value *x = std::strstr(loc, "[]") ? nullptr : find(u->doc, u->record, loc);
+  if (!x || (x->kind != K_REAL && x->kind != K_INT)) return fail(SPM_ERR_INVALID, std::string("no number at ") + loc);
+  *v = x->num;
+  return SPM_OK;
+}
+
+int spm_udf_get_int(const spm_udf *u, const char *loc, int *v) {
+  double d = 0.0;
+  const int rc = spm_udf_get_double(u, loc, &d);
+  if (rc) return rc;
+  if (d != std::floor(d)) return fail(SPM_ERR_INVALID, std::string("not an integer at ") + loc);
+  *v = (int)d;
+  return SPM_OK;
+}
+
+int spm_udf_get_string(const spm_udf *u, const char *loc, char *buf, int len) {
+  if (!u || !loc || !buf || len <= 0) return fail(SPM_ERR_INVALID, "null argument");
+  const Value *x = std::strstr(loc, "[]") ? nullptr : find(u->doc, u->record, loc);
+  if (!x || (x->kind != K_STRING && x->kind != K_SELECT)) return fail(SPM_ERR_INVALID, std::string("no string at ") + loc);
+  if ((int)x->str.size() >= len) return fail(SPM_ERR_INVALID, "buffer too small");
+  std::memcpy(buf, x->str.c_str(), x->str.size() + 1);
+  return SPM_OK;
+}
+
+int spm_udf_size(const spm_udf *u, const char *loc, int *n) {
+  if (!u || !loc || !n) return fail(SPM_ERR_INVALID, "null argument");
+  std::string l(loc);
+  if (l.size() < 2 || l.compare(l.size() - 2, 2, "[]") != 0) l += "[]";
+  const Value *x = find(u->doc, u->record, l);
+  if (!x || x->kind != K_ARRAY) return fail(SPM_ERR_INVALID, std::string("no array at ") + loc);
+  *n = (int)x->items.size();
+  return SPM_OK;
+}
+
+int spm_udf_get_doubles(const spm_udf *u, const char *loc, double *out, int cap, int *n) {
+  if (!u || !loc || !n || (cap > 0 && !out)) return fail(SPM_ERR_INVALID, "null argument");
+  std::vector<const Value *> v;
+  if (!lookup(u->doc, u->record, loc, v)) return fail(SPM_ERR_INVALID, std::string("no values at ") + loc);
+  for (auto *x : v)
+    if (x->kind != K_REAL && x->kind != K_INT) return fail(SPM_ERR_INVALID, std::string("not numbers at ") + loc);
+  *n = (int)v.size();
+  if ((int)v.size() > cap) return fail(SPM_ERR_INVALID, std::string("more values than room at ") + loc);
+  for (size_t k = 0; k < v.size(); ++k) out[k] = v[k]->num;
+  return SPM_OK;
+}
+
+int spm_udf_num_records(const spm_udf *u, int *n) {
+  if (!u || !n) return fail(SPM_ERR_INVALID, "null argument");
+  *n = (int)u->doc.records.size();
+  return SPM_OK;
+}
+
+int spm_udf_jump(spm_udf *u, int record) {
+  if (!u) return fail(SPM_ERR_INVALID, "null argument");
+  if (record < -1 || record >= (int)u->doc.records.size()) return fail(SPM_ERR_INVALID, "no such record");
+  u->record = record;
+  return SPM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// output file: the input's text with includes inlined, then one
+// \begin{record} block per snapshot
+// ---------------------------------------------------------------------
+namespace {
+
+bool inline_includes(const std::string &path, std::string &out, std::string &err, int depth = 0) {
+  if (depth > 8) {
+    err = "include nesting too deep";
+    return false;
+  }
+  std::string src;
+  if (!read_file(path, src)) {
+    err = "cannot read " + path;
+    return false;
+  }
+  const std::string dir = path.find('/') == std::string::npos ? "" : path.substr(0, path.rfind('/') + 1);
+  std::istringstream in(src);
+  std::string line;
+  while (std::getline(in, line)) {
+    const size_t b = line.find_first_not_of(" \t");
+    if (b != std::string::npos && line.compare(b, 9, "\\include{") == 0) {
+      const size_t e = line.find('}', b);
+      std::string inc = e == std::string::npos ? "" : line.substr(b + 9, e - b - 9);
+      if (!inc.empty() && (inc.front() == '"' || inc.front() == '\'')) inc = inc.substr(1, inc.size() - 2);
+      if (inc.empty()) {
+        err = "malformed include in " + path;
+        return false;
+      }
+      if (inc[0] != '/') inc = dir + inc;
+      if (!inline_includes(inc, out, err, depth + 1)) return false;
+      if (!out.empty() && out.back() != '\n') out += '\n';
+    } else {
+      out += line;
+      out += '\n';
+    }
+  }
+  return true;
+}
+
+void put3(std::string &o, const double *v) {
+  char b[96];
+  std::snprintf(b, sizeof(b), "{%.17g,%.17g,%.17g}", v[0], v[1], v[2]);
+  o += b;
+}
+
+}  // namespace
+
+extern "C" int spm_udf_create_output(const char *in_path, const char *out_path) {
+  if (!in_path || !out_path) return fail(SPM_ERR_INVALID, "null argument");
+  spm_udf *probe = nullptr;
+  const int rc = spm_udf_open(in_path, &probe);
+  if (rc) return rc;
+  spm_udf_close(probe);
+  std::string text, err;
+  if (!inline_includes(in_path, text, err)) return fail(SPM_ERR_INVALID, err);
+  std::ofstream f(out_path, std::ios::binary | std::ios::trunc);
+  if (!f) return fail(SPM_ERR_INVALID, std::string("cannot write ") + out_path);
+  f << text;
+  return f.good() ? SPM_OK : fail(SPM_ERR_INVALID, std::string("write failed on ") + out_path);
+}
+
+extern "C" int spm_udf_append_record(const char *out_path, int ts, const spm_particle *p, int n) {
+  if (!out_path || (!p && n > 0) || n < 0) return fail(SPM_ERR_INVALID, "null argument");
+  // Output_udf (avs_output.cxx:343-396): E = 1.0, t = ts, and per particle
+  // R, R_raw, v, q, omega, then the previous-step hydrodynamic, LJ and slip
+  // forces and torques (torque_r written as zero)
+  std::string o = "\\begin{record}\nE:1.0\n";
+  o += "t:" + std::to_string(ts) + "\nParticles[]:[\n";
+  const double zero[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < n; ++k) {
+    const spm_particle &q = p[k];
+    const double qq[4] = {q.q.s, q.q.v[0], q.q.v[1], q.q.v[2]};
+    char b[128];
+    o += "{";
+    put3(o, q.x);
+    put3(o, q.x_nopbc);
+    put3(o, q.v);
+    std::snprintf(b, sizeof(b), "{%.17g,%.17g,%.17g,%.17g}", qq[0], qq[1], qq[2], qq[3]);
+    o += b;
+    put3(o, q.omega);
+    put3(o, q.f_hydro_previous);
+    put3(o, q.torque_hydro_previous);
+    put3(o, q.fr_previous);
+    put3(o, zero);
+    put3(o, q.f_slip_previous);
+    put3(o, q.torque_slip_previous);
+    o += "}\n";
+  }
+  o += "]\nRigidParticles[]:[]\n\\end{record}\n";
+  std::ofstream f(out_path, std::ios::binary | std::ios::app);
+  if (!f) return fail(SPM_ERR_INVALID, std::string("cannot append to ") + out_path);
+  f << o;
+  return f.good() ? SPM_OK : fail(SPM_ERR_INVALID, std::string("write failed on ") + out_path);
+}
+
+// ---------------------------------------------------------------------
+// input.cxx mapping
+// ---------------------------------------------------------------------
+namespace {
+
+struct Reader {
+  const spm_udf *u;
+  int rc = SPM_OK;
+  double d(const std::string &loc) {
+    double v = 0.0;
+    if (rc == SPM_OK) rc = spm_udf_get_double(u, loc.c_str(), &v);
+    return v;
+  }
+  int i(const std::string &loc) {
+    int v = 0;
+    if (rc == SPM_OK) rc = spm_udf_get_int(u, loc.c_str(), &v);
+    return v;
+  }
+  std::string s(const std::string &loc) {
+    char buf[256] = {0};
+    if (rc == SPM_OK) rc = spm_udf_get_string(u, loc.c_str(), buf, (int)sizeof(buf));
+    return buf;
+  }
+  int size(const std::string &loc) {
+    int v = 0;
+    if (rc == SPM_OK) rc = spm_udf_size(u, loc.c_str(), &v);
+    return v;
+  }
+};
+
+int unsupported(const std::string &what) { return fail(SPM_ERR_UNSUPPORTED, what + " is not run by this library"); }
+int invalid(const std::string &what) { return fail(SPM_ERR_INVALID, "invalid " + what); }
+
+}  // namespace
+
+extern "C" int spm_udf_params(const spm_udf *u, spm_params *prm, spm_electrolyte *el, spm_udf_run *run) {
+  if (!u || !prm || !run) return fail(SPM_ERR_INVALID, "null argument");
+  Reader r{u};
+  spm_params p;
+  std::memset(&p, 0, sizeof(p));
+  spm_udf_run rn;
+  std::memset(&rn, 0, sizeof(rn));
+  // constitutive_eq (input.cxx:438-790)
+  const std::string eq = r.s("constitutive_eq.type");
+  if (r.rc) return r.rc;
+  std::string base;
+  if (eq == "Navier_Stokes") {
+    rn.equation = SPM_UDF_NAVIER_STOKES;
+    base = "constitutive_eq.Navier_Stokes.";
+  } else if (eq == "Electrolyte") {
+    rn.equation = SPM_UDF_ELECTROLYTE;
+    base = "constitutive_eq.Electrolyte.";
+    if (!el) return fail(SPM_ERR_INVALID, "Electrolyte input needs an spm_electrolyte");
+  } else {
+    return unsupported("constitutive_eq " + eq);
+  }
+  p.dx = r.d(base + "DX");
+  p.rho = r.d(base + "RHO");
+  p.eta = r.d(base + "ETA");
+  p.kbt = r.d(base + "kBT");
+  if (rn.equation == SPM_UDF_NAVIER_STOKES) {
+    // the Electrolyte branch does not read alpha_v / alpha_o, which keep
+    // their zero initial values (input.cxx:267, 660-667)
+    p.alpha_v = r.d(base + "alpha_v");
+    p.alpha_o = r.d(base + "alpha_o");
+  }
+  double frequency = 0.0;
+  if (rn.equation == SPM_UDF_ELECTROLYTE) {
+    std::memset(el, 0, sizeof(*el));
+    el->elementary_charge = 1.0;  // Elementary_charge (input.cxx:247)
+    el->dielectric = r.d(base + "Dielectric_cst");
+    const std::string init = r.s(base + "INIT_profile");
+    rn.poisson_boltzmann = init == "Poisson_Boltzmann";
+    if (!rn.poisson_boltzmann && init != "Uniform" && r.rc == SPM_OK) return invalid("INIT_profile " + init);
+    const std::string salt = r.s(base + "Add_salt.type");
+    if (salt == "saltfree") {
+      el->n_spec = 1;
+      el->valency[0] = r.d(base + "Add_salt.saltfree.Valency_counterion");
+      el->onsager[0] = r.d(base + "Add_salt.saltfree.Onsager_coeff_counterion");
+    } else if (salt == "salt") {
+      el->n_spec = 2;
+      el->valency[0] = r.d(base + "Add_salt.salt.Valency_positive_ion");
+      el->valency[1] = r.d(base + "Add_salt.salt.Valency_negative_ion");
+      el->onsager[0] = r.d(base + "Add_salt.salt.Onsager_coeff_positive_ion");
+      el->onsager[1] = r.d(base + "Add_salt.salt.Onsager_coeff_negative_ion");
+      el->debye_length = r.d(base + "Add_salt.salt.Debye_length");
+    } else if (r.rc == SPM_OK) {
+      return invalid("Add_salt " + salt);
+    }
+    const std::string ef = r.s(base + "Electric_field.type");
+    if (ef == "ON") {
+      el->external_field = 1;
+      const std::string t = r.s(base + "Electric_field.ON.type");
+      const std::string b = base + "Electric_field.ON." + t + ".";
+      if (t == "AC") {
+        el->ac = 1;
+        frequency = r.d(b + "Frequency");
+        el->angular_frequency = 2.0 * M_PI * frequency;  // PI2 * Frequency, input.cxx:356
+      } else if (t != "DC" && r.rc == SPM_OK) {
+        return invalid("Electric_field type " + t);
+      }
+      el->e_ext[0] = r.d(b + "Ex");
+      el->e_ext[1] = r.d(b + "Ey");
+      el->e_ext[2] = r.d(b + "Ez");
+    } else if (ef != "OFF" && r.rc == SPM_OK) {
+      return invalid("Electric_field " + ef);
+    }
+  }
+  if (r.rc) return r.rc;
+  // object_type (input.cxx:800-1330)
+  const std::string obj = r.s("object_type.type");
+  if (r.rc) return r.rc;
+  if (obj != "spherical_particle") return unsupported("object_type " + obj);
+  const int nspec = r.size("object_type.spherical_particle.Particle_spec[]");
+  if (r.rc) return r.rc;
+  if (nspec < 1 || nspec > SPM_MAX_SPECIES) return invalid("Particle_spec count");
+  p.n_species = nspec;
+  for (int s = 0; s < nspec; ++s) {
+    const std::string b = "object_type.spherical_particle.Particle_spec[" + std::to_string(s) + "].";
+    p.n_particles += r.i(b + "Particle_number");
+    p.mass_ratio[s] = r.d(b + "MASS_RATIO");
+    const double q = r.d(b + "Surface_charge");
+    if (el && rn.equation == SPM_UDF_ELECTROLYTE) el->surface_charge[s] = q;
+    const std::string jp = r.s(b + "janus_propulsion");
+    if (r.rc) return r.rc;
+    if (jp != "OFF") return unsupported("janus_propulsion " + jp);
+  }
+  // geometry and forces (input.cxx:1332-1405)
+  p.a_xi = r.d("A_XI");
+  p.a = r.d("A");
+  p.g = r.d("gravity.G");
+  const std::string gd = r.s("gravity.G_direction");
+  p.epsilon = r.d("EPSILON");
+  const std::string ljp = r.s("LJ_powers");
+  int np[3] = {r.i("mesh.NPX"), r.i("mesh.NPY"), r.i("mesh.NPZ")};
+  if (r.rc) return r.rc;
+  if (gd == "-X") p.g_direction = 0;
+  else if (gd == "-Y") p.g_direction = 1;
+  else if (gd == "-Z") p.g_direction = 2;
+  else return invalid("G_direction " + gd);
+  if (ljp == "12:6") p.lj_powers = SPM_LJ_12_6;
+  else if (ljp == "24:12") p.lj_powers = SPM_LJ_24_12;
+  else if (ljp == "36:18") p.lj_powers = SPM_LJ_36_18;
+  else return invalid("LJ_powers " + ljp);
+  for (int d = 0; d < 3; ++d)
+    if (np[d] < 0 || np[d] > 12) return invalid("mesh");
+  p.nx = 1 << np[0];
+  p.ny = 1 << np[1];
+  p.nz = 1 << np[2];
+  // time_increment (input.cxx:1406-1426)
+  const std::string ti = r.s("time_increment.type");
+  if (ti == "auto") {
+    rn.auto_dt = 1;
+    rn.dt_factor = r.d("time_increment.auto.factor");
+  } else if (ti == "manual") {
+    p.dt = r.d("time_increment.manual.delta_t");
+  } else if (r.rc == SPM_OK) {
+    return invalid("time_increment " + ti);
+  }
+  // switch (input.cxx:1427-1630)
+  const std::string rot = r.s("switch.ROTATION");
+  const std::string ljt = r.s("switch.LJ_truncate");
+  const std::string dist = r.s("switch.INIT_distribution.type");
+  if (r.rc) return r.rc;
+  if (rot == "ON") p.rotation = 1;
+  else if (rot == "OFF") p.rotation = 0;
+  else return invalid("ROTATION " + rot);
+  if (ljt == "ON") p.lj_truncate = SPM_LJ_WCA;
+  else if (ljt == "OFF") p.lj_truncate = SPM_LJ_FULL;
+  else if (ljt == "NONE") p.lj_truncate = SPM_LJ_NONE;
+  else return invalid("LJ_truncate " + ljt);
+  static const char *dists[] = {"NONE", "uniform_random", "random_walk", "FCC", "BCC", "user_specify"};
+  rn.distribution = -1;
+  for (int k = 0; k < 6; ++k)
+    if (dist == dists[k]) rn.distribution = k;
+  if (rn.distribution < 0) return invalid("INIT_distribution " + dist);
+  const char *xyz[3] = {"x", "y", "z"};
+  for (int d = 0; d < 3; ++d) {
+    const std::string fc = r.s(std::string("switch.FIX_CELL.") + xyz[d]);
+    if (r.rc) return r.rc;
+    if (fc == "ON") p.fix_cell[d] = 1;
+    else if (fc == "OFF") p.fix_cell[d] = 0;
+    else return invalid(std::string("FIX_CELL") + xyz[d]);
+  }
+  {
+    char pin[16] = {0};
+    if (spm_udf_get_string(u, "switch.pin.type", pin, sizeof(pin)) == SPM_OK && std::strcmp(pin, "YES") == 0)
+      return unsupported("pinning");
+  }
+  // output (input.cxx:1645-1660)
+  rn.gts = r.i("output.GTS");
+  rn.num_snap = r.i("output.Num_snap");
+  if (r.rc) return r.rc;
+  // derived time step, Set_global_parameters (input.cxx:312-381)
+  if (rn.auto_dt) {
+    const double nu = p.eta * (1. / p.rho);  // NU = ETA * IRHO (input.cxx:288-289)
+    const int trn[3] = {(p.nx + 2) / 3, (p.ny + 2) / 3, (p.nz + 2) / 3};
+    const double L[3] = {p.nx * p.dx, p.ny * p.dx, p.nz * p.dx};
+    const double pi2 = 2. * M_PI;
+    const double kmax2 = (pi2 / L[0] * trn[0]) * (pi2 / L[0] * trn[0]) + (pi2 / L[1] * trn[1]) * (pi2 / L[1] * trn[1]) +
+                         (pi2 / L[2] * trn[2]) * (pi2 / L[2] * trn[2]);
+    double tdump = 1. / (nu * kmax2);
+    if (rn.equation == SPM_UDF_ELECTROLYTE) {
+      const double onsager = el->n_spec == 1 ? el->onsager[0] : std::fmax(el->onsager[0], el->onsager[1]);
+      const double diffusion_time = 1. / (p.kbt * onsager * kmax2);
+      tdump = std::fmin(tdump, diffusion_time);
+      if (el->external_field && el->ac) tdump = std::fmin(tdump, 1.e-2 * (1. / frequency));
+    }
+    p.dt = rn.dt_factor * tdump;
+  }
+  p.precision = SPM_FP64;
+  *prm = p;
+  *run = rn;
+  return SPM_OK;
+}
+
+extern "C" int spm_udf_user_particles(const spm_udf *u, spm_particle *p, int n) {
+  if (!u || (!p && n > 0)) return fail(SPM_ERR_INVALID, "null argument");
+  Reader r{u};
+  const std::string dist = r.s("switch.INIT_distribution.type");
+  const std::string rot = r.s("switch.ROTATION");
+  const std::string orient = r.s("switch.INIT_orientation");
+  const int nspec = r.size("object_type.spherical_particle.Particle_spec[]");
+  if (r.rc) return r.rc;
+  if (dist != "user_specify") return fail(SPM_ERR_INVALID, "INIT_distribution is " + dist + ", not user_specify");
+  // orientation (init_particle.cxx:282-286 and 318-345): the file's q,
+  // normalized, only with ROTATION on and INIT_orientation user_specify
+  const bool user_q = rot == "ON" && orient == "user_specify";
+  if (rot == "ON" && orient == "random") return unsupported("INIT_orientation random");
+  const std::string b0 = "switch.INIT_distribution.user_specify.Particles";
+  const int m = r.size(b0 + "[]");
+  if (r.rc) return r.rc;
+  if (m != n) return fail(SPM_ERR_INVALID, "Particles[] holds " + std::to_string(m) + " entries");
+  // species by Particle_spec counts in order (init_particle.cxx:351-357)
+  std::vector<int> spec;
+  for (int s = 0; s < nspec; ++s) {
+    const int c = r.i("object_type.spherical_particle.Particle_spec[" + std::to_string(s) + "].Particle_number");
+    if (r.rc) return r.rc;
+    spec.insert(spec.end(), c, s);
+  }
+  if ((int)spec.size() != n) return fail(SPM_ERR_INVALID, "Particle_number total differs from Particles[]");
+  std::memset(p, 0, sizeof(spm_particle) * (size_t)n);
+  for (int k = 0; k < n; ++k) {
+    const std::string b = b0 + "[" + std::to_string(k) + "].";
+    const char *c[3] = {"x", "y", "z"};
+    p[k].spec = spec[k];
+    for (int d = 0; d < 3; ++d) {
+      p[k].x[d] = r.d(b + "R." + c[d]);
+      p[k].x_nopbc[d] = p[k].x[d];
+      // v and omega are read, then set to 0.0*RA() (init_particle.cxx:361,372)
+      (void)r.d(b + "v." + c[d]);
+      (void)r.d(b + "omega." + c[d]);
+    }
+    double q[4] = {r.d(b + "q.q0"), r.d(b + "q.q1"), r.d(b + "q.q2"), r.d(b + "q.q3")};
+    if (r.rc) return r.rc;
+    if (user_q) {  // qtn_normalize (quaternion.h:237-245)
+      const double norm = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+      if (!(norm > 0.0)) return fail(SPM_ERR_INVALID, "zero quaternion in Particles[" + std::to_string(k) + "]");
+      const double inv = 1.0 / norm;
+      for (double &x : q) x *= inv;
+    } else {
+      q[0] = 1.0;
+      q[1] = q[2] = q[3] = 0.0;
+    }
+    p[k].q.s = q[0];
+    p[k].q.v[0] = q[1];
+    p[k].q.v[1] = q[2];
+    p[k].q.v[2] = q[3];
+    p[k].q_old = p[k].q;
+  }
+  return SPM_OK;
+}

@@ -1,0 +1,293 @@
+"""UDF reader and record writer (include/spm_udf.h), on the reference's own
+input files (tests/golden/udf: Udf/define.udf, Udf/input.udf and inputs from
+Examples/01, 02, 03, 09, 10, copied unchanged).  CPU only: the reader is
+host code.
+
+Expected values are the numbers written in those files; derived values
+(the auto time step of input.cxx:316-362) are recomputed here in closed
+form.
+"""
+import math
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from kapsel_old_amd import abi
+from kapsel_old_amd.spm import SpmError
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "udf")
+FIXTURES = ["input.udf", "colloid_1.udf", "colloid_32.udf", "sample1.udf", "gravity.udf", "repulsive.udf",
+            "tmp_shear.udf", "squirm_single_a+2.udf", "input_rigid_free1.udf"]
+
+
+@pytest.fixture(scope="module")
+def udf():
+    from kapsel_old_amd.spm import LIB_PATH
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("libspm.so not built")
+    from kapsel_old_amd import udf as m
+    return m
+
+
+def path(name):
+    return os.path.join(GOLDEN, name)
+
+
+def auto_dt(n, nu, factor=1.0, diffusion=None, ac_frequency=None, dx=1.0):
+    """DT = factor * Tdump with Tdump = 1/(NU*KMAX2) (input.cxx:316-362),
+    KMAX2 = sum over axes of (2 pi TRN / L)^2, TRN = (N+2)//3, L = N*DX;
+    n is N or a tuple (NX, NY, NZ)."""
+    ns = (n, n, n) if isinstance(n, int) else n
+    kmax2 = sum((2.0 * math.pi * ((m + 2) // 3) / (m * dx)) ** 2 for m in ns)
+    t = 1.0 / (nu * kmax2)
+    if diffusion is not None:  # kBT * Onsager
+        t = min(t, 1.0 / (diffusion * kmax2))
+    if ac_frequency is not None:
+        t = min(t, 1e-2 / ac_frequency)
+    return factor * t
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_every_fixture_parses(udf, name):
+    with udf.UDF(path(name)) as u:
+        assert u.get_string("object_type.type") in ("spherical_particle", "chain", "rigid")
+        assert u.get_double("A") > 0
+        assert u.size("object_type.%s" % {"spherical_particle": "spherical_particle.Particle_spec",
+                                          "chain": "chain.Chain_spec",
+                                          "rigid": "rigid.Rigid_spec"}[u.get_string("object_type.type")]) >= 1
+
+
+def test_input_udf_values(udf):
+    with udf.UDF(path("input.udf")) as u:
+        assert u.get_string("constitutive_eq.type") == "Navier_Stokes"
+        assert u.get_double("constitutive_eq.Shear_Navier_Stokes.External_field.DC.Shear_rate") == 0.1
+        assert u.get_double("constitutive_eq.Electrolyte.Add_salt.salt.Valency_negative_ion") == -1.0
+        assert u.get_string("constitutive_eq.Electrolyte.Electric_field.ON.type") == "DC"
+        assert u.get_double("constitutive_eq.Electrolyte.Electric_field.ON.AC.Frequency") == 0.1
+        assert u.size("object_type.spherical_particle.Particle_spec[]") == 2
+        assert u.size("object_type.spherical_particle.Particle_spec") == 2
+        assert u.get_double("object_type.spherical_particle.Particle_spec[1].MASS_RATIO") == 0.5
+        np.testing.assert_array_equal(
+            u.get_doubles("object_type.spherical_particle.Particle_spec[].Particle_number"), [5, 5])
+        assert u.get_string("object_type.spherical_particle.Particle_spec[0].janus_axis") == "NONE"
+        assert u.get_int("mesh.NPY") == 5
+        assert u.get_string("LJ_powers") == "36:18"
+        assert u.get_string("gravity.G_direction") == "-Y"
+        assert u.size("switch.INIT_distribution.user_specify.Particles[]") == 2
+        assert u.get_double("switch.INIT_distribution.user_specify.Particles[1].R.x") == 4.0
+        np.testing.assert_array_equal(u.get_doubles("switch.INIT_distribution.user_specify.Particles[].R.y"),
+                                      [16.0, 16.0])
+        assert u.get_int("switch.INIT_distribution.random_walk.iteration") == 100
+        assert u.get_string("switch.INIT_orientation") == "space_align"
+        assert u.get_double("switch.SLIP_tol") == 1e-2
+        assert u.get_int("switch.SLIP_iter") == 4
+        assert u.size("switch.pin.YES.pin[]") == 1 and u.get_int("switch.pin.YES.pin[0]") == 2
+        assert u.size("switch.pin.YES.pin_rot[]") == 0
+        assert u.get_int("output.GTS") == 10 and u.get_int("output.Num_snap") == 100
+        assert u.get_string("output.ON.FileType") == "BINARY"
+        assert u.get_string("output.UDF") == "ON"
+        assert u.get_string("resume.Calculation") == "NEW"
+        assert u.get_int("resume.CONTINUE.Saved_Data.jikan.ts") == 0
+        assert u.totalRecord() == 0
+
+
+def test_params_navier_stokes(udf):
+    with udf.UDF(path("input.udf")) as u:
+        prm, el, run = u.params()
+    assert el is None and run.equation == udf.NAVIER_STOKES
+    assert (prm.nx, prm.ny, prm.nz) == (32, 32, 32)
+    assert (prm.dx, prm.rho, prm.eta, prm.kbt, prm.alpha_v, prm.alpha_o) == (1.0, 1.0, 1.0, 0.0, 1.0, 1.0)
+    assert (prm.a, prm.a_xi, prm.epsilon, prm.g) == (5.0, 2.0, 1.0, 1.0)
+    assert prm.g_direction == 1 and prm.lj_powers == abi.LJ_36_18 and prm.lj_truncate == abi.LJ_WCA
+    assert prm.n_particles == 10 and prm.n_species == 2 and list(prm.mass_ratio[:2]) == [1.5, 0.5]
+    assert prm.rotation == 1 and list(prm.fix_cell) == [1, 1, 1]
+    assert run.auto_dt == 1 and run.dt_factor == 1.0
+    assert run.gts == 10 and run.num_snap == 100
+    assert udf.DISTRIBUTIONS[run.distribution] == "uniform_random"
+    assert prm.dt == pytest.approx(auto_dt(32, 1.0), rel=1e-14)
+    assert prm.precision == abi.FP64
+
+
+def test_params_large_box(udf):
+    with udf.UDF(path("gravity.udf")) as u:
+        prm, el, run = u.params()
+    assert (prm.nx, prm.ny, prm.nz) == (256, 256, 512)
+    assert (prm.dx, prm.rho, prm.eta) == (0.1, 1.0, 5.0)
+    assert prm.n_particles == 3204 and prm.g == 980.665 and prm.g_direction == 2
+    assert prm.dt == pytest.approx(auto_dt((256, 256, 512), 5.0, dx=0.1), rel=1e-14)
+
+
+def test_params_electrolyte(udf):
+    with udf.UDF(path("sample1.udf")) as u:
+        prm, el, run = u.params()
+    assert run.equation == udf.ELECTROLYTE and el is not None
+    assert (prm.rho, prm.eta, prm.kbt) == (0.1, 0.5, 1.0)
+    # the Electrolyte branch leaves alpha_v / alpha_o at zero
+    assert prm.alpha_v == 0.0 and prm.alpha_o == 0.0
+    assert el.n_spec == 2 and list(el.valency) == [1.0, -1.0] and list(el.onsager) == [7.24, 7.24]
+    assert el.debye_length == 10.0 and el.dielectric == 1.0 and el.elementary_charge == 1.0
+    assert el.surface_charge[0] == -100.0
+    assert el.external_field == 1 and el.ac == 0 and list(el.e_ext) == [0.1, 0.0, 0.0]
+    assert run.poisson_boltzmann == 0
+    assert list(prm.fix_cell) == [0, 0, 0]
+    # diffusion time 1/(kBT*Onsager*KMAX2) is the shorter one here
+    assert prm.dt == pytest.approx(auto_dt(64, 5.0, diffusion=7.24), rel=1e-14)
+    assert prm.dt < auto_dt(64, 5.0)
+    with udf.UDF(path("colloid_1.udf")) as u:
+        prm, el, run = u.params()
+    assert run.poisson_boltzmann == 1 and prm.dt == pytest.approx(auto_dt(64, 1.0, diffusion=1.0), rel=1e-14)
+
+
+@pytest.mark.parametrize("name,what", [("tmp_shear.udf", "Shear_Navier_Stokes"),
+                                       ("input_rigid_free1.udf", "rigid"),
+                                       ("squirm_single_a+2.udf", "SQUIRMER")])
+def test_params_rejects_what_is_not_run(udf, name, what):
+    with udf.UDF(path(name)) as u:
+        with pytest.raises(SpmError) as e:
+            u.params()
+    assert e.value.status == abi.SPM_ERR_UNSUPPORTED and what in str(e.value)
+
+
+def test_user_particles_fixture(udf):
+    with udf.UDF(path("sample1.udf")) as u:
+        p = u.user_particles()
+    assert len(p) == 1
+    assert list(p[0]["x"]) == [32.0, 32.0, 32.0] and list(p[0]["x_nopbc"]) == [32.0, 32.0, 32.0]
+    assert p[0]["spec"] == 0 and p[0]["q"]["s"] == 1.0 and p[0]["q_old"]["s"] == 1.0
+    with udf.UDF(path("input.udf")) as u:  # uniform_random, not user_specify
+        with pytest.raises(SpmError):
+            u.user_particles()
+
+
+def _user_file(tmp_path, rotation, orientation):
+    """sample1.udf with three particles of two species, nonzero v and omega,
+    unnormalized q."""
+    shutil.copy(path("define.udf"), tmp_path / "define.udf")
+    src = open(path("sample1.udf")).read()
+    spec_old = '[{1,1.10000000000000,-100.000000000000,"NONE","OFF",{0.0,0.0,0.0}{0.0,0.0,0.0}0.0,0.0}]'
+    spec_new = ('[{2,1.10000000000000,-100.000000000000,"NONE","OFF",{0.0,0.0,0.0}{0.0,0.0,0.0}0.0,0.0}'
+                '{1,2.5,0.0,"NONE","OFF",{0.0,0.0,0.0}{0.0,0.0,0.0}0.0,0.0}]')
+    parts_old = "[{{32.000000,32.000000,32.000000}{0.0,0.0,0.0}{1.0000000,0.0,0.0,0.0}{0.0,0.0,0.0}}]"
+    parts_new = ("[{{1.5,2.5,3.5}{0.1,0.2,0.3}{2.0,0.0,0.0,0.0}{0.4,0.5,0.6}}"
+                 " {{10.0,20.0,30.0}{-0.1,0,0}{1,1,1,1}{0,0,1}}"
+                 " {{40.25,50.5,60.75}{0,0,0}{0.0,3.0,4.0,0.0}{0,0,0}}]")
+    sw_old = ' "ON",\n "ON",\n {\n  "user_specify"'
+    assert spec_old in src and parts_old in src and sw_old in src and '"space_align"' in src
+    src = src.replace(spec_old, spec_new).replace(parts_old, parts_new)
+    src = src.replace(sw_old, ' "%s",\n "ON",\n {\n  "user_specify"' % rotation)
+    src = src.replace('"space_align"', '"%s"' % orientation)
+    f = tmp_path / ("user_%s_%s.udf" % (rotation, orientation))
+    f.write_text(src)
+    return str(f)
+
+
+@pytest.mark.parametrize("rotation,orientation", [("ON", "user_specify"), ("OFF", "user_specify"),
+                                                  ("ON", "space_align")])
+def test_user_particles_init(udf, tmp_path, rotation, orientation):
+    f = _user_file(tmp_path, rotation, orientation)
+    with udf.UDF(f) as u:
+        prm, el, run = u.params()
+        p = u.user_particles()
+    assert prm.n_particles == 3 and prm.n_species == 2 and prm.rotation == (rotation == "ON")
+    assert list(p["spec"]) == [0, 0, 1]
+    np.testing.assert_array_equal(p["x"], [[1.5, 2.5, 3.5], [10, 20, 30], [40.25, 50.5, 60.75]])
+    np.testing.assert_array_equal(p["x_nopbc"], p["x"])
+    # Init_Particle reads v and omega, then overwrites both with 0.0*RA()
+    assert not p["v"].any() and not p["omega"].any()
+    q = np.stack([p["q"]["s"], *p["q"]["v"].T], axis=1)
+    if rotation == "ON" and orientation == "user_specify":
+        # qtn_normalize scales by the reciprocal norm (quaternion.h:237-245):
+        # 3 * (1/5) is 0.6000000000000001, not 3/5
+        raw = np.array([[2.0, 0, 0, 0], [1, 1, 1, 1], [0, 3, 4, 0]])
+        norm = np.sqrt((raw * raw).sum(axis=1))
+        np.testing.assert_array_equal(q, raw * (1.0 / norm)[:, None])
+        np.testing.assert_allclose(q, [[1, 0, 0, 0], [0.5, 0.5, 0.5, 0.5], [0, 0.6, 0.8, 0]], rtol=0, atol=2e-16)
+    else:
+        np.testing.assert_array_equal(q, [[1, 0, 0, 0]] * 3)
+    assert (p["q_old"] == p["q"]).all()
+
+
+def test_user_particles_random_orientation_rejected(udf, tmp_path):
+    f = _user_file(tmp_path, "ON", "random")
+    with udf.UDF(f) as u:
+        with pytest.raises(SpmError) as e:
+            u.user_particles()
+    assert e.value.status == abi.SPM_ERR_UNSUPPORTED
+
+
+def test_records_round_trip(udf, tmp_path):
+    out = tmp_path / "output.udf"
+    udf.create_output(path("input.udf"), out)
+    text = out.read_text()
+    assert "\\include" not in text and "\\begin{global_def}" in text
+    rng = np.random.default_rng(5)
+    snaps = []
+    for ts in (0, 10, 20):
+        p = abi.new_particles(4)
+        for k in ("x", "x_nopbc", "v", "omega", "f_hydro_previous", "torque_hydro_previous", "fr_previous",
+                  "f_slip_previous", "torque_slip_previous"):
+            p[k] = rng.standard_normal((4, 3)) * 10.0 ** rng.integers(-12, 12, (4, 3))
+        p["q"]["s"] = rng.standard_normal(4)
+        p["q"]["v"] = rng.standard_normal((4, 3))
+        p["fr"] = 7.0  # not written: f_r is fr_previous
+        udf.append_record(out, ts, p)
+        snaps.append((ts, p))
+    with udf.UDF(out) as u:
+        assert u.totalRecord() == 3
+        assert u.get_int("mesh.NPX") == 5  # the data section is still there
+        for r, (ts, p) in enumerate(snaps):
+            u.jump(r)
+            assert u.get_double("E") == 1.0 and u.get_int("t") == ts
+            assert u.size("Particles[]") == 4 and u.size("RigidParticles[]") == 0
+            pairs = [("R", "x"), ("R_raw", "x_nopbc"), ("v", "v"), ("omega", "omega"),
+                     ("f_hydro", "f_hydro_previous"), ("torque_hydro", "torque_hydro_previous"),
+                     ("f_r", "fr_previous"), ("f_slip", "f_slip_previous"),
+                     ("torque_slip", "torque_slip_previous")]
+            for name, field in pairs:
+                for d, c in enumerate("xyz"):
+                    np.testing.assert_array_equal(u.get_doubles("Particles[].%s.%s" % (name, c)), p[field][:, d])
+            np.testing.assert_array_equal(u.get_doubles("Particles[].torque_r.y"), np.zeros(4))
+            np.testing.assert_array_equal(u.get_doubles("Particles[].q.q0"), p["q"]["s"])
+            np.testing.assert_array_equal(u.get_doubles("Particles[].q.q3"), p["q"]["v"][:, 2])
+            assert u.get_double("Particles[2].R_raw.z") == p["x_nopbc"][2, 2]
+        u.jump(-1)
+        with pytest.raises(SpmError):
+            u.get_double("t")
+        with pytest.raises(SpmError):
+            u.jump(3)
+
+
+def _write(tmp_path, name, data):
+    shutil.copy(path("define.udf"), tmp_path / "define.udf")
+    f = tmp_path / name
+    f.write_text('\\include{"define.udf"}\n\\begin{data}\n' + data + "\n\\end{data}\n")
+    return str(f)
+
+
+def test_reader_errors(udf, tmp_path):
+    with pytest.raises(SpmError):
+        udf.UDF(tmp_path / "missing.udf")
+    with pytest.raises(SpmError):  # unterminated record
+        udf.UDF(_write(tmp_path, "a.udf", "mesh:{5,5,5"))
+    with pytest.raises(SpmError):  # not in the schema
+        udf.UDF(_write(tmp_path, "b.udf", "no_such_entry:1"))
+    with pytest.raises(SpmError):  # not a number
+        udf.UDF(_write(tmp_path, "c.udf", "A:five"))
+    with pytest.raises(SpmError):  # include that does not resolve
+        f = tmp_path / "d.udf"
+        f.write_text('\\include{"nowhere.udf"}\n')
+        udf.UDF(f)
+    with udf.UDF(_write(tmp_path, "e.udf", "A:2.5 mesh:{5,6,7} LJ_powers:'12:6'")) as u:
+        assert u.get_double("A") == 2.5 and u.get_int("mesh.NPZ") == 7 and u.get_string("LJ_powers") == "12:6"
+        with pytest.raises(SpmError):
+            u.get_int("A")  # 2.5 is not an integer
+        with pytest.raises(SpmError):
+            u.get_double("mesh")  # a record, not a number
+        with pytest.raises(SpmError):
+            u.get_double("mesh.NPW")
+        with pytest.raises(SpmError):
+            u.get_double("A_XI")  # declared but given no value
+        with pytest.raises(SpmError):
+            u.params()  # incomplete input
