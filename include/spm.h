@@ -224,6 +224,20 @@ int spm_fused_stage(spm_ctx *ctx, int stage);
 /* Scans u and zeta for NaN/Inf; SPM_ERR_NONFINITE if any is found. */
 int spm_check_finite(spm_ctx *ctx);
 
+/* ---- checkpoint / resume ------------------------------------------------
+ * Replace Save_Restart_udf / Force_restore_parameters (resume.cxx:10-364)
+ * for exact resumption: the file holds zeta0/zeta1 in the context's own
+ * precision, uk_dc, every particle record, ts and time, the MT19937 state
+ * (which the reference's restart file does not keep) and, for an
+ * electrolyte, the ion spectra and their totals, with a hash over all of
+ * it.  A context created with the same spm_params (and spm_set_electrolyte)
+ * that loads the file continues bit for bit like the one that saved it.
+ * spm_load_state rejects a file from another grid, precision, particle
+ * count, ion count or time step, or a damaged one, and then leaves the
+ * context unchanged.  spm_save_state writes PATH.tmp and renames it. */
+int spm_save_state(spm_ctx *ctx, const char *path);
+int spm_load_state(spm_ctx *ctx, const char *path);
+
 /* ---- measurement (no counterpart in the reference) --------------------
  * With timing on, every kernel launch is bracketed by HIP events on the
  * context's stream and attributed to a kernel family together with the

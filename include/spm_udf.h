@@ -95,6 +95,37 @@ int spm_udf_params(const spm_udf *udf, spm_params *prm, spm_electrolyte *el, spm
  * length.  INIT_orientation random with ROTATION ON is SPM_ERR_UNSUPPORTED. */
 int spm_udf_user_particles(const spm_udf *udf, spm_particle *p, int n);
 
+/* Restart data (resume.CONTINUE.Saved_Data of Udf/define.udf).  The caller
+ * sets the sizes and owns the buffers: zeta0/zeta1 hold nx*ny*2*(nz/2+1)
+ * values in the field layout, concentration n_ion times as many (ion
+ * spectra, n_ion 0 for Navier_Stokes), particles n_particles records. */
+typedef struct spm_restart {
+  int nx, ny, nz;
+  int n_particles;
+  int n_ion;
+  int ts;                /* jikan.ts   */
+  double time;           /* jikan.time */
+  double uk_dc[3];
+  double degree_oblique; /* oblique.degree_oblique (0 without shear) */
+  double *zeta0, *zeta1;
+  double *concentration;
+  spm_particle *particles;
+} spm_restart;
+
+/* Save_Restart_udf (resume.cxx:10-78, Save_Particle_udf): writes the input
+ * file's text (includes inlined, data section re-serialized) with
+ * resume.Calculation = "CONTINUE" and the saved data, every number with 17
+ * significant digits.  A file of text: for small boxes; spm_save_state is
+ * the exact, compact checkpoint. */
+int spm_udf_write_restart(const char *in_path, const char *out_path, const spm_restart *r);
+/* Force_restore_parameters (resume.cxx:274-364, Read_Particle_udf): fills
+ * r from resume.CONTINUE.Saved_Data of a file whose resume.Calculation is
+ * "CONTINUE".  Per particle: the 17 saved vectors, q and q_old normalized,
+ * x_previous = x_nopbc = R, v_slip, omega_slip and the mass / inertia
+ * temporaries zero; other members keep the caller's values.  The sizes in r
+ * must match the file's arrays; on any error r is left unchanged. */
+int spm_udf_read_restart(const spm_udf *udf, spm_restart *r);
+
 #ifdef __cplusplus
 }
 #endif

@@ -941,3 +941,295 @@ extern "C" int spm_udf_user_particles(const spm_udf *u, spm_particle *p, int n) 
   }
   return SPM_OK;
 }
+
+// ---------------------------------------------------------------------
+// restart file: Save_Restart_udf / Force_restore_parameters
+// (resume.cxx:10-78, 274-364; Read_Particle_udf and Save_Particle_udf)
+// ---------------------------------------------------------------------
+namespace {
+
+void put_num(std::string &o, double v) {
+  char b[32];
+  std::snprintf(b, sizeof(b), "%.17g", v);
+  o += b;
+}
+
+void serialize(std::string &o, const Value &v) {
+  switch (v.kind) {
+    case K_INT: {
+      char b[32];
+      std::snprintf(b, sizeof(b), "%.0f", v.num);
+      o += b;
+      break;
+    }
+    case K_REAL:
+      put_num(o, v.num);
+      break;
+    case K_STRING:
+    case K_SELECT:
+      o += '"';
+      o += v.str;
+      o += '"';
+      break;
+    case K_RECORD:
+      o += '{';
+      for (size_t k = 0; k < v.fields.size(); ++k) {
+        if (k) o += ',';
+        serialize(o, v.fields[k].second);
+      }
+      o += '}';
+      break;
+    case K_ARRAY:
+      o += '[';
+      for (size_t k = 0; k < v.items.size(); ++k) {
+        if (k) o += ',';
+        serialize(o, v.items[k]);
+      }
+      o += ']';
+      break;
+    default:
+      break;
+  }
+}
+
+// sParticle member order (Udf/define.udf restart schema)
+const char *const kSaved3[] = {"R", "v", "v_old", "f_hydro", "f_hydro_previous", "f_hydro1", "f_slip",
+                               "f_slip_previous", "fr", "fr_previous", "omega", "omega_old", "torque_hydro",
+                               "torque_hydro_previous", "torque_hydro1", "torque_slip", "torque_slip_previous"};
+
+const double *saved3(const spm_particle &p, int k) {
+  const double *const f[] = {p.x, p.v, p.v_old, p.f_hydro, p.f_hydro_previous, p.f_hydro1, p.f_slip,
+                             p.f_slip_previous, p.fr, p.fr_previous, p.omega, p.omega_old, p.torque_hydro,
+                             p.torque_hydro_previous, p.torque_hydro1, p.torque_slip, p.torque_slip_previous};
+  return f[k];
+}
+
+double *saved3(spm_particle &p, int k) { return const_cast<double *>(saved3(static_cast<const spm_particle &>(p), k)); }
+
+int check_restart(const spm_restart *r) {
+  if (!r || r->nx <= 0 || r->ny <= 0 || r->nz <= 0 || r->n_particles < 0 || r->n_ion < 0 || r->n_ion > 2)
+    return fail(SPM_ERR_INVALID, "invalid spm_restart sizes");
+  if (!r->zeta0 || !r->zeta1 || (r->n_particles > 0 && !r->particles) || (r->n_ion > 0 && !r->concentration))
+    return fail(SPM_ERR_INVALID, "spm_restart buffer missing");
+  return SPM_OK;
+}
+
+const Value *member(const Value *v, const char *name) {
+  if (!v || v->kind != K_RECORD) return nullptr;
+  for (auto &f : v->fields)
+    if (f.first == name) return &f.second;
+  return nullptr;
+}
+
+bool num3(const Value *v, double *out) {
+  const char *c[3] = {"x", "y", "z"};
+  for (int d = 0; d < 3; ++d) {
+    const Value *x = member(v, c[d]);
+    if (!x || (x->kind != K_REAL && x->kind != K_INT)) return false;
+    out[d] = x->num;
+  }
+  return true;
+}
+
+bool quat(const Value *v, double *out) {
+  const char *c[4] = {"q0", "q1", "q2", "q3"};
+  for (int d = 0; d < 4; ++d) {
+    const Value *x = member(v, c[d]);
+    if (!x || (x->kind != K_REAL && x->kind != K_INT)) return false;
+    out[d] = x->num;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int spm_udf_write_restart(const char *in_path, const char *out_path, const spm_restart *r) {
+  if (!in_path || !out_path) return fail(SPM_ERR_INVALID, "null argument");
+  if (int rc = check_restart(r)) return rc;
+  spm_udf *u = nullptr;
+  if (int rc = spm_udf_open(in_path, &u)) return rc;
+  std::unique_ptr<spm_udf> hold(u);
+  bool has_resume = false;
+  for (auto &e : u->doc.entries) has_resume |= e.first == "resume";
+  if (!has_resume) return fail(SPM_ERR_INVALID, "the schema has no resume entry");
+  std::string text, err;
+  if (!inline_includes(in_path, text, err)) return fail(SPM_ERR_INVALID, err);
+  // the data section proper, not the one inside \begin{header}
+  const size_t hdr = text.find("\\end{header}");
+  const size_t b = text.find("\\begin{data}", hdr == std::string::npos ? 0 : hdr);
+  if (b == std::string::npos) return fail(SPM_ERR_INVALID, "no data section");
+  std::string o = text.substr(0, b);
+  o += "\\begin{data}\n";
+  for (auto &e : u->doc.entries) {
+    if (e.first == "resume") continue;
+    auto it = u->doc.data.find(e.first);
+    if (it == u->doc.data.end()) continue;
+    o += e.first + ":";
+    serialize(o, it->second);
+    o += '\n';
+  }
+  // resume:{"CONTINUE",{Saved_Data:{jikan, Particles[], GR_body[], Zeta[][][],
+  //         uk_dc, Concentration[][][][], oblique}}}
+  o += "resume:{\"CONTINUE\",{{\n{" + std::to_string(r->ts) + ",";
+  put_num(o, r->time);
+  o += "}\n[";
+  for (int j = 0; j < r->n_particles; ++j) {
+    const spm_particle &p = r->particles[j];
+    o += j ? ",\n{" : "\n{";
+    for (int k = 0; k < 17; ++k) put3(o, saved3(p, k));
+    char qb[128];
+    std::snprintf(qb, sizeof(qb), "{%.17g,%.17g,%.17g,%.17g}", p.q.s, p.q.v[0], p.q.v[1], p.q.v[2]);
+    o += qb;
+    std::snprintf(qb, sizeof(qb), "{%.17g,%.17g,%.17g,%.17g}", p.q_old.s, p.q_old.v[0], p.q_old.v[1],
+                  p.q_old.v[2]);
+    o += qb;
+    o += '}';
+  }
+  o += "]\n[]\n[";
+  const int nz_ = 2 * (r->nz / 2 + 1);
+  for (int i = 0; i < r->nx; ++i) {
+    o += i ? ",\n[" : "\n[";
+    for (int j = 0; j < r->ny; ++j) {
+      o += j ? ",[" : "[";
+      for (int k = 0; k < nz_; ++k) {
+        const size_t im = ((size_t)i * r->ny + j) * nz_ + k;
+        o += k ? ",{" : "{";
+        put_num(o, r->zeta0[im]);
+        o += ',';
+        put_num(o, r->zeta1[im]);
+        o += '}';
+      }
+      o += ']';
+    }
+    o += ']';
+  }
+  o += "]\n";
+  put3(o, r->uk_dc);
+  o += "\n[";
+  const size_t nf = (size_t)r->nx * r->ny * nz_;
+  for (int n = 0; n < r->n_ion; ++n) {
+    o += n ? ",[" : "[";
+    for (int i = 0; i < r->nx; ++i) {
+      o += i ? ",\n[" : "\n[";
+      for (int j = 0; j < r->ny; ++j) {
+        o += j ? ",[" : "[";
+        for (int k = 0; k < nz_; ++k) {
+          o += k ? ",{" : "{";
+          put_num(o, r->concentration[n * nf + ((size_t)i * r->ny + j) * nz_ + k]);
+          o += '}';
+        }
+        o += ']';
+      }
+      o += ']';
+    }
+    o += ']';
+  }
+  o += "]\n{";
+  put_num(o, r->degree_oblique);
+  o += "}\n}}}\n\\end{data}\n";
+  const std::string tmp = std::string(out_path) + ".tmp";
+  {
+    std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+    if (!f) return fail(SPM_ERR_INVALID, "cannot write " + tmp);
+    f << o;
+    f.close();
+    if (!f) return fail(SPM_ERR_INVALID, "write failed on " + tmp);
+  }
+  if (std::rename(tmp.c_str(), out_path) != 0) return fail(SPM_ERR_INVALID, std::string("cannot rename to ") + out_path);
+  return SPM_OK;
+}
+
+extern "C" int spm_udf_read_restart(const spm_udf *u, spm_restart *r) {
+  if (!u) return fail(SPM_ERR_INVALID, "null argument");
+  if (int rc = check_restart(r)) return rc;
+  char calc[16] = {0};
+  if (spm_udf_get_string(u, "resume.Calculation", calc, sizeof(calc)) != SPM_OK || std::strcmp(calc, "CONTINUE"))
+    return fail(SPM_ERR_INVALID, "resume.Calculation is not CONTINUE");
+  const Value *sd = find(u->doc, -1, "resume.CONTINUE.Saved_Data");
+  const Value *jikan = member(sd, "jikan");
+  const Value *parts = member(sd, "Particles");
+  const Value *zeta = member(sd, "Zeta");
+  const Value *ukdc = member(sd, "uk_dc");
+  const Value *conc = member(sd, "Concentration");
+  const Value *obl = member(sd, "oblique");
+  const Value *ts = member(jikan, "ts"), *tm = member(jikan, "time"), *deg = member(obl, "degree_oblique");
+  if (!ts || !tm || !parts || !zeta || !ukdc || !conc || !deg)
+    return fail(SPM_ERR_INVALID, "resume.CONTINUE.Saved_Data is incomplete");
+  const int nz_ = 2 * (r->nz / 2 + 1);
+  // sizes first, so that a mismatch leaves the caller's buffers untouched
+  if ((int)parts->items.size() != r->n_particles) return fail(SPM_ERR_INVALID, "Particles[] size differs");
+  if ((int)zeta->items.size() != r->nx) return fail(SPM_ERR_INVALID, "Zeta[] size differs from nx");
+  for (auto &a : zeta->items) {
+    if ((int)a.items.size() != r->ny) return fail(SPM_ERR_INVALID, "Zeta[][] size differs from ny");
+    for (auto &b : a.items)
+      if ((int)b.items.size() != nz_) return fail(SPM_ERR_INVALID, "Zeta[][][] size differs from 2*(nz/2+1)");
+  }
+  if ((int)conc->items.size() != r->n_ion) return fail(SPM_ERR_INVALID, "Concentration[] size differs");
+  for (auto &n : conc->items) {
+    if ((int)n.items.size() != r->nx) return fail(SPM_ERR_INVALID, "Concentration[][] size differs");
+    for (auto &a : n.items) {
+      if ((int)a.items.size() != r->ny) return fail(SPM_ERR_INVALID, "Concentration[][][] size differs");
+      for (auto &b : a.items)
+        if ((int)b.items.size() != nz_) return fail(SPM_ERR_INVALID, "Concentration[][][][] size differs");
+    }
+  }
+  // particles (Read_Particle_udf, resume.cxx:366-460)
+  std::vector<spm_particle> ps(r->particles, r->particles + r->n_particles);
+  for (int j = 0; j < r->n_particles; ++j) {
+    const Value *pv = &parts->items[j];
+    spm_particle &p = ps[j];
+    for (int k = 0; k < 17; ++k)
+      if (!num3(member(pv, kSaved3[k]), saved3(p, k))) return fail(SPM_ERR_INVALID, "bad Particles[] entry");
+    double q[4], qo[4];
+    if (!quat(member(pv, "q"), q) || !quat(member(pv, "q_old"), qo))
+      return fail(SPM_ERR_INVALID, "bad Particles[] quaternion");
+    for (double *a : {q, qo}) {  // qtn_normalize (quaternion.h:237-245)
+      const double norm = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3]);
+      if (!(norm > 0.0)) return fail(SPM_ERR_INVALID, "zero quaternion in Particles[]");
+      const double inv = 1.0 / norm;
+      for (int d = 0; d < 4; ++d) a[d] *= inv;
+    }
+    p.q.s = q[0];
+    p.q_old.s = qo[0];
+    for (int d = 0; d < 3; ++d) {
+      p.q.v[d] = q[d + 1];
+      p.q_old.v[d] = qo[d + 1];
+      p.x_previous[d] = p.x[d];
+      p.x_nopbc[d] = p.x[d];
+      p.v_slip[d] = 0.0;
+      p.omega_slip[d] = 0.0;
+      p.mass_center[d] = p.surface_mass_center[d] = p.surface_dv[d] = p.surface_dw[d] = 0.0;
+      for (int l = 0; l < 3; ++l) p.inertia[d][l] = p.surface_inertia[d][l] = 0.0;
+    }
+    p.mass = p.surface_mass = 0.0;
+  }
+  double dc[3];
+  if (!num3(ukdc, dc)) return fail(SPM_ERR_INVALID, "bad uk_dc");
+  // fields (resume.cxx:280-301)
+  const size_t nf = (size_t)r->nx * r->ny * nz_;
+  std::vector<double> z0(nf), z1(nf), cc((size_t)r->n_ion * nf);
+  for (int i = 0; i < r->nx; ++i)
+    for (int j = 0; j < r->ny; ++j)
+      for (int k = 0; k < nz_; ++k) {
+        const Value &e = zeta->items[i].items[j].items[k];
+        const Value *a = member(&e, "zeta0"), *b = member(&e, "zeta1");
+        if (!a || !b || a->kind != K_REAL || b->kind != K_REAL) return fail(SPM_ERR_INVALID, "bad Zeta entry");
+        const size_t im = ((size_t)i * r->ny + j) * nz_ + k;
+        z0[im] = a->num;
+        z1[im] = b->num;
+        for (int n = 0; n < r->n_ion; ++n) {
+          const Value *ck = member(&conc->items[n].items[i].items[j].items[k], "ck");
+          if (!ck || ck->kind != K_REAL) return fail(SPM_ERR_INVALID, "bad Concentration entry");
+          cc[n * nf + im] = ck->num;
+        }
+      }
+  std::copy(ps.begin(), ps.end(), r->particles);
+  std::copy(z0.begin(), z0.end(), r->zeta0);
+  std::copy(z1.begin(), z1.end(), r->zeta1);
+  if (r->n_ion > 0) std::copy(cc.begin(), cc.end(), r->concentration);
+  for (int d = 0; d < 3; ++d) r->uk_dc[d] = dc[d];
+  r->ts = (int)ts->num;
+  r->time = tm->num;
+  r->degree_oblique = deg->num;
+  return SPM_OK;
+}

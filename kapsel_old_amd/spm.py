@@ -29,7 +29,7 @@ SYMBOLS = [
     "spm_synchronize", "spm_check_finite", "spm_set_schedule", "spm_timing_enable", "spm_timing_read", "spm_fused_stage",
     "spm_set_electrolyte", "spm_init_rho_ion", "spm_put_concentration", "spm_get_concentration",
     "spm_put_total_solute", "spm_get_total_solute", "spm_nssolute_solver_euler", "spm_rescale_solute",
-    "spm_make_coulomb_force",
+    "spm_make_coulomb_force", "spm_save_state", "spm_load_state",
 ]
 
 SCHEDULE_FUSED, SCHEDULE_LITERAL = 0, 1
@@ -90,6 +90,8 @@ def lib():
             getattr(L, name).argtypes = [vp]
         L.spm_put_concentration.argtypes = [vp, ip, dp]
         L.spm_get_concentration.argtypes = [vp, ip, dp]
+        L.spm_save_state.argtypes = [vp, C.c_char_p]
+        L.spm_load_state.argtypes = [vp, C.c_char_p]
         L.spm_put_total_solute.argtypes = [vp, dp]
         L.spm_get_total_solute.argtypes = [vp, dp]
         for name in SYMBOLS:
@@ -250,6 +252,13 @@ class SPM:
         mti = C.c_int()
         self._call("get_rng", mt, C.byref(mti))
         return np.array(mt, dtype=np.uint32), mti.value
+
+    def save_state(self, path):
+        """Exact checkpoint (spm_save_state), RNG state included."""
+        self._call("save_state", str(path).encode())
+
+    def load_state(self, path):
+        self._call("load_state", str(path).encode())
 
     def put_rng(self, mt, mti):
         arr = (C.c_uint32 * abi.MT_N)(*[int(x) for x in mt])

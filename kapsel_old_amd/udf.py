@@ -24,6 +24,14 @@ class SpmUdfRun(C.Structure):
                 ("poisson_boltzmann", C.c_int)]
 
 
+class SpmRestart(C.Structure):
+    """spm_restart: resume.CONTINUE.Saved_Data with caller-owned buffers."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("n_particles", C.c_int), ("n_ion", C.c_int),
+                ("ts", C.c_int), ("time", C.c_double), ("uk_dc", C.c_double * 3), ("degree_oblique", C.c_double),
+                ("zeta0", C.POINTER(C.c_double)), ("zeta1", C.POINTER(C.c_double)),
+                ("concentration", C.POINTER(C.c_double)), ("particles", C.c_void_p)]
+
+
 NAVIER_STOKES, ELECTROLYTE = 0, 3
 DISTRIBUTIONS = ["NONE", "uniform_random", "random_walk", "FCC", "BCC", "user_specify"]
 
@@ -50,6 +58,8 @@ def lib():
         L.spm_udf_user_particles.argtypes = [vp, vp, ip]
         L.spm_udf_create_output.argtypes = [cp, cp]
         L.spm_udf_append_record.argtypes = [cp, ip, vp, ip]
+        L.spm_udf_write_restart.argtypes = [cp, cp, C.POINTER(SpmRestart)]
+        L.spm_udf_read_restart.argtypes = [vp, C.POINTER(SpmRestart)]
         _bound = True
     return L
 
@@ -140,6 +150,52 @@ class UDF:
         p = abi.new_particles(n)
         _check(lib().spm_udf_user_particles(self._h, p.ctypes.data_as(C.c_void_p), n), "spm_udf_user_particles")
         return p
+
+
+def _restart_struct(shape, zeta0, zeta1, conc, particles):
+    nx, ny, nz = shape
+    r = SpmRestart()
+    r.nx, r.ny, r.nz = nx, ny, nz
+    r.n_particles = len(particles)
+    r.n_ion = 0 if conc is None else len(conc)
+    dp = C.POINTER(C.c_double)
+    r.zeta0 = zeta0.ctypes.data_as(dp)
+    r.zeta1 = zeta1.ctypes.data_as(dp)
+    r.concentration = conc.ctypes.data_as(dp) if conc is not None else None
+    r.particles = particles.ctypes.data_as(C.c_void_p)
+    return r
+
+
+def _field(a, shape):
+    nx, ny, nz = shape
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(nx, ny, 2 * (nz // 2 + 1))
+
+
+def write_restart(in_path, out_path, shape, zeta0, zeta1, uk_dc, particles, ts, time, conc=None,
+                  degree_oblique=0.0):
+    """Save_Restart_udf: the input with resume = CONTINUE and this state."""
+    z0, z1 = _field(zeta0, shape), _field(zeta1, shape)
+    cc = None if conc is None else np.ascontiguousarray(
+        [_field(c, shape) for c in conc], dtype=np.float64)
+    p = np.ascontiguousarray(particles, dtype=abi.PARTICLE_DTYPE)
+    r = _restart_struct(shape, z0, z1, cc, p)
+    r.ts, r.time, r.degree_oblique = int(ts), float(time), float(degree_oblique)
+    r.uk_dc[:] = [float(x) for x in uk_dc]
+    _check(lib().spm_udf_write_restart(str(in_path).encode(), str(out_path).encode(), C.byref(r)),
+           "spm_udf_write_restart")
+
+
+def read_restart(u, shape, n_particles, n_ion=0, particles=None):
+    """Force_restore_parameters: dict of the saved state of UDF `u`."""
+    nx, ny, nz = shape
+    z0 = np.zeros((nx, ny, 2 * (nz // 2 + 1)))
+    z1 = np.zeros_like(z0)
+    cc = np.zeros((n_ion,) + z0.shape) if n_ion else None
+    p = abi.new_particles(n_particles) if particles is None else np.array(particles, dtype=abi.PARTICLE_DTYPE)
+    r = _restart_struct(shape, z0, z1, cc, p)
+    _check(lib().spm_udf_read_restart(u._h, C.byref(r)), "spm_udf_read_restart")
+    return dict(zeta0=z0, zeta1=z1, concentration=cc, particles=p, uk_dc=np.array(r.uk_dc[:]), ts=r.ts,
+                time=r.time, degree_oblique=r.degree_oblique)
 
 
 def create_output(in_path, out_path):

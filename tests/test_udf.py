@@ -291,3 +291,89 @@ def test_reader_errors(udf, tmp_path):
             u.get_double("A_XI")  # declared but given no value
         with pytest.raises(SpmError):
             u.params()  # incomplete input
+
+
+def _random_state(rng, shape, n, n_ion):
+    nx, ny, nz = shape
+    fs = (nx, ny, 2 * (nz // 2 + 1))
+    z0 = rng.standard_normal(fs) * 10.0 ** rng.integers(-300, 300, fs)
+    z1 = rng.standard_normal(fs)
+    z1.flat[0] = -0.0
+    z1.flat[1] = 5e-324  # smallest subnormal
+    conc = rng.standard_normal((n_ion,) + fs) if n_ion else None
+    p = abi.new_particles(n)
+    for name in p.dtype.names:
+        if name in ("spec", "q", "q_old"):
+            continue
+        p[name] = rng.standard_normal(p[name].shape)
+    p["spec"] = np.arange(n) % 2
+    for q in ("q", "q_old"):
+        p[q]["s"] = rng.standard_normal(n)
+        p[q]["v"] = rng.standard_normal((n, 3))
+    return z0, z1, conc, p
+
+
+SAVED = ["x", "v", "v_old", "f_hydro", "f_hydro_previous", "f_hydro1", "f_slip", "f_slip_previous", "fr",
+         "fr_previous", "omega", "omega_old", "torque_hydro", "torque_hydro_previous", "torque_hydro1",
+         "torque_slip", "torque_slip_previous"]
+
+
+@pytest.mark.parametrize("name,n_ion", [("input.udf", 0), ("sample1.udf", 2)])
+def test_restart_round_trip(udf, tmp_path, name, n_ion):
+    shape = (8, 4, 6)
+    rng = np.random.default_rng(11)
+    z0, z1, conc, p = _random_state(rng, shape, 5, n_ion)
+    out = tmp_path / "restart.udf"
+    udf.write_restart(path(name), out, shape, z0, z1, (0.5, -1e-17, 3.0), p, 1234, 56.78125, conc=conc)
+    assert "\\include" not in out.read_text()
+    with udf.UDF(out) as u, udf.UDF(path(name)) as orig:
+        assert u.get_string("resume.Calculation") == "CONTINUE"
+        # every other entry of the data section is carried over
+        for loc in ("constitutive_eq.type", "LJ_powers", "switch.INIT_distribution.type", "output.ON.Out_dir"):
+            assert u.get_string(loc) == orig.get_string(loc)
+        for loc in ("A", "A_XI", "mesh.NPX", "gravity.G", "output.GTS", "time_increment.auto.factor",
+                    "object_type.spherical_particle.Particle_spec[0].MASS_RATIO", "switch.SLIP_tol"):
+            assert u.get_double(loc) == orig.get_double(loc)
+        keep = abi.new_particles(5)
+        keep["spec"] = [1, 0, 1, 0, 1]
+        keep["QR"] = 2.5
+        r = udf.read_restart(u, shape, 5, n_ion, particles=keep)
+    np.testing.assert_array_equal(r["zeta0"], z0)
+    np.testing.assert_array_equal(r["zeta1"], z1)
+    assert np.signbit(r["zeta1"].flat[0]) and r["zeta1"].flat[1] == 5e-324
+    if n_ion:
+        np.testing.assert_array_equal(r["concentration"], conc)
+    np.testing.assert_array_equal(r["uk_dc"], [0.5, -1e-17, 3.0])
+    assert r["ts"] == 1234 and r["time"] == 56.78125 and r["degree_oblique"] == 0.0
+    q = r["particles"]
+    for k in SAVED:
+        np.testing.assert_array_equal(q[k], p[k], err_msg=k)
+    np.testing.assert_array_equal(q["x_previous"], p["x"])
+    np.testing.assert_array_equal(q["x_nopbc"], p["x"])
+    assert not q["v_slip"].any() and not q["omega_slip"].any() and not q["inertia"].any()
+    assert not q["mass"].any() and not q["surface_dw"].any()
+    # members the restart does not hold keep the caller's values
+    assert list(q["spec"]) == [1, 0, 1, 0, 1] and (q["QR"] == 2.5).all()
+    for name_q in ("q", "q_old"):  # normalized with the reciprocal norm
+        raw = np.stack([p[name_q]["s"], *p[name_q]["v"].T], axis=1)
+        inv = 1.0 / np.sqrt((raw * raw).sum(axis=1))
+        got = np.stack([q[name_q]["s"], *q[name_q]["v"].T], axis=1)
+        np.testing.assert_allclose(got, raw * inv[:, None], rtol=1e-15, atol=0)
+
+
+def test_restart_rejects(udf, tmp_path):
+    shape = (4, 4, 4)
+    with udf.UDF(path("input.udf")) as u:  # resume.Calculation is NEW
+        with pytest.raises(SpmError):
+            udf.read_restart(u, shape, 0)
+    z0, z1, _, p = _random_state(np.random.default_rng(2), shape, 3, 0)
+    out = tmp_path / "r.udf"
+    udf.write_restart(path("input.udf"), out, shape, z0, z1, (0, 0, 0), p, 7, 0.5)
+    with udf.UDF(out) as u:
+        for bad in [((4, 4, 8), 3), ((4, 8, 4), 3), ((8, 4, 4), 3), (shape, 2)]:
+            with pytest.raises(SpmError):
+                udf.read_restart(u, bad[0], bad[1])
+        with pytest.raises(SpmError):
+            udf.read_restart(u, shape, 3, n_ion=1)
+        r = udf.read_restart(u, shape, 3)
+    assert r["ts"] == 7
