@@ -94,6 +94,16 @@ int spm_udf_params(const spm_udf *udf, spm_params *prm, spm_electrolyte *el, spm
  * otherwise, q_old = q; every other field zero.  n must be the array's
  * length.  INIT_orientation random with ROTATION ON is SPM_ERR_UNSUPPORTED. */
 int spm_udf_user_particles(const spm_udf *udf, spm_particle *p, int n);
+/* Init_Particle (init_particle.cxx:12-393) for spherical particles: as
+ * spm_udf_user_particles for user_specify; uniform_random (positions drawn
+ * uniformly in the box, redrawn while closer than 1.05 SIGMA to an earlier
+ * one under periodic images) and BCC (the reference's lattice) place the
+ * particles; INIT_orientation random with ROTATION ON draws q by
+ * RA_on_sphere4D.  Positions and orientations come from glibc's rand()
+ * sequence from its default state, as in the reference.  n is the
+ * Particle_number total.  FCC and random_walk (lattice relaxed by random
+ * walks and steepest descent on the LJ force) are SPM_ERR_UNSUPPORTED. */
+int spm_udf_init_particles(const spm_udf *udf, spm_particle *p, int n);
 
 /* Restart data (resume.CONTINUE.Saved_Data of Udf/define.udf).  The caller
  * sets the sizes and owns the buffers: zeta0/zeta1 hold nx*ny*2*(nz/2+1)

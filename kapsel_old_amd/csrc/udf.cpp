@@ -885,61 +885,197 @@ extern "C" int spm_udf_params(const spm_udf *u, spm_params *prm, spm_electrolyte
   return SPM_OK;
 }
 
-extern "C" int spm_udf_user_particles(const spm_udf *u, spm_particle *p, int n) {
+namespace {
+
+// glibc's rand() (the TYPE_3 additive feedback generator of random_r:
+// r[i] = r[i-31] + r[i-3] mod 2^32 over a state seeded by the Park-Miller
+// recurrence 16807 r mod (2^31 - 1), the first 310 outputs discarded, each
+// output r >> 1).  KAPSEL draws initial positions and orientations with it
+// (RA, RAx, macro.h:75-80); a process that has not called srand runs as
+// srand(1).
+class GlibcRand {
+ public:
+  explicit GlibcRand(uint32_t seed = 1) { srand(seed); }
+  void srand(uint32_t seed) {
+    if (seed == 0) seed = 1;
+    r_[0] = (int32_t)seed;
+    for (int i = 1; i < 31; ++i) {
+      const int64_t hi = r_[i - 1] / 127773, lo = r_[i - 1] % 127773;
+      int64_t w = 16807 * lo - 2836 * hi;
+      if (w < 0) w += 2147483647;
+      r_[i] = (int32_t)w;
+    }
+    for (int i = 31; i < 34; ++i) r_[i] = r_[i - 31];
+    k_ = 0;  // r[34] goes to slot 0
+    for (int i = 0; i < 310; ++i) next();
+  }
+  int rand() { return (int)(next() >> 1); }
+  static constexpr double kRandMax = 2147483647.0;
+  double RA() { return 2.0 * ((double)rand() - kRandMax * 0.5) / kRandMax; }  // uniform in [-1, 1]
+  double RAx(double x) { return (double)rand() / kRandMax * x; }             // uniform in [0, x)
+
+ private:
+  int32_t r_[34];  // the last 34 values, r[i] in slot i % 34
+  int k_ = 0;      // slot of the next value
+  uint32_t next() {
+    const uint32_t v = (uint32_t)r_[(k_ + 3) % 34] + (uint32_t)r_[(k_ + 31) % 34];  // r[i-31] + r[i-3]
+    r_[k_] = (int32_t)v;
+    k_ = (k_ + 1) % 34;
+    return v;
+  }
+};
+
+// RA_on_sphere4D (macro.h:113-123) from two RA_circle draws (macro.h:82-91)
+void random_rqtn(GlibcRand &g, double q[4]) {
+  double u[4];
+  for (int h = 0; h < 2; ++h) {
+    do {
+      u[2 * h] = g.RA();
+      u[2 * h + 1] = g.RA();
+    } while (!(u[2 * h] * u[2 * h] + u[2 * h + 1] * u[2 * h + 1] <= 1));
+  }
+  const double scale = std::sqrt((1.0 - (u[0] * u[0] + u[1] * u[1])) / (u[2] * u[2] + u[3] * u[3]));
+  q[0] = u[0];
+  q[1] = u[1];
+  q[2] = u[2] * scale;
+  q[3] = u[3] * scale;
+}
+
+int nint(double a) { return a >= 0.0 ? int(a + .5) : int(a - .5); }  // Nint (macro.h:38-46)
+
+// Init_Particle (init_particle.cxx:12-393) for spherical particles.
+int init_particles(const spm_udf *u, spm_particle *p, int n, bool user_only) {
   if (!u || (!p && n > 0)) return fail(SPM_ERR_INVALID, "null argument");
   Reader r{u};
   const std::string dist = r.s("switch.INIT_distribution.type");
   const std::string rot = r.s("switch.ROTATION");
   const std::string orient = r.s("switch.INIT_orientation");
   const int nspec = r.size("object_type.spherical_particle.Particle_spec[]");
+  const double A = r.d("A");
+  int np[3] = {r.i("mesh.NPX"), r.i("mesh.NPY"), r.i("mesh.NPZ")};
+  double DX = 0.0;
+  {
+    char eq[64] = {0};
+    if (spm_udf_get_string(u, "constitutive_eq.type", eq, sizeof(eq)) == SPM_OK)
+      DX = r.d(std::string("constitutive_eq.") + eq + ".DX");
+  }
   if (r.rc) return r.rc;
-  if (dist != "user_specify") return fail(SPM_ERR_INVALID, "INIT_distribution is " + dist + ", not user_specify");
-  // orientation (init_particle.cxx:282-286 and 318-345): the file's q,
-  // normalized, only with ROTATION on and INIT_orientation user_specify
-  const bool user_q = rot == "ON" && orient == "user_specify";
-  if (rot == "ON" && orient == "random") return unsupported("INIT_orientation random");
-  const std::string b0 = "switch.INIT_distribution.user_specify.Particles";
-  const int m = r.size(b0 + "[]");
-  if (r.rc) return r.rc;
-  if (m != n) return fail(SPM_ERR_INVALID, "Particles[] holds " + std::to_string(m) + " entries");
+  if (user_only && dist != "user_specify")
+    return fail(SPM_ERR_INVALID, "INIT_distribution is " + dist + ", not user_specify");
+  if (orient != "user_specify" && orient != "random" && orient != "space_align")
+    return fail(SPM_ERR_INVALID, "invalid INIT_orientation " + orient);
   // species by Particle_spec counts in order (init_particle.cxx:351-357)
   std::vector<int> spec;
   for (int s = 0; s < nspec; ++s) {
     const int c = r.i("object_type.spherical_particle.Particle_spec[" + std::to_string(s) + "].Particle_number");
     if (r.rc) return r.rc;
+    if (c < 0) return fail(SPM_ERR_INVALID, "negative Particle_number");
     spec.insert(spec.end(), c, s);
   }
-  if ((int)spec.size() != n) return fail(SPM_ERR_INVALID, "Particle_number total differs from Particles[]");
-  std::memset(p, 0, sizeof(spm_particle) * (size_t)n);
-  for (int k = 0; k < n; ++k) {
-    const std::string b = b0 + "[" + std::to_string(k) + "].";
-    const char *c[3] = {"x", "y", "z"};
-    p[k].spec = spec[k];
-    for (int d = 0; d < 3; ++d) {
-      p[k].x[d] = r.d(b + "R." + c[d]);
-      p[k].x_nopbc[d] = p[k].x[d];
-      // v and omega are read, then set to 0.0*RA() (init_particle.cxx:361,372)
-      (void)r.d(b + "v." + c[d]);
-      (void)r.d(b + "omega." + c[d]);
-    }
-    double q[4] = {r.d(b + "q.q0"), r.d(b + "q.q1"), r.d(b + "q.q2"), r.d(b + "q.q3")};
-    if (r.rc) return r.rc;
-    if (user_q) {  // qtn_normalize (quaternion.h:237-245)
-      const double norm = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-      if (!(norm > 0.0)) return fail(SPM_ERR_INVALID, "zero quaternion in Particles[" + std::to_string(k) + "]");
-      const double inv = 1.0 / norm;
-      for (double &x : q) x *= inv;
-    } else {
-      q[0] = 1.0;
-      q[1] = q[2] = q[3] = 0.0;
-    }
-    p[k].q.s = q[0];
-    p[k].q.v[0] = q[1];
-    p[k].q.v[1] = q[2];
-    p[k].q.v[2] = q[3];
-    p[k].q_old = p[k].q;
+  if ((int)spec.size() != n) return fail(SPM_ERR_INVALID, "n differs from the Particle_number total");
+  double L[3];
+  for (int d = 0; d < 3; ++d) {
+    if (np[d] < 0 || np[d] > 12) return fail(SPM_ERR_INVALID, "invalid mesh");
+    L[d] = (1 << np[d]) * DX;  // L_particle = L = N * DX (input.cxx:304-311)
   }
+  const double sigma = 2.0 * (A * DX);  // SIGMA = 2 RADIUS (input.cxx:1339-1340)
+  GlibcRand g;
+  std::vector<spm_particle> out((size_t)n);
+  std::memset(out.data(), 0, sizeof(spm_particle) * out.size());
+  for (auto &q : out) q.q.s = 1.0;
+  const bool rotation = rot == "ON";
+  if (dist == "user_specify") {  // init_particle.cxx:226-288
+    const std::string b0 = "switch.INIT_distribution.user_specify.Particles";
+    const int m = r.size(b0 + "[]");
+    if (r.rc) return r.rc;
+    if (m != n) return fail(SPM_ERR_INVALID, "Particles[] holds " + std::to_string(m) + " entries");
+    for (int k = 0; k < n; ++k) {
+      const std::string b = b0 + "[" + std::to_string(k) + "].";
+      const char *c[3] = {"x", "y", "z"};
+      for (int d = 0; d < 3; ++d) {
+        out[k].x[d] = r.d(b + "R." + c[d]);
+        // v and omega are read, then set to 0.0*RA() below
+        (void)r.d(b + "v." + c[d]);
+        (void)r.d(b + "omega." + c[d]);
+      }
+      double q[4] = {r.d(b + "q.q0"), r.d(b + "q.q1"), r.d(b + "q.q2"), r.d(b + "q.q3")};
+      if (r.rc) return r.rc;
+      if (orient == "user_specify") {  // qtn_normalize (quaternion.h:237-245)
+        const double norm = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        if (!(norm > 0.0)) return fail(SPM_ERR_INVALID, "zero quaternion in Particles[" + std::to_string(k) + "]");
+        const double inv = 1.0 / norm;
+        for (double &x : q) x *= inv;
+        out[k].q.s = q[0];
+        for (int d = 0; d < 3; ++d) out[k].q.v[d] = q[d + 1];
+      }
+    }
+  } else if (dist == "uniform_random") {  // init_particle.cxx:27-48
+    const double overlap = sigma * 1.05;
+    const long budget = 1000000L + 1000L * n;  // draws before giving up
+    long draws = 0;
+    for (int i = 0; i < n; ++i) {
+      for (;;) {
+        if (++draws > budget) return fail(SPM_ERR_INVALID, "uniform_random: no room for the particles");
+        for (int d = 0; d < 3; ++d) out[i].x[d] = g.RAx(L[d]);
+        int j = 0;
+        for (; j < i; ++j) {  // Distance (interaction.h:110-135)
+          double s2 = 0.0;
+          for (int d = 0; d < 3; ++d) {
+            double x12 = out[j].x[d] - out[i].x[d];
+            x12 -= (double)nint(x12 / L[d]) * L[d];
+            s2 += x12 * x12;
+          }
+          if (std::sqrt(s2) <= overlap) break;
+        }
+        if (j >= i) break;
+      }
+    }
+  } else if (dist == "BCC") {  // init_particle.cxx:291-316
+    const int nn = (int)std::ceil(std::pow((double)n / 2., 1. / 3));
+    const double ax = L[0] / (double)nn, ay = L[1] / (double)nn, az = L[2] / (double)nn;
+    for (int i = 0; i < n; ++i) {
+      const int ix = i % nn, iy = (i % (nn * nn)) / nn, iz = i / (nn * nn);
+      out[i].x[0] = ax / 4.0 + (double)ix * ax;
+      out[i].x[1] = ay / 4.0 + (double)iy * ay;
+      out[i].x[2] = az / 4.0 + (double)iz * az / 2.0;
+      if (iz % 2 == 1) {
+        out[i].x[0] = out[i].x[0] + ax / 2.0;
+        out[i].x[1] = out[i].x[1] + ax / 2.0;  // ax, as the reference has it
+      }
+    }
+  } else {
+    return unsupported("INIT_distribution " + dist);
+  }
+  // orientation (init_particle.cxx:318-345)
+  for (auto &q : out) {
+    if (rotation && orient == "random") {
+      double v[4];
+      random_rqtn(g, v);
+      q.q.s = v[0];
+      for (int d = 0; d < 3; ++d) q.q.v[d] = v[d + 1];
+    } else if (!(rotation && orient == "user_specify" && dist == "user_specify")) {
+      q.q.s = 1.0;
+      q.q.v[0] = q.q.v[1] = q.q.v[2] = 0.0;
+    }
+    q.q_old = q.q;
+  }
+  // species, velocity 0*RA(), angular velocity 0*RA() (init_particle.cxx:349-393)
+  for (int k = 0; k < n; ++k) {
+    out[k].spec = spec[k];
+    for (int d = 0; d < 3; ++d) out[k].x_nopbc[d] = out[k].x[d];
+  }
+  std::copy(out.begin(), out.end(), p);
   return SPM_OK;
+}
+
+}  // namespace
+
+extern "C" int spm_udf_user_particles(const spm_udf *u, spm_particle *p, int n) {
+  return init_particles(u, p, n, true);
+}
+
+extern "C" int spm_udf_init_particles(const spm_udf *u, spm_particle *p, int n) {
+  return init_particles(u, p, n, false);
 }
 
 // ---------------------------------------------------------------------

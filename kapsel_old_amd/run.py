@@ -1,6 +1,7 @@
 """Command-line driver: a run from a UDF input, as the reference's main()
 (sp_3d_ns.cxx:392-640) for constitutive_eq Navier_Stokes / Electrolyte
-with spherical particles placed by INIT_distribution user_specify.
+with spherical particles placed by INIT_distribution user_specify,
+uniform_random or BCC.
 
     python -m kapsel_old_amd.run -I input.udf -O output.udf -R restart.udf [-D define.udf]
 
@@ -57,12 +58,13 @@ def parse_args(argv):
 
 
 def initial_particles(u, prm, run, resumed):
-    """Init_Particle for user_specify; for a resumed run of another
-    distribution only the species matter (the restart replaces the rest)."""
-    if udf.DISTRIBUTIONS[run.distribution] == "user_specify":
-        return u.user_particles()
+    """Init_Particle (user_specify, uniform_random, BCC); for a resumed run
+    of another distribution only the species matter (the restart replaces
+    the rest)."""
+    if udf.DISTRIBUTIONS[run.distribution] in ("user_specify", "uniform_random", "BCC"):
+        return u.init_particles()
     if not resumed:
-        raise SystemExit("INIT_distribution %s is not run by this driver; use user_specify"
+        raise SystemExit("INIT_distribution %s is not run by this driver; use user_specify, uniform_random or BCC"
                          % udf.DISTRIBUTIONS[run.distribution])
     p = abi.new_particles(prm.n_particles)
     counts = [u.get_int("object_type.spherical_particle.Particle_spec[%d].Particle_number" % s)

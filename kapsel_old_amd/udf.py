@@ -56,6 +56,7 @@ def lib():
         L.spm_udf_params.argtypes = [vp, C.POINTER(abi.SpmParams), C.POINTER(abi.SpmElectrolyte),
                                      C.POINTER(SpmUdfRun)]
         L.spm_udf_user_particles.argtypes = [vp, vp, ip]
+        L.spm_udf_init_particles.argtypes = [vp, vp, ip]
         L.spm_udf_create_output.argtypes = [cp, cp]
         L.spm_udf_append_record.argtypes = [cp, ip, vp, ip]
         L.spm_udf_write_restart.argtypes = [cp, cp, C.POINTER(SpmRestart)]
@@ -149,6 +150,15 @@ class UDF:
         n = self.size("switch.INIT_distribution.user_specify.Particles[]")
         p = abi.new_particles(n)
         _check(lib().spm_udf_user_particles(self._h, p.ctypes.data_as(C.c_void_p), n), "spm_udf_user_particles")
+        return p
+
+
+    def init_particles(self):
+        """Particle array Init_Particle builds (user_specify, uniform_random, BCC)."""
+        n = sum(self.get_int("object_type.spherical_particle.Particle_spec[%d].Particle_number" % s)
+                for s in range(self.size("object_type.spherical_particle.Particle_spec[]")))
+        p = abi.new_particles(n)
+        _check(lib().spm_udf_init_particles(self._h, p.ctypes.data_as(C.c_void_p), n), "spm_udf_init_particles")
         return p
 
 

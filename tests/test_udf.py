@@ -209,14 +209,6 @@ def test_user_particles_init(udf, tmp_path, rotation, orientation):
     assert (p["q_old"] == p["q"]).all()
 
 
-def test_user_particles_random_orientation_rejected(udf, tmp_path):
-    f = _user_file(tmp_path, "ON", "random")
-    with udf.UDF(f) as u:
-        with pytest.raises(SpmError) as e:
-            u.user_particles()
-    assert e.value.status == abi.SPM_ERR_UNSUPPORTED
-
-
 def test_records_round_trip(udf, tmp_path):
     out = tmp_path / "output.udf"
     udf.create_output(path("input.udf"), out)
@@ -377,3 +369,109 @@ def test_restart_rejects(udf, tmp_path):
             udf.read_restart(u, shape, 3, n_ion=1)
         r = udf.read_restart(u, shape, 3)
     assert r["ts"] == 7
+
+
+def _glibc():
+    import ctypes
+    try:
+        libc = ctypes.CDLL("libc.so.6")
+    except OSError:
+        pytest.skip("glibc not loadable")
+    libc.rand.restype = ctypes.c_int
+    return libc
+
+
+def _nint(a):
+    return int(a + .5) if a >= 0 else int(a - .5)
+
+
+def test_init_particles_uniform_random_is_glibc(udf):
+    """Positions from glibc's own rand() (default state = srand(1)) through
+    the placement loop of init_particle.cxx:27-48."""
+    libc = _glibc()
+    with udf.UDF(path("input.udf")) as u:
+        p = u.init_particles()
+    L, overlap = 32.0, 2 * 5.0 * 1.05
+    libc.srand(1)
+    out = []
+    for i in range(10):
+        while True:
+            x = [libc.rand() / 2147483647.0 * L for _ in range(3)]
+            near = False
+            for y in out:
+                r2 = 0.0
+                for d in range(3):
+                    x12 = y[d] - x[d]
+                    x12 -= _nint(x12 / L) * L
+                    r2 += x12 * x12
+                near |= math.sqrt(r2) <= overlap
+            if not near:
+                out.append(x)
+                break
+    np.testing.assert_array_equal(p["x"], out)
+    np.testing.assert_array_equal(p["x_nopbc"], out)
+    assert list(p["spec"]) == [0] * 5 + [1] * 5
+    # INIT_orientation space_align: identity; v and omega zero
+    assert (p["q"]["s"] == 1).all() and not p["q"]["v"].any() and not p["v"].any() and not p["omega"].any()
+
+
+def test_init_particles_bcc(udf):
+    with udf.UDF(path("repulsive.udf")) as u:
+        p = u.init_particles()
+        n = u.get_int("object_type.spherical_particle.Particle_spec[0].Particle_number")
+        L = 2.0 ** u.get_int("mesh.NPX")
+    assert len(p) == n == 500
+    nn = 7  # ceil((500/2)^(1/3))
+    a = L / nn
+    # layer 0 and layer 1 (shifted by a/2 in x and y), z spacing a/2
+    np.testing.assert_allclose(p["x"][0], [a / 4] * 3, rtol=1e-15)
+    np.testing.assert_allclose(p["x"][nn + 2], [a / 4 + 2 * a, a / 4 + a, a / 4], rtol=1e-15)
+    np.testing.assert_allclose(p["x"][nn * nn], [a / 4 + a / 2, a / 4 + a / 2, a / 4 + a / 2], rtol=1e-15)
+    assert (p["x"] >= 0).all() and (p["x"] < L).all()
+    d = np.linalg.norm(p["x"][:, None, :] - p["x"][None, :, :], axis=-1)
+    d[np.arange(n), np.arange(n)] = np.inf
+    assert d.min() == pytest.approx(a * math.sqrt(3) / 2, rel=1e-12)
+
+
+def test_init_particles_random_orientation(udf, tmp_path):
+    """RA_on_sphere4D (macro.h:113-123) on glibc's rand() from srand(1)."""
+    libc = _glibc()
+    f = _user_file(tmp_path, "ON", "random")
+    with udf.UDF(f) as u:
+        p = u.init_particles()
+    libc.srand(1)
+
+    def ra():
+        return 2.0 * (libc.rand() - 2147483647 * 0.5) / 2147483647
+
+    def circle():
+        while True:
+            a, b = ra(), ra()
+            if a * a + b * b <= 1:
+                return a, b
+
+    for k in range(3):
+        u0, u1 = circle()
+        u2, u3 = circle()
+        s = math.sqrt((1.0 - (u0 * u0 + u1 * u1)) / (u2 * u2 + u3 * u3))
+        assert [p["q"]["s"][k], *p["q"]["v"][k]] == [u0, u1, u2 * s, u3 * s]
+        assert abs(p["q"]["s"][k] ** 2 + (p["q"]["v"][k] ** 2).sum() - 1.0) < 1e-14
+    assert (p["q_old"] == p["q"]).all()
+    np.testing.assert_array_equal(p["x"], [[1.5, 2.5, 3.5], [10, 20, 30], [40.25, 50.5, 60.75]])
+
+
+def test_init_particles_unsupported(udf, tmp_path):
+    with udf.UDF(path("squirm_single_a+2.udf")) as u:  # user_specify
+        assert len(u.init_particles()) == 1
+    with udf.UDF(path("colloid_32.udf")) as u:  # uniform_random, 32 particles in 64^3
+        assert len(u.init_particles()) == 32
+    shutil.copy(path("define.udf"), tmp_path / "define.udf")
+    for dist in ("FCC", "random_walk"):
+        src = open(path("sample1.udf")).read()
+        assert src.count('"user_specify",\n  {100}') == 1
+        f = tmp_path / ("%s.udf" % dist)
+        f.write_text(src.replace('"user_specify",\n  {100}', '"%s",\n  {100}' % dist))
+        with udf.UDF(f) as u:
+            with pytest.raises(SpmError) as e:
+                u.init_particles()
+        assert e.value.status == abi.SPM_ERR_UNSUPPORTED
