@@ -2,8 +2,9 @@
 
 Usage: python -m kapsel_old_amd.build [--jobs N] [--define NAME=VALUE ...] [--out PATH]
 
---define overrides a launch-shape constant of csrc/fft_core.h (SPM_LINE_POINTS,
-SPM_LINE_RADIX, SPM_Z_POINTS, SPM_FUSED_RADIX, SPM_FUSED_X_DIV, ...) and
+--define overrides a launch-shape constant of csrc/fft_core.h
+(SPM_LINE_POINTS_F64, SPM_LINE_RADIX, SPM_Z_POINTS_F64, SPM_FUSED_RADIX,
+SPM_FUSED_X_POINTS, SPM_ZADV_POINTS_F64, ...) and
 --out writes that build somewhere other than the in-tree library.
 """
 import hashlib

@@ -213,7 +213,7 @@ namespace {
 template <typename T, int N, int SIGN>
 void launch_lines_n(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
                     int nouter, hipStream_t st, int col_cut, int row_keep) {
-  constexpr int R = kRadix(N), C = kLinesC(N);
+  constexpr int R = kRadix(N), C = kLinesC<T>(N);
   const LineTiles lt = make_line_tiles(ncols, nouter, ostride, C);
   hipLaunchKernelGGL((fft_lines_kernel<T, N, R, C, SIGN>), dim3(lt.tiles, 1, nf), dim3(C * N / R), 0, st, fp, tw, lt,
                      estride, col_cut, row_keep);

@@ -10,8 +10,11 @@ namespace spm {
 // measure another shape; the defaults are the measured choices for gfx950
 // at 256^3 (FP64 fields move twice the bytes per point of FP32 ones, so
 // some choices differ by precision).
-#ifndef SPM_LINE_POINTS
-#define SPM_LINE_POINTS 4096  // points (line length x columns) per line-pass block
+#ifndef SPM_LINE_POINTS_F64
+#define SPM_LINE_POINTS_F64 2048  // points (line length x columns) per line-pass block, FP64 fields
+#endif
+#ifndef SPM_LINE_POINTS_F32
+#define SPM_LINE_POINTS_F32 4096
 #endif
 #ifndef SPM_LINE_RADIX
 #define SPM_LINE_RADIX 8  // Stockham radix of the plain y/x passes
@@ -34,11 +37,14 @@ namespace spm {
 #ifndef SPM_ZADV_RADIX
 #define SPM_ZADV_RADIX 4  // radix of the fused z pass (advection products)
 #endif
-#ifndef SPM_ZADV_POINTS
-#define SPM_ZADV_POINTS 2048
+#ifndef SPM_ZADV_POINTS_F64
+#define SPM_ZADV_POINTS_F64 1024  // complex points per fused z-pass block, FP64 fields
 #endif
-#ifndef SPM_FUSED_X_DIV
-#define SPM_FUSED_X_DIV 2  // fused x passes take 1/DIV of a plain block's columns
+#ifndef SPM_ZADV_POINTS_F32
+#define SPM_ZADV_POINTS_F32 2048
+#endif
+#ifndef SPM_FUSED_X_POINTS
+#define SPM_FUSED_X_POINTS 2048  // points (line length x columns) per fused x-pass block
 #endif
 #ifndef SPM_XREMAP_F64
 #define SPM_XREMAP_F64 0  // XCD-grouped tiles in the fused x passes, FP64 (tiles are whole lines)
@@ -49,8 +55,10 @@ namespace spm {
 
 constexpr int clamp_rows(int r) { return r > 32 ? 32 : (r < 1 ? 1 : r); }
 // columns per line-pass block (at most 32)
+template <typename T>
 constexpr int kLinesC(int n) {
-  return n >= SPM_LINE_POINTS ? 1 : (SPM_LINE_POINTS / n > 32 ? 32 : SPM_LINE_POINTS / n);
+  constexpr int pts = sizeof(T) == 8 ? SPM_LINE_POINTS_F64 : SPM_LINE_POINTS_F32;
+  return n >= pts ? 1 : (pts / n > 32 ? 32 : pts / n);
 }
 constexpr int kRadix(int n) { return n >= SPM_LINE_RADIX ? SPM_LINE_RADIX : n; }
 // rows and radix of a plain z-pass block of m complex points
@@ -64,9 +72,14 @@ constexpr int kZRadix(int m) {
                                                                    : m;
 }
 constexpr int kZadvR(int n) { return n >= SPM_ZADV_RADIX ? SPM_ZADV_RADIX : n; }
-constexpr int kZadvRows(int m) { return clamp_rows(SPM_ZADV_POINTS / m); }
+template <typename T>
+constexpr int kZadvRows(int m) {
+  return clamp_rows((sizeof(T) == 8 ? SPM_ZADV_POINTS_F64 : SPM_ZADV_POINTS_F32) / m);
+}
 constexpr int kFusedR(int n) { return n >= SPM_FUSED_RADIX ? SPM_FUSED_RADIX : n; }
-constexpr int kFusedC(int n) { return kLinesC(n) / SPM_FUSED_X_DIV > 0 ? kLinesC(n) / SPM_FUSED_X_DIV : 1; }
+constexpr int kFusedC(int n) {
+  return n >= SPM_FUSED_X_POINTS ? 1 : (SPM_FUSED_X_POINTS / n > 32 ? 32 : SPM_FUSED_X_POINTS / n);
+}
 template <typename T>
 __device__ __forceinline__ int fused_x_tile() {
   return (sizeof(T) == 8 ? SPM_XREMAP_F64 : SPM_XREMAP_F32) ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;

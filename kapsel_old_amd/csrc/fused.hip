@@ -559,7 +559,7 @@ int fz_advect_t(spm_ctx *c) {
   const Grid &g = c->grid;
   const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
   SPM_ZSIZE_SWITCH(g.nz / 2, {
-    constexpr int R = kZadvR(S), ROWS = kZadvRows(S);
+    constexpr int R = kZadvR(S), ROWS = kZadvRows<T>(S);
     hipLaunchKernelGGL((z_advect_kernel<T, S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R), 0,
                        c->stream, g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
                        c->tw_z, c->tw_rz, scale);

@@ -17,13 +17,17 @@
 //              gravity (md_force.cxx:181-191).
 //   integrators MD_solver_position_* / MD_solver_velocity_* and the
 //              quaternion updates, one thread per particle, FP64.
+#include "phi_core.h"
 #include "spm_internal.h"
 
 namespace spm {
 
 namespace {
 
-constexpr int kTile = 8;
+using phi::kTile;
+using phi::PhiAcc;
+using phi::distance0;
+using phi::phi_fast;
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 __device__ __forceinline__ int nint(double a) { return a >= 0.0 ? (int)(a + .5) : (int)(a - .5); }
@@ -33,46 +37,6 @@ __device__ __forceinline__ double hfun(double dx, double x) { return x > 0 ? exp
 __device__ __forceinline__ double phi_of(const PModel &pm, double x) {
   const double dmy = hfun(pm.dx, pm.radius + pm.hxi - x);
   return dmy / (dmy + hfun(pm.dx, x - pm.radius + pm.hxi));
-}
-
-// The same Phi with one exponential: H1 / (H1 + H2) = 1 / (1 + H2 / H1),
-// H2 / H1 = exp(dx^2/x1^2 - dx^2/x2^2) with x1 = a + xi/2 - d,
-// x2 = d - a + xi/2; H1 = 0 (x1 <= 0) gives 0 and H2 = 0 gives 1 as in the
-// two-exponential form.  Equal to phi_of up to rounding; A is the
-// arithmetic type (FP64, or FP32 for the FP32 field path).
-template <typename A>
-__device__ __forceinline__ A phi_fast(const PModel &pm, A d) {
-  const A x1 = (A)(pm.radius + pm.hxi) - d, x2 = d - (A)(pm.radius - pm.hxi);
-  if (!(x1 > (A)0)) return (A)0;
-  if (!(x2 > (A)0)) return (A)1;
-  const A dx2 = (A)(pm.dx * pm.dx);
-  return (A)1 / ((A)1 + exp(dx2 / (x1 * x1) - dx2 / (x2 * x2)));
-}
-
-// accumulation type of make_phi: FP64, or FP32 for FP32 fields
-template <typename T>
-struct PhiAcc {
-  using type = double;
-};
-template <>
-struct PhiAcc<float> {
-  using type = float;
-};
-
-// Distance0 (interaction.h:110-123) from grid point x1 to particle x2.
-// x12/L is taken as x12 * (1/L): the nearest-image count differs only when
-// x12/L is within one rounding of a half-integer (a pair half a box apart).
-__device__ __forceinline__ double distance0(const Grid &g, const double *x1, const double *x2, double *x12) {
-  const double L[3] = {g.lx, g.ly, g.lz};
-  const double iL[3] = {1.0 / g.lx, 1.0 / g.ly, 1.0 / g.lz};
-  double dmy = 0.0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    x12[d] = x2[d] - x1[d];
-    x12[d] -= (double)nint(x12[d] * iL[d]) * L[d];
-    dmy += sq(x12[d]);
-  }
-  return sqrt(dmy);
 }
 
 __device__ __forceinline__ double P(const double *pb, int np, int slot, int n) { return pb[(long)slot * np + n]; }
@@ -238,63 +202,12 @@ __global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const 
   const int tk = tile % tz, tj = (tile / tz) % ty, ti = tile / (tz * ty);
   const int lk = threadIdx.x % kTile, lj = (threadIdx.x / kTile) % kTile, li = threadIdx.x / (kTile * kTile);
   const int gi[3] = {ti * kTile + li, tj * kTile + lj, tk * kTile + lk};
-  const int N[3] = {g.nx, g.ny, g.nz};
-  const double x[3] = {gi[0] * pm.dx, gi[1] * pm.dx, gi[2] * pm.dx};
-  const int s0 = tile_start[tile], cnt = tile_count[tile];
-  const double rcut = pm.radius + pm.hxi;
   const bool sw_up = mode != 1;
-  // n and the particle's members are the same for every thread of the
-  // block, so the compiler reads them with scalar loads.  FP32 fields take
-  // phi and its sums in FP32 (the distance itself in FP64).
   using A = typename PhiAcc<T>::type;
-  A ph = 0, a0 = 0, a1 = 0, a2 = 0;
-  for (int q = 0; q < cnt; ++q) {  // ascending particle index, as the reference adds
-    const int n = tile_list[s0 + q];
-    int cell[3];
-    bool in = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      int o = gi[d] - pxint[3 * n + d];  // |o| < N
-      if (o >= N[d] / 2) o -= N[d];
-      if (o < -(N[d] / 2)) o += N[d];
-      cell[d] = o;
-      in = in && (o <= pm.nh && o >= -pm.nh);
-    }
-    if (!in) continue;
-    double r[3];  // Relative_coord: offset * dx - residue
-#pragma unroll
-    for (int d = 0; d < 3; ++d) r[d] = (double)cell[d] * pm.dx - pres[3 * n + d];
-    double dist;
-    if constexpr (sizeof(T) == 8) {  // Distance0 as the reference takes it
-      const double xp[3] = {P(pb, pm.np, P_X, n), P(pb, pm.np, P_X + 1, n), P(pb, pm.np, P_X + 2, n)};
-      double x12[3];
-      dist = distance0(g, x, xp, x12);
-    } else {
-      dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-    }
-    if (!(dist < rcut)) continue;
-    const A dphi = phi_fast<A>(pm, (A)dist);
-    ph += dphi;
-    if (sw_up) {
-      double vr[3] = {0.0, 0.0, 0.0};
-      if (pm.rotation) {
-        const double w0 = P(pb, pm.np, P_OMEGA, n), w1 = P(pb, pm.np, P_OMEGA + 1, n), w2 = P(pb, pm.np, P_OMEGA + 2, n);
-        vr[0] = w1 * r[2] - w2 * r[1];
-        vr[1] = w2 * r[0] - w0 * r[2];
-        vr[2] = w0 * r[1] - w1 * r[0];
-      }
-      a0 += (A)(P(pb, pm.np, P_V, n) + vr[0]) * dphi;
-      a1 += (A)(P(pb, pm.np, P_V + 1, n) + vr[1]) * dphi;
-      a2 += (A)(P(pb, pm.np, P_V + 2, n) + vr[2]) * dphi;
-    }
-  }
-  if (sw_up && ph > (A)1) {  // make_phi.cxx:280-297
-    const A inv = (A)1 / ph;
-    a0 *= inv;
-    a1 *= inv;
-    a2 *= inv;
-    ph = (A)1;
-  }
+  A ph, a0, a1, a2;
+  phi::phi_point<T>(g, pm, pb, pxint, pres, tile_list + tile_start[tile], tile_count[tile], gi, sw_up, ph, a0, a1,
+                    a2);
+  if (sw_up) phi::phi_clamp(ph, a0, a1, a2);
   const long im = ((long)gi[0] * g.ny + gi[1]) * g.nz_ + gi[2];
   const bool pad = gi[2] == g.nz - 1;  // also clear the two padding slots
   if (mode == 2) {
@@ -779,16 +692,16 @@ static int launch_make_phi(spm_ctx *c, int mode, double *o0, double *o1, double 
   tbeg(c);
   if (c->prm.precision == SPM_FP32) {
     using T = float;
-    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf,
-                       c->pxint, c->pres, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1],
-                       c->tiles[2], (T *)o0, (T *)o1, (T *)o2, (T *)o3, (const T *)c->u[0], (const T *)c->u[1],
-                       (const T *)c->u[2], mode);
+    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(512), 0,
+                       c->stream, c->grid, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count,
+                       c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2], (T *)o0, (T *)o1, (T *)o2, (T *)o3,
+                       (const T *)c->u[0], (const T *)c->u[1], (const T *)c->u[2], mode);
   } else {
     using T = double;
-    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf,
-                       c->pxint, c->pres, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1],
-                       c->tiles[2], (T *)o0, (T *)o1, (T *)o2, (T *)o3, (const T *)c->u[0], (const T *)c->u[1],
-                       (const T *)c->u[2], mode);
+    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(512), 0,
+                       c->stream, c->grid, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count,
+                       c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2], (T *)o0, (T *)o1, (T *)o2, (T *)o3,
+                       (const T *)c->u[0], (const T *)c->u[1], (const T *)c->u[2], mode);
   }
   const double es = c->prm.precision == SPM_FP32 ? 4.0 : 8.0;
   // mode 0 writes 4 fields, mode 1 one, mode 2 reads 3 and writes 3

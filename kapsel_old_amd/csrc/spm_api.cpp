@@ -232,6 +232,12 @@ int time_evolution_hydro(spm_ctx *c) {
   return 0;
 }
 
+// f = up - phi u (make_phi mode f) and its z transform
+static int make_f_z(spm_ctx *c, double *const f3[3]) {
+  RC(p_make_f(c));
+  return fft_pass_z_r2c(c, f3, 3);
+}
+
 // The same step as fused FFT passes (fused.hip).  Leaves zeta, uk_dc and the
 // particles as the literal schedule does; u (a spectrum at the end of a
 // step), phi and up are rebuilt by materialize() when a spm_step call
@@ -258,8 +264,7 @@ int fused_step(spm_ctx *c) {
   RC(thermostat(c));
   // Make_phi_u_particle + Make_f_particle_dt_sole + Add_f_particle + U2zeta_k
   // (with no particles the tile lists stay empty and f = 0)
-  RC(p_make_f(c));
-  RC(fft_pass_z_r2c(c, f3, 3));
+  RC(make_f_z(c, f3));
   RC(fft_pass_y(c, f3, 3, +1));
   RC(fx_project(c));
   return 0;
@@ -303,8 +308,7 @@ int fused_stage(spm_ctx *c, int stage) {
       return 0;
     case 3:
       RC(p_build_tiles(c));
-      RC(p_make_f(c));
-      RC(fft_pass_z_r2c(c, f3, 3));
+      RC(make_f_z(c, f3));
       RC(fft_pass_y(c, f3, 3, +1));
       RC(fx_project(c));
       return 0;
