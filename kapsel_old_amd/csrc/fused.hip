@@ -70,6 +70,35 @@ __device__ __forceinline__ bool tile_kept(const Grid &g, int c0) {
   return any;
 }
 
+// Tile sequence of a block of the x passes: with one block per tile
+// (gridDim = ntiles) step 0 is the block's own tile; with fewer blocks
+// (persistent) block b takes tiles b, b + G, b + 2G, ... or, with the XCD
+// grouping on and G a multiple of 8, the y-th block of XCD x takes tiles
+// lo_x + y, lo_x + y + G/8, ... of its XCD's contiguous range.  ntiles
+// marks the end of the sequence.
+template <typename T>
+__device__ __forceinline__ int x_tile_seq(int step, int ntiles) {
+  const int b = blockIdx.x, G = gridDim.x;
+  if (G >= ntiles) return step == 0 ? fused_x_tile<T>() : ntiles;
+  if ((sizeof(T) == 8 ? SPM_XREMAP_F64 : SPM_XREMAP_F32) && G % 8 == 0) {
+    const int x = b % 8, y = b / 8, per = G / 8;
+    const int lo = (int)((long)x * ntiles / 8), hi = (int)((long)(x + 1) * ntiles / 8);
+    const long t = lo + y + (long)step * per;
+    return t < hi ? (int)t : ntiles;
+  }
+  const long t = b + (long)step * G;
+  return t < ntiles ? (int)t : ntiles;
+}
+
+// first step >= k of the block's sequence whose tile is kept (or the end)
+template <typename T, int C>
+__device__ __forceinline__ int x_next_kept(const Grid &g, int k, int ntiles) {
+  for (;; ++k) {
+    const int t = x_tile_seq<T>(k, ntiles);
+    if (t >= ntiles || tile_kept<C>(g, t * C)) return k;
+  }
+}
+
 // Zeta_k2u_k (fft_wrapper.cxx:408-511) on one mode; component d.
 template <typename T>
 __device__ __forceinline__ cplx<T> zeta2u_mode(const Grid &g, const ModeK &m, cplx<T> d1, cplx<T> d2, int d) {
@@ -107,11 +136,11 @@ __device__ __forceinline__ cplx<T> zeta2u_mode(const Grid &g, const ModeK &m, cp
 // truncation of sp_3d_ns.cxx:75-77 / operate_omega.cxx:106-108).  The thread
 // holding mode 0 sets u(0) = uk_dc, zeroing the FIX_CELL components first.
 // ---------------------------------------------------------------------
-template <typename T, int N, int R, int C>
+template <typename T, int N, int R, int C, bool PERSIST>
 __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__restrict__ z0,
                                                            const T *__restrict__ z1, T *u0, T *u1, T *u2,
                                                            double *uk_dc, int fix_mask,
-                                                           const double2 *__restrict__ tw_g) {
+                                                           const double2 *__restrict__ tw_g, int ntiles) {
   using C2 = cplx<T>;
   static_assert(N >= R, "one first-stage butterfly per thread");
   __shared__ C2 lds[N * C];
@@ -123,50 +152,75 @@ __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__r
   const int c = tid % C, j0 = tid / C;
   const long H = g.hnz;
   const long es = (long)g.ny * H;
-  // columns (j, kz) numbered as one run j*(NZ/2+1) + kz: a tile starts on
-  // a 128-byte boundary
-  const int c0 = fused_x_tile<T>() * C;
-  // columns outside the 2/3 box are zero and the y pass never reads them
-  if (!tile_kept<C>(g, c0)) return;
-  const int col = c0 + c;
-  const int j = col / (int)H, kz = col - j * (int)H;
-  const bool col_ok = col < es && col_kept(g, j, kz);
+  // Tiles of C columns (j, kz) numbered as one run j*(NZ/2+1) + kz (a tile
+  // starts on a 128-byte boundary).  A block takes the kept tiles of its
+  // sequence (x_tile_seq) in turn and loads the next one's zeta while the
+  // current one is transformed; tiles outside the 2/3 box are zero and the
+  // y pass never reads them.
+  int k = x_next_kept<T, C>(g, 0, ntiles);
+  int t = x_tile_seq<T>(k, ntiles);
+  if (t >= ntiles) return;
   // zeta at the thread's first-stage points i = j0 + q N/R (the 2/3 mask
   // applied on load)
-  C2 za[R], zb[R];
+  auto fetch = [&](int tile, C2 *za, C2 *zb) {
+    const int col = tile * C + c;
+    const int j = col / (int)H, kz = col - j * (int)H;
+    const bool col_ok = col < es && col_kept(g, j, kz);
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = j0 + q * (N / R);
-    za[q] = zb[q] = C2{0, 0};
-    const ModeK m = mode_k(g, i, j, kz);
-    if (col_ok && m.keep) {
-      const long e = 2 * (col + i * es);
-      za[q] = *reinterpret_cast<const C2 *>(z0 + e);
-      zb[q] = *reinterpret_cast<const C2 *>(z1 + e);
+    for (int q = 0; q < R; ++q) {
+      const int i = j0 + q * (N / R);
+      za[q] = zb[q] = C2{0, 0};
+      const ModeK m = mode_k(g, i, j, kz);
+      if (col_ok && m.keep) {
+        const long e = 2 * (col + i * es);
+        za[q] = *reinterpret_cast<const C2 *>(z0 + e);
+        zb[q] = *reinterpret_cast<const C2 *>(z1 + e);
+      }
+    }
+  };
+  C2 za[R], zb[R], na[R], nb[R];
+  fetch(t, za, zb);
+  for (;;) {
+    const int kn = x_next_kept<T, C>(g, k + 1, ntiles);
+    const int tn = x_tile_seq<T>(kn, ntiles);
+    if (PERSIST && tn < ntiles) fetch(tn, na, nb);
+    const int c0 = t * C;
+    const int col = c0 + c;
+    const int j = col / (int)H, kz = col - j * (int)H;
+    const bool col_ok = col < es && col_kept(g, j, kz);
+    const bool has0 = c0 == 0;  // the tile holding column (j, kz) = (0, 0)
+    double dc[3] = {0.0, 0.0, 0.0};
+    if (has0)
+      for (int d = 0; d < 3; ++d) dc[d] = (fix_mask & (1 << d)) ? 0.0 : uk_dc[d];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T *out = d == 0 ? u0 : (d == 1 ? u1 : u2);
+      auto ld = [&](int i, int, int q) -> C2 {
+        const ModeK m = mode_k(g, i, j, kz);
+        C2 v = zeta2u_mode<T>(g, m, za[q], zb[q], d);
+        if (has0 && i == 0 && c == 0) v.x = (T)dc[d];
+        return v;
+      };
+      auto st = [&](int i, int, int, C2 v) {
+        if (col_ok) *reinterpret_cast<C2 *>(out + 2 * (col + i * es)) = v;
+      };
+      fft_stages_io<T, N, R, C, C, -1, 1, true, true>(lds, tw, tid, ld, st);
+    }
+    if (has0) {  // the same in every thread of the block
+      __syncthreads();  // every thread has read uk_dc
+      if (tid == 0)
+        for (int d = 0; d < 3; ++d)
+          if (fix_mask & (1 << d)) uk_dc[d] = 0.0;
+    }
+    if (!PERSIST || tn >= ntiles) break;
+    t = tn;
+    k = kn;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      za[q] = na[q];
+      zb[q] = nb[q];
     }
   }
-  const bool has0 = c0 == 0;  // the block holding column (j, kz) = (0, 0)
-  double dc[3] = {0.0, 0.0, 0.0};
-  if (has0)
-    for (int d = 0; d < 3; ++d) dc[d] = (fix_mask & (1 << d)) ? 0.0 : uk_dc[d];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    T *out = d == 0 ? u0 : (d == 1 ? u1 : u2);
-    auto ld = [&](int i, int, int q) -> C2 {
-      const ModeK m = mode_k(g, i, j, kz);
-      C2 v = zeta2u_mode<T>(g, m, za[q], zb[q], d);
-      if (has0 && i == 0 && c == 0) v.x = (T)dc[d];
-      return v;
-    };
-    auto st = [&](int i, int, int, C2 v) {
-      if (col_ok) *reinterpret_cast<C2 *>(out + 2 * (col + i * es)) = v;
-    };
-    fft_stages_io<T, N, R, C, C, -1, 1, true, true>(lds, tw, tid, ld, st);
-  }
-  __syncthreads();  // every thread of block 0 has read uk_dc
-  if (has0 && tid == 0)
-    for (int d = 0; d < 3; ++d)
-      if (fix_mask & (1 << d)) uk_dc[d] = 0.0;
 }
 
 // ---------------------------------------------------------------------
@@ -319,10 +373,10 @@ __global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, T *u0, T 
 // contraction, Omega_k2zeta_k and the slaved-Euler update as epilogue.
 // The transformed products are not stored.
 // ---------------------------------------------------------------------
-template <typename T, int N, int R, int C>
+template <typename T, int N, int R, int C, bool PERSIST>
 __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0, const T *p1, const T *p2,
                                                            const T *p3, const T *p4, T *z0, T *z1, double nu,
-                                                           double dt, const double2 *__restrict__ tw_g) {
+                                                           double dt, const double2 *__restrict__ tw_g, int ntiles) {
   using C2 = cplx<T>;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
@@ -333,86 +387,102 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
   const int c = tid % C;
   const long H = g.hnz;
   const long es = (long)g.ny * H;
-  const int c0 = fused_x_tile<T>() * C;
-  // a column outside the 2/3 box gives only discarded modes, and zeta is
-  // zero there already
-  if (!tile_kept<C>(g, c0)) return;
-  const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
-  const int j = col / (int)H, kz = col - j * (int)H;
-  const bool col_ok = col < es && col_kept(g, j, kz);
-  // The contraction (operate_omega.cxx:60-97) is linear in the five product
-  // spectra P0..P4 = u2u3, u3u1, u1u2, u2^2-u3^2, u3^2-u1^2: accumulate the
-  // two components Omega_k2zeta_k keeps as each product is transformed.
-  C2 acc0[R], acc1[R];
-  int rows[R];
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    acc0[q] = acc1[q] = C2{0, 0};
-    rows[q] = 0;
-  }
-  // Software pipeline over the five spectra: the first-stage points of
-  // spectrum f+1 (i = j0 + q N/R) are loaded while spectrum f is
-  // transformed in LDS; the barriers of the transform wait on LDS only.
+  // kept tiles of the block's sequence (a column outside the 2/3 box gives
+  // only discarded modes, and zeta is zero there already)
+  int k = x_next_kept<T, C>(g, 0, ntiles);
+  int t = x_tile_seq<T>(k, ntiles);
+  if (t >= ntiles) return;
   const int j0 = tid / C;
-  auto fetch = [&](const T *in, C2 *dst) {
+  const T *const pf[5] = {p0, p1, p2, p3, p4};
+  auto fetch = [&](int tile, int f, C2 *dst) {
+    const int col = tile * C + c;
+    const bool ok = col < es && col_kept(g, col / (int)H, col - (col / (int)H) * (int)H);
 #pragma unroll
     for (int q = 0; q < R; ++q)
-      dst[q] = col_ok ? *reinterpret_cast<const C2 *>(in + 2 * (col + (long)(j0 + q * (N / R)) * es)) : C2{0, 0};
+      dst[q] = ok ? *reinterpret_cast<const C2 *>(pf[f] + 2 * (col + (long)(j0 + q * (N / R)) * es)) : C2{0, 0};
   };
+  // Software pipeline over the five spectra and over tiles: the first-stage
+  // points of the next spectrum (the next tile's first one after the
+  // fifth) are loaded while the current one is transformed in LDS; the
+  // barriers of the transform wait on LDS only.
   C2 cur[R], nxt[R];
-  fetch(p0, cur);
+  fetch(t, 0, cur);
+  for (;;) {
+    const int kn = x_next_kept<T, C>(g, k + 1, ntiles);
+    const int tn = x_tile_seq<T>(kn, ntiles);
+    const int col = t * C + c;  // flat column j*(NZ/2+1) + kz
+    const int j = col / (int)H, kz = col - j * (int)H;
+    const bool col_ok = col < es && col_kept(g, j, kz);
+    // The contraction (operate_omega.cxx:60-97) is linear in the five
+    // product spectra P0..P4 = u2u3, u3u1, u1u2, u2^2-u3^2, u3^2-u1^2:
+    // accumulate the two components Omega_k2zeta_k keeps as each product is
+    // transformed.
+    C2 acc0[R], acc1[R];
+    int rows[R];
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
-    if (f < 4) fetch(f == 0 ? p1 : f == 1 ? p2 : f == 2 ? p3 : p4, nxt);
-    auto ld = [&](int, int, int q) -> C2 { return cur[q]; };
-    auto st = [&](int i, int, int q, C2 P) {
-      rows[q] = i;
-      const ModeK m = mode_k(g, i, j, kz);
-      const T k1 = (T)(m.kx * g.wave_x), k2 = (T)(m.ky * g.wave_y), k3 = (T)(m.kz * g.wave_z);
-      // coefficient of P_f in -Omega_s, s = 0, 1, 2
-      T cf[3];
-      if (f == 0) {
-        cf[0] = (k3 - k2) * (k3 + k2); cf[1] = k1 * k2; cf[2] = -(k3 * k1);
-      } else if (f == 1) {
-        cf[0] = -(k1 * k2); cf[1] = (k1 - k3) * (k1 + k3); cf[2] = k2 * k3;
-      } else if (f == 2) {
-        cf[0] = k3 * k1; cf[1] = -(k2 * k3); cf[2] = (k2 - k1) * (k2 + k1);
-      } else if (f == 3) {
-        cf[0] = k2 * k3; cf[1] = 0; cf[2] = -(k1 * k2);
-      } else {
-        cf[0] = 0; cf[1] = k3 * k1; cf[2] = -(k1 * k2);
-      }
-      const T a = m.kx != 0 ? cf[1] : (m.ky != 0 ? cf[2] : cf[0]);
-      const T b = m.kx != 0 ? cf[2] : (m.ky != 0 ? cf[0] : cf[1]);
-      acc0[q].x -= a * P.x;
-      acc0[q].y -= a * P.y;
-      acc1[q].x -= b * P.x;
-      acc1[q].y -= b * P.y;
-    };
-    fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
-    if (f < 4)
+    for (int q = 0; q < R; ++q) {
+      acc0[q] = acc1[q] = C2{0, 0};
+      rows[q] = 0;
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      if (f < 4)
+        fetch(t, f + 1, nxt);
+      else if (PERSIST && tn < ntiles)
+        fetch(tn, 0, nxt);
+      auto ld = [&](int, int, int q) -> C2 { return cur[q]; };
+      auto st = [&](int i, int, int q, C2 P) {
+        rows[q] = i;
+        const ModeK m = mode_k(g, i, j, kz);
+        const T k1 = (T)(m.kx * g.wave_x), k2 = (T)(m.ky * g.wave_y), k3 = (T)(m.kz * g.wave_z);
+        // coefficient of P_f in -Omega_s, s = 0, 1, 2
+        T cf[3];
+        if (f == 0) {
+          cf[0] = (k3 - k2) * (k3 + k2); cf[1] = k1 * k2; cf[2] = -(k3 * k1);
+        } else if (f == 1) {
+          cf[0] = -(k1 * k2); cf[1] = (k1 - k3) * (k1 + k3); cf[2] = k2 * k3;
+        } else if (f == 2) {
+          cf[0] = k3 * k1; cf[1] = -(k2 * k3); cf[2] = (k2 - k1) * (k2 + k1);
+        } else if (f == 3) {
+          cf[0] = k2 * k3; cf[1] = 0; cf[2] = -(k1 * k2);
+        } else {
+          cf[0] = 0; cf[1] = k3 * k1; cf[2] = -(k1 * k2);
+        }
+        const T a = m.kx != 0 ? cf[1] : (m.ky != 0 ? cf[2] : cf[0]);
+        const T b = m.kx != 0 ? cf[2] : (m.ky != 0 ? cf[0] : cf[1]);
+        acc0[q].x -= a * P.x;
+        acc0[q].y -= a * P.y;
+        acc1[q].x -= b * P.x;
+        acc1[q].y -= b * P.y;
+      };
+      fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
 #pragma unroll
       for (int q = 0; q < R; ++q) cur[q] = nxt[q];
-  }
-  if (!col_ok) return;
-  const T dmy0 = (T)(-nu * dt), dmy1 = (T)(1. / nu);
+    }
+    if (col_ok) {
+      const T dmy0 = (T)(-nu * dt), dmy1 = (T)(1. / nu);
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = rows[q];
-    const ModeK m = mode_k(g, i, j, kz);
-    const long e = 2 * (col + i * es);
-    if (!m.keep) continue;  // zeta stays zero outside the 2/3 box
-    const T kk2 = (T)(sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz));
-    const T ik2 = kk2 > (T)0 ? (T)1 / kk2 : (T)0;
-    const T dmy = exp(dmy0 * kk2) - (T)1;
-    C2 za = *reinterpret_cast<const C2 *>(z0 + e);
-    C2 zb = *reinterpret_cast<const C2 *>(z1 + e);
-    za.x += dmy * (za.x - dmy1 * ik2 * acc0[q].x);
-    za.y += dmy * (za.y - dmy1 * ik2 * acc0[q].y);
-    zb.x += dmy * (zb.x - dmy1 * ik2 * acc1[q].x);
-    zb.y += dmy * (zb.y - dmy1 * ik2 * acc1[q].y);
-    *reinterpret_cast<C2 *>(z0 + e) = za;
-    *reinterpret_cast<C2 *>(z1 + e) = zb;
+      for (int q = 0; q < R; ++q) {
+        const int i = rows[q];
+        const ModeK m = mode_k(g, i, j, kz);
+        const long e = 2 * (col + i * es);
+        if (!m.keep) continue;  // zeta stays zero outside the 2/3 box
+        const T kk2 = (T)(sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz));
+        const T ik2 = kk2 > (T)0 ? (T)1 / kk2 : (T)0;
+        const T dmy = exp(dmy0 * kk2) - (T)1;
+        C2 za = *reinterpret_cast<const C2 *>(z0 + e);
+        C2 zb = *reinterpret_cast<const C2 *>(z1 + e);
+        za.x += dmy * (za.x - dmy1 * ik2 * acc0[q].x);
+        za.y += dmy * (za.y - dmy1 * ik2 * acc0[q].y);
+        zb.x += dmy * (zb.x - dmy1 * ik2 * acc1[q].x);
+        zb.y += dmy * (zb.y - dmy1 * ik2 * acc1[q].y);
+        *reinterpret_cast<C2 *>(z0 + e) = za;
+        *reinterpret_cast<C2 *>(z1 + e) = zb;
+      }
+    }
+    if (!PERSIST || tn >= ntiles) break;
+    t = tn;
+    k = kn;
   }
 }
 
@@ -424,9 +494,10 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
 // gradient part removed by the projection to zero, while the projection
 // keeps the k = 0 mode (IK2(0) = 0).  So the result is the same field.
 // ---------------------------------------------------------------------
-template <typename T, int N, int R, int C>
+template <typename T, int N, int R, int C, bool PERSIST>
 __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0, const T *f1, const T *f2, T *z0,
-                                                            T *z1, double *uk_dc, const double2 *__restrict__ tw_g) {
+                                                            T *z1, double *uk_dc, const double2 *__restrict__ tw_g,
+                                                            int ntiles) {
   using C2 = cplx<T>;
   __shared__ C2 lds[N * C];
   const int tid = threadIdx.x;
@@ -437,73 +508,89 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0
   const int c = tid % C;
   const long H = g.hnz;
   const long es = (long)g.ny * H;
-  const int c0 = fused_x_tile<T>() * C;
-  // a column outside the 2/3 box gives only discarded modes, and zeta is
-  // zero there already
-  if (!tile_kept<C>(g, c0)) return;
-  const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
-  const int j = col / (int)H, kz = col - j * (int)H;
-  const bool col_ok = col < es && col_kept(g, j, kz);
-  C2 acc0[R], acc1[R];
-  int rows[R];
-  double f00[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    acc0[q] = acc1[q] = C2{0, 0};
-    rows[q] = 0;
-  }
-  // software pipeline over the three components, as in x_update_kernel
+  // kept tiles of the block's sequence (a column outside the 2/3 box gives
+  // only discarded modes, and zeta is zero there already)
+  int k = x_next_kept<T, C>(g, 0, ntiles);
+  int t = x_tile_seq<T>(k, ntiles);
+  if (t >= ntiles) return;
   const int j0 = tid / C;
-  auto fetch = [&](const T *in, C2 *dst) {
+  const T *const ff[3] = {f0, f1, f2};
+  auto fetch = [&](int tile, int f, C2 *dst) {
+    const int col = tile * C + c;
+    const bool ok = col < es && col_kept(g, col / (int)H, col - (col / (int)H) * (int)H);
 #pragma unroll
     for (int q = 0; q < R; ++q)
-      dst[q] = col_ok ? *reinterpret_cast<const C2 *>(in + 2 * (col + (long)(j0 + q * (N / R)) * es)) : C2{0, 0};
+      dst[q] = ok ? *reinterpret_cast<const C2 *>(ff[f] + 2 * (col + (long)(j0 + q * (N / R)) * es)) : C2{0, 0};
   };
+  // software pipeline over the three components and over tiles, as in
+  // x_update_kernel
   C2 cur[R], nxt[R];
-  fetch(f0, cur);
+  fetch(t, 0, cur);
+  for (;;) {
+    const int kn = x_next_kept<T, C>(g, k + 1, ntiles);
+    const int tn = x_tile_seq<T>(kn, ntiles);
+    const int c0 = t * C;
+    const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
+    const int j = col / (int)H, kz = col - j * (int)H;
+    const bool col_ok = col < es && col_kept(g, j, kz);
+    C2 acc0[R], acc1[R];
+    int rows[R];
+    double f00[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    if (f < 2) fetch(f == 0 ? f1 : f2, nxt);
-    auto ld = [&](int, int, int q) -> C2 { return cur[q]; };
-    auto st = [&](int i, int, int q, C2 F) {
-      rows[q] = i;
-      if (i == 0 && c == 0) f00[f] = F.x;
-      const ModeK m = mode_k(g, i, j, kz);
-      if (!m.keep) return;
-      const T kx = (T)(m.kx * g.wave_x), ky = (T)(m.ky * g.wave_y), kz = (T)(m.kz * g.wave_z);
-      // omega_s of U_k2zeta_k's curl gets (ks[a] F_b - ks[b] F_a), (s, a, b)
-      // cyclic; omega_re from Im F, omega_im from -Re F.  Coefficient of F_f in
-      // omega_s: (0,2) ky, (0,1) -kz, (1,0) kz, (1,2) -kx, (2,1) kx, (2,0) -ky.
-      auto coef = [&](int sidx) -> T {
-        if (sidx == 0) return f == 2 ? ky : (f == 1 ? -kz : (T)0);
-        if (sidx == 1) return f == 0 ? kz : (f == 2 ? -kx : (T)0);
-        return f == 1 ? kx : (f == 0 ? -ky : (T)0);
+    for (int q = 0; q < R; ++q) {
+      acc0[q] = acc1[q] = C2{0, 0};
+      rows[q] = 0;
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      if (f < 2)
+        fetch(t, f + 1, nxt);
+      else if (PERSIST && tn < ntiles)
+        fetch(tn, 0, nxt);
+      auto ld = [&](int, int, int q) -> C2 { return cur[q]; };
+      auto st = [&](int i, int, int q, C2 F) {
+        rows[q] = i;
+        if (i == 0 && c == 0) f00[f] = F.x;
+        const ModeK m = mode_k(g, i, j, kz);
+        if (!m.keep) return;
+        const T kx = (T)(m.kx * g.wave_x), ky = (T)(m.ky * g.wave_y), kz = (T)(m.kz * g.wave_z);
+        // omega_s of U_k2zeta_k's curl gets (ks[a] F_b - ks[b] F_a), (s, a, b)
+        // cyclic; omega_re from Im F, omega_im from -Re F.  Coefficient of F_f in
+        // omega_s: (0,2) ky, (0,1) -kz, (1,0) kz, (1,2) -kx, (2,1) kx, (2,0) -ky.
+        auto coef = [&](int sidx) -> T {
+          if (sidx == 0) return f == 2 ? ky : (f == 1 ? -kz : (T)0);
+          if (sidx == 1) return f == 0 ? kz : (f == 2 ? -kx : (T)0);
+          return f == 1 ? kx : (f == 0 ? -ky : (T)0);
+        };
+        const int s0 = m.kx != 0 ? 1 : (m.ky != 0 ? 2 : 0);
+        const int s1 = m.kx != 0 ? 2 : (m.ky != 0 ? 0 : 1);
+        const T a = coef(s0), b = coef(s1);
+        acc0[q].x += a * F.y;
+        acc0[q].y -= a * F.x;
+        acc1[q].x += b * F.y;
+        acc1[q].y -= b * F.x;
       };
-      const int s0 = m.kx != 0 ? 1 : (m.ky != 0 ? 2 : 0);
-      const int s1 = m.kx != 0 ? 2 : (m.ky != 0 ? 0 : 1);
-      const T a = coef(s0), b = coef(s1);
-      acc0[q].x += a * F.y;
-      acc0[q].y -= a * F.x;
-      acc1[q].x += b * F.y;
-      acc1[q].y -= b * F.x;
-    };
-    fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
-    if (f < 2)
+      fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
 #pragma unroll
       for (int q = 0; q < R; ++q) cur[q] = nxt[q];
-  }
-  const bool has0 = c0 == 0;
-  if (!col_ok) return;
+    }
+    const bool has0 = c0 == 0;
+    if (col_ok) {
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = rows[q];
-    const ModeK m = mode_k(g, i, j, kz);
-    const long e = 2 * (col + i * es);
-    if (has0 && i == 0 && c == 0)
-      for (int d = 0; d < 3; ++d) uk_dc[d] += f00[d];
-    if (!m.keep) continue;  // zeta stays zero outside the 2/3 box
-    *reinterpret_cast<C2 *>(z0 + e) = cadd(*reinterpret_cast<const C2 *>(z0 + e), acc0[q]);
-    *reinterpret_cast<C2 *>(z1 + e) = cadd(*reinterpret_cast<const C2 *>(z1 + e), acc1[q]);
+      for (int q = 0; q < R; ++q) {
+        const int i = rows[q];
+        const ModeK m = mode_k(g, i, j, kz);
+        const long e = 2 * (col + i * es);
+        if (has0 && i == 0 && c == 0)
+          for (int d = 0; d < 3; ++d) uk_dc[d] += f00[d];
+        if (!m.keep) continue;  // zeta stays zero outside the 2/3 box
+        *reinterpret_cast<C2 *>(z0 + e) = cadd(*reinterpret_cast<const C2 *>(z0 + e), acc0[q]);
+        *reinterpret_cast<C2 *>(z1 + e) = cadd(*reinterpret_cast<const C2 *>(z1 + e), acc1[q]);
+      }
+    }
+    if (!PERSIST || tn >= ntiles) break;
+    t = tn;
+    k = kn;
   }
 }
 
@@ -542,14 +629,40 @@ inline T *F(double *p) {
   return reinterpret_cast<T *>(p);
 }
 
+// Blocks of a persistent x pass: `per_cu` per compute unit (0: one block
+// per tile).
+#ifndef SPM_X_ZETA2U_BLOCKS_F64
+#define SPM_X_ZETA2U_BLOCKS_F64 1
+#endif
+#ifndef SPM_X_ZETA2U_BLOCKS_F32
+#define SPM_X_ZETA2U_BLOCKS_F32 0
+#endif
+#ifndef SPM_X_UPDATE_BLOCKS_F64
+#define SPM_X_UPDATE_BLOCKS_F64 1
+#endif
+#ifndef SPM_X_UPDATE_BLOCKS_F32
+#define SPM_X_UPDATE_BLOCKS_F32 0
+#endif
+#ifndef SPM_X_PROJECT_BLOCKS_F64
+#define SPM_X_PROJECT_BLOCKS_F64 1
+#endif
+#ifndef SPM_X_PROJECT_BLOCKS_F32
+#define SPM_X_PROJECT_BLOCKS_F32 0
+#endif
+inline int x_grid(const spm_ctx *c, int ntiles, int per_cu) {
+  const int n = per_cu > 0 ? per_cu * c->n_cu : ntiles;
+  return n < ntiles ? n : ntiles;
+}
+
 template <typename T>
 int fx_zeta2u_t(spm_ctx *c, int fix_mask) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
     constexpr int R = kFusedR(S), C = kFusedC(S);
-    hipLaunchKernelGGL((x_zeta2u_kernel<T, S, R, C>), dim3((g.ny * g.hnz + C - 1) / C), dim3(C * S / R), 0, c->stream,
-                       g, F<T>(c->zeta[0]), F<T>(c->zeta[1]), F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), c->uk_dc,
-                       fix_mask, c->tw_x);
+    const int ntiles = (g.ny * g.hnz + C - 1) / C;
+    constexpr int per_cu = sizeof(T) == 8 ? SPM_X_ZETA2U_BLOCKS_F64 : SPM_X_ZETA2U_BLOCKS_F32;
+    hipLaunchKernelGGL((x_zeta2u_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->zeta[0]), F<T>(c->zeta[1]), F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]),
+                       c->uk_dc, fix_mask, c->tw_x, ntiles);
   });
   return 0;
 }
@@ -572,9 +685,11 @@ int fx_update_t(spm_ctx *c, double nu, double dt) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
     constexpr int R = kFusedR(S), C = kFusedC(S);
-    hipLaunchKernelGGL((x_update_kernel<T, S, R, C>), dim3((g.ny * g.hnz + C - 1) / C), dim3(C * S / R), 0, c->stream,
-                       g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
-                       F<T>(c->zeta[0]), F<T>(c->zeta[1]), nu, dt, c->tw_x);
+    const int ntiles = (g.ny * g.hnz + C - 1) / C;
+    constexpr int per_cu = sizeof(T) == 8 ? SPM_X_UPDATE_BLOCKS_F64 : SPM_X_UPDATE_BLOCKS_F32;
+    hipLaunchKernelGGL((x_update_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->u[0]),
+                       F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]), F<T>(c->zeta[0]),
+                       F<T>(c->zeta[1]), nu, dt, c->tw_x, ntiles);
   });
   return 0;
 }
@@ -584,9 +699,10 @@ int fx_project_t(spm_ctx *c) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
     constexpr int R = kFusedR(S), C = kFusedC(S);
-    hipLaunchKernelGGL((x_project_kernel<T, S, R, C>), dim3((g.ny * g.hnz + C - 1) / C), dim3(C * S / R), 0, c->stream,
-                       g, F<T>(c->f[0]), F<T>(c->f[1]), F<T>(c->f[2]), F<T>(c->zeta[0]), F<T>(c->zeta[1]), c->uk_dc,
-                       c->tw_x);
+    const int ntiles = (g.ny * g.hnz + C - 1) / C;
+    constexpr int per_cu = sizeof(T) == 8 ? SPM_X_PROJECT_BLOCKS_F64 : SPM_X_PROJECT_BLOCKS_F32;
+    hipLaunchKernelGGL((x_project_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->f[0]),
+                       F<T>(c->f[1]), F<T>(c->f[2]), F<T>(c->zeta[0]), F<T>(c->zeta[1]), c->uk_dc, c->tw_x, ntiles);
   });
   return 0;
 }

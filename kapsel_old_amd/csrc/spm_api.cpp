@@ -352,6 +352,11 @@ int dalloc(spm_ctx *c, P **p, size_t count) {
 
 int setup(spm_ctx *c) {
   const spm_params &p = c->prm;
+  {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && n > 0)
+      c->n_cu = n;
+  }
   Grid &g = c->grid;
   g.nx = p.nx;
   g.ny = p.ny;

@@ -117,6 +117,7 @@ struct spm_ctx {
   spm::Grid grid;
   spm::PModel pm;
   int device = 0;
+  int n_cu = 256;  // compute units of the device (persistent launches)
   hipStream_t stream = nullptr;
   std::string err;
   size_t nfield = 0;  // elements per field
