@@ -305,6 +305,35 @@ int spm_rescale_solute(spm_ctx *ctx);
  * Coulomb body force on the fluid into the f field (real space). */
 int spm_make_coulomb_force(spm_ctx *ctx);
 
+/* ---- Lees-Edwards shear in oblique coordinates, fluid only ---------------
+ * SW_EQ == Shear_Navier_Stokes_Lees_Edwards with no particles: spm_step
+ * runs Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390).  The grid shears
+ * with the flow; degree_oblique (the strain since the last remap, a global
+ * of the reference, input.cxx:271) grows by Shear_rate * dt each step and
+ * the grid is remapped when it reaches 1.  Shear_rate_eff (sp_3d_ns.h:
+ * 174-196) is measured from the velocity at the end of each step and drives
+ * the next step's shear term.  zeta holds the oblique components
+ * (U_k2zeta_k_OBL, fft_wrapper.cxx:341-406) and uk_dc the contravariant
+ * mean flow.  FP64, literal schedule. */
+/* Switches the context to the sheared equations; Shear_rate is
+ * constitutive_eq.Shear_Navier_Stokes_Lees_Edwards.External_field.DC.Shear_rate
+ * (input.cxx:600-646).  Call once, after spm_create, with n_particles == 0. */
+int spm_set_lees_edwards(spm_ctx *ctx, double shear_rate);
+/* degree_oblique and Shear_rate_eff in and out (the restart's
+ * resume.CONTINUE.Saved_Data.oblique, resume.cxx:335-339). */
+int spm_put_oblique(spm_ctx *ctx, double degree_oblique, double shear_rate_eff);
+int spm_get_oblique(spm_ctx *ctx, double *degree_oblique, double *shear_rate_eff);
+/* NS_solver_slavedEuler_Shear_OBL (fluid_solver.cxx:398-429) with
+ * Zeta_k2advection_k_OBL (operate_omega.cxx:115-173). */
+int spm_ns_solver_slaved_euler_shear_obl(spm_ctx *ctx);
+/* Zeta_k2u_k_OBL (fft_wrapper.cxx:513-584): the contravariant velocity
+ * spectrum of zeta into u0..u2. */
+int spm_zeta_k2u_k_obl(spm_ctx *ctx);
+/* Solenoidal_uk_OBL (operate_omega.cxx:222-262), contra2co (fft_wrapper.h:
+ * 327-361) and U_k2zeta_k_OBL of the contravariant spectrum in u0..u2 into
+ * zeta and uk_dc (u0..u2 are left holding the covariant spectrum). */
+int spm_u_k2zeta_k_obl(spm_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

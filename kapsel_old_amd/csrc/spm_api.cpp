@@ -327,7 +327,7 @@ void free_ctx(spm_ctx *c) {
                     c->pres, c->noise_dev})
     if (p) (void)hipFree(p);
   for (double *p : {c->conc[0], c->conc[1], c->conc_rhs[0], c->conc_rhs[1], c->surface_normal[0],
-                    c->surface_normal[1], c->surface_normal[2], c->f_ns1[0], c->f_ns1[1], c->red})
+                    c->surface_normal[1], c->surface_normal[2], c->f_ns1[0], c->f_ns1[1], c->red, c->le_red})
     if (p) (void)hipFree(p);
   for (int *p : {c->spec, c->stencil, c->pxint, c->tile_count, c->tile_start, c->tile_cursor, c->tile_list,
                  c->cell_count, c->cell_start, c->cell_cursor, c->cell_list, c->pcell, c->flag})
@@ -743,6 +743,58 @@ int set_electrolyte(spm_ctx *c, const spm_electrolyte *el) {
   return 0;
 }
 
+// SW_EQ == Shear_Navier_Stokes_Lees_Edwards with no particles
+// (Time_evolution_hydro_OBL, sp_3d_ns.cxx:214-390); the shear rate comes
+// from the UDF's constitutive_eq.Shear_Navier_Stokes_Lees_Edwards.
+// External_field.DC.Shear_rate (input.cxx:600-646); degree_oblique is a
+// zero-initialised global (input.cxx:271).
+int set_lees_edwards(spm_ctx *c, double shear_rate) {
+  if (!std::isfinite(shear_rate)) {
+    c->err = "invalid shear rate";
+    return SPM_ERR_INVALID;
+  }
+  if (c->le || c->electro) {
+    c->err = "the context already runs another set of equations";
+    return SPM_ERR_INVALID;
+  }
+  if (c->prm.precision != SPM_FP64) {
+    c->err = "the Lees-Edwards path runs in FP64 only";
+    return SPM_ERR_UNSUPPORTED;
+  }
+  if (c->prm.n_particles > 0) {
+    c->err = "the Lees-Edwards path runs the fluid without particles";
+    return SPM_ERR_UNSUPPORTED;
+  }
+  if (!c->work[2]) RC(dalloc(c, &c->work[2], c->nfield));
+  RC(dalloc(c, &c->le_red, (size_t)le_red_slots()));  // zeroes Shear_rate_eff
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  c->le_rate = shear_rate;
+  c->le_deg = 0.0;
+  c->le = true;
+  return 0;
+}
+
+// degree_oblique lives on the host (it selects the remap), Shear_rate_eff on
+// the device (the step measures it and the next step's advection reads it).
+int put_oblique(spm_ctx *c, double deg, double sre) {
+  SPM_HIP(c, hipMemcpyAsync(c->le_red + le_red_slots() - 2, &sre, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  c->le_deg = deg;
+  return 0;
+}
+
+int get_oblique(spm_ctx *c, double *deg, double *sre) {
+  SPM_HIP(c, hipMemcpyAsync(sre, c->le_red + le_red_slots() - 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  *deg = c->le_deg;
+  return 0;
+}
+
+int le_zeta_k2u_k_api(spm_ctx *c) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  return le_zeta2u_k(c, u3, 0);
+}
+
 thread_local std::string g_create_err;
 
 }  // namespace
@@ -990,13 +1042,16 @@ static int step_n(spm_ctx *c, int nsteps) {
   // The fused passes carry only the modes the 2/3 rule keeps and leave zeta
   // untouched outside them; the reference's first act in a step is the same
   // truncation (Zeta_k2advection_k, operate_omega.cxx:102-108).
-  if (fused && !c->electro && nsteps > 0) {
+  const bool fused_ns = fused && !c->electro && !c->le;
+  if (fused_ns && nsteps > 0) {
     RC(k_truncate(c, c->zeta[0]));
     RC(k_truncate(c, c->zeta[1]));
   }
   for (int s = 0; s < nsteps; ++s) {
     if (c->electro)
       RC(electrolyte_step(c));
+    else if (c->le)
+      RC(le_step(c, fix_mask(c)));
     else
       RC(fused ? fused_step(c) : time_evolution_hydro(c));
     c->time += c->prm.dt;
@@ -1007,7 +1062,7 @@ static int step_n(spm_ctx *c, int nsteps) {
       RC(fold_timing(c));
     }
   }
-  if (fused && !c->electro && nsteps > 0) RC(materialize(c));
+  if (fused_ns && nsteps > 0) RC(materialize(c));
   return 0;
 }
 
@@ -1130,5 +1185,28 @@ int spm_get_total_solute(spm_ctx *c, double *total) {
   for (int n = 0; n < c->el.n_spec; ++n) total[n] = c->total_solute[n];
   return SPM_OK;
 }
+
+// ---- Lees-Edwards shear, fluid only ----
+#define LE_ENTRY(body)                                       \
+  do {                                                       \
+    if (!c) return SPM_ERR_INVALID;                          \
+    if (!c->le) {                                            \
+      c->err = "spm_set_lees_edwards has not been called";   \
+      return SPM_ERR_INVALID;                                \
+    }                                                        \
+    ENTRY(body);                                             \
+  } while (0)
+
+int spm_set_lees_edwards(spm_ctx *c, double shear_rate) { ENTRY(set_lees_edwards(c, shear_rate)); }
+int spm_put_oblique(spm_ctx *c, double degree_oblique, double shear_rate_eff) {
+  LE_ENTRY(put_oblique(c, degree_oblique, shear_rate_eff));
+}
+int spm_get_oblique(spm_ctx *c, double *degree_oblique, double *shear_rate_eff) {
+  if (!degree_oblique || !shear_rate_eff) return SPM_ERR_INVALID;
+  LE_ENTRY(get_oblique(c, degree_oblique, shear_rate_eff));
+}
+int spm_ns_solver_slaved_euler_shear_obl(spm_ctx *c) { LE_ENTRY(le_ns_solver(c)); }
+int spm_zeta_k2u_k_obl(spm_ctx *c) { LE_ENTRY(le_zeta_k2u_k_api(c)); }
+int spm_u_k2zeta_k_obl(spm_ctx *c) { LE_ENTRY(le_u_k2zeta_k(c)); }
 
 }  // extern "C"

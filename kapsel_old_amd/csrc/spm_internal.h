@@ -181,6 +181,12 @@ struct spm_ctx {
 
   // electrolyte (SW_EQ == Electrolyte): ion spectra, their right-hand sides,
   // surface normals, the solvent right-hand side, reduction scratch
+  // Lees-Edwards shear (SW_EQ == Shear_Navier_Stokes_Lees_Edwards), fluid
+  // only: Shear_rate, degree_oblique (host), Shear_rate_eff (device, after
+  // the reduction partials of le_red)
+  bool le = false;
+  double le_rate = 0.0, le_deg = 0.0;
+  double *le_red = nullptr;
   bool electro = false;
   spm_electrolyte el{};
   double *conc[2] = {nullptr, nullptr};
@@ -278,6 +284,12 @@ int p_make_f(spm_ctx *c);
 int p_init_lists(spm_ctx *c);
 int p_calc_f_hydro_field(spm_ctx *c, double *const fld[3]);
 int electrolyte_step(spm_ctx *c);
+// Lees-Edwards (shear.hip)
+int le_red_slots();
+int le_zeta2u_k(spm_ctx *c, double *const u[3], int fix_mask);
+int le_u_k2zeta_k(spm_ctx *c);
+int le_ns_solver(spm_ctx *c);
+int le_step(spm_ctx *c, int fix_mask);
 
 // electrolyte kernels (electrolyte.hip)
 int e_red_slots();

@@ -30,6 +30,8 @@ SYMBOLS = [
     "spm_set_electrolyte", "spm_init_rho_ion", "spm_put_concentration", "spm_get_concentration",
     "spm_put_total_solute", "spm_get_total_solute", "spm_nssolute_solver_euler", "spm_rescale_solute",
     "spm_make_coulomb_force", "spm_save_state", "spm_load_state",
+    "spm_set_lees_edwards", "spm_put_oblique", "spm_get_oblique", "spm_ns_solver_slaved_euler_shear_obl",
+    "spm_zeta_k2u_k_obl", "spm_u_k2zeta_k_obl",
 ]
 
 SCHEDULE_FUSED, SCHEDULE_LITERAL = 0, 1
@@ -92,6 +94,11 @@ def lib():
         L.spm_get_concentration.argtypes = [vp, ip, dp]
         L.spm_save_state.argtypes = [vp, C.c_char_p]
         L.spm_load_state.argtypes = [vp, C.c_char_p]
+        L.spm_set_lees_edwards.argtypes = [vp, C.c_double]
+        L.spm_put_oblique.argtypes = [vp, C.c_double, C.c_double]
+        L.spm_get_oblique.argtypes = [vp, dp, dp]
+        for name in ["spm_ns_solver_slaved_euler_shear_obl", "spm_zeta_k2u_k_obl", "spm_u_k2zeta_k_obl"]:
+            getattr(L, name).argtypes = [vp]
         L.spm_put_total_solute.argtypes = [vp, dp]
         L.spm_get_total_solute.argtypes = [vp, dp]
         for name in SYMBOLS:
@@ -189,6 +196,26 @@ class SPM:
         tt = np.zeros(2)
         tt[:len(t)] = t
         self._call("put_total_solute", _dp(tt))
+
+    # ---- Lees-Edwards shear, fluid only (SW_EQ == Shear_Navier_Stokes_Lees_Edwards) ----
+    def set_lees_edwards(self, shear_rate):
+        self._call("set_lees_edwards", C.c_double(shear_rate))
+
+    def put_oblique(self, degree_oblique, shear_rate_eff):
+        self._call("put_oblique", C.c_double(degree_oblique), C.c_double(shear_rate_eff))
+
+    def get_oblique(self):
+        """(degree_oblique, Shear_rate_eff)."""
+        d, s = C.c_double(), C.c_double()
+        self._call("get_oblique", C.byref(d), C.byref(s))
+        return d.value, s.value
+
+    def NS_solver_slavedEuler_Shear_OBL(self): self._call("ns_solver_slaved_euler_shear_obl")
+    def Zeta_k2u_k_OBL(self): self._call("zeta_k2u_k_obl")
+
+    def U_k2zeta_k_OBL(self):
+        """Solenoidal_uk_OBL, contra2co and U_k2zeta_k_OBL of the spectrum in u."""
+        self._call("u_k2zeta_k_obl")
 
     def fused_stage(self, stage):
         """Test hook: one stage of the fused schedule (see include/spm.h)."""

@@ -89,6 +89,13 @@ def lib():
         L.oracle_make_phi_qq.restype = None
         L.oracle_make_surface_normal.argtypes = [vp, PP]
         L.oracle_make_surface_normal.restype = None
+        L.oracle_set_lees_edwards.argtypes = [vp, C.c_double]
+        L.oracle_put_oblique.argtypes = [vp, C.c_double, C.c_double]
+        L.oracle_get_oblique.argtypes = [vp, dp, dp]
+        for name in ["ns_solver_shear_obl", "zeta_k2u_k_obl", "u_k2zeta_k_obl"]:
+            f = getattr(L, "oracle_" + name)
+            f.argtypes = [vp]
+            f.restype = ip
         _lib = L
     return _lib
 
@@ -182,6 +189,24 @@ class Oracle:
         n, ptrs = self._vec([np.zeros(self.shape) for _ in range(3)])
         lib().oracle_make_surface_normal(self._h, ptrs)
         return n
+
+    # Lees-Edwards shear, fluid only (SW_EQ == Shear_Navier_Stokes_Lees_Edwards)
+    def set_lees_edwards(self, shear_rate):
+        self._call("set_lees_edwards", C.c_double(shear_rate))
+
+    def put_oblique(self, degree_oblique, shear_rate_eff):
+        self._call("put_oblique", C.c_double(degree_oblique), C.c_double(shear_rate_eff))
+
+    def get_oblique(self):
+        d, s = C.c_double(), C.c_double()
+        self._call("get_oblique", C.byref(d), C.byref(s))
+        return d.value, s.value
+
+    def NS_solver_slavedEuler_Shear_OBL(self): self._call("ns_solver_shear_obl")
+    def Zeta_k2u_k_OBL(self): self._call("zeta_k2u_k_obl")
+    def U_k2zeta_k_OBL(self):
+        """Solenoidal_uk_OBL, contra2co and U_k2zeta_k_OBL of the spectrum in u."""
+        self._call("u_k2zeta_k_obl")
 
     def step(self, nsteps=1):
         self._call("step", nsteps)

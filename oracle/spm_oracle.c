@@ -63,6 +63,10 @@ struct oracle_ctx {
   double total_solute[2];
   double surface_charge_e[SPM_MAX_SPECIES];
   double valency_e[2];
+  /* Lees-Edwards shear (SW_EQ == Shear_Navier_Stokes_Lees_Edwards) */
+  int le;
+  double shear_rate, shear_rate_eff, degree_oblique;
+  double *ucp[3], *wv3[3], *wv1;
 };
 
 /* ------------------------------------------------------------------ */
@@ -1420,10 +1424,18 @@ static void electrolyte_step(oracle_ctx *c) {
 
 /* Time_evolution_hydro (sp_3d_ns.cxx:61-212) for Navier_Stokes with free
  * spherical particles, then jikan.time += dt and ts++ (:593-594). */
+static void le_step(oracle_ctx *c);
+
 int oracle_step(oracle_ctx *c, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     if (c->electro) {
       electrolyte_step(c);
+      c->time += c->dt;
+      c->ts++;
+      continue;
+    }
+    if (c->le) {
+      le_step(c);
       c->time += c->dt;
       c->ts++;
       continue;
@@ -1440,6 +1452,371 @@ int oracle_step(oracle_ctx *c, int nsteps) {
     c->time += c->dt;
     c->ts++;
   }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Lees-Edwards shear in oblique coordinates, fluid only              */
+/* (Time_evolution_hydro_OBL, sp_3d_ns.cxx:214-390)                    */
+/* ------------------------------------------------------------------ */
+/* contra2co_single / co2contra_single, fft_wrapper.h:398-424 */
+static void contra2co_single(double deg, double v[3]) {
+  const double d0 = v[0], d1 = v[1], d2 = v[2];
+  v[0] = d0 + deg * d1;
+  v[1] = deg * d0 + (1. + deg * deg) * d1;
+  v[2] = d2;
+}
+static void co2contra_single(double deg, double v[3]) {
+  const double d0 = v[0], d1 = v[1], d2 = v[2];
+  v[0] = (1. + deg * deg) * d0 - deg * d1;
+  v[1] = -deg * d0 + d1;
+  v[2] = d2;
+}
+
+/* K2 / IK2 for the current degree_oblique, sp_3d_ns.cxx:319-334 */
+static void obl_update_k2(oracle_ctx *c) {
+  const double deg = c->degree_oblique;
+  for (size_t im = 0; im < c->nfield; im++) {
+    c->K2[im] = SQ(c->WAVE_X * c->KX_int[im]) + SQ(c->WAVE_Y * c->KY_int[im] - c->WAVE_X * deg * c->KX_int[im]) +
+                SQ(c->WAVE_Z * c->KZ_int[im]);
+    c->IK2[im] = c->K2[im] > 0.0 ? 1. / c->K2[im] : 0.0;
+  }
+}
+
+/* Zeta_k2u_k_OBL, fft_wrapper.cxx:513-584 (u contravariant) */
+static void zeta_k2u_k_obl(oracle_ctx *c, double *const zeta[2], const double uk_dc[3], double *const u[3]) {
+  const double deg = c->degree_oblique;
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->HNZ_; k++) {
+        const size_t im = IDX(c, i, j, 2 * k);
+        double kx = c->WAVE_X * c->KX_int[im], ky = c->WAVE_Y * c->KY_int[im], kz = c->WAVE_Z * c->KZ_int[im];
+        const double ik2 = c->IK2[im];
+        const double d1r = zeta[0][im], d1i = zeta[0][im + 1], d2r = zeta[1][im], d2i = zeta[1][im + 1];
+        double wr[3], wi[3];
+        if (c->KX_int[im] != 0) {
+          wr[0] = -(1. / kx) * (ky * d1r + kz * d2r);
+          wi[0] = -(1. / kx) * (ky * d1i + kz * d2i);
+          wr[1] = d1r; wi[1] = d1i;
+          wr[2] = d2r; wi[2] = d2i;
+        } else if (c->KZ_int[im] != 0) {
+          const double dmy = -(ky / kz);
+          wr[0] = d2r; wi[0] = d2i;
+          wr[1] = d1r; wi[1] = d1i;
+          wr[2] = dmy * d1r; wi[2] = dmy * d1i;
+        } else {
+          wr[0] = d1r; wi[0] = d1i;
+          wr[1] = 0.0; wi[1] = 0.0;
+          wr[2] = d2r; wi[2] = d2i;
+        }
+        contra2co_single(deg, wr);
+        contra2co_single(deg, wi);
+        kx *= ik2;
+        ky *= ik2;
+        kz *= ik2;
+        u[0][im] = (ky * wi[2] - kz * wi[1]);
+        u[0][im + 1] = -(ky * wr[2] - kz * wr[1]);
+        u[1][im] = (kz * wi[0] - kx * wi[2]);
+        u[1][im + 1] = -(kz * wr[0] - kx * wr[2]);
+        u[2][im] = (kx * wi[1] - ky * wi[0]);
+        u[2][im + 1] = -(kx * wr[1] - ky * wr[0]);
+      }
+  u[0][0] = uk_dc[0];
+  u[1][0] = uk_dc[1];
+  u[2][0] = uk_dc[2];
+}
+
+/* Zeta_k2omega_k_OBL, fft_wrapper.cxx:586-632 (every interleaved slot) */
+static void zeta_k2omega_k_obl(oracle_ctx *c, double *const zeta[2], double *const omega[3]) {
+  for (size_t im = 0; im < c->nfield; im++) {
+    const double kx = c->WAVE_X * c->KX_int[im], ky = c->WAVE_Y * c->KY_int[im], kz = c->WAVE_Z * c->KZ_int[im];
+    const double d1 = zeta[0][im], d2 = zeta[1][im];
+    if (c->KX_int[im] != 0) {
+      omega[0][im] = -(1. / kx) * (ky * d1 + kz * d2);
+      omega[1][im] = d1;
+      omega[2][im] = d2;
+    } else if (c->KZ_int[im] != 0) {
+      const double dmy = -(ky / kz);
+      omega[0][im] = d2;
+      omega[1][im] = d1;
+      omega[2][im] = dmy * d1;
+    } else {
+      omega[0][im] = d1;
+      omega[1][im] = 0.0;
+      omega[2][im] = d2;
+    }
+  }
+  for (int d = 0; d < DIM; d++) omega[d][0] = 0.0;
+}
+
+/* U_k2rotation_k, fft_wrapper.cxx:679-718 (in place, Cartesian ks) */
+static void u_k2rotation_k(oracle_ctx *c, double *const u[3]) {
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->HNZ_; k++) {
+        const size_t im = IDX(c, i, j, 2 * k);
+        const double ks[3] = {c->KX_int[im] * c->WAVE_X, c->KY_int[im] * c->WAVE_Y, c->KZ_int[im] * c->WAVE_Z};
+        double re[3], imv[3], rr[3], ri[3];
+        for (int d = 0; d < DIM; d++) {
+          re[d] = u[d][im];
+          imv[d] = u[d][im + 1];
+        }
+        rr[0] = (ks[1] * imv[2] - ks[2] * imv[1]);
+        ri[0] = -(ks[1] * re[2] - ks[2] * re[1]);
+        rr[1] = (ks[2] * imv[0] - ks[0] * imv[2]);
+        ri[1] = -(ks[2] * re[0] - ks[0] * re[2]);
+        rr[2] = (ks[0] * imv[1] - ks[1] * imv[0]);
+        ri[2] = -(ks[0] * re[1] - ks[1] * re[0]);
+        for (int d = 0; d < DIM; d++) {
+          u[d][im] = rr[d];
+          u[d][im + 1] = ri[d];
+        }
+      }
+}
+
+/* Omega_k2zeta_k_OBL, fft_wrapper.cxx:175-198 (every interleaved slot) */
+static void omega_k2zeta_k_obl(oracle_ctx *c, double *const omega[3], double *const zetak[2]) {
+  for (size_t im = 0; im < c->nfield; im++) {
+    if (c->KX_int[im] != 0) {
+      zetak[0][im] = omega[1][im];
+      zetak[1][im] = omega[2][im];
+    } else if (c->KZ_int[im] != 0) {
+      zetak[0][im] = omega[1][im];
+      zetak[1][im] = omega[0][im];
+    } else {
+      zetak[0][im] = omega[0][im];
+      zetak[1][im] = omega[2][im];
+    }
+  }
+}
+
+/* Zeta_k2advection_k_OBL, operate_omega.cxx:115-173 */
+static void zeta_k2advection_k_obl(oracle_ctx *c, double *const adv[2]) {
+  double *const *u = c->u;
+  for (int d = 0; d < DIM - 1; d++) oracle_truncate(c, c->zeta[d]);
+  /* Zeta_k2u_cpuky, operate_omega.h:144-156 */
+  zeta_k2u_k_obl(c, c->zeta, c->uk_dc, u);
+  memcpy(c->wv1, u[1], sizeof(double) * c->nfield);
+  u_k2u(c, u);
+  /* Zeta_k2omega_OBL, operate_omega.h:169-175 */
+  zeta_k2omega_k_obl(c, c->zeta, c->wv3);
+  u_k2u(c, c->wv3);
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) {
+        const size_t im = IDX(c, i, j, k);
+        const double u1 = u[0][im], u2 = u[1][im], u3 = u[2][im];
+        const double w1 = c->wv3[0][im], w2 = c->wv3[1][im], w3 = c->wv3[2][im];
+        u[0][im] = u2 * w3 - u3 * w2;
+        u[1][im] = u3 * w1 - u1 * w3;
+        u[2][im] = u1 * w2 - u2 * w1;
+      }
+  u2u_k(c, u);
+  for (size_t im = 0; im < c->nfield; im++) {
+    u[0][im] -= 2. * c->shear_rate_eff * c->wv1[im];
+    u[1][im] -= 2. * c->shear_rate_eff * c->degree_oblique * c->wv1[im];
+  }
+  u_k2rotation_k(c, u);
+  omega_k2zeta_k_obl(c, u, adv);
+}
+
+/* NS_solver_slavedEuler_Shear_OBL, fluid_solver.cxx:398-429 */
+int oracle_ns_solver_shear_obl(oracle_ctx *c) {
+  if (!c->le) return -1;
+  zeta_k2advection_k_obl(c, c->f_ns0);
+  const double dmy0 = -c->NU * c->dt;
+  const double dmy1 = 1. / c->NU;
+  int rng[4][6];
+  ijk_blocks(c, rng);
+  for (int n = 0; n < 4; n++)
+    for (int i = rng[n][0]; i <= rng[n][1]; i++)
+      for (int j = rng[n][2]; j <= rng[n][3]; j++)
+        for (int k = rng[n][4]; k <= rng[n][5]; k++) {
+          const size_t im = IDX(c, i, j, k);
+          const double dmy = exp(dmy0 * c->K2[im]) - 1.;
+          c->zeta[0][im] += dmy * (c->zeta[0][im] - dmy1 * c->IK2[im] * c->f_ns0[0][im]);
+          c->zeta[1][im] += dmy * (c->zeta[1][im] - dmy1 * c->IK2[im] * c->f_ns0[1][im]);
+        }
+  return 0;
+}
+
+/* U_oblique2u, fft_wrapper.h:265-325: oblique contravariant components on
+ * the sheared grid -> Cartesian components on the orthogonal grid, plus the
+ * mean shear flow */
+static void u_oblique2u(oracle_ctx *c, double *const uu[3]) {
+  const int NX = c->NX, NY = c->NY, NZ = c->NZ;
+  const double deg = c->degree_oblique;
+  for (int d = 0; d < DIM; d++) memcpy(c->wv3[d], uu[d], sizeof(double) * c->nfield);
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++) {
+      double sign = j - NY / 2;
+      if (!(sign == 0)) sign = sign / fabs(sign);
+      int i_oblique = (int)(sign * deg * (j - NY / 2.)) * sign + sign;
+      const double alpha = (i_oblique - deg * (j - NY / 2.)) * sign;
+      const double beta = 1. - alpha;
+      i_oblique = (int)fmod(i + i_oblique + 4. * NX, NX);
+      const int i_plus = (int)fmod(i + sign + 2 * NX, NX);
+      for (int k = 0; k < NZ; k++) {
+        const size_t im = IDX(c, i, j, k), im_ob = IDX(c, i_oblique, j, k), im_p = IDX(c, i_plus, j, k);
+        uu[0][im_ob] = (beta * c->wv3[0][im] + alpha * c->wv3[0][im_p]) +
+                       deg * (beta * c->wv3[1][im] + alpha * c->wv3[1][im_p]) + c->shear_rate_eff * (j - NY / 2.);
+        uu[1][im_ob] = beta * c->wv3[1][im] + alpha * c->wv3[1][im_p];
+        uu[2][im_ob] = beta * c->wv3[2][im] + alpha * c->wv3[2][im_p];
+      }
+    }
+}
+
+/* Calc_shear_rate_eff, sp_3d_ns.h:174-196 (ivolume = Ivolume DX^3) */
+static void calc_shear_rate_eff(oracle_ctx *c) {
+  const double ivolume = (1. / (c->L[0] * c->L[1] * c->L[2])) * POW3(c->DX);
+  double s = 0.0;
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY - 1; j++)
+      for (int k = 0; k < c->NZ; k++) s += (c->ucp[0][IDX(c, i, j + 1, k)] - c->ucp[0][IDX(c, i, j, k)]) / c->DX;
+  s *= ivolume * c->NY / (c->NY - 1);
+  c->shear_rate_eff = s;
+}
+
+/* Solenoidal_uk_OBL, operate_omega.cxx:222-262 */
+static void solenoidal_uk_obl(oracle_ctx *c, double *const u[3]) {
+  const double deg = c->degree_oblique;
+  for (size_t im = 0; im < c->nfield; im++) {
+    const double kx = c->KX_int[im] * c->WAVE_X, ky = c->KY_int[im] * c->WAVE_Y, kz = c->KZ_int[im] * c->WAVE_Z;
+    const double kxc = (1. + deg * deg) * c->KX_int[im] * c->WAVE_X - deg * c->KY_int[im] * c->WAVE_Y;
+    const double kyc = -deg * c->KX_int[im] * c->WAVE_X + c->KY_int[im] * c->WAVE_Y;
+    const double dmy = c->IK2[im] * (u[0][im] * kx + u[1][im] * ky + u[2][im] * kz);
+    u[0][im] -= dmy * kxc;
+    u[1][im] -= dmy * kyc;
+    u[2][im] -= dmy * kz;
+  }
+}
+
+/* contra2co, fft_wrapper.h:327-361 (every slot) */
+static void contra2co_field(oracle_ctx *c, double *const u[3]) {
+  const double deg = c->degree_oblique;
+  for (size_t im = 0; im < c->nfield; im++) {
+    const double w0 = u[0][im], w1 = u[1][im], w2 = u[2][im];
+    u[0][im] = w0 + deg * w1;
+    u[1][im] = deg * w0 + (1. + deg * deg) * w1;
+    u[2][im] = w2;
+  }
+}
+
+/* U_k2zeta_k_OBL, fft_wrapper.cxx:341-406 */
+static void u_k2zeta_k_obl(oracle_ctx *c, double *const u[3], double *const zeta[2], double uk_dc[3]) {
+  uk_dc[0] = u[0][0];
+  uk_dc[1] = u[1][0];
+  uk_dc[2] = u[2][0];
+  co2contra_single(c->degree_oblique, uk_dc);
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->HNZ_; k++) {
+        const size_t im = IDX(c, i, j, 2 * k);
+        const double ks[3] = {c->KX_int[im] * c->WAVE_X, c->KY_int[im] * c->WAVE_Y, c->KZ_int[im] * c->WAVE_Z};
+        double re[3], iv[3];
+        re[0] = (ks[1] * u[2][im + 1] - ks[2] * u[1][im + 1]);
+        iv[0] = -(ks[1] * u[2][im] - ks[2] * u[1][im]);
+        re[1] = (ks[2] * u[0][im + 1] - ks[0] * u[2][im + 1]);
+        iv[1] = -(ks[2] * u[0][im] - ks[0] * u[2][im]);
+        re[2] = (ks[0] * u[1][im + 1] - ks[1] * u[0][im + 1]);
+        iv[2] = -(ks[0] * u[1][im] - ks[1] * u[0][im]);
+        int a, b;
+        if (c->KX_int[im] != 0) {
+          a = 1; b = 2;
+        } else if (c->KZ_int[im] != 0) {
+          a = 1; b = 0;
+        } else {
+          a = 0; b = 2;
+        }
+        zeta[0][im] = re[a];
+        zeta[0][im + 1] = iv[a];
+        zeta[1][im] = re[b];
+        zeta[1][im + 1] = iv[b];
+      }
+}
+
+/* Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390) with no particles: f = 0,
+ * so U2u_oblique and Add_f_particle leave u as it is. */
+static void le_step(oracle_ctx *c) {
+  oracle_ns_solver_shear_obl(c);
+  for (int d = 0; d < DIM; d++)
+    if (c->P.fix_cell[d]) c->uk_dc[d] = 0.0;
+  zeta_k2u_k_obl(c, c->zeta, c->uk_dc, c->u);
+  u_k2u(c, c->u);
+  c->shear_rate_eff = c->shear_rate;
+  const int NX = c->NX, NY = c->NY, NZ = c->NZ;
+  for (int d = 0; d < DIM; d++) memcpy(c->ucp[d], c->u[d], sizeof(double) * c->nfield);
+  c->degree_oblique += c->shear_rate_eff * c->dt;
+  if (c->degree_oblique >= 1.) {
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++) {
+        double sign = j - NY / 2;
+        if (!(sign == 0)) sign = sign / fabs(sign);
+        int i_oblique = (int)(sign * (j - NY / 2)) * sign;
+        i_oblique = (int)fmod(i + i_oblique + 4. * NX, NX);
+        for (int k = 0; k < NZ; k++) {
+          const size_t im = IDX(c, i, j, k), im_obl = IDX(c, i_oblique, j, k);
+          c->u[0][im_obl] = c->ucp[0][im] + c->ucp[1][im];
+          c->u[1][im_obl] = c->ucp[1][im];
+          c->u[2][im_obl] = c->ucp[2][im];
+        }
+      }
+    c->degree_oblique -= 1.;
+    for (int d = 0; d < DIM; d++) memcpy(c->ucp[d], c->u[d], sizeof(double) * c->nfield);
+  }
+  u_oblique2u(c, c->ucp);
+  obl_update_k2(c);
+  calc_shear_rate_eff(c);
+  u2u_k(c, c->u);
+  solenoidal_uk_obl(c, c->u);
+  contra2co_field(c, c->u);
+  u_k2zeta_k_obl(c, c->u, c->zeta, c->uk_dc);
+}
+
+int oracle_set_lees_edwards(oracle_ctx *c, double shear_rate) {
+  if (!c || c->le || c->electro || c->Np > 0) return -1;
+  for (int d = 0; d < DIM; d++) {
+    c->ucp[d] = (double *)calloc(c->nfield, sizeof(double));
+    c->wv3[d] = (double *)calloc(c->nfield, sizeof(double));
+    if (!c->ucp[d] || !c->wv3[d]) return -1;
+  }
+  c->wv1 = (double *)calloc(c->nfield, sizeof(double));
+  if (!c->wv1) return -1;
+  c->le = 1;
+  c->shear_rate = shear_rate;
+  c->shear_rate_eff = 0.0;
+  c->degree_oblique = 0.0;
+  return 0;
+}
+
+int oracle_put_oblique(oracle_ctx *c, double degree_oblique, double shear_rate_eff) {
+  if (!c || !c->le) return -1;
+  c->degree_oblique = degree_oblique;
+  c->shear_rate_eff = shear_rate_eff;
+  obl_update_k2(c);
+  return 0;
+}
+
+int oracle_get_oblique(oracle_ctx *c, double *degree_oblique, double *shear_rate_eff) {
+  if (!c || !c->le) return -1;
+  *degree_oblique = c->degree_oblique;
+  *shear_rate_eff = c->shear_rate_eff;
+  return 0;
+}
+
+/* the contravariant velocity spectrum of zeta (Zeta_k2u_k_OBL) into u */
+int oracle_zeta_k2u_k_obl(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  zeta_k2u_k_obl(c, c->zeta, c->uk_dc, c->u);
+  return 0;
+}
+
+/* Solenoidal_uk_OBL, contra2co and U_k2zeta_k_OBL on the spectrum in u */
+int oracle_u_k2zeta_k_obl(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  solenoidal_uk_obl(c, c->u);
+  contra2co_field(c, c->u);
+  u_k2zeta_k_obl(c, c->u, c->zeta, c->uk_dc);
   return 0;
 }
 
@@ -1547,6 +1924,11 @@ oracle_ctx *oracle_create(const spm_params *prm) {
 
 void oracle_destroy(oracle_ctx *c) {
   if (!c) return;
+  for (int d = 0; d < 3; d++) {
+    free(c->ucp[d]);
+    free(c->wv3[d]);
+  }
+  free(c->wv1);
   free(c->KX_int);
   free(c->KY_int);
   free(c->KZ_int);

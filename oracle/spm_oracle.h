@@ -86,6 +86,14 @@ int oracle_make_coulomb_force(oracle_ctx *c);
 void oracle_make_phi_qq(oracle_ctx *c, double *phi, double *surface);
 void oracle_make_surface_normal(oracle_ctx *c, double *const normal[3]);
 
+/* Lees-Edwards shear, fluid only (Time_evolution_hydro_OBL) */
+int oracle_set_lees_edwards(oracle_ctx *c, double shear_rate);
+int oracle_put_oblique(oracle_ctx *c, double degree_oblique, double shear_rate_eff);
+int oracle_get_oblique(oracle_ctx *c, double *degree_oblique, double *shear_rate_eff);
+int oracle_ns_solver_shear_obl(oracle_ctx *c);
+int oracle_zeta_k2u_k_obl(oracle_ctx *c);
+int oracle_u_k2zeta_k_obl(oracle_ctx *c);
+
 /* Building blocks, on caller buffers of NX*NY*NZ_ doubles. */
 void oracle_A2a_k(oracle_ctx *c, double *a);
 void oracle_A_k2a(oracle_ctx *c, double *a);

@@ -1,0 +1,183 @@
+"""Parity of the Lees-Edwards path (SW_EQ == Shear_Navier_Stokes_Lees_Edwards,
+fluid only) of libspm.so with the CPU oracle: Zeta_k2u_k_OBL, the
+projection back to zeta, NS_solver_slavedEuler_Shear_OBL at a tilted grid
+with a nonzero Shear_rate_eff, and whole Time_evolution_hydro_OBL steps
+across a remap (degree_oblique reaching 1), on boxes with NX != NY.
+Tolerance 1e-10 relative, as for the rest of the FP64 path; degree_oblique
+is host arithmetic and must agree exactly.
+
+At 128^3 and 256^3, where the oracle is slow, size-independent properties:
+an unsheared oblique step moves the velocity as the Navier-Stokes step
+does, and a fluid at rest stays at rest under shear with the measured
+shear rate equal to the imposed one.
+"""
+import numpy as np
+import pytest
+
+from kapsel_old_amd import abi
+from oracle.oracle import Oracle
+from tests.helpers import rel, smooth_field
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def SPM():
+    from kapsel_old_amd.spm import SPM as _SPM
+    return _SPM
+
+
+def check(name, a, b, tol=TOL):
+    e = rel(a, b)
+    print("%s rel err %.3e" % (name, e), flush=True)
+    assert e < tol, (name, e)
+
+
+def spectra(o, grid, seed, scale=0.3, mean=(0.05, -0.02, 0.01)):
+    """Contravariant velocity spectra of a smooth field with a mean flow."""
+    out = []
+    for d in range(3):
+        u = smooth_field(o.shape, grid[2], seed=seed + d, kmax=3, scale=scale)
+        u[:, :, :grid[2]] += mean[d]
+        out.append(o.A2a_k(u))
+    return out
+
+
+def pair(SPM, grid, rate, dt=0.05, eta=0.5, fix_cell=(0, 0, 0), seed=5, deg=0.0, sre=0.0):
+    """An oracle and a GPU context holding the same sheared fluid."""
+    prm = abi.make_params(*grid, dt=dt, eta=eta, fix_cell=fix_cell)
+    objs = [Oracle(prm), SPM(prm)]
+    u = spectra(objs[0], grid, seed)
+    for o in objs:
+        o.set_lees_edwards(rate)
+        o.put_oblique(deg, sre)
+        for d in range(3):
+            o.put_field("u%d" % d, u[d])
+        o.U_k2zeta_k_OBL()
+    return objs
+
+
+def compare_state(objs, tag):
+    o, g = objs
+    for f in ("zeta0", "zeta1"):
+        check("%s %s" % (tag, f), g.get_field(f), o.get_field(f))
+    # with FIX_CELL the mean flow is rounding noise (~1e-15 of a field of order 0.3)
+    np.testing.assert_allclose(g.get_uk_dc(), o.get_uk_dc(), rtol=TOL, atol=1e-13)
+
+
+@pytest.mark.parametrize("deg", [0.0, 0.37, 0.93])
+def test_zeta_k2u_k_obl_and_back(SPM, deg):
+    grid = (32, 16, 8)
+    objs = pair(SPM, grid, 0.0, deg=deg)
+    compare_state(objs, "U_k2zeta_k_OBL deg %.2f" % deg)
+    for o in objs:
+        o.Zeta_k2u_k_OBL()
+    for d in range(3):
+        check("Zeta_k2u_k_OBL u%d" % d, objs[1].get_field("u%d" % d), objs[0].get_field("u%d" % d))
+
+
+@pytest.mark.parametrize("grid", [(32, 16, 8), (16, 32, 16)])
+def test_ns_solver_shear_obl(SPM, grid):
+    # a tilted grid and a measured shear rate of the previous step
+    objs = pair(SPM, grid, 0.4, deg=0.61, sre=0.37)
+    for o in objs:
+        o.NS_solver_slavedEuler_Shear_OBL()
+    compare_state(objs, "NS_solver_slavedEuler_Shear_OBL %s" % (grid,))
+    assert objs[1].get_oblique() == (0.61, 0.37)
+
+
+@pytest.mark.parametrize("fix_cell", [(0, 0, 0), (1, 1, 1)])
+def test_steps_across_a_remap(SPM, fix_cell):
+    grid = (32, 16, 16)
+    # Shear_rate * dt = 0.02 per step from 0.95: the grid is remapped at the
+    # third step
+    objs = pair(SPM, grid, 0.4, fix_cell=fix_cell, deg=0.95, sre=0.4)
+    degs = [[], []]
+    for s in range(6):
+        for i, o in enumerate(objs):
+            o.step(1)
+            degs[i].append(o.get_oblique()[0])
+        compare_state(objs, "step %d" % s)
+        so, sg = objs[0].get_oblique()[1], objs[1].get_oblique()[1]
+        assert abs(sg - so) <= TOL * max(1.0, abs(so)), (s, sg, so)
+    assert degs[0] == degs[1]
+    assert any(b < a for a, b in zip(degs[1], degs[1][1:]))  # it wrapped
+    assert objs[1].get_time()[0] == 6
+
+
+def test_steps_64cube(SPM):
+    grid = (64, 64, 64)
+    objs = pair(SPM, grid, 0.25, dt=0.1, eta=1.0, deg=0.98, sre=0.25)
+    for o in objs:
+        o.step(3)
+    compare_state(objs, "64^3 3 steps")
+    assert objs[0].get_oblique()[0] == objs[1].get_oblique()[0]
+
+
+def test_unsheared_oblique_step_is_the_navier_stokes_step_128(SPM):
+    """128^3, GPU against GPU: with degree_oblique and Shear_rate_eff held at
+    zero the oblique step is the Navier-Stokes step (literal schedule)."""
+    from kapsel_old_amd.spm import SCHEDULE_LITERAL
+    grid = (128, 128, 128)
+    prm = abi.make_params(*grid, dt=0.05, eta=0.5, fix_cell=(1, 1, 1))
+    o = Oracle(prm)  # transforms only
+    ns, le = SPM(prm), SPM(prm)
+    ns.set_schedule(SCHEDULE_LITERAL)
+    le.set_lees_edwards(0.0)
+    u = [smooth_field(o.shape, grid[2], seed=40 + d, kmax=3, scale=0.3) for d in range(3)]
+    for d in range(3):
+        ns.put_field("u%d" % d, u[d])
+    ns.U2zeta_k()
+    ns.Zeta_k2u()
+    uk = [o.A2a_k(ns.get_field("u%d" % d)) for d in range(3)]
+    for d in range(3):
+        le.put_field("u%d" % d, uk[d])
+    le.U_k2zeta_k_OBL()
+    for _ in range(2):
+        le.put_oblique(0.0, 0.0)
+        ns.step(1)
+        le.step(1)
+    ns.Zeta_k2u()
+    le.Zeta_k2u_k_OBL()
+    for d in range(3):
+        want = o.A2a_k(ns.get_field("u%d" % d))
+        got = le.get_field("u%d" % d)
+        got[0, 0, :2] = want[0, 0, :2] = 0.0  # the mean flow: uk_dc, zero with fix_cell
+        check("128^3 oblique vs NS u%d" % d, got, want, 1e-12)
+
+
+def test_fluid_at_rest_under_shear_256(SPM):
+    grid = (256, 256, 256)
+    prm = abi.make_params(*grid, dt=0.1, fix_cell=(0, 0, 0))
+    g = SPM(prm)
+    g.set_lees_edwards(2.5)
+    d = 0.0
+    for s in range(5):  # degree_oblique grows by 0.25 a step and wraps at 1
+        g.step(1)
+        d += 2.5 * 0.1
+        if d >= 1.0:
+            d -= 1.0
+        deg, sre = g.get_oblique()
+        assert deg == d, (s, deg, d)
+        assert abs(sre - 2.5) < 1e-12 * 2.5, (s, sre)
+    for f in ("zeta0", "zeta1"):
+        assert not g.get_field(f).any(), f
+    assert not g.get_uk_dc().any()
+
+
+def test_lees_edwards_argument_errors(SPM):
+    from kapsel_old_amd.spm import SpmError
+    g = SPM(abi.make_params(32, a=2.0, n_particles=1))
+    with pytest.raises(SpmError):
+        g.set_lees_edwards(0.1)  # particles are not supported
+    with pytest.raises(SpmError):
+        g.put_oblique(0.1, 0.1)  # before spm_set_lees_edwards
+    g = SPM(abi.make_params(16, precision=abi.FP32))
+    with pytest.raises(SpmError):
+        g.set_lees_edwards(0.1)  # FP64 only
+    g = SPM(abi.make_params(16))
+    g.set_lees_edwards(0.1)
+    with pytest.raises(SpmError):
+        g.set_lees_edwards(0.1)  # once
