@@ -229,12 +229,13 @@ int spm_check_finite(spm_ctx *ctx);
  * for exact resumption: the file holds zeta0/zeta1 in the context's own
  * precision, uk_dc, every particle record, ts and time, the MT19937 state
  * (which the reference's restart file does not keep) and, for an
- * electrolyte, the ion spectra and their totals, with a hash over all of
- * it.  A context created with the same spm_params (and spm_set_electrolyte)
- * that loads the file continues bit for bit like the one that saved it.
- * spm_load_state rejects a file from another grid, precision, particle
- * count, ion count or time step, or a damaged one, and then leaves the
- * context unchanged.  spm_save_state writes PATH.tmp and renames it. */
+ * electrolyte, the ion spectra and their totals, and under Lees-Edwards
+ * shear degree_oblique and Shear_rate_eff, with a hash over all of it.  A
+ * context created with the same spm_params (and spm_set_electrolyte or
+ * spm_set_lees_edwards) that loads the file continues bit for bit like the
+ * one that saved it.  spm_load_state rejects a file from another grid,
+ * precision, particle count, ion count, time step or shear rate, or a
+ * damaged one, and then leaves the context unchanged.  spm_save_state writes PATH.tmp and renames it. */
 int spm_save_state(spm_ctx *ctx, const char *path);
 int spm_load_state(spm_ctx *ctx, const char *path);
 

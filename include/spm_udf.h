@@ -64,7 +64,7 @@ int spm_udf_create_output(const char *in_path, const char *out_path);
 int spm_udf_append_record(const char *out_path, int ts, const spm_particle *p, int n);
 
 /* Run control read with the parameters (input.cxx:1384-1656). */
-enum spm_udf_equation { SPM_UDF_NAVIER_STOKES = 0, SPM_UDF_ELECTROLYTE = 3 };
+enum spm_udf_equation { SPM_UDF_NAVIER_STOKES = 0, SPM_UDF_LEES_EDWARDS = 2, SPM_UDF_ELECTROLYTE = 3 };
 enum spm_udf_distribution {
   SPM_UDF_NONE = 0, SPM_UDF_UNIFORM_RANDOM = 1, SPM_UDF_RANDOM_WALK = 2, SPM_UDF_FCC = 3, SPM_UDF_BCC = 4,
   SPM_UDF_USER_SPECIFY = 5
@@ -76,14 +76,18 @@ typedef struct spm_udf_run {
   int gts, num_snap;     /* output.GTS, output.Num_snap; MSTEP = GTS*Num_snap */
   int distribution;      /* switch.INIT_distribution.type                  */
   int poisson_boltzmann; /* Electrolyte INIT_profile == "Poisson_Boltzmann" */
+  double shear_rate;     /* Shear_Navier_Stokes_Lees_Edwards External_field Shear_rate */
 } spm_udf_run;
 
 /* input.cxx:438-1813 (subset) and the derived time step of
  * Set_global_parameters (input.cxx:316-381).  `el` may be NULL for
- * Navier_Stokes input; for Electrolyte input it is filled.  Rejects
- * (SPM_ERR_UNSUPPORTED) the equations, object types and switches the
- * library does not run: shear flows, chains, rigid bodies, janus particles,
- * pinning. */
+ * Navier_Stokes input; for Electrolyte input it is filled.
+ * Shear_Navier_Stokes_Lees_Edwards input gives run->shear_rate (its AC
+ * branch reads Shear_rate as a steady shear, input.cxx:646-653); the
+ * library runs it without particles.  Rejects (SPM_ERR_UNSUPPORTED) the
+ * equations, object types and switches the library does not run:
+ * Shear_Navier_Stokes (sheared walls), chains, rigid bodies, janus
+ * particles, pinning. */
 int spm_udf_params(const spm_udf *udf, spm_params *prm, spm_electrolyte *el, spm_udf_run *run);
 
 /* The particles Init_Particle sets up for INIT_distribution user_specify

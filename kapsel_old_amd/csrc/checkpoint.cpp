@@ -7,8 +7,11 @@
 // sp_3d_ns.cxx:477).  This file holds everything the next step reads -- the
 // two vorticity spectra in the context's own precision, uk_dc, every
 // particle record, ts and time, the MT19937 words and index, and for an
-// electrolyte the ion spectra and their conserved totals -- so that a
-// context loaded from it continues bit for bit like the one that saved it.
+// electrolyte the ion spectra and their conserved totals, and under
+// Lees-Edwards shear degree_oblique and Shear_rate_eff (the reference's
+// restart keeps only degree_oblique, resume.cxx:335-339, and takes
+// Shear_rate_eff from the first step) -- so that a context loaded from it
+// continues bit for bit like the one that saved it.
 //
 // Layout (native byte order): the header below, then zeta0, zeta1
 // (nfield elements each, 4 or 8 bytes), the particle records
@@ -24,7 +27,7 @@
 namespace {
 
 constexpr char kMagic[8] = {'S', 'P', 'M', 'S', 'T', 'A', 'T', 'E'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;
 
 struct Header {
   char magic[8];
@@ -35,6 +38,8 @@ struct Header {
   double time, dt;
   double uk_dc[3];
   double total_solute[2];
+  int32_t le, le_pad;  // Lees-Edwards shear: 1 and the three values below
+  double le_rate, le_deg, le_sre;
   uint32_t mt[SPM_MT_N];
 };
 
@@ -88,6 +93,13 @@ extern "C" int spm_save_state(spm_ctx *c, const char *path) {
   for (int n = 0; n < 2; ++n) h.total_solute[n] = c->total_solute[n];
   std::memcpy(h.mt, c->mt, sizeof(h.mt));
   SPM_HIP(c, hipMemcpyAsync(h.uk_dc, c->uk_dc, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+  if (c->le) {
+    h.le = 1;
+    h.le_rate = c->le_rate;
+    h.le_deg = c->le_deg;
+    SPM_HIP(c, hipMemcpyAsync(&h.le_sre, c->le_red + spm::le_red_slots() - 2, sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+  }
   const size_t fb = elem_bytes(c) * c->nfield;
   std::vector<std::vector<char>> data;
   for (void *b : buffers(c)) {
@@ -137,9 +149,10 @@ extern "C" int spm_load_state(spm_ctx *c, const char *path) {
     return fail(c, SPM_ERR_INVALID, "state file version differs");
   const int n_ion = c->electro ? c->el.n_spec : 0;
   if (h.nx != c->prm.nx || h.ny != c->prm.ny || h.nz != c->prm.nz || h.precision != c->prm.precision ||
-      h.n_particles != c->np || h.n_ion != n_ion || h.dt != c->prm.dt)
+      h.n_particles != c->np || h.n_ion != n_ion || h.dt != c->prm.dt || h.le != (c->le ? 1 : 0) ||
+      (c->le && h.le_rate != c->le_rate))
     return fail(c, SPM_ERR_INVALID, "state file was saved by a context of another grid, precision, particle "
-                                    "count, ion count or time step");
+                                    "count, ion count, time step or shear");
   if (h.ts < 0 || h.mti < 0 || h.mti > SPM_MT_N + 1) return fail(c, SPM_ERR_INVALID, "corrupt state header");
   const size_t fb = elem_bytes(c) * c->nfield;
   std::vector<std::vector<char>> data(2 + n_ion, std::vector<char>(fb));
@@ -156,7 +169,11 @@ extern "C" int spm_load_state(spm_ctx *c, const char *path) {
   for (size_t k = 0; k < dst.size(); ++k)
     SPM_HIP(c, hipMemcpyAsync(dst[k], data[k].data(), fb, hipMemcpyHostToDevice, c->stream));
   SPM_HIP(c, hipMemcpyAsync(c->uk_dc, h.uk_dc, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream));
+  if (c->le)
+    SPM_HIP(c, hipMemcpyAsync(c->le_red + spm::le_red_slots() - 2, &h.le_sre, sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
   SPM_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->le) c->le_deg = h.le_deg;
   c->ts = h.ts;
   c->time = h.time;
   std::memcpy(c->mt, h.mt, sizeof(c->mt));

@@ -718,6 +718,9 @@ extern "C" int spm_udf_params(const spm_udf *u, spm_params *prm, spm_electrolyte
   if (eq == "Navier_Stokes") {
     rn.equation = SPM_UDF_NAVIER_STOKES;
     base = "constitutive_eq.Navier_Stokes.";
+  } else if (eq == "Shear_Navier_Stokes_Lees_Edwards") {
+    rn.equation = SPM_UDF_LEES_EDWARDS;
+    base = "constitutive_eq.Shear_Navier_Stokes_Lees_Edwards.";
   } else if (eq == "Electrolyte") {
     rn.equation = SPM_UDF_ELECTROLYTE;
     base = "constitutive_eq.Electrolyte.";
@@ -729,13 +732,20 @@ extern "C" int spm_udf_params(const spm_udf *u, spm_params *prm, spm_electrolyte
   p.rho = r.d(base + "RHO");
   p.eta = r.d(base + "ETA");
   p.kbt = r.d(base + "kBT");
-  if (rn.equation == SPM_UDF_NAVIER_STOKES) {
+  if (rn.equation != SPM_UDF_ELECTROLYTE) {
     // the Electrolyte branch does not read alpha_v / alpha_o, which keep
     // their zero initial values (input.cxx:267, 660-667)
     p.alpha_v = r.d(base + "alpha_v");
     p.alpha_o = r.d(base + "alpha_o");
   }
   double frequency = 0.0;
+  if (rn.equation == SPM_UDF_LEES_EDWARDS) {
+    // input.cxx:634-653: DC and AC both read Shear_rate (the AC branch as a
+    // steady shear); the time step is the Navier-Stokes one (:323-340)
+    const std::string t = r.s(base + "External_field.type");
+    if (t != "DC" && t != "AC" && r.rc == SPM_OK) return invalid("External_field type " + t);
+    rn.shear_rate = r.d(base + "External_field." + t + ".Shear_rate");
+  }
   if (rn.equation == SPM_UDF_ELECTROLYTE) {
     std::memset(el, 0, sizeof(*el));
     el->elementary_charge = 1.0;  // Elementary_charge (input.cxx:247)

@@ -1,7 +1,8 @@
 """Command-line driver: a run from a UDF input, as the reference's main()
 (sp_3d_ns.cxx:392-640) for constitutive_eq Navier_Stokes / Electrolyte
 with spherical particles placed by INIT_distribution user_specify,
-uniform_random or BCC.
+uniform_random or BCC, and Shear_Navier_Stokes_Lees_Edwards without
+particles (Time_evolution_hydro_OBL).
 
     python -m kapsel_old_amd.run -I input.udf -O output.udf -R restart.udf [-D define.udf]
 
@@ -76,9 +77,13 @@ def initial_particles(u, prm, run, resumed):
     return p
 
 
-def restore(ctx, u, prm, n_ion, log):
-    """Force_restore_parameters into the context."""
+def restore(ctx, u, prm, n_ion, sheared, log):
+    """Force_restore_parameters into the context.  Under Lees-Edwards shear
+    only degree_oblique is restored (resume.cxx:335-339); Shear_rate_eff
+    stays as the first step measured it."""
     r = udf.read_restart(u, (prm.nx, prm.ny, prm.nz), prm.n_particles, n_ion, particles=ctx.get_particles())
+    if sheared:
+        ctx.put_oblique(r["degree_oblique"], ctx.get_oblique()[1])
     ctx.put_field("zeta0", r["zeta0"])
     ctx.put_field("zeta1", r["zeta1"])
     ctx.put_uk_dc(r["uk_dc"])
@@ -97,6 +102,9 @@ def main(argv=None):
     prm, el, run = u.params()
     if el is not None and run.poisson_boltzmann:
         raise SystemExit("INIT_profile Poisson_Boltzmann is not run by this driver; use Uniform")
+    sheared = run.equation == udf.LEES_EDWARDS
+    if sheared and prm.n_particles > 0:
+        raise SystemExit("Shear_Navier_Stokes_Lees_Edwards is run without particles (Particle_number 0)")
     if a.fp32:
         prm.precision = abi.FP32
     gts, mstep = run.gts, run.gts * run.num_snap
@@ -113,6 +121,8 @@ def main(argv=None):
     if el is not None:
         ctx.set_electrolyte(el)
         ctx.Init_rho_ion()
+    if sheared:
+        ctx.set_lees_edwards(run.shear_rate)
     if write_udf:
         udf.create_output(a.input, a.output)
 
@@ -122,7 +132,7 @@ def main(argv=None):
         ts = ctx.get_time()[0]
     elif resumed:
         ctx.step(1)  # the first loop iteration runs before the restore
-        ts = restore(ctx, u, prm, n_ion, log) + 1
+        ts = restore(ctx, u, prm, n_ion, sheared, log) + 1
     while ts <= mstep:
         if ts % gts == 0:
             if write_udf:
@@ -135,7 +145,8 @@ def main(argv=None):
     shape = (prm.nx, prm.ny, prm.nz)
     conc = [ctx.get_concentration(n) for n in range(n_ion)] if n_ion else None
     udf.write_restart(a.input, a.restart, shape, ctx.get_field("zeta0"), ctx.get_field("zeta1"), ctx.get_uk_dc(),
-                      ctx.get_particles(), ts_done - 1, time, conc=conc)
+                      ctx.get_particles(), ts_done - 1, time, conc=conc,
+                      degree_oblique=ctx.get_oblique()[0] if sheared else 0.0)
     if a.state:
         ctx.save_state(a.state)
     ctx.close()

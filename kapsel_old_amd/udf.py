@@ -21,7 +21,7 @@ class SpmUdfRun(C.Structure):
     """spm_udf_run: run control read with the parameters."""
     _fields_ = [("equation", C.c_int), ("auto_dt", C.c_int), ("dt_factor", C.c_double),
                 ("gts", C.c_int), ("num_snap", C.c_int), ("distribution", C.c_int),
-                ("poisson_boltzmann", C.c_int)]
+                ("poisson_boltzmann", C.c_int), ("shear_rate", C.c_double)]
 
 
 class SpmRestart(C.Structure):
@@ -32,7 +32,7 @@ class SpmRestart(C.Structure):
                 ("concentration", C.POINTER(C.c_double)), ("particles", C.c_void_p)]
 
 
-NAVIER_STOKES, ELECTROLYTE = 0, 3
+NAVIER_STOKES, LEES_EDWARDS, ELECTROLYTE = 0, 2, 3
 DISTRIBUTIONS = ["NONE", "uniform_random", "random_walk", "FCC", "BCC", "user_specify"]
 
 

@@ -11,6 +11,8 @@ an unsheared oblique step moves the velocity as the Navier-Stokes step
 does, and a fluid at rest stays at rest under shear with the measured
 shear rate equal to the imposed one.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -181,3 +183,86 @@ def test_lees_edwards_argument_errors(SPM):
     g.set_lees_edwards(0.1)
     with pytest.raises(SpmError):
         g.set_lees_edwards(0.1)  # once
+
+
+def test_state_file_round_trip(SPM, tmp_path):
+    from kapsel_old_amd.spm import SpmError
+    grid = (32, 16, 16)
+    prm = abi.make_params(*grid, dt=0.05, eta=0.5, fix_cell=(0, 0, 0))
+    u = spectra(Oracle(prm), grid, 9)
+    a = SPM(prm)
+    a.set_lees_edwards(0.4)
+    a.put_oblique(0.97, 0.4)
+    for d in range(3):
+        a.put_field("u%d" % d, u[d])
+    a.U_k2zeta_k_OBL()
+    a.step(2)
+    path = str(tmp_path / "le.state")
+    a.save_state(path)
+    a.step(3)  # crosses a remap
+    b = SPM(prm)
+    b.set_lees_edwards(0.4)
+    b.load_state(path)
+    b.step(3)
+    for f in ("zeta0", "zeta1"):
+        np.testing.assert_array_equal(b.get_field(f), a.get_field(f), err_msg=f)
+    np.testing.assert_array_equal(b.get_uk_dc(), a.get_uk_dc())
+    assert b.get_oblique() == a.get_oblique() and b.get_time() == a.get_time()
+    for other in (lambda g: g.set_lees_edwards(0.5), lambda g: None):  # another shear rate, no shear
+        g = SPM(prm)
+        other(g)
+        with pytest.raises(SpmError):
+            g.load_state(path)
+
+
+def _driver_input(tmp_path, num_snap, name):
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "udf")
+    src = open(os.path.join(golden, "input.udf")).read()
+    with open(os.path.join(golden, "define.udf")) as f:
+        (tmp_path / "define.udf").write_text(f.read())
+    edits = [('"Navier_Stokes",\n', '"Shear_Navier_Stokes_Lees_Edwards",\n'),
+             ('[{5,1.50000000000000,0.0,"NONE"', '[{0,1.50000000000000,0.0,"NONE"'),
+             ('{5,0.50000000000000,0.0,"NONE"', '{0,0.50000000000000,0.0,"NONE"'),
+             ("output:{10,100,", "output:{2,%d," % num_snap)]
+    for old, new in edits:
+        assert src.count(old) == 1, old
+        src = src.replace(old, new)
+    head, tail = src.split("{//Shear_Navier_Stokes_Lees_Edwards")
+    tail = tail.replace("1.00000000000000e-003", "2.00000000000000e+001", 1)  # DC Shear_rate 20
+    src = head + "{//Shear_Navier_Stokes_Lees_Edwards" + tail
+    out = tmp_path / name
+    out.write_text(src)
+    return str(out)
+
+
+def test_driver_lees_edwards(tmp_path):
+    """python -m kapsel_old_amd.run on Lees-Edwards input: degree_oblique in
+    the restart file, and a run resumed from it ends where the whole run
+    does."""
+    from kapsel_old_amd import udf
+    from kapsel_old_amd.run import main
+    d = str(tmp_path)
+    full, part = _driver_input(tmp_path, 3, "le6.udf"), _driver_input(tmp_path, 2, "le4.udf")
+    print("Lees-Edwards driver run MSTEP 6", flush=True)
+    assert main(["-I" + full, "-O" + d + "/o.udf", "-R" + d + "/r.udf", "--quiet"]) == 0
+    with udf.UDF(full) as u:
+        prm, _, run = u.params()
+    assert run.shear_rate == 20.0
+    want = 0.0
+    for _ in range(7):  # ts = 0..6
+        want += run.shear_rate * prm.dt
+        if want >= 1.0:
+            want -= 1.0
+    with udf.UDF(d + "/r.udf") as u:
+        r = udf.read_restart(u, (32, 32, 32), 0)
+    assert r["ts"] == 6 and r["degree_oblique"] == want
+    assert not r["zeta0"].any() and not r["zeta1"].any()  # a fluid at rest stays at rest
+    print("first part MSTEP 4, then resume to MSTEP 6", flush=True)
+    main(["-I" + part, "-O" + d + "/oa.udf", "-R" + d + "/ra.udf", "--quiet"])
+    text = open(d + "/ra.udf").read()
+    assert text.count("output:{2,2,") == 1
+    open(d + "/ra3.udf", "w").write(text.replace("output:{2,2,", "output:{2,3,"))
+    main(["-I" + d + "/ra3.udf", "-O" + d + "/ob.udf", "-R" + d + "/rb.udf", "--quiet"])
+    with udf.UDF(d + "/rb.udf") as u:
+        rb = udf.read_restart(u, (32, 32, 32), 0)
+    assert rb["ts"] == 6 and rb["degree_oblique"] == want and rb["time"] == r["time"]

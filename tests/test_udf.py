@@ -140,6 +140,36 @@ def test_params_electrolyte(udf):
     assert run.poisson_boltzmann == 1 and prm.dt == pytest.approx(auto_dt(64, 1.0, diffusion=1.0), rel=1e-14)
 
 
+def _lees_edwards_input(tmp_path, ac=False):
+    """input.udf with constitutive_eq.type Shear_Navier_Stokes_Lees_Edwards
+    (its data block holds DC Shear_rate 1e-3, AC Shear_rate 1e-2)."""
+    src = open(path("input.udf")).read()
+    assert src.count('"Navier_Stokes",\n') == 1
+    src = src.replace('"Navier_Stokes",\n', '"Shear_Navier_Stokes_Lees_Edwards",\n')
+    if ac:
+        head, tail = src.split("{//Shear_Navier_Stokes_Lees_Edwards")
+        tail = tail.replace('"DC",', '"AC",', 1)
+        src = head + "{//Shear_Navier_Stokes_Lees_Edwards" + tail
+    with open(path("define.udf")) as f:
+        (tmp_path / "define.udf").write_text(f.read())
+    out = tmp_path / ("le_ac.udf" if ac else "le.udf")
+    out.write_text(src)
+    return str(out)
+
+
+def test_params_lees_edwards(udf, tmp_path):
+    with udf.UDF(_lees_edwards_input(tmp_path)) as u:
+        prm, el, run = u.params()
+    assert el is None and run.equation == udf.LEES_EDWARDS and run.shear_rate == 1e-3
+    # the Lees-Edwards block's own values (kBT 0.1 there), alpha read as for Navier_Stokes
+    assert (prm.dx, prm.rho, prm.eta, prm.kbt, prm.alpha_v, prm.alpha_o) == (1.0, 1.0, 1.0, 0.1, 1.0, 1.0)
+    # the time step ignores the shear (input.cxx:323-340)
+    assert prm.dt == pytest.approx(auto_dt(32, 1.0), rel=1e-14)
+    with udf.UDF(_lees_edwards_input(tmp_path, ac=True)) as u:
+        prm, el, run = u.params()
+    assert run.equation == udf.LEES_EDWARDS and run.shear_rate == 1e-2  # AC read as a steady shear
+
+
 @pytest.mark.parametrize("name,what", [("tmp_shear.udf", "Shear_Navier_Stokes"),
                                        ("input_rigid_free1.udf", "rigid"),
                                        ("squirm_single_a+2.udf", "SQUIRMER")])
