@@ -57,14 +57,11 @@ __device__ __forceinline__ void co2contra1(double deg, double v[3]) {
   v[1] = -deg * d0 + d1;
 }
 
-// Zeta_k2u_k_OBL (fft_wrapper.cxx:513-584): zeta -> contravariant velocity
-// spectrum; mode 0 takes u = uk_dc (real part), whose FIX_CELL components
-// are zeroed first when fix_mask has their bits.  32 B read, 48 B written
-// per mode.
-__global__ void zeta2u_obl_kernel(Grid g, const double *__restrict__ z0, const double *__restrict__ z1, double *u0,
-                                  double *u1, double *u2, double *uk_dc, int fix_mask, double deg) {
-  ModeO m;
-  if (!mode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m)) return;
+// Zeta_k2u_k_OBL (fft_wrapper.cxx:513-584) at one mode: zeta -> the
+// contravariant velocity (real parts ur, imaginary parts ui); mode 0 is
+// left to the caller (it takes uk_dc).
+__device__ __forceinline__ void zeta2u_mode(const Grid &g, const ModeO &m, double deg, const double *__restrict__ z0,
+                                            const double *__restrict__ z1, double ur[3], double ui[3]) {
   double kx = g.wave_x * m.kx, ky = g.wave_y * m.ky, kz = g.wave_z * m.kz;
   const double k2 = k2_obl(g, m, deg);
   const double ik2 = k2 > 0.0 ? 1. / k2 : 0.0;
@@ -90,8 +87,23 @@ __global__ void zeta2u_obl_kernel(Grid g, const double *__restrict__ z0, const d
   kx *= ik2;
   ky *= ik2;
   kz *= ik2;
-  double ur[3] = {(ky * wi[2] - kz * wi[1]), (kz * wi[0] - kx * wi[2]), (kx * wi[1] - ky * wi[0])};
-  const double ui[3] = {-(ky * wr[2] - kz * wr[1]), -(kz * wr[0] - kx * wr[2]), -(kx * wr[1] - ky * wr[0])};
+  ur[0] = (ky * wi[2] - kz * wi[1]);
+  ur[1] = (kz * wi[0] - kx * wi[2]);
+  ur[2] = (kx * wi[1] - ky * wi[0]);
+  ui[0] = -(ky * wr[2] - kz * wr[1]);
+  ui[1] = -(kz * wr[0] - kx * wr[2]);
+  ui[2] = -(kx * wr[1] - ky * wr[0]);
+}
+
+// Zeta_k2u_k_OBL over the spectrum; mode 0 takes u = uk_dc (real part),
+// whose FIX_CELL components are zeroed first when fix_mask has their bits.
+// 32 B read, 48 B written per mode.
+__global__ void zeta2u_obl_kernel(Grid g, const double *__restrict__ z0, const double *__restrict__ z1, double *u0,
+                                  double *u1, double *u2, double *uk_dc, int fix_mask, double deg) {
+  ModeO m;
+  if (!mode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m)) return;
+  double ur[3], ui[3];
+  zeta2u_mode(g, m, deg, z0, z1, ur, ui);
   if (m.im == 0)
     for (int d = 0; d < 3; ++d) {
       if (fix_mask & (1 << d)) uk_dc[d] = 0.0;
@@ -154,20 +166,27 @@ __global__ void cross_kernel(Grid g, double *u0, double *u1, double *u2, const d
 // The tail of Zeta_k2advection_k_OBL (operate_omega.cxx:158-172): the shear
 // terms u0 -= 2 rate uk_cp, u1 -= 2 rate deg uk_cp, U_k2rotation_k
 // (fft_wrapper.cxx:679-718) and Omega_k2zeta_k_OBL (fft_wrapper.cxx:175-198)
-// into adv.  `rate` is Shear_rate_eff on the device.  56 B read, 32 B
-// written per mode.
+// into adv.  `rate` is Shear_rate_eff on the device.  uk_cp, the y velocity
+// spectrum of the step's zeta (operate_omega.cxx:124-128), is formed again
+// here from zeta, which the solver has not yet changed, rather than kept in
+// a field of its own.  80 B read, 32 B written per mode.
 __global__ void advect_tail_obl_kernel(Grid g, const double *__restrict__ u0, const double *__restrict__ u1,
-                                       const double *__restrict__ u2, const double *__restrict__ cp,
+                                       const double *__restrict__ u2, const double *__restrict__ z0,
+                                       const double *__restrict__ z1, const double *__restrict__ uk_dc,
                                        const double *__restrict__ rate_p, double deg, double *a0, double *a1) {
   ModeO m;
   if (!mode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m)) return;
   const double rate = *rate_p;
+  double cr[3], ci[3];
+  zeta2u_mode(g, m, deg, z0, z1, cr, ci);
+  if (m.im == 0) cr[1] = uk_dc[1];
+  const double cp[2] = {cr[1], ci[1]};
   const double ks[3] = {m.kx * g.wave_x, m.ky * g.wave_y, m.kz * g.wave_z};
   double re[3], iv[3];
-  re[0] = u0[m.im] - 2. * rate * cp[m.im];
-  iv[0] = u0[m.im + 1] - 2. * rate * cp[m.im + 1];
-  re[1] = u1[m.im] - 2. * rate * deg * cp[m.im];
-  iv[1] = u1[m.im + 1] - 2. * rate * deg * cp[m.im + 1];
+  re[0] = u0[m.im] - 2. * rate * cp[0];
+  iv[0] = u0[m.im + 1] - 2. * rate * cp[1];
+  re[1] = u1[m.im] - 2. * rate * deg * cp[0];
+  iv[1] = u1[m.im + 1] - 2. * rate * deg * cp[1];
   re[2] = u2[m.im];
   iv[2] = u2[m.im + 1];
   double rr[3], ri[3];
@@ -221,16 +240,12 @@ __global__ void remap_kernel(Grid g, const double *__restrict__ c0, const double
   u2[dst] = c2[src];
 }
 
-// U_oblique2u (fft_wrapper.h:265-325) in gather form: destination row
-// (id, j) takes source rows i = id - io(j) and i + sign(j - NY/2), with
-// the interpolation weights and the mean shear flow of the reference.
-// 48 B read, 24 B written per point.
-__global__ void oblique2u_kernel(Grid g, const double *__restrict__ w0, const double *__restrict__ w1,
-                                 const double *__restrict__ w2, double *u0, double *u1, double *u2, double deg,
-                                 double rate) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.nx * g.ny * g.nz) return;
-  const int k = (int)(t % g.nz), j = (int)((t / g.nz) % g.ny), id = (int)(t / ((long)g.nz * g.ny));
+// The x velocity that U_oblique2u (fft_wrapper.h:265-325) forms at grid
+// point (id, j, k) from the oblique field w0, w1: destination row (id, j)
+// interpolates source rows i = id - io(j) and i + sign(j - NY/2) with the
+// reference's weights, and adds the mean shear flow rate (j - NY/2).
+__device__ __forceinline__ double cart_u0(const Grid &g, const double *__restrict__ w0, const double *__restrict__ w1,
+                                          int id, int j, int k, double deg, double rate) {
   const int NX = g.nx, NY = g.ny;
   double sign = j - NY / 2;
   if (!(sign == 0)) sign = sign / fabs(sign);
@@ -240,14 +255,16 @@ __global__ void oblique2u_kernel(Grid g, const double *__restrict__ w0, const do
   const int i = ((id - io) % NX + NX) % NX;
   const int ip = (int)fmod(i + sign + 2 * NX, NX);
   const long s = ((long)i * NY + j) * g.nz_ + k, sp = ((long)ip * NY + j) * g.nz_ + k;
-  const long dst = ((long)id * NY + j) * g.nz_ + k;
-  u0[dst] = (beta * w0[s] + alpha * w0[sp]) + deg * (beta * w1[s] + alpha * w1[sp]) + rate * (j - NY / 2.);
-  u1[dst] = beta * w1[s] + alpha * w1[sp];
-  u2[dst] = beta * w2[s] + alpha * w2[sp];
+  return (beta * w0[s] + alpha * w0[sp]) + deg * (beta * w1[s] + alpha * w1[sp]) + rate * (j - NY / 2.);
 }
 
-// Calc_shear_rate_eff (sp_3d_ns.h:174-196), two passes in a fixed order:
-// sum of (u0(j+1) - u0(j)) / DX over j < NY-1.
+// Calc_shear_rate_eff (sp_3d_ns.h:174-196) of the Cartesian velocity
+// U_oblique2u makes of the oblique field (sp_3d_ns.cxx:319-341).  For each
+// (i, k) the sum over j < NY-1 of (ux(j+1) - ux(j)) / DX telescopes to
+// (ux(NY-1) - ux(0)) / DX, so only rows 0 and NY-1 of ux are formed; without
+// particles nothing else reads the Cartesian field (with particles
+// Calc_f_hydro_correct_precision_OBL and Make_f_particle_dt_nonsole do).
+// 32 B read per (i, k) column; partial sums in a fixed order.
 __device__ __forceinline__ double block_sum(double v, double *sh) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -260,15 +277,15 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return s;
 }
 
-__global__ __launch_bounds__(kBlock) void shear_rate_partial_kernel(Grid g, const double *__restrict__ u0,
-                                                                    double *part) {
+__global__ __launch_bounds__(kBlock) void shear_rate_rows_kernel(Grid g, const double *__restrict__ w0,
+                                                                 const double *__restrict__ w1, double deg,
+                                                                 double rate, double *part) {
   __shared__ double sh[kBlock / 64];
-  const long n = (long)g.nx * (g.ny - 1) * g.nz;
+  const long n = (long)g.nx * g.nz;
   double v = 0.0;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    const int k = (int)(t % g.nz), j = (int)((t / g.nz) % (g.ny - 1)), i = (int)(t / ((long)g.nz * (g.ny - 1)));
-    const long im = ((long)i * g.ny + j) * g.nz_ + k;
-    v += (u0[im + g.nz_] - u0[im]) / g.dx;
+    const int k = (int)(t % g.nz), i = (int)(t / g.nz);
+    v += (cart_u0(g, w0, w1, i, g.ny - 1, k, deg, rate) - cart_u0(g, w0, w1, i, 0, k, deg, rate)) / g.dx;
   }
   const double s = block_sum(v, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -369,8 +386,8 @@ int le_u_k2zeta_k(spm_ctx *c) {
 }
 
 // NS_solver_slavedEuler_Shear_OBL (fluid_solver.cxx:398-429) with
-// Zeta_k2advection_k_OBL (operate_omega.cxx:115-173).  Scratch: work[0]
-// (uk_cp), work[1..2] (advection), f[0..2] (omega).
+// Zeta_k2advection_k_OBL (operate_omega.cxx:115-173).  Scratch: work[1..2]
+// (advection), f[0..2] (omega).
 int le_ns_solver(spm_ctx *c) {
   const Grid &g = c->grid;
   const double deg = c->le_deg;
@@ -379,7 +396,6 @@ int le_ns_solver(spm_ctx *c) {
   double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
   double *const w3[3] = {c->f[0], c->f[1], c->f[2]};
   RCS(le_zeta2u_k(c, u3, 0));
-  SPM_HIP(c, hipMemcpyAsync(c->work[0], c->u[1], sizeof(double) * c->nfield, hipMemcpyDeviceToDevice, c->stream));
   tbeg(c);
   hipLaunchKernelGGL(zeta2omega_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, c->zeta[0],
                      c->zeta[1], w3[0], w3[1], w3[2]);
@@ -395,8 +411,8 @@ int le_ns_solver(spm_ctx *c) {
   RCS(fft_r2c_many(c, u3, 3));
   tbeg(c);
   hipLaunchKernelGGL(advect_tail_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, u3[0], u3[1],
-                     u3[2], c->work[0], c->le_red + kRedBlocks, deg, c->work[1], c->work[2]);
-  tend(c, SPM_K_SPECTRAL, 88.0 * nmodes(c));
+                     u3[2], c->zeta[0], c->zeta[1], c->uk_dc, c->le_red + kRedBlocks, deg, c->work[1], c->work[2]);
+  tend(c, SPM_K_SPECTRAL, 112.0 * nmodes(c));
   SPM_LAUNCH_CHECK(c);
   tbeg(c);
   hipLaunchKernelGGL(slaved_euler_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, c->zeta[0],
@@ -407,43 +423,36 @@ int le_ns_solver(spm_ctx *c) {
 }
 
 // Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390) with no particles: f is
-// zero, so U2u_oblique and Add_f_particle leave u unchanged.  ucp lives in
-// up[0..2], the copy U_oblique2u reads in f[0..2].
+// zero, so U2u_oblique and Add_f_particle leave u unchanged.  The remap
+// reads a copy of u held in up[0..2].
 int le_step(spm_ctx *c, int fix_mask) {
   const Grid &g = c->grid;
   RCS(le_ns_solver(c));
   double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
-  double *const ucp[3] = {c->up[0], c->up[1], c->up[2]};
-  double *const w3[3] = {c->f[0], c->f[1], c->f[2]};
   RCS(le_zeta2u_k(c, u3, fix_mask));
   RCS(fft_c2r_many(c, u3, 3));
-  const size_t fb = sizeof(double) * c->nfield;
-  for (int d = 0; d < 3; ++d) SPM_HIP(c, hipMemcpyAsync(ucp[d], u3[d], fb, hipMemcpyDeviceToDevice, c->stream));
   // Shear_rate_eff = Shear_rate; degree_oblique += Shear_rate_eff * dt
   const double rate = c->le_rate;
   c->le_deg += rate * c->prm.dt;
   if (c->le_deg >= 1.) {
+    double *const ucp[3] = {c->up[0], c->up[1], c->up[2]};
+    const size_t fb = sizeof(double) * c->nfield;
+    for (int d = 0; d < 3; ++d) SPM_HIP(c, hipMemcpyAsync(ucp[d], u3[d], fb, hipMemcpyDeviceToDevice, c->stream));
     tbeg(c);
     hipLaunchKernelGGL(remap_kernel, dim3(nblk(npoints(c))), dim3(kBlock), 0, c->stream, g, ucp[0], ucp[1], ucp[2],
                        u3[0], u3[1], u3[2]);
     tend(c, SPM_K_REALSPACE, 48.0 * npoints(c));
     SPM_LAUNCH_CHECK(c);
     c->le_deg -= 1.;
-    for (int d = 0; d < 3; ++d) SPM_HIP(c, hipMemcpyAsync(ucp[d], u3[d], fb, hipMemcpyDeviceToDevice, c->stream));
   }
-  for (int d = 0; d < 3; ++d) SPM_HIP(c, hipMemcpyAsync(w3[d], ucp[d], fb, hipMemcpyDeviceToDevice, c->stream));
-  tbeg(c);
-  hipLaunchKernelGGL(oblique2u_kernel, dim3(nblk(npoints(c))), dim3(kBlock), 0, c->stream, g, w3[0], w3[1], w3[2],
-                     ucp[0], ucp[1], ucp[2], c->le_deg, rate);
-  tend(c, SPM_K_REALSPACE, 48.0 * npoints(c));
-  SPM_LAUNCH_CHECK(c);
   // Calc_shear_rate_eff: ivolume = Ivolume DX^3, Ivolume = 1 / (LX LY LZ)
   const double ivolume = (1. / (g.lx * g.ly * g.lz)) * (g.dx * g.dx * g.dx);
   tbeg(c);
-  hipLaunchKernelGGL(shear_rate_partial_kernel, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, g, ucp[0], c->le_red);
+  hipLaunchKernelGGL(shear_rate_rows_kernel, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, g, u3[0], u3[1],
+                     c->le_deg, rate, c->le_red);
   hipLaunchKernelGGL(shear_rate_final_kernel, dim3(1), dim3(kBlock), 0, c->stream, c->le_red, kRedBlocks,
                      ivolume * g.ny / (g.ny - 1), c->le_red + kRedBlocks);
-  tend(c, SPM_K_REALSPACE, 8.0 * npoints(c));
+  tend(c, SPM_K_REALSPACE, 32.0 * g.nx * g.nz);
   SPM_LAUNCH_CHECK(c);
   RCS(fft_r2c_many(c, u3, 3));
   return le_u_k2zeta_k(c);
