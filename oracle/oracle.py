@@ -92,7 +92,11 @@ def lib():
         L.oracle_set_lees_edwards.argtypes = [vp, C.c_double]
         L.oracle_put_oblique.argtypes = [vp, C.c_double, C.c_double]
         L.oracle_get_oblique.argtypes = [vp, dp, dp]
-        for name in ["ns_solver_shear_obl", "zeta_k2u_k_obl", "u_k2zeta_k_obl"]:
+        L.oracle_distance0_obl.argtypes = [vp, dp, dp, dp]
+        L.oracle_distance0_obl.restype = C.c_double
+        for name in ["ns_solver_shear_obl", "zeta_k2u_k_obl", "u_k2zeta_k_obl", "make_phi_u_particle_obl",
+                     "calc_f_hydro_obl", "md_solver_position_obl", "md_solver_velocity_obl", "force_obl",
+                     "u2u_oblique", "u_oblique2u"]:
             f = getattr(L, "oracle_" + name)
             f.argtypes = [vp]
             f.restype = ip
@@ -207,6 +211,21 @@ class Oracle:
     def U_k2zeta_k_OBL(self):
         """Solenoidal_uk_OBL, contra2co and U_k2zeta_k_OBL of the spectrum in u."""
         self._call("u_k2zeta_k_obl")
+
+    # spherical particles under shear; the Cartesian velocity is u0..u2
+    def Make_phi_u_particle_OBL(self): self._call("make_phi_u_particle_obl")
+    def Calc_f_hydro_correct_precision_OBL(self): self._call("calc_f_hydro_obl")
+    def MD_solver_position_OBL(self): self._call("md_solver_position_obl")
+    def MD_solver_velocity_OBL(self): self._call("md_solver_velocity_obl")
+    def Force_OBL(self): self._call("force_obl")
+    def U2u_oblique(self): self._call("u2u_oblique")
+    def U_oblique2u(self): self._call("u_oblique2u")
+
+    def Distance0_OBL(self, x1, x2):
+        a, b = np.array(x1, dtype=np.float64), np.array(x2, dtype=np.float64)
+        x12 = np.zeros(3)
+        r = lib().oracle_distance0_obl(self._h, _dp(a), _dp(b), _dp(x12))
+        return r, x12
 
     def step(self, nsteps=1):
         self._call("step", nsteps)

@@ -866,8 +866,14 @@ static void reset_forces(oracle_ctx *c) {
 
 /* MD_solver_velocity_Euler / _AB2_hydro, particle_solver.cxx:106-147,
  * 247-289 (no self propulsion: self force and torque are 0). */
+static void velocity_update(oracle_ctx *c);
 int oracle_md_solver_velocity(oracle_ctx *c) {
   oracle_force(c);
+  velocity_update(c);
+  return 0;
+}
+
+static void velocity_update(oracle_ctx *c) {
   for (int n = 0; n < c->Np; n++) {
     spm_particle *p = &c->p[n];
     int s = p->spec;
@@ -890,7 +896,6 @@ int oracle_md_solver_velocity(oracle_ctx *c) {
     }
   }
   reset_forces(c);
-  return 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1735,8 +1740,258 @@ static void u_k2zeta_k_obl(oracle_ctx *c, double *const u[3], double *const zeta
       }
 }
 
-/* Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390) with no particles: f = 0,
- * so U2u_oblique and Add_f_particle leave u as it is. */
+/* ---- spherical particles under Lees-Edwards shear ---- */
+
+/* Relative_coord_check_stepover_Y (make_phi.h:197-236): the stencil point
+ * `cell` of a particle anchored at x_int + residue; a point across the y
+ * boundary belongs to the particle's image shifted by -sign degree_oblique
+ * NY cells in x.  Returns sign (+1 / -1 across the upper / lower boundary,
+ * else 0). */
+static int relative_coord_obl(const oracle_ctx *c, const int *cell, const int *x_int, const double *residue,
+                              int *r_mesh, double *r) {
+  const int Ns[3] = {c->NX, c->NY, c->NZ};
+  const double dx = c->DX;
+  double signY = x_int[1] + cell[1];
+  r_mesh[1] = (x_int[1] + cell[1] + Ns[1]) % Ns[1];
+  signY -= r_mesh[1];
+  int sign = (int)signY;
+  if (!(sign == 0)) sign = sign / abs(sign);
+  const double xo = fmod(x_int[0] - sign * c->degree_oblique * Ns[1] + residue[0] / dx + cell[0] + 4 * Ns[0], Ns[0]);
+  r_mesh[0] = (int)xo;
+  const double x_oblique_residue = xo - r_mesh[0];
+  r_mesh[2] = (x_int[2] + cell[2] + Ns[2]) % Ns[2];
+  r[0] = (double)cell[0] * dx - x_oblique_residue * dx;
+  r[1] = (double)cell[1] * dx - residue[1];
+  r[2] = (double)cell[2] * dx - residue[2];
+  return sign;
+}
+
+/* Make_phi_u_primitive_OBL (make_phi.cxx:299-385) with SW_UP, after
+ * Reset_phi_u: the x velocity of a point across the y boundary is taken in
+ * the frame of the particle's image, v_x - sign Shear_rate_eff LY. */
+static void make_phi_u_obl(oracle_ctx *c) {
+  reset_phi_u(c);
+  const double ly = c->L[1];
+  for (int n = 0; n < c->Np; n++) {
+    const spm_particle *pn = &c->p[n];
+    int x_int[3], r_mesh[3];
+    double residue[3], r[3], v_rot[3];
+    particle_cell(c, pn->x, x_int, residue);
+    for (int mesh = 0; mesh < c->np_domain; mesh++) {
+      const int sign = relative_coord_obl(c, &c->sekibun[3 * mesh], x_int, residue, r_mesh, r);
+      double dmy = 0.;
+      for (int d = 0; d < DIM; d++) dmy += SQ(r[d]);
+      dmy = sqrt(dmy);
+      const double dmy_phi = Phi(c, dmy, c->RADIUS);
+      const size_t im = IDX(c, r_mesh[0], r_mesh[1], r_mesh[2]);
+      c->phi[im] += dmy_phi;
+      angular2v(c, pn->omega, r, v_rot);
+      c->up[0][im] += ((pn->v[0] - sign * c->shear_rate_eff * ly + v_rot[0]) * dmy_phi);
+      c->up[1][im] += (pn->v[1] + v_rot[1]) * dmy_phi;
+      c->up[2][im] += (pn->v[2] + v_rot[2]) * dmy_phi;
+    }
+  }
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) {
+        const size_t im = IDX(c, i, j, k);
+        if (c->phi[im] > 1.) {
+          const double idmy_phi = 1. / c->phi[im];
+          c->up[0][im] *= idmy_phi;
+          c->up[1][im] *= idmy_phi;
+          c->up[2][im] *= idmy_phi;
+          c->phi[im] = 1.;
+        }
+      }
+}
+
+/* Calc_f_hydro_correct_precision_OBL (md_force.cxx:391-470, 497-506) on the
+ * Cartesian velocity `fld` (no rigid bodies; the Hydro_force stress fields
+ * are output only and not formed). */
+static void calc_f_hydro_obl(oracle_ctx *c, double *const fld[3]) {
+  const double dmy0 = -c->DX3 * c->RHO;
+  const double dmy = dmy0 / c->dt;
+  const double ly = c->L[1];
+  for (int n = 0; n < c->Np; n++) {
+    spm_particle *pn = &c->p[n];
+    int x_int[3], r_mesh[3];
+    double residue[3], r[3], v_rot[3], dmy_fp[3];
+    double force[3] = {0, 0, 0}, torque[3] = {0, 0, 0};
+    particle_cell(c, pn->x, x_int, residue);
+    for (int mesh = 0; mesh < c->np_domain; mesh++) {
+      const int sign = relative_coord_obl(c, &c->sekibun[3 * mesh], x_int, residue, r_mesh, r);
+      double dmyR = 0.;
+      for (int d = 0; d < DIM; d++) dmyR += SQ(r[d]);
+      dmyR = sqrt(dmyR);
+      const double dmy_phi = Phi(c, dmyR, c->RADIUS);
+      angular2v(c, pn->omega, r, v_rot);
+      const size_t im = IDX(c, r_mesh[0], r_mesh[1], r_mesh[2]);
+      dmy_fp[0] = ((pn->v[0] - sign * c->shear_rate_eff * ly + v_rot[0]) - fld[0][im]) * dmy_phi;
+      dmy_fp[1] = ((pn->v[1] + v_rot[1]) - fld[1][im]) * dmy_phi;
+      dmy_fp[2] = ((pn->v[2] + v_rot[2]) - fld[2][im]) * dmy_phi;
+      for (int d = 0; d < DIM; d++) force[d] += dmy_fp[d];
+      torque[0] += (r[1] * dmy_fp[2] - r[2] * dmy_fp[1]);
+      torque[1] += (r[2] * dmy_fp[0] - r[0] * dmy_fp[2]);
+      torque[2] += (r[0] * dmy_fp[1] - r[1] * dmy_fp[0]);
+    }
+    for (int d = 0; d < DIM; d++) pn->f_hydro[d] = (dmy * force[d]);
+    if (c->P.rotation)
+      for (int d = 0; d < DIM; d++) pn->torque_hydro[d] = (dmy * torque[d]);
+  }
+}
+
+/* PBC_OBL (periodic_boundary.h:22-44): a particle leaving through a y
+ * boundary re-enters at the other one shifted by -sign degree_oblique LY
+ * in x; returns sign and the x velocity change -sign Shear_rate_eff LY. */
+static int pbc_obl(const oracle_ctx *c, double *x, double *delta_vx) {
+  double signY = x[1];
+  x[1] = fmod(x[1] + c->L[1], c->L[1]);
+  signY -= x[1];
+  int sign = (int)signY;
+  if (!(sign == 0)) sign = sign / abs(sign);
+  x[0] -= (double)sign * c->degree_oblique * c->L[1];
+  x[0] = fmod(x[0] + c->L[0], c->L[0]);
+  x[2] = fmod(x[2] + c->L[2], c->L[2]);
+  *delta_vx = -(double)sign * c->shear_rate_eff * c->L[1];
+  return sign;
+}
+
+/* MD_solver_position_Euler_OBL / _AB2_OBL (particle_solver.cxx:292-346). */
+int oracle_md_solver_position_obl(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  for (int n = 0; n < c->Np; n++) {
+    spm_particle *p = &c->p[n];
+    for (int d = 0; d < DIM; d++) {
+      p->x_previous[d] = p->x[d];
+      const double delta_x = (c->ts == 0) ? c->dt * p->v[d] : c->hdt * (3.0 * p->v[d] - p->v_old[d]);
+      p->x_nopbc[d] += delta_x;
+      p->x[d] += delta_x;
+    }
+    double delta_vx;
+    pbc_obl(c, p->x, &delta_vx);
+    p->v[0] += delta_vx;
+    p->v_old[0] += delta_vx;
+    if (c->P.rotation) {
+      if (c->ts == 0)
+        orientation_euler(p, c->dt);
+      else
+        orientation_ab2(p, c->hdt);
+    }
+  }
+  return 0;
+}
+
+/* Distance0_OBL (interaction.h:39-65): the y separation to the nearest
+ * image first, then x measured from the image of x1 shifted by
+ * sign degree_oblique LY. */
+static double distance0_obl(const oracle_ctx *c, const double *x1, const double *x2, double *x12) {
+  double dmy = 0.0;
+  double signY = x2[1] - x1[1];
+  x12[1] = x2[1] - x1[1];
+  x12[1] -= (double)Nint(x12[1] / c->L[1]) * c->L[1];
+  signY -= x12[1];
+  int sign = (int)signY;
+  if (!(sign == 0)) sign = sign / abs(sign);
+  dmy += SQ(x12[1]);
+  x12[0] = x2[0] - (x1[0] + (double)sign * c->degree_oblique * c->L[1]);
+  x12[0] -= (double)Nint(x12[0] / c->L[0]) * c->L[0];
+  dmy += SQ(x12[0]);
+  x12[2] = x2[2] - x1[2];
+  x12[2] -= (double)Nint(x12[2] / c->L[2]) * c->L[2];
+  dmy += SQ(x12[2]);
+  return sqrt(dmy);
+}
+
+double oracle_distance0_obl(oracle_ctx *c, const double *x1, const double *x2, double *x12) {
+  return distance0_obl(c, x1, x2, x12);
+}
+
+/* Force_OBL (particle_solver.h:111-127): Calc_f_Lennard_Jones_OBL, the
+ * all-pairs loop of md_force.cxx:136-179 with Distance0_OBL, and
+ * Add_f_gravity. */
+int oracle_force_obl(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  if (c->P.lj_truncate >= 0)
+    for (int n = 0; n < c->Np; n++)
+      for (int m = n + 1; m < c->Np; m++) {
+        double r_ij_vec[3];
+        const double r_ij = distance0_obl(c, c->p[n].x, c->p[m].x, r_ij_vec);
+        if (r_ij < c->A_R_cutoff * c->LJ_dia) {
+          const double dmy = DMIN(DBL_MAX / r_ij, lennard_jones_f(c, r_ij, c->LJ_dia));
+          for (int d = 0; d < DIM; d++) {
+            const double dmy1 = dmy * -r_ij_vec[d];
+            c->p[n].fr[d] += dmy1;
+            c->p[m].fr[d] -= dmy1;
+          }
+        }
+      }
+  if (c->P.g != 0.0) {
+    const double Gravity_on_fluid = c->P.g * c->RHO * 4. / 3. * M_PI * SQ(c->RADIUS) * c->RADIUS;
+    for (int n = 0; n < c->Np; n++)
+      c->p[n].fr[c->P.g_direction] -= Gravity_on_fluid * (c->P.mass_ratio[c->p[n].spec] - 1.0);
+  }
+  return 0;
+}
+
+/* MD_solver_velocity_Euler_OBL / _AB2_hydro_OBL (particle_solver.cxx:
+ * 348-443): the Navier-Stokes update after Force_OBL. */
+int oracle_md_solver_velocity_obl(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  oracle_force_obl(c);
+  velocity_update(c);
+  return 0;
+}
+
+/* U2u_oblique (fft_wrapper.h:204-263): Cartesian components on the
+ * orthogonal grid -> oblique contravariant components on the sheared grid,
+ * by linear interpolation along x. */
+static void u2u_oblique(oracle_ctx *c, double *const uu[3]) {
+  const int NX = c->NX, NY = c->NY, NZ = c->NZ;
+  const double deg = c->degree_oblique;
+  for (int d = 0; d < DIM; d++) memcpy(c->wv3[d], uu[d], sizeof(double) * c->nfield);
+  for (int i = 0; i < NX; i++)
+    for (int j = 0; j < NY; j++) {
+      double sign = j - NY / 2;
+      if (!(sign == 0)) sign = sign / fabs(sign);
+      int i_oblique = (int)(sign * deg * (j - NY / 2)) * sign;
+      const double alpha = (deg * (j - NY / 2) - i_oblique) * sign;
+      const double beta = 1. - alpha;
+      i_oblique = (int)fmod(i + i_oblique + 4. * NX, NX);
+      const int i_oblique_plus = (int)fmod(i_oblique + sign + 4. * NX, NX);
+      for (int k = 0; k < NZ; k++) {
+        const size_t im = IDX(c, i, j, k), im_ob = IDX(c, i_oblique, j, k), im_ob_p = IDX(c, i_oblique_plus, j, k);
+        uu[0][im] = (beta * c->wv3[0][im_ob] + alpha * c->wv3[0][im_ob_p]) -
+                    deg * (beta * c->wv3[1][im_ob] + alpha * c->wv3[1][im_ob_p]);
+        uu[1][im] = beta * c->wv3[1][im_ob] + alpha * c->wv3[1][im_ob_p];
+        uu[2][im] = beta * c->wv3[2][im_ob] + alpha * c->wv3[2][im_ob_p];
+      }
+    }
+}
+
+/* hooks: the Cartesian velocity is taken from u0..u2 */
+int oracle_make_phi_u_particle_obl(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  make_phi_u_obl(c);
+  return 0;
+}
+int oracle_calc_f_hydro_obl(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  calc_f_hydro_obl(c, c->u);
+  return 0;
+}
+int oracle_u2u_oblique(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  u2u_oblique(c, c->u);
+  return 0;
+}
+int oracle_u_oblique2u(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  u_oblique2u(c, c->u);
+  return 0;
+}
+
+/* Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390).  Without particles f is
+ * zero and the particle block leaves u as it is. */
 static void le_step(oracle_ctx *c) {
   oracle_ns_solver_shear_obl(c);
   for (int d = 0; d < DIM; d++)
@@ -1767,6 +2022,25 @@ static void le_step(oracle_ctx *c) {
   u_oblique2u(c, c->ucp);
   obl_update_k2(c);
   calc_shear_rate_eff(c);
+  if (c->Np > 0) {
+    oracle_md_solver_position_obl(c);
+    reset_phi_u(c);
+    calc_f_hydro_obl(c, c->ucp);
+    oracle_md_solver_velocity_obl(c);
+    if (c->kBT > 0.) oracle_add_random_force_thermostat(c);
+    make_phi_u_obl(c);
+    /* Make_f_particle_dt_nonsole (f_particle.h:41-62), U2u_oblique,
+     * Add_f_particle */
+    for (size_t im = 0; im < c->nfield; im++)
+      for (int d = 0; d < DIM; d++) c->f[d][im] = c->up[d][im] - c->phi[im] * c->ucp[d][im];
+    u2u_oblique(c, c->f);
+    for (int i = 0; i < c->NX; i++)
+      for (int j = 0; j < c->NY; j++)
+        for (int k = 0; k < c->NZ; k++) {
+          const size_t im = IDX(c, i, j, k);
+          for (int d = 0; d < DIM; d++) c->u[d][im] += c->f[d][im];
+        }
+  }
   u2u_k(c, c->u);
   solenoidal_uk_obl(c, c->u);
   contra2co_field(c, c->u);
@@ -1774,7 +2048,7 @@ static void le_step(oracle_ctx *c) {
 }
 
 int oracle_set_lees_edwards(oracle_ctx *c, double shear_rate) {
-  if (!c || c->le || c->electro || c->Np > 0) return -1;
+  if (!c || c->le || c->electro) return -1;
   for (int d = 0; d < DIM; d++) {
     c->ucp[d] = (double *)calloc(c->nfield, sizeof(double));
     c->wv3[d] = (double *)calloc(c->nfield, sizeof(double));

@@ -90,6 +90,15 @@ void oracle_make_surface_normal(oracle_ctx *c, double *const normal[3]);
 int oracle_set_lees_edwards(oracle_ctx *c, double shear_rate);
 int oracle_put_oblique(oracle_ctx *c, double degree_oblique, double shear_rate_eff);
 int oracle_get_oblique(oracle_ctx *c, double *degree_oblique, double *shear_rate_eff);
+/* spherical particles under shear; the Cartesian velocity is u0..u2 */
+int oracle_make_phi_u_particle_obl(oracle_ctx *c);
+int oracle_calc_f_hydro_obl(oracle_ctx *c);
+int oracle_md_solver_position_obl(oracle_ctx *c);
+int oracle_md_solver_velocity_obl(oracle_ctx *c);
+int oracle_force_obl(oracle_ctx *c);
+int oracle_u2u_oblique(oracle_ctx *c);
+int oracle_u_oblique2u(oracle_ctx *c);
+double oracle_distance0_obl(oracle_ctx *c, const double *x1, const double *x2, double *x12);
 int oracle_ns_solver_shear_obl(oracle_ctx *c);
 int oracle_zeta_k2u_k_obl(oracle_ctx *c);
 int oracle_u_k2zeta_k_obl(oracle_ctx *c);

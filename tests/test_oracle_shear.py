@@ -168,3 +168,255 @@ def test_shearing_wave_vorticity_is_carried_by_the_shear():
     err = [abs(_shearing_wave(dt, 0.5)[0] / _shearing_wave(dt, 0.5)[1] - 1) for dt in (0.01, 0.005, 0.0025)]
     assert err[0] < 1e-2
     assert 0.4 < err[1] / err[0] < 0.6 and 0.4 < err[2] / err[1] < 0.6, err
+
+
+# ---- spherical particles under shear --------------------------------------
+
+from tests.helpers import particles  # noqa: E402
+
+
+def _le_oracle(n, a, deg, sre, np_=1, rotation=1, **kw):
+    o = Oracle(abi.make_params(n, a=a, n_particles=np_, rotation=rotation, **kw))
+    o.set_lees_edwards(sre)
+    o.put_oblique(deg, sre)
+    return o
+
+
+def _images(xp, L, deg):
+    """The particle's Lees-Edwards images: the copy b boxes up in y sits
+    b deg LY further along x and moves b Shear_rate_eff LY faster in x."""
+    out = []
+    for b in (-1, 0, 1):
+        for a_ in (-1, 0, 1):
+            for c in (-1, 0, 1):
+                out.append((b, xp + np.array([a_ * L[0] + b * deg * L[1], b * L[1], c * L[2]])))
+    return out
+
+
+def _brute_phi_up(o, p, n, deg, sre):
+    """phi and up of one particle from the distance of every grid point to
+    the particle's nearest sheared image (no clamp: one particle)."""
+    L = np.array([n, n, n], dtype=float)
+    X = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).astype(float)
+    phi = np.zeros((n, n, n))
+    up = np.zeros((3, n, n, n))
+    for b, img in _images(p["x"][0], L, deg):
+        r = X - img
+        dist = np.sqrt((r ** 2).sum(-1))
+        for idx in zip(*np.nonzero(dist < o.prm_a + 2.0)):
+            ph = o.phi_profile(dist[idx])
+            if ph == 0.0:
+                continue
+            rr = r[idx]
+            v = p["v"][0] + np.cross(p["omega"][0], rr) + np.array([b * sre * L[1], 0.0, 0.0])
+            phi[idx] += ph
+            up[:, idx[0], idx[1], idx[2]] += v * ph
+    return phi, up
+
+
+@pytest.mark.parametrize("y", [1.3, 30.6])
+def test_obl_phi_is_the_sheared_image(y):
+    n, a, deg, sre = 32, 3.0, 0.37, 0.2
+    o = _le_oracle(n, a, deg, sre)
+    o.prm_a = a
+    p = particles(1, (n, n, n), a, positions=np.array([[7.2, y, 15.4]]))
+    p["v"][0] = [0.02, -0.01, 0.015]
+    p["omega"][0] = [0.004, -0.003, 0.002]
+    o.put_particles(p)
+    o.Make_phi_u_particle_OBL()
+    phi, up = _brute_phi_up(o, p, n, deg, sre)
+    got = o.get_field("phi")[:, :, :n]
+    assert (got > 0).sum() > 50 and np.abs(got - phi).max() < 1e-11
+    # the stencil does reach across the y boundary
+    rows = np.nonzero(got.sum(axis=(0, 2)))[0]
+    assert 0 in rows and n - 1 in rows
+    for d in range(3):
+        assert np.abs(o.get_field("up%d" % d)[:, :, :n] - up[d]).max() < 1e-11, d
+
+
+def test_obl_f_hydro_is_the_sheared_image():
+    n, a, deg, sre, dt, rho = 32, 3.0, 0.61, 0.3, 0.05, 1.2
+    o = _le_oracle(n, a, deg, sre, dt=dt, rho=rho)
+    o.prm_a = a
+    p = particles(1, (n, n, n), a, positions=np.array([[20.7, 30.9, 11.2]]))
+    p["v"][0] = [0.01, 0.02, -0.01]
+    p["omega"][0] = [-0.002, 0.003, 0.001]
+    o.put_particles(p)
+    rng = np.random.default_rng(4)
+    u = [rng.standard_normal(o.shape) * 0.05 for _ in range(3)]
+    for d in range(3):
+        o.put_field("u%d" % d, u[d])
+    o.Calc_f_hydro_correct_precision_OBL()
+    q = o.get_particles()[0]
+    phi, up = _brute_phi_up(o, p, n, deg, sre)
+    fp = np.stack([(up[d] - phi * u[d][:, :, :n]) for d in range(3)])
+    force = -(rho / dt) * fp.sum(axis=(1, 2, 3))
+    np.testing.assert_allclose(q["f_hydro"], force, rtol=1e-10, atol=1e-14)
+    # torque about the particle: r from the image each point belongs to
+    L = np.array([n, n, n], dtype=float)
+    X = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).astype(float)
+    torque = np.zeros(3)
+    for b, img in _images(p["x"][0], L, deg):
+        r = X - img
+        m = np.sqrt((r ** 2).sum(-1)) < a + 2.0
+        torque += np.cross(r[m], fp[:, m].T).sum(0)
+    np.testing.assert_allclose(q["torque_hydro"], -(rho / dt) * torque, rtol=1e-10, atol=1e-14)
+
+
+def test_obl_routines_reduce_to_the_unsheared_ones():
+    """At degree_oblique = 0 and Shear_rate_eff = 0 the oblique particle
+    routines are the (pinned) Navier-Stokes ones."""
+    n, a = 64, 4.0
+    prm = abi.make_params(n, a=a, n_particles=6, rotation=1, dt=0.05, lj_truncate=abi.LJ_WCA, g=0.3,
+                          mass_ratio=(1.5,))
+    # three close pairs (inside the WCA cutoff), one across the periodic y boundary
+    pos = np.array([[10.0, 20.0, 30.0], [18.6, 20.5, 30.2], [40.0, 1.0, 12.0], [40.5, 62.6, 12.3],
+                    [30.0, 40.0, 50.0], [30.2, 40.4, 58.5]])
+    p = particles(6, (n, n, n), a, seed=7, vel=0.02, omega=0.004, positions=pos)
+    ns, le = Oracle(prm), Oracle(prm)
+    le.set_lees_edwards(0.0)
+    rng = np.random.default_rng(2)
+    u = [rng.standard_normal(ns.shape) * 0.05 for _ in range(3)]
+    for o in (ns, le):
+        o.put_particles(p)
+        for d in range(3):
+            o.put_field("u%d" % d, u[d])
+    ns.Make_phi_u_particle()
+    le.Make_phi_u_particle_OBL()
+    for f in ("phi", "up0", "up1", "up2"):
+        assert rel(le.get_field(f), ns.get_field(f)) < 1e-12, f
+    ns.Calc_f_hydro_correct_precision()
+    le.Calc_f_hydro_correct_precision_OBL()
+    qa, qb = ns.get_particles(), le.get_particles()
+    for k in ("f_hydro", "torque_hydro"):
+        assert rel(qb[k], qa[k]) < 1e-12, k
+    ns.Force()
+    le.Force_OBL()  # all pairs instead of linked cells
+    assert rel(le.get_particles()["fr"], ns.get_particles()["fr"]) < 1e-13
+    assert np.abs(ns.get_particles()["fr"]).max() > 0
+    for o in (ns, le):
+        o.put_particles(p)
+    ns.MD_solver_position()
+    le.MD_solver_position_OBL()
+    qa, qb = ns.get_particles(), le.get_particles()
+    for k in ("x", "x_nopbc", "v", "v_old"):
+        np.testing.assert_array_equal(qb[k], qa[k], err_msg=k)
+
+
+@pytest.mark.parametrize("up", [True, False])
+def test_pbc_obl_crossing(up):
+    n, a, deg, sre, dt = 32, 3.0, 0.3, 0.25, 0.5
+    o = _le_oracle(n, a, deg, sre, dt=dt)
+    y0 = 31.8 if up else 0.1
+    p = particles(1, (n, n, n), a, positions=np.array([[5.0, y0, 9.0]]))
+    v = np.array([0.1, 0.6 if up else -0.6, -0.2])
+    p["v"][0] = v
+    p["v_old"][0] = v
+    o.put_particles(p)
+    o.MD_solver_position_OBL()  # ts = 0: Euler
+    q = o.get_particles()[0]
+    s = 1 if up else -1  # leaves through the upper / lower y boundary
+    raw = np.array([5.0, y0, 9.0]) + dt * v
+    np.testing.assert_allclose(q["x_nopbc"], raw, rtol=0, atol=1e-14)
+    want = np.array([(raw[0] - s * deg * n) % n, raw[1] - s * n, raw[2] % n])
+    np.testing.assert_allclose(q["x"], want, rtol=0, atol=1e-13)
+    # it comes in through the other boundary, where the flow differs by
+    # Shear_rate_eff LY
+    assert q["v"][0] == pytest.approx(v[0] - s * sre * n, abs=1e-15)
+    assert q["v_old"][0] == pytest.approx(v[0] - s * sre * n, abs=1e-15)
+    assert list(q["v"][1:]) == list(v[1:])
+
+
+def test_distance0_obl_is_the_nearest_sheared_image():
+    n, deg = 32, 0.43
+    o = _le_oracle(n, 3.0, deg, 0.1)
+    L = np.array([n, n, n], dtype=float)
+    rng = np.random.default_rng(9)
+    crossed = 0
+    for _ in range(400):
+        x1 = rng.uniform(0, n, 3)
+        off = rng.uniform(-4.5, 4.5, 3)
+        # x2 = x1 + off seen from the copy of the box b boxes down
+        b = int(np.floor((x1[1] + off[1]) / n))
+        x2 = (x1 + off - b * np.array([deg * n, n, 0.0])) % L
+        crossed += b != 0
+        r, x12 = o.Distance0_OBL(x1, x2)
+        best = min(np.linalg.norm(x2 - img) for _, img in _images(x1, L, deg))
+        assert best == pytest.approx(np.linalg.norm(off), abs=1e-12)
+        assert r == pytest.approx(best, abs=1e-12)
+        np.testing.assert_allclose(x12, off, rtol=0, atol=1e-12)
+    assert crossed > 20
+
+
+def test_u2u_oblique_shift_and_inverse():
+    n = 16
+    o = _le_oracle(n, 2.0, 0.5, 0.0)
+    rng = np.random.default_rng(5)
+    w = [rng.standard_normal(o.shape) for _ in range(3)]
+    for d in range(3):
+        o.put_field("u%d" % d, w[d])
+    o.U2u_oblique()
+    got = [o.get_field("u%d" % d) for d in range(3)]
+    # where deg (j - NY/2) is a whole number m the oblique point (i, j) is the
+    # Cartesian point i + m: a pure shift with u0 - deg u1
+    for j in range(0, n, 2):
+        m = (j - n // 2) // 2
+        for i in range(n):
+            s = (i + m) % n
+            np.testing.assert_allclose(got[0][i, j, :n], w[0][s, j, :n] - 0.5 * w[1][s, j, :n], rtol=0, atol=1e-14)
+            np.testing.assert_allclose(got[1][i, j, :n], w[1][s, j, :n], rtol=0, atol=0)
+    # fields constant along x: U2u_oblique undoes U_oblique2u exactly
+    o2 = _le_oracle(n, 2.0, 0.3, 0.0)
+    c = [np.repeat(rng.standard_normal((1,) + o.shape[1:]), n, axis=0) for _ in range(3)]
+    for d in range(3):
+        o2.put_field("u%d" % d, c[d])
+    o2.U_oblique2u()
+    o2.U2u_oblique()
+    for d in range(3):
+        np.testing.assert_allclose(o2.get_field("u%d" % d)[:, :, :n], c[d][:, :, :n], rtol=0, atol=1e-14)
+
+
+def test_particle_step_away_from_the_boundary_is_the_navier_stokes_step():
+    """Particles whose stencils stay clear of the y boundaries, no shear:
+    three Time_evolution_hydro_OBL steps equal three Navier-Stokes steps
+    (f added unprojected and the projection done once at the end, LJ by
+    all pairs, the same thermostat stream)."""
+    n, a = 32, 3.0
+    prm = abi.make_params(n, a=a, n_particles=3, rotation=1, dt=0.05, kbt=0.01, fix_cell=(0, 0, 0),
+                          lj_truncate=abi.LJ_WCA)
+    p = particles(3, (n, n, n), a, vel=0.01, omega=0.002,
+                  positions=np.array([[8.0, 12.0, 10.0], [16.5, 19.0, 20.0], [24.0, 15.5, 6.0]]))
+    ns, le = Oracle(prm), Oracle(prm)
+    le.set_lees_edwards(0.0)
+    # the fluid starts at rest (the two store zeta with other components)
+    for o in (ns, le):
+        o.put_particles(p)
+    for _ in range(3):
+        le.put_oblique(0.0, 0.0)
+        ns.step(1)
+        le.step(1)
+    qa, qb = ns.get_particles(), le.get_particles()
+    for k in ("x", "v", "omega"):
+        assert rel(qb[k], qa[k]) < 1e-12, k
+    assert rel(qa["v"], p["v"]) > 1e-3  # they moved
+    # zeta carries the modes outside the 2/3 box after a step (f is not
+    # band-limited); the next step truncates them first, so compare inside it
+    for f in ("zeta0", "zeta1"):
+        le.put_field(f, le.truncate(le.get_field(f)))
+    ua, ub = velocity(ns, False), velocity(le, True)
+    for d in range(3):
+        assert rel(ub[d][:, :, :n], ua[d][:, :, :n]) < 1e-11, d
+
+
+def test_sphere_in_shear_rotates_at_half_the_vorticity():
+    """A free sphere on the zero-velocity plane of a simple shear flow
+    u_x = rate (y - LY/2) stays where it is and spins at omega_z = -rate/2
+    (Jeffery's result for a sphere; the fluid's vorticity is -rate)."""
+    n, a, rate = 32, 3.0, 0.01
+    o = Oracle(abi.make_params(n, a=a, n_particles=1, rotation=1, dt=0.1, fix_cell=(0, 0, 0)))
+    o.set_lees_edwards(rate)
+    o.put_particles(particles(1, (n, n, n), a, positions=np.array([[16.0, 16.0, 16.0]])))
+    o.step(40)
+    q = o.get_particles()[0]
+    assert abs(q["omega"][2] / rate + 0.5) < 0.01, q["omega"]
+    assert np.abs(q["omega"][:2]).max() < 1e-12 * rate and np.abs(q["v"]).max() < 1e-12 * rate
