@@ -306,9 +306,9 @@ int spm_rescale_solute(spm_ctx *ctx);
  * Coulomb body force on the fluid into the f field (real space). */
 int spm_make_coulomb_force(spm_ctx *ctx);
 
-/* ---- Lees-Edwards shear in oblique coordinates, fluid only ---------------
- * SW_EQ == Shear_Navier_Stokes_Lees_Edwards with no particles: spm_step
- * runs Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390).  The grid shears
+/* ---- Lees-Edwards shear in oblique coordinates ----------------------------
+ * SW_EQ == Shear_Navier_Stokes_Lees_Edwards with free spherical particles
+ * (or none): spm_step runs Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390).  The grid shears
  * with the flow; degree_oblique (the strain since the last remap, a global
  * of the reference, input.cxx:271) grows by Shear_rate * dt each step and
  * the grid is remapped when it reaches 1.  Shear_rate_eff (sp_3d_ns.h:
@@ -318,7 +318,7 @@ int spm_make_coulomb_force(spm_ctx *ctx);
  * mean flow.  FP64, literal schedule. */
 /* Switches the context to the sheared equations; Shear_rate is
  * constitutive_eq.Shear_Navier_Stokes_Lees_Edwards.External_field.DC.Shear_rate
- * (input.cxx:600-646).  Call once, after spm_create, with n_particles == 0. */
+ * (input.cxx:600-646).  Call once, after spm_create. */
 int spm_set_lees_edwards(spm_ctx *ctx, double shear_rate);
 /* degree_oblique and Shear_rate_eff in and out (the restart's
  * resume.CONTINUE.Saved_Data.oblique, resume.cxx:335-339). */
@@ -334,6 +334,22 @@ int spm_zeta_k2u_k_obl(spm_ctx *ctx);
  * 327-361) and U_k2zeta_k_OBL of the contravariant spectrum in u0..u2 into
  * zeta and uk_dc (u0..u2 are left holding the covariant spectrum). */
 int spm_u_k2zeta_k_obl(spm_ctx *ctx);
+/* The particle routines of the sheared step, on the context's particles
+ * with the current degree_oblique and Shear_rate_eff; the Cartesian fluid
+ * velocity they read is u0..u2 (the step passes U_oblique2u of u).
+ * Make_phi_u_particle_OBL (make_phi.cxx:299-385, after Reset_phi_u) into
+ * phi, up0..up2. */
+int spm_make_phi_u_particle_obl(spm_ctx *ctx);
+/* Calc_f_hydro_correct_precision_OBL (md_force.cxx:391-470, 497-506; the
+ * Hydro_force stress fields are not formed). */
+int spm_calc_f_hydro_obl(spm_ctx *ctx);
+/* MD_solver_position_Euler_OBL (ts == 0) / _AB2_OBL with PBC_OBL
+ * (particle_solver.cxx:292-346, periodic_boundary.h:22-44). */
+int spm_md_solver_position_obl(spm_ctx *ctx);
+/* MD_solver_velocity_Euler_OBL (ts == 0) / _AB2_hydro_OBL with Force_OBL:
+ * all-pairs LJ with Distance0_OBL and gravity (particle_solver.cxx:348-443,
+ * particle_solver.h:111-127, md_force.cxx:136-179). */
+int spm_md_solver_velocity_obl(spm_ctx *ctx);
 
 #ifdef __cplusplus
 }

@@ -559,6 +559,8 @@ __device__ __forceinline__ void space2body(double *out, const double *v, const Q
   out[2] = qc.v2;
 }
 
+__device__ __forceinline__ void orientation_step(const PModel &pm, double *pb, int n, double dt, double hdt, int euler);
+
 __global__ void position_kernel(Grid g, PModel pm, double *pb, double dt, double hdt, int euler) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= pm.np) return;
@@ -571,7 +573,13 @@ __global__ void position_kernel(Grid g, PModel pm, double *pb, double dt, double
     PR(pb, pm.np, P_X_NOPBC + d, n) += delta;
     PR(pb, pm.np, P_X + d, n) = fmod((x + delta) + L[d], L[d]);  // PBC, periodic_boundary.h:9-15
   }
-  if (pm.rotation) {
+  if (pm.rotation) orientation_step(pm, pb, n, dt, hdt, euler);
+}
+
+// MD_solver_orientation_Euler / _AB2 (particle_rotation_solver.h:15-39)
+__device__ __forceinline__ void orientation_step(const PModel &pm, double *pb, int n, double dt, double hdt,
+                                                 int euler) {
+  {
     const Q q = qload(pb, pm.np, P_Q, n);
     Q dq;
     if (euler) {  // MD_solver_orientation_Euler
@@ -642,6 +650,287 @@ __global__ void thermostat_kernel(PModel pm, double *pb, const int *__restrict__
     if (pm.rotation) PR(pb, pm.np, P_OMEGA + d, n) += noise[6 * n + d + 3] * no * imoi;
   }
 }
+
+// ---------------------------------------------------------------------
+// spherical particles under Lees-Edwards shear (sp_3d_ns.cxx:343-373).
+// The oblique geometry is written with the reference's expression order and
+// without contraction into fused multiply-adds, so that the mesh index a
+// floor() picks does not depend on rounding the reference does not have.
+// ---------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+// Relative_coord_check_stepover_Y (make_phi.h:197-236): stencil point
+// `cell` of the particle anchored at xi + res; returns sign (+1 / -1 when
+// the point lies across the upper / lower y boundary, where it belongs to
+// the particle's image shifted by -sign deg NY cells in x).
+__device__ __forceinline__ int rel_coord_obl(const Grid &g, const PModel &pm, const int cell[3], const int xi[3],
+                                             const double res[3], double deg, int rm[3], double r[3]) {
+  const int NX = g.nx, NY = g.ny, NZ = g.nz;
+  double signY = xi[1] + cell[1];
+  rm[1] = (xi[1] + cell[1] + NY) % NY;
+  signY -= rm[1];
+  int sign = (int)signY;
+  if (!(sign == 0)) sign = sign / abs(sign);
+  double xo = fmod(xi[0] - sign * deg * NY + res[0] / pm.dx + cell[0] + 4 * NX, (double)NX);
+  if (xo < 0.0) xo += NX;  // only for NY > 4 NX - stencil (the reference asserts there)
+  rm[0] = (int)xo;
+  if (rm[0] >= NX) rm[0] = NX - 1;
+  const double xres = xo - rm[0];
+  rm[2] = (xi[2] + cell[2] + NZ) % NZ;
+  r[0] = (double)cell[0] * pm.dx - xres * pm.dx;
+  r[1] = (double)cell[1] * pm.dx - res[1];
+  r[2] = (double)cell[2] * pm.dx - res[2];
+  return sign;
+}
+
+// Make_phi_u_primitive_OBL (make_phi.cxx:299-364), one wavefront per
+// particle over its stencil: phi and up are scattered with FP64 atomics
+// (the image of a particle across the y boundary lands on a shifted,
+// fractional part of the grid, outside the tile lists of the unsheared
+// path), so sums at points two particles share are ordered differently
+// from run to run, at rounding level.  The x velocity of an image point
+// is v_x - sign Shear_rate_eff LY.
+__global__ __launch_bounds__(256) void make_phi_obl_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+                                                           const int *__restrict__ pxint,
+                                                           const double *__restrict__ pres,
+                                                           const int *__restrict__ stencil, int nst, double deg,
+                                                           const double *__restrict__ sre_p, double *phi,
+                                                           double *up0, double *up1, double *up2) {
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (n >= pm.np) return;
+  const int xi[3] = {pxint[3 * n], pxint[3 * n + 1], pxint[3 * n + 2]};
+  const double res[3] = {pres[3 * n], pres[3 * n + 1], pres[3 * n + 2]};
+  const double vp[3] = {P(pb, pm.np, P_V, n), P(pb, pm.np, P_V + 1, n), P(pb, pm.np, P_V + 2, n)};
+  const double wp[3] = {P(pb, pm.np, P_OMEGA, n), P(pb, pm.np, P_OMEGA + 1, n), P(pb, pm.np, P_OMEGA + 2, n)};
+  const double shift = *sre_p * g.ly;
+  const double rcut = pm.radius + pm.hxi;
+  for (int s = lane; s < nst; s += 64) {
+    const int cell[3] = {stencil[3 * s], stencil[3 * s + 1], stencil[3 * s + 2]};
+    int rm[3];
+    double r[3];
+    const int sign = rel_coord_obl(g, pm, cell, xi, res, deg, rm, r);
+    double dist = 0.0;
+    for (int d = 0; d < 3; ++d) dist += sq(r[d]);
+    dist = sqrt(dist);
+    if (!(dist < rcut)) continue;  // phi = 0 adds nothing
+    const double dphi = phi_fast<double>(pm, dist);
+    double vr[3] = {0.0, 0.0, 0.0};
+    if (pm.rotation) {
+      vr[0] = wp[1] * r[2] - wp[2] * r[1];
+      vr[1] = wp[2] * r[0] - wp[0] * r[2];
+      vr[2] = wp[0] * r[1] - wp[1] * r[0];
+    }
+    const long im = ((long)rm[0] * g.ny + rm[1]) * g.nz_ + rm[2];
+    atomicAdd(phi + im, dphi);
+    atomicAdd(up0 + im, (vp[0] - sign * shift + vr[0]) * dphi);
+    atomicAdd(up1 + im, (vp[1] + vr[1]) * dphi);
+    atomicAdd(up2 + im, (vp[2] + vr[2]) * dphi);
+  }
+}
+
+// the clamp of make_phi.cxx:366-384
+__global__ void phi_clamp_kernel(Grid g, double *phi, double *up0, double *up1, double *up2) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.nx * g.ny * g.nz) return;
+  const long im = (t / g.nz) * g.nz_ + t % g.nz;
+  const double ph = phi[im];
+  if (ph > 1.) {
+    const double inv = 1. / ph;
+    up0[im] *= inv;
+    up1[im] *= inv;
+    up2[im] *= inv;
+    phi[im] = 1.;
+  }
+}
+
+// Calc_f_hydro_correct_precision_OBL (md_force.cxx:391-470, 497-506) on the
+// Cartesian velocity u: one wavefront per particle, sums in a fixed order.
+__global__ __launch_bounds__(256) void f_hydro_obl_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ pxint,
+                                                          const double *__restrict__ pres,
+                                                          const int *__restrict__ stencil, int nst,
+                                                          const double *__restrict__ u0,
+                                                          const double *__restrict__ u1,
+                                                          const double *__restrict__ u2, double deg,
+                                                          const double *__restrict__ sre_p, double dmy) {
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (n >= pm.np) return;
+  const int xi[3] = {pxint[3 * n], pxint[3 * n + 1], pxint[3 * n + 2]};
+  const double res[3] = {pres[3 * n], pres[3 * n + 1], pres[3 * n + 2]};
+  const double vp[3] = {P(pb, pm.np, P_V, n), P(pb, pm.np, P_V + 1, n), P(pb, pm.np, P_V + 2, n)};
+  const double wp[3] = {P(pb, pm.np, P_OMEGA, n), P(pb, pm.np, P_OMEGA + 1, n), P(pb, pm.np, P_OMEGA + 2, n)};
+  const double shift = *sre_p * g.ly;
+  const double rcut = pm.radius + pm.hxi;
+  double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0;
+  for (int s = lane; s < nst; s += 64) {
+    const int cell[3] = {stencil[3 * s], stencil[3 * s + 1], stencil[3 * s + 2]};
+    int rm[3];
+    double r[3];
+    const int sign = rel_coord_obl(g, pm, cell, xi, res, deg, rm, r);
+    double dist = 0.0;
+    for (int d = 0; d < 3; ++d) dist += sq(r[d]);
+    dist = sqrt(dist);
+    if (!(dist < rcut)) continue;
+    const double dphi = phi_fast<double>(pm, dist);
+    double vr[3] = {0.0, 0.0, 0.0};
+    if (pm.rotation) {
+      vr[0] = wp[1] * r[2] - wp[2] * r[1];
+      vr[1] = wp[2] * r[0] - wp[0] * r[2];
+      vr[2] = wp[0] * r[1] - wp[1] * r[0];
+    }
+    const long im = ((long)rm[0] * g.ny + rm[1]) * g.nz_ + rm[2];
+    const double f0 = ((vp[0] - sign * shift + vr[0]) - u0[im]) * dphi;
+    const double f1 = ((vp[1] + vr[1]) - u1[im]) * dphi;
+    const double f2 = ((vp[2] + vr[2]) - u2[im]) * dphi;
+    fx += f0;
+    fy += f1;
+    fz += f2;
+    tx += (r[1] * f2 - r[2] * f1);
+    ty += (r[2] * f0 - r[0] * f2);
+    tz += (r[0] * f1 - r[1] * f0);
+  }
+  fx = wave_sum(fx);
+  fy = wave_sum(fy);
+  fz = wave_sum(fz);
+  tx = wave_sum(tx);
+  ty = wave_sum(ty);
+  tz = wave_sum(tz);
+  if (lane == 0) {
+    PR(pb, pm.np, P_F_HYDRO, n) = dmy * fx;
+    PR(pb, pm.np, P_F_HYDRO + 1, n) = dmy * fy;
+    PR(pb, pm.np, P_F_HYDRO + 2, n) = dmy * fz;
+    if (pm.rotation) {
+      PR(pb, pm.np, P_TORQUE_HYDRO, n) = dmy * tx;
+      PR(pb, pm.np, P_TORQUE_HYDRO + 1, n) = dmy * ty;
+      PR(pb, pm.np, P_TORQUE_HYDRO + 2, n) = dmy * tz;
+    }
+  }
+}
+
+// PBC_OBL (periodic_boundary.h:22-44) after the step of
+// MD_solver_position_*_OBL (particle_solver.cxx:292-346).
+__global__ void position_obl_kernel(Grid g, PModel pm, double *pb, double dt, double hdt, int euler, double deg,
+                                    const double *__restrict__ sre_p) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= pm.np) return;
+  double x[3];
+  for (int d = 0; d < 3; ++d) {
+    x[d] = P(pb, pm.np, P_X + d, n);
+    PR(pb, pm.np, P_X_PREV + d, n) = x[d];
+    const double v = P(pb, pm.np, P_V + d, n);
+    const double delta = euler ? dt * v : hdt * (3.0 * v - P(pb, pm.np, P_V_OLD + d, n));
+    PR(pb, pm.np, P_X_NOPBC + d, n) += delta;
+    x[d] += delta;
+  }
+  double signY = x[1];
+  x[1] = fmod(x[1] + g.ly, g.ly);
+  signY -= x[1];
+  int sign = (int)signY;
+  if (!(sign == 0)) sign = sign / abs(sign);
+  x[0] -= (double)sign * deg * g.ly;
+  x[0] = fmod(x[0] + g.lx, g.lx);
+  if (x[0] < 0.0) {  // LY > LX: the shift can exceed one box (the reference asserts)
+    x[0] -= floor(x[0] / g.lx) * g.lx;
+    if (!(x[0] < g.lx)) x[0] = 0.0;
+  }
+  x[2] = fmod(x[2] + g.lz, g.lz);
+  for (int d = 0; d < 3; ++d) PR(pb, pm.np, P_X + d, n) = x[d];
+  const double dvx = -(double)sign * *sre_p * g.ly;
+  PR(pb, pm.np, P_V, n) += dvx;
+  PR(pb, pm.np, P_V_OLD, n) += dvx;
+  if (pm.rotation) orientation_step(pm, pb, n, dt, hdt, euler);
+}
+
+// Distance0_OBL (interaction.h:39-65)
+__device__ __forceinline__ double distance0_obl(const Grid &g, double deg, const double *x1, const double *x2,
+                                                double *x12) {
+  double dmy = 0.0;
+  double signY = x2[1] - x1[1];
+  x12[1] = x2[1] - x1[1];
+  x12[1] -= (double)nint(x12[1] / g.ly) * g.ly;
+  signY -= x12[1];
+  int sign = (int)signY;
+  if (!(sign == 0)) sign = sign / abs(sign);
+  dmy += sq(x12[1]);
+  x12[0] = x2[0] - (x1[0] + (double)sign * deg * g.ly);
+  x12[0] -= (double)nint(x12[0] / g.lx) * g.lx;
+  dmy += sq(x12[0]);
+  x12[2] = x2[2] - x1[2];
+  x12[2] -= (double)nint(x12[2] / g.lz) * g.lz;
+  dmy += sq(x12[2]);
+  return sqrt(dmy);
+}
+
+// Force_OBL (particle_solver.h:111-127): the all-pairs LJ of
+// md_force.cxx:136-179 with Distance0_OBL, then gravity.  One wavefront
+// per particle i; each pair is measured in the reference's orientation
+// (from the lower index to the higher), and the lane sums are combined by a
+// fixed shuffle tree.
+__global__ __launch_bounds__(256) void force_obl_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ spec,
+                                                        double deg) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= pm.np) return;
+  double F[3] = {0.0, 0.0, 0.0};
+  if (pm.lj_on) {
+    const double xi[3] = {P(pb, pm.np, P_X, i), P(pb, pm.np, P_X + 1, i), P(pb, pm.np, P_X + 2, i)};
+    for (int j = lane; j < pm.np; j += 64) {
+      if (j == i) continue;
+      const double xj[3] = {P(pb, pm.np, P_X, j), P(pb, pm.np, P_X + 1, j), P(pb, pm.np, P_X + 2, j)};
+      double rv[3];
+      const double r = i < j ? distance0_obl(g, deg, xi, xj, rv) : distance0_obl(g, deg, xj, xi, rv);
+      if (r < pm.lj_cutoff) {
+        const double ljf = lj_f(pm, r);
+        const double cap = 1.79769313486231570815e+308 / r;
+        const double dmy = cap < ljf ? cap : ljf;
+        const double sgn = i < j ? 1.0 : -1.0;  // fr[n] += dmy1, fr[m] -= dmy1
+#pragma unroll
+        for (int d = 0; d < 3; ++d) F[d] += sgn * (dmy * -rv[d]);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[d] = wave_sum(F[d]);
+  }
+  if (lane != 0) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) PR(pb, pm.np, P_FR + d, i) += F[d];
+  if (pm.g_on_fluid != 0.0) PR(pb, pm.np, P_FR + pm.g_direction, i) -= pm.g_on_fluid * (pm.mass_ratio[spec[i]] - 1.0);
+}
+
+// Make_f_particle_dt_nonsole (f_particle.h:41-62), U2u_oblique
+// (fft_wrapper.h:204-263, already a gather) and Add_f_particle in one pass:
+// u(i, j) += the oblique components of f = up - phi ucp interpolated
+// between x rows i_oblique and i_oblique + sign.  ucp is the Cartesian
+// velocity.  112 B read, 24 B written per point.
+__global__ void f_add_obl_kernel(Grid g, const double *__restrict__ up0, const double *__restrict__ up1,
+                                 const double *__restrict__ up2, const double *__restrict__ phi,
+                                 const double *__restrict__ c0, const double *__restrict__ c1,
+                                 const double *__restrict__ c2, double *u0, double *u1, double *u2, double deg) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.nx * g.ny * g.nz) return;
+  const int NX = g.nx, NY = g.ny;
+  const int k = (int)(t % g.nz), j = (int)((t / g.nz) % NY), i = (int)(t / ((long)g.nz * NY));
+  double sign = j - NY / 2;
+  if (!(sign == 0)) sign = sign / fabs(sign);
+  int io = (int)(sign * deg * (j - NY / 2)) * sign;
+  const double alpha = (deg * (j - NY / 2) - io) * sign;
+  const double beta = 1. - alpha;
+  // (i + io) mod NX and the next row toward sign, as fmod(... + 4 NX, NX)
+  // gives them, in integer arithmetic that stays in range for any NY
+  io = ((i + io) % NX + NX) % NX;
+  const int iop = ((io + (int)sign) % NX + NX) % NX;
+  const long a = ((long)io * NY + j) * g.nz_ + k, b = ((long)iop * NY + j) * g.nz_ + k;
+  const long im = ((long)i * NY + j) * g.nz_ + k;
+  const double fa0 = up0[a] - phi[a] * c0[a], fb0 = up0[b] - phi[b] * c0[b];
+  const double fa1 = up1[a] - phi[a] * c1[a], fb1 = up1[b] - phi[b] * c1[b];
+  const double fa2 = up2[a] - phi[a] * c2[a], fb2 = up2[b] - phi[b] * c2[b];
+  u0[im] += (beta * fa0 + alpha * fb0) - deg * (beta * fa1 + alpha * fb1);
+  u1[im] += beta * fa1 + alpha * fb1;
+  u2[im] += beta * fa2 + alpha * fb2;
+}
+
+#pragma clang fp contract(on)
 
 inline unsigned blocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -827,6 +1116,73 @@ int p_velocity(spm_ctx *c, int euler) {
   hipLaunchKernelGGL(velocity_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pbuf, c->spec,
                      c->prm.dt, 0.5 * c->prm.dt, euler);
   tend(c, SPM_K_PARTICLE, 8.0 * 60 * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+// ---- Lees-Edwards shear (device Shear_rate_eff at sre) ----
+int p_position_obl(spm_ctx *c, int euler, const double *sre) {
+  if (c->np == 0) return 0;
+  tbeg(c);
+  hipLaunchKernelGGL(position_obl_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->prm.dt, 0.5 * c->prm.dt, euler, c->le_deg, sre);
+  tend(c, SPM_K_PARTICLE, 8.0 * 40 * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int p_calc_f_hydro_obl(spm_ctx *c, double *const ucp[3], const double *sre) {
+  if (c->np == 0) return 0;
+  RCP(p_build_tiles(c));  // anchors at the current positions
+  const double dmy = (-c->pm.dx3 * c->pm.rho) / c->prm.dt;
+  tbeg(c);
+  hipLaunchKernelGGL(f_hydro_obl_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->pxint, c->pres, c->stencil, c->n_stencil, ucp[0], ucp[1], ucp[2], c->le_deg, sre, dmy);
+  tend(c, SPM_K_F_HYDRO, 24.0 * (double)c->n_stencil * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int p_velocity_obl(spm_ctx *c, int euler) {
+  if (c->np == 0) return 0;
+  tbeg(c);
+  hipLaunchKernelGGL(force_obl_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->spec, c->le_deg);
+  tend(c, SPM_K_PARTICLE, 24.0 * (double)c->np * c->np);
+  SPM_LAUNCH_CHECK(c);
+  tbeg(c);
+  hipLaunchKernelGGL(velocity_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pbuf, c->spec,
+                     c->prm.dt, 0.5 * c->prm.dt, euler);
+  tend(c, SPM_K_PARTICLE, 8.0 * 60 * c->np);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+// phi and up at the anchors p_calc_f_hydro_obl built (positions unchanged
+// since)
+int p_make_phi_obl(spm_ctx *c, const double *sre) {
+  RCP(k_zero(c, c->phi, c->nfield));
+  for (int d = 0; d < 3; ++d) RCP(k_zero(c, c->up[d], c->nfield));
+  if (c->np == 0) return 0;
+  tbeg(c);
+  hipLaunchKernelGGL(make_phi_obl_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->pxint, c->pres, c->stencil, c->n_stencil, c->le_deg, sre, c->phi, c->up[0], c->up[1],
+                     c->up[2]);
+  const long npts = (long)c->grid.nx * c->grid.ny * c->grid.nz;
+  hipLaunchKernelGGL(phi_clamp_kernel, dim3(blocks(npts, 256)), dim3(256), 0, c->stream, c->grid, c->phi, c->up[0],
+                     c->up[1], c->up[2]);
+  tend(c, SPM_K_MAKE_PHI, 32.0 * (double)c->n_stencil * c->np + 64.0 * npts);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+// u += U2u_oblique(up - phi ucp)
+int p_f_add_obl(spm_ctx *c, double *const ucp[3]) {
+  const long npts = (long)c->grid.nx * c->grid.ny * c->grid.nz;
+  tbeg(c);
+  hipLaunchKernelGGL(f_add_obl_kernel, dim3(blocks(npts, 256)), dim3(256), 0, c->stream, c->grid, c->up[0], c->up[1],
+                     c->up[2], c->phi, ucp[0], ucp[1], ucp[2], c->u[0], c->u[1], c->u[2], c->le_deg);
+  tend(c, SPM_K_REALSPACE, 136.0 * npts);
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

@@ -1,6 +1,7 @@
 // shear.hip -- Lees-Edwards shear in oblique coordinates (SW_EQ ==
-// Shear_Navier_Stokes_Lees_Edwards), fluid only: Time_evolution_hydro_OBL
-// (sp_3d_ns.cxx:214-390) with no particles.
+// Shear_Navier_Stokes_Lees_Edwards): Time_evolution_hydro_OBL
+// (sp_3d_ns.cxx:214-390); the particle kernels it calls are in
+// particles.hip.
 //
 // The grid shears with the flow: degree_oblique (the strain since the last
 // remap) tilts the y axis, wavevectors keep their indices, and the metric
@@ -240,6 +241,31 @@ __global__ void remap_kernel(Grid g, const double *__restrict__ c0, const double
   u2[dst] = c2[src];
 }
 
+// U_oblique2u (fft_wrapper.h:265-325) in gather form: destination row
+// (id, j) takes source rows i = id - io(j) and i + sign(j - NY/2) of the
+// oblique field w, with the reference's weights, plus the mean shear flow
+// rate (j - NY/2).  48 B read, 24 B written per point.
+__global__ void oblique2u_kernel(Grid g, const double *__restrict__ w0, const double *__restrict__ w1,
+                                 const double *__restrict__ w2, double *u0, double *u1, double *u2, double deg,
+                                 double rate) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.nx * g.ny * g.nz) return;
+  const int k = (int)(t % g.nz), j = (int)((t / g.nz) % g.ny), id = (int)(t / ((long)g.nz * g.ny));
+  const int NX = g.nx, NY = g.ny;
+  double sign = j - NY / 2;
+  if (!(sign == 0)) sign = sign / fabs(sign);
+  const int io = (int)((int)(sign * deg * (j - NY / 2.)) * sign + sign);
+  const double alpha = (io - deg * (j - NY / 2.)) * sign;
+  const double beta = 1. - alpha;
+  const int i = ((id - io) % NX + NX) % NX;
+  const int ip = (int)fmod(i + sign + 2 * NX, NX);
+  const long s = ((long)i * NY + j) * g.nz_ + k, sp = ((long)ip * NY + j) * g.nz_ + k;
+  const long dst = ((long)id * NY + j) * g.nz_ + k;
+  u0[dst] = (beta * w0[s] + alpha * w0[sp]) + deg * (beta * w1[s] + alpha * w1[sp]) + rate * (j - NY / 2.);
+  u1[dst] = beta * w1[s] + alpha * w1[sp];
+  u2[dst] = beta * w2[s] + alpha * w2[sp];
+}
+
 // The x velocity that U_oblique2u (fft_wrapper.h:265-325) forms at grid
 // point (id, j, k) from the oblique field w0, w1: destination row (id, j)
 // interpolates source rows i = id - io(j) and i + sign(j - NY/2) with the
@@ -422,9 +448,19 @@ int le_ns_solver(spm_ctx *c) {
   return 0;
 }
 
-// Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390) with no particles: f is
-// zero, so U2u_oblique and Add_f_particle leave u unchanged.  The remap
-// reads a copy of u held in up[0..2].
+int le_oblique2u(spm_ctx *c, double *const ucp[3]) {
+  tbeg(c);
+  hipLaunchKernelGGL(oblique2u_kernel, dim3(nblk(npoints(c))), dim3(kBlock), 0, c->stream, c->grid, c->u[0], c->u[1],
+                     c->u[2], ucp[0], ucp[1], ucp[2], c->le_deg, c->le_rate);
+  tend(c, SPM_K_REALSPACE, 72.0 * npoints(c));
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+// Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390).  The remap reads a copy
+// of u held in up[0..2]; with particles the Cartesian velocity ucp lives in
+// f[0..2].  Without particles f is zero, so U2u_oblique and Add_f_particle
+// leave u unchanged and the particle block is skipped.
 int le_step(spm_ctx *c, int fix_mask) {
   const Grid &g = c->grid;
   RCS(le_ns_solver(c));
@@ -435,11 +471,11 @@ int le_step(spm_ctx *c, int fix_mask) {
   const double rate = c->le_rate;
   c->le_deg += rate * c->prm.dt;
   if (c->le_deg >= 1.) {
-    double *const ucp[3] = {c->up[0], c->up[1], c->up[2]};
+    double *const copy[3] = {c->up[0], c->up[1], c->up[2]};
     const size_t fb = sizeof(double) * c->nfield;
-    for (int d = 0; d < 3; ++d) SPM_HIP(c, hipMemcpyAsync(ucp[d], u3[d], fb, hipMemcpyDeviceToDevice, c->stream));
+    for (int d = 0; d < 3; ++d) SPM_HIP(c, hipMemcpyAsync(copy[d], u3[d], fb, hipMemcpyDeviceToDevice, c->stream));
     tbeg(c);
-    hipLaunchKernelGGL(remap_kernel, dim3(nblk(npoints(c))), dim3(kBlock), 0, c->stream, g, ucp[0], ucp[1], ucp[2],
+    hipLaunchKernelGGL(remap_kernel, dim3(nblk(npoints(c))), dim3(kBlock), 0, c->stream, g, copy[0], copy[1], copy[2],
                        u3[0], u3[1], u3[2]);
     tend(c, SPM_K_REALSPACE, 48.0 * npoints(c));
     SPM_LAUNCH_CHECK(c);
@@ -454,6 +490,18 @@ int le_step(spm_ctx *c, int fix_mask) {
                      ivolume * g.ny / (g.ny - 1), c->le_red + kRedBlocks);
   tend(c, SPM_K_REALSPACE, 32.0 * g.nx * g.nz);
   SPM_LAUNCH_CHECK(c);
+  if (c->np > 0) {  // sp_3d_ns.cxx:343-373
+    const double *sre = c->le_red + kRedBlocks;
+    double *const ucp[3] = {c->f[0], c->f[1], c->f[2]};
+    const int euler = c->ts == 0;
+    RCS(le_oblique2u(c, ucp));
+    RCS(p_position_obl(c, euler, sre));
+    RCS(p_calc_f_hydro_obl(c, ucp, sre));
+    RCS(p_velocity_obl(c, euler));
+    if (c->prm.kbt > 0.) RCS(host_thermostat(c));
+    RCS(p_make_phi_obl(c, sre));
+    RCS(p_f_add_obl(c, ucp));
+  }
   RCS(fft_r2c_many(c, u3, 3));
   return le_u_k2zeta_k(c);
 }

@@ -749,8 +749,17 @@ int set_electrolyte(spm_ctx *c, const spm_electrolyte *el) {
 // External_field.DC.Shear_rate (input.cxx:600-646); degree_oblique is a
 // zero-initialised global (input.cxx:271).
 int set_lees_edwards(spm_ctx *c, double shear_rate) {
-  if (!std::isfinite(shear_rate)) {
-    c->err = "invalid shear rate";
+  // the strain grows and is remapped at 1 (sp_3d_ns.cxx:280-305): a
+  // negative rate would take degree_oblique below 0, which the reference
+  // never remaps
+  if (!std::isfinite(shear_rate) || shear_rate < 0.0) {
+    c->err = "the shear rate must be finite and >= 0";
+    return SPM_ERR_INVALID;
+  }
+  // one remap per step keeps degree_oblique in [0, 1) only while the strain
+  // of a step stays below 1
+  if (!(shear_rate * c->prm.dt < 1.0)) {
+    c->err = "Shear_rate * dt must be below 1 (one grid remap per step)";
     return SPM_ERR_INVALID;
   }
   if (c->le || c->electro) {
@@ -759,10 +768,6 @@ int set_lees_edwards(spm_ctx *c, double shear_rate) {
   }
   if (c->prm.precision != SPM_FP64) {
     c->err = "the Lees-Edwards path runs in FP64 only";
-    return SPM_ERR_UNSUPPORTED;
-  }
-  if (c->prm.n_particles > 0) {
-    c->err = "the Lees-Edwards path runs the fluid without particles";
     return SPM_ERR_UNSUPPORTED;
   }
   if (!c->work[2]) RC(dalloc(c, &c->work[2], c->nfield));
@@ -777,6 +782,10 @@ int set_lees_edwards(spm_ctx *c, double shear_rate) {
 // degree_oblique lives on the host (it selects the remap), Shear_rate_eff on
 // the device (the step measures it and the next step's advection reads it).
 int put_oblique(spm_ctx *c, double deg, double sre) {
+  if (!(deg >= 0.0 && deg < 1.0) || !std::isfinite(sre)) {
+    c->err = "degree_oblique must lie in [0, 1) and Shear_rate_eff be finite";
+    return SPM_ERR_INVALID;
+  }
   SPM_HIP(c, hipMemcpyAsync(c->le_red + le_red_slots() - 2, &sre, sizeof(double), hipMemcpyHostToDevice, c->stream));
   SPM_HIP(c, hipStreamSynchronize(c->stream));
   c->le_deg = deg;
@@ -790,6 +799,20 @@ int get_oblique(spm_ctx *c, double *deg, double *sre) {
   return 0;
 }
 
+// particle hooks: the Cartesian velocity is u0..u2, Shear_rate_eff the
+// device value spm_put_oblique or the last step left
+const double *le_sre(const spm_ctx *c) { return c->le_red + le_red_slots() - 2; }
+int le_make_phi_api(spm_ctx *c) {
+  RC(p_build_tiles(c));
+  return p_make_phi_obl(c, le_sre(c));
+}
+int le_f_hydro_api(spm_ctx *c) {
+  double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
+  return p_calc_f_hydro_obl(c, u3, le_sre(c));
+}
+int le_position_api(spm_ctx *c) { return p_position_obl(c, c->ts == 0, le_sre(c)); }
+int le_velocity_api(spm_ctx *c) { return p_velocity_obl(c, c->ts == 0); }
+
 int le_zeta_k2u_k_api(spm_ctx *c) {
   double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
   return le_zeta2u_k(c, u3, 0);
@@ -801,6 +824,7 @@ thread_local std::string g_create_err;
 
 namespace spm {
 int electrolyte_step(spm_ctx *c) { return electrolyte_step_impl(c); }
+int host_thermostat(spm_ctx *c) { return thermostat(c); }
 }  // namespace spm
 
 // =====================================================================
@@ -1208,5 +1232,9 @@ int spm_get_oblique(spm_ctx *c, double *degree_oblique, double *shear_rate_eff) 
 int spm_ns_solver_slaved_euler_shear_obl(spm_ctx *c) { LE_ENTRY(le_ns_solver(c)); }
 int spm_zeta_k2u_k_obl(spm_ctx *c) { LE_ENTRY(le_zeta_k2u_k_api(c)); }
 int spm_u_k2zeta_k_obl(spm_ctx *c) { LE_ENTRY(le_u_k2zeta_k(c)); }
+int spm_make_phi_u_particle_obl(spm_ctx *c) { LE_ENTRY(le_make_phi_api(c)); }
+int spm_calc_f_hydro_obl(spm_ctx *c) { LE_ENTRY(le_f_hydro_api(c)); }
+int spm_md_solver_position_obl(spm_ctx *c) { LE_ENTRY(le_position_api(c)); }
+int spm_md_solver_velocity_obl(spm_ctx *c) { LE_ENTRY(le_velocity_api(c)); }
 
 }  // extern "C"

@@ -278,6 +278,16 @@ int p_calc_f_hydro(spm_ctx *c);
 int p_force(spm_ctx *c);
 int p_velocity(spm_ctx *c, int euler);
 int p_thermostat_apply(spm_ctx *c, const double *noise_dev);
+// spherical particles under Lees-Edwards shear; sre is Shear_rate_eff on the device
+int p_position_obl(spm_ctx *c, int euler, const double *sre);
+int p_calc_f_hydro_obl(spm_ctx *c, double *const ucp[3], const double *sre);
+int p_velocity_obl(spm_ctx *c, int euler);
+int p_make_phi_obl(spm_ctx *c, const double *sre);
+int p_f_add_obl(spm_ctx *c, double *const ucp[3]);
+// Add_random_force_thermostat: host noise draw + device update
+int host_thermostat(spm_ctx *c);
+// U_oblique2u of the real-space oblique velocity u into ucp (Cartesian)
+int le_oblique2u(spm_ctx *c, double *const ucp[3]);
 int p_build_tiles(spm_ctx *c);
 int p_make_phi(spm_ctx *c, int sw_up);
 int p_make_f(spm_ctx *c);

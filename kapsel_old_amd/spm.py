@@ -31,7 +31,8 @@ SYMBOLS = [
     "spm_put_total_solute", "spm_get_total_solute", "spm_nssolute_solver_euler", "spm_rescale_solute",
     "spm_make_coulomb_force", "spm_save_state", "spm_load_state",
     "spm_set_lees_edwards", "spm_put_oblique", "spm_get_oblique", "spm_ns_solver_slaved_euler_shear_obl",
-    "spm_zeta_k2u_k_obl", "spm_u_k2zeta_k_obl",
+    "spm_zeta_k2u_k_obl", "spm_u_k2zeta_k_obl", "spm_make_phi_u_particle_obl", "spm_calc_f_hydro_obl",
+    "spm_md_solver_position_obl", "spm_md_solver_velocity_obl",
 ]
 
 SCHEDULE_FUSED, SCHEDULE_LITERAL = 0, 1
@@ -97,7 +98,9 @@ def lib():
         L.spm_set_lees_edwards.argtypes = [vp, C.c_double]
         L.spm_put_oblique.argtypes = [vp, C.c_double, C.c_double]
         L.spm_get_oblique.argtypes = [vp, dp, dp]
-        for name in ["spm_ns_solver_slaved_euler_shear_obl", "spm_zeta_k2u_k_obl", "spm_u_k2zeta_k_obl"]:
+        for name in ["spm_ns_solver_slaved_euler_shear_obl", "spm_zeta_k2u_k_obl", "spm_u_k2zeta_k_obl",
+                     "spm_make_phi_u_particle_obl", "spm_calc_f_hydro_obl", "spm_md_solver_position_obl",
+                     "spm_md_solver_velocity_obl"]:
             getattr(L, name).argtypes = [vp]
         L.spm_put_total_solute.argtypes = [vp, dp]
         L.spm_get_total_solute.argtypes = [vp, dp]
@@ -197,7 +200,7 @@ class SPM:
         tt[:len(t)] = t
         self._call("put_total_solute", _dp(tt))
 
-    # ---- Lees-Edwards shear, fluid only (SW_EQ == Shear_Navier_Stokes_Lees_Edwards) ----
+    # ---- Lees-Edwards shear (SW_EQ == Shear_Navier_Stokes_Lees_Edwards) ----
     def set_lees_edwards(self, shear_rate):
         self._call("set_lees_edwards", C.c_double(shear_rate))
 
@@ -211,6 +214,10 @@ class SPM:
         return d.value, s.value
 
     def NS_solver_slavedEuler_Shear_OBL(self): self._call("ns_solver_slaved_euler_shear_obl")
+    def Make_phi_u_particle_OBL(self): self._call("make_phi_u_particle_obl")
+    def Calc_f_hydro_correct_precision_OBL(self): self._call("calc_f_hydro_obl")
+    def MD_solver_position_OBL(self): self._call("md_solver_position_obl")
+    def MD_solver_velocity_OBL(self): self._call("md_solver_velocity_obl")
     def Zeta_k2u_k_OBL(self): self._call("zeta_k2u_k_obl")
 
     def U_k2zeta_k_OBL(self):

@@ -1,8 +1,9 @@
-"""Parity of the Lees-Edwards path (SW_EQ == Shear_Navier_Stokes_Lees_Edwards,
-fluid only) of libspm.so with the CPU oracle: Zeta_k2u_k_OBL, the
-projection back to zeta, NS_solver_slavedEuler_Shear_OBL at a tilted grid
-with a nonzero Shear_rate_eff, and whole Time_evolution_hydro_OBL steps
-across a remap (degree_oblique reaching 1), on boxes with NX != NY.
+"""Parity of the Lees-Edwards path (SW_EQ == Shear_Navier_Stokes_Lees_Edwards)
+of libspm.so with the CPU oracle: Zeta_k2u_k_OBL, the projection back to
+zeta, NS_solver_slavedEuler_Shear_OBL at a tilted grid with a nonzero
+Shear_rate_eff, the particle routines with particles straddling the y
+boundaries, and whole Time_evolution_hydro_OBL steps across a remap
+(degree_oblique reaching 1), on boxes with NX != NY.
 Tolerance 1e-10 relative, as for the rest of the FP64 path; degree_oblique
 is host arithmetic and must agree exactly.
 
@@ -173,16 +174,23 @@ def test_lees_edwards_argument_errors(SPM):
     from kapsel_old_amd.spm import SpmError
     g = SPM(abi.make_params(32, a=2.0, n_particles=1))
     with pytest.raises(SpmError):
-        g.set_lees_edwards(0.1)  # particles are not supported
-    with pytest.raises(SpmError):
         g.put_oblique(0.1, 0.1)  # before spm_set_lees_edwards
+    with pytest.raises(SpmError):
+        g.Make_phi_u_particle_OBL()
     g = SPM(abi.make_params(16, precision=abi.FP32))
     with pytest.raises(SpmError):
         g.set_lees_edwards(0.1)  # FP64 only
     g = SPM(abi.make_params(16))
+    with pytest.raises(SpmError):
+        g.set_lees_edwards(-0.1)  # the strain is remapped only upward
+    with pytest.raises(SpmError):
+        g.set_lees_edwards(1.0 / g.params.dt)  # a strain of 1 per step
     g.set_lees_edwards(0.1)
     with pytest.raises(SpmError):
         g.set_lees_edwards(0.1)  # once
+    for deg in (-0.1, 1.0, float("nan")):
+        with pytest.raises(SpmError):
+            g.put_oblique(deg, 0.1)
 
 
 def test_state_file_round_trip(SPM, tmp_path):
@@ -228,7 +236,7 @@ def _driver_input(tmp_path, num_snap, name):
         assert src.count(old) == 1, old
         src = src.replace(old, new)
     head, tail = src.split("{//Shear_Navier_Stokes_Lees_Edwards")
-    tail = tail.replace("1.00000000000000e-003", "2.00000000000000e+001", 1)  # DC Shear_rate 20
+    tail = tail.replace("1.00000000000000e-003", "4.00000000000000", 1)  # DC Shear_rate 4: 0.29 per step
     src = head + "{//Shear_Navier_Stokes_Lees_Edwards" + tail
     out = tmp_path / name
     out.write_text(src)
@@ -247,7 +255,7 @@ def test_driver_lees_edwards(tmp_path):
     assert main(["-I" + full, "-O" + d + "/o.udf", "-R" + d + "/r.udf", "--quiet"]) == 0
     with udf.UDF(full) as u:
         prm, _, run = u.params()
-    assert run.shear_rate == 20.0
+    assert run.shear_rate == 4.0 and 0.2 < run.shear_rate * prm.dt < 0.5
     want = 0.0
     for _ in range(7):  # ts = 0..6
         want += run.shear_rate * prm.dt
@@ -266,3 +274,121 @@ def test_driver_lees_edwards(tmp_path):
     with udf.UDF(d + "/rb.udf") as u:
         rb = udf.read_restart(u, (32, 32, 32), 0)
     assert rb["ts"] == 6 and rb["degree_oblique"] == want and rb["time"] == r["time"]
+
+
+# ---- spherical particles under shear --------------------------------------
+
+def _le_particles(n, box, a, seed, near_y, deg):
+    """Non-overlapping particles, the first ones straddling the y boundaries;
+    particle 1 sits inside the WCA range of particle 0's image across the
+    y boundary (x shifted by deg LY there)."""
+    from tests.helpers import particles
+    rng = np.random.default_rng(seed)
+    L = np.array(box, dtype=float)
+    x0 = rng.uniform(0, 1, 3) * L
+    x0[1] = 0.4 * a
+    off = np.array([0.3 * a, -2.15 * a, 0.2 * a])  # below y = 0: the copy of the box one down
+    pos = [x0, (x0 + off + np.array([deg * L[1], L[1], 0.0])) % L]
+    for k in range(2, n):
+        for _ in range(1000):
+            x = rng.uniform(0, 1, 3) * np.array(box, dtype=float)
+            if k < near_y:  # within a radius of y = 0 or y = LY
+                x[1] = (rng.uniform(-a, a)) % box[1]
+            ok = True
+            for y in pos:
+                d = np.abs(x - y)
+                d = np.minimum(d, np.array(box) - d)
+                ok = ok and np.linalg.norm(d) > 2.3 * a
+            if ok:
+                pos.append(x)
+                break
+    return particles(n, box, a, seed=seed, vel=0.02, omega=0.004, positions=np.array(pos))
+
+
+def _le_pair(SPM, grid, np_, a, deg, sre, rate=0.3, seed=11, near_y=3, **kw):
+    prm = abi.make_params(*grid, a=a, n_particles=np_, rotation=1, dt=0.05, fix_cell=(0, 0, 0),
+                          lj_truncate=abi.LJ_WCA, **kw)
+    p = _le_particles(np_, grid, a, seed, near_y, deg)
+    objs = [Oracle(prm), SPM(prm)]
+    for o in objs:
+        o.set_lees_edwards(rate)
+        o.put_oblique(deg, sre)
+        o.put_particles(p)
+    return objs, p
+
+
+def _cmp_particles(objs, keys, tag, tol=TOL):
+    qa, qb = objs[0].get_particles(), objs[1].get_particles()
+    for k in keys:
+        e = rel(qb[k], qa[k])
+        print("%s %s rel err %.3e" % (tag, k, e), flush=True)
+        assert e < tol, (tag, k, e)
+
+
+@pytest.mark.parametrize("grid", [(64, 32, 32), (32, 64, 32)])
+def test_particle_routines_obl(SPM, grid):
+    objs, p = _le_pair(SPM, grid, 8, 3.0, deg=0.37, sre=0.23)
+    rng = np.random.default_rng(3)
+    u = [rng.standard_normal(objs[0].shape) * 0.05 for _ in range(3)]
+    for o in objs:
+        for d in range(3):
+            o.put_field("u%d" % d, u[d])
+        o.Make_phi_u_particle_OBL()
+    for f in ("phi", "up0", "up1", "up2"):
+        check("Make_phi_u_particle_OBL %s %s" % (grid, f), objs[1].get_field(f), objs[0].get_field(f))
+    # the stencils do reach across the y boundary
+    assert objs[1].get_field("phi")[:, 0, :].any() and objs[1].get_field("phi")[:, -1, :].any()
+    for o in objs:
+        o.Calc_f_hydro_correct_precision_OBL()
+    _cmp_particles(objs, ("f_hydro", "torque_hydro"), "Calc_f_hydro_OBL")
+    for o in objs:
+        o.MD_solver_velocity_OBL()
+    _cmp_particles(objs, ("v", "omega", "v_old", "fr_previous", "f_hydro_previous"), "MD_solver_velocity_OBL")
+    assert np.abs(objs[1].get_particles()["fr_previous"][:2]).max() > 0  # the pair across the boundary
+    # velocities that carry some particles through the y boundaries
+    q = objs[0].get_particles()
+    y = q["x"][:, 1]
+    q["v"][:, 1] = np.where(y < grid[1] / 2, -(y + 0.5), grid[1] - y + 0.5) / 0.05 * (np.arange(len(q)) < 3)
+    for o in objs:
+        o.put_particles(q)
+        o.MD_solver_position_OBL()
+    _cmp_particles(objs, ("x", "x_nopbc", "v", "v_old"), "MD_solver_position_OBL", tol=1e-13)
+    qa, qb = objs[0].get_particles(), objs[1].get_particles()
+    for f in ("s", "v"):
+        assert rel(qb["q"][f], qa["q"][f]) < 1e-13, f
+    assert np.abs(qa["x"][:3, 1] - q["x"][:3, 1]).max() > grid[1] / 2  # three went through
+
+
+@pytest.mark.parametrize("kbt", [0.0, 1e-4])
+def test_steps_with_particles_across_a_remap(SPM, kbt):
+    grid = (32, 32, 32)
+    objs, p = _le_pair(SPM, grid, 6, 3.0, deg=0.96, sre=0.3, rate=0.3, kbt=kbt)
+    for s in range(5):
+        for o in objs:
+            o.step(1)
+        compare_state(objs, "particles step %d" % s)
+        _cmp_particles(objs, ("x", "v", "omega"), "particles step %d" % s)
+    assert objs[0].get_oblique()[0] == objs[1].get_oblique()[0] < 0.5  # remapped
+
+
+def test_steps_with_particles_64cube(SPM):
+    objs, p = _le_pair(SPM, (64, 64, 64), 20, 4.0, deg=0.5, sre=0.1, rate=0.1, near_y=5, kbt=1e-4)
+    for o in objs:
+        o.step(3)
+    compare_state(objs, "64^3 particles")
+    _cmp_particles(objs, ("x", "v", "omega"), "64^3 particles")
+
+
+def test_sphere_in_shear_rotates_at_half_the_vorticity_64(SPM):
+    """Jeffery: a free sphere on the zero-velocity plane of u_x = rate (y - LY/2)
+    spins at omega_z = -rate/2 and stays in place."""
+    from tests.helpers import particles
+    n, a, rate = 64, 5.0, 0.01
+    g = SPM(abi.make_params(n, a=a, n_particles=1, rotation=1, dt=0.1, fix_cell=(0, 0, 0)))
+    g.set_lees_edwards(rate)
+    g.put_particles(particles(1, (n, n, n), a, positions=np.array([[32.0, 32.0, 32.0]])))
+    g.step(60)
+    q = g.get_particles()[0]
+    print("omega_z / rate %.5f" % (q["omega"][2] / rate), flush=True)
+    assert abs(q["omega"][2] / rate + 0.5) < 0.01
+    assert np.abs(q["v"]).max() < 1e-10 * rate
