@@ -12,6 +12,9 @@ Time_evolution_hydro.
 Prints one JSON line: time-steps/s (whole job), the roofline of the dominant
 kernel family measured with HIP events on the library's stream, and the CPU
 oracle timed on a bounded sample of the same workload on this host.
+
+--shear-rate R > 0 runs the same box under Lees-Edwards shear instead
+(Time_evolution_hydro_OBL, Shear_rate R, literal FP64 schedule).
 """
 import argparse
 import json
@@ -53,12 +56,14 @@ def algorithmic_step_bytes(n, np_, stencil, w=8, n_fft=20):
     return n_fft * (8 * nn + 16 * n * n * (n // 2 + 1)) * (w / 8) + np_ * stencil * 11 * w
 
 
-def gpu_run(n, np_, steps, warmup, profile_steps, schedule, precision):
+def gpu_run(n, np_, steps, warmup, profile_steps, schedule, precision, shear_rate=0.0):
     from kapsel_old_amd.spm import SPM
     g = SPM(workload_params(n, np_, precision))
     g.set_schedule(schedule)
     g.put_particles(workload_particles(n, np_))
     g.init_fluid()
+    if shear_rate > 0:
+        g.set_lees_edwards(shear_rate)
     g.step(warmup)
     g.synchronize()
     print("bench: warmup done (%d steps)" % warmup, file=sys.stderr, flush=True)
@@ -120,13 +125,15 @@ def load_traffic(schedule, precision, n, np_):
     return out, os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(n, np_, max_steps=2, budget_s=40.0):
+def cpu_baseline(n, np_, max_steps=2, budget_s=40.0, shear_rate=0.0):
     """The CPU oracle (single-threaded FP64 restatement of the reference
     step) on the same workload: init untimed, then up to max_steps steps."""
     from oracle.oracle import Oracle
     o = Oracle(workload_params(n, np_))
     o.put_particles(workload_particles(n, np_))
     o.init_fluid()
+    if shear_rate > 0:
+        o.set_lees_edwards(shear_rate)
     done = 0
     t0 = time.perf_counter()
     while done < max_steps:
@@ -152,6 +159,7 @@ def main():
     ap.add_argument("--schedule", choices=["fused", "literal"], default="fused")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB")
+    ap.add_argument("--shear-rate", type=float, default=0.0)
     args = ap.parse_args()
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
@@ -161,10 +169,13 @@ def main():
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
     n, np_ = args.grid, args.particles
+    sheared = args.shear_rate > 0
+    if sheared:  # the sheared step has one schedule and runs in FP64
+        args.schedule, args.precision = "literal", "fp64"
     schedule = 0 if args.schedule == "fused" else 1
     precision = abi.FP64 if args.precision == "fp64" else abi.FP32
     elapsed, prof_elapsed, ms, by, cnt, stencil = gpu_run(n, np_, args.steps, args.warmup, args.profile_steps,
-                                                          schedule, precision)
+                                                          schedule, precision, args.shear_rate)
     wbytes = 8 if precision == abi.FP64 else 4
     value = args.steps / elapsed
     dom = int(np.argmax(ms))
@@ -172,14 +183,14 @@ def main():
     per_launch_bytes = by[dom] / max(cnt[dom], 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
     b_step = algorithmic_step_bytes(n, np_, stencil, w=wbytes)
-    traffic, traffic_src = load_traffic(args.schedule, args.precision, n, np_)
+    traffic, traffic_src = (None, None) if sheared else load_traffic(args.schedule, args.precision, n, np_)
     fam_traffic = traffic.get(FAMILIES[dom]) if traffic else None
     families = {FAMILIES[f]: {"ms_per_step": ms[f] / args.profile_steps, "launches_per_step":
                               cnt[f] / args.profile_steps,
                               "GBps": (by[f] / (ms[f] * 1e-3) / 1e9) if ms[f] > 0 else None}
                 for f in range(len(FAMILIES))}
     out = {
-        "metric": "time-steps/s at 256^3 w/ 4000 spheres",
+        "metric": "time-steps/s at 256^3 w/ 4000 spheres" + (" under Lees-Edwards shear" if sheared else ""),
         "value": value,
         "unit": "time-steps/s",
         "n_gpus": 1,
@@ -191,8 +202,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if precision == abi.FP64 else "f32 fields, f64 particles",
         "data": "synthetic",
-        "config": {"workload": "SPM time step (Time_evolution_hydro), %d^3 grid, %d spheres a=5 xi=2, "
-                               "WCA, kBT=0.1, dt=0.05, %s fields" % (n, np_, args.precision.upper()),
+        "config": {"workload": "SPM time step (%s), %d^3 grid, %d spheres a=5 xi=2, "
+                               "WCA, kBT=0.1, dt=0.05, %s fields%s"
+                               % ("Time_evolution_hydro_OBL" if sheared else "Time_evolution_hydro", n, np_,
+                                  args.precision.upper(),
+                                  (", Lees-Edwards Shear_rate %g" % args.shear_rate) if sheared else ""),
                    "grid": [n, n, n], "n_particles": np_, "stencil_points": stencil, "precision": args.precision,
                    "schedule": args.schedule},
         "hbm_TBps_algorithmic": b_step * value / 1e12,
@@ -208,7 +222,7 @@ def main():
         "profiled_ms_per_step": 1e3 * prof_elapsed / args.profile_steps,
     }
     if not args.no_cpu_baseline and precision == abi.FP64:
-        v, done, secs = cpu_baseline(n, np_)
+        v, done, secs = cpu_baseline(n, np_, shear_rate=args.shear_rate)
         out["cpu_baseline"] = {"value": v, "unit": "time-steps/s", "cores": 1, "kind": "port",
                                "sample": "%d step(s) of the same %d^3/%d workload after an untimed init, "
                                          "%.1f s, oracle/liboracle.so single-threaded" % (done, n, np_, secs)}

@@ -83,8 +83,7 @@ typedef struct spm_udf_run {
  * Set_global_parameters (input.cxx:316-381).  `el` may be NULL for
  * Navier_Stokes input; for Electrolyte input it is filled.
  * Shear_Navier_Stokes_Lees_Edwards input gives run->shear_rate (its AC
- * branch reads Shear_rate as a steady shear, input.cxx:646-653); the
- * library runs it without particles.  Rejects (SPM_ERR_UNSUPPORTED) the
+ * branch reads Shear_rate as a steady shear, input.cxx:646-653).  Rejects (SPM_ERR_UNSUPPORTED) the
  * equations, object types and switches the library does not run:
  * Shear_Navier_Stokes (sheared walls), chains, rigid bodies, janus
  * particles, pinning. */

@@ -244,7 +244,8 @@ __global__ void remap_kernel(Grid g, const double *__restrict__ c0, const double
 // U_oblique2u (fft_wrapper.h:265-325) in gather form: destination row
 // (id, j) takes source rows i = id - io(j) and i + sign(j - NY/2) of the
 // oblique field w, with the reference's weights, plus the mean shear flow
-// rate (j - NY/2).  48 B read, 24 B written per point.
+// rate (j - NY/2).  Each source row serves two destination rows: 24 B read,
+// 24 B written per point.
 __global__ void oblique2u_kernel(Grid g, const double *__restrict__ w0, const double *__restrict__ w1,
                                  const double *__restrict__ w2, double *u0, double *u1, double *u2, double deg,
                                  double rate) {
@@ -452,7 +453,7 @@ int le_oblique2u(spm_ctx *c, double *const ucp[3]) {
   tbeg(c);
   hipLaunchKernelGGL(oblique2u_kernel, dim3(nblk(npoints(c))), dim3(kBlock), 0, c->stream, c->grid, c->u[0], c->u[1],
                      c->u[2], ucp[0], ucp[1], ucp[2], c->le_deg, c->le_rate);
-  tend(c, SPM_K_REALSPACE, 72.0 * npoints(c));
+  tend(c, SPM_K_REALSPACE, 48.0 * npoints(c));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

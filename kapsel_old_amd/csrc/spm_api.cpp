@@ -1210,7 +1210,7 @@ int spm_get_total_solute(spm_ctx *c, double *total) {
   return SPM_OK;
 }
 
-// ---- Lees-Edwards shear, fluid only ----
+// ---- Lees-Edwards shear ----
 #define LE_ENTRY(body)                                       \
   do {                                                       \
     if (!c) return SPM_ERR_INVALID;                          \

@@ -1,8 +1,8 @@
 """Command-line driver: a run from a UDF input, as the reference's main()
 (sp_3d_ns.cxx:392-640) for constitutive_eq Navier_Stokes / Electrolyte
 with spherical particles placed by INIT_distribution user_specify,
-uniform_random or BCC, and Shear_Navier_Stokes_Lees_Edwards without
-particles (Time_evolution_hydro_OBL).
+uniform_random or BCC, and Shear_Navier_Stokes_Lees_Edwards
+(Time_evolution_hydro_OBL) with or without particles.
 
     python -m kapsel_old_amd.run -I input.udf -O output.udf -R restart.udf [-D define.udf]
 
@@ -103,8 +103,6 @@ def main(argv=None):
     if el is not None and run.poisson_boltzmann:
         raise SystemExit("INIT_profile Poisson_Boltzmann is not run by this driver; use Uniform")
     sheared = run.equation == udf.LEES_EDWARDS
-    if sheared and prm.n_particles > 0:
-        raise SystemExit("Shear_Navier_Stokes_Lees_Edwards is run without particles (Particle_number 0)")
     if a.fp32:
         prm.precision = abi.FP32
     gts, mstep = run.gts, run.gts * run.num_snap
