@@ -405,6 +405,7 @@ int setup(spm_ctx *c) {
     a_r_cutoff = std::min(nmax * p.dx * .5 / m.lj_dia, 2.5);
   }
   m.lj_cutoff = a_r_cutoff * m.lj_dia;
+  c->a_r_cutoff = a_r_cutoff;
   m.lj_on = (p.lj_truncate >= 0 && m.lj_cutoff > 0.0) ? 1 : 0;
   m.epsilon = p.epsilon;
   m.lj_powers = p.lj_powers;
@@ -772,6 +773,16 @@ int set_lees_edwards(spm_ctx *c, double shear_rate) {
   }
   if (!c->work[2]) RC(dalloc(c, &c->work[2], c->nfield));
   RC(dalloc(c, &c->le_red, (size_t)le_red_slots()));  // zeroes Shear_rate_eff
+  // under shear the LJ diameter is min((SIGMA + XI) / 2^(1/p), SIGMA) with
+  // the LJ_powers exponent p (input.cxx:389-403); R_cutoff = A_R_cutoff LJ_dia
+  {
+    PModel &m = c->pm;
+    const double sigma = m.lj_dia, xi = c->prm.a_xi * c->prm.dx;
+    const double dummy_pow = c->prm.lj_powers == 0 ? std::pow(2., 1. / 6.)
+                             : c->prm.lj_powers == 1 ? std::pow(2., 1. / 12.) : std::pow(2., 1. / 18.);
+    m.lj_dia = std::min((sigma + xi) / dummy_pow, sigma);
+    m.lj_cutoff = c->a_r_cutoff * m.lj_dia;
+  }
   SPM_HIP(c, hipStreamSynchronize(c->stream));
   c->le_rate = shear_rate;
   c->le_deg = 0.0;

@@ -186,6 +186,7 @@ struct spm_ctx {
   // the reduction partials of le_red)
   bool le = false;
   double le_rate = 0.0, le_deg = 0.0;
+  double a_r_cutoff = 0.0;  // A_R_cutoff (input.cxx:1461-1480), lj_cutoff = a_r_cutoff * lj_dia
   double *le_red = nullptr;
   bool electro = false;
   spm_electrolyte el{};

@@ -2056,6 +2056,12 @@ int oracle_set_lees_edwards(oracle_ctx *c, double shear_rate) {
   }
   c->wv1 = (double *)calloc(c->nfield, sizeof(double));
   if (!c->wv1) return -1;
+  /* LJ_dia under shear, input.cxx:389-403 */
+  {
+    const double dummy_pow = c->P.lj_powers == 0 ? pow(2., 1. / 6.) : c->P.lj_powers == 1 ? pow(2., 1. / 12.)
+                                                                                          : pow(2., 1. / 18.);
+    c->LJ_dia = DMIN((c->SIGMA + c->XI) / dummy_pow, c->SIGMA);
+  }
   c->le = 1;
   c->shear_rate = shear_rate;
   c->shear_rate_eff = 0.0;

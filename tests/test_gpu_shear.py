@@ -290,7 +290,7 @@ def _le_particles(n, box, a, seed, near_y, deg):
     L = np.array(box, dtype=float)
     x0 = rng.uniform(0, 1, 3) * L
     x0[1] = 0.4 * a
-    off = np.array([0.3 * a, -2.15 * a, 0.2 * a])  # below y = 0: the copy of the box one down
+    off = np.array([0.3 * a, -2.0 * a, 0.2 * a])  # below y = 0: the copy of the box one down
     pos = [x0, (x0 + off + np.array([deg * L[1], L[1], 0.0])) % L]
     for k in range(2, n):
         for _ in range(1000):
@@ -328,9 +328,11 @@ def _cmp_particles(objs, keys, tag, tol=TOL):
         assert e < tol, (tag, k, e)
 
 
-@pytest.mark.parametrize("grid", [(64, 32, 32), (32, 64, 32)])
-def test_particle_routines_obl(SPM, grid):
-    objs, p = _le_pair(SPM, grid, 8, 3.0, deg=0.37, sre=0.23)
+@pytest.mark.parametrize("grid,a,a_xi", [((64, 32, 32), 3.0, 2.0), ((32, 64, 32), 3.0, 2.0),
+                                         ((32, 32, 32), 4.0, 0.5)])
+def test_particle_routines_obl(SPM, grid, a, a_xi):
+    # a_xi = 0.5 < 0.245 a: the sheared LJ diameter (SIGMA + XI) / 2^(1/6) < SIGMA
+    objs, p = _le_pair(SPM, grid, 8 if a < 4 else 5, a, deg=0.37, sre=0.23, a_xi=a_xi)
     rng = np.random.default_rng(3)
     u = [rng.standard_normal(objs[0].shape) * 0.05 for _ in range(3)]
     for o in objs:

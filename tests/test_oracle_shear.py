@@ -420,3 +420,23 @@ def test_sphere_in_shear_rotates_at_half_the_vorticity():
     q = o.get_particles()[0]
     assert abs(q["omega"][2] / rate + 0.5) < 0.01, q["omega"]
     assert np.abs(q["omega"][:2]).max() < 1e-12 * rate and np.abs(q["v"]).max() < 1e-12 * rate
+
+
+@pytest.mark.parametrize("a_xi,want_sigma", [(0.5, (8.0 + 0.5) / 2 ** (1 / 6)), (2.0, 8.0)])
+def test_sheared_lj_diameter(a_xi, want_sigma):
+    """Under shear the LJ diameter is min((SIGMA + XI) / 2^(1/6), SIGMA)
+    (12:6 powers): the WCA force of one pair against the closed form
+    24 eps / r^2 (2 (s/r)^12 - (s/r)^6) r_vec with that s, cut at 2^(1/6) s."""
+    n, a = 32, 4.0
+    o = Oracle(abi.make_params(n, a=a, a_xi=a_xi, n_particles=2, rotation=0, lj_truncate=abi.LJ_WCA,
+                               epsilon=0.7))
+    o.set_lees_edwards(0.0)
+    for r in (0.97 * want_sigma, 1.05 * want_sigma, 1.13 * want_sigma):
+        x = np.array([[10.0, 12.0, 9.0], [10.0 + r * 0.6, 12.0 + r * 0.8, 9.0]])
+        o.put_particles(particles(2, (n, n, n), a, positions=x))
+        o.Force_OBL()
+        fr = o.get_particles()["fr"]
+        s6 = (want_sigma / r) ** 6
+        mag = 24 * 0.7 / r ** 2 * (2 * s6 * s6 - s6) if r < 2 ** (1 / 6) * want_sigma else 0.0
+        np.testing.assert_allclose(fr[0], -mag * r * np.array([0.6, 0.8, 0.0]), rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(fr[1], -fr[0], rtol=0, atol=0)
