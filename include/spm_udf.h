@@ -103,10 +103,23 @@ int spm_udf_user_particles(const spm_udf *udf, spm_particle *p, int n);
  * one under periodic images) and BCC (the reference's lattice) place the
  * particles; INIT_orientation random with ROTATION ON draws q by
  * RA_on_sphere4D.  Positions and orientations come from glibc's rand()
- * sequence from its default state, as in the reference.  n is the
- * Particle_number total.  FCC and random_walk (lattice relaxed by random
- * walks and steepest descent on the LJ force) are SPM_ERR_UNSUPPORTED. */
+ * sequence from its default state, as in the reference.  FCC and
+ * random_walk (init_particle.cxx:50-224) place an FCC lattice, then move
+ * every particle once by a random step of 1e-3 RADIUS and once down its LJ
+ * force by 0.05 RADIUS (Random_Walk / Steepest_descent, fluct.h:66-120,
+ * with the linked-cell LJ of md_force.cxx:23-134, whose double count of
+ * pairs under 3 cells per axis is kept because it decides the positions);
+ * random_walk repeats the pair random_walk.iteration times with a step of
+ * 0.025 RADIUS.  The random steps come from MT19937 seeded with 0 (the
+ * reference's MT_seed(GIVEN_SEED, 0) precedes Init_Particle).  Boxes under
+ * 16 points per axis are SPM_ERR_UNSUPPORTED for these two.  n is the
+ * Particle_number total. */
 int spm_udf_init_particles(const spm_udf *udf, spm_particle *p, int n);
+/* As spm_udf_init_particles, and the MT19937 state the initialisation
+ * leaves (the seed-0 state unless FCC or random_walk drew from it): the
+ * reference's thermostat goes on from it, so a caller hands it to
+ * spm_put_rng. */
+int spm_udf_init_particles_rng(const spm_udf *udf, spm_particle *p, int n, uint32_t mt[SPM_MT_N], int *mti);
 
 /* Restart data (resume.CONTINUE.Saved_Data of Udf/define.udf).  The caller
  * sets the sizes and owns the buffers: zeta0/zeta1 hold nx*ny*2*(nz/2+1)

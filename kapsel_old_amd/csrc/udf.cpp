@@ -11,6 +11,8 @@
 // member that is itself a record (the bodies of a select's alternatives, as
 // in constitutive_eq) appears in braces in schema order whether chosen or
 // not.  Comments run from // to the end of the line.
+#include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -953,8 +955,232 @@ void random_rqtn(GlibcRand &g, double q[4]) {
 
 int nint(double a) { return a >= 0.0 ? int(a + .5) : int(a - .5); }  // Nint (macro.h:38-46)
 
-// Init_Particle (init_particle.cxx:12-393) for spherical particles.
-int init_particles(const spm_udf *u, spm_particle *p, int n, bool user_only) {
+// MT19937 (mt19937ar.c: init_genrand, genrand_int32, genrand_real3) and
+// Gauss2 (fluct.h:48-60), the generator the reference seeds with
+// MT_seed(GIVEN_SEED, 0) (sp_3d_ns.cxx:477) before Init_Particle; FCC and
+// random_walk draw their walks from it and the thermostat goes on from the
+// state they leave.
+class Mt {
+ public:
+  explicit Mt(uint32_t s) {
+    mt_[0] = s;
+    for (mti_ = 1; mti_ < kN; ++mti_) mt_[mti_] = 1812433253U * (mt_[mti_ - 1] ^ (mt_[mti_ - 1] >> 30)) + mti_;
+  }
+  uint32_t next() {
+    static const uint32_t mag01[2] = {0x0U, 0x9908b0dfU};
+    uint32_t y;
+    if (mti_ >= kN) {
+      int kk;
+      for (kk = 0; kk < kN - kM; kk++) {
+        y = (mt_[kk] & 0x80000000U) | (mt_[kk + 1] & 0x7fffffffU);
+        mt_[kk] = mt_[kk + kM] ^ (y >> 1) ^ mag01[y & 1U];
+      }
+      for (; kk < kN - 1; kk++) {
+        y = (mt_[kk] & 0x80000000U) | (mt_[kk + 1] & 0x7fffffffU);
+        mt_[kk] = mt_[kk + (kM - kN)] ^ (y >> 1) ^ mag01[y & 1U];
+      }
+      y = (mt_[kN - 1] & 0x80000000U) | (mt_[0] & 0x7fffffffU);
+      mt_[kN - 1] = mt_[kM - 1] ^ (y >> 1) ^ mag01[y & 1U];
+      mti_ = 0;
+    }
+    y = mt_[mti_++];
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680U;
+    y ^= (y << 15) & 0xefc60000U;
+    y ^= (y >> 18);
+    return y;
+  }
+  double real3() { return (((double)next()) + 0.5) * (1.0 / 4294967296.0); }
+  void gauss2(double *out) {  // Marsaglia's polar method
+    double x1, x2, rsq;
+    do {
+      x1 = 2. * real3() - 1.;
+      x2 = 2. * real3() - 1.;
+      rsq = x1 * x1 + x2 * x2;
+    } while (rsq >= 1.0 || rsq == 0.0);
+    const double factor = std::sqrt(-2. * std::log(rsq) / rsq);
+    out[0] = factor * x1;
+    out[1] = factor * x2;
+  }
+  void state(uint32_t *mt, int *mti) const {
+    std::memcpy(mt, mt_, sizeof(mt_));
+    *mti = mti_;
+  }
+
+ private:
+  static constexpr int kN = SPM_MT_N, kM = 397;
+  uint32_t mt_[SPM_MT_N];
+  int mti_;
+};
+
+// The LJ model of the initial relaxation: Lennard_Jones_f
+// (interaction.h:156-181) with LJ_dia and the cutoff A_R_cutoff LJ_dia.
+struct InitLj {
+  int powers;
+  double epsilon, dia, cutoff;
+  double f(double x) const {
+    double dmy = dia / x;
+    double coeff = 24. * epsilon;
+    if (powers == 0) {
+      dmy = dmy * dmy * (dmy * dmy) * (dmy * dmy);
+    } else if (powers == 1) {
+      const double d2 = dmy * dmy;
+      dmy = d2 * d2 * d2 * d2 * d2 * d2;
+      coeff = 48. * epsilon;
+    } else {
+      const double d2 = dmy * dmy;
+      dmy = d2 * d2 * d2 * d2 * d2 * d2 * d2 * d2 * d2;
+      coeff = 72. * epsilon;
+    }
+    return coeff / (x * x) * (2.0 * (dmy * dmy) - dmy);
+  }
+};
+
+// Calc_f_Lennard_Jones_shear_cap_primitive_lnk (md_force.cxx:23-134) with
+// Distance0 and no cap, as Force_random_walk calls it: cells of 16 grid
+// spacings, the 27 neighbours of each cell with periodic wrap, pairs i > j
+// in the order the cell lists hold them.  Where fewer than 3 cells span an
+// axis the reference visits a neighbour cell twice and counts those pairs
+// twice; so does this, since it decides where the particles start.
+void init_lj_lnk(std::vector<spm_particle> &p, const int N[3], double dx, const double L[3], const InitLj &lj) {
+  const int cell_length = 16;
+  const int lc[3] = {N[0] / cell_length, N[1] / cell_length, N[2] / cell_length};
+  const double lc_r = cell_length * dx;
+  const int lcyz = lc[1] * lc[2], lcxyz = lc[0] * lcyz;
+  const int n = (int)p.size();
+  std::vector<int> head((size_t)lcxyz, -1), lscl((size_t)std::max(n, 1));
+  for (int i = 0; i < n; ++i) {
+    int mc[3];
+    for (int d = 0; d < 3; ++d) mc[d] = int(p[i].x[d] / lc_r);
+    const int cn = mc[0] * lcyz + mc[1] * lc[2] + mc[2];
+    lscl[i] = head[cn];
+    head[cn] = i;
+  }
+  int ic[3], icl[3], icls[3];
+  for (ic[0] = 0; ic[0] < lc[0]; ic[0]++)
+    for (ic[1] = 0; ic[1] < lc[1]; ic[1]++)
+      for (ic[2] = 0; ic[2] < lc[2]; ic[2]++) {
+        const int cn = ic[0] * lcyz + ic[1] * lc[2] + ic[2];
+        for (icl[0] = ic[0] - 1; icl[0] <= ic[0] + 1; icl[0]++)
+          for (icl[1] = ic[1] - 1; icl[1] <= ic[1] + 1; icl[1]++)
+            for (icl[2] = ic[2] - 1; icl[2] <= ic[2] + 1; icl[2]++) {
+              for (int d = 0; d < 3; ++d) {
+                icls[d] = icl[d];
+                if (icls[d] < 0) icls[d] = lc[d] - 1;
+                if (icls[d] > lc[d] - 1) icls[d] = 0;
+              }
+              const int cl = ((icls[0] + lc[0]) % lc[0]) * lcyz + ((icls[1] + lc[1]) % lc[1]) * lc[2] +
+                             ((icls[2] + lc[2]) % lc[2]);
+              for (int i = head[cn]; i != -1; i = lscl[i])
+                for (int j = head[cl]; j != -1; j = lscl[j]) {
+                  if (!(i > j)) continue;
+                  double x12[3], r2 = 0.0;
+                  for (int d = 0; d < 3; ++d) {  // Distance0 (interaction.h:110-123)
+                    x12[d] = p[j].x[d] - p[i].x[d];
+                    x12[d] -= (double)nint(x12[d] / L[d]) * L[d];
+                    r2 += x12[d] * x12[d];
+                  }
+                  const double r = std::sqrt(r2);
+                  if (r < lj.cutoff) {
+                    const double dmy = std::min(DBL_MAX / r, lj.f(r));
+                    for (int d = 0; d < 3; ++d) {
+                      const double dmy1 = dmy * -x12[d];
+                      p[i].fr[d] += dmy1;
+                      p[j].fr[d] -= dmy1;
+                    }
+                  }
+                }
+            }
+      }
+}
+
+// Random_Walk and Steepest_descent (fluct.h:66-120): every particle moves
+// by RADIUS dr_factor, along a Gaussian direction or along its force (fr
+// accumulates over the calls, as in the reference), wrapped into the box.
+void random_walk(std::vector<spm_particle> &p, Mt &mt, double radius, double dr_factor, const double L[3]) {
+  const double dr = radius * dr_factor;
+  for (auto &q : p) {
+    double dmy[4];
+    mt.gauss2(dmy);
+    mt.gauss2(dmy + 2);
+    double h = 0.;
+    for (int d = 0; d < 3; ++d) h += dmy[d] * dmy[d];
+    h = std::sqrt(h);
+    if (h > 0.) h = dr / h;
+    for (int d = 0; d < 3; ++d) {
+      q.x[d] += h * dmy[d];
+      q.x[d] -= std::floor(q.x[d] * (1. / L[d])) * L[d];
+    }
+  }
+}
+
+void steepest_descent(std::vector<spm_particle> &p, double radius, const int N[3], double dx, const double L[3],
+                      const InitLj &lj) {
+  const double dr = radius * 0.5e-1;
+  init_lj_lnk(p, N, dx, L, lj);
+  for (auto &q : p) {
+    double h = 0.;
+    for (int d = 0; d < 3; ++d) h += q.fr[d] * q.fr[d];
+    h = std::sqrt(h);
+    if (h > 0.) h = dr / h;
+    for (int d = 0; d < 3; ++d) {
+      q.x[d] += h * q.fr[d];
+      q.x[d] -= std::floor(q.x[d] * (1. / L[d])) * L[d];
+    }
+  }
+}
+
+// FCC placement (init_particle.cxx:57-150): nn cells per axis chosen for
+// the smallest anisotropy, four particles per cell, a last partial layer
+// stretched along z.
+int fcc_lattice(std::vector<spm_particle> &p, const double L[3], double sigma) {
+  const int n = (int)p.size();
+  double lratio[3];
+  const double min_l = std::min(L[0], std::min(L[1], L[2]));
+  for (int d = 0; d < 3; ++d) lratio[d] = L[d] / min_l;
+  const double dmy = std::pow((double)n / (4. * lratio[0] * lratio[1] * lratio[2]), 1. / 3);
+  int nn_base = (int)std::ceil(dmy);
+  const int nn_up = (int)std::ceil(dmy), nn_low = (int)dmy;
+  const int nxny_up = 2 * nn_up * nn_up * (int)lratio[0] * (int)lratio[1];
+  const int nxny_low = 2 * nn_low * nn_low * (int)lratio[0] * (int)lratio[1];
+  const int nz_up = nxny_up > 0 ? (int)std::ceil((double)n / nxny_up) : 0;
+  const int nz_low = nxny_low > 0 ? (int)std::ceil((double)n / nxny_low) : 0;
+  const double density_xy_up = std::sqrt((double)nxny_up / (L[0] * L[1]));
+  const double density_xy_low = std::sqrt((double)nxny_low / (L[0] * L[1]));
+  const double density_z_up = nz_up / L[2], density_z_low = nz_low / L[2];
+  const double skew_up = std::fabs(density_xy_up / density_z_up - 1.);
+  const double skew_low = std::fabs(density_xy_low / density_z_low - 1.);
+  if (nxny_low > 0 && nz_low > 0 && skew_low <= skew_up)
+    nn_base = nn_low;
+  else if (nxny_up > 0 && nz_up > 0)
+    nn_base = nn_up;
+  int nn[3];
+  for (int d = 0; d < 3; ++d) nn[d] = (int)(nn_base * std::ceil(lratio[d]));
+  const bool just_packed = n - 4 * nn[0] * nn[1] * nn[2] == 0;
+  int nz = 0;
+  if (!just_packed) nz = (int)std::ceil((double)n / (2 * nn[0] * nn[1]));
+  double lattice[3], origin[3];
+  for (int d = 0; d < 3; ++d) {
+    lattice[d] = L[d] / nn[d];
+    if (!just_packed && d == 2) lattice[d] = L[d] / (nz * .5);
+    origin[d] = lattice[d] * .25;
+    if (lattice[d] < sigma * std::sqrt(2.))
+      return fail(SPM_ERR_INVALID, "FCC: beyond closed packing along axis " + std::to_string(d));
+  }
+  const int zlayer = 2 * nn[0] * nn[1];
+  for (int i = 0; i < n; ++i) {
+    const int ix = i % nn[0], iy = (i % zlayer) / nn[0], iz = i / zlayer;
+    p[i].x[0] = origin[0] + (double)ix * lattice[0] + (lattice[0] / 2.0) * ((iy + iz) % 2);
+    p[i].x[1] = origin[1] + (double)iy * lattice[1] / 2.0;
+    p[i].x[2] = origin[2] + (double)iz * lattice[2] / 2.0;
+  }
+  return SPM_OK;
+}
+
+// Init_Particle (init_particle.cxx:12-393) for spherical particles; mt and
+// mti (when given) receive the MT19937 state it leaves.
+int init_particles(const spm_udf *u, spm_particle *p, int n, bool user_only, uint32_t *mt_out = nullptr,
+                   int *mti_out = nullptr) {
   if (!u || (!p && n > 0)) return fail(SPM_ERR_INVALID, "null argument");
   Reader r{u};
   const std::string dist = r.s("switch.INIT_distribution.type");
@@ -1053,9 +1279,54 @@ int init_particles(const spm_udf *u, spm_particle *p, int n, bool user_only) {
         out[i].x[1] = out[i].x[1] + ax / 2.0;  // ax, as the reference has it
       }
     }
+  } else if (dist == "FCC" || dist == "random_walk") {  // init_particle.cxx:50-224
+    // the LJ model the relaxation uses: LJ_dia = SIGMA, or under shear
+    // min((SIGMA + XI) / 2^(1/p), SIGMA) (input.cxx:389-403)
+    const std::string ljp = r.s("LJ_powers");
+    const double eps = r.d("EPSILON"), a_xi = r.d("A_XI");
+    int iters = 0;
+    if (dist == "random_walk") iters = r.i("switch.INIT_distribution.random_walk.iteration");
+    char eq[64] = {0};
+    (void)spm_udf_get_string(u, "constitutive_eq.type", eq, sizeof(eq));
+    if (r.rc) return r.rc;
+    InitLj lj;
+    lj.powers = ljp == "12:6" ? 0 : ljp == "24:12" ? 1 : ljp == "36:18" ? 2 : -1;
+    if (lj.powers < 0) return fail(SPM_ERR_INVALID, "invalid LJ_powers " + ljp);
+    if (iters < 0) return fail(SPM_ERR_INVALID, "negative random_walk iteration");
+    const int N[3] = {1 << np[0], 1 << np[1], 1 << np[2]};
+    if (N[0] < 16 || N[1] < 16 || N[2] < 16)
+      return unsupported("FCC / random_walk on a box under 16 grid points per axis (its LJ cells)");
+    const double radius = A * DX;
+    const double pw[3] = {std::pow(2., 1. / 6.), std::pow(2., 1. / 12.), std::pow(2., 1. / 18.)};
+    lj.epsilon = eps;
+    lj.dia = sigma;
+    if (std::string(eq) == "Shear_Navier_Stokes" || std::string(eq) == "Shear_Navier_Stokes_Lees_Edwards")
+      lj.dia = std::min((sigma + a_xi * DX) / pw[lj.powers], sigma);
+    if (int rc = fcc_lattice(out, L, sigma)) return rc;
+    for (auto &q : out)
+      for (int d = 0; d < 3; ++d) q.fr[d] = q.fr_previous[d] = 0.0;
+    Mt mt(0);
+    lj.cutoff = pw[lj.powers] * lj.dia;  // A_R_cutoff by LJ_powers while the lattice settles
+    random_walk(out, mt, radius, 1.e-3, L);
+    steepest_descent(out, radius, N, DX, L, lj);
+    // (the reference repeats the pair while a particle lies outside [0, L)
+    // in y or z; both moves wrap into the box, so it runs once)
+    if (dist == "random_walk") {
+      for (auto &q : out)
+        for (int d = 0; d < 3; ++d) q.fr[d] = q.fr_previous[d] = 0.0;
+      lj.cutoff = pw[0] * lj.dia;  // A_R_cutoff = 2^(1/6)
+      for (int it = 0; it < iters; ++it) {
+        random_walk(out, mt, radius, 0.5e-1 * .5, L);
+        steepest_descent(out, radius, N, DX, L, lj);
+      }
+    }
+    for (auto &q : out)
+      for (int d = 0; d < 3; ++d) q.fr[d] = q.fr_previous[d] = 0.0;  // init_particle.cxx:369-370
+    if (mt_out) mt.state(mt_out, mti_out);
   } else {
     return unsupported("INIT_distribution " + dist);
   }
+  if (mt_out && dist != "FCC" && dist != "random_walk") Mt(0).state(mt_out, mti_out);
   // orientation (init_particle.cxx:318-345)
   for (auto &q : out) {
     if (rotation && orient == "random") {
@@ -1086,6 +1357,12 @@ extern "C" int spm_udf_user_particles(const spm_udf *u, spm_particle *p, int n) 
 
 extern "C" int spm_udf_init_particles(const spm_udf *u, spm_particle *p, int n) {
   return init_particles(u, p, n, false);
+}
+
+extern "C" int spm_udf_init_particles_rng(const spm_udf *u, spm_particle *p, int n, uint32_t mt[SPM_MT_N],
+                                          int *mti) {
+  if (!mt || !mti) return fail(SPM_ERR_INVALID, "null argument");
+  return init_particles(u, p, n, false, mt, mti);
 }
 
 // ---------------------------------------------------------------------

@@ -1,7 +1,7 @@
 """Command-line driver: a run from a UDF input, as the reference's main()
 (sp_3d_ns.cxx:392-640) for constitutive_eq Navier_Stokes / Electrolyte
 with spherical particles placed by INIT_distribution user_specify,
-uniform_random or BCC, and Shear_Navier_Stokes_Lees_Edwards
+uniform_random, BCC, FCC or random_walk, and Shear_Navier_Stokes_Lees_Edwards
 (Time_evolution_hydro_OBL) with or without particles.
 
     python -m kapsel_old_amd.run -I input.udf -O output.udf -R restart.udf [-D define.udf]
@@ -59,14 +59,14 @@ def parse_args(argv):
 
 
 def initial_particles(u, prm, run, resumed):
-    """Init_Particle (user_specify, uniform_random, BCC); for a resumed run
-    of another distribution only the species matter (the restart replaces
-    the rest)."""
-    if udf.DISTRIBUTIONS[run.distribution] in ("user_specify", "uniform_random", "BCC"):
-        return u.init_particles()
+    """Init_Particle and the MT19937 state it leaves (None: the context's
+    own seed-0 state); for a resumed run of another distribution only the
+    species matter (the restart replaces the rest)."""
+    if udf.DISTRIBUTIONS[run.distribution] in ("user_specify", "uniform_random", "BCC", "FCC", "random_walk"):
+        p, mt, mti = u.init_particles(rng=True)
+        return p, (mt, mti)
     if not resumed:
-        raise SystemExit("INIT_distribution %s is not run by this driver; use user_specify, uniform_random or BCC"
-                         % udf.DISTRIBUTIONS[run.distribution])
+        raise SystemExit("INIT_distribution %s is not run by this driver" % udf.DISTRIBUTIONS[run.distribution])
     p = abi.new_particles(prm.n_particles)
     counts = [u.get_int("object_type.spherical_particle.Particle_spec[%d].Particle_number" % s)
               for s in range(prm.n_species)]
@@ -74,7 +74,7 @@ def initial_particles(u, prm, run, resumed):
     L = np.array([prm.nx, prm.ny, prm.nz]) * prm.dx
     p["x"] = (np.arange(prm.n_particles)[:, None] + 0.5) / max(prm.n_particles, 1) * L  # placeholders
     p["x_nopbc"] = p["x"]
-    return p
+    return p, None
 
 
 def restore(ctx, u, prm, n_ion, sheared, log):
@@ -114,7 +114,10 @@ def main(argv=None):
 
     ctx = SPM(prm)
     ctx.set_schedule(SCHEDULE_LITERAL if a.schedule == "literal" else SCHEDULE_FUSED)
-    ctx.put_particles(initial_particles(u, prm, run, resumed or a.resume_state))
+    p, rng = initial_particles(u, prm, run, resumed or a.resume_state)
+    ctx.put_particles(p)
+    if rng is not None:  # the thermostat goes on from the state the initialisation left
+        ctx.put_rng(*rng)
     ctx.init_fluid()
     if el is not None:
         ctx.set_electrolyte(el)

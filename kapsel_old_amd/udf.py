@@ -57,6 +57,7 @@ def lib():
                                      C.POINTER(SpmUdfRun)]
         L.spm_udf_user_particles.argtypes = [vp, vp, ip]
         L.spm_udf_init_particles.argtypes = [vp, vp, ip]
+        L.spm_udf_init_particles_rng.argtypes = [vp, vp, ip, C.POINTER(C.c_uint32), C.POINTER(ip)]
         L.spm_udf_create_output.argtypes = [cp, cp]
         L.spm_udf_append_record.argtypes = [cp, ip, vp, ip]
         L.spm_udf_write_restart.argtypes = [cp, cp, C.POINTER(SpmRestart)]
@@ -153,13 +154,21 @@ class UDF:
         return p
 
 
-    def init_particles(self):
-        """Particle array Init_Particle builds (user_specify, uniform_random, BCC)."""
+    def init_particles(self, rng=False):
+        """Particle array Init_Particle builds (user_specify, uniform_random,
+        BCC, FCC, random_walk); with rng=True also the MT19937 state it
+        leaves, (particles, mt, mti), for SPM.put_rng."""
         n = sum(self.get_int("object_type.spherical_particle.Particle_spec[%d].Particle_number" % s)
                 for s in range(self.size("object_type.spherical_particle.Particle_spec[]")))
         p = abi.new_particles(n)
-        _check(lib().spm_udf_init_particles(self._h, p.ctypes.data_as(C.c_void_p), n), "spm_udf_init_particles")
-        return p
+        if not rng:
+            _check(lib().spm_udf_init_particles(self._h, p.ctypes.data_as(C.c_void_p), n), "spm_udf_init_particles")
+            return p
+        mt = (C.c_uint32 * abi.MT_N)()
+        mti = C.c_int()
+        _check(lib().spm_udf_init_particles_rng(self._h, p.ctypes.data_as(C.c_void_p), n, mt, C.byref(mti)),
+               "spm_udf_init_particles_rng")
+        return p, np.array(mt, dtype=np.uint32), mti.value
 
 
 def _restart_struct(shape, zeta0, zeta1, conc, particles):

@@ -495,13 +495,130 @@ def test_init_particles_unsupported(udf, tmp_path):
         assert len(u.init_particles()) == 1
     with udf.UDF(path("colloid_32.udf")) as u:  # uniform_random, 32 particles in 64^3
         assert len(u.init_particles()) == 32
+    # FCC / random_walk need the LJ cells of 16 points per axis
+    f = _lattice_input(tmp_path, "FCC", 2, 1.0, 3, "12:6", 0)
+    with udf.UDF(f) as u:
+        with pytest.raises(SpmError) as e:
+            u.init_particles()
+    assert e.value.status == abi.SPM_ERR_UNSUPPORTED
+    # a lattice closer than SIGMA sqrt(2) (init_particle.cxx:128-139)
+    f = _lattice_input(tmp_path, "FCC", 128, 3.0, 5, "12:6", 0)
+    with udf.UDF(f) as u:
+        with pytest.raises(SpmError) as e:
+            u.init_particles()
+    assert e.value.status == abi.SPM_ERR_INVALID and "closed packing" in str(e.value)
+
+
+def _lattice_input(tmp_path, dist, per_species, a, npx, powers, iters):
+    """input.udf with INIT_distribution dist, per_species particles of each
+    of its two species, radius a, a 2^npx box, LJ_powers and the
+    random_walk iteration count."""
+    src = open(path("input.udf")).read()
+    edits = [('  "uniform_random",\n', '  "%s",\n' % dist),
+             ('[{5,1.50000000000000,', '[{%d,1.50000000000000,' % per_species),
+             ('{5,0.50000000000000,', '{%d,0.50000000000000,' % per_species),
+             ("A:5.00000000000000", "A:%.17g" % a),
+             ("mesh:{5,5,5}", "mesh:{%d,%d,%d}" % (npx, npx, npx)),
+             ('LJ_powers:"36:18"', 'LJ_powers:"%s"' % powers),
+             ("{100}", "{%d}" % iters)]
+    for old, new in edits:
+        assert src.count(old) == 1, old
+        src = src.replace(old, new)
     shutil.copy(path("define.udf"), tmp_path / "define.udf")
-    for dist in ("FCC", "random_walk"):
-        src = open(path("sample1.udf")).read()
-        assert src.count('"user_specify",\n  {100}') == 1
-        f = tmp_path / ("%s.udf" % dist)
-        f.write_text(src.replace('"user_specify",\n  {100}', '"%s",\n  {100}' % dist))
-        with udf.UDF(f) as u:
-            with pytest.raises(SpmError) as e:
-                u.init_particles()
-        assert e.value.status == abi.SPM_ERR_UNSUPPORTED
+    f = tmp_path / ("%s_%d_%d.udf" % (dist, per_species, npx))
+    f.write_text(src)
+    return str(f)
+
+
+class _NumpyMt:
+    """genrand_real3 and Gauss2 on numpy's MT19937 (legacy seeding is
+    init_genrand), with the C library's log and sqrt."""
+    def __init__(self, seed):
+        self.rs = np.random.RandomState(seed)
+
+    def real3(self):
+        return (float(self.rs.randint(0, 2 ** 32, dtype=np.uint64)) + 0.5) * (1.0 / 4294967296.0)
+
+    def gauss2(self):
+        while True:
+            x1, x2 = 2. * self.real3() - 1., 2. * self.real3() - 1.
+            rsq = x1 * x1 + x2 * x2
+            if rsq < 1.0 and rsq != 0.0:
+                f = math.sqrt(-2. * math.log(rsq) / rsq)
+                return [f * x1, f * x2]
+
+
+def _fcc_sites(n, L):
+    """The FCC placement for n = 4 nn^3 in a cubic box of side L: four
+    sublattices of an nn^3 cubic lattice of spacing L / nn, offset L / (4 nn)."""
+    nn = round((n / 4) ** (1 / 3))
+    lat = L / nn
+    i = np.arange(n)
+    ix, iy, iz = i % nn, (i % (2 * nn * nn)) // nn, i // (2 * nn * nn)
+    x = lat / 4 + ix * lat + lat / 2 * ((iy + iz) % 2)
+    return np.stack([x, lat / 4 + iy * lat / 2, lat / 4 + iz * lat / 2], axis=1)
+
+
+def _walk(x, mt, dr, L):
+    """Random_Walk: every particle moves by dr along a Gaussian direction."""
+    out = x.copy()
+    for k in range(len(x)):
+        g = np.array(mt.gauss2() + mt.gauss2())[:3]
+        out[k] += dr / np.linalg.norm(g) * g
+    return out - np.floor(out / L) * L
+
+
+def _descend(x, fr, dr, L):
+    h = np.linalg.norm(fr, axis=1)
+    out = x + np.where(h > 0, dr / np.where(h > 0, h, 1), 0)[:, None] * fr
+    return out - np.floor(out / L) * L
+
+
+def test_init_particles_fcc_without_forces_is_lattice_plus_walk(udf, tmp_path):
+    """32 spheres of radius 2 in 64^3: the FCC sites are 22.6 apart, beyond
+    the WCA range, so the steepest descent does not move them; each sits
+    1e-3 RADIUS from its site along the Gaussian direction drawn from
+    MT19937 seeded 0, and the generator is left where those draws end."""
+    f = _lattice_input(tmp_path, "FCC", 16, 2.0, 6, "12:6", 0)
+    with udf.UDF(f) as u:
+        p, mt, mti = u.init_particles(rng=True)
+    sites = _fcc_sites(32, 64.0)
+    ref = _NumpyMt(0)
+    want = _walk(sites, ref, 2.0 * 1e-3, 64.0)
+    np.testing.assert_allclose(p["x"], want, rtol=0, atol=1e-13)
+    assert np.abs(np.linalg.norm(p["x"] - sites, axis=1) - 2e-3).max() < 1e-12
+    np.testing.assert_array_equal(p["x_nopbc"], p["x"])
+    assert not p["fr"].any() and not p["v"].any()
+    st = ref.rs.get_state()
+    np.testing.assert_array_equal(mt, st[1])
+    assert mti == st[2]
+    assert list(p["spec"]) == [0] * 16 + [1] * 16
+
+
+@pytest.mark.parametrize("dist,iters", [("FCC", 0), ("random_walk", 3)])
+def test_init_particles_relaxation_with_forces(udf, tmp_path, dist, iters):
+    """256 spheres of radius 2.6 in 32^3 (two LJ cells per axis, where the
+    reference counts some pairs twice): nearest FCC neighbours sit inside
+    the WCA range.  The placement equals the walk re-run from numpy's
+    MT19937 with forces from the oracle's linked-cell LJ (pinned against
+    brute force), fr accumulating over the descents as in the reference."""
+    from oracle.oracle import Oracle
+    a, L = 2.6, 32.0
+    f = _lattice_input(tmp_path, dist, 128, a, 5, "12:6", iters)
+    with udf.UDF(f) as u:
+        p, mt, mti = u.init_particles(rng=True)
+    o = Oracle(abi.make_params(32, a=a, lj_truncate=abi.LJ_WCA, lj_powers=abi.LJ_12_6, epsilon=1.0))
+    ref = _NumpyMt(0)
+    x = _walk(_fcc_sites(256, L), ref, a * 1e-3, L)
+    fr = o.lj(x)
+    assert np.abs(fr).max() > 0
+    x = _descend(x, fr, a * 0.05, L)
+    fr = np.zeros_like(x)
+    for _ in range(iters):
+        x = _walk(x, ref, a * 0.025, L)
+        fr = fr + o.lj(x)
+        x = _descend(x, fr, a * 0.05, L)
+    np.testing.assert_allclose(p["x"], x, rtol=0, atol=1e-11)
+    st = ref.rs.get_state()
+    np.testing.assert_array_equal(mt, st[1])
+    assert mti == st[2]
