@@ -350,6 +350,12 @@ int fft_c2r(spm_ctx *c, double *a) { return fft_many(c, &a, 1, false); }
 
 namespace spm {
 
+#define RC_(x)            \
+  do {                    \
+    const int rc_ = (x);  \
+    if (rc_) return rc_;  \
+  } while (0)
+
 // Single passes, used by the fused schedule (fused.hip supplies the x and
 // z passes that carry pointwise work).  kind: 0 = y pass, 1 = z forward,
 // 2 = z inverse.  They carry only the modes the 2/3 rule keeps: kz <
@@ -394,6 +400,53 @@ static int fft_pass_t(spm_ctx *c, double *const *a, int n, int kind, int sign) {
 static int fft_pass(spm_ctx *c, double *const *a, int n, int kind, int sign) {
   return c->prm.precision == SPM_FP32 ? fft_pass_t<float>(c, a, n, kind, sign)
                                       : fft_pass_t<double>(c, a, n, kind, sign);
+}
+
+// Whole transforms with the band-limited y and z passes: the inverse of
+// spectra that vanish outside the 2/3 box, and the forward transform whose
+// modes outside the box are not read on (they are left unspecified).  The
+// x pass is the full one.
+template <typename T>
+static int fft_band_t(spm_ctx *c, double *const *a, int n, bool fwd) {
+  const Grid &g = c->grid;
+  for (int b = 0; b < n; b += 5) {
+    const int nf = (n - b) < 5 ? (n - b) : 5;
+    FieldPtrs<T> fp{};
+    for (int q = 0; q < nf; ++q) fp.p[q] = reinterpret_cast<T *>(a[b + q]);
+    const long h = g.hnz;
+    const double bytes = 2.0 * nf * (double)g.nx * g.ny * g.nz_ * sizeof(T);
+    int rc = 0;
+    if (fwd) {
+      RC_(fft_pass_t<T>(c, a + b, nf, 1, +1));
+      RC_(fft_pass_t<T>(c, a + b, nf, 0, +1));
+      tbeg(c);
+      rc |= launch_lines<T, +1>(g.nx, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1, c->stream);
+      tend(c, SPM_K_FFT_X, bytes);
+    } else {
+      tbeg(c);
+      hipLaunchKernelGGL((ooura_planes_kernel<T>), dim3((unsigned)((2L * g.nx * g.ny + 255) / 256), nf), dim3(256), 0,
+                         c->stream, fp, g.nx, g.ny, g.nz_, g.nz / 2);
+      tend(c, SPM_K_SPECTRAL, 2.0 * nf * 2.0 * g.nx * g.ny * 2 * sizeof(T));
+      tbeg(c);
+      rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1, c->stream);
+      tend(c, SPM_K_FFT_X, bytes);
+      if (!rc) RC_(fft_pass_t<T>(c, a + b, nf, 0, -1));
+      if (!rc) RC_(fft_pass_t<T>(c, a + b, nf, 2, -1));
+    }
+    if (rc) {
+      c->err = "unsupported FFT size";
+      return SPM_ERR_UNSUPPORTED;
+    }
+    SPM_LAUNCH_CHECK(c);
+  }
+  return 0;
+}
+
+int fft_r2c_many_band(spm_ctx *c, double *const *a, int n) {
+  return c->prm.precision == SPM_FP32 ? fft_band_t<float>(c, a, n, true) : fft_band_t<double>(c, a, n, true);
+}
+int fft_c2r_many_band(spm_ctx *c, double *const *a, int n) {
+  return c->prm.precision == SPM_FP32 ? fft_band_t<float>(c, a, n, false) : fft_band_t<double>(c, a, n, false);
 }
 
 int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign) { return fft_pass(c, a, n, 0, sign); }

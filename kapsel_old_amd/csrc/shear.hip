@@ -428,14 +428,16 @@ int le_ns_solver(spm_ctx *c) {
                      c->zeta[1], w3[0], w3[1], w3[2]);
   tend(c, SPM_K_SPECTRAL, 80.0 * nmodes(c));
   SPM_LAUNCH_CHECK(c);
+  // zeta is truncated, so u and omega vanish outside the 2/3 box
   double *const uw[6] = {u3[0], u3[1], u3[2], w3[0], w3[1], w3[2]};
-  RCS(fft_c2r_many(c, uw, 6));
+  RCS(fft_c2r_many_band(c, uw, 6));
   tbeg(c);
   hipLaunchKernelGGL(cross_kernel, dim3(nblk(npoints(c))), dim3(kBlock), 0, c->stream, g, u3[0], u3[1], u3[2], w3[0],
                      w3[1], w3[2]);
   tend(c, SPM_K_REALSPACE, 72.0 * npoints(c));
   SPM_LAUNCH_CHECK(c);
-  RCS(fft_r2c_many(c, u3, 3));
+  // only the kept modes of the advection enter the update
+  RCS(fft_r2c_many_band(c, u3, 3));
   tbeg(c);
   hipLaunchKernelGGL(advect_tail_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, u3[0], u3[1],
                      u3[2], c->zeta[0], c->zeta[1], c->uk_dc, c->le_red + kRedBlocks, deg, c->work[1], c->work[2]);
@@ -467,7 +469,7 @@ int le_step(spm_ctx *c, int fix_mask) {
   RCS(le_ns_solver(c));
   double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
   RCS(le_zeta2u_k(c, u3, fix_mask));
-  RCS(fft_c2r_many(c, u3, 3));
+  RCS(fft_c2r_many_band(c, u3, 3));  // the updated zeta is still zero outside the box
   // Shear_rate_eff = Shear_rate; degree_oblique += Shear_rate_eff * dt
   const double rate = c->le_rate;
   c->le_deg += rate * c->prm.dt;

@@ -249,6 +249,10 @@ int fft_c2r(spm_ctx *c, double *a);
 // batched versions over several fields (same grid)
 int fft_r2c_many(spm_ctx *c, double *const *a, int n);
 int fft_c2r_many(spm_ctx *c, double *const *a, int n);
+// with the band-limited y and z passes (inverse of spectra zero outside the
+// 2/3 box; forward whose modes outside it are left unspecified)
+int fft_r2c_many_band(spm_ctx *c, double *const *a, int n);
+int fft_c2r_many_band(spm_ctx *c, double *const *a, int n);
 int fft_init_tables(spm_ctx *c);
 int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign);
 int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n);
