@@ -402,10 +402,11 @@ static int fft_pass(spm_ctx *c, double *const *a, int n, int kind, int sign) {
                                       : fft_pass_t<double>(c, a, n, kind, sign);
 }
 
-// Whole transforms with the band-limited y and z passes: the inverse of
-// spectra that vanish outside the 2/3 box, and the forward transform whose
-// modes outside the box are not read on (they are left unspecified).  The
-// x pass is the full one.
+// Whole transforms with band-limited passes: the inverse of spectra that
+// vanish outside the 2/3 box, and the forward transform whose modes outside
+// the box are not read on (they are left unspecified).  The x pass runs over
+// the kz < TRN_Z columns of every ky row (as NY outer blocks of NZ/2+1
+// columns), the y and z passes are those of the fused schedule.
 template <typename T>
 static int fft_band_t(spm_ctx *c, double *const *a, int n, bool fwd) {
   const Grid &g = c->grid;
@@ -415,21 +416,22 @@ static int fft_band_t(spm_ctx *c, double *const *a, int n, bool fwd) {
     for (int q = 0; q < nf; ++q) fp.p[q] = reinterpret_cast<T *>(a[b + q]);
     const long h = g.hnz;
     const double bytes = 2.0 * nf * (double)g.nx * g.ny * g.nz_ * sizeof(T);
+    const double fz = (double)g.trn_z / g.hnz;
     int rc = 0;
     if (fwd) {
       RC_(fft_pass_t<T>(c, a + b, nf, 1, +1));
       RC_(fft_pass_t<T>(c, a + b, nf, 0, +1));
       tbeg(c);
-      rc |= launch_lines<T, +1>(g.nx, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1, c->stream);
-      tend(c, SPM_K_FFT_X, bytes);
+      rc |= launch_lines<T, +1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream, g.trn_z);
+      tend(c, SPM_K_FFT_X, bytes * fz);
     } else {
       tbeg(c);
       hipLaunchKernelGGL((ooura_planes_kernel<T>), dim3((unsigned)((2L * g.nx * g.ny + 255) / 256), nf), dim3(256), 0,
                          c->stream, fp, g.nx, g.ny, g.nz_, g.nz / 2);
       tend(c, SPM_K_SPECTRAL, 2.0 * nf * 2.0 * g.nx * g.ny * 2 * sizeof(T));
       tbeg(c);
-      rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1, c->stream);
-      tend(c, SPM_K_FFT_X, bytes);
+      rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream, g.trn_z);
+      tend(c, SPM_K_FFT_X, bytes * fz);
       if (!rc) RC_(fft_pass_t<T>(c, a + b, nf, 0, -1));
       if (!rc) RC_(fft_pass_t<T>(c, a + b, nf, 2, -1));
     }
