@@ -164,19 +164,22 @@ __global__ void cross_kernel(Grid g, double *u0, double *u1, double *u2, const d
   u2[im] = a1 * b2 - a2 * b1;
 }
 
-// The tail of Zeta_k2advection_k_OBL (operate_omega.cxx:158-172): the shear
-// terms u0 -= 2 rate uk_cp, u1 -= 2 rate deg uk_cp, U_k2rotation_k
-// (fft_wrapper.cxx:679-718) and Omega_k2zeta_k_OBL (fft_wrapper.cxx:175-198)
-// into adv.  `rate` is Shear_rate_eff on the device.  uk_cp, the y velocity
-// spectrum of the step's zeta (operate_omega.cxx:124-128), is formed again
-// here from zeta, which the solver has not yet changed, rather than kept in
-// a field of its own.  80 B read, 32 B written per mode.
-__global__ void advect_tail_obl_kernel(Grid g, const double *__restrict__ u0, const double *__restrict__ u1,
-                                       const double *__restrict__ u2, const double *__restrict__ z0,
-                                       const double *__restrict__ z1, const double *__restrict__ uk_dc,
-                                       const double *__restrict__ rate_p, double deg, double *a0, double *a1) {
+// The tail of Zeta_k2advection_k_OBL (operate_omega.cxx:158-172) and the
+// update of NS_solver_slavedEuler_Shear_OBL (fluid_solver.cxx:407-428) in
+// one pass over the kept modes (the update reads the advection only there):
+// the shear terms u0 -= 2 rate uk_cp, u1 -= 2 rate deg uk_cp,
+// U_k2rotation_k (fft_wrapper.cxx:679-718), Omega_k2zeta_k_OBL
+// (fft_wrapper.cxx:175-198), then zeta += (exp(-nu dt K2) - 1)(zeta -
+// adv / (nu K2)) with the oblique K2.  `rate` is Shear_rate_eff on the
+// device.  uk_cp, the y velocity spectrum of the step's zeta
+// (operate_omega.cxx:124-128), is formed again here from zeta rather than
+// kept in a field of its own.  80 B read, 32 B written per kept mode.
+__global__ void advect_update_obl_kernel(Grid g, const double *__restrict__ u0, const double *__restrict__ u1,
+                                         const double *__restrict__ u2, double *z0, double *z1,
+                                         const double *__restrict__ uk_dc, const double *__restrict__ rate_p,
+                                         double deg, double nu, double dt) {
   ModeO m;
-  if (!mode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m)) return;
+  if (!mode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m) || !m.keep) return;
   const double rate = *rate_p;
   double cr[3], ci[3];
   zeta2u_mode(g, m, deg, z0, z1, cr, ci);
@@ -199,19 +202,7 @@ __global__ void advect_tail_obl_kernel(Grid g, const double *__restrict__ u0, co
   ri[2] = -(ks[0] * re[1] - ks[1] * re[0]);
   const int a = m.kx != 0 ? 1 : (m.kz != 0 ? 1 : 0);
   const int b = m.kx != 0 ? 2 : (m.kz != 0 ? 0 : 2);
-  a0[m.im] = rr[a];
-  a0[m.im + 1] = ri[a];
-  a1[m.im] = rr[b];
-  a1[m.im + 1] = ri[b];
-}
-
-// NS_solver_slavedEuler_Shear_OBL's update (fluid_solver.cxx:407-428) on
-// the 2/3-rule box with the oblique K2.  48 B read, 32 B written per kept
-// mode.
-__global__ void slaved_euler_obl_kernel(Grid g, double *z0, double *z1, const double *__restrict__ a0,
-                                        const double *__restrict__ a1, double nu, double dt, double deg) {
-  ModeO m;
-  if (!mode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m) || !m.keep) return;
+  const double adv0[2] = {rr[a], ri[a]}, adv1[2] = {rr[b], ri[b]};
   const double k2 = k2_obl(g, m, deg);
   const double ik2 = k2 > 0.0 ? 1. / k2 : 0.0;
   const double dmy0 = -nu * dt, dmy1 = 1. / nu;
@@ -219,8 +210,8 @@ __global__ void slaved_euler_obl_kernel(Grid g, double *z0, double *z1, const do
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const long im = m.im + s;
-    z0[im] += dmy * (z0[im] - dmy1 * ik2 * a0[im]);
-    z1[im] += dmy * (z1[im] - dmy1 * ik2 * a1[im]);
+    z0[im] += dmy * (z0[im] - dmy1 * ik2 * adv0[s]);
+    z1[im] += dmy * (z1[im] - dmy1 * ik2 * adv1[s]);
   }
 }
 
@@ -413,8 +404,8 @@ int le_u_k2zeta_k(spm_ctx *c) {
 }
 
 // NS_solver_slavedEuler_Shear_OBL (fluid_solver.cxx:398-429) with
-// Zeta_k2advection_k_OBL (operate_omega.cxx:115-173).  Scratch: work[1..2]
-// (advection), f[0..2] (omega).
+// Zeta_k2advection_k_OBL (operate_omega.cxx:115-173).  Scratch: f[0..2]
+// (omega).
 int le_ns_solver(spm_ctx *c) {
   const Grid &g = c->grid;
   const double deg = c->le_deg;
@@ -439,14 +430,11 @@ int le_ns_solver(spm_ctx *c) {
   // only the kept modes of the advection enter the update
   RCS(fft_r2c_many_band(c, u3, 3));
   tbeg(c);
-  hipLaunchKernelGGL(advect_tail_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, u3[0], u3[1],
-                     u3[2], c->zeta[0], c->zeta[1], c->uk_dc, c->le_red + kRedBlocks, deg, c->work[1], c->work[2]);
-  tend(c, SPM_K_SPECTRAL, 112.0 * nmodes(c));
-  SPM_LAUNCH_CHECK(c);
-  tbeg(c);
-  hipLaunchKernelGGL(slaved_euler_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, c->zeta[0],
-                     c->zeta[1], c->work[1], c->work[2], c->prm.eta * (1. / c->prm.rho), c->prm.dt, deg);
-  tend(c, SPM_K_SPECTRAL, 80.0 * nmodes(c));
+  hipLaunchKernelGGL(advect_update_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, u3[0], u3[1],
+                     u3[2], c->zeta[0], c->zeta[1], c->uk_dc, c->le_red + kRedBlocks, deg,
+                     c->prm.eta * (1. / c->prm.rho), c->prm.dt);
+  tend(c, SPM_K_SPECTRAL, 112.0 * nmodes(c) * ((double)(2 * g.trn_x - 1) / g.nx) *
+                              ((double)(2 * g.trn_y - 1) / g.ny) * ((double)g.trn_z / g.hnz));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

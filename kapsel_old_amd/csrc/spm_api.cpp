@@ -771,7 +771,6 @@ int set_lees_edwards(spm_ctx *c, double shear_rate) {
     c->err = "the Lees-Edwards path runs in FP64 only";
     return SPM_ERR_UNSUPPORTED;
   }
-  if (!c->work[2]) RC(dalloc(c, &c->work[2], c->nfield));
   RC(dalloc(c, &c->le_red, (size_t)le_red_slots()));  // zeroes Shear_rate_eff
   // under shear the LJ diameter is min((SIGMA + XI) / 2^(1/p), SIGMA) with
   // the LJ_powers exponent p (input.cxx:389-403); R_cutoff = A_R_cutoff LJ_dia
