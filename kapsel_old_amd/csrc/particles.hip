@@ -903,7 +903,8 @@ __global__ __launch_bounds__(256) void force_obl_kernel(Grid g, PModel pm, doubl
 // u(i, j) += the oblique components of f = up - phi ucp interpolated
 // between x rows i_oblique and i_oblique + sign.  ucp is the Cartesian
 // velocity.  Each source row serves two destination rows, so 56 B of f
-// sources and 24 B of u are read and 24 B written per point.
+// sources and 24 B of u are read and 24 B written per point where a
+// particle reaches; elsewhere only phi is read.
 __global__ void f_add_obl_kernel(Grid g, const double *__restrict__ up0, const double *__restrict__ up1,
                                  const double *__restrict__ up2, const double *__restrict__ phi,
                                  const double *__restrict__ c0, const double *__restrict__ c1,
@@ -922,6 +923,9 @@ __global__ void f_add_obl_kernel(Grid g, const double *__restrict__ up0, const d
   io = ((i + io) % NX + NX) % NX;
   const int iop = ((io + (int)sign) % NX + NX) % NX;
   const long a = ((long)io * NY + j) * g.nz_ + k, b = ((long)iop * NY + j) * g.nz_ + k;
+  // phi = 0 means no particle reached the point, so up is zero there too and
+  // f vanishes: u stays as it is (adding zeros would not change it)
+  if (phi[a] == 0.0 && phi[b] == 0.0) return;
   const long im = ((long)i * NY + j) * g.nz_ + k;
   const double fa0 = up0[a] - phi[a] * c0[a], fb0 = up0[b] - phi[b] * c0[b];
   const double fa1 = up1[a] - phi[a] * c1[a], fb1 = up1[b] - phi[b] * c1[b];
@@ -1183,7 +1187,7 @@ int p_f_add_obl(spm_ctx *c, double *const ucp[3]) {
   tbeg(c);
   hipLaunchKernelGGL(f_add_obl_kernel, dim3(blocks(npts, 256)), dim3(256), 0, c->stream, c->grid, c->up[0], c->up[1],
                      c->up[2], c->phi, ucp[0], ucp[1], ucp[2], c->u[0], c->u[1], c->u[2], c->le_deg);
-  tend(c, SPM_K_REALSPACE, 104.0 * npts);
+  tend(c, SPM_K_REALSPACE, 8.0 * npts);  // phi only: the rest depends on where particles reach
   SPM_LAUNCH_CHECK(c);
   return 0;
 }
