@@ -408,7 +408,7 @@ static int fft_pass(spm_ctx *c, double *const *a, int n, int kind, int sign) {
 // the kz < TRN_Z columns of every ky row (as NY outer blocks of NZ/2+1
 // columns), the y and z passes are those of the fused schedule.
 template <typename T>
-static int fft_band_t(spm_ctx *c, double *const *a, int n, bool fwd) {
+static int fft_band_t(spm_ctx *c, double *const *a, int n, bool fwd, bool with_z = true) {
   const Grid &g = c->grid;
   for (int b = 0; b < n; b += 5) {
     const int nf = (n - b) < 5 ? (n - b) : 5;
@@ -419,7 +419,7 @@ static int fft_band_t(spm_ctx *c, double *const *a, int n, bool fwd) {
     const double fz = (double)g.trn_z / g.hnz;
     int rc = 0;
     if (fwd) {
-      RC_(fft_pass_t<T>(c, a + b, nf, 1, +1));
+      if (with_z) RC_(fft_pass_t<T>(c, a + b, nf, 1, +1));
       RC_(fft_pass_t<T>(c, a + b, nf, 0, +1));
       tbeg(c);
       rc |= launch_lines<T, +1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream, g.trn_z);
@@ -433,7 +433,7 @@ static int fft_band_t(spm_ctx *c, double *const *a, int n, bool fwd) {
       rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.hnz, (long)g.ny * h, h, g.ny, c->stream, g.trn_z);
       tend(c, SPM_K_FFT_X, bytes * fz);
       if (!rc) RC_(fft_pass_t<T>(c, a + b, nf, 0, -1));
-      if (!rc) RC_(fft_pass_t<T>(c, a + b, nf, 2, -1));
+      if (!rc && with_z) RC_(fft_pass_t<T>(c, a + b, nf, 2, -1));
     }
     if (rc) {
       c->err = "unsupported FFT size";
@@ -449,6 +449,14 @@ int fft_r2c_many_band(spm_ctx *c, double *const *a, int n) {
 }
 int fft_c2r_many_band(spm_ctx *c, double *const *a, int n) {
   return c->prm.precision == SPM_FP32 ? fft_band_t<float>(c, a, n, false) : fft_band_t<double>(c, a, n, false);
+}
+int fft_c2r_band_xy(spm_ctx *c, double *const *a, int n) {
+  return c->prm.precision == SPM_FP32 ? fft_band_t<float>(c, a, n, false, false)
+                                      : fft_band_t<double>(c, a, n, false, false);
+}
+int fft_r2c_band_yx(spm_ctx *c, double *const *a, int n) {
+  return c->prm.precision == SPM_FP32 ? fft_band_t<float>(c, a, n, true, false)
+                                      : fft_band_t<double>(c, a, n, true, false);
 }
 
 int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign) { return fft_pass(c, a, n, 0, sign); }

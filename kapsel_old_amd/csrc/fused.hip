@@ -369,6 +369,66 @@ __global__ __launch_bounds__(ROWS *M / R) void z_advect_kernel(Grid g, T *u0, T 
 }
 
 // ---------------------------------------------------------------------
+// z pass of the sheared solver (Zeta_k2advection_k_OBL, operate_omega.cxx:
+// 115-146): C2R of the velocity (u0, u1, u2) and the vorticity (w0, w1,
+// w2), the real-space u x w (operate_omega.cxx:123-146), R2C of its three
+// components into u0..u2.  kz < TRN_Z entries only are read and written
+// (the spectra are truncated; the update reads kept modes only).
+// ---------------------------------------------------------------------
+template <typename T, int M, int R, int ROWS>
+__global__ __launch_bounds__(ROWS *M / R) void z_cross_kernel(Grid g, T *u0, T *u1, T *u2, const T *w0, const T *w1,
+                                                             const T *w2, const double2 *__restrict__ tw_g,
+                                                             const double2 *__restrict__ twr_g, T scale) {
+  using C2 = cplx<T>;
+  constexpr int NT = ROWS * M / R, CP = ROWS + 1, E = ROWS * M / NT;
+  __shared__ C2 lds[(M + 1) * CP];
+  const int tid = threadIdx.x;
+  __shared__ double2 tw[M];
+  for (int t = tid; t < M; t += blockDim.x) tw[t] = tw_g[t];
+  __shared__ double2 twr[M + 1];
+  for (int t = tid; t < M + 1; t += blockDim.x) twr[t] = twr_g[t];
+  const long row0 = (long)blockIdx.x * ROWS;
+  const int nrows = g.nx * g.ny;
+  C2 v[6][E];
+  constexpr int KF = ZRows<M, R, ROWS>::KF;
+  C2 cur[KF], nxt[KF];
+  const int kz_cut = g.trn_z;
+  const T *const in[6] = {u0, u1, u2, w0, w1, w2};
+  z_rows_fetch<T, M, R, ROWS>(in[0], row0, nrows, g.nz_, kz_cut, tid, cur);
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    if (d < 5) z_rows_fetch<T, M, R, ROWS>(in[d + 1], row0, nrows, g.nz_, kz_cut, tid, nxt);
+    z_c2r_regs_to_lds<T, M, R, ROWS>(lds, cur, twr, tid);
+    if (d < 5)
+#pragma unroll
+      for (int t = 0; t < KF; ++t) cur[t] = nxt[t];
+    fft_stages<T, M, R, ROWS, CP, -1, 1>(lds, tw, tid);
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, m = f % M, r = f / M;
+      const C2 x = lds[m * CP + r];
+      v[d][t] = C2{x.x * scale, x.y * scale};
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    T *out = q == 0 ? u0 : q == 1 ? u1 : u2;
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+      const int f = tid + t * NT, m = f % M, r = f / M;
+      // each complex slot packs two real points: the cross product of each
+      const int a = (q + 1) % 3, b = (q + 2) % 3;  // (u x w)_q = u_a w_b - u_b w_a
+      const C2 pr{v[a][t].x * v[3 + b][t].x - v[b][t].x * v[3 + a][t].x,
+                  v[a][t].y * v[3 + b][t].y - v[b][t].y * v[3 + a][t].y};
+      lds[m * CP + r] = pr;
+    }
+    __syncthreads();
+    z_r2c_lds_to_row<T, M, R, ROWS>(lds, out, row0, nrows, g.nz_, kz_cut, tw, twr, tid);
+  }
+}
+
+// ---------------------------------------------------------------------
 // x pass (forward) of the five product spectra with the advection
 // contraction, Omega_k2zeta_k and the slaved-Euler update as epilogue.
 // The transformed products are not stored.
@@ -760,6 +820,22 @@ int fx_project(spm_ctx *c) {
   const Band b = band(c);
   tend(c, SPM_K_FFT_X, (3.0 * b.col + 4.0 * b.col * b.kx) * field_bytes(c));
   if (rc) return rc;
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int fz_cross_obl(spm_ctx *c, double *const u[3], double *const w[3]) {
+  using T = double;
+  const Grid &g = c->grid;
+  const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+  tbeg(c);
+  SPM_ZSIZE_SWITCH(g.nz / 2, {
+    constexpr int R = kZadvR(S), ROWS = kZadvRows<T>(S);
+    hipLaunchKernelGGL((z_cross_kernel<T, S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R), 0,
+                       c->stream, g, u[0], u[1], u[2], w[0], w[1], w[2], c->tw_z, c->tw_rz, scale);
+  });
+  // six truncated spectra read, three written, kz < TRN_Z entries
+  tend(c, SPM_K_FFT_Z_C2R, 9.0 * c->nfield * sizeof(T) * ((double)g.trn_z / g.hnz));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

@@ -150,20 +150,6 @@ __global__ void zeta2omega_obl_kernel(Grid g, const double *__restrict__ z0, con
   }
 }
 
-// u <- u x omega at every grid point (operate_omega.cxx:123-146): 48 B read,
-// 24 B written per point.
-__global__ void cross_kernel(Grid g, double *u0, double *u1, double *u2, const double *__restrict__ w0,
-                             const double *__restrict__ w1, const double *__restrict__ w2) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.nx * g.ny * g.nz) return;
-  const long im = (t / g.nz) * g.nz_ + t % g.nz;
-  const double a1 = u0[im], a2 = u1[im], a3 = u2[im];
-  const double b1 = w0[im], b2 = w1[im], b3 = w2[im];
-  u0[im] = a2 * b3 - a3 * b2;
-  u1[im] = a3 * b1 - a1 * b3;
-  u2[im] = a1 * b2 - a2 * b1;
-}
-
 // The tail of Zeta_k2advection_k_OBL (operate_omega.cxx:158-172) and the
 // update of NS_solver_slavedEuler_Shear_OBL (fluid_solver.cxx:407-428) in
 // one pass over the kept modes (the update reads the advection only there):
@@ -419,16 +405,13 @@ int le_ns_solver(spm_ctx *c) {
                      c->zeta[1], w3[0], w3[1], w3[2]);
   tend(c, SPM_K_SPECTRAL, 80.0 * nmodes(c));
   SPM_LAUNCH_CHECK(c);
-  // zeta is truncated, so u and omega vanish outside the 2/3 box
+  // zeta is truncated, so u and omega vanish outside the 2/3 box: x and y
+  // passes, then one z pass that also forms u x omega in real space, then
+  // the y and x passes of the product (only its kept modes are read on)
   double *const uw[6] = {u3[0], u3[1], u3[2], w3[0], w3[1], w3[2]};
-  RCS(fft_c2r_many_band(c, uw, 6));
-  tbeg(c);
-  hipLaunchKernelGGL(cross_kernel, dim3(nblk(npoints(c))), dim3(kBlock), 0, c->stream, g, u3[0], u3[1], u3[2], w3[0],
-                     w3[1], w3[2]);
-  tend(c, SPM_K_REALSPACE, 72.0 * npoints(c));
-  SPM_LAUNCH_CHECK(c);
-  // only the kept modes of the advection enter the update
-  RCS(fft_r2c_many_band(c, u3, 3));
+  RCS(fft_c2r_band_xy(c, uw, 6));
+  RCS(fz_cross_obl(c, u3, w3));
+  RCS(fft_r2c_band_yx(c, u3, 3));
   tbeg(c);
   hipLaunchKernelGGL(advect_update_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, u3[0], u3[1],
                      u3[2], c->zeta[0], c->zeta[1], c->uk_dc, c->le_red + kRedBlocks, deg,

@@ -253,6 +253,11 @@ int fft_c2r_many(spm_ctx *c, double *const *a, int n);
 // 2/3 box; forward whose modes outside it are left unspecified)
 int fft_r2c_many_band(spm_ctx *c, double *const *a, int n);
 int fft_c2r_many_band(spm_ctx *c, double *const *a, int n);
+// the same without the z pass (x and y of an inverse; y and x of a forward)
+int fft_c2r_band_xy(spm_ctx *c, double *const *a, int n);
+int fft_r2c_band_yx(spm_ctx *c, double *const *a, int n);
+// z pass of the sheared solver: C2R of u and w, u x w, R2C into u (FP64)
+int fz_cross_obl(spm_ctx *c, double *const u[3], double *const w[3]);
 int fft_init_tables(spm_ctx *c);
 int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign);
 int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n);
