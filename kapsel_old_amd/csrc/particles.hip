@@ -108,51 +108,65 @@ __global__ void tile_fill_kernel(PModel pm, const int *__restrict__ pxint, int t
   if (ti >= 0) tile_list[atomicAdd(&tile_cursor[ti], 1)] = n;
 }
 
-// Single-block exclusive scan of count[0..n) into start[] and cursor[],
-// 4096 elements per round: each thread takes 4 consecutive counts, the
-// wavefronts scan their 64 partial sums with shuffles, and one LDS exchange
-// combines the 16 wavefronts.
+// Exclusive scan of count[0..n) into start[] and cursor[], one block per
+// 4096 counts.  Each block first sums all the counts before its chunk (at
+// most n ints, read from L2), so blocks never wait on one another; then
+// each thread takes 4 consecutive counts, the wavefronts scan their 64
+// partial sums with shuffles, and one LDS exchange combines the 16
+// wavefronts.  Integer sums: the result does not depend on the order.
 __global__ __launch_bounds__(1024) void scan_kernel(const int *__restrict__ count, int *start, int *cursor, int n) {
   __shared__ int wsum[16];
+  __shared__ int carry_s;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  int carry = 0;
-  for (int base = 0; base < n; base += 4096) {
-    const int i0 = base + 4 * t;
-    int v[4];
+  const int base = blockIdx.x * 4096;
+  int acc = 0;
+  for (int i = t; i < base && i < n; i += 1024) acc += count[i];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = (i0 + q < n) ? count[i0 + q] : 0;
-    const int s = v[0] + v[1] + v[2] + v[3];
-    int x = s;  // inclusive scan over the wavefront
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) wsum[w] = acc;
+  __syncthreads();
+  if (t == 0) {
+    int c = 0;
+    for (int k = 0; k < 16; ++k) c += wsum[k];
+    carry_s = c;
+  }
+  __syncthreads();
+  const int carry = carry_s;
+  const int i0 = base + 4 * t;
+  int v[4];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(x, off, 64);
-      if (lane >= off) x += y;
+  for (int q = 0; q < 4; ++q) v[q] = (i0 + q < n) ? count[i0 + q] : 0;
+  const int s = v[0] + v[1] + v[2] + v[3];
+  int x = s;  // inclusive scan over the wavefront
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    int y = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const int z = __shfl_up(y, off, 64);
+      if (lane >= off) y += z;
     }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    if (w == 0) {
-      int y = lane < 16 ? wsum[lane] : 0;
+    if (lane < 16) wsum[lane] = y;  // inclusive over wavefronts
+  }
+  __syncthreads();
+  int run = carry + (w > 0 ? wsum[w - 1] : 0) + x - s;
 #pragma unroll
-      for (int off = 1; off < 16; off <<= 1) {
-        const int z = __shfl_up(y, off, 64);
-        if (lane >= off) y += z;
-      }
-      if (lane < 16) wsum[lane] = y;  // inclusive over wavefronts
+  for (int q = 0; q < 4; ++q) {
+    if (i0 + q < n) {
+      start[i0 + q] = run;
+      cursor[i0 + q] = run;
     }
-    __syncthreads();
-    int run = carry + (w > 0 ? wsum[w - 1] : 0) + x - s;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (i0 + q < n) {
-        start[i0 + q] = run;
-        cursor[i0 + q] = run;
-      }
-      run += v[q];
-    }
-    carry += wsum[15];
-    __syncthreads();  // wsum is rewritten next round
+    run += v[q];
   }
 }
+
+inline unsigned scan_blocks(int n) { return (unsigned)((n + 4095) / 4096 > 0 ? (n + 4095) / 4096 : 1); }
 
 // Sort each list segment ascending (segments are short: insertion sort).
 __global__ void sort_segments_kernel(const int *__restrict__ start, const int *__restrict__ count, int *list, int n) {
@@ -971,7 +985,7 @@ int p_build_tiles(spm_ctx *c) {
   const long slots = (long)c->np * k[0] * k[1] * k[2];
   hipLaunchKernelGGL(anchor_count_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
                      c->pxint, c->pres, c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_count);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->tile_count, c->tile_start, c->tile_cursor,
+  hipLaunchKernelGGL(scan_kernel, dim3(scan_blocks(c->ntiles)), dim3(1024), 0, c->stream, c->tile_count, c->tile_start, c->tile_cursor,
                      c->ntiles);
   hipLaunchKernelGGL(tile_fill_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->pm, c->pxint,
                      c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_cursor, c->tile_list);
@@ -1099,7 +1113,7 @@ int p_force(spm_ctx *c) {
     hipLaunchKernelGGL(izero_kernel, dim3(blocks(c->ncells, 256)), dim3(256), 0, c->stream, c->cell_count, c->ncells);
     hipLaunchKernelGGL(cell_count_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf,
                        c->cells[0], c->cells[1], c->cells[2], c->cell_count, c->pcell);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->cell_count, c->cell_start, c->cell_cursor,
+    hipLaunchKernelGGL(scan_kernel, dim3(scan_blocks(c->ncells)), dim3(1024), 0, c->stream, c->cell_count, c->cell_start, c->cell_cursor,
                        c->ncells);
     hipLaunchKernelGGL(cell_fill_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pcell,
                        c->cell_cursor, c->cell_list);
