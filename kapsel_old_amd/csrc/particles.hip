@@ -380,8 +380,19 @@ __global__ __launch_bounds__(256) void f_hydro_kernel(Grid g, PModel pm, double 
   const double vp[3] = {P(pb, pm.np, P_V, n), P(pb, pm.np, P_V + 1, n), P(pb, pm.np, P_V + 2, n)};
   const double wp[3] = {P(pb, pm.np, P_OMEGA, n), P(pb, pm.np, P_OMEGA + 1, n), P(pb, pm.np, P_OMEGA + 2, n)};
   const double rcut = pm.radius + pm.hxi;
+  // single-precision screen as in make_phi: a point whose distance is below
+  // rcut has a float distance below rcut + 1e-3 dx, so the FP64 distance
+  // and the gather run only near the sphere (phi = 0 adds nothing)
+  const float rscreen2 = (float)((rcut + 1e-3 * pm.dx) * (rcut + 1e-3 * pm.dx));
+  const float fres[3] = {(float)res[0], (float)res[1], (float)res[2]};
   double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0;
   for (int s = lane; s < nst; s += 64) {
+    {
+      const float f0 = (float)stencil[3 * s] * (float)pm.dx - fres[0];
+      const float f1 = (float)stencil[3 * s + 1] * (float)pm.dx - fres[1];
+      const float f2 = (float)stencil[3 * s + 2] * (float)pm.dx - fres[2];
+      if (!(f0 * f0 + f1 * f1 + f2 * f2 < rscreen2)) continue;
+    }
     int rm[3];
     double r[3], x[3];
 #pragma unroll
