@@ -96,7 +96,7 @@ def lib():
         L.oracle_distance0_obl.restype = C.c_double
         for name in ["ns_solver_shear_obl", "zeta_k2u_k_obl", "u_k2zeta_k_obl", "make_phi_u_particle_obl",
                      "calc_f_hydro_obl", "md_solver_position_obl", "md_solver_velocity_obl", "force_obl",
-                     "u2u_oblique", "u_oblique2u"]:
+                     "u2u_oblique", "u_oblique2u", "remap_obl"]:
             f = getattr(L, "oracle_" + name)
             f.argtypes = [vp]
             f.restype = ip
@@ -220,6 +220,7 @@ class Oracle:
     def Force_OBL(self): self._call("force_obl")
     def U2u_oblique(self): self._call("u2u_oblique")
     def U_oblique2u(self): self._call("u_oblique2u")
+    def Remap_OBL(self): self._call("remap_obl")
 
     def Distance0_OBL(self, x1, x2):
         a, b = np.array(x1, dtype=np.float64), np.array(x2, dtype=np.float64)

@@ -1990,19 +1990,11 @@ int oracle_u_oblique2u(oracle_ctx *c) {
   return 0;
 }
 
-/* Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390).  Without particles f is
- * zero and the particle block leaves u as it is. */
-static void le_step(oracle_ctx *c) {
-  oracle_ns_solver_shear_obl(c);
-  for (int d = 0; d < DIM; d++)
-    if (c->P.fix_cell[d]) c->uk_dc[d] = 0.0;
-  zeta_k2u_k_obl(c, c->zeta, c->uk_dc, c->u);
-  u_k2u(c, c->u);
-  c->shear_rate_eff = c->shear_rate;
+/* The integer strain (sp_3d_ns.cxx:280-305): once degree_oblique reaches 1
+ * the grid is remapped, u0 += u1 with the rows shifted by j - NY/2, the
+ * copy ucp refreshed and degree_oblique reduced by 1. */
+static void remap_obl(oracle_ctx *c) {
   const int NX = c->NX, NY = c->NY, NZ = c->NZ;
-  for (int d = 0; d < DIM; d++) memcpy(c->ucp[d], c->u[d], sizeof(double) * c->nfield);
-  c->degree_oblique += c->shear_rate_eff * c->dt;
-  if (c->degree_oblique >= 1.) {
     for (int i = 0; i < NX; i++)
       for (int j = 0; j < NY; j++) {
         double sign = j - NY / 2;
@@ -2018,6 +2010,28 @@ static void le_step(oracle_ctx *c) {
       }
     c->degree_oblique -= 1.;
     for (int d = 0; d < DIM; d++) memcpy(c->ucp[d], c->u[d], sizeof(double) * c->nfield);
+}
+
+int oracle_remap_obl(oracle_ctx *c) {
+  if (!c || !c->le) return -1;
+  for (int d = 0; d < DIM; d++) memcpy(c->ucp[d], c->u[d], sizeof(double) * c->nfield);
+  remap_obl(c);
+  return 0;
+}
+
+/* Time_evolution_hydro_OBL (sp_3d_ns.cxx:214-390).  Without particles f is
+ * zero and the particle block leaves u as it is. */
+static void le_step(oracle_ctx *c) {
+  oracle_ns_solver_shear_obl(c);
+  for (int d = 0; d < DIM; d++)
+    if (c->P.fix_cell[d]) c->uk_dc[d] = 0.0;
+  zeta_k2u_k_obl(c, c->zeta, c->uk_dc, c->u);
+  u_k2u(c, c->u);
+  c->shear_rate_eff = c->shear_rate;
+  for (int d = 0; d < DIM; d++) memcpy(c->ucp[d], c->u[d], sizeof(double) * c->nfield);
+  c->degree_oblique += c->shear_rate_eff * c->dt;
+  if (c->degree_oblique >= 1.) {
+    remap_obl(c);
   }
   u_oblique2u(c, c->ucp);
   obl_update_k2(c);

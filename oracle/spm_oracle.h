@@ -98,6 +98,7 @@ int oracle_md_solver_velocity_obl(oracle_ctx *c);
 int oracle_force_obl(oracle_ctx *c);
 int oracle_u2u_oblique(oracle_ctx *c);
 int oracle_u_oblique2u(oracle_ctx *c);
+int oracle_remap_obl(oracle_ctx *c); /* the integer-strain remap of the real-space u */
 double oracle_distance0_obl(oracle_ctx *c, const double *x1, const double *x2, double *x12);
 int oracle_ns_solver_shear_obl(oracle_ctx *c);
 int oracle_zeta_k2u_k_obl(oracle_ctx *c);

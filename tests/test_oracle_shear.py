@@ -440,3 +440,38 @@ def test_sheared_lj_diameter(a_xi, want_sigma):
         mag = 24 * 0.7 / r ** 2 * (2 * s6 * s6 - s6) if r < 2 ** (1 / 6) * want_sigma else 0.0
         np.testing.assert_allclose(fr[0], -mag * r * np.array([0.6, 0.8, 0.0]), rtol=1e-12, atol=1e-15)
         np.testing.assert_allclose(fr[1], -fr[0], rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("grid", [(16, 16, 8), (8, 32, 8)])
+def test_remap_keeps_the_cartesian_flow(grid):
+    """The remap at degree_oblique = 1 (sp_3d_ns.cxx:280-305) changes the
+    grid and the oblique components but not the flow: U_oblique2u of the
+    field before it, at degree_oblique 1, equals U_oblique2u of the
+    remapped field at degree_oblique 0 (both interpolations are whole-row
+    shifts there)."""
+    prm = abi.make_params(*grid, dt=0.05)
+    rng = np.random.default_rng(8)
+    shape = (grid[0], grid[1], 2 * (grid[2] // 2 + 1))
+    u = [rng.standard_normal(shape) for _ in range(3)]
+    cart = []
+    for remap in (False, True):
+        o = Oracle(prm)
+        o.set_lees_edwards(0.7)
+        o.put_oblique(1.0, 0.7)
+        for d in range(3):
+            o.put_field("u%d" % d, u[d])
+        if remap:
+            o.Remap_OBL()
+            assert o.get_oblique()[0] == 0.0
+        o.U_oblique2u()
+        cart.append([o.get_field("u%d" % d)[:, :, :grid[2]] for d in range(3)])
+    for d in range(3):
+        np.testing.assert_allclose(cart[1][d], cart[0][d], rtol=0, atol=1e-13)
+    # and the remap is not the identity: the oblique components did change
+    o = Oracle(prm)
+    o.set_lees_edwards(0.7)
+    o.put_oblique(1.0, 0.7)
+    for d in range(3):
+        o.put_field("u%d" % d, u[d])
+    o.Remap_OBL()
+    assert np.abs(o.get_field("u0")[:, :, :grid[2]] - u[0][:, :, :grid[2]]).max() > 0.5
