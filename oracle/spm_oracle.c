@@ -1671,13 +1671,23 @@ static void u_oblique2u(oracle_ctx *c, double *const uu[3]) {
     }
 }
 
-/* Calc_shear_rate_eff, sp_3d_ns.h:174-196 (ivolume = Ivolume DX^3) */
+/* Calc_shear_rate_eff, sp_3d_ns.h:174-196 (ivolume = Ivolume DX^3).  The
+ * reference sums the NX (NY-1) NZ differences in an OpenMP reduction, whose
+ * order is not fixed; a plain loop over 10^6 and more terms carries a
+ * rounding of up to ~1e-10 relative, so the oracle adds them with Kahan
+ * compensation, which is within rounding of the exact sum that every order
+ * approximates. */
 static void calc_shear_rate_eff(oracle_ctx *c) {
   const double ivolume = (1. / (c->L[0] * c->L[1] * c->L[2])) * POW3(c->DX);
-  double s = 0.0;
+  double s = 0.0, comp = 0.0;
   for (int i = 0; i < c->NX; i++)
     for (int j = 0; j < c->NY - 1; j++)
-      for (int k = 0; k < c->NZ; k++) s += (c->ucp[0][IDX(c, i, j + 1, k)] - c->ucp[0][IDX(c, i, j, k)]) / c->DX;
+      for (int k = 0; k < c->NZ; k++) {
+        const double y = (c->ucp[0][IDX(c, i, j + 1, k)] - c->ucp[0][IDX(c, i, j, k)]) / c->DX - comp;
+        const double t = s + y;
+        comp = (t - s) - y;
+        s = t;
+      }
   s *= ivolume * c->NY / (c->NY - 1);
   c->shear_rate_eff = s;
 }

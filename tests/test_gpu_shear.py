@@ -437,3 +437,13 @@ def test_driver_lees_edwards_with_particles(tmp_path):
     # resumed runs do); only the run itself and the clock are checked
     assert np.isfinite(b["zeta0"]).all() and np.isfinite(b["particles"]["v"]).all()
     assert b["time"] == pytest.approx(ref["time"], rel=1e-15)
+
+
+def test_steps_with_particles_128cube(SPM):
+    """128^3, 40 spheres of radius 5 (five across the y faces), thermostat on,
+    two sheared steps against the oracle."""
+    objs, p = _le_pair(SPM, (128, 128, 128), 40, 5.0, deg=0.9, sre=0.02, rate=0.02, near_y=5, kbt=1e-4)
+    for o in objs:
+        o.step(2)
+    compare_state(objs, "128^3 particles")
+    _cmp_particles(objs, ("x", "v", "omega"), "128^3 particles")
