@@ -118,12 +118,24 @@ __global__ void zeta2u_obl_kernel(Grid g, const double *__restrict__ z0, const d
   u2[m.im + 1] = ui[2];
 }
 
-// Zeta_k2omega_k_OBL (fft_wrapper.cxx:586-632): 32 B read, 48 B written
+// Zeta_k2u_k_OBL (mode 0: uk_dc, real part) and Zeta_k2omega_k_OBL
+// (fft_wrapper.cxx:513-632) from one read of zeta: 32 B read, 96 B written
 // per mode; omega(0) = 0.
-__global__ void zeta2omega_obl_kernel(Grid g, const double *__restrict__ z0, const double *__restrict__ z1,
-                                      double *w0, double *w1, double *w2) {
+__global__ void zeta2uw_obl_kernel(Grid g, const double *__restrict__ z0, const double *__restrict__ z1, double *u0,
+                                   double *u1, double *u2, double *w0, double *w1, double *w2,
+                                   const double *__restrict__ uk_dc, double deg) {
   ModeO m;
   if (!mode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m)) return;
+  double ur[3], ui[3];
+  zeta2u_mode(g, m, deg, z0, z1, ur, ui);
+  if (m.im == 0)
+    for (int d = 0; d < 3; ++d) ur[d] = uk_dc[d];
+  u0[m.im] = ur[0];
+  u0[m.im + 1] = ui[0];
+  u1[m.im] = ur[1];
+  u1[m.im + 1] = ui[1];
+  u2[m.im] = ur[2];
+  u2[m.im + 1] = ui[2];
   const double kx = g.wave_x * m.kx, ky = g.wave_y * m.ky, kz = g.wave_z * m.kz;
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
@@ -399,11 +411,10 @@ int le_ns_solver(spm_ctx *c) {
   RCS(k_truncate(c, c->zeta[1]));
   double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
   double *const w3[3] = {c->f[0], c->f[1], c->f[2]};
-  RCS(le_zeta2u_k(c, u3, 0));
   tbeg(c);
-  hipLaunchKernelGGL(zeta2omega_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, c->zeta[0],
-                     c->zeta[1], w3[0], w3[1], w3[2]);
-  tend(c, SPM_K_SPECTRAL, 80.0 * nmodes(c));
+  hipLaunchKernelGGL(zeta2uw_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, g, c->zeta[0],
+                     c->zeta[1], u3[0], u3[1], u3[2], w3[0], w3[1], w3[2], c->uk_dc, deg);
+  tend(c, SPM_K_SPECTRAL, 128.0 * nmodes(c));
   SPM_LAUNCH_CHECK(c);
   // zeta is truncated, so u and omega vanish outside the 2/3 box: x and y
   // passes, then one z pass that also forms u x omega in real space, then
