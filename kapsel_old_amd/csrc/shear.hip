@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void shear_rate_final_kernel(const double *
 // U_k2zeta_k_OBL (fft_wrapper.cxx:341-406), with uk_dc = co2contra of the
 // k = 0 velocity.  48 B read, 80 B written per mode.
 __global__ void project_zeta_obl_kernel(Grid g, double *u0, double *u1, double *u2, double *z0, double *z1,
-                                        double *uk_dc, double deg) {
+                                        double *uk_dc, double deg, int store_u) {
   ModeO m;
   if (!mode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m)) return;
   const double kx = m.kx * g.wave_x, ky = m.ky * g.wave_y, kz = m.kz * g.wave_z;
@@ -341,7 +341,7 @@ __global__ void project_zeta_obl_kernel(Grid g, double *u0, double *u1, double *
     v[2] -= dmy * kz;
     contra2co1(deg, v);
     for (int d = 0; d < 3; ++d) {
-      u[d][im] = v[d];
+      if (store_u) u[d][im] = v[d];
       (s == 0 ? re : iv)[d] = v[d];
     }
   }
@@ -392,11 +392,11 @@ int le_zeta2u_k(spm_ctx *c, double *const u[3], int fix_mask) {
 }
 
 // Solenoidal_uk_OBL + contra2co + U_k2zeta_k_OBL of the spectra in c->u.
-int le_u_k2zeta_k(spm_ctx *c) {
+int le_u_k2zeta_k(spm_ctx *c, bool store_u) {
   tbeg(c);
   hipLaunchKernelGGL(project_zeta_obl_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, c->grid, c->u[0],
-                     c->u[1], c->u[2], c->zeta[0], c->zeta[1], c->uk_dc, c->le_deg);
-  tend(c, SPM_K_SPECTRAL, 128.0 * nmodes(c));
+                     c->u[1], c->u[2], c->zeta[0], c->zeta[1], c->uk_dc, c->le_deg, store_u ? 1 : 0);
+  tend(c, SPM_K_SPECTRAL, (store_u ? 128.0 : 80.0) * nmodes(c));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }
@@ -446,7 +446,7 @@ int le_oblique2u(spm_ctx *c, double *const ucp[3]) {
 // of u held in up[0..2]; with particles the Cartesian velocity ucp lives in
 // f[0..2].  Without particles f is zero, so U2u_oblique and Add_f_particle
 // leave u unchanged and the particle block is skipped.
-int le_step(spm_ctx *c, int fix_mask) {
+int le_step(spm_ctx *c, int fix_mask, bool store_u) {
   const Grid &g = c->grid;
   RCS(le_ns_solver(c));
   double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
@@ -488,7 +488,7 @@ int le_step(spm_ctx *c, int fix_mask) {
     RCS(p_f_add_obl(c, ucp));
   }
   RCS(fft_r2c_many(c, u3, 3));
-  return le_u_k2zeta_k(c);
+  return le_u_k2zeta_k(c, store_u);
 }
 
 }  // namespace spm

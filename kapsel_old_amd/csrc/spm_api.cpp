@@ -1085,7 +1085,7 @@ static int step_n(spm_ctx *c, int nsteps) {
     if (c->electro)
       RC(electrolyte_step(c));
     else if (c->le)
-      RC(le_step(c, fix_mask(c)));
+      RC(le_step(c, fix_mask(c), s == nsteps - 1));
     else
       RC(fused ? fused_step(c) : time_evolution_hydro(c));
     c->time += c->prm.dt;

@@ -307,9 +307,11 @@ int electrolyte_step(spm_ctx *c);
 // Lees-Edwards (shear.hip)
 int le_red_slots();
 int le_zeta2u_k(spm_ctx *c, double *const u[3], int fix_mask);
-int le_u_k2zeta_k(spm_ctx *c);
+// store_u: leave u holding the covariant spectrum, as the reference does
+// (the next step does not read it, so only the last step of a call stores it)
+int le_u_k2zeta_k(spm_ctx *c, bool store_u = true);
 int le_ns_solver(spm_ctx *c);
-int le_step(spm_ctx *c, int fix_mask);
+int le_step(spm_ctx *c, int fix_mask, bool store_u = true);
 
 // electrolyte kernels (electrolyte.hip)
 int e_red_slots();

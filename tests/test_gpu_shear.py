@@ -116,6 +116,9 @@ def test_steps_64cube(SPM):
     for o in objs:
         o.step(3)
     compare_state(objs, "64^3 3 steps")
+    # u is left holding the covariant spectrum of the last step, as the reference leaves it
+    for d in range(3):
+        check("64^3 u%d after the call" % d, objs[1].get_field("u%d" % d), objs[0].get_field("u%d" % d))
     assert objs[0].get_oblique()[0] == objs[1].get_oblique()[0]
 
 
@@ -446,4 +449,6 @@ def test_steps_with_particles_128cube(SPM):
     for o in objs:
         o.step(2)
     compare_state(objs, "128^3 particles")
+    for d in range(3):
+        check("128^3 u%d after the call" % d, objs[1].get_field("u%d" % d), objs[0].get_field("u%d" % d))
     _cmp_particles(objs, ("x", "v", "omega"), "128^3 particles")
