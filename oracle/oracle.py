@@ -357,6 +357,12 @@ class Oracle:
 _ref = None
 
 
+def ref_available():
+    """Whether oracle/_ref/libref.so was built (checks the file only, so a
+    skip condition evaluated at collection loads nothing)."""
+    return os.path.exists(REF_PATH)
+
+
 def ref_lib():
     """Handle on oracle/_ref/libref.so, or None where it was not built."""
     global _ref
@@ -372,6 +378,18 @@ def ref_lib():
         L.init_genrand.restype = None
         L.genrand_int32.restype = C.c_ulong
         L.genrand_real3.restype = C.c_double
+        dp, vp = C.POINTER(C.c_double), C.c_void_p
+        for name in ["ref_orientation_euler", "ref_orientation_ab2"]:
+            getattr(L, name).argtypes = [vp, C.c_double]
+            getattr(L, name).restype = None
+        L.ref_rigid_body_rotation.argtypes = [dp, dp, dp, C.c_int]
+        L.ref_rigid_body_rotation.restype = None
+        L.ref_qdot.argtypes = [dp, dp, dp, C.c_int]
+        L.ref_qdot.restype = None
+        L.ref_qtn_normalize.argtypes = [dp]
+        L.ref_qtn_normalize.restype = None
+        L.ref_particle_layout.argtypes = [C.POINTER(C.c_size_t), C.c_int]
+        L.ref_particle_layout.restype = C.c_int
         _ref = L
     return _ref
 

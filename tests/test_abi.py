@@ -8,6 +8,7 @@ import subprocess
 import pytest
 
 from kapsel_old_amd import abi
+from oracle.oracle import ref_available, ref_lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "spm.h")
@@ -43,6 +44,34 @@ def test_header_layout_matches_ctypes(tmp_path):
     for f, v in zip(["nx", "dx", "n_particles", "mass_ratio", "rotation", "fix_cell", "seed", "precision",
                      "device"], vals[2 + len(fields):]):
         assert getattr(abi.SpmParams, f).offset == v, f
+
+
+PARTICLE_MEMBERS = [f[0] for f in abi.SpmParticle._fields_]
+
+
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref (reference variable.h build) absent")
+def test_particle_layout_matches_reference_variable_h(tmp_path):
+    """Every member offset of spm_particle (include/spm.h) and of the ctypes
+    mirror equals offsetof() of the reference's own Particle (variable.h:27-75)
+    as compiled from the reference tree into oracle/_ref."""
+    L = ref_lib()
+    buf = (C.c_size_t * 64)()
+    n = L.ref_particle_layout(buf, 64)
+    assert n == 1 + len(PARTICLE_MEMBERS)
+    ref_size, ref_off = buf[0], list(buf[1:n])
+    prog = tmp_path / "offs.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){",
+             'printf("%zu\\n", sizeof(spm_particle));']
+    lines += ['printf("%%zu\\n", offsetof(spm_particle, %s));' % m for m in PARTICLE_MEMBERS]
+    lines.append("return 0;}")
+    prog.write_text("\n".join(lines))
+    exe = tmp_path / "offs"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(prog)], check=True)
+    hdr = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert hdr[0] == ref_size == C.sizeof(abi.SpmParticle) == 1000
+    for m, r, h in zip(PARTICLE_MEMBERS, ref_off, hdr[1:]):
+        assert h == r, m
+        assert getattr(abi.SpmParticle, m).offset == r, m
 
 
 def test_library_exports_every_declared_symbol():

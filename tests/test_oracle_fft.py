@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from kapsel_old_amd import abi
-from oracle.oracle import Oracle, ref_A2a_k, ref_A_k2a, ref_lib
+from oracle.oracle import Oracle, ref_A2a_k, ref_A_k2a, ref_available
 from tests.helpers import real_field
 
 GRIDS = [(8, 8, 8), (16, 8, 32), (32, 32, 32), (64, 16, 8)]
@@ -33,7 +33,7 @@ def test_forward_matches_numpy(grid):
     assert np.abs(k - ref).max() / np.abs(ref).max() < 1e-14
 
 
-@pytest.mark.skipif(ref_lib() is None, reason="oracle/_ref (reference Ooura build) absent")
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref (reference Ooura build) absent")
 @pytest.mark.parametrize("grid", GRIDS)
 def test_forward_and_inverse_match_reference_ooura(grid):
     nz = grid[2]
@@ -93,7 +93,7 @@ def test_single_mode_sign_convention():
     assert abs(k[0, 0, 7] - n ** 3 / 2) < 1e-9 and abs(k[0, 0, 6]) < 1e-9
 
 
-@pytest.mark.skipif(ref_lib() is None, reason="oracle/_ref (reference Ooura build) absent")
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref (reference Ooura build) absent")
 @pytest.mark.parametrize("grid", GRIDS)
 def test_inverse_of_a_non_hermitian_spectrum_matches_reference_ooura(grid):
     """A derivative's Nyquist modes make a spectrum that is not Hermitian on

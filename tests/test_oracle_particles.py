@@ -14,12 +14,15 @@
   Add_f_gravity (md_force.cxx:181-191)     buoyancy-corrected weight
   MD solvers (particle_solver.cxx)         AB2 exact for linear velocity;
                                            quaternion rotation about an axis
+  MD_solver_orientation_Euler / _AB2       bit for bit against the reference's
+    (particle_rotation_solver.h:15-39)     own rigid_body.cxx / quaternion.h
+                                           build on random non-coaxial states
 """
 import numpy as np
 import pytest
 
 from kapsel_old_amd import abi
-from oracle.oracle import Oracle
+from oracle.oracle import Oracle, ref_available, ref_lib
 from tests.helpers import particles
 
 
@@ -216,3 +219,75 @@ def test_quaternion_rotation_second_order(axis):
         errs.append(np.abs(got - exact).max())
     assert errs[0] < 1e-3
     assert 3.0 < errs[0] / errs[1] < 5.0  # second order
+
+
+def _random_unit_quaternion(rng):
+    q = rng.standard_normal(4)
+    return q / np.linalg.norm(q)
+
+
+def _set_q(rec, q4):
+    rec["s"] = q4[0]
+    rec["v"] = q4[1:]
+
+
+def _q_of(rec):
+    return np.array([rec["s"], *rec["v"]])
+
+
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref (reference rigid_body.cxx build) absent")
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_orientation_solvers_match_reference_build(seed):
+    """MD_solver_orientation_Euler (step 0, dt) and _AB2 (later steps, dt/2)
+    of particle_rotation_solver.h:15-39, run by the reference's own
+    rigid_body.cxx / quaternion.h, on random non-coaxial q, q_old, omega and
+    omega_old: the oracle's q and q_old agree bit for bit."""
+    L = ref_lib()
+    rng = np.random.default_rng(seed)
+    dt = 0.05
+    for ts, ref_fn, h in [(0, L.ref_orientation_euler, dt), (7, L.ref_orientation_ab2, dt / 2)]:
+        o = Oracle(abi.make_params(32, a=4.0, n_particles=1, rotation=1, dt=dt))
+        p = particles(1, (32, 32, 32), 4.0, positions=np.array([[16.0, 16.0, 16.0]]))
+        _set_q(p["q"][0], _random_unit_quaternion(rng))
+        _set_q(p["q_old"][0], _random_unit_quaternion(rng))
+        p["omega"][0] = rng.standard_normal(3)
+        p["omega_old"][0] = rng.standard_normal(3)
+        ref = p.copy()
+        o.put_particles(p)
+        o.put_time(ts, ts * dt)
+        o.MD_solver_position()
+        got = o.get_particles()[0]
+        ref_fn(ref.ctypes.data, h)
+        assert np.array_equal(_q_of(got["q"]), _q_of(ref[0]["q"])), ts
+        assert np.array_equal(_q_of(got["q_old"]), _q_of(ref[0]["q_old"])), ts
+        # the step moved q: a frame or product-order slip would show here
+        assert np.abs(_q_of(got["q"]) - _q_of(p[0]["q"])).max() > 1e-3
+
+
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref (reference rigid_body.cxx build) absent")
+def test_orientation_ab2_frame_and_order_are_observable():
+    """The reference build itself separates the frames: qdot in the body
+    frame (q (0,w)) differs from the space frame ((0,w) q) for a general q,
+    and SPACE2BODY inverts BODY2SPACE; so the bit-for-bit agreement above
+    pins the product order and the frame of the oracle's AB2 step."""
+    import ctypes as C
+    L = ref_lib()
+    rng = np.random.default_rng(9)
+    q = _random_unit_quaternion(rng)
+    w = rng.standard_normal(3)
+    dp = C.POINTER(C.c_double)
+    out_b, out_s = np.zeros(4), np.zeros(4)
+    L.ref_qdot(out_b.ctypes.data_as(dp), q.ctypes.data_as(dp), w.ctypes.data_as(dp), 0)
+    L.ref_qdot(out_s.ctypes.data_as(dp), q.ctypes.data_as(dp), w.ctypes.data_as(dp), 1)
+    assert np.abs(out_b - out_s).max() > 1e-2
+    v = rng.standard_normal(3)
+    sp, back = np.zeros(3), np.zeros(3)
+    L.ref_rigid_body_rotation(sp.ctypes.data_as(dp), v.ctypes.data_as(dp), q.ctypes.data_as(dp), 0)
+    L.ref_rigid_body_rotation(back.ctypes.data_as(dp), sp.ctypes.data_as(dp), q.ctypes.data_as(dp), 1)
+    assert np.allclose(back, v, rtol=0, atol=1e-14)
+    # BODY2SPACE is the rotation matrix of q (closed form) applied to v
+    s, x, y, z = q
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - s * z), 2 * (x * z + s * y)],
+                  [2 * (x * y + s * z), 1 - 2 * (x * x + z * z), 2 * (y * z - s * x)],
+                  [2 * (x * z - s * y), 2 * (y * z + s * x), 1 - 2 * (x * x + y * y)]])
+    assert np.allclose(sp, R @ v, rtol=0, atol=1e-14)

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from kapsel_old_amd import abi
-from oracle.oracle import Oracle, ref_lib
+from oracle.oracle import Oracle, ref_available, ref_lib
 
 
 def oracle_with_seed(seed):
@@ -30,7 +30,7 @@ def test_numpy_mt19937_state_agrees():
     assert np.array_equal(ours, theirs)
 
 
-@pytest.mark.skipif(ref_lib() is None, reason="oracle/_ref (reference mt19937ar.c build) absent")
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref (reference mt19937ar.c build) absent")
 @pytest.mark.parametrize("seed", [0, 12345, 4357])
 def test_matches_reference_mt19937ar(seed):
     L = ref_lib()

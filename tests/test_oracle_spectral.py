@@ -9,7 +9,9 @@ convention, mapped to the reference's exp(+i k.x) spectra).
   Solenoidal_uk    (operate_omega.cxx:186)   k . u = 0, idempotent
   U2advection_k    (operate_omega.cxx:14)    = curl(u x omega); 0 for ABC flow
   NS_solver_slavedEuler (fluid_solver.cxx:328) exact viscous decay of a
-                                             shear mode
+                                             shear mode; with advection live,
+                                             the slaved-Euler closed form
+                                             e w + (1 - e) curl(u x w)/(nu k^2)
 """
 import numpy as np
 import pytest
@@ -196,3 +198,54 @@ def test_slaved_euler_exact_shear_decay():
     assert np.abs(z0).max() + np.abs(z1).max() > 1.0
     assert np.abs(o.get_field("zeta0") - decay * z0).max() < 1e-12 * np.abs(z0).max() + 1e-14
     assert np.abs(o.get_field("zeta1") - decay * z1).max() < 1e-12 * (np.abs(z1).max() + 1)
+
+
+def _slaved_euler_closed_form(n, dt, eta, rho, u, seed_flip=False):
+    """One slaved-Euler step of the vorticity equation, written from the
+    mathematics: d(omega)/dt = curl(u x omega) + nu lap(omega), integrated
+    with the linear part exact over dt and the nonlinear term N held fixed,
+    omega' = e omega + (1 - e) N / (nu k^2), e = exp(-nu k^2 dt); omega and N
+    reduced to the reference's two components (fft_wrapper.cxx:298-313)."""
+    w = curl(u, n)
+    uxw = [u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]]
+    N = curl(uxw, n)
+    KX, KY, KZ = kvecs(n)
+    z_w = zeta_components([np.fft.rfftn(c) for c in w], KX, KY)
+    z_n = zeta_components([np.fft.rfftn(c) for c in N], KX, KY)
+    k2 = (2 * np.pi / n) ** 2 * (KX ** 2 + KY ** 2 + KZ ** 2).astype(float)
+    nu = eta / rho
+    e = np.exp(-nu * k2 * dt)
+    k2[0, 0, 0] = 1.0
+    slave = np.where(KX ** 2 + KY ** 2 + KZ ** 2 > 0, (1 - e) / (nu * k2), 0.0)
+    sign = -1.0 if seed_flip else 1.0
+    # in numpy's exp(-ik.x) convention, as ooura_to_np returns the oracle's
+    return [e * zw + sign * slave * zn for zw, zn in zip(z_w, z_n)]
+
+
+def test_slaved_euler_advection_term_closed_form():
+    """NS_solver_slavedEuler (fluid_solver.cxx:328-354) with the advection
+    term live: a random band-limited solenoidal flow plus a mean flow
+    (uk_dc), kept whole by the 2/3 rule (|k| <= 5 < TRN = 6 at 16^3), so the
+    products are alias-free on the kept modes and the numpy curl(u x omega)
+    is exact there.  The step must equal the closed form on every kept mode
+    and leave the rest zero; the same closed form with the sign of the
+    advection term flipped must not match."""
+    n, dt, eta, rho = 16, 0.05, 0.7, 1.3
+    o = Oracle(abi.make_params(n, dt=dt, eta=eta, rho=rho, fix_cell=(0, 0, 0)))
+    mean = np.array([0.3, -0.2, 0.15])
+    u = [1.5 * c + mean[d] for d, c in enumerate(band_limited_solenoidal(n, 5, seed=11))]
+    for d in range(3):
+        o.put_field("u%d" % d, to_layout(u[d], o.shape, n))
+    o.U2zeta_k()
+    assert np.allclose(o.get_uk_dc(), [c.sum() for c in u], rtol=1e-12)
+    o.NS_solver_slavedEuler()
+    got = [ooura_to_np(o.get_field("zeta%d" % c)) for c in range(2)]
+    want = _slaved_euler_closed_form(n, dt, eta, rho, u)
+    flip = _slaved_euler_closed_form(n, dt, eta, rho, u, seed_flip=True)
+    keep = ooura_to_np(o.truncate(np.ones(o.shape))) != 0
+    scale = max(np.abs(w[keep]).max() for w in want)
+    for c in range(2):
+        assert np.abs(got[c][keep] - want[c][keep]).max() < 1e-11 * scale
+        assert np.all(got[c][~keep] == 0)
+    # the advection term carries weight here: flipping its sign is caught
+    assert max(np.abs(got[c][keep] - flip[c][keep]).max() for c in range(2)) > 1e-2 * scale
