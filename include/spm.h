@@ -29,6 +29,7 @@
 #ifndef SPM_H
 #define SPM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -223,6 +224,25 @@ int spm_set_schedule(spm_ctx *ctx, int schedule);
 int spm_fused_stage(spm_ctx *ctx, int stage);
 /* Scans u and zeta for NaN/Inf; SPM_ERR_NONFINITE if any is found. */
 int spm_check_finite(spm_ctx *ctx);
+
+/* ---- transforms of a caller's field, without a context -----------------
+ * A2a_k (direction +1, fft_wrapper.h:432-527) and A_k2a (direction -1,
+ * fft_wrapper.h:534-631) take a bare `double *` in the reference and
+ * allocate a work copy on every call (:437,461,540,566).  spm_fft3d
+ * transforms, in place, one FP64 field that lives in device memory in the
+ * reference layout [NX][NY][NZ/2+1 complex] (real data in k < NZ): +1 is
+ * the forward transform with the Ooura sign exp(+i k.x), -1 the inverse
+ * with the 1/(NX NY NZ) scale and rdft3dsort's reading of the kz = 0 and
+ * kz = NZ/2 planes.  Sizes are powers of two in [8, 1024].  `workspace`
+ * (device memory of at least spm_fft_workspace_size() bytes, rewritten by
+ * every call) holds the roots of unity; with workspace == NULL the call
+ * allocates one, waits for `stream` and frees it.  `stream` is a
+ * hipStream_t (NULL: the default stream); with a workspace the call only
+ * queues work on it.  Errors: SPM_ERR_INVALID for bad sizes, a null field
+ * or a short workspace (text in spm_last_error(NULL)). */
+size_t spm_fft_workspace_size(int nx, int ny, int nz);
+int spm_fft3d(int nx, int ny, int nz, int direction, double *field, void *workspace, size_t workspace_bytes,
+              void *stream);
 
 /* ---- checkpoint / resume ------------------------------------------------
  * Replace Save_Restart_udf / Force_restore_parameters (resume.cxx:10-364)

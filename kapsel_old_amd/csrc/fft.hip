@@ -463,4 +463,145 @@ int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign) { return fft_pass(
 int fft_pass_z_r2c(spm_ctx *c, double *const *a, int n) { return fft_pass(c, a, n, 1, +1); }
 int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n) { return fft_pass(c, a, n, 2, -1); }
 
+// ---------------------------------------------------------------------
+// Transforms of one caller-owned FP64 field with no context (the
+// reference's A2a_k / A_k2a are free functions on a caller's array,
+// fft_wrapper.h:432-631).  The roots of unity live in a workspace that
+// every call rewrites: four tables, each starting on a 256-byte boundary,
+//   x: exp(+2 pi i t/NX), t < NX      y: exp(+2 pi i t/NY), t < NY
+//   z: exp(+2 pi i t/(NZ/2)), t < NZ/2     rz: exp(+2 pi i k/NZ), k <= NZ/2
+// computed on the device (sincospi of an exact dyadic argument), so the
+// call queues kernels only.
+// ---------------------------------------------------------------------
+namespace {
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct RootTables {
+  size_t off[4];  // byte offsets of x, y, z, rz
+  int len[4];
+  size_t bytes;
+};
+
+RootTables root_tables(int nx, int ny, int nz) {
+  RootTables t{};
+  t.len[0] = nx;
+  t.len[1] = ny;
+  t.len[2] = nz / 2;
+  t.len[3] = nz / 2 + 1;
+  size_t o = 0;
+  for (int q = 0; q < 4; ++q) {
+    t.off[q] = o;
+    o += align256(sizeof(double2) * t.len[q]);
+  }
+  t.bytes = o;
+  return t;
+}
+
+bool fft_size_ok(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+}  // namespace
+
+// one thread per table entry over the four tables
+__global__ void roots_kernel(char *ws, RootTables t, int nx, int ny, int nz) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = 0, e = g;
+  while (q < 4 && e >= t.len[q]) e -= t.len[q++];
+  if (q >= 4) return;
+  const int n = q == 0 ? nx : (q == 1 ? ny : (q == 2 ? nz / 2 : nz));
+  double sn, cs;
+  sincospi(2.0 * (double)e / (double)n, &sn, &cs);
+  reinterpret_cast<double2 *>(ws + t.off[q])[e] = double2{cs, sn};
+}
+
+size_t fft3d_workspace_bytes(int nx, int ny, int nz) {
+  if (!fft_size_ok(nx) || !fft_size_ok(ny) || !fft_size_ok(nz)) return 0;
+  return root_tables(nx, ny, nz).bytes;
+}
+
+// Queues one transform of `a` on `st` using the roots in `ws` (at least
+// fft3d_workspace_bytes bytes of device memory).  direction +1: A2a_k,
+// -1: A_k2a.
+int fft3d_raw(int nx, int ny, int nz, int direction, double *a, void *ws, hipStream_t st) {
+  if (!fft_size_ok(nx) || !fft_size_ok(ny) || !fft_size_ok(nz)) {
+    global_error() = "transform sizes must be powers of two in [8, 1024]";
+    return SPM_ERR_INVALID;
+  }
+  if (!a || !ws || (direction != 1 && direction != -1)) {
+    global_error() = "null field or workspace, or direction not +1 / -1";
+    return SPM_ERR_INVALID;
+  }
+  const RootTables t = root_tables(nx, ny, nz);
+  char *w = static_cast<char *>(ws);
+  const int nent = t.len[0] + t.len[1] + t.len[2] + t.len[3];
+  hipLaunchKernelGGL(roots_kernel, dim3((nent + 255) / 256), dim3(256), 0, st, w, t, nx, ny, nz);
+  const double2 *tw_x = reinterpret_cast<const double2 *>(w + t.off[0]);
+  const double2 *tw_y = reinterpret_cast<const double2 *>(w + t.off[1]);
+  const double2 *tw_z = reinterpret_cast<const double2 *>(w + t.off[2]);
+  const double2 *tw_rz = reinterpret_cast<const double2 *>(w + t.off[3]);
+  FieldPtrs<double> fp{};
+  fp.p[0] = a;
+  const int hnz = nz / 2 + 1, nz_ = 2 * hnz, nrows = nx * ny;
+  int rc = 0;
+  if (direction > 0) {
+    rc |= launch_z<double>(nz / 2, true, fp, 1, tw_z, tw_rz, nrows, nz_, 1.0, st);
+    rc |= launch_lines<double, +1>(ny, fp, 1, tw_y, hnz, hnz, (long)ny * hnz, nx, st);
+    rc |= launch_lines<double, +1>(nx, fp, 1, tw_x, ny * hnz, (long)ny * hnz, 0, 1, st);
+  } else {
+    hipLaunchKernelGGL((ooura_planes_kernel<double>), dim3((unsigned)((2L * nx * ny + 255) / 256), 1), dim3(256), 0,
+                       st, fp, nx, ny, nz_, nz / 2);
+    rc |= launch_lines<double, -1>(nx, fp, 1, tw_x, ny * hnz, (long)ny * hnz, 0, 1, st);
+    rc |= launch_lines<double, -1>(ny, fp, 1, tw_y, hnz, hnz, (long)ny * hnz, nx, st);
+    rc |= launch_z<double>(nz / 2, false, fp, 1, tw_z, tw_rz, nrows, nz_, 1.0 / ((double)nx * ny * nz), st);
+  }
+  if (rc) {
+    global_error() = "unsupported transform size";
+    return SPM_ERR_UNSUPPORTED;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    global_error() = std::string("kernel launch: ") + hipGetErrorString(e);
+    return SPM_ERR_HIP;
+  }
+  return 0;
+}
+
 }  // namespace spm
+
+extern "C" {
+
+size_t spm_fft_workspace_size(int nx, int ny, int nz) { return spm::fft3d_workspace_bytes(nx, ny, nz); }
+
+int spm_fft3d(int nx, int ny, int nz, int direction, double *field, void *workspace, size_t workspace_bytes,
+              void *stream) {
+  spm::global_error().clear();
+  const size_t need = spm::fft3d_workspace_bytes(nx, ny, nz);
+  if (need == 0) {
+    spm::global_error() = "transform sizes must be powers of two in [8, 1024]";
+    return SPM_ERR_INVALID;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (workspace) {
+    if (workspace_bytes < need) {
+      spm::global_error() = "workspace smaller than spm_fft_workspace_size()";
+      return SPM_ERR_INVALID;
+    }
+    return spm::fft3d_raw(nx, ny, nz, direction, field, workspace, st);
+  }
+  // no workspace given: allocate one for this call and free it once the
+  // stream has finished with it
+  void *ws = nullptr;
+  hipError_t e = hipMalloc(&ws, need);
+  if (e != hipSuccess) {
+    spm::global_error() = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return SPM_ERR_OOM;
+  }
+  int rc = spm::fft3d_raw(nx, ny, nz, direction, field, ws, st);
+  e = hipStreamSynchronize(st);
+  if (!rc && e != hipSuccess) {
+    spm::global_error() = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+    rc = SPM_ERR_HIP;
+  }
+  (void)hipFree(ws);
+  return rc;
+}
+
+}  // extern "C"

@@ -828,11 +828,15 @@ int le_zeta_k2u_k_api(spm_ctx *c) {
   return le_zeta2u_k(c, u3, 0);
 }
 
-thread_local std::string g_create_err;
-
 }  // namespace
 
 namespace spm {
+// text of the last error of a call without a context (spm_create, the
+// stateless transforms), per thread; spm_last_error(NULL) returns it
+std::string &global_error() {
+  static thread_local std::string s;
+  return s;
+}
 int electrolyte_step(spm_ctx *c) { return electrolyte_step_impl(c); }
 int host_thermostat(spm_ctx *c) { return thermostat(c); }
 }  // namespace spm
@@ -855,43 +859,43 @@ const char *spm_strerror(int status) {
   }
 }
 
-const char *spm_last_error(const spm_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+const char *spm_last_error(const spm_ctx *ctx) { return ctx ? ctx->err.c_str() : spm::global_error().c_str(); }
 
 int spm_create(const spm_params *prm, spm_ctx **out) {
   if (!prm || !out) return SPM_ERR_INVALID;
   *out = nullptr;
-  g_create_err.clear();
+  spm::global_error().clear();
   const spm_params &p = *prm;
   if (!is_pow2(p.nx, 8, 1024) || !is_pow2(p.ny, 8, 1024) || !is_pow2(p.nz, 8, 1024)) {
-    g_create_err = "grid sizes must be powers of two in [8, 1024]";
+    spm::global_error() = "grid sizes must be powers of two in [8, 1024]";
     return SPM_ERR_UNSUPPORTED;
   }
   if (p.n_particles < 0 || p.n_species < 1 || p.n_species > SPM_MAX_SPECIES || !(p.dt > 0) || !(p.dx > 0) ||
       !(p.rho > 0) || !(p.eta > 0) || !(p.a > 0) || !(p.a_xi > 0) || p.g_direction < 0 || p.g_direction > 2) {
-    g_create_err = "invalid parameter";
+    spm::global_error() = "invalid parameter";
     return SPM_ERR_INVALID;
   }
   for (int s = 0; s < p.n_species; ++s)
     if (!(p.mass_ratio[s] > 0)) {
-      g_create_err = "mass ratios must be positive";
+      spm::global_error() = "mass ratios must be positive";
       return SPM_ERR_INVALID;
     }
   if (p.precision != SPM_FP64 && p.precision != SPM_FP32) {
-    g_create_err = "precision must be SPM_FP64 or SPM_FP32";
+    spm::global_error() = "precision must be SPM_FP64 or SPM_FP32";
     return SPM_ERR_INVALID;
   }
   const int nh = (int)p.a + (int)p.a_xi + 2;
   if (p.n_particles > 0 && (p.nx < 2 * nh + 1 || p.ny < 2 * nh + 1 || p.nz < 2 * nh + 1)) {
-    g_create_err = "box narrower than a particle stencil (2*(a+xi+2)+1 points)";
+    spm::global_error() = "box narrower than a particle stencil (2*(a+xi+2)+1 points)";
     return SPM_ERR_UNSUPPORTED;
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-    g_create_err = "no HIP device";
+    spm::global_error() = "no HIP device";
     return SPM_ERR_NO_DEVICE;
   }
   if (p.device < 0 || p.device >= ndev) {
-    g_create_err = "device ordinal out of range";
+    spm::global_error() = "device ordinal out of range";
     return SPM_ERR_INVALID;
   }
   spm_ctx *c = new (std::nothrow) spm_ctx();
@@ -900,9 +904,9 @@ int spm_create(const spm_params *prm, spm_ctx **out) {
   c->device = p.device;
   int rc = setup(c);
   if (rc) {
-    g_create_err = c->err;
+    spm::global_error() = c->err;
     free_ctx(c);
-    return rc == SPM_ERR_HIP && g_create_err.find("out of memory") != std::string::npos ? SPM_ERR_OOM : rc;
+    return rc == SPM_ERR_HIP && spm::global_error().find("out of memory") != std::string::npos ? SPM_ERR_OOM : rc;
   }
   *out = c;
   return SPM_OK;

@@ -239,6 +239,8 @@ struct spm_ctx {
 // launchers implemented in the .hip files
 // ---------------------------------------------------------------------
 namespace spm {
+// last error of a call without a context (this thread)
+std::string &global_error();
 // timing brackets around a launch; no-ops unless timing is enabled
 void tbeg(spm_ctx *c);
 void tend(spm_ctx *c, int family, double bytes);
@@ -259,6 +261,10 @@ int fft_r2c_band_yx(spm_ctx *c, double *const *a, int n);
 // z pass of the sheared solver: C2R of u and w, u x w, R2C into u (FP64)
 int fz_cross_obl(spm_ctx *c, double *const u[3], double *const w[3]);
 int fft_init_tables(spm_ctx *c);
+// context-free transform of one FP64 field (spm_fft3d): roots in `ws`, at
+// least fft3d_workspace_bytes(); 0 bytes means the sizes are unsupported
+size_t fft3d_workspace_bytes(int nx, int ny, int nz);
+int fft3d_raw(int nx, int ny, int nz, int direction, double *a, void *ws, hipStream_t st);
 int fft_pass_y(spm_ctx *c, double *const *a, int n, int sign);
 int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n);
 int fft_pass_z_r2c(spm_ctx *c, double *const *a, int n);

@@ -18,7 +18,7 @@ HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.j
 
 def declared_functions():
     src = "\n".join(open(h).read() for h in HEADERS)
-    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int|void)\s*\*?\s*(spm_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int|void|size_t)\s*\*?\s*(spm_\w+)\s*\(", src, re.M)))
 
 
 def test_header_layout_matches_ctypes(tmp_path):
@@ -103,3 +103,17 @@ def test_python_front_end_covers_the_call_surface():
               "MD_solver_position", "MD_solver_velocity", "Add_random_force_thermostat",
               "Make_f_particle_dt_sole", "U2zeta_k", "Zeta_k2u", "init_fluid", "step"]:
         assert callable(getattr(SPM, m))
+
+
+def test_fft_workspace_size_and_size_checks_without_device():
+    """spm_fft_workspace_size is host arithmetic, and spm_fft3d rejects bad
+    sizes before touching a device."""
+    from kapsel_old_amd.spm import lib
+    L = lib()
+    ws = L.spm_fft_workspace_size(256, 256, 256)
+    # four 256-byte aligned root tables of 16-byte entries: 256, 256, 128, 129
+    assert ws == 4096 + 4096 + 2048 + 2304
+    assert L.spm_fft_workspace_size(48, 64, 64) == 0
+    assert L.spm_fft_workspace_size(2048, 64, 64) == 0
+    assert L.spm_fft3d(48, 64, 64, 1, None, None, 0, None) == abi.SPM_ERR_INVALID
+    assert b"powers of two" in L.spm_last_error(None)
