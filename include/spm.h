@@ -243,6 +243,16 @@ int spm_check_finite(spm_ctx *ctx);
 size_t spm_fft_workspace_size(int nx, int ny, int nz);
 int spm_fft3d(int nx, int ny, int nz, int direction, double *field, void *workspace, size_t workspace_bytes,
               void *stream);
+/* No counterpart in the reference: spm_fft3d for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same transform and results;
+ * the root tables go to a 65536-byte static device array of the library
+ * (one per GPU, held until the process ends) instead of a workspace, so the
+ * call allocates nothing and only queues kernels on `stream`.  Sizes are
+ * powers of two in [8, 1024] (the tables then fit the array); others get
+ * SPM_ERR_INVALID.  Every call on a GPU rewrites that GPU's array: calls
+ * that overlap on one GPU (two streams or two threads) race. */
+int spm_fft3d_noalloc(int nx, int ny, int nz, int direction, double *field, void *stream);
 
 /* ---- checkpoint / resume ------------------------------------------------
  * Replace Save_Restart_udf / Force_restore_parameters (resume.cxx:10-364)

@@ -33,7 +33,7 @@ SYMBOLS = [
     "spm_set_lees_edwards", "spm_put_oblique", "spm_get_oblique", "spm_ns_solver_slaved_euler_shear_obl",
     "spm_zeta_k2u_k_obl", "spm_u_k2zeta_k_obl", "spm_make_phi_u_particle_obl", "spm_calc_f_hydro_obl",
     "spm_md_solver_position_obl", "spm_md_solver_velocity_obl",
-    "spm_fft_workspace_size", "spm_fft3d",
+    "spm_fft_workspace_size", "spm_fft3d", "spm_fft3d_noalloc",
 ]
 
 SCHEDULE_FUSED, SCHEDULE_LITERAL = 0, 1
@@ -106,6 +106,7 @@ def lib():
         L.spm_fft_workspace_size.argtypes = [ip, ip, ip]
         L.spm_fft_workspace_size.restype = C.c_size_t
         L.spm_fft3d.argtypes = [ip, ip, ip, ip, vp, vp, C.c_size_t, vp]
+        L.spm_fft3d_noalloc.argtypes = [ip, ip, ip, ip, vp, vp]
         L.spm_put_total_solute.argtypes = [vp, dp]
         L.spm_get_total_solute.argtypes = [vp, dp]
         for name in SYMBOLS:
