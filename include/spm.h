@@ -263,8 +263,7 @@ int spm_fft3d_noalloc(int nx, int ny, int nz, int direction, double *field, void
  * shear degree_oblique and Shear_rate_eff, with a hash over all of it.  A
  * context created with the same spm_params (and spm_set_electrolyte or
  * spm_set_lees_edwards) that loads the file continues bit for bit like the
- * one that saved it (under Lees-Edwards shear, where particle stencils
- * overlap, up to the run-to-run rounding order of the phi scatter).  spm_load_state rejects a file from another grid,
+ * one that saved it.  spm_load_state rejects a file from another grid,
  * precision, particle count, ion count, time step or shear rate, or a
  * damaged one, and then leaves the context unchanged.  spm_save_state writes PATH.tmp and renames it. */
 int spm_save_state(spm_ctx *ctx, const char *path);

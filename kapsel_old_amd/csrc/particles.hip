@@ -708,65 +708,158 @@ __device__ __forceinline__ int rel_coord_obl(const Grid &g, const PModel &pm, co
   return sign;
 }
 
-// Make_phi_u_primitive_OBL (make_phi.cxx:299-364), one wavefront per
-// particle over its stencil: phi and up are scattered with FP64 atomics
-// (the image of a particle across the y boundary lands on a shifted,
-// fractional part of the grid, outside the tile lists of the unsheared
-// path), so sums at points two particles share are ordered differently
-// from run to run, at rounding level.  The x velocity of an image point
-// is v_x - sign Shear_rate_eff LY.
-__global__ __launch_bounds__(256) void make_phi_obl_kernel(Grid g, PModel pm, const double *__restrict__ pb,
-                                                           const int *__restrict__ pxint,
-                                                           const double *__restrict__ pres,
-                                                           const int *__restrict__ stencil, int nst, double deg,
-                                                           const double *__restrict__ sre_p, double *phi,
-                                                           double *up0, double *up1, double *up2) {
-  const int lane = threadIdx.x & 63;
-  const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (n >= pm.np) return;
-  const int xi[3] = {pxint[3 * n], pxint[3 * n + 1], pxint[3 * n + 2]};
-  const double res[3] = {pres[3 * n], pres[3 * n + 1], pres[3 * n + 2]};
-  const double vp[3] = {P(pb, pm.np, P_V, n), P(pb, pm.np, P_V + 1, n), P(pb, pm.np, P_V + 2, n)};
-  const double wp[3] = {P(pb, pm.np, P_OMEGA, n), P(pb, pm.np, P_OMEGA + 1, n), P(pb, pm.np, P_OMEGA + 2, n)};
-  const double shift = *sre_p * g.ly;
-  const double rcut = pm.radius + pm.hxi;
-  for (int s = lane; s < nst; s += 64) {
-    const int cell[3] = {stencil[3 * s], stencil[3 * s + 1], stencil[3 * s + 2]};
-    int rm[3];
-    double r[3];
-    const int sign = rel_coord_obl(g, pm, cell, xi, res, deg, rm, r);
-    double dist = 0.0;
-    for (int d = 0; d < 3; ++d) dist += sq(r[d]);
-    dist = sqrt(dist);
-    if (!(dist < rcut)) continue;  // phi = 0 adds nothing
-    const double dphi = phi_fast<double>(pm, dist);
-    double vr[3] = {0.0, 0.0, 0.0};
-    if (pm.rotation) {
-      vr[0] = wp[1] * r[2] - wp[2] * r[1];
-      vr[1] = wp[2] * r[0] - wp[0] * r[2];
-      vr[2] = wp[0] * r[1] - wp[1] * r[0];
-    }
-    const long im = ((long)rm[0] * g.ny + rm[1]) * g.nz_ + rm[2];
-    atomicAdd(phi + im, dphi);
-    atomicAdd(up0 + im, (vp[0] - sign * shift + vr[0]) * dphi);
-    atomicAdd(up1 + im, (vp[1] + vr[1]) * dphi);
-    atomicAdd(up2 + im, (vp[2] + vr[2]) * dphi);
+// Make_phi_u_primitive_OBL (make_phi.cxx:299-364, clamp :366-384) as an
+// owner-computes gather, so every sum has a fixed order and the sheared
+// FP64 path is run-to-run identical.
+//
+// Lists: each particle is entered in the tiles its stencil cube reaches
+// (region 0) and, when the cube crosses the upper / lower y face, in the
+// tiles of the same cube moved by the image shift -sign deg NY along x
+// (regions 1 / 2, one tile wider for the fractional part).  Entering a
+// few tiles too many only costs time: the gather keeps a point only where
+// the stencil map of the scatter lands on it.
+__device__ __forceinline__ int obl_region_tiles(const Grid &g, const PModel &pm, const double *pb, int n,
+                                                const int xi[3], double deg, int region, int t0[3], int cn[3]) {
+  const int nt[3] = {g.nx / kTile, g.ny / kTile, g.nz / kTile};
+  if (region == 1 && xi[1] + pm.nh < g.ny) return 0;
+  if (region == 2 && xi[1] - pm.nh >= 0) return 0;
+  // one cell of margin along x: the scatter's floor() of the shifted x
+  // coordinate can land one past the cube where a residue rounds to 1
+  int cx = xi[0], wx = pm.nh + 1;
+  if (region > 0) {
+    const double sign = region == 1 ? 1.0 : -1.0;
+    const double xo = (double)xi[0] - sign * deg * g.ny + (P(pb, pm.np, P_X, n) / pm.dx - (double)xi[0]);
+    cx = (int)floor(xo);
   }
+  const int lo[3] = {cx - wx, xi[1] - pm.nh, xi[2] - pm.nh};
+  const int hi[3] = {cx + wx, xi[1] + pm.nh, xi[2] + pm.nh};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int a = (lo[d] >= 0) ? lo[d] / kTile : -((-lo[d] + kTile - 1) / kTile);
+    const int b = (hi[d] >= 0) ? hi[d] / kTile : -((-hi[d] + kTile - 1) / kTile);
+    cn[d] = b - a + 1;
+    if (cn[d] > nt[d]) cn[d] = nt[d];
+    t0[d] = ((a % nt[d]) + nt[d]) % nt[d];
+  }
+  return 1;
 }
 
-// the clamp of make_phi.cxx:366-384
-__global__ void phi_clamp_kernel(Grid g, double *phi, double *up0, double *up1, double *up2) {
+// thread (n, region, slot); mode 0 counts, mode 1 fills
+__global__ void obl_tiles_kernel(Grid g, PModel pm, const double *__restrict__ pb, const int *__restrict__ pxint,
+                                 double deg, int k0, int k1, int k2, int *tile_count, int *tile_cursor, int *tile_list,
+                                 int mode) {
+  const int per = k0 * k1 * k2;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.nx * g.ny * g.nz) return;
-  const long im = (t / g.nz) * g.nz_ + t % g.nz;
-  const double ph = phi[im];
-  if (ph > 1.) {
-    const double inv = 1. / ph;
-    up0[im] *= inv;
-    up1[im] *= inv;
-    up2[im] *= inv;
-    phi[im] = 1.;
+  const int n = (int)(t / (3L * per)), region = (int)((t / per) % 3), s = (int)(t % per);
+  if (n >= pm.np) return;
+  const int xi[3] = {pxint[3 * n], pxint[3 * n + 1], pxint[3 * n + 2]};
+  int t0[3], cn[3];
+  if (!obl_region_tiles(g, pm, pb, n, xi, deg, region, t0, cn)) return;
+  const int nt[3] = {g.nx / kTile, g.ny / kTile, g.nz / kTile}, k[3] = {k0, k1, k2};
+  const int ti = tile_slot(s, k, t0, cn, nt);
+  if (ti < 0) return;
+  if (mode == 0)
+    atomicAdd(&tile_count[ti], 1);
+  else
+    tile_list[atomicAdd(&tile_cursor[ti], 1)] = n;
+}
+
+// One block per 8x8x8 tile, one point per thread.  For every particle of
+// the tile's sorted list (each once) the stencil cells that can map to this
+// point are the y offsets c1 = j - xi1 (+NY across the upper face, -NY
+// across the lower one) within the cube, and for each the x offsets next
+// to the inverse of the sheared x map; a cell counts where the scatter's
+// own map (rel_coord_obl) lands on this point and its distance is below
+// a + xi/2, so the sums are the scatter's, added in ascending particle
+// order as the reference's particle loop adds them.
+__global__ __launch_bounds__(512) void make_phi_obl_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+                                                           const int *__restrict__ pxint,
+                                                           const double *__restrict__ pres,
+                                                           const int *__restrict__ tile_start,
+                                                           const int *__restrict__ tile_count,
+                                                           const int *__restrict__ tile_list, double deg,
+                                                           const double *__restrict__ sre_p, double *phi,
+                                                           double *up0, double *up1, double *up2) {
+  const int tz = g.nz / kTile, ty = g.ny / kTile;
+  const int tile = blockIdx.x;
+  const int tk = tile % tz, tj = (tile / tz) % ty, ti = tile / (tz * ty);
+  const int lk = threadIdx.x % kTile, lj = (threadIdx.x / kTile) % kTile, li = threadIdx.x / (kTile * kTile);
+  const int gi[3] = {ti * kTile + li, tj * kTile + lj, tk * kTile + lk};
+  const int NX = g.nx, NY = g.ny, NZ = g.nz;
+  const double shift = *sre_p * g.ly;
+  const double rcut = pm.radius + pm.hxi;
+  const int *lst = tile_list + tile_start[tile];
+  const int cnt = tile_count[tile];
+  double ph = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  int prev = -1;
+  for (int q = 0; q < cnt; ++q) {
+    const int n = lst[q];
+    if (n == prev) continue;  // a particle entered through two regions
+    prev = n;
+    const int xi[3] = {pxint[3 * n], pxint[3 * n + 1], pxint[3 * n + 2]};
+    int c2 = gi[2] - xi[2];
+    if (c2 >= NZ / 2) c2 -= NZ;
+    if (c2 < -(NZ / 2)) c2 += NZ;
+    if (c2 > pm.nh || c2 < -pm.nh) continue;
+    const double res[3] = {pres[3 * n], pres[3 * n + 1], pres[3 * n + 2]};
+    // screen: cells farther than a + xi/2 (plus a rounding margin) from the
+    // particle add nothing, so only the survivors run the exact map
+    const double rz = c2 * pm.dx - res[2];
+    const double rc2 = (rcut + 1e-6 * pm.dx) * (rcut + 1e-6 * pm.dx);
+    for (int yc = 0; yc < 3; ++yc) {
+      const int c1 = gi[1] - xi[1] + (yc == 1 ? NY : (yc == 2 ? -NY : 0));
+      if (c1 > pm.nh || c1 < -pm.nh) continue;
+      const double ryz = sq(c1 * pm.dx - res[1]) + sq(rz);
+      if (!(ryz < rc2)) continue;
+      const int sign = yc == 1 ? 1 : (yc == 2 ? -1 : 0);
+      const double base = (double)xi[0] - sign * deg * NY + res[0] / pm.dx;
+      const double fb = floor(base);
+      int c0 = gi[0] - (int)fb;
+      c0 = ((c0 % NX) + NX + NX / 2) % NX - NX / 2;
+      for (int dc = -1; dc <= 1; ++dc) {
+        const int cell[3] = {c0 + dc, c1, c2};
+        if (cell[0] > pm.nh || cell[0] < -pm.nh) continue;
+        if (!(sq((cell[0] - (base - fb)) * pm.dx) + ryz < rc2)) continue;
+        int rm[3];
+        double r[3];
+        const int sg = rel_coord_obl(g, pm, cell, xi, res, deg, rm, r);
+        if (rm[0] != gi[0] || rm[1] != gi[1] || rm[2] != gi[2]) continue;
+        double dist = 0.0;
+        for (int d = 0; d < 3; ++d) dist += sq(r[d]);
+        dist = sqrt(dist);
+        if (!(dist < rcut)) continue;  // phi = 0 adds nothing
+        const double dphi = phi_fast<double>(pm, dist);
+        const double vp[3] = {P(pb, pm.np, P_V, n), P(pb, pm.np, P_V + 1, n), P(pb, pm.np, P_V + 2, n)};
+        double vr[3] = {0.0, 0.0, 0.0};
+        if (pm.rotation) {
+          const double wp[3] = {P(pb, pm.np, P_OMEGA, n), P(pb, pm.np, P_OMEGA + 1, n),
+                                P(pb, pm.np, P_OMEGA + 2, n)};
+          vr[0] = wp[1] * r[2] - wp[2] * r[1];
+          vr[1] = wp[2] * r[0] - wp[0] * r[2];
+          vr[2] = wp[0] * r[1] - wp[1] * r[0];
+        }
+        ph += dphi;
+        a0 += (vp[0] - sg * shift + vr[0]) * dphi;
+        a1 += (vp[1] + vr[1]) * dphi;
+        a2 += (vp[2] + vr[2]) * dphi;
+      }
+    }
   }
+  if (ph > 1.) {  // make_phi.cxx:366-384
+    const double inv = 1. / ph;
+    a0 *= inv;
+    a1 *= inv;
+    a2 *= inv;
+    ph = 1.;
+  }
+  const long im = ((long)gi[0] * g.ny + gi[1]) * g.nz_ + gi[2];
+  phi[im] = ph;
+  up0[im] = a0;
+  up1[im] = a1;
+  up2[im] = a2;
+  if (gi[2] == NZ - 1)
+    phi[im + 1] = phi[im + 2] = up0[im + 1] = up0[im + 2] = up1[im + 1] = up1[im + 2] = up2[im + 1] = up2[im + 2] =
+        0.0;
 }
 
 // Calc_f_hydro_correct_precision_OBL (md_force.cxx:391-470, 497-506) on the
@@ -1188,20 +1281,43 @@ int p_velocity_obl(spm_ctx *c, int euler) {
   return 0;
 }
 
+int obl_tile_span(const spm_ctx *c, int axis) {
+  const int width = 2 * c->pm.nh + 1 + (axis == 0 ? 2 : 0);
+  return std::min(c->tiles[axis], (width + 7) / 8 + 1);
+}
+
 // phi and up at the anchors p_calc_f_hydro_obl built (positions unchanged
-// since)
+// since), through tile lists that include the sheared images
 int p_make_phi_obl(spm_ctx *c, const double *sre) {
-  RCP(k_zero(c, c->phi, c->nfield));
-  for (int d = 0; d < 3; ++d) RCP(k_zero(c, c->up[d], c->nfield));
-  if (c->np == 0) return 0;
+  if (c->np == 0) {
+    RCP(k_zero(c, c->phi, c->nfield));
+    for (int d = 0; d < 3; ++d) RCP(k_zero(c, c->up[d], c->nfield));
+    return 0;
+  }
+  const int k[3] = {obl_tile_span(c, 0), obl_tile_span(c, 1), obl_tile_span(c, 2)};
+  const long slots = 3L * c->np * k[0] * k[1] * k[2];
+  if (slots > c->tile_list_obl_cap) {
+    c->err = "sheared tile lists exceed their allocation";
+    return SPM_ERR_INVALID;
+  }
   tbeg(c);
-  hipLaunchKernelGGL(make_phi_obl_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
-                     c->pxint, c->pres, c->stencil, c->n_stencil, c->le_deg, sre, c->phi, c->up[0], c->up[1],
-                     c->up[2]);
-  const long npts = (long)c->grid.nx * c->grid.ny * c->grid.nz;
-  hipLaunchKernelGGL(phi_clamp_kernel, dim3(blocks(npts, 256)), dim3(256), 0, c->stream, c->grid, c->phi, c->up[0],
-                     c->up[1], c->up[2]);
-  tend(c, SPM_K_MAKE_PHI, 32.0 * (double)c->n_stencil * c->np + 64.0 * npts);
+  hipLaunchKernelGGL(izero_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_count_obl,
+                     c->ntiles);
+  hipLaunchKernelGGL(obl_tiles_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->pxint, c->le_deg, k[0], k[1], k[2], c->tile_count_obl, c->tile_cursor_obl, c->tile_list_obl, 0);
+  hipLaunchKernelGGL(scan_kernel, dim3(scan_blocks(c->ntiles)), dim3(1024), 0, c->stream, c->tile_count_obl,
+                     c->tile_start_obl, c->tile_cursor_obl, c->ntiles);
+  hipLaunchKernelGGL(obl_tiles_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->pxint, c->le_deg, k[0], k[1], k[2], c->tile_count_obl, c->tile_cursor_obl, c->tile_list_obl, 1);
+  hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_start_obl,
+                     c->tile_count_obl, c->tile_list_obl, c->ntiles);
+  tend(c, SPM_K_PARTICLE, 16.0 * c->ntiles);
+  SPM_LAUNCH_CHECK(c);
+  tbeg(c);
+  hipLaunchKernelGGL(make_phi_obl_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf,
+                     c->pxint, c->pres, c->tile_start_obl, c->tile_count_obl, c->tile_list_obl, c->le_deg, sre, c->phi,
+                     c->up[0], c->up[1], c->up[2]);
+  tend(c, SPM_K_MAKE_PHI, 4.0 * 8.0 * (double)c->nfield);
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

@@ -330,7 +330,8 @@ void free_ctx(spm_ctx *c) {
                     c->surface_normal[1], c->surface_normal[2], c->f_ns1[0], c->f_ns1[1], c->red, c->le_red})
     if (p) (void)hipFree(p);
   for (int *p : {c->spec, c->stencil, c->pxint, c->tile_count, c->tile_start, c->tile_cursor, c->tile_list,
-                 c->cell_count, c->cell_start, c->cell_cursor, c->cell_list, c->pcell, c->flag})
+                 c->cell_count, c->cell_start, c->cell_cursor, c->cell_list, c->pcell, c->flag, c->tile_count_obl,
+                 c->tile_start_obl, c->tile_cursor_obl, c->tile_list_obl})
     if (p) (void)hipFree(p);
   for (double2 *p : {c->tw_x, c->tw_y, c->tw_z, c->tw_rz})
     if (p) (void)hipFree(p);
@@ -772,6 +773,12 @@ int set_lees_edwards(spm_ctx *c, double shear_rate) {
     return SPM_ERR_UNSUPPORTED;
   }
   RC(dalloc(c, &c->le_red, (size_t)le_red_slots()));  // zeroes Shear_rate_eff
+  // tile lists of the phi gather, images across the y faces included
+  RC(dalloc(c, &c->tile_count_obl, (size_t)c->ntiles));
+  RC(dalloc(c, &c->tile_start_obl, (size_t)c->ntiles));
+  RC(dalloc(c, &c->tile_cursor_obl, (size_t)c->ntiles));
+  c->tile_list_obl_cap = std::max<long>(1, 3L * c->np * obl_tile_span(c, 0) * obl_tile_span(c, 1) * obl_tile_span(c, 2));
+  RC(dalloc(c, &c->tile_list_obl, (size_t)c->tile_list_obl_cap));
   // under shear the LJ diameter is min((SIGMA + XI) / 2^(1/p), SIGMA) with
   // the LJ_powers exponent p (input.cxx:389-403); R_cutoff = A_R_cutoff LJ_dia
   {

@@ -153,6 +153,11 @@ struct spm_ctx {
   int *tile_count = nullptr, *tile_start = nullptr, *tile_cursor = nullptr;
   int *tile_list = nullptr;
   int tile_list_cap = 0;
+  // the same under Lees-Edwards shear, each particle also entered in the
+  // tiles of its images across the y faces (allocated by spm_set_lees_edwards)
+  int *tile_count_obl = nullptr, *tile_start_obl = nullptr, *tile_cursor_obl = nullptr;
+  int *tile_list_obl = nullptr;
+  long tile_list_obl_cap = 0;
 
   // LJ cell list
   int cells[3] = {0, 0, 0};
@@ -299,6 +304,8 @@ int p_position_obl(spm_ctx *c, int euler, const double *sre);
 int p_calc_f_hydro_obl(spm_ctx *c, double *const ucp[3], const double *sre);
 int p_velocity_obl(spm_ctx *c, int euler);
 int p_make_phi_obl(spm_ctx *c, const double *sre);
+// tiles per axis that one region of a sheared particle's list can span
+int obl_tile_span(const spm_ctx *c, int axis);
 int p_f_add_obl(spm_ctx *c, double *const ucp[3]);
 // Add_random_force_thermostat: host noise draw + device update
 int host_thermostat(spm_ctx *c);

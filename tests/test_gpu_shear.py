@@ -452,3 +452,24 @@ def test_steps_with_particles_128cube(SPM):
     for d in range(3):
         check("128^3 u%d after the call" % d, objs[1].get_field("u%d" % d), objs[0].get_field("u%d" % d))
     _cmp_particles(objs, ("x", "v", "omega"), "128^3 particles")
+
+
+def test_sheared_steps_are_run_to_run_identical(SPM):
+    """phi and up of the sheared step are gathered per grid point in
+    ascending particle order (no floating-point atomics): two runs of the
+    same three steps, with particles whose stencils overlap across the y
+    faces, end bit for bit alike, and Make_phi_u_particle_OBL on its own
+    also matches the oracle there."""
+    outs = []
+    for _ in range(2):
+        objs, p = _le_pair(SPM, (64, 64, 32), 12, 4.0, deg=0.41, sre=0.05, rate=0.05, near_y=6, kbt=1e-4)
+        g = objs[1]
+        g.step(3)
+        g.Make_phi_u_particle_OBL()
+        outs.append([g.get_field(f) for f in ("phi", "up0", "up1", "up2", "zeta0", "zeta1")]
+                    + [g.get_particles()["v"].copy()])
+        objs[0].close()
+        g.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert (outs[0][0] > 1.0 - 1e-12).any() and outs[0][0][:, 0, :].any()
