@@ -8,6 +8,7 @@
 #include <new>
 
 #include "spm_internal.h"
+#include "../../include/spm_closure.h"
 
 using namespace spm;
 
@@ -217,15 +218,28 @@ int thermostat(spm_ctx *c) {
   return p_thermostat_apply(c, c->noise_dev);
 }
 
+// The particle part of the step (sp_3d_ns.cxx:81-95,183-185): positions,
+// hydrodynamic force and torque (plus the attached closure's correction),
+// velocities, thermostat.  With prescribed motion the particles keep their
+// positions and velocities and only the force is formed.
+int particle_motion(spm_ctx *c) {
+  const bool free_motion = c->motion == SPM_MOTION_FREE;
+  if (free_motion) RC(p_position(c, c->ts == 0));
+  RC(p_calc_f_hydro(c));
+  if (c->closure) RC(closure_apply(c));
+  if (free_motion) {
+    RC(p_velocity(c, c->ts == 0));
+    RC(thermostat(c));
+  }
+  return 0;
+}
+
 // Time_evolution_hydro (sp_3d_ns.cxx:61-212), Navier_Stokes with free
 // spherical particles, no slip.
 int time_evolution_hydro(spm_ctx *c) {
   RC(ns_solver(c));
   RC(zeta_k2u(c, fix_mask(c)));
-  RC(p_position(c, c->ts == 0));
-  RC(p_calc_f_hydro(c));  // Reset_phi_u is skipped: make_phi below rewrites phi/up
-  RC(p_velocity(c, c->ts == 0));
-  RC(thermostat(c));
+  RC(particle_motion(c));
   RC(p_make_phi(c, 1));
   RC(make_f_sole_add(c));
   RC(u2zeta_k(c));
@@ -257,11 +271,8 @@ int fused_step(spm_ctx *c) {
   RC(fx_zeta2u(c, fix_mask(c)));
   RC(fft_pass_y(c, u3, 3, -1));
   RC(fft_pass_z_c2r(c, u3, 3));
-  // particles
-  RC(p_position(c, c->ts == 0));
-  RC(p_calc_f_hydro(c));  // also builds the tile lists at the new positions
-  RC(p_velocity(c, c->ts == 0));
-  RC(thermostat(c));
+  // particles (p_calc_f_hydro also builds the tile lists at the new positions)
+  RC(particle_motion(c));
   // Make_phi_u_particle + Make_f_particle_dt_sole + Add_f_particle + U2zeta_k
   // (with no particles the tile lists stay empty and f = 0)
   RC(make_f_z(c, f3));

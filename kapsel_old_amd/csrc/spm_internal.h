@@ -204,6 +204,11 @@ struct spm_ctx {
   spm::QTable qtab{};
   double valency_e[2] = {0.0, 0.0};
 
+  // learned closure added to f_hydro / torque_hydro (include/spm_closure.h)
+  // and the particle motion of the step (free, or prescribed: held fixed)
+  struct spm_closure *closure = nullptr;
+  int motion = 0;
+
   // optional per-kernel-family timing (HIP events on `stream`)
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -316,6 +321,9 @@ int p_make_phi(spm_ctx *c, int sw_up);
 int p_make_f(spm_ctx *c);
 int p_init_lists(spm_ctx *c);
 int p_calc_f_hydro_field(spm_ctx *c, double *const fld[3]);
+// closure.hip: the attached network's force/torque correction (after
+// p_calc_f_hydro, whose tile lists and anchors it reads)
+int closure_apply(spm_ctx *c);
 int electrolyte_step(spm_ctx *c);
 // Lees-Edwards (shear.hip)
 int le_red_slots();

@@ -34,6 +34,10 @@ SYMBOLS = [
     "spm_zeta_k2u_k_obl", "spm_u_k2zeta_k_obl", "spm_make_phi_u_particle_obl", "spm_calc_f_hydro_obl",
     "spm_md_solver_position_obl", "spm_md_solver_velocity_obl",
     "spm_fft_workspace_size", "spm_fft3d", "spm_fft3d_noalloc",
+    "spm_closure_create", "spm_closure_destroy", "spm_closure_init", "spm_closure_get_params",
+    "spm_closure_put_params", "spm_closure_set_scaling", "spm_closure_get_scaling", "spm_closure_forward",
+    "spm_closure_gradient", "spm_closure_set_data", "spm_closure_train", "spm_set_closure", "spm_closure_features",
+    "spm_set_particle_motion",
 ]
 
 SCHEDULE_FUSED, SCHEDULE_LITERAL = 0, 1
@@ -107,6 +111,21 @@ def lib():
         L.spm_fft_workspace_size.restype = C.c_size_t
         L.spm_fft3d.argtypes = [ip, ip, ip, ip, vp, vp, C.c_size_t, vp]
         L.spm_fft3d_noalloc.argtypes = [ip, ip, ip, ip, vp, vp]
+        fp = C.POINTER(C.c_float)
+        L.spm_closure_create.argtypes = [ip, ip, C.POINTER(vp)]
+        L.spm_closure_destroy.argtypes = [vp]
+        L.spm_closure_init.argtypes = [vp, C.c_uint64]
+        L.spm_closure_get_params.argtypes = [vp, fp]
+        L.spm_closure_put_params.argtypes = [vp, fp]
+        L.spm_closure_set_scaling.argtypes = [vp, fp, fp, fp]
+        L.spm_closure_get_scaling.argtypes = [vp, fp, fp, fp]
+        L.spm_closure_forward.argtypes = [vp, fp, fp, ip]
+        L.spm_closure_gradient.argtypes = [vp, fp, fp, ip, fp, fp]
+        L.spm_closure_set_data.argtypes = [vp, fp, fp, ip]
+        L.spm_closure_train.argtypes = [vp, ip, ip, C.c_float, fp]
+        L.spm_set_closure.argtypes = [vp, vp]
+        L.spm_closure_features.argtypes = [vp, fp]
+        L.spm_set_particle_motion.argtypes = [vp, ip]
         L.spm_put_total_solute.argtypes = [vp, dp]
         L.spm_get_total_solute.argtypes = [vp, dp]
         for name in SYMBOLS:
@@ -175,6 +194,24 @@ class SPM:
     def set_schedule(self, schedule):
         """SCHEDULE_FUSED (default) or SCHEDULE_LITERAL for spm_step."""
         self._call("set_schedule", schedule)
+
+    # ---- learned closure and particle motion (include/spm_closure.h) ----
+    def set_closure(self, closure):
+        """Attach a Closure (None detaches): each step adds its force and
+        torque correction after Calc_f_hydro_correct_precision."""
+        self._closure = closure
+        self._call("set_closure", closure._h if closure is not None else None)
+
+    def closure_features(self):
+        """The closure's 114 inputs of every particle, float32 [n][114]."""
+        x = np.zeros((self.n_particles, 114), dtype=np.float32)
+        self._call("closure_features", x.ctypes.data_as(C.POINTER(C.c_float)))
+        return x
+
+    def set_particle_motion(self, prescribed):
+        """False: free particles (default); True: positions, velocities and
+        angular velocities held fixed while the step forms the force."""
+        self._call("set_particle_motion", 1 if prescribed else 0)
 
     # ---- electrolyte (SW_EQ == Electrolyte) ----
     def set_electrolyte(self, el):
@@ -302,3 +339,83 @@ class SPM:
     def put_rng(self, mt, mti):
         arr = (C.c_uint32 * abi.MT_N)(*[int(x) for x in mt])
         self._call("put_rng", arr, mti)
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Closure:
+    """The learned force/torque closure's network on one GPU (an
+    spm_closure): 114 inputs -> 128 -> 128 -> 128 -> 6, BF16 weights on the
+    matrix cores with FP32 accumulation, trained with Adam on the GPU."""
+
+    NPARAM = 51600
+    IN, OUT = 114, 6
+
+    def __init__(self, max_batch, device=0, seed=0):
+        h = C.c_void_p()
+        rc = lib().spm_closure_create(device, max_batch, C.byref(h))
+        if rc:
+            raise SpmError(rc, "spm_closure_create", lib().spm_last_error(None).decode())
+        self._h = h
+        self.max_batch = max_batch
+        if seed:
+            self.init(seed)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().spm_closure_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _call(self, name, *args):
+        rc = getattr(lib(), "spm_closure_" + name)(self._h, *args)
+        if rc:
+            raise SpmError(rc, "spm_closure_" + name, lib().spm_last_error(None).decode())
+
+    def init(self, seed):
+        self._call("init", seed)
+
+    def get_params(self):
+        p = np.zeros(self.NPARAM, dtype=np.float32)
+        self._call("get_params", _fp(p))
+        return p
+
+    def put_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._call("put_params", _fp(p))
+
+    def set_scaling(self, shift, scale, oscale):
+        a, b, c = (np.ascontiguousarray(v, dtype=np.float32) for v in (shift, scale, oscale))
+        self._call("set_scaling", _fp(a), _fp(b), _fp(c))
+
+    def get_scaling(self):
+        a, b, c = np.zeros(114, np.float32), np.zeros(114, np.float32), np.zeros(6, np.float32)
+        self._call("get_scaling", _fp(a), _fp(b), _fp(c))
+        return a, b, c
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.OUT), dtype=np.float32)
+        self._call("forward", _fp(x), _fp(y), x.shape[0])
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.NPARAM, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._call("gradient", _fp(x), _fp(t), x.shape[0], _fp(g), _fp(loss))
+        return float(loss[0]), g
+
+    def set_data(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        self._call("set_data", _fp(x), _fp(t), x.shape[0])
+
+    def train(self, steps, batch, lr=1e-3):
+        loss = np.zeros(steps, dtype=np.float32)
+        self._call("train", steps, batch, C.c_float(lr), _fp(loss))
+        return loss
