@@ -15,7 +15,8 @@
  *            [3, 84)    u(x_p + a o) - v_p at the 27 offsets o in {-1,0,1}^3
  *                       (o lexicographic, x slowest; components x, y, z),
  *                       trilinear in the grid
- *            [84, 111)  phi(x_p + a o) at the same points (after the clamp)
+ *            [84, 111)  phi(x_p + a o): the particles' profiles summed at
+ *                       the same points, clamped at 1
  *            [111, 114) gap vector to the nearest other particle: the centre
  *                       separation r times (1 - 2a / |r|), minimum image
  *                       (zero without another particle)

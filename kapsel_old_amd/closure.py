@@ -159,3 +159,91 @@ def augment(x, t):
     """The training set under all 48 lattice symmetries."""
     pairs = [transform(x, t, S) for S in cube_symmetries()]
     return np.concatenate([p[0] for p in pairs]), np.concatenate([p[1] for p in pairs])
+
+
+def free_run_speed(n_grid, a, pos, dt, steps, precision, closure=None, warmup=20):
+    """Steps per second of free suspensions (the production use) with the
+    closure attached or not."""
+    import time
+    n = len(pos)
+    g = SPM(_params(n_grid, a, n, dt, precision))
+    p = abi.new_particles(n)
+    p["x"], p["x_nopbc"] = pos, pos
+    g.put_particles(p)
+    g.init_fluid()
+    if closure is not None:
+        g.set_closure(closure)
+    g.step(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    g.close()
+    return steps / (t1 - t0)
+
+
+def report(seeds=8, steps=600, train_steps=40000, log=print):
+    """The learned closure at the sizes of the coarse/fine comparison: pairs
+    of 64^3 (a = 3) and 256^3 (a = 12) resistance runs at volume fractions
+    0.05 .. 0.3 (`seeds` suspensions each), training on the GPU with the 48
+    lattice symmetries, evaluation on two held-out suspensions of 512
+    spheres in 64^3 (volume fraction 0.22) with the network attached to an
+    FP32 coarse step, and the speed of that step against the FP64 fine step."""
+    import time
+    t0 = time.time()
+    cfg = [(phi, 100 + k) for k in range(seeds) for phi in (0.05, 0.1, 0.15, 0.2, 0.25, 0.3)]
+    x, t, fine, coarse = dataset(cfg, steps=steps)
+    t_data = time.time() - t0
+    log("training pairs: %d suspensions, %d particles, %.0f s" % (len(cfg), len(x), t_data))
+    xa, ta = augment(x, t)
+    t1 = time.time()
+    net, loss = train_closure(xa, ta, steps=train_steps, batch=1024, lr=5e-4, max_batch=8192)
+    t_train = time.time() - t1
+    log("training: %d samples after symmetries, %d Adam steps of 1024, loss %.4f -> %.4f, %.1f s"
+        % (len(xa), train_steps, loss[:20].mean(), loss[-20:].mean(), t_train))
+    errs = {"fp64_coarse": [], "fp32_coarse_closure": [], "torque_fp64_coarse": [], "torque_fp32_coarse_closure": []}
+    for seed in (900, 901):
+        pos, v, w = random_suspension(512, 64, 3.0, None, seed)
+        fc, tc = resistance_run(64, 3.0, pos, v, w, 0.05, steps)
+        ff, tf = resistance_run(256, 12.0, pos * 4, v / 4, w / 16, 0.05 * 16, steps)
+        fx, tx = resistance_run(64, 3.0, pos, v, w, 0.05, steps, precision=abi.FP32, closure=net)
+        errs["fp64_coarse"].append(drag_error(fc, ff))
+        errs["fp32_coarse_closure"].append(drag_error(fx, ff))
+        errs["torque_fp64_coarse"].append(drag_error(tc, tf / 4))
+        errs["torque_fp32_coarse_closure"].append(drag_error(tx, tf / 4))
+    stats = {}
+    for k, v in errs.items():
+        e = np.concatenate(v)
+        stats[k] = {"median": float(np.median(e)), "mean": float(e.mean()), "p95": float(np.percentile(e, 95))}
+        log("%-28s median %.4f mean %.4f p95 %.4f" % (k, stats[k]["median"], stats[k]["mean"], stats[k]["p95"]))
+    pos, _, _ = random_suspension(512, 64, 3.0, None, 900)
+    speed = {"fp32_64cube_closure": free_run_speed(64, 3.0, pos, 0.05, 1000, abi.FP32, closure=net),
+             "fp64_64cube": free_run_speed(64, 3.0, pos, 0.05, 1000, abi.FP64),
+             "fp64_256cube_fine": free_run_speed(256, 12.0, pos * 4, 0.8, 200, abi.FP64)}
+    for k, v in speed.items():
+        log("%-22s %.1f steps/s" % (k, v))
+    ratio = speed["fp32_64cube_closure"] / speed["fp64_256cube_fine"]
+    log("FP32 coarse + closure vs FP64 fine: %.1fx the steps/s" % ratio)
+    return {"pairs": len(cfg), "particles": int(len(x)), "samples_after_symmetries": int(len(xa)),
+            "train_steps": train_steps, "batch": 1024, "lr": 5e-4, "loss_first": float(loss[:20].mean()),
+            "loss_last": float(loss[-20:].mean()), "seconds_pairs": t_data, "seconds_training": t_train,
+            "drag_error": stats, "steps_per_s": speed, "speedup_vs_fine": ratio,
+            "coarse": "64^3, a = 3, xi = 2 grid points, dt = 0.05", "fine": "256^3, a = 12, xi = 2, dt = 0.8",
+            "evaluation": "2 held-out suspensions of 512 spheres in 64^3 (volume fraction 0.22), "
+                          "resistance problem, 600 steps"}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="coarse/fine closure report")
+    ap.add_argument("--seeds", type=int, default=8)
+    ap.add_argument("--train-steps", type=int, default=40000)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    r = report(seeds=a.seeds, train_steps=a.train_steps, log=lambda s: print(s, flush=True))
+    print(json.dumps(r))
+    if a.out:
+        with open(a.out, "w") as fh:
+            json.dump(r, fh, indent=1)

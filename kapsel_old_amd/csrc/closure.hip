@@ -5,10 +5,11 @@
 // correction to Calc_f_hydro_correct_precision's force and torque
 // (md_force.cxx:263-389).
 //
-// Layout.  A block takes a tile of 64 samples (particles); its four
-// wavefronts take 16 rows each.  Activations of a tile sit in LDS as
-// [sample][128] BF16; each layer is 8 output tiles of 16 x 16 per
-// wavefront over 8 k-steps of 16.  Weights are kept in FP32 (master) and
+// Layout.  The forward pass takes tiles of 16 samples (particles) per
+// block, its four wavefronts splitting the output columns; the backward pass
+// tiles of 64, one 16-row slice per wavefront.  Activations of a tile sit in
+// LDS as [sample][128] BF16; a layer is a set of 16 x 16 output tiles over
+// 8 k-steps of 16.  Weights are kept in FP32 (master) and
 // twice in BF16: [out][in] for the forward product (four consecutive k of
 // one output column per lane) and [in][out] for the backward product
 // dH = dZ W^T.  Weight gradients dW = H^T dZ are products over the tile's
@@ -99,16 +100,20 @@ __device__ __forceinline__ void mfma_rows(const bf16_t *act, int row0, const bf1
   }
 }
 
-// Forward pass of a tile of 64 samples.  With `target`, also the output
-// gradient of the mean squared error (normalised units) and the tile's sum
-// of squared errors; with wk.z, the activations a backward pass needs.
+// Forward pass of a tile of 16 samples: the four wavefronts split each
+// layer's 128 output columns (two 16 x 16 tiles each; the last layer's
+// single tile by wavefront 0), so a small batch still spreads over many
+// blocks and each wavefront's chain of matrix instructions stays short.
+// With `target`, also the output gradient of the mean squared error
+// (normalised units) and the tile's sum of squared errors; with wk.z, the
+// activations a backward pass needs.
+constexpr int FTILE = 16;
 __global__ __launch_bounds__(THREADS) void mlp_fwd_kernel(Net net, const float *__restrict__ x, int n, float *y,
                                                           const float *__restrict__ target, Work wk) {
-  __shared__ bf16_t act[2][TILE * W];
-  __shared__ float red[THREADS / 64];
+  __shared__ bf16_t act[2][FTILE * W];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const long s0 = (long)blockIdx.x * TILE;
-  for (int e = tid; e < TILE * W; e += THREADS) {
+  const long s0 = (long)blockIdx.x * FTILE;
+  for (int e = tid; e < FTILE * W; e += THREADS) {
     const int r = e / W, col = e % W;
     const long s = s0 + r;
     float v = 0.f;
@@ -118,18 +123,17 @@ __global__ __launch_bounds__(THREADS) void mlp_fwd_kernel(Net net, const float *
     if (wk.h[0] && s < n) wk.h[0][s * W + col] = h;
   }
   __syncthreads();
-  const int row0 = 16 * wave;
   int cur = 0;
   for (int l = 0; l < 3; ++l) {
-    f32x4 acc[8];
-    mfma_rows<8>(act[cur], row0, net.wt[l], acc, lane);
+    f32x4 acc[2];
+    mfma_rows<2>(act[cur], 0, net.wt[l] + 32 * wave * W, acc, lane);
     const float *b = net.p + OFF_B[l];
 #pragma unroll
-    for (int nt = 0; nt < 8; ++nt) {
-      const int o = nt * 16 + (lane & 15);
+    for (int nt = 0; nt < 2; ++nt) {
+      const int o = 32 * wave + nt * 16 + (lane & 15);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int r = row0 + 4 * (lane >> 4) + j;
+        const int r = 4 * (lane >> 4) + j;
         const long s = s0 + r;
         const float z = acc[nt][j] + b[o];
         const bf16_t h = f2bf(gelu(z));
@@ -143,32 +147,30 @@ __global__ __launch_bounds__(THREADS) void mlp_fwd_kernel(Net net, const float *
     __syncthreads();
     cur ^= 1;
   }
-  f32x4 acc4[1];
-  mfma_rows<1>(act[cur], row0, net.wt[3], acc4, lane);
   float sse = 0.f;
-  const int o = lane & 15;
+  if (wave == 0) {
+    f32x4 acc4[1];
+    mfma_rows<1>(act[cur], 0, net.wt[3], acc4, lane);
+    const int o = lane & 15;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const long s = s0 + row0 + 4 * (lane >> 4) + j;
-    if (s >= n) continue;
-    const float yn = acc4[0][j] + net.p[OFF_B[3] + o];
-    if (o < OUT) {
-      if (y) y[s * OUT + o] = yn * net.oscale[o];
-      if (target) {
-        const float d = yn - target[s * OUT + o] / net.oscale[o];
-        wk.dy[s * OUTP + o] = 2.f * d / ((float)n * OUT);
-        sse += d * d;
+    for (int j = 0; j < 4; ++j) {
+      const long s = s0 + 4 * (lane >> 4) + j;
+      if (s >= n) continue;
+      const float yn = acc4[0][j] + net.p[OFF_B[3] + o];
+      if (o < OUT) {
+        if (y) y[s * OUT + o] = yn * net.oscale[o];
+        if (target) {
+          const float d = yn - target[s * OUT + o] / net.oscale[o];
+          wk.dy[s * OUTP + o] = 2.f * d / ((float)n * OUT);
+          sse += d * d;
+        }
+      } else if (target) {
+        wk.dy[s * OUTP + o] = 0.f;
       }
-    } else if (target) {
-      wk.dy[s * OUTP + o] = 0.f;
     }
-  }
-  if (target) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) sse += __shfl_xor(sse, off, 64);
-    if (lane == 0) red[wave] = sse;
-    __syncthreads();
-    if (tid == 0) wk.loss[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (target && lane == 0) wk.loss[blockIdx.x] = sse;
   }
 }
 
@@ -336,21 +338,32 @@ __global__ void loss_sum_kernel(const float *part, int nblocks, float *out) {
 // ---------------------------------------------------------------------
 namespace {
 
-// phi of the grid point gi (after the clamp), over its tile's list
+// phi at the point x (after the clamp): the sum of the particles' profiles
+// Phi(|x - x_p|) (profile.h:29-49) over the list of the tile holding the
+// grid point below x, which has every particle whose profile reaches x
+// (a particle reaching x has that grid point, within sqrt(3) dx of x,
+// inside its stencil).
 __device__ __forceinline__ double phi_at(const Grid &g, const PModel &pm, const double *__restrict__ pb,
-                                         const int *__restrict__ pxint, const double *__restrict__ pres,
                                          const int *__restrict__ tile_start, const int *__restrict__ tile_count,
-                                         const int *__restrict__ tile_list, const int gi[3]) {
+                                         const int *__restrict__ tile_list, const int gi[3], const double x[3]) {
   const int tile = ((gi[0] / phi::kTile) * (g.ny / phi::kTile) + gi[1] / phi::kTile) * (g.nz / phi::kTile) +
                    gi[2] / phi::kTile;
-  double ph, a0, a1, a2;
-  phi::phi_point<double>(g, pm, pb, pxint, pres, tile_list + tile_start[tile], tile_count[tile], gi, false, ph, a0,
-                         a1, a2);
+  const int *lst = tile_list + tile_start[tile];
+  const int cnt = tile_count[tile];
+  const double rcut = pm.radius + pm.hxi;
+  double ph = 0.0;
+  for (int q = 0; q < cnt; ++q) {
+    const int n = lst[q];
+    const double xp[3] = {phi::P(pb, pm.np, P_X, n), phi::P(pb, pm.np, P_X + 1, n), phi::P(pb, pm.np, P_X + 2, n)};
+    double x12[3];
+    const double d = phi::distance0(g, x, xp, x12);
+    if (d < rcut) ph += phi::phi_fast<double>(pm, d);
+  }
   return ph > 1.0 ? 1.0 : ph;
 }
 
-// One thread per (particle, offset): u - v_p and phi at x_p + h o,
-// trilinear in the 8 grid points around that point (periodic), so the
+// One thread per (particle, offset): u - v_p (trilinear in the 8 grid
+// points around the sample point, periodic) and phi at x_p + h o, so the
 // inputs move smoothly with the particle and turn with the lattice's
 // symmetries.  The centre thread also writes the relative (slip) velocity:
 // the phi-weighted mean of v + w x r - u over the particle, which
@@ -358,11 +371,11 @@ __device__ __forceinline__ double phi_at(const Grid &g, const PModel &pm, const 
 // F = -(rho dx^3 / dt) sum phi (v + w x r - u) (md_force.cxx:263-389),
 // taken here as -F dt / (rho V_p), V_p = 4 pi a^3 / 3.
 template <typename T>
-__global__ void closure_samples_kernel(Grid g, PModel pm, const double *__restrict__ pb, const int *__restrict__ pxint,
-                                       const double *__restrict__ pres, const int *__restrict__ tile_start,
-                                       const int *__restrict__ tile_count, const int *__restrict__ tile_list,
-                                       const T *__restrict__ u0, const T *__restrict__ u1, const T *__restrict__ u2,
-                                       double h, double slip_scale, float *x) {
+__global__ void closure_samples_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+                                       const int *__restrict__ tile_start, const int *__restrict__ tile_count,
+                                       const int *__restrict__ tile_list, const T *__restrict__ u0,
+                                       const T *__restrict__ u1, const T *__restrict__ u2, double h,
+                                       double slip_scale, float *x) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 27L * pm.np) return;
   const int n = (int)(t / 27), o = (int)(t % 27);
@@ -370,60 +383,80 @@ __global__ void closure_samples_kernel(Grid g, PModel pm, const double *__restri
   const int N[3] = {g.nx, g.ny, g.nz};
   const double L[3] = {g.lx, g.ly, g.lz};
   int i0[3];
-  double fr[3];
+  double fr[3], xs[3];
   for (int d = 0; d < 3; ++d) {
-    double xs = phi::P(pb, pm.np, P_X + d, n) + od[d] * h;
-    xs -= L[d] * floor(xs / L[d]);
-    const double gx = xs / pm.dx;
+    xs[d] = phi::P(pb, pm.np, P_X + d, n) + od[d] * h;
+    xs[d] -= L[d] * floor(xs[d] / L[d]);
+    const double gx = xs[d] / pm.dx;
     const double fl = floor(gx);
     i0[d] = ((int)fl % N[d] + N[d]) % N[d];
     fr[d] = gx - fl;
   }
-  double us[3] = {0.0, 0.0, 0.0}, ph = 0.0;
+  double us[3] = {0.0, 0.0, 0.0};
   for (int c = 0; c < 8; ++c) {
     const int cb[3] = {(c >> 2) & 1, (c >> 1) & 1, c & 1};
-    int gi[3];
+    long im = 0;
     double wgt = 1.0;
-    for (int d = 0; d < 3; ++d) {
-      gi[d] = (i0[d] + cb[d]) % N[d];
-      wgt *= cb[d] ? fr[d] : 1.0 - fr[d];
-    }
-    const long im = ((long)gi[0] * g.ny + gi[1]) * g.nz_ + gi[2];
+    for (int d = 0; d < 3; ++d) wgt *= cb[d] ? fr[d] : 1.0 - fr[d];
+    im = ((long)((i0[0] + cb[0]) % N[0]) * g.ny + (i0[1] + cb[1]) % N[1]) * g.nz_ + (i0[2] + cb[2]) % N[2];
     us[0] += wgt * (double)u0[im];
     us[1] += wgt * (double)u1[im];
     us[2] += wgt * (double)u2[im];
-    ph += wgt * phi_at(g, pm, pb, pxint, pres, tile_start, tile_count, tile_list, gi);
   }
   float *xr = x + (long)n * mlp::IN;
   for (int d = 0; d < 3; ++d) xr[3 + 3 * o + d] = (float)(us[d] - phi::P(pb, pm.np, P_V + d, n));
-  xr[84 + o] = (float)ph;
+  xr[84 + o] = (float)phi_at(g, pm, pb, tile_start, tile_count, tile_list, i0, xs);
   if (o == 13)
     for (int d = 0; d < 3; ++d) xr[d] = (float)(slip_scale * phi::P(pb, pm.np, P_F_HYDRO + d, n));
 }
 
-// gap vector to the nearest other particle (minimum image), all pairs
-__global__ void closure_gap_kernel(Grid g, PModel pm, const double *__restrict__ pb, float *x) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+// gap vector to the nearest other particle (minimum image): one wavefront
+// per particle, lanes striding over the others, then a shuffle reduction
+// keeping the smallest distance and, among equal ones, the lowest index
+__global__ __launch_bounds__(256) void closure_gap_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+                                                          float *x) {
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (n >= pm.np) return;
   const double L[3] = {g.lx, g.ly, g.lz};
+  const double iL[3] = {1.0 / g.lx, 1.0 / g.ly, 1.0 / g.lz};
   const double xn[3] = {phi::P(pb, pm.np, P_X, n), phi::P(pb, pm.np, P_X + 1, n), phi::P(pb, pm.np, P_X + 2, n)};
-  double best = 1e300, rb[3] = {0.0, 0.0, 0.0};
-  for (int m = 0; m < pm.np; ++m) {
+  double best = 1e300;
+  int bi = pm.np;
+  for (int m = lane; m < pm.np; m += 64) {
     if (m == n) continue;
-    double r[3], r2 = 0.0;
+    double r2 = 0.0;
     for (int d = 0; d < 3; ++d) {
-      r[d] = phi::P(pb, pm.np, P_X + d, m) - xn[d];
-      r[d] -= L[d] * rint(r[d] / L[d]);
-      r2 += r[d] * r[d];
+      double r = phi::P(pb, pm.np, P_X + d, m) - xn[d];
+      r -= L[d] * rint(r * iL[d]);
+      r2 += r * r;
     }
     if (r2 < best) {
       best = r2;
-      for (int d = 0; d < 3; ++d) rb[d] = r[d];
+      bi = m;
     }
   }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ob = __shfl_xor(best, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    if (ob < best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  if (lane != 0) return;
   float *xr = x + (long)n * mlp::IN;
-  const double f = best < 1e300 ? 1.0 - 2.0 * pm.radius / sqrt(best) : 0.0;
-  for (int d = 0; d < 3; ++d) xr[111 + d] = (float)(rb[d] * f);
+  if (bi >= pm.np) {
+    for (int d = 0; d < 3; ++d) xr[111 + d] = 0.f;
+    return;
+  }
+  const double f = 1.0 - 2.0 * pm.radius / sqrt(best);
+  for (int d = 0; d < 3; ++d) {
+    double r = phi::P(pb, pm.np, P_X + d, bi) - xn[d];
+    r -= L[d] * rint(r * iL[d]);
+    xr[111 + d] = (float)(r * f);
+  }
 }
 
 // f_hydro += y[0..3), torque_hydro += y[3..6) (torque only with rotation,
@@ -539,6 +572,7 @@ spm::mlp::Bf16Ptrs bf_of(spm_closure *cl) {
 }
 
 inline unsigned tiles_of(int n) { return (unsigned)((n + spm::mlp::TILE - 1) / spm::mlp::TILE); }
+inline unsigned ftiles_of(int n) { return (unsigned)((n + spm::mlp::FTILE - 1) / spm::mlp::FTILE); }
 
 int refresh(spm_closure *cl) {
   hipLaunchKernelGGL(spm::mlp::bf16_refresh_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, cl->stream, cl->p,
@@ -550,7 +584,7 @@ int refresh(spm_closure *cl) {
 // forward of n samples at device x (training saves with target != null)
 int fwd(spm_closure *cl, const float *x, float *y, const float *target, int n) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(spm::mlp::mlp_fwd_kernel, dim3(tiles_of(n)), dim3(spm::mlp::THREADS), 0, cl->stream, net_of(cl),
+  hipLaunchKernelGGL(spm::mlp::mlp_fwd_kernel, dim3(ftiles_of(n)), dim3(spm::mlp::THREADS), 0, cl->stream, net_of(cl),
                      x, n, y, target, work_of(cl, target != nullptr));
   CL_HIP(cl, hipGetLastError());
   return 0;
@@ -561,7 +595,7 @@ int fwd_bwd(spm_closure *cl, const float *x, const float *target, int n) {
   CL_RC(fwd(cl, x, nullptr, target, n));
   hipLaunchKernelGGL(spm::mlp::mlp_bwd_kernel, dim3(tiles_of(n)), dim3(spm::mlp::THREADS), 0, cl->stream, net_of(cl),
                      work_of(cl, true), n, cl->partial);
-  hipLaunchKernelGGL(spm::mlp::loss_sum_kernel, dim3(1), dim3(64), 0, cl->stream, cl->loss, (int)tiles_of(n),
+  hipLaunchKernelGGL(spm::mlp::loss_sum_kernel, dim3(1), dim3(64), 0, cl->stream, cl->loss, (int)ftiles_of(n),
                      cl->loss_sum);
   CL_HIP(cl, hipGetLastError());
   return 0;
@@ -608,15 +642,15 @@ int closure_apply(spm_ctx *c) {
   tbeg(c);
   if (c->prm.precision == SPM_FP32)
     hipLaunchKernelGGL(closure_samples_kernel<float>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, c->stream,
-                       c->grid, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count, c->tile_list,
+                       c->grid, c->pm, c->pbuf, c->tile_start, c->tile_count, c->tile_list,
                        (const float *)c->u[0], (const float *)c->u[1], (const float *)c->u[2],
                        c->prm.a * c->prm.dx, slip_scale(c), cl->x);
   else
     hipLaunchKernelGGL(closure_samples_kernel<double>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, c->stream,
-                       c->grid, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count, c->tile_list,
+                       c->grid, c->pm, c->pbuf, c->tile_start, c->tile_count, c->tile_list,
                        (const double *)c->u[0], (const double *)c->u[1], (const double *)c->u[2],
                        c->prm.a * c->prm.dx, slip_scale(c), cl->x);
-  hipLaunchKernelGGL(closure_gap_kernel, dim3((c->np + 127) / 128), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf,
+  hipLaunchKernelGGL(closure_gap_kernel, dim3((c->np + 3) / 4), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
                      cl->x);
   tend(c, SPM_K_PARTICLE, 4.0 * 114 * c->np);
   SPM_LAUNCH_CHECK(c);
@@ -671,7 +705,7 @@ int spm_closure_create(int device, int max_batch, spm_closure **out) {
   const size_t nb = (size_t)max_batch, tiles = tiles_of(max_batch);
   if ((rc = cl_alloc(cl, &cl->p, NPARAM)) || (rc = cl_alloc(cl, &cl->m, NPARAM)) || (rc = cl_alloc(cl, &cl->v, NPARAM)) ||
       (rc = cl_alloc(cl, &cl->scal, 272)) || (rc = cl_alloc(cl, &cl->dy, nb * 16)) ||
-      (rc = cl_alloc(cl, &cl->loss, tiles)) || (rc = cl_alloc(cl, &cl->loss_sum, 1)) ||
+      (rc = cl_alloc(cl, &cl->loss, ftiles_of(max_batch))) || (rc = cl_alloc(cl, &cl->loss_sum, 1)) ||
       (rc = cl_alloc(cl, &cl->partial, tiles * NPARAM)) || (rc = cl_alloc(cl, &cl->grad, NPARAM)) ||
       (rc = cl_alloc(cl, &cl->x, nb * spm::mlp::IN)) || (rc = cl_alloc(cl, &cl->y, nb * spm::mlp::OUT)) ||
       (rc = cl_alloc(cl, &cl->t, nb * spm::mlp::OUT)))
@@ -895,15 +929,15 @@ int spm_closure_features(spm_ctx *c, float *x) {
     const long nthr = 27L * c->np;
     if (c->prm.precision == SPM_FP32)
       hipLaunchKernelGGL(spm::closure_samples_kernel<float>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0,
-                         c->stream, c->grid, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count,
+                         c->stream, c->grid, c->pm, c->pbuf, c->tile_start, c->tile_count,
                          c->tile_list, (const float *)c->u[0], (const float *)c->u[1], (const float *)c->u[2],
                          c->prm.a * c->prm.dx, spm::slip_scale(c), d);
     else
       hipLaunchKernelGGL(spm::closure_samples_kernel<double>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0,
-                         c->stream, c->grid, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count,
+                         c->stream, c->grid, c->pm, c->pbuf, c->tile_start, c->tile_count,
                          c->tile_list, (const double *)c->u[0], (const double *)c->u[1], (const double *)c->u[2],
                          c->prm.a * c->prm.dx, spm::slip_scale(c), d);
-    hipLaunchKernelGGL(spm::closure_gap_kernel, dim3((c->np + 127) / 128), dim3(128), 0, c->stream, c->grid, c->pm,
+    hipLaunchKernelGGL(spm::closure_gap_kernel, dim3((c->np + 3) / 4), dim3(256), 0, c->stream, c->grid, c->pm,
                        c->pbuf, d);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(x, d, sizeof(float) * (size_t)c->np * spm::mlp::IN, hipMemcpyDeviceToHost,

@@ -168,21 +168,45 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int *__restrict__ coun
 
 inline unsigned scan_blocks(int n) { return (unsigned)((n + 4095) / 4096 > 0 ? (n + 4095) / 4096 : 1); }
 
-// Sort each list segment ascending (segments are short: insertion sort).
-__global__ void sort_segments_kernel(const int *__restrict__ start, const int *__restrict__ count, int *list, int n) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// Sort each list segment ascending, one 64-thread block per segment.  A
+// segment of up to kSortLds entries is copied to LDS and every entry goes
+// straight to its rank (the number of smaller entries, equal ones ranked by
+// position, so repeated entries keep their count); a longer one falls back
+// to an insertion sort by one lane.
+constexpr int kSortLds = 1024;
+__global__ __launch_bounds__(64) void sort_segments_kernel(const int *__restrict__ start, const int *__restrict__ count,
+                                                           int *list, int n) {
+  const int s = blockIdx.x;
   if (s >= n) return;
   int *a = list + start[s];
   const int m = count[s];
-  for (int i = 1; i < m; ++i) {
-    const int v = a[i];
-    int j = i - 1;
-    while (j >= 0 && a[j] > v) {
-      a[j + 1] = a[j];
-      --j;
+  if (m <= 1) return;
+  const int lane = threadIdx.x;
+  if (m <= kSortLds) {
+    __shared__ int buf[kSortLds];
+    for (int i = lane; i < m; i += 64) buf[i] = a[i];
+    __syncthreads();
+    for (int i = lane; i < m; i += 64) {
+      const int v = buf[i];
+      int r = 0;
+      for (int j = 0; j < m; ++j) {
+        const int w = buf[j];
+        r += (w < v) || (w == v && j < i);
+      }
+      a[r] = v;
     }
-    a[j + 1] = v;
+    return;
   }
+  if (lane == 0)
+    for (int i = 1; i < m; ++i) {
+      const int v = a[i];
+      int j = i - 1;
+      while (j >= 0 && a[j] > v) {
+        a[j + 1] = a[j];
+        --j;
+      }
+      a[j + 1] = v;
+    }
 }
 
 __global__ void izero_kernel(int *a, int n) {
@@ -1093,7 +1117,7 @@ int p_build_tiles(spm_ctx *c) {
                      c->ntiles);
   hipLaunchKernelGGL(tile_fill_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->pm, c->pxint,
                      c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_cursor, c->tile_list);
-  hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_start,
+  hipLaunchKernelGGL(sort_segments_kernel, dim3(c->ntiles), dim3(64), 0, c->stream, c->tile_start,
                      c->tile_count, c->tile_list, c->ntiles);
   tend(c, SPM_K_PARTICLE, 16.0 * c->ntiles);
   SPM_LAUNCH_CHECK(c);
@@ -1221,7 +1245,7 @@ int p_force(spm_ctx *c) {
                        c->ncells);
     hipLaunchKernelGGL(cell_fill_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pcell,
                        c->cell_cursor, c->cell_list);
-    hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ncells, 256)), dim3(256), 0, c->stream, c->cell_start,
+    hipLaunchKernelGGL(sort_segments_kernel, dim3(c->ncells), dim3(64), 0, c->stream, c->cell_start,
                        c->cell_count, c->cell_list, c->ncells);
   }
   hipLaunchKernelGGL(force_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
@@ -1309,7 +1333,7 @@ int p_make_phi_obl(spm_ctx *c, const double *sre) {
                      c->tile_start_obl, c->tile_cursor_obl, c->ntiles);
   hipLaunchKernelGGL(obl_tiles_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
                      c->pxint, c->le_deg, k[0], k[1], k[2], c->tile_count_obl, c->tile_cursor_obl, c->tile_list_obl, 1);
-  hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_start_obl,
+  hipLaunchKernelGGL(sort_segments_kernel, dim3(c->ntiles), dim3(64), 0, c->stream, c->tile_start_obl,
                      c->tile_count_obl, c->tile_list_obl, c->ntiles);
   tend(c, SPM_K_PARTICLE, 16.0 * c->ntiles);
   SPM_LAUNCH_CHECK(c);

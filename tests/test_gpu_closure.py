@@ -143,9 +143,9 @@ def _trilinear(field, pts):
 
 
 def test_features_match_numpy(SPM):
-    """spm_closure_features against numpy: trilinear u and phi at x_p + a o
-    (phi from the oracle's Make_phi_u_particle, which clamps), the slip
-    velocity from f_hydro, and the nearest-neighbour gap vector."""
+    """spm_closure_features against numpy: trilinear u at x_p + a o, phi
+    there as the clamped sum of the particles' profiles (the oracle's Phi),
+    the slip velocity from f_hydro, and the nearest-neighbour gap vector."""
     from kapsel_old_amd import abi
     from oracle.oracle import Oracle
     from tests.helpers import particles, smooth_field
@@ -161,8 +161,6 @@ def test_features_match_numpy(SPM):
         o.put_field("u%d" % d, u)
     g.Calc_f_hydro_correct_precision()
     x = g.closure_features()
-    o.Make_phi_u_particle()
-    phi = o.get_field("phi")[:, :, :n]
     q = g.get_particles()
     offs = np.array([(k // 9 - 1, (k // 3) % 3 - 1, k % 3 - 1) for k in range(27)])
     us = [g.get_field("u%d" % d)[:, :, :n] for d in range(3)]
@@ -171,7 +169,12 @@ def test_features_match_numpy(SPM):
         for d in range(3):
             want = _trilinear(us[d], pts) - q["v"][m][d]
             assert np.abs(x[m, 3 + 3 * np.arange(27) + d] - want).max() < 1e-6
-        assert np.abs(x[m, 84:111] - _trilinear(phi, pts)).max() < 1e-6
+        phi = []
+        for pt in pts:
+            r = q["x"] - pt
+            r -= n * np.rint(r / n)
+            phi.append(min(1.0, sum(o.phi_profile(d) for d in np.linalg.norm(r, axis=1))))
+        assert np.abs(x[m, 84:111] - np.array(phi)).max() < 1e-6
         slip = -q["f_hydro"][m] * prm.dt / (prm.rho * 4.0 / 3.0 * np.pi * a ** 3)
         assert np.allclose(x[m, :3], slip, rtol=1e-6, atol=1e-9)
         r = q["x"] - q["x"][m]
