@@ -125,26 +125,72 @@ def load_traffic(schedule, precision, n, np_):
     return out, os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(n, np_, max_steps=2, budget_s=40.0, shear_rate=0.0):
-    """The CPU oracle (single-threaded FP64 restatement of the reference
-    step) on the same workload: init untimed, then up to max_steps steps."""
-    from oracle.oracle import Oracle
+def cpu_baseline(n, np_, samples=3, shear_rate=0.0, threads=(1,)):
+    """The CPU oracle (FP64 restatement of the reference step) on the same
+    workload: init and one step untimed, then `samples` single steps timed
+    for each thread count; returns {threads: (median steps/s, seconds per
+    sample list)}."""
+    from oracle.oracle import Oracle, set_threads
     o = Oracle(workload_params(n, np_))
     o.put_particles(workload_particles(n, np_))
     o.init_fluid()
     if shear_rate > 0:
         o.set_lees_edwards(shear_rate)
-    done = 0
-    t0 = time.perf_counter()
-    while done < max_steps:
-        o.step(1)
-        done += 1
-        if time.perf_counter() - t0 > budget_s / max_steps * done and done >= 1:
-            if time.perf_counter() - t0 > budget_s:
-                break
-    t1 = time.perf_counter()
+    out = {}
+    for th in threads:
+        set_threads(th)
+        o.step(1)  # untimed: first touch with this thread count
+        secs = []
+        for _ in range(samples):
+            t0 = time.perf_counter()
+            o.step(1)
+            secs.append(time.perf_counter() - t0)
+        out[th] = (1.0 / float(np.median(secs)), secs)
     o.close()
-    return done / (t1 - t0), done, t1 - t0
+    return out
+
+
+def one_step_calls(n, np_, steps, precision=abi.FP64):
+    """Steps per second when every step is its own spm_step(1) call (the
+    per-call rebuild of u, phi and up that the fused schedule does once per
+    call is then paid every step)."""
+    from kapsel_old_amd.spm import SPM
+    g = SPM(workload_params(n, np_, precision))
+    g.put_particles(workload_particles(n, np_))
+    g.init_fluid()
+    for _ in range(3):
+        g.step(1)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step(1)
+    g.synchronize()
+    t1 = time.perf_counter()
+    g.close()
+    return steps / (t1 - t0)
+
+
+def closure_line():
+    """The learned closure at the coarse/fine sizes of its report: live
+    steps/s of the FP32 64^3 step (512 spheres a = 3) with a network attached
+    and of the FP64 256^3 step with the same spheres at a = 12, and the drag
+    errors of the committed report (profiles/closure_report_64cube_vs_256cube.json)."""
+    from kapsel_old_amd import closure as cl
+    from kapsel_old_amd.spm import Closure
+    pos, _, _ = cl.random_suspension(512, 64, 3.0, None, 900)
+    net = Closure(1024, seed=1)
+    coarse = cl.free_run_speed(64, 3.0, pos, 0.05, 1000, abi.FP32, closure=net)
+    fine = cl.free_run_speed(256, 12.0, pos * 4, 0.8, 100, abi.FP64)
+    out = {"workload": "512 spheres, volume fraction 0.22: FP32 64^3 a=3 + closure vs FP64 256^3 a=12",
+           "steps_per_s_fp32_coarse_closure": coarse, "steps_per_s_fp64_fine": fine, "speedup": coarse / fine}
+    path = os.path.join(ROOT, "profiles", "closure_report_64cube_vs_256cube.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            rep = json.load(fh)
+        out["drag_error_median_fp64_coarse"] = rep["drag_error"]["fp64_coarse"]["median"]
+        out["drag_error_median_fp32_coarse_closure"] = rep["drag_error"]["fp32_coarse_closure"]["median"]
+        out["drag_error_source"] = os.path.relpath(path, ROOT)
+    return out
 
 
 def main():
@@ -160,6 +206,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB")
     ap.add_argument("--shear-rate", type=float, default=0.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline line (no FP32, one-step-call or closure objects)")
     args = ap.parse_args()
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
@@ -209,7 +257,9 @@ def main():
                                   (", Lees-Edwards Shear_rate %g" % args.shear_rate) if sheared else ""),
                    "grid": [n, n, n], "n_particles": np_, "stencil_points": stencil, "precision": args.precision,
                    "schedule": args.schedule},
-        "hbm_TBps_algorithmic": b_step * value / 1e12,
+        # the 20-transform model of the step's bytes over the measured time: an
+        # effective bandwidth (the fused schedule runs 14 transforms)
+        "hbm_TBps_effective": b_step * value / 1e12,
         "step_floor_ms_at_8TBps": b_step / 8e12 * 1e3,
         "roofline": {"bound": "hbm", "kernel": FAMILIES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -221,11 +271,34 @@ def main():
         "kernel_families": families,
         "profiled_ms_per_step": 1e3 * prof_elapsed / args.profile_steps,
     }
+    if not args.no_extras and not sheared and precision == abi.FP64:
+        e32, p32, ms32, by32, cnt32, _ = gpu_run(n, np_, args.steps, args.warmup, args.profile_steps, schedule,
+                                                abi.FP32)
+        d32 = int(np.argmax(ms32))
+        pl32 = ms32[d32] / max(cnt32[d32], 1)
+        out["fp32"] = {"value": args.steps / e32, "ms_per_step": 1e3 * e32 / args.steps,
+                       "dtype": "f32 fields, f64 particles",
+                       "roofline": {"bound": "hbm", "kernel": FAMILIES[d32],
+                                    "achieved": by32[d32] / max(cnt32[d32], 1) / (pl32 * 1e-3) / 1e9,
+                                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": by32[d32] / max(cnt32[d32], 1) / (pl32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                    "us_per_launch": pl32 * 1e3}}
+        out["one_step_per_call"] = {"value": one_step_calls(n, np_, max(20, args.steps // 2)),
+                                    "unit": "time-steps/s", "note": "spm_step(1) per step, FP64 fused"}
+        out["closure"] = closure_line()
     if not args.no_cpu_baseline and precision == abi.FP64:
-        v, done, secs = cpu_baseline(n, np_, shear_rate=args.shear_rate)
-        out["cpu_baseline"] = {"value": v, "unit": "time-steps/s", "cores": 1, "kind": "port",
-                               "sample": "%d step(s) of the same %d^3/%d workload after an untimed init, "
-                                         "%.1f s, oracle/liboracle.so single-threaded" % (done, n, np_, secs)}
+        nthreads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        base = cpu_baseline(n, np_, shear_rate=args.shear_rate, threads=(1, nthreads) if nthreads > 1 else (1,))
+        v1, s1 = base[1]
+        out["cpu_baseline"] = {"value": v1, "unit": "time-steps/s", "cores": 1, "kind": "port",
+                               "sample": "median of %d single steps of the same %d^3/%d workload after an untimed "
+                                         "init and step, oracle/liboracle.so on 1 thread (%s s)"
+                                         % (len(s1), n, np_, ", ".join("%.2f" % x for x in s1))}
+        if nthreads > 1:
+            vn, sn = base[nthreads]
+            out["cpu_baseline"]["all_cores"] = {"value": vn, "cores": nthreads,
+                                                "sample": "median of %d single steps on %d OpenMP threads (%s s)"
+                                                          % (len(sn), nthreads, ", ".join("%.2f" % x for x in sn))}
     print(json.dumps(out), flush=True)
 
 

@@ -39,6 +39,8 @@ def lib():
             f.argtypes = [vp]
             f.restype = ip
         L.oracle_step.argtypes = [vp, ip]
+        L.oracle_set_threads.argtypes = [ip]
+        L.oracle_set_threads.restype = None
         L.oracle_stencil_size.argtypes = [vp]
         L.oracle_stencil.argtypes = [vp, C.POINTER(C.c_int)]
         L.oracle_put_field.argtypes = [vp, ip, dp]
@@ -102,6 +104,12 @@ def lib():
             f.restype = ip
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (line transforms,
+    per-particle force sums); results do not depend on it."""
+    lib().oracle_set_threads(int(n))
 
 
 def _dp(a):

@@ -19,6 +19,9 @@
 
 #include <float.h>
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -54,7 +57,6 @@ struct oracle_ctx {
   uint32_t mt[MT_N];
   int mti;
   fft_plan px, py, pz;          /* pz has n = NZ/2 */
-  double *rowbuf;
   double *ztw;                  /* cos, sin of 2*pi*k/NZ, k <= NZ/2 */
   /* electrolyte (SW_EQ == Electrolyte) */
   int electro;
@@ -86,6 +88,16 @@ static size_t IDX(const oracle_ctx *c, int i, int j, int k) {
 /* X(k) = sum a(x) exp(+2 pi i k.x/N), half spectrum in [NX][NY][NZ_]  */
 /* (fftsg3d.c:109-120, fft_wrapper.h:432-631)                          */
 /* ------------------------------------------------------------------ */
+/* Threads of the parallel loops (the line transforms and the per-particle
+ * force sums); 1 runs the restatement serially.  No effect without OpenMP. */
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
 static int plan_init(fft_plan *pl, int n) {
   pl->n = n;
   pl->c = (double *)malloc(sizeof(double) * n);
@@ -152,8 +164,12 @@ static void fft_batch(const fft_plan *pl, double *a, long stride, int batch, int
 /* A2a_k, fft_wrapper.h:432-527: real field (k < NZ) -> half spectrum. */
 void oracle_A2a_k(oracle_ctx *c, double *a) {
   const int NX = c->NX, NY = c->NY, NZ = c->NZ, M = NZ / 2, H = c->HNZ_;
-  double *z = c->rowbuf;
-  for (int i = 0; i < NX; i++)
+  /* every line is transformed on its own, so the lines are shared out over
+   * OpenMP threads (oracle_set_threads); the result does not depend on how */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < NX; i++) {
+    double zbuf[2 * 1024 + 2];
+    double *z = zbuf;
     for (int j = 0; j < NY; j++) {
       double *row = a + IDX(c, i, j, 0);
       memcpy(z, row, sizeof(double) * NZ);
@@ -172,7 +188,10 @@ void oracle_A2a_k(oracle_ctx *c, double *a) {
         row[2 * k + 1] = ei + (wr * oi + wi * orr);
       }
     }
+  }
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < NX; i++) fft_batch(&c->py, a + IDX(c, i, 0, 0), H, H, +1);
+#pragma omp parallel for schedule(static)
   for (int j = 0; j < NY; j++) fft_batch(&c->px, a + IDX(c, 0, j, 0), (long)NY * H, H, +1);
 }
 
@@ -214,10 +233,14 @@ void oracle_A_k2a(oracle_ctx *c, double *a) {
   const int NX = c->NX, NY = c->NY, NZ = c->NZ, M = NZ / 2, H = c->HNZ_;
   ooura_hermitian_planes(c, a);
   const double scale = 1.0 / ((double)NX * NY * NZ);
-  double *z = c->rowbuf;
+#pragma omp parallel for schedule(static)
   for (int j = 0; j < NY; j++) fft_batch(&c->px, a + IDX(c, 0, j, 0), (long)NY * H, H, -1);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < NX; i++) fft_batch(&c->py, a + IDX(c, i, 0, 0), H, H, -1);
-  for (int i = 0; i < NX; i++)
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < NX; i++) {
+    double zbuf[2 * 1024 + 2];
+    double *z = zbuf;
     for (int j = 0; j < NY; j++) {
       double *row = a + IDX(c, i, j, 0);
       for (int k = 0; k < M; k++) {
@@ -235,6 +258,7 @@ void oracle_A_k2a(oracle_ctx *c, double *a) {
       row[NZ] = 0.0;
       row[NZ + 1] = 0.0;
     }
+  }
 }
 
 /* ------------------------------------------------------------------ */
@@ -623,6 +647,7 @@ static void calc_f_hydro_field(oracle_ctx *c, double *const fld[3]) {
   reset_phi_u(c);
   const double dmy0 = -c->DX3 * c->RHO;
   const double dmy = dmy0 / c->dt;
+#pragma omp parallel for schedule(dynamic, 16)
   for (int n = 0; n < c->Np; n++) {
     spm_particle *pn = &c->p[n];
     int x_int[3], r_mesh[3];
@@ -2202,9 +2227,8 @@ oracle_ctx *oracle_create(const spm_params *prm) {
     fail |= !c->u[d] || !c->up[d] || !c->f[d];
   }
   c->phi = (double *)calloc(c->nfield, sizeof(double));
-  c->rowbuf = (double *)malloc(sizeof(double) * (c->NZ + 2));
   c->ztw = (double *)malloc(sizeof(double) * (c->NZ + 2));
-  fail |= !c->phi || !c->rowbuf || !c->ztw;
+  fail |= !c->phi || !c->ztw;
   if (c->ztw)
     for (int k = 0; k <= c->NZ / 2; k++) {
       long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)c->NZ;
@@ -2254,7 +2278,6 @@ void oracle_destroy(oracle_ctx *c) {
     free(c->f_ns1[n]);
   }
   for (int d = 0; d < 3; d++) free(c->surface_normal[d]);
-  free(c->rowbuf);
   free(c->ztw);
   plan_free(&c->px);
   plan_free(&c->py);

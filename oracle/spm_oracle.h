@@ -122,6 +122,8 @@ uint32_t oracle_genrand_int32(oracle_ctx *c);
 double oracle_genrand_real3(oracle_ctx *c);
 void oracle_gauss2(oracle_ctx *c, double out[2]);
 
+void oracle_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif
