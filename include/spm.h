@@ -75,7 +75,9 @@ enum spm_precision { SPM_FP64 = 0, SPM_FP32 = 1 };
  * follow Set_global_parameters (input.cxx:274-433).
  */
 typedef struct spm_params {
-  int nx, ny, nz;            /* grid; each a power of two, 8..1024       */
+  int nx, ny, nz;            /* grid; each a power of two, 8..1024, or   *
+                              * 48, 80, 96, 160, 192 (literal schedule,   *
+                              * no Lees-Edwards shear)                    */
   double dx;                 /* DX                                        */
   double rho, eta;           /* RHO, ETA (NU = ETA/RHO)                   */
   double kbt;                /* kBT; thermostat runs when > 0             */
@@ -208,7 +210,9 @@ int spm_synchronize(spm_ctx *ctx);
 /* How spm_step runs a step (no counterpart in the reference):
  *   SPM_SCHEDULE_FUSED (default) -- pointwise work fused into the FFT passes;
  *     on return zeta, uk_dc, u, phi, up and the particles hold the reference's
- *     end-of-step state, the f field is scratch;
+ *     end-of-step state, the f field is scratch; power-of-two grids only (a
+ *     grid with a mixed length runs LITERAL, and asking for FUSED there is
+ *     SPM_ERR_UNSUPPORTED);
  *   SPM_SCHEDULE_LITERAL -- one kernel sequence per reference routine, every
  *     field (f included) as the reference leaves it. */
 enum spm_schedule { SPM_SCHEDULE_FUSED = 0, SPM_SCHEDULE_LITERAL = 1 };
@@ -233,7 +237,9 @@ int spm_check_finite(spm_ctx *ctx);
  * reference layout [NX][NY][NZ/2+1 complex] (real data in k < NZ): +1 is
  * the forward transform with the Ooura sign exp(+i k.x), -1 the inverse
  * with the 1/(NX NY NZ) scale and rdft3dsort's reading of the kz = 0 and
- * kz = NZ/2 planes.  Sizes are powers of two in [8, 1024].  `workspace`
+ * kz = NZ/2 planes.  Sizes are powers of two in [8, 1024] or one of the
+ * mixed lengths 48, 80, 96, 160, 192 (factors 2, 3, 5; the reference's
+ * Ooura transform has powers of two only).  `workspace`
  * (device memory of at least spm_fft_workspace_size() bytes, rewritten by
  * every call) holds the roots of unity; with workspace == NULL the call
  * allocates one, waits for `stream` and frees it.  `stream` is a
@@ -249,7 +255,7 @@ int spm_fft3d(int nx, int ny, int nz, int direction, double *field, void *worksp
  * the root tables go to a 65536-byte static device array of the library
  * (one per GPU, held until the process ends) instead of a workspace, so the
  * call allocates nothing and only queues kernels on `stream`.  Sizes are
- * powers of two in [8, 1024] (the tables then fit the array); others get
+ * those of spm_fft3d (the tables then fit the array); others get
  * SPM_ERR_INVALID.  Every call on a GPU rewrites that GPU's array: calls
  * that overlap on one GPU (two streams or two threads) race. */
 int spm_fft3d_noalloc(int nx, int ny, int nz, int direction, double *field, void *stream);

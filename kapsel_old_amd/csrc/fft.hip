@@ -205,6 +205,136 @@ __global__ void ooura_planes_kernel(FieldPtrs<T> fp, int nx, int ny, int nz_, in
 }
 
 // ---------------------------------------------------------------------
+// Lengths with factors 3 and 5 (grids of 48, 80, 96, 160 or 192 points per
+// axis; the reference allows powers of two only, input.cxx:1397-1402): the
+// same passes with the mixed-radix stages of fft_core.h on two LDS buffers.
+// A block of 256 threads takes C columns (y, x) or ROWS rows (z).
+// ---------------------------------------------------------------------
+constexpr int kMixedThreads = 256;
+template <typename T>
+constexpr int kMixedC() { return sizeof(T) == 8 ? 8 : 16; }  // one 128-byte run per row
+constexpr int kMixedRows(int m) { return m >= 1024 ? 1 : (1024 / m > 32 ? 32 : 1024 / m); }
+
+template <typename T, int N, int C, int SIGN>
+__global__ __launch_bounds__(kMixedThreads) void fft_lines_mixed_kernel(FieldPtrs<T> fp,
+                                                                       const double2 *__restrict__ tw_g, LineTiles lt,
+                                                                       long estride) {
+  using C2 = cplx<T>;
+  __shared__ C2 b0[N * C], b1[N * C];
+  __shared__ double2 tw[N];
+  const int tid = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  for (int t = tid; t < N; t += kMixedThreads) tw[t] = tw_g[t];
+  C2 *g = reinterpret_cast<C2 *>(fp.p[blockIdx.z]);
+  for (int e = tid; e < N * C; e += kMixedThreads) {
+    const long col = line_column(lt, tile, e % C, C);
+    b0[e] = col >= 0 ? g[col + (e / C) * estride] : C2{0, 0};
+  }
+  __syncthreads();
+  mixed_stages<T, N, 1, C, SIGN>(b0, b1, tw, tid, kMixedThreads);
+  const C2 *res = (mixed_nstages(N) % 2 == 0) ? b0 : b1;
+  for (int e = tid; e < N * C; e += kMixedThreads) {
+    const long col = line_column(lt, tile, e % C, C);
+    if (col >= 0) g[col + (e / C) * estride] = res[e];
+  }
+}
+
+// z pass forward: ROWS rows of NZ = 2M reals -> M + 1 complex each
+template <typename T, int M, int ROWS>
+__global__ __launch_bounds__(kMixedThreads) void fft_z_r2c_mixed_kernel(FieldPtrs<T> fp,
+                                                                       const double2 *__restrict__ tw_g,
+                                                                       const double2 *__restrict__ twr_g, int nrows,
+                                                                       int nz_) {
+  using C2 = cplx<T>;
+  constexpr int H = M + 1;
+  __shared__ C2 b0[M * ROWS], b1[M * ROWS];
+  __shared__ double2 tw[M], twr[M + 1];
+  T *a = fp.p[blockIdx.z];
+  const int tid = threadIdx.x;
+  for (int t = tid; t < M; t += kMixedThreads) tw[t] = tw_g[t];
+  for (int t = tid; t < M + 1; t += kMixedThreads) twr[t] = twr_g[t];
+  const long row0 = (long)blockIdx.x * ROWS;
+  for (int f = tid; f < ROWS * M; f += kMixedThreads) {
+    const int m = f % M, r = f / M;
+    const long row = row0 + r;
+    b0[m * ROWS + r] = (row < nrows) ? *reinterpret_cast<const C2 *>(a + row * nz_ + 2 * m) : C2{0, 0};
+  }
+  __syncthreads();
+  mixed_stages<T, M, 1, ROWS, +1>(b0, b1, tw, tid, kMixedThreads);
+  const C2 *res = (mixed_nstages(M) % 2 == 0) ? b0 : b1;
+  for (int f = tid; f < ROWS * H; f += kMixedThreads) {
+    const int k = f % H, r = f / H;
+    const long row = row0 + r;
+    const C2 z = res[(k % M) * ROWS + r];
+    C2 zc = res[((M - k) % M) * ROWS + r];
+    zc.y = -zc.y;
+    const C2 e{(T)0.5 * (z.x + zc.x), (T)0.5 * (z.y + zc.y)};
+    const C2 o{(T)0.5 * (z.y - zc.y), (T)-0.5 * (z.x - zc.x)};  // (z - zc) / (2i)
+    const double2 w = twr[k];
+    if (row < nrows) *reinterpret_cast<C2 *>(a + row * nz_ + 2 * k) = cadd(e, cmul(o, C2{(T)w.x, (T)w.y}));
+  }
+}
+
+// z pass inverse: M + 1 complex -> NZ = 2M reals per row, times `scale`
+template <typename T, int M, int ROWS>
+__global__ __launch_bounds__(kMixedThreads) void fft_z_c2r_mixed_kernel(FieldPtrs<T> fp,
+                                                                       const double2 *__restrict__ tw_g,
+                                                                       const double2 *__restrict__ twr_g, int nrows,
+                                                                       int nz_, T scale) {
+  using C2 = cplx<T>;
+  constexpr int H = M + 1, HP = M / 2 + 1;
+  __shared__ C2 b0[H * ROWS], b1[M * ROWS];
+  __shared__ double2 tw[M], twr[M + 1];
+  T *a = fp.p[blockIdx.z];
+  const int tid = threadIdx.x;
+  for (int t = tid; t < M; t += kMixedThreads) tw[t] = tw_g[t];
+  for (int t = tid; t < M + 1; t += kMixedThreads) twr[t] = twr_g[t];
+  const long row0 = (long)blockIdx.x * ROWS;
+  for (int f = tid; f < ROWS * H; f += kMixedThreads) {
+    const int k = f % H, r = f / H;
+    const long row = row0 + r;
+    b0[k * ROWS + r] = (row < nrows) ? *reinterpret_cast<const C2 *>(a + row * nz_ + 2 * k) : C2{0, 0};
+  }
+  __syncthreads();
+  // Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) exp(-2 pi i k/NZ), in place per pair
+  for (int f = tid; f < ROWS * HP; f += kMixedThreads) {
+    const int p = f % HP, r = f / HP;
+    const int k = p, kk = M - p;
+    const C2 xk = b0[k * ROWS + r], xkk = b0[kk * ROWS + r];
+    {
+      const C2 cj{xkk.x, -xkk.y};
+      const C2 e = cadd(xk, cj), d = csub(xk, cj);
+      const double2 w = twr[k];
+      const C2 o = cmul(d, C2{(T)w.x, (T)-w.y});
+      b0[k * ROWS + r] = C2{e.x - o.y, e.y + o.x};
+    }
+    if (p > 0 && kk != k) {
+      const C2 cj{xk.x, -xk.y};
+      const C2 e = cadd(xkk, cj), d = csub(xkk, cj);
+      const double2 w = twr[kk];
+      const C2 o = cmul(d, C2{(T)w.x, (T)-w.y});
+      b0[kk * ROWS + r] = C2{e.x - o.y, e.y + o.x};
+    }
+  }
+  __syncthreads();
+  mixed_stages<T, M, 1, ROWS, -1>(b0, b1, tw, tid, kMixedThreads);
+  const C2 *res = (mixed_nstages(M) % 2 == 0) ? b0 : b1;
+  for (int f = tid; f < ROWS * H; f += kMixedThreads) {
+    const int m = f % H, r = f / H;
+    const long row = row0 + r;
+    if (row < nrows) {
+      C2 v{0, 0};
+      if (m < M) {
+        v = res[m * ROWS + r];
+        v.x *= scale;
+        v.y *= scale;
+      }
+      *reinterpret_cast<C2 *>(a + row * nz_ + 2 * m) = v;  // m == M writes the zero padding
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
 // host side: configuration per size and launch
 // ---------------------------------------------------------------------
 namespace {
@@ -217,6 +347,28 @@ void launch_lines_n(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols
   const LineTiles lt = make_line_tiles(ncols, nouter, ostride, C);
   hipLaunchKernelGGL((fft_lines_kernel<T, N, R, C, SIGN>), dim3(lt.tiles, 1, nf), dim3(C * N / R), 0, st, fp, tw, lt,
                      estride, col_cut, row_keep);
+}
+
+template <typename T, int N, int SIGN>
+void launch_lines_mixed(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
+                        int nouter, hipStream_t st) {
+  constexpr int C = kMixedC<T>();
+  const LineTiles lt = make_line_tiles(ncols, nouter, ostride, C);
+  hipLaunchKernelGGL((fft_lines_mixed_kernel<T, N, C, SIGN>), dim3(lt.tiles, 1, nf), dim3(kMixedThreads), 0, st, fp,
+                     tw, lt, estride);
+}
+
+template <typename T, int M>
+void launch_z_mixed(bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr, int nrows,
+                    int nz_, T scale, hipStream_t st) {
+  constexpr int ROWS = kMixedRows(M);
+  dim3 grid((nrows + ROWS - 1) / ROWS, 1, nf);
+  if (fwd)
+    hipLaunchKernelGGL((fft_z_r2c_mixed_kernel<T, M, ROWS>), grid, dim3(kMixedThreads), 0, st, fp, tw, twr, nrows,
+                       nz_);
+  else
+    hipLaunchKernelGGL((fft_z_c2r_mixed_kernel<T, M, ROWS>), grid, dim3(kMixedThreads), 0, st, fp, tw, twr, nrows,
+                       nz_, scale);
 }
 
 // one switch arm per supported line length
@@ -233,6 +385,15 @@ int launch_lines(int n, const FieldPtrs<T> &fp, int nf, const double2 *tw, int n
     case 256: launch_lines_n<T, 256, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
     case 512: launch_lines_n<T, 512, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
     case 1024: launch_lines_n<T, 1024, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st, col_cut, row_keep); return 0;
+  }
+  // mixed radix: whole transforms only (no band limits)
+  if (col_cut != INT_MAX || row_keep != n) return -1;
+  switch (n) {
+    case 48: launch_lines_mixed<T, 48, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 80: launch_lines_mixed<T, 80, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 96: launch_lines_mixed<T, 96, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 160: launch_lines_mixed<T, 160, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
+    case 192: launch_lines_mixed<T, 192, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
   }
   return -1;
 }
@@ -262,6 +423,14 @@ int launch_z(int m, bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw,
     case 128: launch_z_n<T, 128>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
     case 256: launch_z_n<T, 256>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
     case 512: launch_z_n<T, 512>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
+  }
+  if (kz_cut != INT_MAX) return -1;  // mixed radix: whole transforms only
+  switch (m) {
+    case 24: launch_z_mixed<T, 24>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 40: launch_z_mixed<T, 40>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 48: launch_z_mixed<T, 48>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 80: launch_z_mixed<T, 80>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
+    case 96: launch_z_mixed<T, 96>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
   }
   return -1;
 }
@@ -470,7 +639,8 @@ int fft_pass_z_c2r(spm_ctx *c, double *const *a, int n) { return fft_pass(c, a, 
 // every call rewrites: four tables, each starting on a 256-byte boundary,
 //   x: exp(+2 pi i t/NX), t < NX      y: exp(+2 pi i t/NY), t < NY
 //   z: exp(+2 pi i t/(NZ/2)), t < NZ/2     rz: exp(+2 pi i k/NZ), k <= NZ/2
-// computed on the device (sincospi of an exact dyadic argument), so the
+// computed on the device (sincospi of 2t/n: exact for powers of two, one
+// rounding of the argument for the mixed lengths), so the
 // call queues kernels only.
 // ---------------------------------------------------------------------
 namespace {
@@ -497,7 +667,7 @@ RootTables root_tables(int nx, int ny, int nz) {
   return t;
 }
 
-bool fft_size_ok(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+bool fft_size_ok(int n) { return grid_size_ok(n); }
 }  // namespace
 
 // one thread per table entry over the four tables
@@ -522,7 +692,7 @@ size_t fft3d_workspace_bytes(int nx, int ny, int nz) {
 // -1: A_k2a.
 int fft3d_raw(int nx, int ny, int nz, int direction, double *a, void *ws, hipStream_t st) {
   if (!fft_size_ok(nx) || !fft_size_ok(ny) || !fft_size_ok(nz)) {
-    global_error() = "transform sizes must be powers of two in [8, 1024]";
+    global_error() = "transform sizes must be powers of two in [8, 1024] or 48, 80, 96, 160, 192";
     return SPM_ERR_INVALID;
   }
   if (!a || !ws || (direction != 1 && direction != -1)) {
@@ -575,7 +745,7 @@ int spm_fft3d(int nx, int ny, int nz, int direction, double *field, void *worksp
   spm::global_error().clear();
   const size_t need = spm::fft3d_workspace_bytes(nx, ny, nz);
   if (need == 0) {
-    spm::global_error() = "transform sizes must be powers of two in [8, 1024]";
+    spm::global_error() = "transform sizes must be powers of two in [8, 1024] or 48, 80, 96, 160, 192";
     return SPM_ERR_INVALID;
   }
   hipStream_t st = static_cast<hipStream_t>(stream);

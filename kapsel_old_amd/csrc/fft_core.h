@@ -134,11 +134,100 @@ __device__ __forceinline__ void dft8(C2 *a) {
   a[7] = csub(e3, o3);
 }
 
+// radix 3: y_m = sum_r a_r w^(rm), w = exp(SIGN 2 pi i / 3)
+template <typename C2, int SIGN>
+__device__ __forceinline__ void dft3(C2 *a) {
+  using R = decltype(a[0].x);
+  const R h = (R)(SIGN * 0.86602540378443864676372317075294);  // SIGN sin(2 pi / 3)
+  const C2 t1 = cadd(a[1], a[2]);
+  const C2 t2{a[0].x - (R)0.5 * t1.x, a[0].y - (R)0.5 * t1.y};
+  const C2 d = csub(a[1], a[2]);
+  const C2 id{-h * d.y, h * d.x};  // i SIGN sin(2 pi/3) (a1 - a2)
+  a[0] = cadd(a[0], t1);
+  a[1] = cadd(t2, id);
+  a[2] = csub(t2, id);
+}
+
+// radix 5: y_m = sum_r a_r w^(rm), w = exp(SIGN 2 pi i / 5), from the
+// symmetric sums a_1 + a_4, a_2 + a_3 and differences a_1 - a_4, a_2 - a_3
+template <typename C2, int SIGN>
+__device__ __forceinline__ void dft5(C2 *a) {
+  using R = decltype(a[0].x);
+  const R c1 = (R)0.30901699437494742410229341718282, c2 = (R)-0.80901699437494742410229341718282;
+  const R s1 = (R)(SIGN * 0.95105651629515357211643933337938), s2 = (R)(SIGN * 0.58778525229247312916870595463907);
+  const C2 b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
+  const C2 d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+  const C2 e1{a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y};
+  const C2 e2{a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y};
+  const C2 f1{s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y};
+  const C2 f2{s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y};
+  a[0] = cadd(a[0], cadd(b1, b2));
+  a[1] = C2{e1.x - f1.y, e1.y + f1.x};  // e1 + i f1
+  a[4] = C2{e1.x + f1.y, e1.y - f1.x};
+  a[2] = C2{e2.x - f2.y, e2.y + f2.x};
+  a[3] = C2{e2.x + f2.y, e2.y - f2.x};
+}
+
 template <typename C2, int RS, int SIGN>
 __device__ __forceinline__ void dft(C2 *a) {
   if constexpr (RS == 2) dft2<C2, SIGN>(a);
+  else if constexpr (RS == 3) dft3<C2, SIGN>(a);
   else if constexpr (RS == 4) dft4<C2, SIGN>(a);
+  else if constexpr (RS == 5) dft5<C2, SIGN>(a);
   else dft8<C2, SIGN>(a);
+}
+
+// ---------------------------------------------------------------------
+// Mixed-radix lengths (factors 2, 3, 5): out-of-place Stockham stages on two
+// LDS buffers of N points x C columns ([point][column]), radices 4, 2, 3,
+// 5 in that order of preference; each thread loops over butterflies.
+// ---------------------------------------------------------------------
+constexpr int mixed_radix(int rem) {
+  return rem % 4 == 0 ? 4 : (rem % 2 == 0 ? 2 : (rem % 3 == 0 ? 3 : (rem % 5 == 0 ? 5 : 0)));
+}
+constexpr bool radix235(int n) {
+  if (n < 1) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+constexpr int mixed_nstages(int n) {
+  int s = 0;
+  for (int rem = n; rem > 1; rem /= mixed_radix(rem)) ++s;
+  return s;
+}
+
+// stages from span NS on; src holds the data, the result ends in src when
+// the number of remaining stages is even, else in dst
+template <typename T, int N, int NS, int C, int SIGN>
+__device__ __forceinline__ void mixed_stages(cplx<T> *src, cplx<T> *dst, const double2 *__restrict__ tw, int tid,
+                                             int nt) {
+  using C2 = cplx<T>;
+  if constexpr (NS < N) {
+    constexpr int RS = mixed_radix(N / NS);
+    static_assert(RS > 0, "lengths with factors 2, 3 and 5 only");
+    constexpr int NB = N / RS;
+    for (int w = tid; w < NB * C; w += nt) {
+      const int c = w % C, j = w / C, k = j % NS;
+      C2 a[RS];
+#pragma unroll
+      for (int r = 0; r < RS; ++r) a[r] = src[(j + r * NB) * C + c];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < RS; ++r) {
+          const double2 t = tw[(r * k * (N / (NS * RS))) % N];
+          a[r] = cmul(a[r], C2{(T)t.x, (T)(SIGN * t.y)});
+        }
+      }
+      dft<C2, RS, SIGN>(a);
+      const int base = (j / NS) * NS * RS + k;
+#pragma unroll
+      for (int r = 0; r < RS; ++r) dst[(base + r * NS) * C + c] = a[r];
+    }
+    __syncthreads();
+    mixed_stages<T, N, NS * mixed_radix(N / NS), C, SIGN>(dst, src, tw, tid, nt);
+  }
 }
 
 // One radix-RS Stockham stage of span NS on C columns of N points held in

@@ -30,7 +30,7 @@ extern "C" int spm_fft3d_noalloc(int nx, int ny, int nz, int direction, double *
   spm::global_error().clear();
   const size_t need = spm::fft3d_workspace_bytes(nx, ny, nz);
   if (need == 0) {
-    spm::global_error() = "transform sizes must be powers of two in [8, 1024]";
+    spm::global_error() = "transform sizes must be powers of two in [8, 1024] or 48, 80, 96, 160, 192";
     return SPM_ERR_INVALID;
   }
   if (need > kScratchBytes) {

@@ -115,6 +115,9 @@ void gauss2(spm_ctx *c, double *out) {
 }
 
 bool is_pow2(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+// the fused passes and the sheared solver take power-of-two grids; others
+// run the literal schedule
+bool pow2_grid(const spm_ctx *c) { return grid_pow2(c->grid.nx) && grid_pow2(c->grid.ny) && grid_pow2(c->grid.nz); }
 
 double *field_ptr(spm_ctx *c, int f) {
   switch (f) {
@@ -387,6 +390,7 @@ int setup(spm_ctx *c) {
   g.wave_y = PI2 / g.ly;
   g.wave_z = PI2 / g.lz;
   c->nfield = (size_t)g.nx * g.ny * g.nz_;
+  if (!pow2_grid(c)) c->schedule = SPM_SCHEDULE_LITERAL;
 
   PModel &m = c->pm;
   m.np = p.n_particles;
@@ -783,6 +787,10 @@ int set_lees_edwards(spm_ctx *c, double shear_rate) {
     c->err = "the Lees-Edwards path runs in FP64 only";
     return SPM_ERR_UNSUPPORTED;
   }
+  if (!pow2_grid(c)) {
+    c->err = "the Lees-Edwards path needs power-of-two grids";
+    return SPM_ERR_UNSUPPORTED;
+  }
   RC(dalloc(c, &c->le_red, (size_t)le_red_slots()));  // zeroes Shear_rate_eff
   // tile lists of the phi gather, images across the y faces included
   RC(dalloc(c, &c->tile_count_obl, (size_t)c->ntiles));
@@ -884,8 +892,8 @@ int spm_create(const spm_params *prm, spm_ctx **out) {
   *out = nullptr;
   spm::global_error().clear();
   const spm_params &p = *prm;
-  if (!is_pow2(p.nx, 8, 1024) || !is_pow2(p.ny, 8, 1024) || !is_pow2(p.nz, 8, 1024)) {
-    spm::global_error() = "grid sizes must be powers of two in [8, 1024]";
+  if (!grid_size_ok(p.nx) || !grid_size_ok(p.ny) || !grid_size_ok(p.nz)) {
+    spm::global_error() = "grid sizes must be powers of two in [8, 1024] or 48, 80, 96, 160, 192";
     return SPM_ERR_UNSUPPORTED;
   }
   if (p.n_particles < 0 || p.n_species < 1 || p.n_species > SPM_MAX_SPECIES || !(p.dt > 0) || !(p.dx > 0) ||
@@ -1167,6 +1175,10 @@ int spm_fused_stage(spm_ctx *c, int stage) { ENTRY(fused_stage(c, stage)); }
 
 int spm_set_schedule(spm_ctx *c, int schedule) {
   if (!c || (schedule != SPM_SCHEDULE_FUSED && schedule != SPM_SCHEDULE_LITERAL)) return SPM_ERR_INVALID;
+  if (schedule == SPM_SCHEDULE_FUSED && !pow2_grid(c)) {
+    c->err = "the fused schedule needs power-of-two grids";
+    return SPM_ERR_UNSUPPORTED;
+  }
   c->schedule = schedule;
   return SPM_OK;
 }

@@ -249,6 +249,10 @@ struct spm_ctx {
 // launchers implemented in the .hip files
 // ---------------------------------------------------------------------
 namespace spm {
+// Grid points per axis the transforms take: powers of two in [8, 1024],
+// or one of the mixed-radix lengths 48, 80, 96, 160, 192 (factors 2, 3, 5).
+inline bool grid_pow2(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+inline bool grid_size_ok(int n) { return grid_pow2(n) || n == 48 || n == 80 || n == 96 || n == 160 || n == 192; }
 // last error of a call without a context (this thread)
 std::string &global_error();
 // timing brackets around a launch; no-ops unless timing is enabled

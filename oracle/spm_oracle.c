@@ -129,6 +129,30 @@ static void plan_free(fft_plan *pl) {
  * sign = +1 computes sum x_j exp(+2 pi i jk/n), -1 the conjugate kernel. */
 static void fft_batch(const fft_plan *pl, double *a, long stride, int batch, int sign) {
   const int n = pl->n;
+  if (n & (n - 1)) {
+    /* lengths with factors 3 and 5 (the GPU path's 48, 80, 96, 160, 192,
+     * which the reference cannot run): the DFT by its definition */
+    double t[2 * 192];
+    for (int b = 0; b < batch; b++) {
+      for (int k = 0; k < n; k++) {
+        double sr = 0.0, si = 0.0;
+        for (int j = 0; j < n; j++) {
+          const int m = (int)(((long)j * k) % n);
+          const double xr = a[2 * ((long)j * stride + b)], xi = a[2 * ((long)j * stride + b) + 1];
+          const double wr = pl->c[m], wi = sign * pl->s[m];
+          sr += xr * wr - xi * wi;
+          si += xr * wi + xi * wr;
+        }
+        t[2 * k] = sr;
+        t[2 * k + 1] = si;
+      }
+      for (int k = 0; k < n; k++) {
+        a[2 * ((long)k * stride + b)] = t[2 * k];
+        a[2 * ((long)k * stride + b) + 1] = t[2 * k + 1];
+      }
+    }
+    return;
+  }
   for (int j = 0; j < n; j++) {
     int r = pl->rev[j];
     if (r > j) {
@@ -2152,10 +2176,14 @@ int oracle_u_k2zeta_k_obl(oracle_ctx *c) {
 /* ------------------------------------------------------------------ */
 /* lifetime and transfers                                              */
 /* ------------------------------------------------------------------ */
-static int is_pow2(int n) { return n >= 8 && (n & (n - 1)) == 0; }
+/* powers of two from 8, and the mixed lengths 48, 80, 96, 160, 192 that the
+ * GPU path adds (the reference's Ooura transform takes powers of two only) */
+static int size_ok(int n) {
+  return (n >= 8 && n <= 1024 && (n & (n - 1)) == 0) || n == 48 || n == 80 || n == 96 || n == 160 || n == 192;
+}
 
 oracle_ctx *oracle_create(const spm_params *prm) {
-  if (!prm || !is_pow2(prm->nx) || !is_pow2(prm->ny) || !is_pow2(prm->nz) || prm->n_particles < 0 ||
+  if (!prm || !size_ok(prm->nx) || !size_ok(prm->ny) || !size_ok(prm->nz) || prm->n_particles < 0 ||
       prm->n_species < 1 || prm->n_species > SPM_MAX_SPECIES || !(prm->dt > 0))
     return NULL;
   oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));

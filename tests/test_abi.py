@@ -113,7 +113,9 @@ def test_fft_workspace_size_and_size_checks_without_device():
     ws = L.spm_fft_workspace_size(256, 256, 256)
     # four 256-byte aligned root tables of 16-byte entries: 256, 256, 128, 129
     assert ws == 4096 + 4096 + 2048 + 2304
-    assert L.spm_fft_workspace_size(48, 64, 64) == 0
-    assert L.spm_fft_workspace_size(2048, 64, 64) == 0
-    assert L.spm_fft3d(48, 64, 64, 1, None, None, 0, None) == abi.SPM_ERR_INVALID
+    # mixed lengths 48, 80, 96, 160, 192: tables of 48, 80, 48, 49 entries
+    assert L.spm_fft_workspace_size(48, 80, 96) == 768 + 1280 + 768 + 1024
+    for bad in (12, 100, 2048, 4):
+        assert L.spm_fft_workspace_size(bad, 64, 64) == 0
+    assert L.spm_fft3d(100, 64, 64, 1, None, None, 0, None) == abi.SPM_ERR_INVALID
     assert b"powers of two" in L.spm_last_error(None)

@@ -309,7 +309,7 @@ def test_nonfinite_detected(SPM):
 def test_invalid_inputs_rejected(SPM):
     from kapsel_old_amd.spm import SpmError
     with pytest.raises(SpmError):
-        SPM(abi.make_params(48))  # not a power of two
+        SPM(abi.make_params(100))  # neither a power of two nor a mixed length
     g = SPM(abi.make_params(32, n_particles=1))
     bad = particles(1, (32, 32, 32), 4.0, positions=np.array([[33.0, 1.0, 1.0]]))
     with pytest.raises(SpmError):

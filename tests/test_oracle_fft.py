@@ -110,3 +110,51 @@ def test_inverse_of_a_non_hermitian_spectrum_matches_reference_ooura(grid):
     assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
     avg = np.fft.irfftn(X[:, :, 0::2] - 1j * X[:, :, 1::2], s=(nx, ny, nz), axes=(0, 1, 2))
     assert np.abs(avg - want).max() > 1e-3 * np.abs(want).max()
+
+
+# Grids with factors 3 and 5 (the library's mixed-radix lengths; the
+# reference's Ooura transform takes powers of two only, so numpy is the
+# reference here).
+MIXED = [(48, 48, 48), (80, 16, 96), (16, 96, 80), (8, 8, 192), (160, 8, 16), (8, 192, 8)]
+
+
+@pytest.mark.parametrize("grid", MIXED)
+def test_mixed_forward_matches_numpy(grid):
+    o = Oracle(abi.make_params(*grid))
+    a = real_field(o.shape, grid[2], seed=5)
+    k = o.A2a_k(a)
+    ref = numpy_spectrum(a, grid[2])
+    assert np.abs(k - ref).max() / np.abs(ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("grid", MIXED)
+def test_mixed_inverse_matches_numpy_and_round_trips(grid):
+    nx, ny, nz = grid
+    o = Oracle(abi.make_params(*grid))
+    a = real_field(o.shape, nz, seed=6)
+    spec = numpy_spectrum(a, nz)            # Hermitian: the spectrum of a real field
+    b = o.A_k2a(spec.copy())
+    assert np.abs(b[:, :, :nz] - a[:, :, :nz]).max() < 1e-13 * np.abs(a).max()
+    assert np.all(b[:, :, nz:] == 0.0)
+
+
+def test_mixed_brute_force_dft_48():
+    nx, ny, nz = 48, 8, 80
+    o = Oracle(abi.make_params(nx, ny, nz))
+    a = real_field(o.shape, nz, seed=7)
+    k = o.A2a_k(a)
+    rng = np.random.default_rng(1)
+    x, y, z = np.arange(nx), np.arange(ny), np.arange(nz)
+    for _ in range(30):
+        k1, k2, k3 = rng.integers(0, nx), rng.integers(0, ny), rng.integers(0, nz // 2 + 1)
+        ph = np.exp(2j * np.pi * (k1 * x[:, None, None] / nx + k2 * y[None, :, None] / ny
+                                  + k3 * z[None, None, :] / nz))
+        X = (a[:, :, :nz] * ph).sum()
+        assert abs(k[k1, k2, 2 * k3] - X.real) < 1e-11
+        assert abs(k[k1, k2, 2 * k3 + 1] - X.imag) < 1e-11
+
+
+@pytest.mark.parametrize("n", [12, 100, 7, 256 * 3, 2048])
+def test_oracle_rejects_unsupported_lengths(n):
+    with pytest.raises(Exception):
+        Oracle(abi.make_params(n, 8, 8))
