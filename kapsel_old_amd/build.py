@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspm.so")
 SOURCES = ["fft.hip", "fused.hip", "spectral.hip", "particles.hip", "electrolyte.hip", "shear.hip", "noalloc.hip", "closure.hip",
-           "spm_api.cpp", "checkpoint.cpp", "udf.cpp"]
+           "spm_api.cpp", "ensemble.cpp", "checkpoint.cpp", "udf.cpp"]
 # plain host C++ (no device code), built with the host compiler
 HOST_SOURCES = {"udf.cpp"}
 CXX = os.environ.get("CXX", "g++")

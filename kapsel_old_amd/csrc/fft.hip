@@ -21,6 +21,11 @@
 // keeps 8 points in registers per stage, stages exchange through LDS laid
 // out [point][column] so a wavefront touches consecutive LDS words.
 #include <climits>
+#include <cstdio>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "fft_core.h"
 
@@ -208,32 +213,38 @@ __global__ void ooura_planes_kernel(FieldPtrs<T> fp, int nx, int ny, int nz_, in
 // Lengths with factors 3 and 5 (grids of 48, 80, 96, 160 or 192 points per
 // axis; the reference allows powers of two only, input.cxx:1397-1402): the
 // same passes with the mixed-radix stages of fft_core.h on two LDS buffers.
-// A block of 256 threads takes C columns (y, x) or ROWS rows (z).
+// A block of TH threads takes C columns (y, x) or ROWS rows (z); which
+// (C, TH) or ROWS a launch uses is chosen per length and pass by timing
+// (tune_mixed_transforms), the defaults below otherwise.
 // ---------------------------------------------------------------------
 constexpr int kMixedThreads = 256;
+constexpr size_t kMixedMaxLds = 65536;
 template <typename T>
 constexpr int kMixedC() { return sizeof(T) == 8 ? 8 : 16; }  // one 128-byte run per row
 constexpr int kMixedRows(int m) { return m >= 1024 ? 1 : (1024 / m > 32 ? 32 : 1024 / m); }
+template <typename T>
+constexpr size_t lines_mixed_lds(int n, int c) { return 2 * (size_t)n * c * sizeof(cplx<T>) + 16 * (size_t)n; }
+template <typename T>
+constexpr size_t z_mixed_lds(int m, int rows) { return (2 * (size_t)m + 1) * rows * sizeof(cplx<T>) + 16 * (2 * (size_t)m + 1); }
 
-template <typename T, int N, int C, int SIGN>
-__global__ __launch_bounds__(kMixedThreads) void fft_lines_mixed_kernel(FieldPtrs<T> fp,
-                                                                       const double2 *__restrict__ tw_g, LineTiles lt,
-                                                                       long estride) {
+template <typename T, int N, int C, int TH, int SIGN>
+__global__ __launch_bounds__(TH) void fft_lines_mixed_kernel(FieldPtrs<T> fp, const double2 *__restrict__ tw_g,
+                                                             LineTiles lt, long estride) {
   using C2 = cplx<T>;
   __shared__ C2 b0[N * C], b1[N * C];
   __shared__ double2 tw[N];
   const int tid = threadIdx.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  for (int t = tid; t < N; t += kMixedThreads) tw[t] = tw_g[t];
+  for (int t = tid; t < N; t += TH) tw[t] = tw_g[t];
   C2 *g = reinterpret_cast<C2 *>(fp.p[blockIdx.z]);
-  for (int e = tid; e < N * C; e += kMixedThreads) {
+  for (int e = tid; e < N * C; e += TH) {
     const long col = line_column(lt, tile, e % C, C);
     b0[e] = col >= 0 ? g[col + (e / C) * estride] : C2{0, 0};
   }
   __syncthreads();
-  mixed_stages<T, N, 1, C, SIGN>(b0, b1, tw, tid, kMixedThreads);
+  mixed_stages<T, N, 1, C, SIGN>(b0, b1, tw, tid, TH);
   const C2 *res = (mixed_nstages(N) % 2 == 0) ? b0 : b1;
-  for (int e = tid; e < N * C; e += kMixedThreads) {
+  for (int e = tid; e < N * C; e += TH) {
     const long col = line_column(lt, tile, e % C, C);
     if (col >= 0) g[col + (e / C) * estride] = res[e];
   }
@@ -349,26 +360,110 @@ void launch_lines_n(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols
                      estride, col_cut, row_keep);
 }
 
+// Launch-shape variants of the mixed-radix passes.  Lines: (C, TH) =
+// (C0/2, 128), (C0, 256), (2 C0, 256), (C0, 128) with C0 = kMixedC<T>();
+// z: ROWS = R0/2, R0, 2 R0 with R0 = kMixedRows(M).  A variant whose LDS
+// would pass 64 KiB does not exist (its launch returns -1).
+constexpr int kLinesVariants = 4, kZVariants = 3, kLinesDefault = 1, kZDefault = 1;
+
+template <typename T, int N, int C, int TH, int SIGN>
+int lines_mixed_go(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
+                   int nouter, hipStream_t st) {
+  if constexpr (lines_mixed_lds<T>(N, C) > kMixedMaxLds || C < 1) {
+    return -1;
+  } else {
+    const LineTiles lt = make_line_tiles(ncols, nouter, ostride, C);
+    hipLaunchKernelGGL((fft_lines_mixed_kernel<T, N, C, TH, SIGN>), dim3(lt.tiles, 1, nf), dim3(TH), 0, st, fp, tw,
+                       lt, estride);
+    return 0;
+  }
+}
+
 template <typename T, int N, int SIGN>
-void launch_lines_mixed(const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride, long ostride,
-                        int nouter, hipStream_t st) {
-  constexpr int C = kMixedC<T>();
-  const LineTiles lt = make_line_tiles(ncols, nouter, ostride, C);
-  hipLaunchKernelGGL((fft_lines_mixed_kernel<T, N, C, SIGN>), dim3(lt.tiles, 1, nf), dim3(kMixedThreads), 0, st, fp,
-                     tw, lt, estride);
+int launch_lines_mixed(int v, const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride,
+                       long ostride, int nouter, hipStream_t st) {
+  constexpr int C0 = kMixedC<T>();
+  switch (v) {
+    case 0: return lines_mixed_go<T, N, C0 / 2, 128, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st);
+    case 1: return lines_mixed_go<T, N, C0, 256, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st);
+    case 2: return lines_mixed_go<T, N, 2 * C0, 256, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st);
+    case 3: return lines_mixed_go<T, N, C0, 128, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st);
+  }
+  return -1;
+}
+
+template <typename T, int M, int ROWS>
+int z_mixed_go(bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr, int nrows, int nz_,
+               T scale, hipStream_t st) {
+  if constexpr (z_mixed_lds<T>(M, ROWS) > kMixedMaxLds || ROWS < 1) {
+    return -1;
+  } else {
+    dim3 grid((nrows + ROWS - 1) / ROWS, 1, nf);
+    if (fwd)
+      hipLaunchKernelGGL((fft_z_r2c_mixed_kernel<T, M, ROWS>), grid, dim3(kMixedThreads), 0, st, fp, tw, twr, nrows,
+                         nz_);
+    else
+      hipLaunchKernelGGL((fft_z_c2r_mixed_kernel<T, M, ROWS>), grid, dim3(kMixedThreads), 0, st, fp, tw, twr, nrows,
+                         nz_, scale);
+    return 0;
+  }
 }
 
 template <typename T, int M>
-void launch_z_mixed(bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr, int nrows,
-                    int nz_, T scale, hipStream_t st) {
-  constexpr int ROWS = kMixedRows(M);
-  dim3 grid((nrows + ROWS - 1) / ROWS, 1, nf);
-  if (fwd)
-    hipLaunchKernelGGL((fft_z_r2c_mixed_kernel<T, M, ROWS>), grid, dim3(kMixedThreads), 0, st, fp, tw, twr, nrows,
-                       nz_);
-  else
-    hipLaunchKernelGGL((fft_z_c2r_mixed_kernel<T, M, ROWS>), grid, dim3(kMixedThreads), 0, st, fp, tw, twr, nrows,
-                       nz_, scale);
+int launch_z_mixed(int v, bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr, int nrows,
+                   int nz_, T scale, hipStream_t st) {
+  constexpr int R0 = kMixedRows(M);
+  switch (v) {
+    case 0: return z_mixed_go<T, M, R0 / 2>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st);
+    case 1: return z_mixed_go<T, M, R0>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st);
+    case 2: return z_mixed_go<T, M, 2 * R0>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st);
+  }
+  return -1;
+}
+
+// runtime length -> template
+template <typename T, int SIGN>
+int lines_mixed_rt(int n, int v, const FieldPtrs<T> &fp, int nf, const double2 *tw, int ncols, long estride,
+                   long ostride, int nouter, hipStream_t st) {
+  switch (n) {
+    case 48: return launch_lines_mixed<T, 48, SIGN>(v, fp, nf, tw, ncols, estride, ostride, nouter, st);
+    case 80: return launch_lines_mixed<T, 80, SIGN>(v, fp, nf, tw, ncols, estride, ostride, nouter, st);
+    case 96: return launch_lines_mixed<T, 96, SIGN>(v, fp, nf, tw, ncols, estride, ostride, nouter, st);
+    case 160: return launch_lines_mixed<T, 160, SIGN>(v, fp, nf, tw, ncols, estride, ostride, nouter, st);
+    case 192: return launch_lines_mixed<T, 192, SIGN>(v, fp, nf, tw, ncols, estride, ostride, nouter, st);
+  }
+  return -1;
+}
+
+template <typename T>
+int z_mixed_rt(int m, int v, bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw, const double2 *twr,
+               int nrows, int nz_, T scale, hipStream_t st) {
+  switch (m) {
+    case 24: return launch_z_mixed<T, 24>(v, fwd, fp, nf, tw, twr, nrows, nz_, scale, st);
+    case 40: return launch_z_mixed<T, 40>(v, fwd, fp, nf, tw, twr, nrows, nz_, scale, st);
+    case 48: return launch_z_mixed<T, 48>(v, fwd, fp, nf, tw, twr, nrows, nz_, scale, st);
+    case 80: return launch_z_mixed<T, 80>(v, fwd, fp, nf, tw, twr, nrows, nz_, scale, st);
+    case 96: return launch_z_mixed<T, 96>(v, fwd, fp, nf, tw, twr, nrows, nz_, scale, st);
+  }
+  return -1;
+}
+
+// The autotuner's choices, per pass kind, element size, length and the
+// shape of the pass (columns and outer count for lines, rows for z).
+enum MixedKind { MIX_LINES = 0, MIX_ZFWD = 1, MIX_ZINV = 2 };
+struct MixedKey {
+  int kind, esize, n, a, b;
+  bool operator<(const MixedKey &o) const {
+    return std::tie(kind, esize, n, a, b) < std::tie(o.kind, o.esize, o.n, o.a, o.b);
+  }
+};
+std::mutex g_tune_mu;
+std::map<MixedKey, int> g_tune;
+
+int tuned_variant(const MixedKey &k, int dflt) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(k);
+  return it == g_tune.end() ? dflt : it->second;
 }
 
 // one switch arm per supported line length
@@ -388,14 +483,8 @@ int launch_lines(int n, const FieldPtrs<T> &fp, int nf, const double2 *tw, int n
   }
   // mixed radix: whole transforms only (no band limits)
   if (col_cut != INT_MAX || row_keep != n) return -1;
-  switch (n) {
-    case 48: launch_lines_mixed<T, 48, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 80: launch_lines_mixed<T, 80, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 96: launch_lines_mixed<T, 96, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 160: launch_lines_mixed<T, 160, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-    case 192: launch_lines_mixed<T, 192, SIGN>(fp, nf, tw, ncols, estride, ostride, nouter, st); return 0;
-  }
-  return -1;
+  const int v = tuned_variant(MixedKey{MIX_LINES, (int)sizeof(T), n, ncols, nouter}, kLinesDefault);
+  return lines_mixed_rt<T, SIGN>(n, v, fp, nf, tw, ncols, estride, ostride, nouter, st);
 }
 
 template <typename T, int M>
@@ -425,14 +514,8 @@ int launch_z(int m, bool fwd, const FieldPtrs<T> &fp, int nf, const double2 *tw,
     case 512: launch_z_n<T, 512>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st, kz_cut); return 0;
   }
   if (kz_cut != INT_MAX) return -1;  // mixed radix: whole transforms only
-  switch (m) {
-    case 24: launch_z_mixed<T, 24>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 40: launch_z_mixed<T, 40>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 48: launch_z_mixed<T, 48>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 80: launch_z_mixed<T, 80>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-    case 96: launch_z_mixed<T, 96>(fwd, fp, nf, tw, twr, nrows, nz_, scale, st); return 0;
-  }
-  return -1;
+  const int v = tuned_variant(MixedKey{fwd ? MIX_ZFWD : MIX_ZINV, (int)sizeof(T), m, nrows, 0}, kZDefault);
+  return z_mixed_rt<T>(m, v, fwd, fp, nf, tw, twr, nrows, nz_, scale, st);
 }
 
 int upload_roots(spm_ctx *c, double2 **dst, int n, int count) {
@@ -514,6 +597,110 @@ int fft_r2c_many(spm_ctx *c, double *const *a, int n) { return fft_many(c, a, n,
 int fft_c2r_many(spm_ctx *c, double *const *a, int n) { return fft_many(c, a, n, false); }
 int fft_r2c(spm_ctx *c, double *a) { return fft_many(c, &a, 1, true); }
 int fft_c2r(spm_ctx *c, double *a) { return fft_many(c, &a, 1, false); }
+
+// ---------------------------------------------------------------------
+// Autotuner of the mixed-radix launch shapes: every variant of every
+// mixed-length pass of the context's grid is timed on the context's two
+// scratch fields (work[0], work[1], zeroed first; they carry nothing
+// between steps) and the fastest is kept for that pass shape, for every
+// context of the process.  Shapes already in the table are not re-timed.
+// ---------------------------------------------------------------------
+namespace {
+struct TuneLog {
+  MixedKey key;
+  std::vector<float> ms;  // per variant, -1 where it does not exist
+  int pick;
+};
+std::vector<TuneLog> g_tune_log;
+
+template <typename T>
+int tune_mixed_t(spm_ctx *c) {
+  const Grid &g = c->grid;
+  FieldPtrs<T> fp{};
+  fp.p[0] = reinterpret_cast<T *>(c->work[0]);
+  fp.p[1] = reinterpret_cast<T *>(c->work[1]);
+  const int nf = 2, nrows = g.nx * g.ny;
+  const long h = g.hnz;
+  const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+  struct Pass {
+    MixedKey key;
+    int nvar;
+    std::function<int(int)> go;
+  };
+  std::vector<Pass> passes;
+  const int es = (int)sizeof(T);
+  if (!grid_pow2(g.nz)) {
+    const int m = g.nz / 2;
+    passes.push_back({MixedKey{MIX_ZFWD, es, m, nrows, 0}, kZVariants, [&, m](int v) {
+                        return z_mixed_rt<T>(m, v, true, fp, nf, c->tw_z, c->tw_rz, nrows, g.nz_, (T)1, c->stream);
+                      }});
+    passes.push_back({MixedKey{MIX_ZINV, es, m, nrows, 0}, kZVariants, [&, m](int v) {
+                        return z_mixed_rt<T>(m, v, false, fp, nf, c->tw_z, c->tw_rz, nrows, g.nz_, scale, c->stream);
+                      }});
+  }
+  if (!grid_pow2(g.ny))
+    passes.push_back({MixedKey{MIX_LINES, es, g.ny, g.hnz, g.nx}, kLinesVariants, [&](int v) {
+                        return lines_mixed_rt<T, +1>(g.ny, v, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx,
+                                                     c->stream);
+                      }});
+  if (!grid_pow2(g.nx))
+    passes.push_back({MixedKey{MIX_LINES, es, g.nx, g.ny * g.hnz, 1}, kLinesVariants, [&](int v) {
+                        return lines_mixed_rt<T, +1>(g.nx, v, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1,
+                                                     c->stream);
+                      }});
+  if (passes.empty()) return 0;
+  for (int q = 0; q < 2; ++q)
+    if (int r0 = k_zero(c, c->work[q], c->nfield)) return r0;
+  hipEvent_t e0, e1;
+  SPM_HIP(c, hipEventCreate(&e0));
+  SPM_HIP(c, hipEventCreate(&e1));
+  constexpr int kReps = 10;
+  int rc = 0;
+  for (Pass &p : passes) {
+    if (tuned_variant(p.key, -1) >= 0) continue;
+    TuneLog log{p.key, std::vector<float>(p.nvar, -1.f), -1};
+    for (int v = 0; v < p.nvar && rc == 0; ++v) {
+      if (p.go(v) != 0) continue;  // no such variant for this length
+      if (hipEventRecord(e0, c->stream) != hipSuccess) rc = SPM_ERR_HIP;
+      for (int r = 0; r < kReps; ++r) (void)p.go(v);
+      if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) rc = SPM_ERR_HIP;
+      float ms = 0.f;
+      if (rc == 0 && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = SPM_ERR_HIP;
+      log.ms[v] = ms / kReps;
+      if (log.pick < 0 || log.ms[v] < log.ms[log.pick]) log.pick = v;
+    }
+    if (hipGetLastError() != hipSuccess) rc = SPM_ERR_HIP;
+    if (rc) break;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[p.key] = log.pick;
+    g_tune_log.push_back(log);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) c->err = "autotuning the mixed-radix passes failed";
+  return rc;
+}
+}  // namespace
+
+int tune_mixed_transforms(spm_ctx *c) {
+  return c->prm.precision == SPM_FP32 ? tune_mixed_t<float>(c) : tune_mixed_t<double>(c);
+}
+
+std::string tune_report() {
+  static const char *kinds[] = {"lines", "z-r2c", "z-c2r"};
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  std::string out;
+  char line[256];
+  for (const TuneLog &l : g_tune_log) {
+    int n = std::snprintf(line, sizeof line, "%s fp%d n=%d shape=%dx%d pick=%d ms=", kinds[l.key.kind],
+                          8 * l.key.esize, l.key.n, l.key.a, l.key.b, l.pick);
+    for (size_t v = 0; v < l.ms.size() && n < (int)sizeof line - 16; ++v)
+      n += std::snprintf(line + n, sizeof line - n, "%s%.4f", v ? "," : "", l.ms[v]);
+    out += line;
+    out += "\n";
+  }
+  return out;
+}
 
 }  // namespace spm
 

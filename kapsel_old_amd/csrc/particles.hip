@@ -1357,6 +1357,28 @@ int p_f_add_obl(spm_ctx *c, double *const ucp[3]) {
   return 0;
 }
 
+// One block: read the slot, copy its 6 np normals, advance the counter.  A
+// slot past the pool (the host refills before that can happen) yields NaN,
+// which the finiteness check reports, instead of a read out of bounds.
+__global__ void noise_pool_kernel(const double *__restrict__ pool, int *counter, int nslots, int n6,
+                                  double *__restrict__ dst) {
+  __shared__ int slot;
+  if (threadIdx.x == 0) slot = *counter;
+  __syncthreads();
+  const bool ok = slot >= 0 && slot < nslots;
+  for (int i = threadIdx.x; i < n6; i += blockDim.x) dst[i] = ok ? pool[(long)slot * n6 + i] : __builtin_nan("");
+  __syncthreads();
+  if (threadIdx.x == 0) *counter = slot + 1;
+}
+
+int p_noise_from_pool(spm_ctx *c) {
+  if (c->np == 0) return 0;
+  hipLaunchKernelGGL(noise_pool_kernel, dim3(1), dim3(256), 0, c->stream, c->pool_dev, c->pool_counter, c->pool_steps,
+                     6 * c->np, c->noise_dev);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
 int p_thermostat_apply(spm_ctx *c, const double *noise_dev) {
   if (c->np == 0) return 0;
   tbeg(c);

@@ -422,6 +422,18 @@ int k_zero(spm_ctx *c, double *a, size_t n) {
   return 0;
 }
 
+__global__ void zero_int_kernel(int *a, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// Zeroes n ints with a kernel (no memset node when captured in a graph).
+int k_zero_int(spm_ctx *c, int *a, size_t n) {
+  hipLaunchKernelGGL(zero_int_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, a, (long)n);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
 // Sets *c->flag to 1 if any used element of `a` is NaN or Inf.
 int k_check_finite(spm_ctx *c, const double *a, bool spectrum) {
   const long n = (long)c->grid.nx * c->grid.ny * (spectrum ? c->grid.nz_ : c->grid.nz);

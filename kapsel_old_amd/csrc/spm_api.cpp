@@ -207,6 +207,10 @@ int reset_phi_u(spm_ctx *c) {
 // staging slots let the host draw step n+1 while step n is in flight.
 int thermostat(spm_ctx *c) {
   if (!(c->prm.kbt > 0.) || c->np == 0) return 0;
+  if (c->pool_mode) {  // drawn ahead into the pool, in the same order
+    RC(p_noise_from_pool(c));
+    return p_thermostat_apply(c, c->noise_dev);
+  }
   const int s = c->noise_slot;
   SPM_HIP(c, hipEventSynchronize(c->noise_done[s]));
   double *h = c->noise_host[s];
@@ -338,8 +342,10 @@ void free_ctx(spm_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (double *p : {c->zeta[0], c->zeta[1], c->u[0], c->u[1], c->u[2], c->up[0], c->up[1], c->up[2], c->phi, c->f[0],
                     c->f[1], c->f[2], c->work[0], c->work[1], c->work[2], c->work[3], c->work[4], c->uk_dc, c->pbuf,
-                    c->pres, c->noise_dev})
+                    c->pres, c->noise_dev, c->pool_dev})
     if (p) (void)hipFree(p);
+  if (c->pool_counter) (void)hipFree(c->pool_counter);
+  if (c->pool_host) (void)hipHostFree(c->pool_host);
   for (double *p : {c->conc[0], c->conc[1], c->conc_rhs[0], c->conc_rhs[1], c->surface_normal[0],
                     c->surface_normal[1], c->surface_normal[2], c->f_ns1[0], c->f_ns1[1], c->red, c->le_red})
     if (p) (void)hipFree(p);
@@ -870,6 +876,70 @@ int host_thermostat(spm_ctx *c) { return thermostat(c); }
 // =====================================================================
 // C ABI
 // =====================================================================
+// ---------------------------------------------------------------------
+// pieces of spm_step for ensembles (ensemble.cpp), which capture one step
+// of every box into a graph and replay it
+// ---------------------------------------------------------------------
+namespace spm {
+
+// phase 0: what spm_step does before its steps; 1: one step; 2: what it
+// does after them.  No host synchronisation on any phase.
+int box_step_phase(spm_ctx *c, int phase) {
+  const bool fused = c->schedule == SPM_SCHEDULE_FUSED;
+  if (phase == 0) {
+    if (fused) {
+      RC(k_truncate(c, c->zeta[0]));
+      RC(k_truncate(c, c->zeta[1]));
+    }
+    return 0;
+  }
+  if (phase == 1) return fused ? fused_step(c) : time_evolution_hydro(c);
+  return fused ? materialize(c) : 0;
+}
+
+bool box_uses_noise(const spm_ctx *c) {
+  return c->prm.kbt > 0. && c->np > 0 && c->motion == SPM_MOTION_FREE;
+}
+
+// device pool and pinned staging for `steps` steps of noise
+int pool_setup(spm_ctx *c, int steps) {
+  if (!box_uses_noise(c) || c->pool_steps >= steps) return 0;
+  if (c->pool_dev) (void)hipFree(c->pool_dev);
+  if (c->pool_host) (void)hipHostFree(c->pool_host);
+  c->pool_dev = nullptr;
+  c->pool_host = nullptr;
+  const size_t n = (size_t)steps * 6 * c->np;
+  RC(dalloc(c, &c->pool_dev, n));
+  if (!c->pool_counter) RC(dalloc(c, &c->pool_counter, 1));
+  SPM_HIP(c, hipHostMalloc((void **)&c->pool_host, sizeof(double) * n, hipHostMallocDefault));
+  c->pool_steps = steps;
+  return 0;
+}
+
+// Draws the next k steps of noise (the order thermostat() draws them in,
+// step by step) and queues their copy and the counter reset on c->stream.
+// The caller makes sure no queued work still reads the pool.
+int pool_fill(spm_ctx *c, int k) {
+  if (!box_uses_noise(c)) return 0;
+  if (k > c->pool_steps) {
+    c->err = "noise pool smaller than the steps asked for";
+    return SPM_ERR_INVALID;
+  }
+  double *h = c->pool_host;
+  for (int s = 0; s < k; ++s)
+    for (int n = 0; n < c->np; ++n) {
+      double *d = h + ((size_t)s * c->np + n) * 6;
+      gauss2(c, d);
+      gauss2(c, d + 2);
+      gauss2(c, d + 4);
+    }
+  SPM_HIP(c, hipMemcpyAsync(c->pool_dev, h, sizeof(double) * 6 * c->np * k, hipMemcpyHostToDevice, c->stream));
+  RC(k_zero_int(c, c->pool_counter, 1));
+  return 0;
+}
+
+}  // namespace spm
+
 extern "C" {
 
 const char *spm_strerror(int status) {
@@ -1129,6 +1199,7 @@ static int step_n(spm_ctx *c, int nsteps) {
   if (fused_ns && nsteps > 0) RC(materialize(c));
   return 0;
 }
+
 
 static int check_finite(spm_ctx *c) {
   SPM_HIP(c, hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));

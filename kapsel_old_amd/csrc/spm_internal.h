@@ -175,6 +175,15 @@ struct spm_ctx {
   hipEvent_t noise_done[2] = {nullptr, nullptr};
   int noise_slot = 0;
   double noise_v = 0.0, noise_o = 0.0;
+  // noise drawn ahead for pool_steps whole steps (ensembles replaying a
+  // captured step): with pool_mode the thermostat takes slot *pool_counter
+  // of pool_dev on the device and advances the counter, so a replayed graph
+  // needs no host work per step
+  bool pool_mode = false;
+  int pool_steps = 0;
+  double *pool_dev = nullptr;   // pool_steps * 6 * np
+  double *pool_host = nullptr;  // pinned staging of the same size
+  int *pool_counter = nullptr;  // device
 
   // host staging
   std::vector<double> pstage;
@@ -300,6 +309,7 @@ int k_solenoidal_trunc(spm_ctx *c, double *u0, double *u1, double *u2);
 int k_make_f(spm_ctx *c);
 int k_add_f(spm_ctx *c);
 int k_zero(spm_ctx *c, double *a, size_t n);
+int k_zero_int(spm_ctx *c, int *a, size_t n);
 int k_check_finite(spm_ctx *c, const double *a, bool spectrum);
 
 // particles.hip
@@ -308,6 +318,18 @@ int p_calc_f_hydro(spm_ctx *c);
 int p_force(spm_ctx *c);
 int p_velocity(spm_ctx *c, int euler);
 int p_thermostat_apply(spm_ctx *c, const double *noise_dev);
+// noise_dev = slot *pool_counter of pool_dev, then the counter advances
+int p_noise_from_pool(spm_ctx *c);
+// ensembles (ensemble.cpp): spm_step in phases (0 before the steps, 1 one
+// step, 2 after them), the thermostat's noise pool, and the timing
+// autotuner of the mixed-radix launch shapes for the context's grid
+int box_step_phase(spm_ctx *c, int phase);
+bool box_uses_noise(const spm_ctx *c);
+int pool_setup(spm_ctx *c, int steps);
+int pool_fill(spm_ctx *c, int k);
+int tune_mixed_transforms(spm_ctx *c);
+// the autotuner's table as text, one line per tuned pass
+std::string tune_report();
 // spherical particles under Lees-Edwards shear; sre is Shear_rate_eff on the device
 int p_position_obl(spm_ctx *c, int euler, const double *sre);
 int p_calc_f_hydro_obl(spm_ctx *c, double *const ucp[3], const double *sre);

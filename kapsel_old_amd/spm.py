@@ -38,6 +38,9 @@ SYMBOLS = [
     "spm_closure_put_params", "spm_closure_set_scaling", "spm_closure_get_scaling", "spm_closure_forward",
     "spm_closure_gradient", "spm_closure_set_data", "spm_closure_train", "spm_set_closure", "spm_closure_features",
     "spm_set_particle_motion",
+    "spm_ensemble_create", "spm_ensemble_destroy", "spm_ensemble_box", "spm_ensemble_size", "spm_ensemble_step",
+    "spm_ensemble_set_graph", "spm_ensemble_autotune", "spm_ensemble_tuning", "spm_autotune_report",
+    "spm_ensemble_last_error",
 ]
 
 SCHEDULE_FUSED, SCHEDULE_LITERAL = 0, 1
@@ -126,6 +129,18 @@ def lib():
         L.spm_set_closure.argtypes = [vp, vp]
         L.spm_closure_features.argtypes = [vp, fp]
         L.spm_set_particle_motion.argtypes = [vp, ip]
+        L.spm_ensemble_create.argtypes = [ip, C.POINTER(abi.SpmParams), C.POINTER(vp)]
+        L.spm_ensemble_destroy.argtypes = [vp]
+        L.spm_ensemble_box.argtypes = [vp, ip]
+        L.spm_ensemble_box.restype = vp
+        L.spm_ensemble_size.argtypes = [vp]
+        L.spm_ensemble_step.argtypes = [vp, ip]
+        L.spm_ensemble_set_graph.argtypes = [vp, ip]
+        L.spm_ensemble_autotune.argtypes = [vp, ip]
+        L.spm_ensemble_tuning.argtypes = [vp, C.POINTER(ip), dp, dp]
+        L.spm_autotune_report.argtypes = [C.c_char_p, ip]
+        L.spm_ensemble_last_error.argtypes = [vp]
+        L.spm_ensemble_last_error.restype = C.c_char_p
         L.spm_put_total_solute.argtypes = [vp, dp]
         L.spm_get_total_solute.argtypes = [vp, dp]
         for name in SYMBOLS:
@@ -154,7 +169,22 @@ class SPM:
         self.shape = (p.nx, p.ny, 2 * (p.nz // 2 + 1))
         self.n_particles = p.n_particles
 
+    @classmethod
+    def _wrap(cls, handle, params, owner):
+        """A view of a context another object owns (an ensemble's box)."""
+        self = cls.__new__(cls)
+        self.params = params
+        self._h = C.c_void_p(handle)
+        self._owner = owner
+        p = params
+        self.shape = (p.nx, p.ny, 2 * (p.nz // 2 + 1))
+        self.n_particles = p.n_particles
+        return self
+
     def close(self):
+        if getattr(self, "_owner", None) is not None:
+            self._h = None  # the owner destroys it
+            return
         if getattr(self, "_h", None):
             lib().spm_destroy(self._h)
             self._h = None
@@ -419,3 +449,62 @@ class Closure:
         loss = np.zeros(steps, dtype=np.float32)
         self._call("train", steps, batch, C.c_float(lr), _fp(loss))
         return loss
+
+
+class Ensemble:
+    """Independent SPM boxes on one GPU advanced together (an spm_ensemble,
+    include/spm_ensemble.h): one captured graph replay steps every box.
+    `boxes[b]` is an SPM view of box b for particles, fields and the rest."""
+
+    def __init__(self, params_list):
+        n = len(params_list)
+        arr = (abi.SpmParams * n)(*params_list)
+        h = C.c_void_p()
+        rc = lib().spm_ensemble_create(n, arr, C.byref(h))
+        if rc:
+            raise SpmError(rc, "spm_ensemble_create", lib().spm_last_error(None).decode())
+        self._h = h
+        self.params = list(params_list)
+        self.boxes = [SPM._wrap(lib().spm_ensemble_box(h, b), params_list[b], self) for b in range(n)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for bx in self.boxes:
+                bx._h = None
+            lib().spm_ensemble_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __len__(self):
+        return len(self.boxes)
+
+    def _call(self, name, *args):
+        rc = getattr(lib(), "spm_ensemble_" + name)(self._h, *args)
+        if rc:
+            raise SpmError(rc, "spm_ensemble_" + name, lib().spm_ensemble_last_error(self._h).decode())
+
+    def step(self, nsteps=1):
+        self._call("step", nsteps)
+
+    def set_graph(self, on):
+        self._call("set_graph", 1 if on else 0)
+
+    def autotune(self, steps_per_candidate=3):
+        self._call("autotune", steps_per_candidate)
+
+    def tuning(self):
+        """(lanes in use, ms per ensemble step at 1/2/4/8 lanes, per-box ms)."""
+        lanes = C.c_int()
+        ms = np.zeros(4)
+        box = np.zeros(len(self.boxes))
+        self._call("tuning", C.byref(lanes), _dp(ms), _dp(box))
+        return lanes.value, ms, box
+
+
+def autotune_report():
+    """The transform autotuner's choices, one line per tuned pass."""
+    n = lib().spm_autotune_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().spm_autotune_report(buf, n + 1)
+    return buf.value.decode()

@@ -18,7 +18,7 @@ HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.j
 
 def declared_functions():
     src = "\n".join(open(h).read() for h in HEADERS)
-    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int|void|size_t)\s*\*?\s*(spm_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int|void|size_t|spm_ctx)\s*\*?\s*(spm_\w+)\s*\(", src, re.M)))
 
 
 def test_header_layout_matches_ctypes(tmp_path):
