@@ -193,6 +193,56 @@ def closure_line():
     return out
 
 
+def ensemble_line(nbox=64, steps=200, seed=4):
+    """The batched ensemble: 64 boxes with grid edges drawn from
+    {32, 48, 64, 80, 96}, 1-50 spheres a = 3-6 each, WCA, kBT = 0.05, FP32
+    fields, `steps` steps of every box from one graph replay per step, after
+    the autotuner has chosen the lane count (its steps are untimed); the
+    host-launched rate of the same ensemble beside it."""
+    from kapsel_old_amd.ensemble import build, ensemble_boxes, grid_points
+    from kapsel_old_amd.spm import autotune_report
+    boxes = ensemble_boxes(nbox=nbox, seed=seed)
+    ens = build(boxes)
+    ens.autotune(3)
+    ens.step(2)
+    t0 = time.perf_counter()
+    ens.step(steps)
+    el = time.perf_counter() - t0
+    lanes, tune_ms, box_ms = ens.tuning()
+    ens.set_graph(False)
+    k = max(10, steps // 10)
+    t0 = time.perf_counter()
+    ens.step(k)
+    el_host = (time.perf_counter() - t0) / k
+    ens.close()
+    pts = grid_points(boxes)
+    return {"workload": "%d boxes, edges from {32,48,64,80,96}, %d spheres in all (1-50 per box, a=3-6, xi=2), "
+                        "WCA, kBT=0.05, FP32 fields, %d steps" % (nbox, sum(len(p) for _, p in boxes), steps),
+            "value": nbox * steps / el, "unit": "box-steps/s",
+            "ensemble_steps_per_s": steps / el, "ms_per_ensemble_step": 1e3 * el / steps,
+            "grid_point_steps_per_s": pts * steps / el, "grid_points": pts,
+            "lanes": lanes, "ms_per_step_at_1_2_4_8_lanes": [float(x) for x in tune_ms],
+            "sum_of_lone_box_ms": float(box_ms.sum()),
+            "host_launched_ms_per_ensemble_step": 1e3 * el_host,
+            "mixed_radix_tuning": _tuning_summary(autotune_report())}
+
+
+def _tuning_summary(report):
+    """Passes the transform autotuner timed, how many it moved off the
+    default launch shape, and the mean time saved on those."""
+    picks, gains = 0, []
+    lines = [ln for ln in report.splitlines() if ln.strip()]
+    for ln in lines:
+        pick = int(ln.split("pick=")[1].split()[0])
+        ms = [float(x) for x in ln.split("ms=")[1].split(",")]
+        dflt = 1
+        if pick != dflt and ms[dflt] > 0:
+            picks += 1
+            gains.append(1.0 - ms[pick] / ms[dflt])
+    return {"passes_timed": len(lines), "non_default_picks": picks,
+            "mean_time_saved_on_those": float(np.mean(gains)) if gains else 0.0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,8 +257,13 @@ def main():
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB")
     ap.add_argument("--shear-rate", type=float, default=0.0)
     ap.add_argument("--no-extras", action="store_true",
-                    help="only the headline line (no FP32, one-step-call or closure objects)")
+                    help="only the headline line (no FP32, one-step-call, closure or ensemble objects)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="print only the batched-ensemble object")
     args = ap.parse_args()
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_line()}), flush=True)
+        return
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
         return
@@ -286,6 +341,7 @@ def main():
         out["one_step_per_call"] = {"value": one_step_calls(n, np_, max(20, args.steps // 2)),
                                     "unit": "time-steps/s", "note": "spm_step(1) per step, FP64 fused"}
         out["closure"] = closure_line()
+        out["ensemble"] = ensemble_line()
     if not args.no_cpu_baseline and precision == abi.FP64:
         nthreads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
         base = cpu_baseline(n, np_, shear_rate=args.shear_rate, threads=(1, nthreads) if nthreads > 1 else (1,))
