@@ -234,6 +234,23 @@ def report(seeds=8, steps=600, train_steps=40000, log=print):
                           "resistance problem, 600 steps"}
 
 
+def mlp_workload(batch=65536, train_steps=40, minibatch=8192, forwards=10, seed=0):
+    """The network's kernels alone, for profiling: `train_steps` Adam steps
+    on minibatches of random samples, then `forwards` forwards of `batch`
+    samples."""
+    from .spm import Closure
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((batch, 114)).astype(np.float32)
+    t = (0.1 * rng.standard_normal((batch, 6))).astype(np.float32)
+    net = Closure(batch, seed=3)
+    net.set_scaling(np.zeros(114), np.ones(114), np.ones(6))
+    net.set_data(x, t)
+    loss = net.train(train_steps, minibatch, 1e-3)
+    for _ in range(forwards):
+        net.forward(x)
+    return float(loss[0]), float(loss[-1])
+
+
 if __name__ == "__main__":
     import argparse
     import json
@@ -241,7 +258,12 @@ if __name__ == "__main__":
     ap.add_argument("--seeds", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=40000)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--mlp-workload", type=int, default=0, metavar="BATCH",
+                    help="only run the network's kernels (40 training steps, 10 forwards of BATCH samples)")
     a = ap.parse_args()
+    if a.mlp_workload:
+        print("loss first/last", mlp_workload(a.mlp_workload), flush=True)
+        raise SystemExit(0)
     r = report(seeds=a.seeds, train_steps=a.train_steps, log=lambda s: print(s, flush=True))
     print(json.dumps(r))
     if a.out:

@@ -51,3 +51,19 @@ def build(boxes):
 
 def grid_points(boxes):
     return sum(prm.nx * prm.ny * prm.nz for prm, _ in boxes)
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="run the standard ensemble draw")
+    ap.add_argument("--boxes", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--autotune", type=int, default=2, metavar="STEPS", help="steps per lane candidate (0: none)")
+    ap.add_argument("--steps", type=int, default=10)
+    a = ap.parse_args()
+    ens = build(ensemble_boxes(nbox=a.boxes, seed=a.seed))
+    if a.autotune:
+        ens.autotune(a.autotune)
+    ens.step(a.steps)
+    lanes, ms, box_ms = ens.tuning()
+    print("boxes %d lanes %d ms/step at 1,2,4,8 lanes %s" % (a.boxes, lanes, ms), flush=True)
