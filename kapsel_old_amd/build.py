@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspm.so")
+EXE = os.path.join(HERE, "bin", "spm")
 SOURCES = ["fft.hip", "fused.hip", "spectral.hip", "particles.hip", "electrolyte.hip", "shear.hip", "noalloc.hip", "closure.hip",
            "spm_api.cpp", "ensemble.cpp", "checkpoint.cpp", "udf.cpp"]
 # plain host C++ (no device code), built with the host compiler
@@ -58,7 +59,25 @@ def build(jobs=4, verbose=False, defines=(), out=LIB):
     os.replace(tmp, out)
     if verbose:
         print("built", out)
+    if out == LIB:
+        exe = build_program()
+        if verbose:
+            print("built", exe)
     return out
+
+
+def build_program():
+    """kapsel_old_amd/bin/spm: the C++ command-line program (csrc/spm_main.cpp),
+    host code over the library's C ABI, linked to the in-tree libspm.so."""
+    os.makedirs(os.path.dirname(EXE), exist_ok=True)
+    tmp = EXE + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "spm_main.cpp"), "-o", tmp,
+           "-L" + HERE, "-l:libspm.so", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("building the spm program failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    os.replace(tmp, EXE)
+    return EXE
 
 
 if __name__ == "__main__":
