@@ -7,6 +7,7 @@
 #include <cstring>
 #include <new>
 
+#include "mt19937.h"
 #include "spm_internal.h"
 #include "../../include/spm_closure.h"
 
@@ -61,58 +62,13 @@ static int fold_timing(spm_ctx *c) {
 
 namespace {
 
-// ---------------------------------------------------------------------
-// MT19937 (mt19937ar.c:65-171) and the polar Gaussian pair (fluct.h:48-60),
-// on the host: the thermal-noise stream is sequential with rejection and
-// uses the host libm's log/sqrt, so drawing it here keeps it identical to
-// the reference's stream.
-// ---------------------------------------------------------------------
-void mt_seed(spm_ctx *c, unsigned long s) {
-  c->mt[0] = (uint32_t)(s & 0xffffffffUL);
-  for (c->mti = 1; c->mti < SPM_MT_N; c->mti++)
-    c->mt[c->mti] = (uint32_t)(1812433253UL * (c->mt[c->mti - 1] ^ (c->mt[c->mti - 1] >> 30)) + c->mti);
-}
+// The thermal-noise stream (MT19937 + polar Gaussian pairs, mt19937.h) is
+// drawn on the host: it is sequential with rejection and uses the host
+// libm's log/sqrt, so drawing it here keeps it identical to the reference's.
+static_assert(SPM_MT_N == spm::kMtN, "the state size of include/spm.h");
+void mt_seed(spm_ctx *c, unsigned long s) { spm::mt_init(c->mt, c->mti, (uint32_t)(s & 0xffffffffUL)); }
 
-uint32_t mt_next(spm_ctx *c) {
-  constexpr int N = SPM_MT_N, M = 397;
-  static const uint32_t mag01[2] = {0x0U, 0x9908b0dfU};
-  uint32_t y;
-  if (c->mti >= N) {
-    if (c->mti == N + 1) mt_seed(c, 5489UL);
-    int kk;
-    for (kk = 0; kk < N - M; kk++) {
-      y = (c->mt[kk] & 0x80000000U) | (c->mt[kk + 1] & 0x7fffffffU);
-      c->mt[kk] = c->mt[kk + M] ^ (y >> 1) ^ mag01[y & 1U];
-    }
-    for (; kk < N - 1; kk++) {
-      y = (c->mt[kk] & 0x80000000U) | (c->mt[kk + 1] & 0x7fffffffU);
-      c->mt[kk] = c->mt[kk + (M - N)] ^ (y >> 1) ^ mag01[y & 1U];
-    }
-    y = (c->mt[N - 1] & 0x80000000U) | (c->mt[0] & 0x7fffffffU);
-    c->mt[N - 1] = c->mt[M - 1] ^ (y >> 1) ^ mag01[y & 1U];
-    c->mti = 0;
-  }
-  y = c->mt[c->mti++];
-  y ^= (y >> 11);
-  y ^= (y << 7) & 0x9d2c5680U;
-  y ^= (y << 15) & 0xefc60000U;
-  y ^= (y >> 18);
-  return y;
-}
-
-inline double mt_real3(spm_ctx *c) { return (((double)mt_next(c)) + 0.5) * (1.0 / 4294967296.0); }
-
-void gauss2(spm_ctx *c, double *out) {
-  double x1, x2, rsq;
-  do {
-    x1 = 2. * mt_real3(c) - 1.;
-    x2 = 2. * mt_real3(c) - 1.;
-    rsq = x1 * x1 + x2 * x2;
-  } while (rsq >= 1.0 || rsq == 0.0);
-  const double factor = std::sqrt(-2. * std::log(rsq) / rsq);
-  out[0] = factor * x1;
-  out[1] = factor * x2;
-}
+void gauss2(spm_ctx *c, double *out) { spm::mt_gauss2(c->mt, c->mti, out); }
 
 bool is_pow2(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
 // the fused passes and the sheared solver take power-of-two grids; others

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/spm_udf.h"
+#include "mt19937.h"
 
 namespace {
 
@@ -174,19 +175,28 @@ class Parser {
   std::vector<Token> &tok;
   size_t pos = 0;
 
+  // The token list always ends with one T_END; the cursor never moves past
+  // it, so a file cut anywhere reads as T_END from there on.
   const Token &peek() const { return tok[pos]; }
-  const Token &next() { return tok[pos++]; }
+  void adv() {
+    if (tok[pos].t != T_END) ++pos;
+  }
+  const Token &next() {
+    const Token &t = tok[pos];
+    adv();
+    return t;
+  }
   void expect(Tok t, const char *what) {
     if (peek().t != t) throw std::runtime_error(std::string("expected ") + what + " at line " + std::to_string(peek().line));
-    ++pos;
+    adv();
   }
   void skip_to(const std::string &dir) {
-    while (peek().t != T_END && !(peek().t == T_DIRECTIVE && peek().s == dir)) ++pos;
+    while (peek().t != T_END && !(peek().t == T_DIRECTIVE && peek().s == dir)) adv();
     if (peek().t == T_END) throw std::runtime_error("missing \\" + dir);
-    ++pos;
+    adv();
   }
   void skip_commas() {
-    while (peek().t == T_COMMA) ++pos;
+    while (peek().t == T_COMMA) adv();
   }
 
   // name with optional [] dimensions; returns the dimension count
@@ -196,7 +206,7 @@ class Parser {
     std::string name = t.s;
     dims = 0;
     while (peek().t == T_LBRACK) {
-      ++pos;
+      adv();
       expect(T_RBRACK, "]");
       ++dims;
     }
@@ -223,7 +233,7 @@ class Parser {
         ty->fields.push_back({nm, ft});
         skip_commas();
       }
-      ++pos;
+      adv();
     } else if (t.t == T_WORD) {
       if (t.s == "int") {
         ty->kind = K_INT;
@@ -236,10 +246,10 @@ class Parser {
         expect(T_LBRACE, "{");
         while (peek().t != T_RBRACE) {
           if (peek().t == T_STRING || peek().t == T_WORD) ty->options.push_back(next().s);
-          else if (peek().t == T_COMMA) ++pos;
+          else if (peek().t == T_COMMA) adv();
           else throw std::runtime_error("bad select at line " + std::to_string(peek().line));
         }
-        ++pos;
+        adv();
       } else {
         ty->kind = K_REF;
         ty->ref = t.s;
@@ -247,7 +257,7 @@ class Parser {
     } else {
       throw std::runtime_error("expected a type at line " + std::to_string(t.line));
     }
-    if (peek().t == T_STRING) ++pos;  // description
+    if (peek().t == T_STRING) adv();  // description
     return ty;
   }
 
@@ -255,7 +265,7 @@ class Parser {
     while (!(peek().t == T_DIRECTIVE && peek().s.rfind("end:", 0) == 0)) {
       if (peek().t == T_END) throw std::runtime_error("unterminated schema section");
       if (peek().t == T_WORD && peek().s == "class") {
-        ++pos;
+        adv();
         const std::string nm = next().s;
         expect(T_COLON, ":");
         doc.classes[nm] = type_expr();
@@ -274,7 +284,7 @@ class Parser {
       doc.entries.push_back({nm, ty});
       skip_commas();
     }
-    ++pos;
+    adv();
   }
 
   TypeP resolve(TypeP t) {
@@ -327,7 +337,7 @@ class Parser {
           v.items.push_back(value(ty->elem));
           skip_commas();
         }
-        ++pos;
+        adv();
         break;
       }
       default:
@@ -350,7 +360,7 @@ class Parser {
       out[nm] = value(ty);
       skip_commas();
     }
-    ++pos;
+    adv();
   }
 };
 
@@ -962,55 +972,18 @@ int nint(double a) { return a >= 0.0 ? int(a + .5) : int(a - .5); }  // Nint (ma
 // state they leave.
 class Mt {
  public:
-  explicit Mt(uint32_t s) {
-    mt_[0] = s;
-    for (mti_ = 1; mti_ < kN; ++mti_) mt_[mti_] = 1812433253U * (mt_[mti_ - 1] ^ (mt_[mti_ - 1] >> 30)) + mti_;
-  }
-  uint32_t next() {
-    static const uint32_t mag01[2] = {0x0U, 0x9908b0dfU};
-    uint32_t y;
-    if (mti_ >= kN) {
-      int kk;
-      for (kk = 0; kk < kN - kM; kk++) {
-        y = (mt_[kk] & 0x80000000U) | (mt_[kk + 1] & 0x7fffffffU);
-        mt_[kk] = mt_[kk + kM] ^ (y >> 1) ^ mag01[y & 1U];
-      }
-      for (; kk < kN - 1; kk++) {
-        y = (mt_[kk] & 0x80000000U) | (mt_[kk + 1] & 0x7fffffffU);
-        mt_[kk] = mt_[kk + (kM - kN)] ^ (y >> 1) ^ mag01[y & 1U];
-      }
-      y = (mt_[kN - 1] & 0x80000000U) | (mt_[0] & 0x7fffffffU);
-      mt_[kN - 1] = mt_[kM - 1] ^ (y >> 1) ^ mag01[y & 1U];
-      mti_ = 0;
-    }
-    y = mt_[mti_++];
-    y ^= (y >> 11);
-    y ^= (y << 7) & 0x9d2c5680U;
-    y ^= (y << 15) & 0xefc60000U;
-    y ^= (y >> 18);
-    return y;
-  }
-  double real3() { return (((double)next()) + 0.5) * (1.0 / 4294967296.0); }
-  void gauss2(double *out) {  // Marsaglia's polar method
-    double x1, x2, rsq;
-    do {
-      x1 = 2. * real3() - 1.;
-      x2 = 2. * real3() - 1.;
-      rsq = x1 * x1 + x2 * x2;
-    } while (rsq >= 1.0 || rsq == 0.0);
-    const double factor = std::sqrt(-2. * std::log(rsq) / rsq);
-    out[0] = factor * x1;
-    out[1] = factor * x2;
-  }
+  explicit Mt(uint32_t s) { spm::mt_init(mt_, mti_, s); }
+  uint32_t next() { return spm::mt_next(mt_, mti_); }
+  double real3() { return spm::mt_real3(mt_, mti_); }
+  void gauss2(double *out) { spm::mt_gauss2(mt_, mti_, out); }
   void state(uint32_t *mt, int *mti) const {
     std::memcpy(mt, mt_, sizeof(mt_));
     *mti = mti_;
   }
 
  private:
-  static constexpr int kN = SPM_MT_N, kM = 397;
   uint32_t mt_[SPM_MT_N];
-  int mti_;
+  int mti_ = 0;
 };
 
 // The LJ model of the initial relaxation: Lennard_Jones_f
