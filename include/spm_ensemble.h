@@ -16,10 +16,11 @@
  * processing time first on their measured step times, so the branches run
  * side by side on the device's hardware queues; one graph launch then
  * advances the whole ensemble by one step with no host work per box.
- * The thermostat's normals are drawn on the host ahead of time, for up to
- * SPM_ENSEMBLE_CHUNK steps, into a device pool per box that the graph
- * reads through a device-side counter; the draw order is the one spm_step
- * uses, so the MT19937 stream is unchanged.  A box's first step (ts == 0,
+ * The thermostat's normals are drawn on the host ahead of time,
+ * SPM_ENSEMBLE_CHUNK steps at a time, into a device ring per box that the
+ * graph reads through a device-side counter (the host draws the next chunk
+ * while the device runs the current one); the draw order is the one
+ * spm_step uses, so the MT19937 stream is unchanged.  A box's first step (ts == 0,
  * the Euler start of the AB2 integrators) runs outside the graph.
  *
  * Boxes must use the plain Navier-Stokes path: no electrolyte, no

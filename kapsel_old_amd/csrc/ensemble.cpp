@@ -28,6 +28,7 @@ struct spm_ensemble {
   hipStream_t lanes[SPM_ENSEMBLE_MAX_LANES] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[SPM_ENSEMBLE_MAX_LANES] = {};
+  hipEvent_t copy_done[2] = {};  // the pool copies of each half of the ring
   int nlanes = 1;
   std::vector<std::vector<int>> split;  // boxes of each lane
   bool use_graph = true;
@@ -186,40 +187,62 @@ int capture(spm_ensemble *e) {
   return 0;
 }
 
-int fill_pools(spm_ensemble *e, int k);
-
-// k steps of every box with ts > 0 (k <= SPM_ENSEMBLE_CHUNK), optionally
-// timed (ms for the k steps, device time between events on the master)
-int run_chunk(spm_ensemble *e, int k, float *ms) {
+// nsteps steps of every box with ts > 0, replayed (or host-launched) on the
+// master stream, optionally timed (ms: device time between events around
+// the steps).  The noise pools are rings of 2 SPM_ENSEMBLE_CHUNK slots: the
+// host draws chunk i + 1 into one half while the device reads chunk i from
+// the other; a half's staging is rewritten only after the copy that last
+// read it has finished (copy_done), and its device slots only by a copy
+// queued behind the replays that read them.
+int run_steps(spm_ensemble *e, int nsteps, float *ms) {
   const int nb = (int)e->box.size();
-  // the pools are refilled only when nothing queued still reads them; the
-  // copies are ordered before the replays on the master stream
-  if (int rc = fill_pools(e, k)) return rc;
+  if (int rc = wait_all(e)) return rc;
+  for (int b = 0; b < nb; ++b) {
+    spm_ctx *c = e->box[b];
+    c->stream = e->master;
+    if (int rc = spm::pool_reset(c)) return box_fail(e, b, rc);
+  }
   if (e->use_graph && (!e->exec || e->exec_lanes != e->nlanes))
     if (int rc = capture(e)) return rc;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (ms) {
     EH(hipEventCreate(&t0));
     EH(hipEventCreate(&t1));
-    EH(hipEventRecord(t0, e->master));
   }
-  for (int s = 0; s < k; ++s) {
-    if (e->use_graph) {
-      EH(hipGraphLaunch(e->exec, e->master));
-      for (int b = 0; b < nb; ++b) {
-        e->box[b]->time += e->box[b]->prm.dt;
-        e->box[b]->ts++;
-      }
-    } else {
-      // lanes wait for the pool copies on the master, the master for the lanes
-      EH(hipEventRecord(e->fork, e->master));
-      for (int l = 0; l < e->nlanes; ++l) EH(hipStreamWaitEvent(e->lanes[l], e->fork, 0));
-      if (int rc = eager_step(e, true, nullptr)) return rc;
-      for (int l = 0; l < e->nlanes; ++l) {
-        EH(hipEventRecord(e->join[l], e->lanes[l]));
-        EH(hipStreamWaitEvent(e->master, e->join[l], 0));
+  bool timing_started = false;
+  for (int done = 0, chunk = 0; done < nsteps; ++chunk) {
+    const int k = std::min(nsteps - done, SPM_ENSEMBLE_CHUNK);
+    const int half = chunk & 1;
+    EH(hipEventSynchronize(e->copy_done[half]));
+    for (int b = 0; b < nb; ++b) {
+      spm_ctx *c = e->box[b];
+      c->stream = e->master;
+      if (int rc = spm::pool_fill(c, k, half * SPM_ENSEMBLE_CHUNK)) return box_fail(e, b, rc);
+    }
+    EH(hipEventRecord(e->copy_done[half], e->master));
+    if (ms && !timing_started) {
+      EH(hipEventRecord(t0, e->master));
+      timing_started = true;
+    }
+    for (int s = 0; s < k; ++s) {
+      if (e->use_graph) {
+        EH(hipGraphLaunch(e->exec, e->master));
+        for (int b = 0; b < nb; ++b) {
+          e->box[b]->time += e->box[b]->prm.dt;
+          e->box[b]->ts++;
+        }
+      } else {
+        // lanes wait for the pool copies on the master, the master for the lanes
+        EH(hipEventRecord(e->fork, e->master));
+        for (int l = 0; l < e->nlanes; ++l) EH(hipStreamWaitEvent(e->lanes[l], e->fork, 0));
+        if (int rc = eager_step(e, true, nullptr)) return rc;
+        for (int l = 0; l < e->nlanes; ++l) {
+          EH(hipEventRecord(e->join[l], e->lanes[l]));
+          EH(hipStreamWaitEvent(e->master, e->join[l], 0));
+        }
       }
     }
+    done += k;
   }
   if (ms) {
     EH(hipEventRecord(t1, e->master));
@@ -253,17 +276,6 @@ bool any_unstarted(const spm_ensemble *e) {
   return false;
 }
 
-// draws k steps of noise into every box's pool and waits for the copies
-int fill_pools(spm_ensemble *e, int k) {
-  if (int rc = wait_all(e)) return rc;
-  for (size_t b = 0; b < e->box.size(); ++b) {
-    spm_ctx *c = e->box[b];
-    c->stream = e->master;
-    if (int rc = spm::pool_fill(c, k)) return box_fail(e, (int)b, rc);
-  }
-  return 0;
-}
-
 int phase_all(spm_ensemble *e, int phase) {
   for (size_t b = 0; b < e->box.size(); ++b) {
     spm_ctx *c = e->box[b];
@@ -274,7 +286,7 @@ int phase_all(spm_ensemble *e, int phase) {
 }
 
 // nsteps steps of every box, as spm_step(box, nsteps) would take them
-int ensemble_step(spm_ensemble *e, int nsteps, float *ms_first_chunk = nullptr) {
+int ensemble_step(spm_ensemble *e, int nsteps, float *ms = nullptr) {
   if (nsteps <= 0) return 0;
   if (int rc = check_boxes(e)) return rc;
   if (int rc = phase_all(e, 0)) return rc;
@@ -283,13 +295,8 @@ int ensemble_step(spm_ensemble *e, int nsteps, float *ms_first_chunk = nullptr) 
     if (int rc = plain_step_all(e)) return rc;
     --left;
   }
-  bool first = true;
-  while (left > 0) {
-    const int k = std::min(left, SPM_ENSEMBLE_CHUNK);
-    if (int rc = run_chunk(e, k, first ? ms_first_chunk : nullptr)) return rc;
-    first = false;
-    left -= k;
-  }
+  if (left > 0)
+    if (int rc = run_steps(e, left, ms)) return rc;
   if (int rc = phase_all(e, 2)) return rc;
   for (size_t b = 0; b < e->box.size(); ++b) e->box[b]->stream = e->own_stream[b];
   return 0;
@@ -330,8 +337,12 @@ int spm_ensemble_create(int nbox, const spm_params *params, spm_ensemble **out) 
   }
   int rc = 0;
   if (hipSetDevice(e->device) != hipSuccess || hipStreamCreate(&e->master) != hipSuccess ||
-      hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&e->copy_done[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&e->copy_done[1], hipEventDisableTiming) != hipSuccess)
     rc = SPM_ERR_HIP;
+  for (int h = 0; h < 2 && !rc; ++h)  // both halves of the pools start free
+    if (hipEventRecord(e->copy_done[h], e->master) != hipSuccess) rc = SPM_ERR_HIP;
   for (int l = 0; l < SPM_ENSEMBLE_MAX_LANES && !rc; ++l)
     if (hipStreamCreate(&e->lanes[l]) != hipSuccess ||
         hipEventCreateWithFlags(&e->join[l], hipEventDisableTiming) != hipSuccess)
@@ -339,7 +350,7 @@ int spm_ensemble_create(int nbox, const spm_params *params, spm_ensemble **out) 
   for (int b = 0; b < nbox && !rc; ++b) {
     spm_ctx *c = e->box[b];
     e->cost.push_back(estimate_cost(c));
-    rc = spm::pool_setup(c, SPM_ENSEMBLE_CHUNK);
+    rc = spm::pool_setup(c, 2 * SPM_ENSEMBLE_CHUNK);
     if (!rc) rc = spm::tune_mixed_transforms(c);
     if (rc) {
       const std::string msg = "box " + std::to_string(b) + ": " + c->err;
@@ -375,6 +386,8 @@ int spm_ensemble_destroy(spm_ensemble *e) {
     if (e->join[l]) (void)hipEventDestroy(e->join[l]);
   }
   if (e->fork) (void)hipEventDestroy(e->fork);
+  for (hipEvent_t ev : e->copy_done)
+    if (ev) (void)hipEventDestroy(ev);
   if (e->master) (void)hipStreamDestroy(e->master);
   delete e;
   return SPM_OK;
@@ -422,7 +435,13 @@ int spm_ensemble_autotune(spm_ensemble *e, int steps_per_candidate) {
   if (any_unstarted(e))
     if (int rc = plain_step_all(e)) return restore(rc);
   {
-    if (int rc = fill_pools(e, 1)) return restore(rc);
+    for (int b = 0; b < nb; ++b) {  // one step of noise in slot 0
+      spm_ctx *c = e->box[b];
+      c->stream = e->master;
+      int rc = spm::pool_reset(c);
+      if (!rc) rc = spm::pool_fill(c, 1, 0);
+      if (rc) return restore(box_fail(e, b, rc));
+    }
     if (int rc = wait_all(e)) return restore(rc);
     std::vector<hipEvent_t> ev(2 * nb, nullptr);
     for (auto &x : ev) EH(hipEventCreate(&x));
@@ -445,8 +464,8 @@ int spm_ensemble_autotune(spm_ensemble *e, int steps_per_candidate) {
     e->split.clear();
     set_lanes(e, cand[k]);
     float ms = 0.f;
-    if (int rc = run_chunk(e, 1, nullptr)) return restore(rc);  // capture + first replay, untimed
-    if (int rc = run_chunk(e, steps_per_candidate, &ms)) return restore(rc);
+    if (int rc = run_steps(e, 1, nullptr)) return restore(rc);  // capture + first replay, untimed
+    if (int rc = run_steps(e, steps_per_candidate, &ms)) return restore(rc);
     e->tune_ms[k] = ms / steps_per_candidate;
     if (best < 0 || e->tune_ms[k] < e->tune_ms[best]) best = k;
   }

@@ -209,6 +209,95 @@ __global__ __launch_bounds__(64) void sort_segments_kernel(const int *__restrict
     }
 }
 
+// The whole tile-list build of a small box in one block: counts in LDS,
+// the exclusive scan, the fill with LDS cursors and the per-tile sort, with
+// barriers between the phases (the lists equal those of the multi-kernel
+// build: each tile's members sorted ascending).  For boxes of at most
+// kSmallTiles tiles, where the five launches of the general build each cost
+// more than their work.
+constexpr int kSmallTiles = 4096;
+__global__ __launch_bounds__(1024) void tiles_small_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+                                                           int *pxint, double *pres, int tx, int ty, int tz, int k0,
+                                                           int k1, int k2, int ntiles, int *tile_count,
+                                                           int *tile_start, int *tile_list) {
+  __shared__ int cnt[kSmallTiles], cur[kSmallTiles];
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (int i = t; i < ntiles; i += 1024) cnt[i] = 0;
+  __syncthreads();
+  const int per = k0 * k1 * k2;
+  const long slots = (long)pm.np * per;
+  const int N[3] = {g.nx, g.ny, g.nz};
+  const int nt[3] = {tx, ty, tz}, k[3] = {k0, k1, k2};
+  const double idx = 1. / pm.dx;
+  for (long q = t; q < slots; q += 1024) {
+    const int n = (int)(q / per), sl = (int)(q % per);
+    int xi[3], t0[3], cn[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double dmy = P(pb, pm.np, P_X + d, n) * idx;
+      xi[d] = (dmy >= 0.0 && dmy < (double)N[d]) ? (int)dmy : 0;
+      if (sl == 0) {
+        pxint[3 * n + d] = xi[d];
+        pres[3 * n + d] = (dmy - (double)xi[d]) * pm.dx;
+      }
+      tile_span(xi[d], pm.nh, nt[d], t0[d], cn[d]);
+    }
+    const int ti = tile_slot(sl, k, t0, cn, nt);
+    if (ti >= 0) atomicAdd(&cnt[ti], 1);
+  }
+  __syncthreads();
+  // exclusive scan: 4 consecutive counts per thread, wavefront shuffles,
+  // one LDS exchange across the 16 wavefronts
+  int v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = (4 * t + q < ntiles) ? cnt[4 * t + q] : 0;
+  int x = v[0] + v[1] + v[2] + v[3];
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int before = 0;
+  for (int k2w = 0; k2w < w; ++k2w) before += wsum[k2w];
+  int run = before + x - (v[0] + v[1] + v[2] + v[3]);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = 4 * t + q;
+    if (i < ntiles) {
+      cur[i] = run;
+      tile_start[i] = run;
+      tile_count[i] = v[q];
+    }
+    run += v[q];
+  }
+  __syncthreads();
+  for (long q = t; q < slots; q += 1024) {
+    const int n = (int)(q / per), sl = (int)(q % per);
+    int t0[3], cn[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) tile_span(pxint[3 * n + d], pm.nh, nt[d], t0[d], cn[d]);
+    const int ti = tile_slot(sl, k, t0, cn, nt);
+    if (ti >= 0) tile_list[atomicAdd(&cur[ti], 1)] = n;
+  }
+  __syncthreads();
+  for (int i = t; i < ntiles; i += 1024) {
+    int *a = tile_list + (cur[i] - cnt[i]);
+    const int m = cnt[i];
+    for (int p = 1; p < m; ++p) {
+      const int val = a[p];
+      int j = p - 1;
+      while (j >= 0 && a[j] > val) {
+        a[j + 1] = a[j];
+        --j;
+      }
+      a[j + 1] = val;
+    }
+  }
+}
+
 __global__ void izero_kernel(int *a, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) a[t] = 0;
@@ -388,21 +477,37 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // u is gathered in the field precision T; sums run in FP64.
-template <typename T>
-__global__ __launch_bounds__(256) void f_hydro_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ pxint,
-                                                      const double *__restrict__ pres,
-                                                      const int *__restrict__ stencil, int nst,
-                                                      const T *__restrict__ u0, const T *__restrict__ u1,
-                                                      const T *__restrict__ u2, double dmy) {
+// W wavefronts per particle: wave w of a particle takes stencil points
+// w*64 + lane, stepping 64 W; the W wave sums are added in wave order.  W = 1
+// (many particles) keeps every wave busy with its own particle; few
+// particles (small boxes) take W up to 16 so a stencil of up to 5185 points
+// is not walked by one latency-bound wave.  Blocks hold 256 threads, or one
+// particle of 64 W threads when that is more.
+template <int W>
+constexpr int f_hydro_block() { return 64 * W > 256 ? 64 * W : 256; }
+
+template <typename T, int W>
+__global__ __launch_bounds__(f_hydro_block<W>()) void f_hydro_kernel(Grid g, PModel pm, double *pb,
+                                                                    const int *__restrict__ pxint,
+                                                                    const double *__restrict__ pres,
+                                                                    const int *__restrict__ stencil, int nst,
+                                                                    const T *__restrict__ u0,
+                                                                    const T *__restrict__ u1,
+                                                                    const T *__restrict__ u2, double dmy) {
+  constexpr int BS = f_hydro_block<W>(), PPB = BS / (64 * W);
+  __shared__ double part[BS / 64][6];
   const int lane = threadIdx.x & 63;
-  const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (n >= pm.np) return;
+  const int wv = (threadIdx.x >> 6) % W;
+  const int n = blockIdx.x * PPB + (int)(threadIdx.x >> 6) / W;
+  const bool active = n < pm.np;
+  if (W == 1 && !active) return;
+  const int nn = active ? n : 0;
   const int N[3] = {g.nx, g.ny, g.nz};
-  const int xi[3] = {pxint[3 * n], pxint[3 * n + 1], pxint[3 * n + 2]};
-  const double res[3] = {pres[3 * n], pres[3 * n + 1], pres[3 * n + 2]};
-  const double xp[3] = {P(pb, pm.np, P_X, n), P(pb, pm.np, P_X + 1, n), P(pb, pm.np, P_X + 2, n)};
-  const double vp[3] = {P(pb, pm.np, P_V, n), P(pb, pm.np, P_V + 1, n), P(pb, pm.np, P_V + 2, n)};
-  const double wp[3] = {P(pb, pm.np, P_OMEGA, n), P(pb, pm.np, P_OMEGA + 1, n), P(pb, pm.np, P_OMEGA + 2, n)};
+  const int xi[3] = {pxint[3 * nn], pxint[3 * nn + 1], pxint[3 * nn + 2]};
+  const double res[3] = {pres[3 * nn], pres[3 * nn + 1], pres[3 * nn + 2]};
+  const double xp[3] = {P(pb, pm.np, P_X, nn), P(pb, pm.np, P_X + 1, nn), P(pb, pm.np, P_X + 2, nn)};
+  const double vp[3] = {P(pb, pm.np, P_V, nn), P(pb, pm.np, P_V + 1, nn), P(pb, pm.np, P_V + 2, nn)};
+  const double wp[3] = {P(pb, pm.np, P_OMEGA, nn), P(pb, pm.np, P_OMEGA + 1, nn), P(pb, pm.np, P_OMEGA + 2, nn)};
   const double rcut = pm.radius + pm.hxi;
   // single-precision screen as in make_phi: a point whose distance is below
   // rcut has a float distance below rcut + 1e-3 dx, so the FP64 distance
@@ -410,7 +515,7 @@ __global__ __launch_bounds__(256) void f_hydro_kernel(Grid g, PModel pm, double 
   const float rscreen2 = (float)((rcut + 1e-3 * pm.dx) * (rcut + 1e-3 * pm.dx));
   const float fres[3] = {(float)res[0], (float)res[1], (float)res[2]};
   double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0;
-  for (int s = lane; s < nst; s += 64) {
+  for (int s = active ? wv * 64 + lane : nst; s < nst; s += 64 * W) {
     {
       const float f0 = (float)stencil[3 * s] * (float)pm.dx - fres[0];
       const float f1 = (float)stencil[3 * s + 1] * (float)pm.dx - fres[1];
@@ -456,7 +561,22 @@ __global__ __launch_bounds__(256) void f_hydro_kernel(Grid g, PModel pm, double 
   tx = wave_sum(tx);
   ty = wave_sum(ty);
   tz = wave_sum(tz);
-  if (lane == 0) {
+  if constexpr (W > 1) {
+    const int slot = threadIdx.x >> 6;
+    if (lane == 0) {
+      part[slot][0] = fx; part[slot][1] = fy; part[slot][2] = fz;
+      part[slot][3] = tx; part[slot][4] = ty; part[slot][5] = tz;
+    }
+    __syncthreads();
+    if (wv == 0 && lane == 0) {
+      fx = fy = fz = tx = ty = tz = 0.0;
+      for (int w = 0; w < W; ++w) {
+        fx += part[slot + w][0]; fy += part[slot + w][1]; fz += part[slot + w][2];
+        tx += part[slot + w][3]; ty += part[slot + w][4]; tz += part[slot + w][5];
+      }
+    }
+  }
+  if (active && wv == 0 && lane == 0) {
     PR(pb, pm.np, P_F_HYDRO, n) = (dmy * fx) * 1.0;
     PR(pb, pm.np, P_F_HYDRO + 1, n) = (dmy * fy) * 1.0;
     PR(pb, pm.np, P_F_HYDRO + 2, n) = (dmy * fz) * 1.0;
@@ -1105,12 +1225,20 @@ int p_position(spm_ctx *c, int euler) {
 int p_build_tiles(spm_ctx *c) {
   if (c->np == 0) return 0;
   tbeg(c);
-  hipLaunchKernelGGL(izero_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_count, c->ntiles);
   // tiles a stencil cube of 2 nh + 1 points can reach per axis (the same
   // bound sizes tile_list)
   int k[3];
   for (int d = 0; d < 3; ++d) k[d] = std::min(c->tiles[d], (2 * c->pm.nh + 1 + 7) / 8 + 1);
   const long slots = (long)c->np * k[0] * k[1] * k[2];
+  if (c->ntiles <= kSmallTiles && slots <= (1L << 16)) {
+    hipLaunchKernelGGL(tiles_small_kernel, dim3(1), dim3(1024), 0, c->stream, c->grid, c->pm, c->pbuf, c->pxint,
+                       c->pres, c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->ntiles, c->tile_count,
+                       c->tile_start, c->tile_list);
+    tend(c, SPM_K_PARTICLE, 16.0 * c->ntiles);
+    SPM_LAUNCH_CHECK(c);
+    return 0;
+  }
+  hipLaunchKernelGGL(izero_kernel, dim3(blocks(c->ntiles, 256)), dim3(256), 0, c->stream, c->tile_count, c->ntiles);
   hipLaunchKernelGGL(anchor_count_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
                      c->pxint, c->pres, c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_count);
   hipLaunchKernelGGL(scan_kernel, dim3(scan_blocks(c->ntiles)), dim3(1024), 0, c->stream, c->tile_count, c->tile_start, c->tile_cursor,
@@ -1166,6 +1294,24 @@ int p_make_f(spm_ctx *c) { return launch_make_phi(c, 2, nullptr, c->f[0], c->f[1
 
 int p_calc_f_hydro(spm_ctx *c) { return p_calc_f_hydro_field(c, c->u); }
 
+template <typename T, int W>
+void launch_f_hydro_w(spm_ctx *c, const T *u0, const T *u1, const T *u2, double dmy) {
+  constexpr int BS = f_hydro_block<W>(), PPB = BS / (64 * W);
+  hipLaunchKernelGGL((f_hydro_kernel<T, W>), dim3((unsigned)((c->np + PPB - 1) / PPB)), dim3(BS), 0, c->stream,
+                     c->grid, c->pm, c->pbuf, c->pxint, c->pres, c->stencil, c->n_stencil, u0, u1, u2, dmy);
+}
+
+template <typename T>
+void launch_f_hydro(spm_ctx *c, int w, const T *u0, const T *u1, const T *u2, double dmy) {
+  switch (w) {
+    case 1: launch_f_hydro_w<T, 1>(c, u0, u1, u2, dmy); break;
+    case 2: launch_f_hydro_w<T, 2>(c, u0, u1, u2, dmy); break;
+    case 4: launch_f_hydro_w<T, 4>(c, u0, u1, u2, dmy); break;
+    case 8: launch_f_hydro_w<T, 8>(c, u0, u1, u2, dmy); break;
+    default: launch_f_hydro_w<T, 16>(c, u0, u1, u2, dmy); break;
+  }
+}
+
 // Calc_f_hydro_correct_precision(p, fld, jikan) on any real-space field
 // (the electrolyte step passes the f field, sp_3d_ns.cxx:97-106).
 int p_calc_f_hydro_field(spm_ctx *c, double *const fld[3]) {
@@ -1175,14 +1321,14 @@ int p_calc_f_hydro_field(spm_ctx *c, double *const fld[3]) {
   if (rc) return rc;
   const double dmy = (-c->pm.dx3 * c->pm.rho) / c->prm.dt;
   tbeg(c);
+  // waves per particle: about 8 waves per compute unit in all, 1 to 16 each
+  const long want = 8L * c->n_cu;
+  int w = 1;
+  while (w < 16 && (long)c->np * (2 * w) <= want) w *= 2;
   if (c->prm.precision == SPM_FP32)
-    hipLaunchKernelGGL(f_hydro_kernel<float>, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm,
-                       c->pbuf, c->pxint, c->pres, c->stencil, c->n_stencil, (const float *)fld[0],
-                       (const float *)fld[1], (const float *)fld[2], dmy);
+    launch_f_hydro<float>(c, w, (const float *)fld[0], (const float *)fld[1], (const float *)fld[2], dmy);
   else
-    hipLaunchKernelGGL(f_hydro_kernel<double>, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm,
-                       c->pbuf, c->pxint, c->pres, c->stencil, c->n_stencil, (const double *)fld[0],
-                       (const double *)fld[1], (const double *)fld[2], dmy);
+    launch_f_hydro<double>(c, w, (const double *)fld[0], (const double *)fld[1], (const double *)fld[2], dmy);
   // gathers u (3 doubles) at every stencil point of every particle
   tend(c, SPM_K_F_HYDRO, 3.0 * (c->prm.precision == SPM_FP32 ? 4.0 : 8.0) * (double)c->n_stencil * c->np);
   SPM_LAUNCH_CHECK(c);
@@ -1357,16 +1503,16 @@ int p_f_add_obl(spm_ctx *c, double *const ucp[3]) {
   return 0;
 }
 
-// One block: read the slot, copy its 6 np normals, advance the counter.  A
-// slot past the pool (the host refills before that can happen) yields NaN,
-// which the finiteness check reports, instead of a read out of bounds.
+// One block: read the counter, copy the 6 np normals of slot
+// counter mod nslots, advance the counter.  The pool is a ring the host
+// refills half by half while the other half is read.
 __global__ void noise_pool_kernel(const double *__restrict__ pool, int *counter, int nslots, int n6,
                                   double *__restrict__ dst) {
   __shared__ int slot;
   if (threadIdx.x == 0) slot = *counter;
   __syncthreads();
-  const bool ok = slot >= 0 && slot < nslots;
-  for (int i = threadIdx.x; i < n6; i += blockDim.x) dst[i] = ok ? pool[(long)slot * n6 + i] : __builtin_nan("");
+  const long base = (long)(slot % nslots) * n6;
+  for (int i = threadIdx.x; i < n6; i += blockDim.x) dst[i] = pool[base + i];
   __syncthreads();
   if (threadIdx.x == 0) *counter = slot + 1;
 }

@@ -873,25 +873,33 @@ int pool_setup(spm_ctx *c, int steps) {
 }
 
 // Draws the next k steps of noise (the order thermostat() draws them in,
-// step by step) and queues their copy and the counter reset on c->stream.
-// The caller makes sure no queued work still reads the pool.
-int pool_fill(spm_ctx *c, int k) {
+// step by step) into pool slots slot0 .. slot0 + k - 1 and queues their
+// copy on c->stream.  The caller makes sure no queued work still reads
+// those slots and no earlier copy still reads their staging.
+int pool_fill(spm_ctx *c, int k, int slot0) {
   if (!box_uses_noise(c)) return 0;
-  if (k > c->pool_steps) {
+  if (slot0 < 0 || slot0 + k > c->pool_steps) {
     c->err = "noise pool smaller than the steps asked for";
     return SPM_ERR_INVALID;
   }
-  double *h = c->pool_host;
+  const size_t n6 = (size_t)6 * c->np;
+  double *h = c->pool_host + (size_t)slot0 * n6;
   for (int s = 0; s < k; ++s)
     for (int n = 0; n < c->np; ++n) {
-      double *d = h + ((size_t)s * c->np + n) * 6;
+      double *d = h + (size_t)s * n6 + (size_t)n * 6;
       gauss2(c, d);
       gauss2(c, d + 2);
       gauss2(c, d + 4);
     }
-  SPM_HIP(c, hipMemcpyAsync(c->pool_dev, h, sizeof(double) * 6 * c->np * k, hipMemcpyHostToDevice, c->stream));
-  RC(k_zero_int(c, c->pool_counter, 1));
+  SPM_HIP(c, hipMemcpyAsync(c->pool_dev + (size_t)slot0 * n6, h, sizeof(double) * n6 * k, hipMemcpyHostToDevice,
+                            c->stream));
   return 0;
+}
+
+// the thermostat's next read is pool slot 0
+int pool_reset(spm_ctx *c) {
+  if (!box_uses_noise(c)) return 0;
+  return k_zero_int(c, c->pool_counter, 1);
 }
 
 }  // namespace spm

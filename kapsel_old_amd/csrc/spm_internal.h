@@ -176,9 +176,9 @@ struct spm_ctx {
   int noise_slot = 0;
   double noise_v = 0.0, noise_o = 0.0;
   // noise drawn ahead for pool_steps whole steps (ensembles replaying a
-  // captured step): with pool_mode the thermostat takes slot *pool_counter
-  // of pool_dev on the device and advances the counter, so a replayed graph
-  // needs no host work per step
+  // captured step): with pool_mode the thermostat takes slot
+  // *pool_counter mod pool_steps of pool_dev on the device and advances the
+  // counter, so a replayed graph needs no host work per step
   bool pool_mode = false;
   int pool_steps = 0;
   double *pool_dev = nullptr;   // pool_steps * 6 * np
@@ -326,7 +326,8 @@ int p_noise_from_pool(spm_ctx *c);
 int box_step_phase(spm_ctx *c, int phase);
 bool box_uses_noise(const spm_ctx *c);
 int pool_setup(spm_ctx *c, int steps);
-int pool_fill(spm_ctx *c, int k);
+int pool_fill(spm_ctx *c, int k, int slot0);
+int pool_reset(spm_ctx *c);
 int tune_mixed_transforms(spm_ctx *c);
 // the autotuner's table as text, one line per tuned pass
 std::string tune_report();

@@ -60,10 +60,11 @@ def test_boxes_bit_identical_to_lone_contexts(precision):
     boxes = _small_boxes(precision)
     ens = build(boxes)
     lone = _lone(boxes)
-    # 1 + 32 + 7: the Euler start, a full noise chunk and a partial one
-    ens.step(40)
+    # 1 + 32 + 32 + 7: the Euler start, both halves of the noise ring, then
+    # the first half again, partly
+    ens.step(72)
     for s in lone:
-        s.step(40)
+        s.step(72)
     for b, (e, s) in enumerate(zip(ens.boxes, lone)):
         _same(e, s, "box %d" % b)
     # a second call continues where the first stopped
@@ -72,7 +73,7 @@ def test_boxes_bit_identical_to_lone_contexts(precision):
         s.step(3)
     for b, (e, s) in enumerate(zip(ens.boxes, lone)):
         _same(e, s, "box %d, second call" % b)
-    print("ensemble of %d boxes bit-identical to lone contexts after 43 steps" % len(boxes), flush=True)
+    print("ensemble of %d boxes bit-identical to lone contexts after 75 steps" % len(boxes), flush=True)
 
 
 def test_graph_and_host_launched_steps_agree():
