@@ -181,8 +181,13 @@ def closure_line():
     net = Closure(1024, seed=1)
     coarse = cl.free_run_speed(64, 3.0, pos, 0.05, 1000, abi.FP32, closure=net)
     fine = cl.free_run_speed(256, 12.0, pos * 4, 0.8, 100, abi.FP64)
+    coarse_g = cl.free_run_speed(64, 3.0, pos, 0.05, 1000, abi.FP32, closure=net, graph=True)
+    fine_g = cl.free_run_speed(256, 12.0, pos * 4, 0.8, 100, abi.FP64, graph=True)
     out = {"workload": "512 spheres, volume fraction 0.22: FP32 64^3 a=3 + closure vs FP64 256^3 a=12",
-           "steps_per_s_fp32_coarse_closure": coarse, "steps_per_s_fp64_fine": fine, "speedup": coarse / fine}
+           "steps_per_s_fp32_coarse_closure": coarse, "steps_per_s_fp64_fine": fine, "speedup": coarse / fine,
+           "graph_replayed": {"steps_per_s_fp32_coarse_closure": coarse_g, "steps_per_s_fp64_fine": fine_g,
+                              "speedup": coarse_g / fine_g,
+                              "note": "both steps replayed from a captured graph (one-box spm_ensemble)"}}
     path = os.path.join(ROOT, "profiles", "closure_report_64cube_vs_256cube.json")
     if os.path.exists(path):
         with open(path) as fh:
@@ -198,16 +203,20 @@ def ensemble_line(nbox=64, steps=200, seed=4):
     {32, 48, 64, 80, 96}, 1-50 spheres a = 3-6 each, WCA, kBT = 0.05, FP32
     fields, `steps` steps of every box from one graph replay per step, after
     the autotuner has chosen the lane count (its steps are untimed); the
-    host-launched rate of the same ensemble beside it."""
+    host-launched rate of the same ensemble beside it.  `value` is the median
+    of 3 timed runs of `steps` steps."""
     from kapsel_old_amd.ensemble import build, ensemble_boxes, grid_points
     from kapsel_old_amd.spm import autotune_report
     boxes = ensemble_boxes(nbox=nbox, seed=seed)
     ens = build(boxes)
     ens.autotune(3)
     ens.step(2)
-    t0 = time.perf_counter()
-    ens.step(steps)
-    el = time.perf_counter() - t0
+    runs = []
+    for _ in range(3):  # the median of 3 timed runs: the spread is about 10%
+        t0 = time.perf_counter()
+        ens.step(steps)
+        runs.append(time.perf_counter() - t0)
+    el = float(np.median(runs))
     lanes, tune_ms, box_ms = ens.tuning()
     ens.set_graph(False)
     k = max(10, steps // 10)
@@ -220,6 +229,7 @@ def ensemble_line(nbox=64, steps=200, seed=4):
                         "WCA, kBT=0.05, FP32 fields, %d steps" % (nbox, sum(len(p) for _, p in boxes), steps),
             "value": nbox * steps / el, "unit": "box-steps/s",
             "ensemble_steps_per_s": steps / el, "ms_per_ensemble_step": 1e3 * el / steps,
+            "ms_per_ensemble_step_runs": [1e3 * r / steps for r in runs],
             "grid_point_steps_per_s": pts * steps / el, "grid_points": pts,
             "lanes": lanes, "ms_per_step_at_1_2_4_8_lanes": [float(x) for x in tune_ms],
             "sum_of_lone_box_ms": float(box_ms.sum()),

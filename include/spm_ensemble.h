@@ -24,8 +24,10 @@
  * the Euler start of the AB2 integrators) runs outside the graph.
  *
  * Boxes must use the plain Navier-Stokes path: no electrolyte, no
- * Lees-Edwards shear, no attached closure and timing off (those steps are
- * rejected with SPM_ERR_UNSUPPORTED).
+ * Lees-Edwards shear and timing off; a box may have a closure network
+ * attached (spm_set_closure), one network per box (those steps are
+ * rejected with SPM_ERR_UNSUPPORTED).  A one-box ensemble is the way to
+ * replay a single context's step from a graph.
  */
 #ifndef SPM_ENSEMBLE_H
 #define SPM_ENSEMBLE_H

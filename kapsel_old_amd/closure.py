@@ -161,25 +161,34 @@ def augment(x, t):
     return np.concatenate([p[0] for p in pairs]), np.concatenate([p[1] for p in pairs])
 
 
-def free_run_speed(n_grid, a, pos, dt, steps, precision, closure=None, warmup=20):
+def free_run_speed(n_grid, a, pos, dt, steps, precision, closure=None, warmup=20, graph=False):
     """Steps per second of free suspensions (the production use) with the
-    closure attached or not."""
+    closure attached or not; graph=True replays the step from a captured
+    graph (a one-box ensemble, include/spm_ensemble.h) instead of launching
+    its kernels from the host."""
     import time
+    from .spm import Ensemble
     n = len(pos)
-    g = SPM(_params(n_grid, a, n, dt, precision))
+    prm = _params(n_grid, a, n, dt, precision)
+    ens = Ensemble([prm]) if graph else None
+    g = ens.boxes[0] if graph else SPM(prm)
     p = abi.new_particles(n)
     p["x"], p["x_nopbc"] = pos, pos
     g.put_particles(p)
     g.init_fluid()
     if closure is not None:
         g.set_closure(closure)
-    g.step(warmup)
+    run = ens.step if graph else g.step
+    run(warmup)
     g.synchronize()
     t0 = time.perf_counter()
-    g.step(steps)
+    run(steps)
     g.synchronize()
     t1 = time.perf_counter()
-    g.close()
+    if graph:
+        ens.close()
+    else:
+        g.close()
     return steps / (t1 - t0)
 
 

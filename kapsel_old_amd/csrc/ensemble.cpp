@@ -111,12 +111,20 @@ int wait_all(spm_ensemble *e) {
 int check_boxes(spm_ensemble *e) {
   for (size_t b = 0; b < e->box.size(); ++b) {
     const spm_ctx *c = e->box[b];
-    if (c->electro || c->le || c->closure || c->timing) {
+    if (c->electro || c->le || c->timing) {
       char m[128];
       std::snprintf(m, sizeof m, "box %zu: ensembles run the plain Navier-Stokes step (no electrolyte, "
-                    "shear, closure or timing)", b);
+                    "shear or timing)", b);
       return fail(e, SPM_ERR_UNSUPPORTED, m);
     }
+    // a network's input and output buffers serve one box at a time, and the
+    // boxes of different lanes run at once
+    for (size_t o = 0; o < b && c->closure; ++o)
+      if (e->box[o]->closure == c->closure) {
+        char m[128];
+        std::snprintf(m, sizeof m, "boxes %zu and %zu share one closure network", o, b);
+        return fail(e, SPM_ERR_UNSUPPORTED, m);
+      }
   }
   return 0;
 }

@@ -214,8 +214,10 @@ __global__ __launch_bounds__(64) void sort_segments_kernel(const int *__restrict
 // barriers between the phases (the lists equal those of the multi-kernel
 // build: each tile's members sorted ascending).  For boxes of at most
 // kSmallTiles tiles, where the five launches of the general build each cost
-// more than their work.
+// more than their work.  With more than kSmallSlots (particle, tile) pairs
+// the one block's serial share outweighs the launches saved.
 constexpr int kSmallTiles = 4096;
+constexpr long kSmallSlots = 4096;
 __global__ __launch_bounds__(1024) void tiles_small_kernel(Grid g, PModel pm, const double *__restrict__ pb,
                                                            int *pxint, double *pres, int tx, int ty, int tz, int k0,
                                                            int k1, int k2, int ntiles, int *tile_count,
@@ -1230,7 +1232,7 @@ int p_build_tiles(spm_ctx *c) {
   int k[3];
   for (int d = 0; d < 3; ++d) k[d] = std::min(c->tiles[d], (2 * c->pm.nh + 1 + 7) / 8 + 1);
   const long slots = (long)c->np * k[0] * k[1] * k[2];
-  if (c->ntiles <= kSmallTiles && slots <= (1L << 16)) {
+  if (c->ntiles <= kSmallTiles && slots <= kSmallSlots) {
     hipLaunchKernelGGL(tiles_small_kernel, dim3(1), dim3(1024), 0, c->stream, c->grid, c->pm, c->pbuf, c->pxint,
                        c->pres, c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->ntiles, c->tile_count,
                        c->tile_start, c->tile_list);

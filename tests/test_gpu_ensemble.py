@@ -151,3 +151,26 @@ def test_unsupported_boxes_rejected():
     bad = [abi.make_params(32), abi.make_params(32, device=1)]
     with pytest.raises(SpmError):
         Ensemble(bad)
+
+
+def test_box_with_closure_replays_like_lone_context():
+    """A coarse box with a network attached, replayed from the graph (a
+    one-box ensemble), equals the lone context stepped with spm_step; two
+    boxes may not share one network."""
+    from kapsel_old_amd.spm import Closure, Ensemble, SpmError
+    boxes = _small_boxes(abi.FP32)[:1]
+    net1, net2 = Closure(64, seed=5), Closure(64, seed=5)
+    ens = build(boxes)
+    ens.boxes[0].set_closure(net1)
+    lone = _lone(boxes)[0]
+    lone.set_closure(net2)
+    ens.step(40)
+    lone.step(40)
+    _same(ens.boxes[0], lone, "closure box")
+    two = build(_small_boxes(abi.FP32)[:2])
+    two.boxes[0].set_closure(net1)
+    two.boxes[1].set_closure(net1)
+    with pytest.raises(SpmError):
+        two.step(1)
+    two.boxes[0].set_closure(None)
+    two.boxes[1].set_closure(None)
