@@ -174,6 +174,108 @@ __global__ __launch_bounds__(THREADS) void mlp_fwd_kernel(Net net, const float *
   }
 }
 
+// Inference forward of 64 samples per block (no training outputs): each
+// layer's BF16 weights are staged once in LDS and shared by the four
+// wavefronts, each wavefront takes 16 samples and all output columns, so a
+// block reads the weights once per 64 samples instead of once per 16.
+// Rows of the LDS arrays are padded to 132 entries (264 bytes), so the 16
+// rows a matrix operand reads fall in different banks.  Every output is the
+// same chain of matrix instructions in the same k order as mlp_fwd_kernel's,
+// so the results are identical.
+constexpr int BTILE = 64, LDW = W + 4;
+static_assert(IN % 2 == 0, "inputs staged as pairs");
+__global__ __launch_bounds__(THREADS) void mlp_infer_kernel(Net net, const float *__restrict__ x, int n,
+                                                            float *__restrict__ y) {
+  // one activation buffer: a wavefront reads its 16 rows for a layer's
+  // products before it writes that layer's outputs over them, and no other
+  // wavefront touches them
+  __shared__ bf16_t act[BTILE * LDW];
+  __shared__ bf16_t wsh[W * LDW];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const long s0 = (long)blockIdx.x * BTILE;
+  {
+    // inputs as column pairs: thread t always takes pair t % 64, rows
+    // t / 64 + 4 i, so its shift and scale are loaded once
+    const int cp = tid % 64, c0 = 2 * cp;
+    const bool in = c0 < IN;
+    const float sh0 = in ? net.shift[c0] : 0.f, sh1 = in ? net.shift[c0 + 1] : 0.f;
+    const float sc0 = in ? net.scale[c0] : 0.f, sc1 = in ? net.scale[c0 + 1] : 0.f;
+    float2 v[BTILE / 4];
+#pragma unroll
+    for (int i = 0; i < BTILE / 4; ++i) {
+      const long s = s0 + tid / 64 + 4 * i;
+      v[i] = (in && s < n) ? *reinterpret_cast<const float2 *>(x + s * IN + c0) : float2{0.f, 0.f};
+    }
+#pragma unroll
+    for (int i = 0; i < BTILE / 4; ++i) {
+      const int r = tid / 64 + 4 * i;
+      const long s = s0 + r;
+      const bool ok = in && s < n;
+      act[r * LDW + c0] = f2bf(ok ? (v[i].x - sh0) * sc0 : 0.f);
+      act[r * LDW + c0 + 1] = f2bf(ok ? (v[i].y - sh1) * sc1 : 0.f);
+    }
+  }
+  const int row0 = 16 * wave;
+  for (int l = 0; l < 4; ++l) {
+    const int no = NOUT[l];
+    // stage W_l ([out][in], 8 bf16 per 16-byte load) after every wavefront
+    // has finished reading the previous layer's weights
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < (W * W / 8) / THREADS; ++i) {
+      const int e = tid + i * THREADS;
+      if (e < no * (W / 8)) {
+        const int o = e / (W / 8), k8 = (e % (W / 8)) * 8;
+        const uint4 v = *reinterpret_cast<const uint4 *>(net.wt[l] + o * W + k8);
+        uint2 *d = reinterpret_cast<uint2 *>(wsh + o * LDW + k8);
+        d[0] = uint2{v.x, v.y};
+        d[1] = uint2{v.z, v.w};
+      }
+    }
+    __syncthreads();
+    const float *b = net.p + OFF_B[l];
+    if (l < 3) {
+      f32x4 acc[8];
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < W / 16; ++kk) {
+        const int k = kk * 16 + 4 * (lane >> 4);
+        const bf16x4 a = ld_bf4(act + (row0 + (lane & 15)) * LDW + k);
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+          const bf16x4 bw = ld_bf4(wsh + (nt * 16 + (lane & 15)) * LDW + k);
+          acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[nt], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int nt = 0; nt < 8; ++nt) {
+        const int o = nt * 16 + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int r = row0 + 4 * (lane >> 4) + j;
+          act[r * LDW + o] = f2bf(gelu(acc[nt][j] + b[o]));
+        }
+      }
+    } else {
+      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < W / 16; ++kk) {
+        const int k = kk * 16 + 4 * (lane >> 4);
+        const bf16x4 a = ld_bf4(act + (row0 + (lane & 15)) * LDW + k);
+        const bf16x4 bw = ld_bf4(wsh + (lane & 15) * LDW + k);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
+      }
+      const int o = lane & 15;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const long s = s0 + row0 + 4 * (lane >> 4) + j;
+        if (s < n && o < OUT) y[s * OUT + o] = (acc[j] + b[o]) * net.oscale[o];
+      }
+    }
+  }
+}
+
 // Backward pass of a tile: per-tile weight and bias gradients into
 // partial[blockIdx.x][NPARAM].
 __global__ __launch_bounds__(THREADS) void mlp_bwd_kernel(Net net, Work wk, int n, float *partial) {
@@ -573,6 +675,9 @@ spm::mlp::Bf16Ptrs bf_of(spm_closure *cl) {
 
 inline unsigned tiles_of(int n) { return (unsigned)((n + spm::mlp::TILE - 1) / spm::mlp::TILE); }
 inline unsigned ftiles_of(int n) { return (unsigned)((n + spm::mlp::FTILE - 1) / spm::mlp::FTILE); }
+// batches from which inference takes mlp_infer_kernel (below it, the
+// 16-sample blocks of mlp_fwd_kernel spread a small batch over more CUs)
+constexpr int kInferMin = 2048;
 
 int refresh(spm_closure *cl) {
   hipLaunchKernelGGL(spm::mlp::bf16_refresh_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, cl->stream, cl->p,
@@ -584,6 +689,12 @@ int refresh(spm_closure *cl) {
 // forward of n samples at device x (training saves with target != null)
 int fwd(spm_closure *cl, const float *x, float *y, const float *target, int n) {
   if (n <= 0) return 0;
+  if (!target && n >= kInferMin) {  // inference of a large batch: weights shared by 64 samples
+    hipLaunchKernelGGL(spm::mlp::mlp_infer_kernel, dim3((unsigned)((n + spm::mlp::BTILE - 1) / spm::mlp::BTILE)),
+                       dim3(spm::mlp::THREADS), 0, cl->stream, net_of(cl), x, n, y);
+    CL_HIP(cl, hipGetLastError());
+    return 0;
+  }
   hipLaunchKernelGGL(spm::mlp::mlp_fwd_kernel, dim3(ftiles_of(n)), dim3(spm::mlp::THREADS), 0, cl->stream, net_of(cl),
                      x, n, y, target, work_of(cl, target != nullptr));
   CL_HIP(cl, hipGetLastError());

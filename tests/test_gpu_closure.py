@@ -48,6 +48,24 @@ def test_forward_matches_oracle(Closure, n):
     assert all(np.allclose(a, b) for a, b in zip(net.get_scaling(), (shift, scale, oscale)))
 
 
+def test_large_batch_inference_kernel_equals_small_batch_kernel(Closure):
+    """Batches from 2048 samples run the 64-sample LDS-staged kernel; its
+    outputs equal, bit for bit, those of the 16-sample kernel that smaller
+    batches run (the same matrix-instruction chain per output), and match
+    the oracle."""
+    n = 5000
+    x, _, shift, scale, oscale = _problem(n, 77)
+    p = oc.init_params(9)
+    net = Closure(n)
+    net.put_params(p)
+    net.set_scaling(shift, scale, oscale)
+    big = net.forward(x)
+    small = np.concatenate([net.forward(x[i:i + 1000]) for i in range(0, n, 1000)])
+    assert np.array_equal(big, small)
+    want, _ = oc.forward(p, x, shift, scale, oscale)
+    assert np.abs(big - want).max() / np.abs(want).max() < 1e-2
+
+
 def test_gradient_matches_oracle(Closure):
     n = 150
     x, t, shift, scale, oscale = _problem(n, 3)
