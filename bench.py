@@ -152,8 +152,9 @@ def cpu_baseline(n, np_, samples=3, shear_rate=0.0, threads=(1,)):
 
 def one_step_calls(n, np_, steps, precision=abi.FP64):
     """Steps per second when every step is its own spm_step(1) call (the
-    per-call rebuild of u, phi and up that the fused schedule does once per
-    call is then paid every step)."""
+    fused schedule's rebuild of u, phi and up is deferred to the first call
+    that reads them, so a caller stepping one step per call pays it only
+    when it reads the fields)."""
     from kapsel_old_amd.spm import SPM
     g = SPM(workload_params(n, np_, precision))
     g.put_particles(workload_particles(n, np_))

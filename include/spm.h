@@ -209,8 +209,10 @@ int spm_synchronize(spm_ctx *ctx);
 
 /* How spm_step runs a step (no counterpart in the reference):
  *   SPM_SCHEDULE_FUSED (default) -- pointwise work fused into the FFT passes;
- *     on return zeta, uk_dc, u, phi, up and the particles hold the reference's
- *     end-of-step state, the f field is scratch; power-of-two grids only (a
+ *     after spm_step, zeta, uk_dc, u, phi, up and the particles read as the
+ *     reference's end-of-step state (u, phi and up, which the next step does
+ *     not read, are formed by the first call that reads or replaces them),
+ *     the f field is scratch; power-of-two grids only (a
  *     grid with a mixed length runs LITERAL, and asking for FUSED there is
  *     SPM_ERR_UNSUPPORTED);
  *   SPM_SCHEDULE_LITERAL -- one kernel sequence per reference routine, every

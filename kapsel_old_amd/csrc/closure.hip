@@ -1033,6 +1033,7 @@ int spm_closure_features(spm_ctx *c, float *x) {
   if (!c || (!x && c->np > 0)) return SPM_ERR_INVALID;
   if (c->np == 0) return 0;
   (void)hipSetDevice(c->device);
+  if (int rs = spm::settle(c)) return rs;  // the inputs read u
   float *d = nullptr;
   SPM_HIP(c, hipMalloc((void **)&d, sizeof(float) * (size_t)c->np * spm::mlp::IN));
   int rc = spm::p_build_tiles(c);

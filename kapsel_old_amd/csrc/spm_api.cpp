@@ -838,8 +838,15 @@ int host_thermostat(spm_ctx *c) { return thermostat(c); }
 // ---------------------------------------------------------------------
 namespace spm {
 
+int settle(spm_ctx *c) {
+  if (!c->stale) return 0;
+  c->stale = false;
+  return materialize(c);
+}
+
 // phase 0: what spm_step does before its steps; 1: one step; 2: what it
-// does after them.  No host synchronisation on any phase.
+// does after them (for fused boxes: marking u, phi and up stale).  No host
+// synchronisation on any phase.
 int box_step_phase(spm_ctx *c, int phase) {
   const bool fused = c->schedule == SPM_SCHEDULE_FUSED;
   if (phase == 0) {
@@ -850,7 +857,8 @@ int box_step_phase(spm_ctx *c, int phase) {
     return 0;
   }
   if (phase == 1) return fused ? fused_step(c) : time_evolution_hydro(c);
-  return fused ? materialize(c) : 0;
+  if (fused) c->stale = true;
+  return 0;
 }
 
 bool box_uses_noise(const spm_ctx *c) {
@@ -982,6 +990,8 @@ int spm_stencil_size(const spm_ctx *ctx) { return ctx ? ctx->n_stencil : -1; }
 // Fields cross the boundary as doubles; an FP32 context converts.
 int spm_put_field(spm_ctx *c, int field, const double *host) {
   if (!c || !host) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  RC(spm::settle(c));
   double *p = field_ptr(c, field);
   if (!p) return SPM_ERR_INVALID;
   SPM_HIP(c, hipSetDevice(c->device));
@@ -998,6 +1008,8 @@ int spm_put_field(spm_ctx *c, int field, const double *host) {
 
 int spm_get_field(spm_ctx *c, int field, double *host) {
   if (!c || !host) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  RC(spm::settle(c));
   double *p = field_ptr(c, field);
   if (!p) return SPM_ERR_INVALID;
   SPM_HIP(c, hipSetDevice(c->device));
@@ -1015,6 +1027,8 @@ int spm_get_field(spm_ctx *c, int field, double *host) {
 
 int spm_put_particles(spm_ctx *c, const spm_particle *p, int n) {
   if (!c || (!p && n > 0) || n != c->np) return SPM_ERR_INVALID;
+  SPM_HIP(c, hipSetDevice(c->device));
+  RC(spm::settle(c));  // phi and up of the particles they were formed from
   const double L[3] = {c->grid.lx, c->grid.ly, c->grid.lz};
   std::vector<int> spec(n > 0 ? n : 1);
   for (int i = 0; i < n; ++i) {
@@ -1062,6 +1076,7 @@ int spm_get_particles(spm_ctx *c, spm_particle *p, int n) {
 int spm_put_uk_dc(spm_ctx *c, const double uk_dc[3]) {
   if (!c || !uk_dc) return SPM_ERR_INVALID;
   SPM_HIP(c, hipSetDevice(c->device));
+  RC(spm::settle(c));  // u's mean mode is formed from the old uk_dc
   SPM_HIP(c, hipMemcpyAsync(c->uk_dc, uk_dc, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream));
   SPM_HIP(c, hipStreamSynchronize(c->stream));
   return SPM_OK;
@@ -1104,12 +1119,23 @@ int spm_get_rng(spm_ctx *c, uint32_t mt[SPM_MT_N], int *mti) {
   return SPM_OK;
 }
 
-#define ENTRY(body)                                \
+// every entry point but spm_step first rebuilds what a fused step left
+// stale (u, phi, up), so callers see the reference's end-of-step fields
+#define ENTRY_NO_SETTLE(body)                      \
   do {                                             \
     if (!c) return SPM_ERR_INVALID;                \
     c->err.clear();                                \
     SPM_HIP(c, hipSetDevice(c->device));           \
     int rc_ = (body);                              \
+    return rc_;                                    \
+  } while (0)
+#define ENTRY(body)                                \
+  do {                                             \
+    if (!c) return SPM_ERR_INVALID;                \
+    c->err.clear();                                \
+    SPM_HIP(c, hipSetDevice(c->device));           \
+    int rc_ = spm::settle(c);                      \
+    if (!rc_) rc_ = (body);                        \
     return rc_;                                    \
   } while (0)
 
@@ -1137,6 +1163,7 @@ static int calc_f_hydro_api(spm_ctx *c) {
 
 static int step_n(spm_ctx *c, int nsteps) {
   const bool fused = c->schedule == SPM_SCHEDULE_FUSED;
+  if (!(fused && !c->electro && !c->le)) RC(spm::settle(c));
   // The fused passes carry only the modes the 2/3 rule keeps and leave zeta
   // untouched outside them; the reference's first act in a step is the same
   // truncation (Zeta_k2advection_k, operate_omega.cxx:102-108).
@@ -1160,7 +1187,8 @@ static int step_n(spm_ctx *c, int nsteps) {
       RC(fold_timing(c));
     }
   }
-  if (fused_ns && nsteps > 0) RC(materialize(c));
+  // u, phi and up are rebuilt when something reads them (settle)
+  if (fused_ns && nsteps > 0) c->stale = true;
   return 0;
 }
 
@@ -1194,7 +1222,7 @@ int spm_u2zeta_k(spm_ctx *c) { ENTRY(u2zeta_k(c)); }
 
 int spm_step(spm_ctx *c, int nsteps) {
   if (nsteps < 0) return SPM_ERR_INVALID;
-  ENTRY(step_n(c, nsteps));
+  ENTRY_NO_SETTLE(step_n(c, nsteps));
 }
 
 int spm_synchronize(spm_ctx *c) {

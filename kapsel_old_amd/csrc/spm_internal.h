@@ -124,6 +124,10 @@ struct spm_ctx {
   int ts = 0;
   double time = 0.0;
   int schedule = SPM_SCHEDULE_FUSED;
+  // u, phi and up still to be rebuilt from zeta and the particles: a fused
+  // spm_step defers that to the first entry point that reads or replaces
+  // them (settle), since the next fused step does not read them
+  bool stale = false;
 
   // fields (double path)
   double *zeta[2] = {nullptr, nullptr};
@@ -324,6 +328,8 @@ int p_noise_from_pool(spm_ctx *c);
 // step, 2 after them), the thermostat's noise pool, and the timing
 // autotuner of the mixed-radix launch shapes for the context's grid
 int box_step_phase(spm_ctx *c, int phase);
+// rebuilds u, phi and up if a fused step left them stale
+int settle(spm_ctx *c);
 bool box_uses_noise(const spm_ctx *c);
 int pool_setup(spm_ctx *c, int steps);
 int pool_fill(spm_ctx *c, int k, int slot0);

@@ -434,3 +434,24 @@ def test_config2_mid_size_steps(SPM, schedule):
     mt_o, mti_o = o.get_rng()
     mt_g, mti_g = g.get_rng()
     assert mti_o == mti_g and np.array_equal(mt_o, mt_g)
+
+
+def test_deferred_rebuild_is_invisible(SPM):
+    """A fused spm_step leaves u, phi and up to be rebuilt by the first call
+    that reads or replaces them: reading them at once, or after replacing
+    the particles and uk_dc, gives the fields of the end of the step."""
+    grid, n, a = (32, 32, 32), 4, 4.0
+    prm = abi.make_params(*grid, a=a, n_particles=n, kbt=0.05, lj_truncate=abi.LJ_WCA, seed=3, rotation=1)
+    p = particles(n, grid, a, seed=8, min_factor=2.2)
+    ga, gb = SPM(prm), SPM(prm)
+    for g in (ga, gb):
+        g.put_particles(p)
+        g.init_fluid()
+        g.step(3)
+    want = {nm: ga.get_field(nm) for nm in ("u0", "u1", "u2", "phi", "up0", "up2")}
+    moved = gb.get_particles()
+    moved["x"] = (moved["x"] + 1.5) % 32.0
+    gb.put_particles(moved)
+    gb.put_uk_dc(np.array([0.3, 0.2, 0.1]))
+    for nm, w in want.items():
+        assert np.array_equal(gb.get_field(nm), w), nm
