@@ -389,6 +389,25 @@ int spm_md_solver_position_obl(spm_ctx *ctx);
  * particle_solver.h:111-127, md_force.cxx:136-179). */
 int spm_md_solver_velocity_obl(spm_ctx *ctx);
 
+/* Mean_shear_stress(SHOW, ...) of a sheared context (sp_3d_ns.h:209-358,
+ * the Lees-Edwards DC line, printed by the reference after every step,
+ * sp_3d_ns.cxx:598-601): the stresses of the last step from the sums of its
+ * Hydro_force and Hydro_force_new fields (md_force.cxx:432-620 summed as
+ * Calc_hydro_stress does, fluid_solver.h:195-221) and the LJ pair virial
+ * of its Force_OBL (particle_solver.h:111-127).  Without particles the
+ * stresses are zero and the viscosity is ETA; with Shear_rate 0 the
+ * viscosity divides by zero as the reference's does. */
+typedef struct spm_le_stress {
+  double time;             /* jikan.time                                  */
+  double shear_rate;       /* Shear_rate (the imposed rate)               */
+  double strain;           /* Shear_strain_realized                       */
+  double lj_dev_stress;    /* dev_shear_stress_lj                         */
+  double hydro_stress;     /* hydro_stress[1][0] (Hydro_force)            */
+  double hydro_stress_new; /* hydro_stress_new[1][0] (Hydro_force_new)    */
+  double viscosity;        /* (hydro_stress_new + lj_dev_stress) / rate + ETA */
+} spm_le_stress;
+int spm_le_shear_stress(spm_ctx *ctx, spm_le_stress *out);
+
 #ifdef __cplusplus
 }
 #endif

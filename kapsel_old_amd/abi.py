@@ -192,3 +192,8 @@ def random_positions(n, box, min_dist, seed=12345, max_tries=1_000_000):
         pos[k] = cand
         k += 1
     return pos
+
+
+# spm_le_stress (include/spm.h), in order
+LE_STRESS_KEYS = ("time", "shear_rate", "strain", "lj_dev_stress", "hydro_stress", "hydro_stress_new",
+                  "viscosity")

@@ -775,7 +775,7 @@ __device__ __forceinline__ void orientation_step(const PModel &pm, double *pb, i
 
 // MD_solver_velocity_Euler / _AB2_hydro + reset_Forces.
 __global__ void velocity_kernel(PModel pm, double *pb, const int *__restrict__ spec, double dt, double hdt,
-                                int euler) {
+                                int euler, int obl = 0) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= pm.np) return;
   const int s = spec[n];
@@ -794,6 +794,10 @@ __global__ void velocity_kernel(PModel pm, double *pb, const int *__restrict__ s
       PR(pb, pm.np, P_V + d, n) = v + dmy * (2.0 * (fh + 0.0) + fr + P(pb, pm.np, P_FR_PREV + d, n));
       PR(pb, pm.np, P_OMEGA + d, n) = w + dmy_rot * 2.0 * (th + 0.0);
     }
+    // the sheared solvers keep the impulse of fr for the stress output
+    // (momentum_depend_fr, particle_solver.cxx:373,422)
+    if (obl)
+      PR(pb, pm.np, P_MOMENTUM_DEPEND_FR + d, n) = euler ? dt * fr : hdt * (fr + P(pb, pm.np, P_FR_PREV + d, n));
     // reset_Forces, particle_solver.cxx:11-34
     PR(pb, pm.np, P_FR_PREV + d, n) = fr;
     PR(pb, pm.np, P_FR + d, n) = 0.0;
@@ -1016,7 +1020,8 @@ __global__ __launch_bounds__(256) void f_hydro_obl_kernel(Grid g, PModel pm, dou
                                                           const double *__restrict__ u0,
                                                           const double *__restrict__ u1,
                                                           const double *__restrict__ u2, double deg,
-                                                          const double *__restrict__ sre_p, double dmy) {
+                                                          const double *__restrict__ sre_p, double dmy,
+                                                          double *__restrict__ spart) {
   const int lane = threadIdx.x & 63;
   const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (n >= pm.np) return;
@@ -1027,6 +1032,11 @@ __global__ __launch_bounds__(256) void f_hydro_obl_kernel(Grid g, PModel pm, dou
   const double shift = *sre_p * g.ly;
   const double rcut = pm.radius + pm.hxi;
   double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0;
+  // moments of the Hydro_force fields of md_force.cxx:432-620 (only their
+  // sums reach the output, Calc_hydro_stress, fluid_solver.h:195-221):
+  // ry is the point's y index unwrapped across the y faces,
+  // r_mesh[1] + sign L_particle[1] (md_force.cxx:498)
+  double sA = 0, sB = 0, sC = 0, sD = 0, sV = 0, sI = 0;
   for (int s = lane; s < nst; s += 64) {
     const int cell[3] = {stencil[3 * s], stencil[3 * s + 1], stencil[3 * s + 2]};
     int rm[3];
@@ -1053,6 +1063,15 @@ __global__ __launch_bounds__(256) void f_hydro_obl_kernel(Grid g, PModel pm, dou
     tx += (r[1] * f2 - r[2] * f1);
     ty += (r[2] * f0 - r[0] * f2);
     tz += (r[0] * f1 - r[1] * f0);
+    if (spart) {
+      const double ry = (double)rm[1] + sign * g.ly;
+      sA += f0 * ry;
+      sB += ry * dphi;
+      sC += ry * dphi * r[2];
+      sD += ry * dphi * r[1];
+      sV += dphi;
+      sI += dphi * (dist * dist);
+    }
   }
   fx = wave_sum(fx);
   fy = wave_sum(fy);
@@ -1060,7 +1079,21 @@ __global__ __launch_bounds__(256) void f_hydro_obl_kernel(Grid g, PModel pm, dou
   tx = wave_sum(tx);
   ty = wave_sum(ty);
   tz = wave_sum(tz);
+  if (spart) {
+    sA = wave_sum(sA);
+    sB = wave_sum(sB);
+    sC = wave_sum(sC);
+    sD = wave_sum(sD);
+    sV = wave_sum(sV);
+    sI = wave_sum(sI);
+  }
   if (lane == 0) {
+    if (spart) {  // with the raw sums and the momentum_depend_fr[0] the step began with
+      double *o = spart + (long)kStressSlots * n;
+      o[0] = sA; o[1] = sB; o[2] = sC; o[3] = sD; o[4] = sV; o[5] = sI;
+      o[6] = fx; o[7] = ty; o[8] = tz;
+      o[9] = P(pb, pm.np, P_MOMENTUM_DEPEND_FR, n);
+    }
     PR(pb, pm.np, P_F_HYDRO, n) = dmy * fx;
     PR(pb, pm.np, P_F_HYDRO + 1, n) = dmy * fy;
     PR(pb, pm.np, P_F_HYDRO + 2, n) = dmy * fz;
@@ -1132,11 +1165,12 @@ __device__ __forceinline__ double distance0_obl(const Grid &g, double deg, const
 // (from the lower index to the higher), and the lane sums are combined by a
 // fixed shuffle tree.
 __global__ __launch_bounds__(256) void force_obl_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ spec,
-                                                        double deg) {
+                                                        double deg, double *__restrict__ vir) {
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (i >= pm.np) return;
   double F[3] = {0.0, 0.0, 0.0};
+  double V = 0.0;  // the pair virial (dmy -r_x) r_y of md_force.cxx:174 over pairs (i, j > i)
   if (pm.lj_on) {
     const double xi[3] = {P(pb, pm.np, P_X, i), P(pb, pm.np, P_X + 1, i), P(pb, pm.np, P_X + 2, i)};
     for (int j = lane; j < pm.np; j += 64) {
@@ -1151,12 +1185,15 @@ __global__ __launch_bounds__(256) void force_obl_kernel(Grid g, PModel pm, doubl
         const double sgn = i < j ? 1.0 : -1.0;  // fr[n] += dmy1, fr[m] -= dmy1
 #pragma unroll
         for (int d = 0; d < 3; ++d) F[d] += sgn * (dmy * -rv[d]);
+        if (i < j) V += (dmy * -rv[0]) * rv[1];
       }
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) F[d] = wave_sum(F[d]);
+    V = wave_sum(V);
   }
   if (lane != 0) return;
+  if (vir) vir[i] = V;
 #pragma unroll
   for (int d = 0; d < 3; ++d) PR(pb, pm.np, P_FR + d, i) += F[d];
   if (pm.g_on_fluid != 0.0) PR(pb, pm.np, P_FR + pm.g_direction, i) -= pm.g_on_fluid * (pm.mass_ratio[spec[i]] - 1.0);
@@ -1432,7 +1469,8 @@ int p_calc_f_hydro_obl(spm_ctx *c, double *const ucp[3], const double *sre) {
   const double dmy = (-c->pm.dx3 * c->pm.rho) / c->prm.dt;
   tbeg(c);
   hipLaunchKernelGGL(f_hydro_obl_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
-                     c->pxint, c->pres, c->stencil, c->n_stencil, ucp[0], ucp[1], ucp[2], c->le_deg, sre, dmy);
+                     c->pxint, c->pres, c->stencil, c->n_stencil, ucp[0], ucp[1], ucp[2], c->le_deg, sre, dmy,
+                     c->le_stress);
   tend(c, SPM_K_F_HYDRO, 24.0 * (double)c->n_stencil * c->np);
   SPM_LAUNCH_CHECK(c);
   return 0;
@@ -1442,12 +1480,12 @@ int p_velocity_obl(spm_ctx *c, int euler) {
   if (c->np == 0) return 0;
   tbeg(c);
   hipLaunchKernelGGL(force_obl_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
-                     c->spec, c->le_deg);
+                     c->spec, c->le_deg, c->le_stress ? c->le_stress + (long)kStressSlots * c->np : nullptr);
   tend(c, SPM_K_PARTICLE, 24.0 * (double)c->np * c->np);
   SPM_LAUNCH_CHECK(c);
   tbeg(c);
   hipLaunchKernelGGL(velocity_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pbuf, c->spec,
-                     c->prm.dt, 0.5 * c->prm.dt, euler);
+                     c->prm.dt, 0.5 * c->prm.dt, euler, 1);
   tend(c, SPM_K_PARTICLE, 8.0 * 60 * c->np);
   SPM_LAUNCH_CHECK(c);
   return 0;

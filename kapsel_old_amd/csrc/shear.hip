@@ -446,6 +446,8 @@ int le_oblique2u(spm_ctx *c, double *const ucp[3]) {
 // of u held in up[0..2]; with particles the Cartesian velocity ucp lives in
 // f[0..2].  Without particles f is zero, so U2u_oblique and Add_f_particle
 // leave u unchanged and the particle block is skipped.
+__global__ void strain_kernel(const double *__restrict__ sre, double *strain, double dt) { *strain += *sre * dt; }
+
 int le_step(spm_ctx *c, int fix_mask, bool store_u) {
   const Grid &g = c->grid;
   RCS(le_ns_solver(c));
@@ -474,6 +476,9 @@ int le_step(spm_ctx *c, int fix_mask, bool store_u) {
   hipLaunchKernelGGL(shear_rate_final_kernel, dim3(1), dim3(kBlock), 0, c->stream, c->le_red, kRedBlocks,
                      ivolume * g.ny / (g.ny - 1), c->le_red + kRedBlocks);
   tend(c, SPM_K_REALSPACE, 32.0 * g.nx * g.nz);
+  // Shear_strain_realized += Shear_rate_eff dt (sp_3d_ns.cxx:600), on the device
+  hipLaunchKernelGGL(strain_kernel, dim3(1), dim3(1), 0, c->stream, c->le_red + kRedBlocks, c->le_strain,
+                     c->prm.dt);
   SPM_LAUNCH_CHECK(c);
   if (c->np > 0) {  // sp_3d_ns.cxx:343-373
     const double *sre = c->le_red + kRedBlocks;

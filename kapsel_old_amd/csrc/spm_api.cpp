@@ -303,7 +303,8 @@ void free_ctx(spm_ctx *c) {
   if (c->pool_counter) (void)hipFree(c->pool_counter);
   if (c->pool_host) (void)hipHostFree(c->pool_host);
   for (double *p : {c->conc[0], c->conc[1], c->conc_rhs[0], c->conc_rhs[1], c->surface_normal[0],
-                    c->surface_normal[1], c->surface_normal[2], c->f_ns1[0], c->f_ns1[1], c->red, c->le_red})
+                    c->surface_normal[1], c->surface_normal[2], c->f_ns1[0], c->f_ns1[1], c->red, c->le_red,
+                    c->le_stress, c->le_strain})
     if (p) (void)hipFree(p);
   for (int *p : {c->spec, c->stencil, c->pxint, c->tile_count, c->tile_start, c->tile_cursor, c->tile_list,
                  c->cell_count, c->cell_start, c->cell_cursor, c->cell_list, c->pcell, c->flag, c->tile_count_obl,
@@ -754,6 +755,8 @@ int set_lees_edwards(spm_ctx *c, double shear_rate) {
     return SPM_ERR_UNSUPPORTED;
   }
   RC(dalloc(c, &c->le_red, (size_t)le_red_slots()));  // zeroes Shear_rate_eff
+  RC(dalloc(c, &c->le_stress, (size_t)(kStressSlots + 1) * (c->np > 0 ? c->np : 1)));
+  RC(dalloc(c, &c->le_strain, 1));  // Shear_strain_realized = 0 (input.cxx:612)
   // tile lists of the phi gather, images across the y faces included
   RC(dalloc(c, &c->tile_count_obl, (size_t)c->ntiles));
   RC(dalloc(c, &c->tile_start_obl, (size_t)c->ntiles));
@@ -794,6 +797,51 @@ int get_oblique(spm_ctx *c, double *deg, double *sre) {
   SPM_HIP(c, hipMemcpyAsync(sre, c->le_red + le_red_slots() - 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   SPM_HIP(c, hipStreamSynchronize(c->stream));
   *deg = c->le_deg;
+  return 0;
+}
+
+// Mean_shear_stress (sp_3d_ns.h:209-358) from the moments f_hydro_obl left
+// per particle.  Per particle n (ry its points' unwrapped y, phi the
+// profile, f0 the x force density, V = sum phi, It = 2/3 sum phi |r|^2):
+//   sum Hydro_force = sum_n [rho_p sum f0 ry - (mdfr_x / V) sum ry phi]
+//                     - (sum_n sum f0) / (sum_n V / RHO) sum_n sum ry phi
+//   sum Hydro_force_new = sum_n [rho_p sum f0 ry - RHO (sum f0 / V) sum ry phi
+//                     - RHO / It (T_y sum ry phi r_z - T_z sum ry phi r_y)]
+// (md_force.cxx:497-617; the raw force and torque sums), each times
+// -Ivolume DX^3 / dt (Calc_hydro_stress); the LJ part is the pair virial
+// times Ivolume.
+int le_shear_stress(spm_ctx *c, spm_le_stress *out) {
+  const int np = c->np;
+  std::vector<double> st((size_t)(kStressSlots + 1) * (np > 0 ? np : 1), 0.0);
+  double strain = 0.0;
+  if (np > 0)
+    SPM_HIP(c, hipMemcpyAsync(st.data(), c->le_stress, sizeof(double) * st.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+  SPM_HIP(c, hipMemcpyAsync(&strain, c->le_strain, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  SPM_HIP(c, hipStreamSynchronize(c->stream));
+  const Grid &g = c->grid;
+  const double rho = c->prm.rho;
+  double hf = 0.0, hfn = 0.0, sum_force = 0.0, sum_volume = 0.0, sum_b = 0.0, vir = 0.0;
+  for (int n = 0; n < np; ++n) {
+    const double *o = &st[(size_t)kStressSlots * n];
+    const double rhop = c->pm.mass_ratio[c->spec_host[n]] * rho;
+    hf += rhop * o[0] - (o[9] / o[4]) * o[1];
+    hfn += rhop * o[0] + (-rho * o[6] / o[4]) * o[1] + (-rho / ((2.0 / 3.0) * o[5])) * (o[7] * o[2] - o[8] * o[3]);
+    sum_force += o[6];
+    sum_volume += o[4];
+    sum_b += o[1];
+    vir += st[(size_t)kStressSlots * np + n];
+  }
+  if (np > 0) hf -= sum_force / (sum_volume / rho) * sum_b;
+  const double ivol = 1.0 / (g.lx * g.ly * g.lz);
+  const double dmy = ivol * (g.dx * g.dx * g.dx) / c->prm.dt;
+  out->time = c->time;
+  out->shear_rate = c->le_rate;
+  out->strain = strain;
+  out->lj_dev_stress = vir * ivol;
+  out->hydro_stress = -hf * dmy;
+  out->hydro_stress_new = -hfn * dmy;
+  out->viscosity = (out->hydro_stress_new + out->lj_dev_stress) / c->le_rate + c->prm.eta;
   return 0;
 }
 
@@ -1337,6 +1385,10 @@ int spm_get_oblique(spm_ctx *c, double *degree_oblique, double *shear_rate_eff) 
   LE_ENTRY(get_oblique(c, degree_oblique, shear_rate_eff));
 }
 int spm_ns_solver_slaved_euler_shear_obl(spm_ctx *c) { LE_ENTRY(le_ns_solver(c)); }
+int spm_le_shear_stress(spm_ctx *c, spm_le_stress *out) {
+  if (!out) return SPM_ERR_INVALID;
+  LE_ENTRY(le_shear_stress(c, out));
+}
 int spm_zeta_k2u_k_obl(spm_ctx *c) { LE_ENTRY(le_zeta_k2u_k_api(c)); }
 int spm_u_k2zeta_k_obl(spm_ctx *c) { LE_ENTRY(le_u_k2zeta_k(c)); }
 int spm_make_phi_u_particle_obl(spm_ctx *c) { LE_ENTRY(le_make_phi_api(c)); }

@@ -102,6 +102,9 @@ struct PModel {
   int g_direction;
 };
 
+// moments per particle the sheared f_hydro writes for the stress output
+constexpr int kStressSlots = 10;
+
 // surface charge (times the elementary charge) of each particle species
 struct QTable {
   double q[SPM_MAX_SPECIES];
@@ -206,6 +209,11 @@ struct spm_ctx {
   double le_rate = 0.0, le_deg = 0.0;
   double a_r_cutoff = 0.0;  // A_R_cutoff (input.cxx:1461-1480), lj_cutoff = a_r_cutoff * lj_dia
   double *le_red = nullptr;
+  // stress output (Mean_shear_stress): per particle kStressSlots moments
+  // from the last f_hydro_obl, then one LJ pair virial per particle; the
+  // realized strain (Shear_strain_realized) on the device
+  double *le_stress = nullptr;
+  double *le_strain = nullptr;
   bool electro = false;
   spm_electrolyte el{};
   double *conc[2] = {nullptr, nullptr};

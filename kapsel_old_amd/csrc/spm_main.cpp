@@ -4,7 +4,7 @@
 // (include/spm.h, include/spm_udf.h).
 //
 //   spm -I input.udf -O output.udf -R restart.udf [-D define.udf]
-//       [--state FILE] [--resume-state FILE] [--fp32] [--schedule literal] [--quiet]
+//       [--state FILE] [--resume-state FILE] [--fp32] [--schedule literal] [--stress FILE] [--quiet]
 //
 // Values may be attached (-Iinput.udf) as the reference takes them.  The
 // time loop is the reference's: steps ts = 0..MSTEP (MSTEP = GTS * Num_snap),
@@ -16,7 +16,9 @@
 // index of the step running: the restart holds ts = (steps done) - 1 and a
 // restored context is given ts + 1.  --state writes the exact binary
 // checkpoint (spm_save_state) after the last step, --resume-state starts
-// from one (no counterpart in the reference).
+// from one (no counterpart in the reference).  Under Lees-Edwards shear
+// --stress FILE writes Mean_shear_stress's line after every step, as the
+// reference prints it to stderr (sp_3d_ns.h:209-358, sp_3d_ns.cxx:598-601).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -29,7 +31,7 @@
 namespace {
 
 struct Args {
-  std::string input, output, restart, define, state, resume_state;
+  std::string input, output, restart, define, state, resume_state, stress;
   bool fp32 = false, literal = false, quiet = false;
 };
 
@@ -66,6 +68,7 @@ Args parse(int argc, char **argv) {
     };
     if (s == "--state") long_value(a.state);
     else if (s == "--resume-state") long_value(a.resume_state);
+    else if (s == "--stress") long_value(a.stress);
     else if (s == "--fp32") a.fp32 = true;
     else if (s == "--quiet") a.quiet = true;
     else if (s == "--schedule") {
@@ -89,7 +92,7 @@ Args parse(int argc, char **argv) {
   }
   if (a.input.empty() || a.output.empty() || a.restart.empty())
     die("usage: spm -I input.udf -O output.udf -R restart.udf [-D define.udf] [--state FILE] "
-        "[--resume-state FILE] [--fp32] [--schedule fused|literal] [--quiet]");
+        "[--resume-state FILE] [--fp32] [--schedule fused|literal] [--stress FILE] [--quiet]");
   return a;
 }
 
@@ -209,6 +212,14 @@ int main(int argc, char **argv) {
     log("############################ Parameters are restored.");
     ts = r.ts + 1;
   }
+  FILE *fstress = nullptr;
+  if (!a.stress.empty()) {
+    if (!sheared) die("--stress needs Lees-Edwards input");
+    fstress = std::fopen(a.stress.c_str(), "w");
+    if (!fstress) die("cannot write " + a.stress);
+    std::fprintf(fstress, "#1:time 2:shear_rate 3:shear_strain_temporal 4:lj_dev_stress_temporal "
+                          "5:shear_stress_temporal_old 6:shear_stress_temporal_new 7:viscosity \n");
+  }
   while (ts <= mstep) {
     if (ts % gts == 0) {
       if (write_udf) {
@@ -217,10 +228,18 @@ int main(int argc, char **argv) {
       }
       log("# Step: %9d/%9d", ts, mstep);
     }
-    const int next = (ts / gts + 1) * gts < mstep + 1 ? (ts / gts + 1) * gts : mstep + 1;
+    int next = (ts / gts + 1) * gts < mstep + 1 ? (ts / gts + 1) * gts : mstep + 1;
+    if (fstress) next = ts + 1;  // a line after every step
     check(spm_step(ctx, next - ts), "spm_step", ctx);
     ts = next;
+    if (fstress) {
+      spm_le_stress st{};
+      check(spm_le_shear_stress(ctx, &st), "spm_le_shear_stress", ctx);
+      std::fprintf(fstress, "%g %g %g %g %g %g %g\n", st.time, st.shear_rate, st.strain, st.lj_dev_stress,
+                   st.hydro_stress, st.hydro_stress_new, st.viscosity);
+    }
   }
+  if (fstress) std::fclose(fstress);
 
   int ts_done = 0;
   double time = 0.0;

@@ -19,7 +19,9 @@ loop goes on from the saved ts.
 Extra flags (no counterpart in the reference): --state FILE also writes an
 exact binary checkpoint (spm_save_state) after the last step; --resume-state
 FILE starts from one instead of the input's resume data; --fp32 runs the
-FP32 path; --schedule literal selects the per-routine schedule.
+FP32 path; --schedule literal selects the per-routine schedule; --stress
+FILE (Lees-Edwards input) writes Mean_shear_stress's line after every step,
+as the reference prints it to stderr (sp_3d_ns.h:209-358).
 
 The context counts ts after each step (spm_step), the reference's jikan.ts
 is the index of the step running; so the restart holds ts = (steps done) - 1
@@ -54,8 +56,14 @@ def parse_args(argv):
     ap.add_argument("--resume-state")
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--schedule", choices=["fused", "literal"], default="fused")
+    ap.add_argument("--stress")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args(norm)
+
+
+def _g(x):
+    """printf's %g"""
+    return "%g" % x
 
 
 def initial_particles(u, prm, run, resumed):
@@ -134,14 +142,26 @@ def main(argv=None):
     elif resumed:
         ctx.step(1)  # the first loop iteration runs before the restore
         ts = restore(ctx, u, prm, n_ion, sheared, log) + 1
+    fstress = None
+    if a.stress:
+        if not sheared:
+            raise SystemExit("--stress needs Lees-Edwards input")
+        fstress = open(a.stress, "w")
+        fstress.write("#1:time 2:shear_rate 3:shear_strain_temporal 4:lj_dev_stress_temporal "
+                      "5:shear_stress_temporal_old 6:shear_stress_temporal_new 7:viscosity \n")
     while ts <= mstep:
         if ts % gts == 0:
             if write_udf:
                 udf.append_record(a.output, ts, ctx.get_particles())
             log("# Step: %9d/%9d" % (ts, mstep))
-        n = min((ts // gts + 1) * gts, mstep + 1) - ts
+        n = 1 if fstress else min((ts // gts + 1) * gts, mstep + 1) - ts
         ctx.step(n)
         ts += n
+        if fstress:
+            st = ctx.le_shear_stress()
+            fstress.write(" ".join(_g(st[k]) for k in abi.LE_STRESS_KEYS) + "\n")
+    if fstress:
+        fstress.close()
     ts_done, time = ctx.get_time()
     shape = (prm.nx, prm.ny, prm.nz)
     conc = [ctx.get_concentration(n) for n in range(n_ion)] if n_ion else None

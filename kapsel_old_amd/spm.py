@@ -37,7 +37,7 @@ SYMBOLS = [
     "spm_closure_create", "spm_closure_destroy", "spm_closure_init", "spm_closure_get_params",
     "spm_closure_put_params", "spm_closure_set_scaling", "spm_closure_get_scaling", "spm_closure_forward",
     "spm_closure_gradient", "spm_closure_set_data", "spm_closure_train", "spm_set_closure", "spm_closure_features",
-    "spm_set_particle_motion",
+    "spm_set_particle_motion", "spm_le_shear_stress",
     "spm_ensemble_create", "spm_ensemble_destroy", "spm_ensemble_box", "spm_ensemble_size", "spm_ensemble_step",
     "spm_ensemble_set_graph", "spm_ensemble_autotune", "spm_ensemble_tuning", "spm_autotune_report",
     "spm_ensemble_last_error",
@@ -129,6 +129,7 @@ def lib():
         L.spm_set_closure.argtypes = [vp, vp]
         L.spm_closure_features.argtypes = [vp, fp]
         L.spm_set_particle_motion.argtypes = [vp, ip]
+        L.spm_le_shear_stress.argtypes = [vp, dp]
         L.spm_ensemble_create.argtypes = [ip, C.POINTER(abi.SpmParams), C.POINTER(vp)]
         L.spm_ensemble_destroy.argtypes = [vp]
         L.spm_ensemble_box.argtypes = [vp, ip]
@@ -273,6 +274,14 @@ class SPM:
         self._call("put_total_solute", _dp(tt))
 
     # ---- Lees-Edwards shear (SW_EQ == Shear_Navier_Stokes_Lees_Edwards) ----
+    def le_shear_stress(self):
+        """Mean_shear_stress's Lees-Edwards line of the last step (dict:
+        time, shear_rate, strain, lj_dev_stress, hydro_stress,
+        hydro_stress_new, viscosity)."""
+        out = np.zeros(7)
+        self._call("le_shear_stress", _dp(out))
+        return dict(zip(abi.LE_STRESS_KEYS, out.tolist()))
+
     def set_lees_edwards(self, shear_rate):
         self._call("set_lees_edwards", C.c_double(shear_rate))
 

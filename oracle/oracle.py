@@ -96,6 +96,7 @@ def lib():
         L.oracle_get_oblique.argtypes = [vp, dp, dp]
         L.oracle_distance0_obl.argtypes = [vp, dp, dp, dp]
         L.oracle_distance0_obl.restype = C.c_double
+        L.oracle_le_shear_stress.argtypes = [vp, dp]
         for name in ["ns_solver_shear_obl", "zeta_k2u_k_obl", "u_k2zeta_k_obl", "make_phi_u_particle_obl",
                      "calc_f_hydro_obl", "md_solver_position_obl", "md_solver_velocity_obl", "force_obl",
                      "u2u_oblique", "u_oblique2u", "remap_obl"]:
@@ -229,6 +230,12 @@ class Oracle:
     def U2u_oblique(self): self._call("u2u_oblique")
     def U_oblique2u(self): self._call("u_oblique2u")
     def Remap_OBL(self): self._call("remap_obl")
+
+    def le_shear_stress(self):
+        """Mean_shear_stress's Lees-Edwards line as a dict."""
+        out = np.zeros(7)
+        self._call("le_shear_stress", _dp(out))
+        return dict(zip(abi.LE_STRESS_KEYS, out.tolist()))
 
     def Distance0_OBL(self, x1, x2):
         a, b = np.array(x1, dtype=np.float64), np.array(x2, dtype=np.float64)

@@ -68,6 +68,10 @@ struct oracle_ctx {
   /* Lees-Edwards shear (SW_EQ == Shear_Navier_Stokes_Lees_Edwards) */
   int le;
   double shear_rate, shear_rate_eff, degree_oblique;
+  /* Mean_shear_stress inputs: the sums over the grid of Hydro_force and
+   * Hydro_force_new, dev_shear_stress_lj before the Ivolume factor, and
+   * Shear_strain_realized */
+  double hydro_force_sum, hydro_force_new_sum, lj_shear_stress, strain;
   double *ucp[3], *wv3[3], *wv1;
 };
 
@@ -1864,15 +1868,21 @@ static void make_phi_u_obl(oracle_ctx *c) {
       }
 }
 
-/* Calc_f_hydro_correct_precision_OBL (md_force.cxx:391-470, 497-506) on the
- * Cartesian velocity `fld` (no rigid bodies; the Hydro_force stress fields
- * are output only and not formed). */
+/* Calc_f_hydro_correct_precision_OBL (md_force.cxx:391-620) on the
+ * Cartesian velocity `fld` (no rigid bodies).  The Hydro_force and
+ * Hydro_force_new fields feed only Calc_hydro_stress (fluid_solver.h:
+ * 195-221), which sums them over the grid, so their sums are kept, added
+ * point by point in the reference's three particle loops. */
 static void calc_f_hydro_obl(oracle_ctx *c, double *const fld[3]) {
   const double dmy0 = -c->DX3 * c->RHO;
   const double dmy = dmy0 / c->dt;
   const double ly = c->L[1];
+  double *volume = (double *)calloc((size_t)(c->Np > 0 ? c->Np : 1), sizeof(double));
+  double *itrace = (double *)calloc((size_t)(c->Np > 0 ? c->Np : 1), sizeof(double));
+  double hf = 0.0, hfn = 0.0, sum_force = 0.0, sum_volume = 0.0;
   for (int n = 0; n < c->Np; n++) {
     spm_particle *pn = &c->p[n];
+    const double rhop = c->P.mass_ratio[pn->spec] * c->RHO; /* RHO_particle, input.cxx:279 */
     int x_int[3], r_mesh[3];
     double residue[3], r[3], v_rot[3], dmy_fp[3];
     double force[3] = {0, 0, 0}, torque[3] = {0, 0, 0};
@@ -1892,11 +1902,53 @@ static void calc_f_hydro_obl(oracle_ctx *c, double *const fld[3]) {
       torque[0] += (r[1] * dmy_fp[2] - r[2] * dmy_fp[1]);
       torque[1] += (r[2] * dmy_fp[0] - r[0] * dmy_fp[2]);
       torque[2] += (r[0] * dmy_fp[1] - r[1] * dmy_fp[0]);
+      const double dmy_ry = (r_mesh[1] + sign * ly); /* md_force.cxx:498 */
+      hf += dmy_fp[0] * dmy_ry * rhop;
+      hfn += dmy_fp[0] * dmy_ry * rhop;
+      volume[n] += dmy_phi;
+      itrace[n] += dmy_phi * SQ(dmyR);
     }
+    itrace[n] *= 2.0 / 3.0;
     for (int d = 0; d < DIM; d++) pn->f_hydro[d] = (dmy * force[d]);
     if (c->P.rotation)
       for (int d = 0; d < DIM; d++) pn->torque_hydro[d] = (dmy * torque[d]);
+    /* md_force.cxx:527-555 */
+    for (int mesh = 0; mesh < c->np_domain; mesh++) {
+      const int sign = relative_coord_obl(c, &c->sekibun[3 * mesh], x_int, residue, r_mesh, r);
+      double dmyR = 0.;
+      for (int d = 0; d < DIM; d++) dmyR += SQ(r[d]);
+      dmyR = sqrt(dmyR);
+      const double dmy_phi = Phi(c, dmyR, c->RADIUS);
+      const double dmy_ry = (r_mesh[1] + sign * ly);
+      const double dmy_stress_p = pn->momentum_depend_fr[0] / volume[n];
+      hf -= dmy_stress_p * dmy_ry * dmy_phi;
+      const double dmy_stress_v = -c->RHO * force[0] / volume[n];
+      const double dmy_stress_w = -c->RHO * (torque[1] * r[2] - torque[2] * r[1]) / itrace[n];
+      hfn += (dmy_stress_v + dmy_stress_w) * dmy_ry * dmy_phi;
+    }
+    sum_force += force[0];
+    sum_volume += volume[n];
   }
+  sum_volume /= c->RHO;
+  /* md_force.cxx:580-620 */
+  for (int n = 0; n < c->Np; n++) {
+    int x_int[3], r_mesh[3];
+    double residue[3], r[3];
+    particle_cell(c, c->p[n].x, x_int, residue);
+    for (int mesh = 0; mesh < c->np_domain; mesh++) {
+      const int sign = relative_coord_obl(c, &c->sekibun[3 * mesh], x_int, residue, r_mesh, r);
+      double dmyR = 0.;
+      for (int d = 0; d < DIM; d++) dmyR += SQ(r[d]);
+      dmyR = sqrt(dmyR);
+      const double dmy_phi = Phi(c, dmyR, c->RADIUS);
+      const double dmy_ry = (r_mesh[1] + sign * ly);
+      hf -= sum_force * dmy_ry * dmy_phi / sum_volume;
+    }
+  }
+  c->hydro_force_sum = hf;
+  c->hydro_force_new_sum = hfn;
+  free(volume);
+  free(itrace);
 }
 
 /* PBC_OBL (periodic_boundary.h:22-44): a particle leaving through a y
@@ -1970,6 +2022,7 @@ double oracle_distance0_obl(oracle_ctx *c, const double *x1, const double *x2, d
  * Add_f_gravity. */
 int oracle_force_obl(oracle_ctx *c) {
   if (!c || !c->le) return -1;
+  c->lj_shear_stress = 0.0; /* dev_shear_stress_lj = 0, particle_solver.h:113 */
   if (c->P.lj_truncate >= 0)
     for (int n = 0; n < c->Np; n++)
       for (int m = n + 1; m < c->Np; m++) {
@@ -1982,6 +2035,7 @@ int oracle_force_obl(oracle_ctx *c) {
             c->p[n].fr[d] += dmy1;
             c->p[m].fr[d] -= dmy1;
           }
+          c->lj_shear_stress += ((dmy * -r_ij_vec[0]) * (r_ij_vec[1])); /* md_force.cxx:174 */
         }
       }
   if (c->P.g != 0.0) {
@@ -1997,7 +2051,33 @@ int oracle_force_obl(oracle_ctx *c) {
 int oracle_md_solver_velocity_obl(oracle_ctx *c) {
   if (!c || !c->le) return -1;
   oracle_force_obl(c);
+  /* the impulse of fr the stress output reads (particle_solver.cxx:373,422) */
+  for (int n = 0; n < c->Np; n++)
+    for (int d = 0; d < DIM; d++)
+      c->p[n].momentum_depend_fr[d] =
+          (c->ts == 0) ? c->dt * c->p[n].fr[d] : c->hdt * (c->p[n].fr[d] + c->p[n].fr_previous[d]);
   velocity_update(c);
+  return 0;
+}
+
+/* Mean_shear_stress(SHOW, ...) for Lees-Edwards DC (sp_3d_ns.h:209-358):
+ * time, Shear_rate, Shear_strain_realized, dev_shear_stress_lj,
+ * hydro_stress[1][0] (Hydro_force), hydro_stress_new[1][0]
+ * (Hydro_force_new), viscosity (Calc_hydro_stress, fluid_solver.h:195-221:
+ * stress = -sum * Ivolume DX^3 / dt; Force_OBL's dev stress times
+ * Ivolume). */
+int oracle_le_shear_stress(oracle_ctx *c, double out[7]) {
+  if (!c || !c->le) return -1;
+  const double Ivolume = 1. / (c->L[0] * c->L[1] * c->L[2]);
+  const double dmy = Ivolume * c->DX3 / c->dt;
+  const double dev = c->lj_shear_stress * Ivolume;
+  out[0] = c->time;
+  out[1] = c->shear_rate;
+  out[2] = c->strain;
+  out[3] = dev;
+  out[4] = -c->hydro_force_sum * dmy;
+  out[5] = -c->hydro_force_new_sum * dmy;
+  out[6] = (out[5] + dev) / c->shear_rate + c->P.eta;
   return 0;
 }
 
@@ -2118,6 +2198,7 @@ static void le_step(oracle_ctx *c) {
   solenoidal_uk_obl(c, c->u);
   contra2co_field(c, c->u);
   u_k2zeta_k_obl(c, c->u, c->zeta, c->uk_dc);
+  c->strain += c->shear_rate_eff * c->dt; /* sp_3d_ns.cxx:600 */
 }
 
 int oracle_set_lees_edwards(oracle_ctx *c, double shear_rate) {

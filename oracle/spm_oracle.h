@@ -96,6 +96,7 @@ int oracle_calc_f_hydro_obl(oracle_ctx *c);
 int oracle_md_solver_position_obl(oracle_ctx *c);
 int oracle_md_solver_velocity_obl(oracle_ctx *c);
 int oracle_force_obl(oracle_ctx *c);
+int oracle_le_shear_stress(oracle_ctx *c, double out[7]);
 int oracle_u2u_oblique(oracle_ctx *c);
 int oracle_u_oblique2u(oracle_ctx *c);
 int oracle_remap_obl(oracle_ctx *c); /* the integer-strain remap of the real-space u */

@@ -61,3 +61,21 @@ def test_program_resumes_like_python_driver(tmp_path):
     main(["-I" + full, "-O" + d + "/o_c_py.udf", "-R" + d + "/r_c_py.udf", "--resume-state", d + "/s_a.bin",
           "--quiet"])
     assert same_file(d + "/r_c_cc.udf", d + "/r_c_py.udf")
+
+
+def test_program_and_driver_write_the_same_stress_lines(tmp_path):
+    """--stress on Lees-Edwards input: Mean_shear_stress's header and one
+    line per step (MSTEP + 1 steps), the same bytes from both drivers."""
+    from kapsel_old_amd.run import main
+    from tests.test_gpu_shear import _driver_input
+    inp = _driver_input(tmp_path, 3, "lep6.udf", particles=True)
+    d = str(tmp_path)
+    run_exe("-I" + inp, "-O" + d + "/o_cc.udf", "-R" + d + "/r_cc.udf", "--stress", d + "/s_cc.txt")
+    main(["-I" + inp, "-O" + d + "/o_py.udf", "-R" + d + "/r_py.udf", "--stress", d + "/s_py.txt", "--quiet"])
+    assert same_file(d + "/s_cc.txt", d + "/s_py.txt")
+    lines = open(d + "/s_cc.txt").read().splitlines()
+    assert lines[0].startswith("#1:time 2:shear_rate 3:shear_strain_temporal")
+    assert len(lines) == 1 + 7
+    vals = [[float(x) for x in ln.split()] for ln in lines[1:]]
+    assert all(len(v) == 7 for v in vals) and vals[-1][1] == 4.0
+    assert same_file(d + "/r_cc.udf", d + "/r_py.udf")

@@ -473,3 +473,39 @@ def test_sheared_steps_are_run_to_run_identical(SPM):
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
     assert (outs[0][0] > 1.0 - 1e-12).any() and outs[0][0][:, 0, :].any()
+
+
+def test_le_shear_stress_matches_oracle(SPM):
+    """Mean_shear_stress's line (spm_le_shear_stress) step by step against
+    the oracle: particles straddling the y faces, a WCA pair across the
+    sheared face (LJ virial), thermostat off."""
+    objs, p = _le_pair(SPM, (32, 32, 32), 8, 3.0, deg=0.37, sre=0.23, rate=0.3)
+    for s in range(4):
+        for o in objs:
+            o.step(1)
+        want, got = objs[0].le_shear_stress(), objs[1].le_shear_stress()
+        print("step", s, got, flush=True)
+        for k in abi.LE_STRESS_KEYS:
+            assert abs(got[k] - want[k]) <= 1e-9 * max(abs(want[k]), 1e-12), (s, k, got[k], want[k])
+    assert abs(want["lj_dev_stress"]) > 0 and abs(want["hydro_stress_new"]) > 0
+
+
+def test_le_viscosity_of_a_dilute_sphere_64(SPM):
+    """One free sphere (a = 6, volume fraction 0.35%) in simple shear: the
+    steady suspension viscosity is 1 + c phi with c near Einstein's 2.5
+    (measured about 2.4 here)."""
+    from tests.helpers import particles
+    n, a, rate = 64, 6.0, 0.01
+    g = SPM(abi.make_params(n, a=a, n_particles=1, rotation=1, dt=0.1, fix_cell=(0, 0, 0)))
+    g.set_lees_edwards(rate)
+    g.put_particles(particles(1, (n, n, n), a, positions=np.array([[32.0, 32.0, 32.0]])))
+    g.init_fluid()
+    g.step(600)
+    vis = []
+    for _ in range(40):
+        g.step(1)
+        vis.append(g.le_shear_stress()["viscosity"])
+    phi = 4.0 / 3.0 * np.pi * a ** 3 / n ** 3
+    c = (np.mean(vis) - 1.0) / phi
+    print("viscosity %.5f coefficient %.3f" % (np.mean(vis), c), flush=True)
+    assert 2.1 < c < 2.6

@@ -475,3 +475,62 @@ def test_remap_keeps_the_cartesian_flow(grid):
         o.put_field("u%d" % d, u[d])
     o.Remap_OBL()
     assert np.abs(o.get_field("u0")[:, :, :grid[2]] - u[0][:, :, :grid[2]]).max() > 0.5
+
+
+def test_le_stress_of_a_dilute_sphere_is_einsteins():
+    """Mean_shear_stress (sp_3d_ns.h:209-358) pinned by physics: one free
+    sphere (a = 4, volume fraction 0.82%) in simple shear reaches a steady
+    suspension viscosity 1 + c phi with c near Einstein's 2.5 (the measured
+    c is about 2.3: the periodic array and the diffuse interface lower it
+    slightly).  A lost factor (DX^3, RHO, the 1/dt of Calc_hydro_stress, the
+    Ivolume) or a sign would miss the band by far.  Without particles the
+    stresses are zero and the viscosity is ETA; the strain grows by the
+    measured shear rate times dt per step."""
+    n, a, rate, dt = 32, 4.0, 0.01, 0.1
+    o = Oracle(abi.make_params(n, a=a, n_particles=1, rotation=1, dt=dt, fix_cell=(0, 0, 0)))
+    o.set_lees_edwards(rate)
+    from tests.helpers import particles
+    o.put_particles(particles(1, (n, n, n), a, positions=np.array([[16.0, 16.0, 16.0]])))
+    o.init_fluid()
+    o.step(400)
+    vis = []
+    for _ in range(40):
+        o.step(1)
+        vis.append(o.le_shear_stress()["viscosity"])
+    phi = 4.0 / 3.0 * np.pi * a ** 3 / n ** 3
+    c = (np.mean(vis) - 1.0) / phi
+    print("viscosity %.5f, coefficient %.3f (Einstein 2.5)" % (np.mean(vis), c), flush=True)
+    assert 2.0 < c < 2.6
+    st = o.le_shear_stress()
+    assert abs(st["hydro_stress"] - st["hydro_stress_new"]) < 0.02 * abs(st["hydro_stress_new"])
+    assert st["strain"] == pytest.approx(rate * 440 * dt, rel=1e-2)  # the measured rate, a little below
+    assert st["time"] == pytest.approx(440 * dt) and st["shear_rate"] == rate and st["lj_dev_stress"] == 0.0
+    f = Oracle(abi.make_params(16, dt=dt, fix_cell=(0, 0, 0)))
+    f.set_lees_edwards(rate)
+    f.step(3)
+    st = f.le_shear_stress()
+    assert st["hydro_stress"] == st["hydro_stress_new"] == 0.0 and st["viscosity"] == 1.0
+
+
+def test_le_lj_virial_is_the_pair_sum():
+    """dev_shear_stress_lj (particle_solver.h:111-127, md_force.cxx:174) of
+    two particles in WCA contact across the sheared y face: the pair term
+    (f_x)(r_y) with f the WCA force along the nearest sheared-image
+    separation, times 1/(LX LY LZ)."""
+    n, a, deg = 32, 3.0, 0.4
+    prm = abi.make_params(n, a=a, n_particles=2, rotation=1, dt=0.05, fix_cell=(0, 0, 0), lj_truncate=abi.LJ_WCA)
+    o = Oracle(prm)
+    o.set_lees_edwards(0.05)
+    o.put_oblique(deg, 0.05)
+    from tests.helpers import particles
+    x0 = np.array([10.0, 1.0, 16.0])
+    x1 = np.array([(10.0 + 2.0 + deg * n) % n, n - 4.0, 16.5])  # image below y = 0, within 2^(1/6) 2a
+    p = particles(2, (n, n, n), a, positions=np.array([x0, x1]))
+    o.put_particles(p)
+    o.Force_OBL()
+    r, x12 = o.Distance0_OBL(x0, x1)
+    sig = 2.0 * a
+    assert r < 2 ** (1 / 6) * sig
+    f = 24.0 * (2.0 * sig ** 12 / r ** 14 - sig ** 6 / r ** 8)  # WCA (epsilon 1): F(r)/r
+    want = (f * -x12[0]) * x12[1] / n ** 3
+    assert o.le_shear_stress()["lj_dev_stress"] == pytest.approx(want, rel=1e-12)
