@@ -328,6 +328,15 @@ int spm_set_electrolyte(spm_ctx *ctx, const spm_electrolyte *el);
  * distribution (Poisson_Boltzmann off): Set_uniform_ion_charge_density_salt
  * / _nosalt, then Count_solute_each.  Needs the particles. */
 int spm_init_rho_ion(spm_ctx *ctx);
+/* Init_rho_ion with Poisson_Boltzmann on (operate_electrolyte.cxx:
+ * 786-805) for salt (n_spec 2) and more than one particle: the uniform
+ * start, then Set_steadystate_ion_density (:374-412): the ions relax by
+ * Ion_diffusion_solver_Euler with the fluid at rest until the free energy
+ * of Calc_free_energy_PB changes by less than 1e-4, at most 9999 times
+ * (`iterations`, may be NULL, receives the count).  u, phi, up and f are
+ * working memory meanwhile, as in the reference.  The iterative profile of
+ * a single particle or of salt-free ions is SPM_ERR_UNSUPPORTED. */
+int spm_init_rho_ion_pb(spm_ctx *ctx, int *iterations);
 /* Concentration[species] spectrum (reference layout) in and out. */
 int spm_put_concentration(spm_ctx *ctx, int species, const double *host);
 int spm_get_concentration(spm_ctx *ctx, int species, double *host);

@@ -117,15 +117,17 @@ __global__ void divergence_kernel(Grid g, const double *__restrict__ u0, const d
 __global__ void nssolute_update_kernel(Grid g, double *z0, double *z1, const double *__restrict__ r0,
                                        const double *__restrict__ r1, double *c0, double *c1,
                                        const double *__restrict__ q0, const double *__restrict__ q1, int nspec,
-                                       double nu, double dt) {
+                                       double nu, double dt, int fluid) {
   EMode m;
   double k2;
   if (!emode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m, k2) || !m.keep) return;
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
     const long t = m.im + e;
-    z0[t] += (dt * (r0[t] + -(nu * k2 * z0[t])));
-    z1[t] += (dt * (r1[t] + -(nu * k2 * z1[t])));
+    if (fluid) {
+      z0[t] += (dt * (r0[t] + -(nu * k2 * z0[t])));
+      z1[t] += (dt * (r1[t] + -(nu * k2 * z1[t])));
+    }
     c0[t] += (dt * q0[t]);
     if (nspec > 1) c1[t] += (dt * q1[t]);
   }
@@ -208,6 +210,30 @@ __global__ void add_scaled_kernel(Grid g, double *u0, double *u1, double *u2, co
 }
 
 // a = v on every real-space point (k < NZ; the padding is left alone).
+// Charge_field_k2Coulomb_potential_k_PBC (operate_electrolyte.cxx:76-88):
+// the potential spectrum rho_k / (eps k^2), zero at k = 0.  16 B read and
+// written per mode.
+__global__ void coulomb_potential_kernel(Grid g, double *pot, double ieps) {
+  EMode m;
+  double k2;
+  if (!emode_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, m, k2)) return;
+  const double f = (k2 > 0.0 ? 1.0 / k2 : 0.0) * ieps;
+  pot[m.im] *= f;
+  pot[m.im + 1] *= f;
+}
+
+// Calc_free_energy_PB (operate_electrolyte.cxx:340-355): half the potential,
+// less the applied field's E.x (Add_external_electric_field_x, :16-43)
+__global__ void half_potential_kernel(Grid g, double *pot, double e0, double e1, double e2, int external) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.nx * g.ny * g.nz) return;
+  const int k = (int)(t % g.nz), j = (int)((t / g.nz) % g.ny), i = (int)(t / ((long)g.nz * g.ny));
+  const long im = ((long)i * g.ny + j) * g.nz_ + k;
+  double v = pot[im] * 0.5;
+  if (external) v -= ((e0 * (double)i + e1 * (double)j + e2 * (double)k) * g.dx);
+  pot[im] = v;
+}
+
 __global__ void fill_real_kernel(Grid g, double *a, double v) {
   long im;
   if (!point_of(g, (long)blockIdx.x * blockDim.x + threadIdx.x, im)) return;
@@ -235,7 +261,10 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return s;
 }
 
-// mode 0: sum |a|; 1: sum (1 - phi) a; 2: sum (1 - phi)
+// mode 0: sum |a|; 1: sum (1 - phi) a; 2: sum (1 - phi);
+// 3: sum (1 - phi) a (log a - 1) (Calc_free_energy_PB's ideal-gas term,
+// operate_electrolyte.cxx:318-330); 4: sum a phi (its electrostatic term,
+// phi then holds the potential, :356-363)
 __global__ __launch_bounds__(kBlock) void reduce_partial_kernel(Grid g, int mode, const double *__restrict__ a,
                                                                 const double *__restrict__ phi, double *part) {
   __shared__ double sh[kBlock / 64];
@@ -243,7 +272,13 @@ __global__ __launch_bounds__(kBlock) void reduce_partial_kernel(Grid g, int mode
   double v = 0.0;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < npts; t += (long)gridDim.x * blockDim.x) {
     const long im = (t / g.nz) * g.nz_ + t % g.nz;
-    v += mode == 0 ? fabs(a[im]) : (mode == 1 ? (1. - phi[im]) * a[im] : (1. - phi[im]));
+    switch (mode) {
+      case 0: v += fabs(a[im]); break;
+      case 1: v += (1. - phi[im]) * a[im]; break;
+      case 2: v += (1. - phi[im]); break;
+      case 3: v += (1. - phi[im]) * a[im] * (log(a[im]) - 1.); break;
+      default: v += a[im] * phi[im]; break;
+    }
   }
   const double s = block_sum(v, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -298,13 +333,27 @@ int e_divergence(spm_ctx *c, double *const flux[3], double *div) {
   return 0;
 }
 
-int e_nssolute_update(spm_ctx *c, double nu, double dt) {
+int e_coulomb_potential(spm_ctx *c, double *pot) {
+  hipLaunchKernelGGL(coulomb_potential_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, c->grid, pot,
+                     1.0 / c->el.dielectric);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int e_half_potential(spm_ctx *c, double *pot, const double ext[3], int external) {
+  hipLaunchKernelGGL(half_potential_kernel, dim3(nblk(npoints(c))), dim3(kBlock), 0, c->stream, c->grid, pot, ext[0],
+                     ext[1], ext[2], external);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int e_nssolute_update(spm_ctx *c, double nu, double dt, bool fluid) {
   tbeg(c);
   hipLaunchKernelGGL(nssolute_update_kernel, dim3(nblk(nmodes(c))), dim3(kBlock), 0, c->stream, c->grid,
                      c->zeta[0], c->zeta[1], c->f_ns1[0], c->f_ns1[1], c->conc[0],
                      c->el.n_spec > 1 ? c->conc[1] : c->conc[0], c->conc_rhs[0],
-                     c->el.n_spec > 1 ? c->conc_rhs[1] : c->conc_rhs[0], c->el.n_spec, nu, dt);
-  tend(c, SPM_K_SPECTRAL, (64.0 + 48.0 * c->el.n_spec) * nmodes(c));
+                     c->el.n_spec > 1 ? c->conc_rhs[1] : c->conc_rhs[0], c->el.n_spec, nu, dt, fluid ? 1 : 0);
+  tend(c, SPM_K_SPECTRAL, ((fluid ? 64.0 : 0.0) + 48.0 * c->el.n_spec) * nmodes(c));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

@@ -497,7 +497,7 @@ int setup(spm_ctx *c) {
 // electrolyte (SW_EQ == Electrolyte), literal schedule, FP64
 // ---------------------------------------------------------------------
 // result slots of c->red (after the partial sums)
-enum { SLOT_SURF = 0, SLOT_SCALE0 = 1, SLOT_VOLUME = 3, SLOT_COUNT = 4 };
+enum { SLOT_SURF = 0, SLOT_SCALE0 = 1, SLOT_VOLUME = 3, SLOT_COUNT = 4, SLOT_FE = 6 };
 
 // Make_phi_qq_particle (operate_electrolyte.cxx:179-242) into phi, surface.
 int make_phi_qq(spm_ctx *c, double *phi, double *surface) {
@@ -684,6 +684,96 @@ int init_rho_ion(spm_ctx *c) {
   // Count_solute_each (solute_rhs.h:143-156), phi and up[0] as working memory
   RC(p_make_phi(c, 0));
   for (int n = 0; n < E.n_spec; ++n) {
+    RC(k2a_out(c, c->conc[n], c->up[0]));
+    RC(e_reduce(c, 1, c->up[0], c->phi, SLOT_COUNT + n, 0, 0.0));
+    RC(e_read_slot(c, SLOT_COUNT + n, &c->total_solute[n]));
+  }
+  return 0;
+}
+
+// Calc_free_energy_PB (operate_electrolyte.cxx:302-371): the ideal-gas
+// term kBT sum (1 - phi) C (log C - 1) dx^3 over the species and the
+// electrostatic term sum rho_e (Phi / 2 - E.x) dx^3, Phi the periodic
+// Poisson potential of the charge density (phi, up[0..2] as working memory).
+int free_energy_pb(spm_ctx *c, double *F) {
+  double *charge = c->up[1], *val = c->up[2];
+  RC(p_make_phi(c, 0));
+  for (int n = 0; n < c->el.n_spec; ++n) {
+    RC(k2a_out(c, c->conc[n], val));
+    RC(e_reduce(c, 3, val, c->phi, SLOT_FE + n, 0, 0.0));
+  }
+  RC(conc_k2charge(c, charge, c->up[0], val));
+  // A2a_k_out, Charge_field_k2Coulomb_potential_k_PBC, A_k2a
+  SPM_HIP(c, hipMemcpyAsync(val, charge, sizeof(double) * c->nfield, hipMemcpyDeviceToDevice, c->stream));
+  RC(fft_r2c(c, val));
+  RC(e_coulomb_potential(c, val));
+  RC(fft_c2r(c, val));
+  double ext[3];
+  for (int d = 0; d < 3; ++d) {
+    ext[d] = c->el.e_ext[d];
+    if (c->el.ac) ext[d] *= std::sin(c->el.angular_frequency * c->time);
+  }
+  RC(e_half_potential(c, val, ext, c->el.external_field));
+  RC(e_reduce(c, 4, charge, val, SLOT_FE + 2, 0, 0.0));
+  double fe[3] = {0.0, 0.0, 0.0};
+  for (int q = 0; q < 3; ++q)
+    if (q < c->el.n_spec || q == 2) RC(e_read_slot(c, SLOT_FE + q, &fe[q]));
+  *F = (fe[0] + fe[1]) * c->prm.kbt + fe[2];  // e_reduce has applied dx^3
+  return 0;
+}
+
+// Set_steadystate_ion_density (operate_electrolyte.cxx:374-412): with the
+// fluid at rest (u = 0, zeta = 0, uk_dc = 0) Ion_diffusion_solver_Euler
+// (fluid_solver.cxx:281-304: the Nernst-Planck right-hand side, the Euler
+// update of the ion spectra on the kept modes, Rescale_solute) until the
+// free energy changes by less than TOL = 1e-4 (operate_electrolyte.h:21),
+// at most 9999 times.  The zero vorticity is held in f_ns1 (the solvent
+// right-hand side the step does not need here, rewritten after zeta is
+// read).
+int steady_ion_density(spm_ctx *c, int *iterations) {
+  double f_prev = 0.0;
+  RC(free_energy_pb(c, &f_prev));
+  double *dc0 = nullptr;
+  SPM_HIP(c, hipMalloc((void **)&dc0, 3 * sizeof(double)));
+  SPM_HIP(c, hipMemsetAsync(dc0, 0, 3 * sizeof(double), c->stream));
+  double *const zsave[2] = {c->zeta[0], c->zeta[1]};
+  double *const dcsave = c->uk_dc;
+  const double nu = c->prm.eta * (1. / c->prm.rho);
+  int m = 1, rc = 0;
+  for (; m < 10000 && !rc; ++m) {
+    rc = k_zero(c, c->f_ns1[0], c->nfield);
+    if (!rc) rc = k_zero(c, c->f_ns1[1], c->nfield);
+    c->zeta[0] = c->f_ns1[0];
+    c->zeta[1] = c->f_ns1[1];
+    c->uk_dc = dc0;
+    if (!rc) rc = rhs_nernst_planck(c);
+    if (!rc) rc = e_nssolute_update(c, nu, c->prm.dt, false);
+    c->zeta[0] = zsave[0];
+    c->zeta[1] = zsave[1];
+    c->uk_dc = dcsave;
+    if (!rc) rc = rescale_solute(c);
+    double f = 0.0;
+    if (!rc) rc = free_energy_pb(c, &f);
+    if (!rc && std::fabs(f - f_prev) < 1e-4) break;
+    f_prev = f;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(dc0);
+  if (iterations) *iterations = m < 10000 ? m : 9999;
+  return rc;
+}
+
+int init_rho_ion_pb(spm_ctx *c, int *iterations) {
+  if (c->el.n_spec != 2 || c->np < 2) {
+    c->err = "the Poisson-Boltzmann start is built for salt with more than one particle (uniform start, then "
+             "Set_steadystate_ion_density); the iterative profile of one particle or salt-free ions is not";
+    return SPM_ERR_UNSUPPORTED;
+  }
+  RC(init_rho_ion(c));  // Set_uniform_ion_charge_density_salt, Total_solute
+  RC(steady_ion_density(c, iterations));
+  // Count_solute_each (solute_rhs.h:143-156)
+  RC(p_make_phi(c, 0));
+  for (int n = 0; n < c->el.n_spec; ++n) {
     RC(k2a_out(c, c->conc[n], c->up[0]));
     RC(e_reduce(c, 1, c->up[0], c->phi, SLOT_COUNT + n, 0, 0.0));
     RC(e_read_slot(c, SLOT_COUNT + n, &c->total_solute[n]));
@@ -1333,6 +1423,7 @@ int spm_timing_read(spm_ctx *c, double ms[SPM_NUM_KERNEL_FAMILIES], double bytes
 
 int spm_set_electrolyte(spm_ctx *c, const spm_electrolyte *el) { ENTRY(set_electrolyte(c, el)); }
 int spm_init_rho_ion(spm_ctx *c) { ELECTRO_ENTRY(init_rho_ion(c)); }
+int spm_init_rho_ion_pb(spm_ctx *c, int *iterations) { ELECTRO_ENTRY(init_rho_ion_pb(c, iterations)); }
 int spm_nssolute_solver_euler(spm_ctx *c) { ELECTRO_ENTRY(nssolute_solver(c)); }
 int spm_rescale_solute(spm_ctx *c) { ELECTRO_ENTRY(rescale_solute(c)); }
 int spm_make_coulomb_force(spm_ctx *c) { ELECTRO_ENTRY(coulomb_force(c)); }

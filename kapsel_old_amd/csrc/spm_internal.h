@@ -380,7 +380,10 @@ int e_red_slots();
 int e_poisson_grad(spm_ctx *c, const double *pot, double *const grad[3], int trunc, const double cst[3]);
 int e_grad_scaled(spm_ctx *c, const double *a, double scale, double *const out[3]);
 int e_divergence(spm_ctx *c, double *const flux[3], double *div);
-int e_nssolute_update(spm_ctx *c, double nu, double dt);
+// fluid false: only the ion spectra (Ion_diffusion_solver_Euler)
+int e_nssolute_update(spm_ctx *c, double nu, double dt, bool fluid = true);
+int e_coulomb_potential(spm_ctx *c, double *pot);
+int e_half_potential(spm_ctx *c, double *pot, const double ext[3], int external);
 int e_charge_add(spm_ctx *c, double *charge, const double *conc, const double *phi, double ze);
 int e_np_flux(spm_ctx *c, double *const flux[3], const double *cx, double *const grad[3], double *const normal[3],
               double *const u[3], double onsager, double ze);

@@ -123,7 +123,8 @@ int main(int argc, char **argv) {
   check_udf(spm_udf_params(u, &prm, &el, &run), "spm_udf_params");
   const bool electro = run.equation == SPM_UDF_ELECTROLYTE;
   const bool sheared = run.equation == SPM_UDF_LEES_EDWARDS;
-  if (electro && run.poisson_boltzmann) die("INIT_profile Poisson_Boltzmann is not run; use Uniform");
+  if (electro && run.poisson_boltzmann && !(el.n_spec == 2 && prm.n_particles > 1))
+    die("INIT_profile Poisson_Boltzmann is run for salt with several particles only");
   if (a.fp32) prm.precision = SPM_FP32;
   const int gts = run.gts, mstep = run.gts * run.num_snap;
   const bool resumed = string_is(u, "resume.Calculation", "CONTINUE");
@@ -171,7 +172,13 @@ int main(int argc, char **argv) {
   check(spm_init_fluid(ctx), "spm_init_fluid", ctx);
   if (electro) {
     check(spm_set_electrolyte(ctx, &el), "spm_set_electrolyte", ctx);
-    check(spm_init_rho_ion(ctx), "spm_init_rho_ion", ctx);
+    if (run.poisson_boltzmann) {
+      int iters = 0;
+      check(spm_init_rho_ion_pb(ctx, &iters), "spm_init_rho_ion_pb", ctx);
+      log("# Poisson-Boltzmann start: %d iterations", iters);
+    } else {
+      check(spm_init_rho_ion(ctx), "spm_init_rho_ion", ctx);
+    }
   }
   if (sheared) check(spm_set_lees_edwards(ctx, run.shear_rate), "spm_set_lees_edwards", ctx);
   if (write_udf) check_udf(spm_udf_create_output(a.input.c_str(), a.output.c_str()), "spm_udf_create_output");

@@ -108,8 +108,8 @@ def main(argv=None):
     log = (lambda *m: None) if a.quiet else (lambda *m: print(*m, file=sys.stderr, flush=True))
     u = udf.UDF(a.input)
     prm, el, run = u.params()
-    if el is not None and run.poisson_boltzmann:
-        raise SystemExit("INIT_profile Poisson_Boltzmann is not run by this driver; use Uniform")
+    if el is not None and run.poisson_boltzmann and not (el.n_spec == 2 and prm.n_particles > 1):
+        raise SystemExit("INIT_profile Poisson_Boltzmann is run for salt with several particles only")
     sheared = run.equation == udf.LEES_EDWARDS
     if a.fp32:
         prm.precision = abi.FP32
@@ -129,7 +129,10 @@ def main(argv=None):
     ctx.init_fluid()
     if el is not None:
         ctx.set_electrolyte(el)
-        ctx.Init_rho_ion()
+        if run.poisson_boltzmann:
+            log("# Poisson-Boltzmann start: %d iterations" % ctx.Init_rho_ion_PB())
+        else:
+            ctx.Init_rho_ion()
     if sheared:
         ctx.set_lees_edwards(run.shear_rate)
     if write_udf:

@@ -27,7 +27,7 @@ SYMBOLS = [
     "spm_calc_f_hydro", "spm_force", "spm_md_solver_velocity", "spm_add_random_force_thermostat",
     "spm_make_phi_u_particle", "spm_make_f_particle_dt_sole", "spm_u2zeta_k", "spm_step",
     "spm_synchronize", "spm_check_finite", "spm_set_schedule", "spm_timing_enable", "spm_timing_read", "spm_fused_stage",
-    "spm_set_electrolyte", "spm_init_rho_ion", "spm_put_concentration", "spm_get_concentration",
+    "spm_set_electrolyte", "spm_init_rho_ion", "spm_init_rho_ion_pb", "spm_put_concentration", "spm_get_concentration",
     "spm_put_total_solute", "spm_get_total_solute", "spm_nssolute_solver_euler", "spm_rescale_solute",
     "spm_make_coulomb_force", "spm_save_state", "spm_load_state",
     "spm_set_lees_edwards", "spm_put_oblique", "spm_get_oblique", "spm_ns_solver_slaved_euler_shear_obl",
@@ -96,6 +96,7 @@ def lib():
         L.spm_timing_enable.argtypes = [vp, ip]
         L.spm_timing_read.argtypes = [vp, dp, dp, C.POINTER(C.c_long)]
         L.spm_set_electrolyte.argtypes = [vp, C.POINTER(abi.SpmElectrolyte)]
+        L.spm_init_rho_ion_pb.argtypes = [vp, C.POINTER(ip)]
         for name in ["spm_init_rho_ion", "spm_nssolute_solver_euler", "spm_rescale_solute",
                      "spm_make_coulomb_force"]:
             getattr(L, name).argtypes = [vp]
@@ -250,6 +251,14 @@ class SPM:
         self._call("set_electrolyte", C.byref(el))
 
     def Init_rho_ion(self): self._call("init_rho_ion")
+
+    def Init_rho_ion_PB(self):
+        """Init_rho_ion with Poisson_Boltzmann on (salt, several particles):
+        the uniform start relaxed to the steady state; returns the
+        relaxation's iteration count."""
+        it = C.c_int()
+        self._call("init_rho_ion_pb", C.byref(it))
+        return it.value
     def NSsolute_solver_Euler(self): self._call("nssolute_solver_euler")
     def Rescale_solute(self): self._call("rescale_solute")
     def Make_Coulomb_force_x_on_fluid(self): self._call("make_coulomb_force")

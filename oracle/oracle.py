@@ -79,6 +79,7 @@ def lib():
         L.oracle_gauss2.argtypes = [vp, dp]
         L.oracle_gauss2.restype = None
         L.oracle_set_electrolyte.argtypes = [vp, C.POINTER(abi.SpmElectrolyte)]
+        L.oracle_init_rho_ion_pb.argtypes = [vp, C.c_double, C.POINTER(ip)]
         for name in ["init_rho_ion", "nssolute_solver_euler", "rescale_solute", "make_coulomb_force"]:
             f = getattr(L, "oracle_" + name)
             f.argtypes = [vp]
@@ -169,6 +170,14 @@ class Oracle:
         self._call("set_electrolyte", C.byref(el))
 
     def Init_rho_ion(self): self._call("init_rho_ion")
+
+    def Init_rho_ion_PB(self, tol=1e-4):
+        """Init_rho_ion with Poisson_Boltzmann on (salt, several particles),
+        relaxed until the free energy changes by less than tol (the
+        reference's 1e-4); returns the iteration count."""
+        it = C.c_int()
+        self._call("init_rho_ion_pb", C.c_double(tol), C.byref(it))
+        return it.value
     def NSsolute_solver_Euler(self): self._call("nssolute_solver_euler")
     def Rescale_solute(self): self._call("rescale_solute")
     def Make_Coulomb_force_x_on_fluid(self): self._call("make_coulomb_force")

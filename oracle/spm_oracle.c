@@ -1429,6 +1429,95 @@ int oracle_init_rho_ion(oracle_ctx *c) {
   return 0;
 }
 
+/* Calc_free_energy_PB, operate_electrolyte.cxx:302-371 (the particles'
+ * phi and the scratch fields as the library uses them). */
+static double free_energy_pb(oracle_ctx *c) {
+  double *charge = c->up[1], *val = c->up[2];
+  double ideal = 0.0, es = 0.0;
+  oracle_make_phi_particle(c);
+  for (int n = 0; n < c->E.n_spec; n++) {
+    a_k2a_out(c, c->conc[n], val);
+    for (int i = 0; i < c->NX; i++)
+      for (int j = 0; j < c->NY; j++)
+        for (int k = 0; k < c->NZ; k++) {
+          const size_t im = IDX(c, i, j, k);
+          ideal += (1. - c->phi[im]) * val[im] * (log(val[im]) - 1.);
+        }
+  }
+  ideal *= (c->kBT * c->DX3);
+  conc_k2charge_field(c, charge, c->up[0], val);
+  memcpy(val, charge, sizeof(double) * c->nfield); /* A2a_k_out */
+  oracle_A2a_k(c, val);
+  coulomb_potential_k(c, val);
+  oracle_A_k2a(c, val);
+  double ext[3];
+  external_field_now(c, ext);
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++)
+      for (int k = 0; k < c->NZ; k++) {
+        const size_t im = IDX(c, i, j, k);
+        val[im] *= 0.5;
+        if (c->E.external_field) val[im] -= ((ext[0] * (double)i + ext[1] * (double)j + ext[2] * (double)k) * c->DX);
+        es += charge[im] * val[im];
+      }
+  es *= c->DX3;
+  return ideal + es;
+}
+
+/* Init_rho_ion with Poisson_Boltzmann on, salt, more than one particle
+ * (operate_electrolyte.cxx:786-805): the uniform start, then
+ * Set_steadystate_ion_density (:374-412): Ion_diffusion_solver_Euler
+ * (fluid_solver.cxx:281-304) with u = zeta = uk_dc = 0 until the free
+ * energy changes by less than tol (the reference's TOL = 1e-4,
+ * operate_electrolyte.h:21; tests also run it tighter), at most 9999
+ * iterations. */
+int oracle_init_rho_ion_pb(oracle_ctx *c, double tol, int *iterations) {
+  if (!c->electro || c->E.n_spec != 2 || c->Np < 2) return -1;
+  int rc = oracle_init_rho_ion(c);
+  if (rc) return rc;
+  double *z0 = (double *)calloc(c->nfield, sizeof(double));
+  double *z1 = (double *)calloc(c->nfield, sizeof(double));
+  if (!z0 || !z1) return -2;
+  double *zsave[2] = {c->zeta[0], c->zeta[1]};
+  double dcsave[3] = {c->uk_dc[0], c->uk_dc[1], c->uk_dc[2]};
+  double f_prev = free_energy_pb(c);
+  int m;
+  for (m = 1; m < 10000; m++) {
+    memset(z0, 0, sizeof(double) * c->nfield);
+    memset(z1, 0, sizeof(double) * c->nfield);
+    c->zeta[0] = z0;
+    c->zeta[1] = z1;
+    for (int d = 0; d < DIM; d++) c->uk_dc[d] = 0.0;
+    rhs_ns_nernst_planck(c);
+    int rng_buf[4][6];
+    const int (*rng)[6] = ijk_blocks(c, rng_buf);
+    for (int b = 0; b < 4; b++)
+      for (int i = rng[b][0]; i <= rng[b][1]; i++)
+        for (int j = rng[b][2]; j <= rng[b][3]; j++)
+          for (int k = rng[b][4]; k <= rng[b][5]; k++) {
+            const size_t im = IDX(c, i, j, k);
+            for (int n = 0; n < c->E.n_spec; n++) c->conc[n][im] += (c->dt * c->conc_rhs[n][im]);
+          }
+    c->zeta[0] = zsave[0];
+    c->zeta[1] = zsave[1];
+    for (int d = 0; d < DIM; d++) c->uk_dc[d] = dcsave[d];
+    oracle_rescale_solute(c);
+    const double f = free_energy_pb(c);
+    if (fabs(f - f_prev) < tol) break;
+    f_prev = f;
+  }
+  free(z0);
+  free(z1);
+  if (iterations) *iterations = m < 10000 ? m : 9999;
+  /* Count_solute_each */
+  oracle_make_phi_particle(c);
+  for (int n = 0; n < c->E.n_spec; n++) {
+    a_k2a_out(c, c->conc[n], c->up[0]);
+    c->total_solute[n] = count_single_solute(c, c->up[0], c->phi);
+  }
+  return 0;
+}
+
 int oracle_put_concentration(oracle_ctx *c, int n, const double *host) {
   if (!c->electro || n < 0 || n >= c->E.n_spec) return -1;
   memcpy(c->conc[n], host, sizeof(double) * c->nfield);

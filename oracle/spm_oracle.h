@@ -76,6 +76,7 @@ int oracle_step(oracle_ctx *c, int nsteps);
 /* Electrolyte (SW_EQ == Electrolyte), mirroring spm_set_electrolyte .. */
 int oracle_set_electrolyte(oracle_ctx *c, const spm_electrolyte *el);
 int oracle_init_rho_ion(oracle_ctx *c);
+int oracle_init_rho_ion_pb(oracle_ctx *c, double tol, int *iterations);
 int oracle_put_concentration(oracle_ctx *c, int species, const double *host);
 int oracle_get_concentration(oracle_ctx *c, int species, double *host);
 int oracle_put_total_solute(oracle_ctx *c, const double *total);

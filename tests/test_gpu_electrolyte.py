@@ -155,3 +155,45 @@ def test_electrolyte_rejects_bad_use(SPM):
     h = SPM(prm32)
     with pytest.raises(SpmError):
         h.set_electrolyte(electrolyte())
+
+
+@pytest.mark.parametrize("e_ext", [(0.0, 0.0, 0.0), (0.01, 0.0, 0.0)], ids=["no_field", "field"])
+def test_poisson_boltzmann_start_matches_oracle(SPM, e_ext):
+    """Init_rho_ion with Poisson_Boltzmann on (uniform start relaxed by
+    Set_steadystate_ion_density until the free energy settles): the same
+    iteration count and ion spectra as the oracle, then three steps."""
+    grid, n, a = (32, 32, 32), 3, 4.0
+    prm = abi.make_params(*grid, a=a, n_particles=n, rotation=1, dt=0.05, kbt=1.0, fix_cell=(0, 0, 0))
+    p = particles(n, grid, a, seed=7, vel=0.01, omega=0.002)
+    el = electrolyte(onsager=(1.0, 1.0), dielectric=1.3, e_ext=e_ext)
+    objs, its = [], []
+    for cls in (Oracle, SPM):
+        o = cls(prm)
+        o.put_particles(p)
+        o.init_fluid()
+        o.set_electrolyte(el)
+        its.append(o.Init_rho_ion_PB())
+        objs.append(o)
+    o, g = objs
+    print("iterations oracle %d gpu %d" % tuple(its), flush=True)
+    assert its[0] == its[1] and 10 < its[0] < 9999
+    for s in range(2):
+        check("conc%d" % s, g.get_concentration(s), o.get_concentration(s))
+    check("total_solute", g.get_total_solute(), o.get_total_solute())
+    for step in range(3):
+        o.step(1)
+        g.step(1)
+        for s in range(2):
+            check("step %d conc%d" % (step, s), g.get_concentration(s), o.get_concentration(s))
+        check("step %d zeta0" % step, g.get_field("zeta0"), o.get_field("zeta0"))
+
+
+def test_poisson_boltzmann_start_rejects_unbuilt_cases(SPM):
+    from kapsel_old_amd.spm import SpmError
+    prm = abi.make_params(32, a=4.0, n_particles=1, rotation=1, dt=0.05, kbt=1.0, fix_cell=(0, 0, 0))
+    g = SPM(prm)
+    g.put_particles(particles(1, (32, 32, 32), 4.0, seed=2))
+    g.init_fluid()
+    g.set_electrolyte(electrolyte())
+    with pytest.raises(SpmError):
+        g.Init_rho_ion_PB()  # one particle: the iterative profile is not built

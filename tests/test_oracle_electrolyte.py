@@ -235,3 +235,54 @@ def test_solvent_update_of_a_shear_mode_is_viscous_decay():
     for d in range(2):
         want = z_before[d] * (1.0 - 0.05 * 1.0 * k2)
         assert np.abs(o.get_field("zeta%d" % d) - want).max() < 1e-12 * max(1.0, np.abs(want).max())
+
+
+def _pb_system(tol, pb=True):
+    prm = abi.make_params(N, a=4.0, n_particles=3, rotation=1, dt=0.05, kbt=1.0, fix_cell=(0, 0, 0))
+    o = Oracle(prm)
+    p = particles(3, (N, N, N), 4.0, seed=7)
+    o.put_particles(p)
+    o.init_fluid()
+    o.set_electrolyte(abi.make_electrolyte(n_spec=2, onsager=(1.0, 1.0), dielectric=1.3, debye_length=3.0,
+                                           surface_charge=(5.0,)))
+    it = o.Init_rho_ion_PB(tol) if pb else o.Init_rho_ion()
+    return o, it
+
+
+def _boltzmann_residual(o):
+    """std over the fluid of ln c+ + e Phi / kBT and ln c- - e Phi / kBT
+    (zero at Poisson-Boltzmann equilibrium), and the std of e Phi / kBT,
+    with Phi solved by numpy from the total charge."""
+    rho, phi = charge_density(o)
+    kx, ky, kz = wavenumbers()
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    P = np.where(k2 > 0, np.fft.rfftn(rho) / np.where(k2 > 0, k2, 1.0) / 1.3, 0.0)
+    pot = np.fft.irfftn(P, s=rho.shape, axes=(0, 1, 2))
+    c = [spectrum_to_real(o.get_concentration(s)) for s in range(2)]
+    m = phi == 0.0
+    return (np.log(c[0][m]) + pot[m]).std(), (np.log(c[1][m]) - pot[m]).std(), pot[m].std()
+
+
+def test_poisson_boltzmann_start_reaches_boltzmann_equilibrium():
+    """Set_steadystate_ion_density (operate_electrolyte.cxx:374-412) pinned
+    by physics: relaxed far enough, the ions obey Boltzmann's law in the
+    Poisson potential of the total charge, c+- ~ exp(-+ e Phi / kBT), to
+    a residual of 2% of the potential's variation (the uniform start is
+    100% off); at the reference's stopping rule (free energy change below
+    1e-4) the residual is already below half the uniform start's.  The
+    relaxation keeps the total charge neutral."""
+    o_u, _ = _pb_system(0, pb=False)
+    ru = _boltzmann_residual(o_u)
+    o_ref, it_ref = _pb_system(1e-4)
+    rr = _boltzmann_residual(o_ref)
+    o_eq, it_eq = _pb_system(1e-12)
+    re = _boltzmann_residual(o_eq)
+    print("iterations %d / %d; residuals uniform %.2e, reference stop %.2e, converged %.2e (pot std %.2e)"
+          % (it_ref, it_eq, ru[0], rr[0], re[0], re[2]), flush=True)
+    assert abs(ru[0] - ru[2]) < 1e-12 * ru[2]  # uniform: the residual is the potential itself
+    assert rr[0] < 0.5 * ru[0] and rr[1] < 0.5 * ru[1]
+    assert re[0] < 0.02 * re[2] and re[1] < 0.02 * re[2]
+    assert 10 < it_ref < it_eq < 9999
+    rho, _ = charge_density(o_eq)
+    phi_qq, surf = o_eq.make_phi_qq()
+    assert abs(rho.sum()) < 1e-9 * abs(surf[:, :, :N].sum())
