@@ -14,7 +14,7 @@
 // continues bit for bit like the one that saved it.
 //
 // Layout (native byte order): the header below, then zeta0, zeta1
-// (nfield elements each, 4 or 8 bytes), the particle records
+// (NX*NY*2(NZ/2+1) elements each in the reference layout, 4 or 8 bytes), the particle records
 // (n_particles * sizeof(spm_particle)), the ion spectra (n_ion * nfield
 // elements), and an FNV-1a 64-bit hash of every byte before it.
 #include <cstdio>
@@ -100,11 +100,12 @@ extern "C" int spm_save_state(spm_ctx *c, const char *path) {
     SPM_HIP(c, hipMemcpyAsync(&h.le_sre, c->le_red + spm::le_red_slots() - 2, sizeof(double), hipMemcpyDeviceToHost,
                               c->stream));
   }
-  const size_t fb = elem_bytes(c) * c->nfield;
+  // fields in the reference layout (independent of the device row pitch)
+  const size_t fb = elem_bytes(c) * spm::ref_count(c->grid);
   std::vector<std::vector<char>> data;
   for (void *b : buffers(c)) {
     data.emplace_back(fb);
-    SPM_HIP(c, hipMemcpyAsync(data.back().data(), b, fb, hipMemcpyDeviceToHost, c->stream));
+    if (int rc = spm::rows_to_host(c, data.back().data(), b, elem_bytes(c))) return rc;
   }
   SPM_HIP(c, hipStreamSynchronize(c->stream));
   std::vector<spm_particle> p(c->np > 0 ? c->np : 1);
@@ -154,7 +155,7 @@ extern "C" int spm_load_state(spm_ctx *c, const char *path) {
     return fail(c, SPM_ERR_INVALID, "state file was saved by a context of another grid, precision, particle "
                                     "count, ion count, time step or shear");
   if (h.ts < 0 || h.mti < 0 || h.mti > SPM_MT_N + 1) return fail(c, SPM_ERR_INVALID, "corrupt state header");
-  const size_t fb = elem_bytes(c) * c->nfield;
+  const size_t fb = elem_bytes(c) * spm::ref_count(c->grid);
   std::vector<std::vector<char>> data(2 + n_ion, std::vector<char>(fb));
   std::vector<spm_particle> p(c->np > 0 ? c->np : 1);
   bool ok = get(data[0].data(), fb) && get(data[1].data(), fb) && get(p.data(), sizeof(spm_particle) * (size_t)c->np);
@@ -167,7 +168,7 @@ extern "C" int spm_load_state(spm_ctx *c, const char *path) {
   SPM_HIP(c, hipSetDevice(c->device));
   const std::vector<void *> dst = buffers(c);
   for (size_t k = 0; k < dst.size(); ++k)
-    SPM_HIP(c, hipMemcpyAsync(dst[k], data[k].data(), fb, hipMemcpyHostToDevice, c->stream));
+    if (int rc = spm::rows_to_device(c, dst[k], data[k].data(), elem_bytes(c))) return rc;
   SPM_HIP(c, hipMemcpyAsync(c->uk_dc, h.uk_dc, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream));
   if (c->le)
     SPM_HIP(c, hipMemcpyAsync(c->le_red + spm::le_red_slots() - 2, &h.le_sre, sizeof(double), hipMemcpyHostToDevice,

@@ -550,9 +550,9 @@ static int fft_many_t(spm_ctx *c, double *const *a, int n, bool fwd) {
     const int nf = (n - b) < 5 ? (n - b) : 5;
     for (int q = 0; q < nf; ++q) fp.p[q] = reinterpret_cast<T *>(a[b + q]);
     const int nrows = g.nx * g.ny;
-    const long h = g.hnz;
+    const long h = g.hp;
     // every pass reads and writes each field once: 2 * NX*NY*NZ_ elements
-    const double bytes = 2.0 * nf * (double)nrows * g.nz_ * sizeof(T);
+    const double bytes = 2.0 * nf * (double)ref_count(g) * sizeof(T);
     int rc = 0;
     if (fwd) {
       tbeg(c);
@@ -562,7 +562,7 @@ static int fft_many_t(spm_ctx *c, double *const *a, int n, bool fwd) {
       rc |= launch_lines<T, +1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
       tend(c, SPM_K_FFT_Y, bytes);
       tbeg(c);
-      rc |= launch_lines<T, +1>(g.nx, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1, c->stream);
+      rc |= launch_lines<T, +1>(g.nx, fp, nf, c->tw_x, g.ny * g.hp, (long)g.ny * h, 0, 1, c->stream);
       tend(c, SPM_K_FFT_X, bytes);
     } else {
       const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
@@ -571,7 +571,7 @@ static int fft_many_t(spm_ctx *c, double *const *a, int n, bool fwd) {
                          c->stream, fp, g.nx, g.ny, g.nz_, g.nz / 2);
       tend(c, SPM_K_SPECTRAL, 2.0 * nf * 2.0 * g.nx * g.ny * 2 * sizeof(T));
       tbeg(c);
-      rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1, c->stream);
+      rc |= launch_lines<T, -1>(g.nx, fp, nf, c->tw_x, g.ny * g.hp, (long)g.ny * h, 0, 1, c->stream);
       tend(c, SPM_K_FFT_X, bytes);
       tbeg(c);
       rc |= launch_lines<T, -1>(g.ny, fp, nf, c->tw_y, g.hnz, h, (long)g.ny * h, g.nx, c->stream);
@@ -620,7 +620,7 @@ int tune_mixed_t(spm_ctx *c) {
   fp.p[0] = reinterpret_cast<T *>(c->work[0]);
   fp.p[1] = reinterpret_cast<T *>(c->work[1]);
   const int nf = 2, nrows = g.nx * g.ny;
-  const long h = g.hnz;
+  const long h = g.hp;
   const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
   struct Pass {
     MixedKey key;
@@ -644,8 +644,8 @@ int tune_mixed_t(spm_ctx *c) {
                                                      c->stream);
                       }});
   if (!grid_pow2(g.nx))
-    passes.push_back({MixedKey{MIX_LINES, es, g.nx, g.ny * g.hnz, 1}, kLinesVariants, [&](int v) {
-                        return lines_mixed_rt<T, +1>(g.nx, v, fp, nf, c->tw_x, g.ny * g.hnz, (long)g.ny * h, 0, 1,
+    passes.push_back({MixedKey{MIX_LINES, es, g.nx, g.ny * g.hp, 1}, kLinesVariants, [&](int v) {
+                        return lines_mixed_rt<T, +1>(g.nx, v, fp, nf, c->tw_x, g.ny * g.hp, (long)g.ny * h, 0, 1,
                                                      c->stream);
                       }});
   if (passes.empty()) return 0;
@@ -724,10 +724,10 @@ static int fft_pass_t(spm_ctx *c, double *const *a, int n, int kind, int sign) {
     FieldPtrs<T> fp{};
     const int nf = (n - b) < 5 ? (n - b) : 5;
     for (int q = 0; q < nf; ++q) fp.p[q] = reinterpret_cast<T *>(a[b + q]);
-    const long h = g.hnz;
+    const long h = g.hp;
     // bytes moved with the band limits: field fractions kept in kz and ky
     const double fz = (double)g.trn_z / g.hnz, fy = (double)(2 * g.trn_y - 1) / g.ny;
-    const double fb = nf * (double)c->nfield * sizeof(T);
+    const double fb = nf * (double)ref_count(g) * sizeof(T);
     int rc;
     tbeg(c);
     if (kind == 0) {
@@ -770,8 +770,8 @@ static int fft_band_t(spm_ctx *c, double *const *a, int n, bool fwd, bool with_z
     const int nf = (n - b) < 5 ? (n - b) : 5;
     FieldPtrs<T> fp{};
     for (int q = 0; q < nf; ++q) fp.p[q] = reinterpret_cast<T *>(a[b + q]);
-    const long h = g.hnz;
-    const double bytes = 2.0 * nf * (double)g.nx * g.ny * g.nz_ * sizeof(T);
+    const long h = g.hp;
+    const double bytes = 2.0 * nf * (double)ref_count(g) * sizeof(T);
     const double fz = (double)g.trn_z / g.hnz;
     int rc = 0;
     if (fwd) {

@@ -60,7 +60,7 @@ __device__ __forceinline__ bool col_kept(const Grid &g, int j, int kz) {
 // in every thread, so a block may return on it before its first barrier.
 template <int C>
 __device__ __forceinline__ bool tile_kept(const Grid &g, int c0) {
-  const int H = g.hnz, ncol = g.ny * H;
+  const int H = g.hp, ncol = g.ny * H;
   bool any = false;
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__r
   __shared__ double2 tw[N];
   for (int t = tid; t < N; t += blockDim.x) tw[t] = tw_g[t];
   const int c = tid % C, j0 = tid / C;
-  const long H = g.hnz;
+  const long H = g.hp;
   const long es = (long)g.ny * H;
   // Tiles of C columns (j, kz) numbered as one run j*(NZ/2+1) + kz (a tile
   // starts on a 128-byte boundary).  A block takes the kept tiles of its
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
   __shared__ double2 tw[N];
   for (int t = tid; t < N; t += blockDim.x) tw[t] = tw_g[t];
   const int c = tid % C;
-  const long H = g.hnz;
+  const long H = g.hp;
   const long es = (long)g.ny * H;
   // kept tiles of the block's sequence (a column outside the 2/3 box gives
   // only discarded modes, and zeta is zero there already)
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0
   __shared__ double2 tw[N];
   for (int t = tid; t < N; t += blockDim.x) tw[t] = tw_g[t];
   const int c = tid % C;
-  const long H = g.hnz;
+  const long H = g.hp;
   const long es = (long)g.ny * H;
   // kept tiles of the block's sequence (a column outside the 2/3 box gives
   // only discarded modes, and zeta is zero there already)
@@ -681,7 +681,7 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0
   }
 
 inline double field_bytes(const spm_ctx *c) {
-  return (double)c->nfield * (c->prm.precision == SPM_FP32 ? sizeof(float) : sizeof(double));
+  return (double)ref_count(c->grid) * (c->prm.precision == SPM_FP32 ? sizeof(float) : sizeof(double));
 }
 
 template <typename T>
@@ -719,7 +719,7 @@ int fx_zeta2u_t(spm_ctx *c, int fix_mask) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
     constexpr int R = kFusedR(S), C = kFusedC(S);
-    const int ntiles = (g.ny * g.hnz + C - 1) / C;
+    const int ntiles = (g.ny * g.hp + C - 1) / C;
     constexpr int per_cu = sizeof(T) == 8 ? SPM_X_ZETA2U_BLOCKS_F64 : SPM_X_ZETA2U_BLOCKS_F32;
     hipLaunchKernelGGL((x_zeta2u_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->zeta[0]), F<T>(c->zeta[1]), F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]),
                        c->uk_dc, fix_mask, c->tw_x, ntiles);
@@ -745,7 +745,7 @@ int fx_update_t(spm_ctx *c, double nu, double dt) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
     constexpr int R = kFusedR(S), C = kFusedC(S);
-    const int ntiles = (g.ny * g.hnz + C - 1) / C;
+    const int ntiles = (g.ny * g.hp + C - 1) / C;
     constexpr int per_cu = sizeof(T) == 8 ? SPM_X_UPDATE_BLOCKS_F64 : SPM_X_UPDATE_BLOCKS_F32;
     hipLaunchKernelGGL((x_update_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->u[0]),
                        F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]), F<T>(c->zeta[0]),
@@ -759,7 +759,7 @@ int fx_project_t(spm_ctx *c) {
   const Grid &g = c->grid;
   SPM_SIZE_SWITCH(g.nx, {
     constexpr int R = kFusedR(S), C = kFusedC(S);
-    const int ntiles = (g.ny * g.hnz + C - 1) / C;
+    const int ntiles = (g.ny * g.hp + C - 1) / C;
     constexpr int per_cu = sizeof(T) == 8 ? SPM_X_PROJECT_BLOCKS_F64 : SPM_X_PROJECT_BLOCKS_F32;
     hipLaunchKernelGGL((x_project_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->f[0]),
                        F<T>(c->f[1]), F<T>(c->f[2]), F<T>(c->zeta[0]), F<T>(c->zeta[1]), c->uk_dc, c->tw_x, ntiles);
@@ -835,7 +835,7 @@ int fz_cross_obl(spm_ctx *c, double *const u[3], double *const w[3]) {
                        c->stream, g, u[0], u[1], u[2], w[0], w[1], w[2], c->tw_z, c->tw_rz, scale);
   });
   // six truncated spectra read, three written, kz < TRN_Z entries
-  tend(c, SPM_K_FFT_Z_C2R, 9.0 * c->nfield * sizeof(T) * ((double)g.trn_z / g.hnz));
+  tend(c, SPM_K_FFT_Z_C2R, 9.0 * ref_count(g) * sizeof(T) * ((double)g.trn_z / g.hnz));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

@@ -1308,7 +1308,7 @@ static int launch_make_phi(spm_ctx *c, int mode, double *o0, double *o1, double 
   }
   const double es = c->prm.precision == SPM_FP32 ? 4.0 : 8.0;
   // mode 0 writes 4 fields, mode 1 one, mode 2 reads 3 and writes 3
-  tend(c, SPM_K_MAKE_PHI, (mode == 0 ? 4.0 : mode == 1 ? 1.0 : 6.0) * es * (double)c->nfield);
+  tend(c, SPM_K_MAKE_PHI, (mode == 0 ? 4.0 : mode == 1 ? 1.0 : 6.0) * es * (double)ref_count(c->grid));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }
@@ -1399,7 +1399,7 @@ int p_make_phi_qq(spm_ctx *c, double *phi, double *surface) {
   hipLaunchKernelGGL(make_phi_qq_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
                      c->pxint, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2],
                      c->qtab, phi, surface);
-  tend(c, SPM_K_MAKE_PHI, 16.0 * c->nfield);
+  tend(c, SPM_K_MAKE_PHI, 16.0 * ref_count(c->grid));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }
@@ -1414,7 +1414,7 @@ int p_surface_normal(spm_ctx *c) {
   hipLaunchKernelGGL(surface_normal_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pxint,
                      c->pres, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2],
                      c->surface_normal[0], c->surface_normal[1], c->surface_normal[2]);
-  tend(c, SPM_K_MAKE_PHI, 24.0 * c->nfield);
+  tend(c, SPM_K_MAKE_PHI, 24.0 * ref_count(c->grid));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }
@@ -1527,7 +1527,7 @@ int p_make_phi_obl(spm_ctx *c, const double *sre) {
   hipLaunchKernelGGL(make_phi_obl_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf,
                      c->pxint, c->pres, c->tile_start_obl, c->tile_count_obl, c->tile_list_obl, c->le_deg, sre, c->phi,
                      c->up[0], c->up[1], c->up[2]);
-  tend(c, SPM_K_MAKE_PHI, 4.0 * 8.0 * (double)c->nfield);
+  tend(c, SPM_K_MAKE_PHI, 4.0 * 8.0 * (double)ref_count(c->grid));
   SPM_LAUNCH_CHECK(c);
   return 0;
 }

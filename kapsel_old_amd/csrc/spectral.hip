@@ -285,7 +285,7 @@ template <typename T>
 __global__ void finite_kernel(Grid g, const T *__restrict__ a, int spectrum, int *flag) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long nrow = (long)g.nx * g.ny;
-  const int w = spectrum ? g.nz_ : g.nz;
+  const int w = spectrum ? 2 * g.hnz : g.nz;
   if (t >= nrow * w) return;
   const T v = a[(t / w) * g.nz_ + t % w];
   if (!isfinite(v)) atomicOr(flag, 1);
@@ -436,7 +436,7 @@ int k_zero_int(spm_ctx *c, int *a, size_t n) {
 
 // Sets *c->flag to 1 if any used element of `a` is NaN or Inf.
 int k_check_finite(spm_ctx *c, const double *a, bool spectrum) {
-  const long n = (long)c->grid.nx * c->grid.ny * (spectrum ? c->grid.nz_ : c->grid.nz);
+  const long n = (long)c->grid.nx * c->grid.ny * (spectrum ? 2 * c->grid.hnz : c->grid.nz);
   SPM_BY_PRECISION(c, hipLaunchKernelGGL(finite_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                                          c->grid, F<T>(a), spectrum ? 1 : 0, c->flag));
   SPM_LAUNCH_CHECK(c);

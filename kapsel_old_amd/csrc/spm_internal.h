@@ -80,7 +80,8 @@ enum PSlot : int {
 
 // Grid-wide constants handed to kernels by value.
 struct Grid {
-  int nx, ny, nz, nz_, hnz;  // hnz = NZ/2+1
+  int nx, ny, nz, nz_, hnz;  // hnz = NZ/2+1; nz_ = row pitch in elements (>= 2 hnz)
+  int hp;                    // row pitch in complex elements (nz_ / 2)
   int trn_x, trn_y, trn_z;   // (N+2)/3, input.cxx:297
   double wave_x, wave_y, wave_z;
   double lx, ly, lz;
@@ -323,6 +324,11 @@ int k_add_f(spm_ctx *c);
 int k_zero(spm_ctx *c, double *a, size_t n);
 int k_zero_int(spm_ctx *c, int *a, size_t n);
 int k_check_finite(spm_ctx *c, const double *a, bool spectrum);
+// field copies between the reference layout on the host and the device
+// pitch (spm_api.cpp)
+inline size_t ref_count(const Grid &g) { return (size_t)g.nx * g.ny * 2 * g.hnz; }
+int rows_to_device(spm_ctx *c, void *dev, const void *host, size_t elem);
+int rows_to_host(spm_ctx *c, void *host, const void *dev, size_t elem);
 
 // particles.hip
 int p_position(spm_ctx *c, int euler);
