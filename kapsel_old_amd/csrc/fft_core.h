@@ -400,6 +400,33 @@ __device__ __forceinline__ long line_column(const LineTiles &t, int tile, int c,
 }
 
 
+// Loads through a buffer resource over a whole field: an offset past the
+// field's end reads as zero (the hardware's range check), so a load that
+// must read zero where a column or mode is not kept needs no branch around
+// it.  With no branch the compiler counts the loads in flight exactly, and
+// loads issued ahead for the next field stay in flight while the current
+// field's complete (a load guarded by `if` is drained at the join after it).
+// Fields must be under 4 GiB (the resource's byte count and offsets are 32
+// bits); the fused schedule checks that.
+struct FieldBuf {
+  __amdgpu_buffer_rsrc_t r;
+};
+template <typename T>
+__device__ __forceinline__ FieldBuf field_buf(const T *p, size_t n) {
+  return FieldBuf{__builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), (short)0, (int)(unsigned)(n * sizeof(T)),
+                                                    0x00020000)};
+}
+// the complex value at element offset `off`, or zero where `ok` is false
+template <typename T>
+__device__ __forceinline__ cplx<T> buf_load(const FieldBuf &b, long off, bool ok) {
+  const unsigned o = ok ? (unsigned)(off * (long)sizeof(T)) : 0xfffffff0u;
+  if constexpr (sizeof(T) == 8) {
+    return __builtin_bit_cast(cplx<T>, __builtin_amdgcn_raw_buffer_load_b128(b.r, (int)o, 0, 0));
+  } else {
+    return __builtin_bit_cast(cplx<T>, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)o, 0, 0));
+  }
+}
+
 template <typename T>
 struct FieldPtrs {
   T *p[5];

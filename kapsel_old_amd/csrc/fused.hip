@@ -31,6 +31,18 @@
 
 namespace spm {
 
+// Minimum waves per SIMD the compiler must fit (a VGPR cap: 4 waves of
+// 512-thread blocks is two blocks per CU at 128 VGPRs), FP64 x passes.
+#ifndef SPM_X_ZETA2U_WAVES_F64
+#define SPM_X_ZETA2U_WAVES_F64 4
+#endif
+#ifndef SPM_X_UPDATE_WAVES_F64
+#define SPM_X_UPDATE_WAVES_F64 1
+#endif
+#ifndef SPM_X_PROJECT_WAVES_F64
+#define SPM_X_PROJECT_WAVES_F64 1
+#endif
+
 namespace {
 
 template <typename T>
@@ -137,7 +149,7 @@ __device__ __forceinline__ cplx<T> zeta2u_mode(const Grid &g, const ModeK &m, cp
 // holding mode 0 sets u(0) = uk_dc, zeroing the FIX_CELL components first.
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C, bool PERSIST>
-__global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__restrict__ z0,
+__global__ __launch_bounds__(C *N / R, sizeof(T) == 8 ? SPM_X_ZETA2U_WAVES_F64 : 1) void x_zeta2u_kernel(Grid g, const T *__restrict__ z0,
                                                            const T *__restrict__ z1, T *u0, T *u1, T *u2,
                                                            double *uk_dc, int fix_mask,
                                                            const double2 *__restrict__ tw_g, int ntiles) {
@@ -152,6 +164,7 @@ __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__r
   const int c = tid % C, j0 = tid / C;
   const long H = g.hp;
   const long es = (long)g.ny * H;
+  const FieldBuf bz0 = field_buf(z0, 2 * (size_t)g.nx * es), bz1 = field_buf(z1, 2 * (size_t)g.nx * es);
   // Tiles of C columns (j, kz) numbered as one run j*(NZ/2+1) + kz (a tile
   // starts on a 128-byte boundary).  A block takes the kept tiles of its
   // sequence (x_tile_seq) in turn and loads the next one's zeta while the
@@ -169,17 +182,21 @@ __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__r
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int i = j0 + q * (N / R);
-      za[q] = zb[q] = C2{0, 0};
       const ModeK m = mode_k(g, i, j, kz);
-      if (col_ok && m.keep) {
-        const long e = 2 * (col + i * es);
-        za[q] = *reinterpret_cast<const C2 *>(z0 + e);
-        zb[q] = *reinterpret_cast<const C2 *>(z1 + e);
-      }
+      const bool ok = col_ok && m.keep;
+      const long e = 2 * (col + i * es);
+      za[q] = buf_load<T>(bz0, e, ok);
+      zb[q] = buf_load<T>(bz1, e, ok);
     }
   };
   C2 za[R], zb[R], na[R], nb[R];
   fetch(t, za, zb);
+  // uk_dc as it was on entry (the block holding mode 0 reads it before it
+  // clears the FIX_CELL components), loaded outside the tile loop so that no
+  // conditional load sits in the loop's pipeline of loads
+  double dc_in[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dc_in[d] = uk_dc[d];
   for (;;) {
     const int kn = x_next_kept<T, C>(g, k + 1, ntiles);
     const int tn = x_tile_seq<T>(kn, ntiles);
@@ -189,9 +206,9 @@ __global__ __launch_bounds__(C *N / R) void x_zeta2u_kernel(Grid g, const T *__r
     const int j = col / (int)H, kz = col - j * (int)H;
     const bool col_ok = col < es && col_kept(g, j, kz);
     const bool has0 = c0 == 0;  // the tile holding column (j, kz) = (0, 0)
-    double dc[3] = {0.0, 0.0, 0.0};
-    if (has0)
-      for (int d = 0; d < 3; ++d) dc[d] = (fix_mask & (1 << d)) ? 0.0 : uk_dc[d];
+    double dc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dc[d] = (fix_mask & (1 << d)) ? 0.0 : dc_in[d];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       T *out = d == 0 ? u0 : (d == 1 ? u1 : u2);
@@ -434,7 +451,7 @@ __global__ __launch_bounds__(ROWS *M / R) void z_cross_kernel(Grid g, T *u0, T *
 // The transformed products are not stored.
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C, bool PERSIST>
-__global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0, const T *p1, const T *p2,
+__global__ __launch_bounds__(C *N / R, sizeof(T) == 8 ? SPM_X_UPDATE_WAVES_F64 : 1) void x_update_kernel(Grid g, const T *p0, const T *p1, const T *p2,
                                                            const T *p3, const T *p4, T *z0, T *z1, double nu,
                                                            double dt, const double2 *__restrict__ tw_g, int ntiles) {
   using C2 = cplx<T>;
@@ -453,13 +470,15 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
   int t = x_tile_seq<T>(k, ntiles);
   if (t >= ntiles) return;
   const int j0 = tid / C;
-  const T *const pf[5] = {p0, p1, p2, p3, p4};
+  const size_t nel = 2 * (size_t)g.nx * es;  // elements per field
+  const FieldBuf bp[5] = {field_buf(p0, nel), field_buf(p1, nel), field_buf(p2, nel), field_buf(p3, nel),
+                          field_buf(p4, nel)};
   auto fetch = [&](int tile, int f, C2 *dst) {
     const int col = tile * C + c;
     const bool ok = col < es && col_kept(g, col / (int)H, col - (col / (int)H) * (int)H);
 #pragma unroll
     for (int q = 0; q < R; ++q)
-      dst[q] = ok ? *reinterpret_cast<const C2 *>(pf[f] + 2 * (col + (long)(j0 + q * (N / R)) * es)) : C2{0, 0};
+      dst[q] = buf_load<T>(bp[f], 2 * (col + (long)(j0 + q * (N / R)) * es), ok);
   };
   // Software pipeline over the five spectra and over tiles: the first-stage
   // points of the next spectrum (the next tile's first one after the
@@ -555,7 +574,7 @@ __global__ __launch_bounds__(C *N / R) void x_update_kernel(Grid g, const T *p0,
 // keeps the k = 0 mode (IK2(0) = 0).  So the result is the same field.
 // ---------------------------------------------------------------------
 template <typename T, int N, int R, int C, bool PERSIST>
-__global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0, const T *f1, const T *f2, T *z0,
+__global__ __launch_bounds__(C *N / R, sizeof(T) == 8 ? SPM_X_PROJECT_WAVES_F64 : 1) void x_project_kernel(Grid g, const T *f0, const T *f1, const T *f2, T *z0,
                                                             T *z1, double *uk_dc, const double2 *__restrict__ tw_g,
                                                             int ntiles) {
   using C2 = cplx<T>;
@@ -574,13 +593,14 @@ __global__ __launch_bounds__(C *N / R) void x_project_kernel(Grid g, const T *f0
   int t = x_tile_seq<T>(k, ntiles);
   if (t >= ntiles) return;
   const int j0 = tid / C;
-  const T *const ff[3] = {f0, f1, f2};
+  const size_t nel = 2 * (size_t)g.nx * es;  // elements per field
+  const FieldBuf bf[3] = {field_buf(f0, nel), field_buf(f1, nel), field_buf(f2, nel)};
   auto fetch = [&](int tile, int f, C2 *dst) {
     const int col = tile * C + c;
     const bool ok = col < es && col_kept(g, col / (int)H, col - (col / (int)H) * (int)H);
 #pragma unroll
     for (int q = 0; q < R; ++q)
-      dst[q] = ok ? *reinterpret_cast<const C2 *>(ff[f] + 2 * (col + (long)(j0 + q * (N / R)) * es)) : C2{0, 0};
+      dst[q] = buf_load<T>(bf[f], 2 * (col + (long)(j0 + q * (N / R)) * es), ok);
   };
   // software pipeline over the three components and over tiles, as in
   // x_update_kernel
@@ -692,7 +712,7 @@ inline T *F(double *p) {
 // Blocks of a persistent x pass: `per_cu` per compute unit (0: one block
 // per tile).
 #ifndef SPM_X_ZETA2U_BLOCKS_F64
-#define SPM_X_ZETA2U_BLOCKS_F64 1
+#define SPM_X_ZETA2U_BLOCKS_F64 0
 #endif
 #ifndef SPM_X_ZETA2U_BLOCKS_F32
 #define SPM_X_ZETA2U_BLOCKS_F32 0
@@ -704,7 +724,7 @@ inline T *F(double *p) {
 #define SPM_X_UPDATE_BLOCKS_F32 0
 #endif
 #ifndef SPM_X_PROJECT_BLOCKS_F64
-#define SPM_X_PROJECT_BLOCKS_F64 1
+#define SPM_X_PROJECT_BLOCKS_F64 0
 #endif
 #ifndef SPM_X_PROJECT_BLOCKS_F32
 #define SPM_X_PROJECT_BLOCKS_F32 0

@@ -74,6 +74,12 @@ bool is_pow2(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1))
 // the fused passes and the sheared solver take power-of-two grids; others
 // run the literal schedule
 bool pow2_grid(const spm_ctx *c) { return grid_pow2(c->grid.nx) && grid_pow2(c->grid.ny) && grid_pow2(c->grid.nz); }
+// The fused x passes address each field through a buffer resource, whose
+// byte counts and offsets are 32 bits: fields under 4 GiB.
+bool fused_fits(const spm_ctx *c) {
+  const size_t elem = c->prm.precision == SPM_FP32 ? sizeof(float) : sizeof(double);
+  return c->nfield * elem < ((size_t)1 << 32) - 64;
+}
 
 double *field_ptr(spm_ctx *c, int f) {
   switch (f) {
@@ -381,7 +387,7 @@ int setup(spm_ctx *c) {
   g.wave_y = PI2 / g.ly;
   g.wave_z = PI2 / g.lz;
   c->nfield = (size_t)g.nx * g.ny * g.nz_;
-  if (!pow2_grid(c)) c->schedule = SPM_SCHEDULE_LITERAL;
+  if (!pow2_grid(c) || !fused_fits(c)) c->schedule = SPM_SCHEDULE_LITERAL;
 
   PModel &m = c->pm;
   m.np = p.n_particles;
@@ -1429,8 +1435,8 @@ int spm_fused_stage(spm_ctx *c, int stage) { ENTRY(fused_stage(c, stage)); }
 
 int spm_set_schedule(spm_ctx *c, int schedule) {
   if (!c || (schedule != SPM_SCHEDULE_FUSED && schedule != SPM_SCHEDULE_LITERAL)) return SPM_ERR_INVALID;
-  if (schedule == SPM_SCHEDULE_FUSED && !pow2_grid(c)) {
-    c->err = "the fused schedule needs power-of-two grids";
+  if (schedule == SPM_SCHEDULE_FUSED && (!pow2_grid(c) || !fused_fits(c))) {
+    c->err = "the fused schedule needs power-of-two grids and fields under 4 GiB";
     return SPM_ERR_UNSUPPORTED;
   }
   c->schedule = schedule;
