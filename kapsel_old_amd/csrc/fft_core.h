@@ -432,4 +432,134 @@ struct FieldPtrs {
   T *p[5];
 };
 
+// ---------------------------------------------------------------------
+// Direct-to-LDS tile loads (FP64 complex, 16 bytes per lane).  gfx950's
+// buffer_load_dwordx4 ... lds writes lane l's 16 bytes to LDS at the wave's
+// base address (M0) + 16 l, so a load in flight holds no VGPRs: a block can
+// keep the next spectrum's whole tile in flight while it transforms the
+// current one in another LDS buffer.
+// ---------------------------------------------------------------------
+// Hides v's value from the optimizer, so that what is computed from it is
+// computed where it is used instead of being kept live from an earlier use.
+template <typename V>
+__device__ __forceinline__ V opaque(V v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+// The same, placed after every memory access before it: what is computed
+// from v starts after those accesses (a loop whose iterations would
+// otherwise be interleaved, each holding many registers, runs them in turn).
+template <typename V>
+__device__ __forceinline__ V opaque_after_memory(V v) {
+  asm volatile("" : "+v"(v) : : "memory");
+  return v;
+}
+
+using lds_void = __attribute__((address_space(3))) void;
+
+// one 16-byte load per lane into LDS at wave_base + 16 * lane; an offset
+// past the field's end reads as zero
+__device__ __forceinline__ void dma16(const FieldBuf &b, void *wave_base, unsigned byte_off) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(b.r, (lds_void *)wave_base, 16, byte_off, 0, 0, 0);
+}
+
+// Barrier for hand-offs through LDS while direct-to-LDS loads are in
+// flight: it waits for this wave's LDS reads and writes, then for every wave
+// of the block, but not for the wave's vector-memory loads (__syncthreads()
+// would wait for those too, since an LDS load writes LDS).  The compiler
+// still waits for an LDS load before the first read of its destination.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Output point of register slot q of the last Stockham stage (radix
+// N / NS_last) for a thread whose first-stage column row is j0.
+template <int N, int R>
+__device__ __forceinline__ int last_stage_point(int j0, int q) {
+  constexpr int NS = [] {
+    int ns = 1;
+    while (ns * R < N) ns *= R;
+    return ns;
+  }();
+  constexpr int RS = N / NS;
+  const int s = q / RS, r = q % RS;
+  const int j = j0 + s * (N / R);
+  return (j / NS) * NS * RS + j % NS + r * NS;
+}
+
+// One Stockham stage over a tile held in LDS (lds[point * C + column]); the
+// first stage (NS == 1) reads the points a thread's own lanes loaded, so it
+// needs no barrier before it.  After the stage's reads, every wave passes an
+// LDS barrier and the first stage then calls hook() (which may start loads
+// into a buffer the previous transform has finished reading).  The last
+// stage hands its outputs to st(point, column, q, value).
+template <typename T, int N, int R, int RS, int NS, int C, int SIGN, class HOOK, class ST>
+__device__ __forceinline__ void stockham_stage_lds(cplx<T> *lds, const double2 *__restrict__ tw, int tid,
+                                                   HOOK &hook, ST &st) {
+  using C2 = cplx<T>;
+  constexpr int NB = N / RS;
+  constexpr int PER = R / RS;
+  constexpr bool last = NS * RS == N;
+  // addresses recomputed in each stage rather than kept live in registers
+  // across the transforms of a fused pass
+  tid = opaque(tid);
+  const int c = tid % C;
+  const int j0 = tid / C;
+  C2 a[R];
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int j = j0 + s * (N / R);
+#pragma unroll
+    for (int r = 0; r < RS; ++r) a[s * RS + r] = lds[(j + r * NB) * C + c];
+    if constexpr (NS > 1) {
+      const int k = j % NS;
+      const double2 t1 = tw[k * (N / (NS * RS))];
+      C2 w[RS];
+      w[1] = C2{(T)t1.x, (T)(SIGN * t1.y)};
+#pragma unroll
+      for (int r = 2; r < RS; ++r) w[r] = (r % 2 == 0) ? cmul(w[r / 2], w[r / 2]) : cmul(w[r - 1], w[1]);
+#pragma unroll
+      for (int r = 1; r < RS; ++r) a[s * RS + r] = cmul(a[s * RS + r], w[r]);
+    }
+    dft<C2, RS, SIGN>(&a[s * RS]);
+  }
+  if constexpr (NS == 1 || !last) {
+    lds_barrier();
+    if constexpr (NS == 1) hook();
+  }
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int j = j0 + s * (N / R);
+    const int base = (j / NS) * NS * RS + j % NS;
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      if constexpr (last)
+        st(base + r * NS, c, s * RS + r, a[s * RS + r]);
+      else
+        lds[(base + r * NS) * C + c] = a[s * RS + r];
+    }
+  }
+  if constexpr (!last) lds_barrier();
+}
+
+template <typename T, int N, int R, int C, int SIGN, int NS, class HOOK, class ST>
+__device__ __forceinline__ void fft_stages_lds(cplx<T> *lds, const double2 *__restrict__ tw, int tid, HOOK &hook,
+                                               ST &st) {
+  if constexpr (NS < N) {
+    constexpr int RS = (N / NS >= R) ? R : N / NS;
+    stockham_stage_lds<T, N, R, RS, NS, C, SIGN>(lds, tw, tid, hook, st);
+    fft_stages_lds<T, N, R, C, SIGN, NS * RS>(lds, tw, tid, hook, st);
+  }
+}
+
+// Starts the direct-to-LDS load of a tile's column points: thread tid's
+// slot q is point tid / C + q N / R of column tid % C, the point its own
+// first Stockham stage reads; off[q] is that point's byte offset in the
+// field (0xfffffff0 where the column is not read: zeros).
+template <int N, int R, int C>
+__device__ __forceinline__ void dma_tile(const FieldBuf &b, cplx<double> *buf, int tid, const unsigned *off) {
+  constexpr int NT = C * N / R;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+  for (int q = 0; q < R; ++q) dma16(b, &buf[q * NT + wave * 64], off[q]);
+}
+
 }  // namespace spm

@@ -142,6 +142,19 @@ __device__ __forceinline__ cplx<T> zeta2u_mode(const Grid &g, const ModeK &m, cp
   return C2{(kx * wi1 - ky * wi0), -(kx * wr1 - ky * wr0)};
 }
 
+// Makes the values of two register arrays count as used and redefined here:
+// what is computed from them before this point stays before it, and what is
+// computed after it stays after it (a field's contribution to accumulators is
+// then computed while its transform is in LDS rather than sunk, with the
+// values it needs held in registers, to the end of the pass).
+template <int R, typename C2>
+__device__ __forceinline__ void pin_acc(C2 *acc0, C2 *acc1) {
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    asm volatile("" : "+v"(acc0[q].x), "+v"(acc0[q].y), "+v"(acc1[q].x), "+v"(acc1[q].y));
+  }
+}
+
 // ---------------------------------------------------------------------
 // x pass (inverse) with Zeta_k2u_k prologue: zeta (2 spectra) -> 3 spectra
 // transformed along x.  Modes outside the 2/3 box read as zero (the in-place
@@ -211,6 +224,9 @@ __global__ __launch_bounds__(C *N / R, sizeof(T) == 8 ? SPM_X_ZETA2U_WAVES_F64 :
     for (int d = 0; d < 3; ++d) dc[d] = (fix_mask & (1 << d)) ? 0.0 : dc_in[d];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
+      // each component's prologue is computed where its transform starts,
+      // not hoisted (with its values held in registers) to the first one
+      pin_acc<R>(za, zb);
       T *out = d == 0 ? u0 : (d == 1 ? u1 : u2);
       auto ld = [&](int i, int, int q) -> C2 {
         const ModeK m = mode_k(g, i, j, kz);
@@ -535,6 +551,7 @@ __global__ __launch_bounds__(C *N / R, sizeof(T) == 8 ? SPM_X_UPDATE_WAVES_F64 :
         acc1[q].y -= b * P.y;
       };
       fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
+      pin_acc<R>(acc0, acc1);
 #pragma unroll
       for (int q = 0; q < R; ++q) cur[q] = nxt[q];
     }
@@ -562,6 +579,120 @@ __global__ __launch_bounds__(C *N / R, sizeof(T) == 8 ? SPM_X_UPDATE_WAVES_F64 :
     if (!PERSIST || tn >= ntiles) break;
     t = tn;
     k = kn;
+  }
+}
+
+// Coefficients (a, b) of the product spectrum P_f in the two components
+// Omega_k2zeta_k keeps of -Omega (operate_omega.cxx:60-98) at mode m.
+template <typename T>
+__device__ __forceinline__ void advect_coef(const Grid &g, const ModeK &m, int f, T &a, T &b) {
+  const T k1 = (T)(m.kx * g.wave_x), k2 = (T)(m.ky * g.wave_y), k3 = (T)(m.kz * g.wave_z);
+  T cf[3];
+  if (f == 0) {
+    cf[0] = (k3 - k2) * (k3 + k2); cf[1] = k1 * k2; cf[2] = -(k3 * k1);
+  } else if (f == 1) {
+    cf[0] = -(k1 * k2); cf[1] = (k1 - k3) * (k1 + k3); cf[2] = k2 * k3;
+  } else if (f == 2) {
+    cf[0] = k3 * k1; cf[1] = -(k2 * k3); cf[2] = (k2 - k1) * (k2 + k1);
+  } else if (f == 3) {
+    cf[0] = k2 * k3; cf[1] = 0; cf[2] = -(k1 * k2);
+  } else {
+    cf[0] = 0; cf[1] = k3 * k1; cf[2] = -(k1 * k2);
+  }
+  a = m.kx != 0 ? cf[1] : (m.ky != 0 ? cf[2] : cf[0]);
+  b = m.kx != 0 ? cf[2] : (m.ky != 0 ? cf[0] : cf[1]);
+}
+
+// x_update for FP64 fields with direct-to-LDS loads: one block per tile, two
+// LDS tile buffers, the next product spectrum in flight into one buffer while
+// the current one is transformed in the other; no VGPRs hold loads in
+// flight, so two blocks fit a CU.  Same arithmetic as x_update_kernel.
+template <int N, int R, int C>
+__global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const double *p0, const double *p1,
+                                                                   const double *p2, const double *p3,
+                                                                   const double *p4, double *z0, double *z1,
+                                                                   double nu, double dt,
+                                                                   const double2 *__restrict__ tw_g) {
+  using T = double;
+  using C2 = cplx<T>;
+  __shared__ C2 bufA[N * C];
+  __shared__ C2 bufB[N * C];
+  __shared__ double2 tw[N];
+  const int tid = threadIdx.x;
+  const int t = fused_x_tile<T>();
+  if (!tile_kept<C>(g, t * C)) return;  // zeta stays zero outside the 2/3 box
+  for (int i = tid; i < N; i += blockDim.x) tw[i] = tw_g[i];
+  const int c = tid % C, j0 = tid / C;
+  const long H = g.hp;
+  const long es = (long)g.ny * H;
+  const int col = t * C + c;  // flat column j*(NZ/2+1) + kz
+  const int j = col / (int)H, kz = col - j * (int)H;
+  const bool col_ok = col < es && col_kept(g, j, kz);
+  const size_t nel = 2 * (size_t)g.nx * es;
+  const FieldBuf bp[5] = {field_buf(p0, nel), field_buf(p1, nel), field_buf(p2, nel), field_buf(p3, nel),
+                          field_buf(p4, nel)};
+  // byte offsets of the thread's first-stage points (the same in every field)
+  unsigned off[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    off[q] = opaque(col_ok ? (unsigned)(2 * (col + (long)(j0 + q * (N / R)) * es) * (long)sizeof(T)) : 0xfffffff0u);
+  dma_tile<N, R, C>(bp[0], bufA, tid, off);
+  const FieldBuf bz0 = field_buf<T>(z0, nel), bz1 = field_buf<T>(z1, nel);
+  C2 acc0[R], acc1[R], za[R], zb[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) acc0[q] = acc1[q] = C2{0, 0};
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    C2 *buf = f % 2 == 0 ? bufA : bufB;
+    C2 *oth = f % 2 == 0 ? bufB : bufA;
+    auto hook = [&]() {
+      if (f < 4) {
+        dma_tile<N, R, C>(bp[f + 1], oth, tid, off);
+      } else {
+        // zeta at the modes this thread updates, loaded while the last
+        // spectrum is transformed
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int i = last_stage_point<N, R>(j0, q);
+          const bool ok = col_ok && mode_k(g, i, j, kz).keep;
+          const long e = 2 * (col + (long)i * es);
+          za[q] = buf_load<T>(bz0, e, ok);
+          zb[q] = buf_load<T>(bz1, e, ok);
+        }
+      }
+    };
+    auto st = [&](int i, int, int q, C2 P) {
+      T a, b;
+      advect_coef<T>(g, mode_k(g, opaque(i), j, kz), f, a, b);
+      acc0[q].x -= a * P.x;
+      acc0[q].y -= a * P.y;
+      acc1[q].x -= b * P.x;
+      acc1[q].y -= b * P.y;
+      __builtin_amdgcn_sched_barrier(0);  // points in turn: fewer live registers
+    };
+    fft_stages_lds<T, N, R, C, +1, 1>(buf, tw, tid, hook, st);
+    pin_acc<R>(acc0, acc1);
+  }
+  if (!col_ok) return;
+  const T dmy0 = (T)(-nu * dt), dmy1 = (T)(1. / nu);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    // one mode after the other (each takes an exponential and a division)
+    __builtin_amdgcn_sched_barrier(0);
+    const int i = opaque_after_memory(last_stage_point<N, R>(j0, q));
+    const ModeK m = mode_k(g, i, j, kz);
+    if (!m.keep) continue;  // zeta stays zero outside the 2/3 box
+    const long e = 2 * (col + (long)i * es);
+    const T kk2 = (T)(sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz));
+    const T ik2 = kk2 > (T)0 ? (T)1 / kk2 : (T)0;
+    const T dmy = exp(dmy0 * kk2) - (T)1;
+    C2 a = za[q], b = zb[q];
+    a.x += dmy * (a.x - dmy1 * ik2 * acc0[q].x);
+    a.y += dmy * (a.y - dmy1 * ik2 * acc0[q].y);
+    b.x += dmy * (b.x - dmy1 * ik2 * acc1[q].x);
+    b.y += dmy * (b.y - dmy1 * ik2 * acc1[q].y);
+    *reinterpret_cast<C2 *>(z0 + e) = a;
+    *reinterpret_cast<C2 *>(z1 + e) = b;
   }
 }
 
@@ -651,6 +782,7 @@ __global__ __launch_bounds__(C *N / R, sizeof(T) == 8 ? SPM_X_PROJECT_WAVES_F64 
         acc1[q].y -= b * F.x;
       };
       fft_stages_io<T, N, R, C, C, +1, 1, true, true>(lds, tw, tid, ld, st);
+      pin_acc<R>(acc0, acc1);
 #pragma unroll
       for (int q = 0; q < R; ++q) cur[q] = nxt[q];
     }
@@ -671,6 +803,104 @@ __global__ __launch_bounds__(C *N / R, sizeof(T) == 8 ? SPM_X_PROJECT_WAVES_F64 
     if (!PERSIST || tn >= ntiles) break;
     t = tn;
     k = kn;
+  }
+}
+
+// Coefficients (a, b) of the component F_f of f in the two components of
+// U_k2zeta_k's curl kept for zeta at mode m: omega_s gets (ks[a] F_b -
+// ks[b] F_a), (s, a, b) cyclic, so F_f enters omega_s with (0,2) ky,
+// (0,1) -kz, (1,0) kz, (1,2) -kx, (2,1) kx, (2,0) -ky.
+template <typename T>
+__device__ __forceinline__ void project_coef(const Grid &g, const ModeK &m, int f, T &a, T &b) {
+  const T kx = (T)(m.kx * g.wave_x), ky = (T)(m.ky * g.wave_y), kz = (T)(m.kz * g.wave_z);
+  auto coef = [&](int sidx) -> T {
+    if (sidx == 0) return f == 2 ? ky : (f == 1 ? -kz : (T)0);
+    if (sidx == 1) return f == 0 ? kz : (f == 2 ? -kx : (T)0);
+    return f == 1 ? kx : (f == 0 ? -ky : (T)0);
+  };
+  const int s0 = m.kx != 0 ? 1 : (m.ky != 0 ? 2 : 0);
+  const int s1 = m.kx != 0 ? 2 : (m.ky != 0 ? 0 : 1);
+  a = coef(s0);
+  b = coef(s1);
+}
+
+// x_project for FP64 fields with direct-to-LDS loads (see
+// x_update_dma_kernel): the three components of f through two LDS buffers,
+// one block per tile.  Same arithmetic as x_project_kernel.
+template <int N, int R, int C>
+__global__ __launch_bounds__(C *N / R, 4) void x_project_dma_kernel(Grid g, const double *f0, const double *f1,
+                                                                    const double *f2, double *z0, double *z1,
+                                                                    double *uk_dc,
+                                                                    const double2 *__restrict__ tw_g) {
+  using T = double;
+  using C2 = cplx<T>;
+  __shared__ C2 bufA[N * C];
+  __shared__ C2 bufB[N * C];
+  __shared__ double2 tw[N];
+  const int tid = threadIdx.x;
+  const int t = fused_x_tile<T>();
+  if (!tile_kept<C>(g, t * C)) return;  // zeta stays zero outside the 2/3 box
+  for (int i = tid; i < N; i += blockDim.x) tw[i] = tw_g[i];
+  const int c = tid % C, j0 = tid / C;
+  const long H = g.hp;
+  const long es = (long)g.ny * H;
+  const int c0 = t * C;
+  const int col = c0 + c;  // flat column j*(NZ/2+1) + kz
+  const int j = col / (int)H, kz = col - j * (int)H;
+  const bool col_ok = col < es && col_kept(g, j, kz);
+  const size_t nel = 2 * (size_t)g.nx * es;
+  const FieldBuf bf[3] = {field_buf(f0, nel), field_buf(f1, nel), field_buf(f2, nel)};
+  unsigned off[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    off[q] = opaque(col_ok ? (unsigned)(2 * (col + (long)(j0 + q * (N / R)) * es) * (long)sizeof(T)) : 0xfffffff0u);
+  dma_tile<N, R, C>(bf[0], bufA, tid, off);
+  const FieldBuf bz0 = field_buf<T>(z0, nel), bz1 = field_buf<T>(z1, nel);
+  C2 acc0[R], acc1[R], za[R], zb[R];
+  double f00[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < R; ++q) acc0[q] = acc1[q] = C2{0, 0};
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    C2 *buf = f % 2 == 0 ? bufA : bufB;
+    C2 *oth = f % 2 == 0 ? bufB : bufA;
+    auto hook = [&]() {
+      if (f < 2) {
+        dma_tile<N, R, C>(bf[f + 1], oth, tid, off);
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int i = last_stage_point<N, R>(j0, q);
+          const bool ok = col_ok && mode_k(g, i, j, kz).keep;
+          const long e = 2 * (col + (long)i * es);
+          za[q] = buf_load<T>(bz0, e, ok);
+          zb[q] = buf_load<T>(bz1, e, ok);
+        }
+      }
+    };
+    auto st = [&](int i, int, int q, C2 F) {
+      // slot 0 of thread 0 is mode 0 of the tile's first column
+      if (q == 0) f00[f] = F.x;
+      T a, b;  // not used where the mode is discarded
+      project_coef<T>(g, mode_k(g, opaque(i), j, kz), f, a, b);
+      acc0[q].x += a * F.y;
+      acc0[q].y -= a * F.x;
+      acc1[q].x += b * F.y;
+      acc1[q].y -= b * F.x;
+    };
+    fft_stages_lds<T, N, R, C, +1, 1>(buf, tw, tid, hook, st);
+    pin_acc<R>(acc0, acc1);
+  }
+  if (!col_ok) return;
+  if (c0 == 0 && tid == 0)  // mode (0, 0, 0)
+    for (int d = 0; d < 3; ++d) uk_dc[d] += f00[d];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = last_stage_point<N, R>(j0, q);
+    if (!mode_k(g, i, j, kz).keep) continue;  // zeta stays zero outside the 2/3 box
+    const long e = 2 * (col + (long)i * es);
+    *reinterpret_cast<C2 *>(z0 + e) = cadd(za[q], acc0[q]);
+    *reinterpret_cast<C2 *>(z1 + e) = cadd(zb[q], acc1[q]);
   }
 }
 
@@ -729,6 +959,11 @@ inline T *F(double *p) {
 #ifndef SPM_X_PROJECT_BLOCKS_F32
 #define SPM_X_PROJECT_BLOCKS_F32 0
 #endif
+// FP64 x_update / x_project through direct-to-LDS loads (0: the register-
+// prefetch kernels)
+#ifndef SPM_X_DMA_F64
+#define SPM_X_DMA_F64 1
+#endif
 inline int x_grid(const spm_ctx *c, int ntiles, int per_cu) {
   const int n = per_cu > 0 ? per_cu * c->n_cu : ntiles;
   return n < ntiles ? n : ntiles;
@@ -767,6 +1002,12 @@ int fx_update_t(spm_ctx *c, double nu, double dt) {
     constexpr int R = kFusedR(S), C = kFusedC(S);
     const int ntiles = (g.ny * g.hp + C - 1) / C;
     constexpr int per_cu = sizeof(T) == 8 ? SPM_X_UPDATE_BLOCKS_F64 : SPM_X_UPDATE_BLOCKS_F32;
+    if constexpr (sizeof(T) == 8 && SPM_X_DMA_F64) {
+      hipLaunchKernelGGL((x_update_dma_kernel<S, R, C>), dim3(ntiles), dim3(C * S / R), 0, c->stream, g,
+                         c->u[0], c->u[1], c->u[2], c->work[0], c->work[1], c->zeta[0], c->zeta[1], nu, dt,
+                         c->tw_x);
+      return 0;
+    }
     hipLaunchKernelGGL((x_update_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->u[0]),
                        F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]), F<T>(c->zeta[0]),
                        F<T>(c->zeta[1]), nu, dt, c->tw_x, ntiles);
@@ -781,6 +1022,11 @@ int fx_project_t(spm_ctx *c) {
     constexpr int R = kFusedR(S), C = kFusedC(S);
     const int ntiles = (g.ny * g.hp + C - 1) / C;
     constexpr int per_cu = sizeof(T) == 8 ? SPM_X_PROJECT_BLOCKS_F64 : SPM_X_PROJECT_BLOCKS_F32;
+    if constexpr (sizeof(T) == 8 && SPM_X_DMA_F64) {
+      hipLaunchKernelGGL((x_project_dma_kernel<S, R, C>), dim3(ntiles), dim3(C * S / R), 0, c->stream, g,
+                         c->f[0], c->f[1], c->f[2], c->zeta[0], c->zeta[1], c->uk_dc, c->tw_x);
+      return 0;
+    }
     hipLaunchKernelGGL((x_project_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->f[0]),
                        F<T>(c->f[1]), F<T>(c->f[2]), F<T>(c->zeta[0]), F<T>(c->zeta[1]), c->uk_dc, c->tw_x, ntiles);
   });
