@@ -42,6 +42,9 @@ namespace spm {
 #ifndef SPM_X_PROJECT_WAVES_F64
 #define SPM_X_PROJECT_WAVES_F64 1
 #endif
+#ifndef SPM_ZADV_WAVES
+#define SPM_ZADV_WAVES 1  // minimum waves per SIMD of z_advect_io (a VGPR cap)
+#endif
 
 namespace {
 
@@ -458,6 +461,118 @@ __global__ __launch_bounds__(ROWS *M / R) void z_cross_kernel(Grid g, T *u0, T *
     }
     __syncthreads();
     z_r2c_lds_to_row<T, M, R, ROWS>(lds, out, row0, nrows, g.nz_, kz_cut, tw, twr, tid);
+  }
+}
+
+// Slot of the last Stockham stage (radix N / NS_last) that holds point
+// j0 + off of a thread whose first-stage points are j0 + q N / R: the last
+// stage's points are those same R points in another order.
+template <int N, int R>
+constexpr int last_slot_of(int off) {
+  int ns = 1;
+  while (ns * R < N) ns *= R;
+  const int rs = N / ns;
+  for (int q = 0; q < R; ++q)
+    if ((q / rs) * (N / R) + (q % rs) * ns == off) return q;
+  return -1;
+}
+
+// z_advect with the C2R pre-twiddle computed in the first Stockham stage of
+// each inverse transform (from the pair X[k], X[M-k] in LDS), the inverse's
+// last stage handing the real-space values to registers, and the forward's
+// first stage taking the products from those registers: per block 3 x 9 +
+// 5 x 9 LDS passes instead of 3 x 13 + 5 x 11, and fewer barriers.  Same
+// arithmetic as z_advect_kernel.
+template <typename T, int M, int R, int ROWS>
+__global__ __launch_bounds__(ROWS *M / R, SPM_ZADV_WAVES) void z_advect_io_kernel(Grid g, T *u0, T *u1, T *u2, T *p3, T *p4,
+                                                                 const double2 *__restrict__ tw_g,
+                                                                 const double2 *__restrict__ twr_g, T scale) {
+  using C2 = cplx<T>;
+  using Z = ZRows<M, R, ROWS>;
+  constexpr int NT = Z::NT, CP = Z::CP, H = Z::H;
+  // the kept entries k < TRN_Z <= (2M + 2) / 3 + 1 of the block's rows
+  constexpr int KF = (ROWS * ((2 * M + 2) / 3 + 1) + NT - 1) / NT;
+  static_assert(M >= R, "one first-stage butterfly per thread");
+  __shared__ C2 lds[H * CP];
+  const int tid = threadIdx.x;
+  __shared__ double2 tw[M];
+  for (int t = tid; t < M; t += blockDim.x) tw[t] = tw_g[t];
+  __shared__ double2 twr[M + 1];
+  for (int t = tid; t < M + 1; t += blockDim.x) twr[t] = twr_g[t];
+  const long row0 = (long)blockIdx.x * ROWS;
+  const int nrows = g.nx * g.ny;
+  const int kz_cut = g.trn_z;
+  C2 v[3][R];  // real-space u (packed pairs) at the thread's R points of its row
+  C2 cur[KF], nxt[KF];
+  // element t of a thread: f = tid + t NT is entry k = f % kz_cut of row f / kz_cut
+  auto fetch = [&](const T *a, C2 *dst) {
+#pragma unroll
+    for (int t = 0; t < KF; ++t) {
+      const int f = tid + t * NT, k = f % kz_cut, r = f / kz_cut;
+      const long row = row0 + r;
+      dst[t] = (r < ROWS && row < nrows) ? *reinterpret_cast<const C2 *>(a + row * g.nz_ + 2 * k) : C2{0, 0};
+    }
+  };
+  fetch(u0, cur);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d < 2) fetch(d == 0 ? u1 : u2, nxt);
+#pragma unroll
+    for (int t = 0; t < KF; ++t) {
+      const int f = tid + t * NT, k = f % kz_cut, r = f / kz_cut;
+      if (r < ROWS) lds[k * CP + r] = cur[t];
+    }
+    __syncthreads();
+    if (d < 2)
+#pragma unroll
+      for (int t = 0; t < KF; ++t) cur[t] = nxt[t];
+    // the C2R pre-twiddle that packs the half spectrum into M complex values
+    // (entries k >= TRN_Z are zero and not held in LDS)
+    auto ld = [&](int k, int c, int) -> C2 {
+      const C2 xk = k < kz_cut ? lds[k * CP + c] : C2{0, 0};
+      const C2 xkk = M - k < kz_cut ? lds[(M - k) * CP + c] : C2{0, 0};
+      const C2 cj{xkk.x, -xkk.y};
+      const C2 e = cadd(xk, cj), dd = csub(xk, cj);
+      const C2 w = C2{(T)twr[k].x, (T)twr[k].y};
+      const C2 o = cmul(dd, C2{w.x, -w.y});
+      return C2{e.x - o.y, e.y + o.x};
+    };
+    auto st = [&](int, int, int q, C2 x) { v[d][q] = C2{x.x * scale, x.y * scale}; };
+    fft_stages_io<T, M, R, ROWS, CP, -1, 1, true, true>(lds, tw, tid, ld, st);
+    __syncthreads();  // every read of this transform's LDS precedes the next rows' writes
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    T *out = q == 0 ? u0 : q == 1 ? u1 : q == 2 ? u2 : q == 3 ? p3 : p4;
+    // first-stage slot s holds point j0 + s M / R: the inverse's last-stage
+    // slot of that point
+    auto ld = [&](int, int, int s) -> C2 {
+      const int p = last_slot_of<M, R>(s * (M / R));
+      const T a1 = v[0][p].x, a2 = v[1][p].x, a3 = v[2][p].x;
+      const T b1 = v[0][p].y, b2 = v[1][p].y, b3 = v[2][p].y;
+      if (q == 0) return C2{a2 * a3, b2 * b3};
+      if (q == 1) return C2{a3 * a1, b3 * b1};
+      if (q == 2) return C2{a1 * a2, b1 * b2};
+      if (q == 3) return C2{(a2 - a3) * (a2 + a3), (b2 - b3) * (b2 + b3)};
+      return C2{(a3 - a1) * (a3 + a1), (b3 - b1) * (b3 + b1)};
+    };
+    auto none = [](int, int, int, C2) {};
+    fft_stages_io<T, M, R, ROWS, CP, +1, 1, true, false>(lds, tw, tid, ld, none);
+    // R2C post-twiddle: split the M packed values into the half spectrum
+    for (int f = tid; f < ROWS * H; f += NT) {
+      const int k = f % H, r = f / H;
+      const long row = row0 + r;
+      const C2 z = lds[(k % M) * CP + r];
+      C2 zc = lds[((M - k) % M) * CP + r];
+      zc.y = -zc.y;
+      const C2 e{(T)0.5 * (z.x + zc.x), (T)0.5 * (z.y + zc.y)};
+      const C2 o{(T)0.5 * (z.y - zc.y), (T)-0.5 * (z.x - zc.x)};
+      const C2 w = C2{(T)twr[k].x, (T)twr[k].y};
+      const C2 x = cadd(e, cmul(o, w));
+      if (row < nrows && k < kz_cut) *reinterpret_cast<C2 *>(out + row * g.nz_ + 2 * k) = x;
+    }
+    // the next product's first stage reads registers only and waits at its
+    // own barrier before it writes LDS
   }
 }
 
@@ -959,6 +1074,11 @@ inline T *F(double *p) {
 #ifndef SPM_X_PROJECT_BLOCKS_F32
 #define SPM_X_PROJECT_BLOCKS_F32 0
 #endif
+// z_advect with the pre- and post-twiddles and the products fused into
+// the Stockham stages (0: the kernel with separate LDS passes)
+#ifndef SPM_ZADV_IO
+#define SPM_ZADV_IO 1
+#endif
 // FP64 x_update / x_project through direct-to-LDS loads (0: the register-
 // prefetch kernels)
 #ifndef SPM_X_DMA_F64
@@ -988,9 +1108,14 @@ int fz_advect_t(spm_ctx *c) {
   const T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
   SPM_ZSIZE_SWITCH(g.nz / 2, {
     constexpr int R = kZadvR(S), ROWS = kZadvRows<T>(S);
-    hipLaunchKernelGGL((z_advect_kernel<T, S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R), 0,
-                       c->stream, g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
-                       c->tw_z, c->tw_rz, scale);
+    if constexpr (SPM_ZADV_IO && S >= R)
+      hipLaunchKernelGGL((z_advect_io_kernel<T, S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS),
+                         dim3(ROWS * S / R), 0, c->stream, g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]),
+                         F<T>(c->work[0]), F<T>(c->work[1]), c->tw_z, c->tw_rz, scale);
+    else
+      hipLaunchKernelGGL((z_advect_kernel<T, S, R, ROWS>), dim3((g.nx * g.ny + ROWS - 1) / ROWS), dim3(ROWS * S / R),
+                         0, c->stream, g, F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]),
+                         F<T>(c->work[1]), c->tw_z, c->tw_rz, scale);
   });
   return 0;
 }
