@@ -199,6 +199,39 @@ def closure_line():
     return out
 
 
+def electrolyte_line(steps=20, warmup=3):
+    """Charged colloids at the electrolyte workload: 128^3, 32 spheres
+    a = 5 with Z = 50, +-1 salt with Debye length 3, Onsager 0.1, E = (0.01,
+    0, 0), uniform start; steps/s of the FP64 path and of FP32 transport
+    (FP64 Poisson side), each `steps` steps after `warmup`."""
+    from kapsel_old_amd.spm import SPM
+    n, npart, a = 128, 32, 5.0
+    el = abi.make_electrolyte(n_spec=2, valency=(1.0, -1.0), onsager=(0.1, 0.1), dielectric=0.018,
+                              debye_length=3.0, surface_charge=(50.0,), e_ext=(0.01, 0.0, 0.0))
+    pos = abi.random_positions(npart, (n, n, n), 2.2 * a, seed=12345)
+    p = abi.new_particles(npart)
+    p["x"] = pos
+    p["x_nopbc"] = pos
+    out = {"workload": "electrolyte, 128^3 grid, 32 spheres a=5 Z=50, +-1 salt lambda_D=3, E=(0.01,0,0), "
+                       "literal schedule"}
+    for prec, key in ((abi.FP64, "fp64"), (abi.FP32, "fp32_transport")):
+        g = SPM(abi.make_params(n, a=a, a_xi=2.0, n_particles=npart, rotation=1, dt=0.05, kbt=1.0,
+                                fix_cell=(0, 0, 0), precision=prec))
+        g.put_particles(p)
+        g.init_fluid()
+        g.set_electrolyte(el)
+        g.Init_rho_ion()
+        g.step(warmup)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.step(steps)
+        g.synchronize()
+        out[key] = {"value": steps / (time.perf_counter() - t0), "unit": "time-steps/s"}
+        g.close()
+    out["fp32_over_fp64"] = out["fp32_transport"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def ensemble_line(nbox=64, steps=200, seed=4):
     """The batched ensemble: 64 boxes with grid edges drawn from
     {32, 48, 64, 80, 96}, 1-50 spheres a = 3-6 each, WCA, kBT = 0.05, FP32
@@ -352,6 +385,7 @@ def main():
         out["one_step_per_call"] = {"value": one_step_calls(n, np_, max(20, args.steps // 2)),
                                     "unit": "time-steps/s", "note": "spm_step(1) per step, FP64 fused"}
         out["closure"] = closure_line()
+        out["electrolyte"] = electrolyte_line()
         out["ensemble"] = ensemble_line()
     if not args.no_cpu_baseline and precision == abi.FP64:
         nthreads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)

@@ -322,7 +322,11 @@ typedef struct spm_electrolyte {
 /* Switches the context to the electrolyte equations (Time_evolution_hydro's
  * SW_EQ == Electrolyte branches, sp_3d_ns.cxx:46-57,89-121,183) and
  * allocates the ion fields.  Call once, after spm_create.  Electrolyte
- * steps run the literal schedule (FP64). */
+ * steps run the literal schedule.  In an SPM_FP32 context the transport
+ * fields (ion spectra and right-hand sides, fluxes, velocity, vorticity,
+ * phi, f) are FP32 while the Poisson solve (charge density, potential and
+ * its gradient in Fourier space, the particles' charge profile, surface
+ * normals) and every grid sum stay FP64. */
 int spm_set_electrolyte(spm_ctx *ctx, const spm_electrolyte *el);
 /* Init_rho_ion (operate_electrolyte.cxx:695-815) with the uniform initial
  * distribution (Poisson_Boltzmann off): Set_uniform_ion_charge_density_salt
@@ -337,7 +341,8 @@ int spm_init_rho_ion(spm_ctx *ctx);
  * working memory meanwhile, as in the reference.  The iterative profile of
  * a single particle or of salt-free ions is SPM_ERR_UNSUPPORTED. */
 int spm_init_rho_ion_pb(spm_ctx *ctx, int *iterations);
-/* Concentration[species] spectrum (reference layout) in and out. */
+/* Concentration[species] spectrum (reference layout, FP64 on the host) in
+ * and out. */
 int spm_put_concentration(spm_ctx *ctx, int species, const double *host);
 int spm_get_concentration(spm_ctx *ctx, int species, double *host);
 /* Total_solute[0 .. n_spec) in and out. */

@@ -597,6 +597,8 @@ int fft_r2c_many(spm_ctx *c, double *const *a, int n) { return fft_many(c, a, n,
 int fft_c2r_many(spm_ctx *c, double *const *a, int n) { return fft_many(c, a, n, false); }
 int fft_r2c(spm_ctx *c, double *a) { return fft_many(c, &a, 1, true); }
 int fft_c2r(spm_ctx *c, double *a) { return fft_many(c, &a, 1, false); }
+int fft_r2c_f64(spm_ctx *c, double *a) { return fft_many_t<double>(c, &a, 1, true); }
+int fft_c2r_many_f64(spm_ctx *c, double *const *a, int n) { return fft_many_t<double>(c, a, n, false); }
 
 // ---------------------------------------------------------------------
 // Autotuner of the mixed-radix launch shapes: every variant of every

@@ -1389,10 +1389,11 @@ int p_copy_hydro1(spm_ctx *c) {
 
 // phi (no clamp) and the unnormalised surface charge profile; builds the
 // tile lists at the current positions first.
+// phi and surface are FP64 fields whatever the context's field type.
 int p_make_phi_qq(spm_ctx *c, double *phi, double *surface) {
   if (c->np == 0) {
-    RCP(k_zero(c, phi, c->nfield));
-    return k_zero(c, surface, c->nfield);
+    RCP(k_zero_f64(c, phi, c->nfield));
+    return k_zero_f64(c, surface, c->nfield);
   }
   RCP(p_build_tiles(c));
   tbeg(c);
@@ -1405,9 +1406,9 @@ int p_make_phi_qq(spm_ctx *c, double *phi, double *surface) {
 }
 
 // Make_surface_normal into c->surface_normal (tile lists must be current).
-int p_surface_normal(spm_ctx *c) {
+int p_surface_normal(spm_ctx *c) {  // FP64 normals whatever the field type
   if (c->np == 0) {
-    for (int d = 0; d < 3; ++d) RCP(k_zero(c, c->surface_normal[d], c->nfield));
+    for (int d = 0; d < 3; ++d) RCP(k_zero_f64(c, c->surface_normal[d], c->nfield));
     return 0;
   }
   tbeg(c);

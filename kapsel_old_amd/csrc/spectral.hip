@@ -422,6 +422,14 @@ int k_zero(spm_ctx *c, double *a, size_t n) {
   return 0;
 }
 
+int k_zero_f64(spm_ctx *c, double *a, size_t n) {
+  tbeg(c);
+  hipLaunchKernelGGL(zero_kernel, dim3(nblk((long)n)), dim3(kBlock), 0, c->stream, a, (long)n);
+  tend(c, SPM_K_REALSPACE, 8.0 * (double)n);
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
 __global__ void zero_int_kernel(int *a, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;

@@ -310,7 +310,7 @@ void free_ctx(spm_ctx *c) {
   if (c->pool_host) (void)hipHostFree(c->pool_host);
   for (double *p : {c->conc[0], c->conc[1], c->conc_rhs[0], c->conc_rhs[1], c->surface_normal[0],
                     c->surface_normal[1], c->surface_normal[2], c->f_ns1[0], c->f_ns1[1], c->red, c->le_red,
-                    c->le_stress, c->le_strain})
+                    c->le_stress, c->le_strain, c->el_q, c->el_p, c->el_pq, c->el_g[0], c->el_g[1], c->el_g[2]})
     if (p) (void)hipFree(p);
   for (int *p : {c->spec, c->stencil, c->pxint, c->tile_count, c->tile_start, c->tile_cursor, c->tile_list,
                  c->cell_count, c->cell_start, c->cell_cursor, c->cell_list, c->pcell, c->flag, c->tile_count_obl,
@@ -528,8 +528,14 @@ int setup(spm_ctx *c) {
 }
 
 // ---------------------------------------------------------------------
-// electrolyte (SW_EQ == Electrolyte), literal schedule, FP64
+// electrolyte (SW_EQ == Electrolyte), literal schedule.  Transport fields
+// (ion spectra, fluxes, velocity, vorticity, phi, f) of the context's field
+// type; the Poisson side (charge density, potential and its gradient, the
+// particles' charge profile, surface normals, the sums) in FP64.  An FP64
+// context keeps the Poisson fields in its own working fields, an FP32 one in
+// the FP64 buffers el_* (every field buffer holds nfield doubles).
 // ---------------------------------------------------------------------
+inline bool el32(const spm_ctx *c) { return c->prm.precision == SPM_FP32; }
 // result slots of c->red (after the partial sums)
 enum { SLOT_SURF = 0, SLOT_SCALE0 = 1, SLOT_VOLUME = 3, SLOT_COUNT = 4, SLOT_FE = 6 };
 
@@ -541,17 +547,19 @@ int make_phi_qq(spm_ctx *c, double *phi, double *surface) {
   double abs_total = 0.;
   for (int n = 0; n < c->np; ++n) abs_total += std::fabs(c->qtab.q[c->spec_host[n]]);
   RC(e_reduce(c, 0, surface, nullptr, SLOT_SURF, 1, abs_total));
-  RC(e_scale_by(c, surface, (long)c->nfield, SLOT_SURF));
+  RC(e_scale_by(c, surface, (long)c->nfield, SLOT_SURF, true));
   return 0;
 }
 
-// A_k2a_out (fft_wrapper.h:639-651).
+// A_k2a_out (fft_wrapper.h:639-651), fields of the context's type.
 int k2a_out(spm_ctx *c, const double *a_k, double *a_x) {
-  SPM_HIP(c, hipMemcpyAsync(a_x, a_k, sizeof(double) * c->nfield, hipMemcpyDeviceToDevice, c->stream));
+  const size_t eb = el32(c) ? sizeof(float) : sizeof(double);
+  SPM_HIP(c, hipMemcpyAsync(a_x, a_k, eb * c->nfield, hipMemcpyDeviceToDevice, c->stream));
   return fft_c2r(c, a_x);
 }
 
-// Conc_k2charge_field (operate_electrolyte.cxx:45-74).
+// Conc_k2charge_field (operate_electrolyte.cxx:45-74): charge and phi_p
+// FP64, dmy of the field type.
 int conc_k2charge(spm_ctx *c, double *charge, double *phi_p, double *dmy) {
   RC(make_phi_qq(c, phi_p, charge));
   for (int n = 0; n < c->el.n_spec; ++n) {
@@ -569,17 +577,27 @@ int rhs_nernst_planck(spm_ctx *c) {
   double *const fl[3] = {c->up[0], c->up[1], c->up[2]};
   double *const gp[3] = {c->f[0], c->f[1], c->f[2]};
   double *const p5[5] = {c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]};
-  // potential gradient in real space (:263-274)
-  RC(conc_k2charge(c, c->u[0], c->u[1], c->u[2]));
-  RC(fft_r2c(c, c->u[0]));
+  // potential gradient in real space (:263-274), FP64: in f (FP64
+  // context) or el_g, then copied to f as the field type (FP32 context),
+  // where the step's Calc_f_hydro reads it
+  double *const g64[3] = {c->el_g[0], c->el_g[1], c->el_g[2]};
+  double *const *grad = el32(c) ? g64 : gp;
+  double *charge = el32(c) ? c->el_q : c->u[0];
+  RC(conc_k2charge(c, charge, el32(c) ? c->el_pq : c->u[1], c->u[2]));
+  RC(el32(c) ? fft_r2c_f64(c, charge) : fft_r2c(c, charge));
   double cst[3];
   {
     double amp = (double)((long)c->grid.nx * c->grid.ny * c->grid.nz);
     if (c->el.ac) amp = std::sin(c->el.angular_frequency * c->time);
     for (int d = 0; d < 3; ++d) cst[d] = c->el.e_ext[d] * amp;
   }
-  RC(e_poisson_grad(c, c->u[0], gp, 1, cst));
-  RC(fft_c2r_many(c, gp, 3));
+  RC(e_poisson_grad(c, charge, grad, 1, cst));
+  if (el32(c)) {
+    RC(fft_c2r_many_f64(c, grad, 3));
+    RC(e_to_t(c, grad, gp));
+  } else {
+    RC(fft_c2r_many(c, grad, 3));
+  }
   // Rhs_NS_solute (:181-227)
   RC(zeta_k2u(c, 0));
   RC(p_surface_normal(c));  // tile lists are current from make_phi_qq
@@ -588,7 +606,7 @@ int rhs_nernst_planck(spm_ctx *c) {
     RC(e_grad_scaled(c, c->conc[n], c->prm.kbt * c->el.onsager[n], fl));
     RC(fft_c2r_many(c, fl, 3));
     RC(k2a_out(c, c->conc[n], c->conc_rhs[n]));
-    RC(e_np_flux(c, fl, c->conc_rhs[n], gp, c->surface_normal, u3, c->el.onsager[n], c->valency_e[n]));
+    RC(e_np_flux(c, fl, c->conc_rhs[n], grad, c->surface_normal, u3, c->el.onsager[n], c->valency_e[n]));
     RC(fft_r2c_many(c, fl, 3));
     RC(e_divergence(c, fl, c->conc_rhs[n]));
   }
@@ -618,21 +636,25 @@ int rescale_solute(spm_ctx *c) {
 }
 
 // Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177), free
-// particles, into f; charge in up[0], potential in up[1].
+// particles, into f.  FP64 context: charge in up[0], potential in up[1],
+// gradient in f; FP32 context: el_q, el_p, el_g.
 int coulomb_force(spm_ctx *c) {
   double *const f3[3] = {c->f[0], c->f[1], c->f[2]};
-  RC(conc_k2charge(c, c->up[0], c->f[0], c->f[1]));
-  SPM_HIP(c, hipMemcpyAsync(c->up[1], c->up[0], sizeof(double) * c->nfield, hipMemcpyDeviceToDevice, c->stream));
-  RC(fft_r2c(c, c->up[1]));
+  double *const g64[3] = {c->el_g[0], c->el_g[1], c->el_g[2]};
+  double *const *grad = el32(c) ? g64 : f3;
+  double *charge = el32(c) ? c->el_q : c->up[0], *pot = el32(c) ? c->el_p : c->up[1];
+  RC(conc_k2charge(c, charge, el32(c) ? c->el_pq : c->f[0], c->f[1]));
+  SPM_HIP(c, hipMemcpyAsync(pot, charge, sizeof(double) * c->nfield, hipMemcpyDeviceToDevice, c->stream));
+  RC(el32(c) ? fft_r2c_f64(c, pot) : fft_r2c(c, pot));
   const double zero[3] = {0.0, 0.0, 0.0};
-  RC(e_poisson_grad(c, c->up[1], f3, 0, zero));
-  RC(fft_c2r_many(c, f3, 3));
+  RC(e_poisson_grad(c, pot, grad, 0, zero));
+  RC(el32(c) ? fft_c2r_many_f64(c, grad, 3) : fft_c2r_many(c, grad, 3));
   double ext[3];
   for (int d = 0; d < 3; ++d) {
     ext[d] = c->el.e_ext[d];
     if (c->el.ac) ext[d] *= std::sin(c->el.angular_frequency * c->time);
   }
-  return e_coulomb(c, c->up[0], f3, ext, c->el.external_field);
+  return e_coulomb(c, charge, grad, f3, ext, c->el.external_field);
 }
 
 // Solenoidal_u (operate_omega.h:96-103) on u.
@@ -730,18 +752,19 @@ int init_rho_ion(spm_ctx *c) {
 // electrostatic term sum rho_e (Phi / 2 - E.x) dx^3, Phi the periodic
 // Poisson potential of the charge density (phi, up[0..2] as working memory).
 int free_energy_pb(spm_ctx *c, double *F) {
-  double *charge = c->up[1], *val = c->up[2];
+  double *charge = el32(c) ? c->el_q : c->up[1], *val = c->up[2];
   RC(p_make_phi(c, 0));
   for (int n = 0; n < c->el.n_spec; ++n) {
     RC(k2a_out(c, c->conc[n], val));
     RC(e_reduce(c, 3, val, c->phi, SLOT_FE + n, 0, 0.0));
   }
-  RC(conc_k2charge(c, charge, c->up[0], val));
-  // A2a_k_out, Charge_field_k2Coulomb_potential_k_PBC, A_k2a
+  RC(conc_k2charge(c, charge, el32(c) ? c->el_pq : c->up[0], val));
+  // A2a_k_out, Charge_field_k2Coulomb_potential_k_PBC, A_k2a (FP64)
+  if (el32(c)) val = c->el_p;
   SPM_HIP(c, hipMemcpyAsync(val, charge, sizeof(double) * c->nfield, hipMemcpyDeviceToDevice, c->stream));
-  RC(fft_r2c(c, val));
+  RC(el32(c) ? fft_r2c_f64(c, val) : fft_r2c(c, val));
   RC(e_coulomb_potential(c, val));
-  RC(fft_c2r(c, val));
+  RC(el32(c) ? fft_c2r_many_f64(c, &val, 1) : fft_c2r(c, val));
   double ext[3];
   for (int d = 0; d < 3; ++d) {
     ext[d] = c->el.e_ext[d];
@@ -797,14 +820,54 @@ int steady_ion_density(spm_ctx *c, int *iterations) {
   return rc;
 }
 
+// FP32 transport: the relaxation stops when the free energy changes by less
+// than 1e-4, a change that FP64 sums over FP32 ion fields resolve only to
+// their rounding (the test then ends at a different, noise-chosen
+// iteration).  So an FP32 context relaxes the ions in an FP64 twin built
+// from the same parameters, particles and time, and takes the relaxed
+// spectra rounded to FP32; its own Count_solute_each follows.
+int pb_relax_in_f64(spm_ctx *c, int *iterations) {
+  spm_params p = c->prm;
+  p.precision = SPM_FP64;
+  spm_ctx *t = nullptr;
+  if (int rc = spm_create(&p, &t)) {
+    c->err = std::string("FP64 context for the Poisson-Boltzmann start: ") + spm_last_error(nullptr);
+    return rc;
+  }
+  int rc = 0;
+  std::vector<spm_particle> pp(c->np > 0 ? c->np : 1);
+  std::vector<double> host(ref_count(c->grid));
+  double total[2] = {0.0, 0.0};
+  if (!rc) rc = spm_get_particles(c, pp.data(), c->np);
+  if (!rc) rc = spm_put_particles(t, pp.data(), c->np);
+  if (!rc) rc = spm_put_time(t, c->ts, c->time);
+  if (!rc) rc = spm_init_fluid(t);
+  if (!rc) rc = spm_set_electrolyte(t, &c->el);
+  if (!rc) rc = spm_init_rho_ion_pb(t, iterations);
+  for (int n = 0; n < c->el.n_spec && !rc; ++n) {
+    rc = spm_get_concentration(t, n, host.data());
+    if (!rc) rc = spm_put_concentration(c, n, host.data());
+  }
+  if (!rc) rc = spm_get_total_solute(t, total);
+  if (rc && c->err.empty()) c->err = std::string("FP64 twin of the Poisson-Boltzmann start: ") + spm_last_error(t);
+  spm_destroy(t);
+  if (!rc)
+    for (int n = 0; n < 2; ++n) c->total_solute[n] = total[n];
+  return rc;
+}
+
 int init_rho_ion_pb(spm_ctx *c, int *iterations) {
   if (c->el.n_spec != 2 || c->np < 2) {
     c->err = "the Poisson-Boltzmann start is built for salt with more than one particle (uniform start, then "
              "Set_steadystate_ion_density); the iterative profile of one particle or salt-free ions is not";
     return SPM_ERR_UNSUPPORTED;
   }
-  RC(init_rho_ion(c));  // Set_uniform_ion_charge_density_salt, Total_solute
-  RC(steady_ion_density(c, iterations));
+  if (el32(c)) {
+    RC(pb_relax_in_f64(c, iterations));
+  } else {
+    RC(init_rho_ion(c));  // Set_uniform_ion_charge_density_salt, Total_solute
+    RC(steady_ion_density(c, iterations));
+  }
   // Count_solute_each (solute_rhs.h:143-156)
   RC(p_make_phi(c, 0));
   for (int n = 0; n < c->el.n_spec; ++n) {
@@ -828,10 +891,6 @@ int set_electrolyte(spm_ctx *c, const spm_electrolyte *el) {
     c->err = "the context already runs the electrolyte equations";
     return SPM_ERR_INVALID;
   }
-  if (c->prm.precision != SPM_FP64) {
-    c->err = "the electrolyte path runs in FP64 only";
-    return SPM_ERR_UNSUPPORTED;
-  }
   c->el = *el;
   for (int n = 0; n < 2; ++n) {
     RC(dalloc(c, &c->conc[n], c->nfield));
@@ -839,6 +898,12 @@ int set_electrolyte(spm_ctx *c, const spm_electrolyte *el) {
     RC(dalloc(c, &c->f_ns1[n], c->nfield));
   }
   for (int d = 0; d < 3; ++d) RC(dalloc(c, &c->surface_normal[d], c->nfield));
+  if (el32(c)) {  // the FP64 Poisson side of FP32 transport
+    RC(dalloc(c, &c->el_q, c->nfield));
+    RC(dalloc(c, &c->el_p, c->nfield));
+    RC(dalloc(c, &c->el_pq, c->nfield));
+    for (int d = 0; d < 3; ++d) RC(dalloc(c, &c->el_g[d], c->nfield));
+  }
   RC(dalloc(c, &c->red, (size_t)e_red_slots()));
   for (int s = 0; s < SPM_MAX_SPECIES; ++s) c->qtab.q[s] = el->surface_charge[s] * el->elementary_charge;
   for (int n = 0; n < 2; ++n) c->valency_e[n] = el->valency[n] * el->elementary_charge;
@@ -1487,10 +1552,19 @@ int spm_nssolute_solver_euler(spm_ctx *c) { ELECTRO_ENTRY(nssolute_solver(c)); }
 int spm_rescale_solute(spm_ctx *c) { ELECTRO_ENTRY(rescale_solute(c)); }
 int spm_make_coulomb_force(spm_ctx *c) { ELECTRO_ENTRY(coulomb_force(c)); }
 
+// ion spectra in the reference layout, FP64 on the host (rounded to FP32
+// into an FP32 context's spectra)
 int spm_put_concentration(spm_ctx *c, int n, const double *host) {
   if (!c || !host || !c->electro || n < 0 || n >= c->el.n_spec) return SPM_ERR_INVALID;
   SPM_HIP(c, hipSetDevice(c->device));
-  RC(spm::rows_to_device(c, c->conc[n], host, sizeof(double)));
+  if (c->prm.precision == SPM_FP32) {
+    const size_t m = spm::ref_count(c->grid);
+    std::vector<float> tmp(m);
+    for (size_t i = 0; i < m; ++i) tmp[i] = (float)host[i];
+    RC(spm::rows_to_device(c, c->conc[n], tmp.data(), sizeof(float)));
+  } else {
+    RC(spm::rows_to_device(c, c->conc[n], host, sizeof(double)));
+  }
   SPM_HIP(c, hipStreamSynchronize(c->stream));
   return SPM_OK;
 }
@@ -1498,6 +1572,14 @@ int spm_put_concentration(spm_ctx *c, int n, const double *host) {
 int spm_get_concentration(spm_ctx *c, int n, double *host) {
   if (!c || !host || !c->electro || n < 0 || n >= c->el.n_spec) return SPM_ERR_INVALID;
   SPM_HIP(c, hipSetDevice(c->device));
+  if (c->prm.precision == SPM_FP32) {
+    const size_t m = spm::ref_count(c->grid);
+    std::vector<float> tmp(m);
+    RC(spm::rows_to_host(c, tmp.data(), c->conc[n], sizeof(float)));
+    SPM_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < m; ++i) host[i] = tmp[i];
+    return SPM_OK;
+  }
   RC(spm::rows_to_host(c, host, c->conc[n], sizeof(double)));
   SPM_HIP(c, hipStreamSynchronize(c->stream));
   return SPM_OK;

@@ -222,6 +222,9 @@ struct spm_ctx {
   double *surface_normal[3] = {nullptr, nullptr, nullptr};
   double *f_ns1[2] = {nullptr, nullptr};
   double *red = nullptr;  // partial sums, then result scalars
+  // FP64 Poisson side of an FP32 context (an FP64 context uses its fields):
+  // charge density, potential, its gradient, the particles' charge profile
+  double *el_q = nullptr, *el_p = nullptr, *el_g[3] = {nullptr, nullptr, nullptr}, *el_pq = nullptr;
   double total_solute[2] = {0.0, 0.0};
   spm::QTable qtab{};
   double valency_e[2] = {0.0, 0.0};
@@ -284,6 +287,10 @@ void tend(spm_ctx *c, int family, double bytes);
 // fft.hip: 3-D real transforms in the reference layout (in place).
 int fft_r2c(spm_ctx *c, double *a);
 int fft_c2r(spm_ctx *c, double *a);
+// FP64 transforms whatever the context's field type (the Poisson side of an
+// FP32 electrolyte context; its row pitch is also a valid FP64 pitch)
+int fft_r2c_f64(spm_ctx *c, double *a);
+int fft_c2r_many_f64(spm_ctx *c, double *const *a, int n);
 // batched versions over several fields (same grid)
 int fft_r2c_many(spm_ctx *c, double *const *a, int n);
 int fft_c2r_many(spm_ctx *c, double *const *a, int n);
@@ -322,6 +329,7 @@ int k_solenoidal_trunc(spm_ctx *c, double *u0, double *u1, double *u2);
 int k_make_f(spm_ctx *c);
 int k_add_f(spm_ctx *c);
 int k_zero(spm_ctx *c, double *a, size_t n);
+int k_zero_f64(spm_ctx *c, double *a, size_t n);  // n doubles, whatever the field type
 int k_zero_int(spm_ctx *c, int *a, size_t n);
 int k_check_finite(spm_ctx *c, const double *a, bool spectrum);
 // field copies between the reference layout on the host and the device
@@ -393,11 +401,14 @@ int e_half_potential(spm_ctx *c, double *pot, const double ext[3], int external)
 int e_charge_add(spm_ctx *c, double *charge, const double *conc, const double *phi, double ze);
 int e_np_flux(spm_ctx *c, double *const flux[3], const double *cx, double *const grad[3], double *const normal[3],
               double *const u[3], double onsager, double ze);
-int e_coulomb(spm_ctx *c, const double *charge, double *const f[3], const double ext[3], int external);
+// f = charge (-grad + E_ext); grad FP64, f of the field type (may be grad in FP64)
+int e_coulomb(spm_ctx *c, const double *charge, double *const grad[3], double *const f[3], const double ext[3],
+              int external);
+int e_to_t(spm_ctx *c, double *const a[3], double *const b[3]);  // FP64 real fields -> field type
 int e_add_scaled(spm_ctx *c, double *const u[3], double *const f[3], double dmy);
 int e_fill_real(spm_ctx *c, double *a, double v);
 int e_reduce(spm_ctx *c, int mode, const double *a, const double *phi, int slot, int ratio, double numer);
-int e_scale_by(spm_ctx *c, double *a, long n, int slot);
+int e_scale_by(spm_ctx *c, double *a, long n, int slot, bool f64 = false);
 int e_read_slot(spm_ctx *c, int slot, double *out);
 int p_copy_hydro1(spm_ctx *c);
 int p_make_phi_qq(spm_ctx *c, double *phi, double *surface);

@@ -151,10 +151,6 @@ def test_electrolyte_rejects_bad_use(SPM):
     g.set_electrolyte(electrolyte())
     with pytest.raises(SpmError):
         g.set_electrolyte(electrolyte())  # twice
-    prm32 = abi.make_params(32, a=4.0, n_particles=2, precision=abi.FP32)
-    h = SPM(prm32)
-    with pytest.raises(SpmError):
-        h.set_electrolyte(electrolyte())
 
 
 @pytest.mark.parametrize("e_ext", [(0.0, 0.0, 0.0), (0.01, 0.0, 0.0)], ids=["no_field", "field"])
