@@ -83,9 +83,10 @@ def test_state_resume_is_exact(SPM, tmp_path, schedule, precision, kbt):
     assert_same(sa, sb)
 
 
-def test_state_resume_electrolyte(SPM, tmp_path):
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32], ids=["fp64", "fp32_transport"])
+def test_state_resume_electrolyte(SPM, tmp_path, precision):
     n = 5
-    prm = abi.make_params(*GRID, a=4.0, n_particles=n, dt=0.05, kbt=1.0, fix_cell=(0, 0, 0))
+    prm = abi.make_params(*GRID, a=4.0, n_particles=n, dt=0.05, kbt=1.0, fix_cell=(0, 0, 0), precision=precision)
     el = abi.make_electrolyte(n_spec=2, valency=(1.0, -1.0), onsager=(0.1, 0.1), dielectric=1.0, debye_length=3.0,
                               surface_charge=(5.0,), e_ext=(0.01, 0.0, 0.0))
     p = particles(n, GRID, 4.0, seed=31, vel=0.02, omega=0.005)
