@@ -66,6 +66,39 @@ __device__ __forceinline__ int tile_slot(int s, const int k[3], const int t0[3],
   return ((t0[0] + a) % nt[0] * nt[1] + (t0[1] + b) % nt[1]) * nt[2] + (t0[2] + cc) % nt[2];
 }
 
+// Whether a particle with anchor xi and residue res reaches tile ti: some
+// point of the tile's box lies within the stencil cube (+-nh) and within
+// rcut + sqrt(3) dx of the particle (plus 0.01 dx, far above the rounding
+// of the residue form of the distance).  The lists keep a particle only in
+// the tiles it reaches, so every per-point sum over a list (phi, up, the
+// shell charge and normals, all zero beyond rcut) is that over the cube's
+// tiles, and so is phi at an off-grid point taken from the list of the
+// tile of the grid point below it (that point lies within sqrt(3) dx).  A
+// box whose offsets cross the periodic wrap is kept.
+__device__ __forceinline__ bool tile_reached(const Grid &g, const PModel &pm, const int xi[3], const double res[3],
+                                             int ti, int ty, int tz) {
+  const int tc[3] = {ti / (tz * ty), (ti / tz) % ty, ti % tz};
+  const int N[3] = {g.nx, g.ny, g.nz};
+  const double reach = pm.radius + pm.hxi + (1.7320508075688772 + 1e-2) * pm.dx;
+  double d2 = 0.0;
+  bool cross = false, out = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int o0 = tc[d] * kTile - xi[d];
+    if (o0 >= N[d] / 2) o0 -= N[d];
+    if (o0 < -(N[d] / 2)) o0 += N[d];
+    int o1 = o0 + kTile - 1;
+    cross = cross || o1 >= N[d] / 2;
+    out = out || o1 < -pm.nh || o0 > pm.nh;
+    o0 = o0 < -pm.nh ? -pm.nh : o0;
+    o1 = o1 > pm.nh ? pm.nh : o1;
+    const double r0 = (double)o0 * pm.dx - res[d], r1 = (double)o1 * pm.dx - res[d];
+    const double r = r0 > 0.0 ? r0 : (r1 < 0.0 ? -r1 : 0.0);
+    d2 += r * r;
+  }
+  return cross || (!out && d2 < reach * reach);
+}
+
 __global__ void anchor_count_kernel(Grid g, PModel pm, const double *__restrict__ pb, int *pxint, double *pres,
                                     int tx, int ty, int tz, int k0, int k1, int k2, int *tile_count) {
   const int per = k0 * k1 * k2;
@@ -75,15 +108,17 @@ __global__ void anchor_count_kernel(Grid g, PModel pm, const double *__restrict_
   const double idx = 1. / pm.dx;
   const int N[3] = {g.nx, g.ny, g.nz};
   int xi[3];
+  double res[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double dmy = P(pb, pm.np, P_X + d, n) * idx;
     // positions are kept in [0, L) by PBC; the clamp only keeps indices in
     // range if a non-finite or escaped position ever arrives
     xi[d] = (dmy >= 0.0 && dmy < (double)N[d]) ? (int)dmy : 0;
+    res[d] = (dmy - (double)xi[d]) * pm.dx;
     if (s == 0) {
       pxint[3 * n + d] = xi[d];
-      pres[3 * n + d] = (dmy - (double)xi[d]) * pm.dx;
+      pres[3 * n + d] = res[d];
     }
   }
   int t0[3], cn[3];
@@ -91,21 +126,23 @@ __global__ void anchor_count_kernel(Grid g, PModel pm, const double *__restrict_
 #pragma unroll
   for (int d = 0; d < 3; ++d) tile_span(xi[d], pm.nh, nt[d], t0[d], cn[d]);
   const int ti = tile_slot(s, k, t0, cn, nt);
-  if (ti >= 0) atomicAdd(&tile_count[ti], 1);
+  if (ti >= 0 && tile_reached(g, pm, xi, res, ti, ty, tz)) atomicAdd(&tile_count[ti], 1);
 }
 
-__global__ void tile_fill_kernel(PModel pm, const int *__restrict__ pxint, int tx, int ty, int tz, int k0, int k1,
-                                 int k2, int *tile_cursor, int *tile_list) {
+__global__ void tile_fill_kernel(Grid g, PModel pm, const int *__restrict__ pxint, const double *__restrict__ pres,
+                                 int tx, int ty, int tz, int k0, int k1, int k2, int *tile_cursor, int *tile_list) {
   const int per = k0 * k1 * k2;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int n = (int)(t / per), s = (int)(t % per);
   if (n >= pm.np) return;
   int t0[3], cn[3];
   const int nt[3] = {tx, ty, tz}, k[3] = {k0, k1, k2};
+  const int xi[3] = {pxint[3 * n], pxint[3 * n + 1], pxint[3 * n + 2]};
+  const double res[3] = {pres[3 * n], pres[3 * n + 1], pres[3 * n + 2]};
 #pragma unroll
-  for (int d = 0; d < 3; ++d) tile_span(pxint[3 * n + d], pm.nh, nt[d], t0[d], cn[d]);
+  for (int d = 0; d < 3; ++d) tile_span(xi[d], pm.nh, nt[d], t0[d], cn[d]);
   const int ti = tile_slot(s, k, t0, cn, nt);
-  if (ti >= 0) tile_list[atomicAdd(&tile_cursor[ti], 1)] = n;
+  if (ti >= 0 && tile_reached(g, pm, xi, res, ti, ty, tz)) tile_list[atomicAdd(&tile_cursor[ti], 1)] = n;
 }
 
 // Exclusive scan of count[0..n) into start[] and cursor[], one block per
@@ -235,18 +272,20 @@ __global__ __launch_bounds__(1024) void tiles_small_kernel(Grid g, PModel pm, co
   for (long q = t; q < slots; q += 1024) {
     const int n = (int)(q / per), sl = (int)(q % per);
     int xi[3], t0[3], cn[3];
+    double res[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double dmy = P(pb, pm.np, P_X + d, n) * idx;
       xi[d] = (dmy >= 0.0 && dmy < (double)N[d]) ? (int)dmy : 0;
+      res[d] = (dmy - (double)xi[d]) * pm.dx;
       if (sl == 0) {
         pxint[3 * n + d] = xi[d];
-        pres[3 * n + d] = (dmy - (double)xi[d]) * pm.dx;
+        pres[3 * n + d] = res[d];
       }
       tile_span(xi[d], pm.nh, nt[d], t0[d], cn[d]);
     }
     const int ti = tile_slot(sl, k, t0, cn, nt);
-    if (ti >= 0) atomicAdd(&cnt[ti], 1);
+    if (ti >= 0 && tile_reached(g, pm, xi, res, ti, ty, tz)) atomicAdd(&cnt[ti], 1);
   }
   __syncthreads();
   // exclusive scan: 4 consecutive counts per thread, wavefront shuffles,
@@ -279,10 +318,12 @@ __global__ __launch_bounds__(1024) void tiles_small_kernel(Grid g, PModel pm, co
   for (long q = t; q < slots; q += 1024) {
     const int n = (int)(q / per), sl = (int)(q % per);
     int t0[3], cn[3];
+    const int xi[3] = {pxint[3 * n], pxint[3 * n + 1], pxint[3 * n + 2]};
+    const double res[3] = {pres[3 * n], pres[3 * n + 1], pres[3 * n + 2]};
 #pragma unroll
-    for (int d = 0; d < 3; ++d) tile_span(pxint[3 * n + d], pm.nh, nt[d], t0[d], cn[d]);
+    for (int d = 0; d < 3; ++d) tile_span(xi[d], pm.nh, nt[d], t0[d], cn[d]);
     const int ti = tile_slot(sl, k, t0, cn, nt);
-    if (ti >= 0) tile_list[atomicAdd(&cur[ti], 1)] = n;
+    if (ti >= 0 && tile_reached(g, pm, xi, res, ti, ty, tz)) tile_list[atomicAdd(&cur[ti], 1)] = n;
   }
   __syncthreads();
   for (int i = t; i < ntiles; i += 1024) {
@@ -1282,8 +1323,8 @@ int p_build_tiles(spm_ctx *c) {
                      c->pxint, c->pres, c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_count);
   hipLaunchKernelGGL(scan_kernel, dim3(scan_blocks(c->ntiles)), dim3(1024), 0, c->stream, c->tile_count, c->tile_start, c->tile_cursor,
                      c->ntiles);
-  hipLaunchKernelGGL(tile_fill_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->pm, c->pxint,
-                     c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_cursor, c->tile_list);
+  hipLaunchKernelGGL(tile_fill_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->grid, c->pm, c->pxint,
+                     c->pres, c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_cursor, c->tile_list);
   hipLaunchKernelGGL(sort_segments_kernel, dim3(c->ntiles), dim3(64), 0, c->stream, c->tile_start,
                      c->tile_count, c->tile_list, c->ntiles);
   tend(c, SPM_K_PARTICLE, 16.0 * c->ntiles);
