@@ -122,7 +122,6 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp,
   constexpr int NT = ROWS * M / R;
   constexpr int CP = ROWS + 1;
   constexpr int H = M + 1;
-  constexpr int HP = M / 2 + 1;  // pairs (k, M-k)
   __shared__ C2 lds[H * CP];
   T *a = fp.p[blockIdx.z];
   const int tid = threadIdx.x;
@@ -139,28 +138,18 @@ __global__ __launch_bounds__(ROWS *M / R) void fft_z_c2r_kernel(FieldPtrs<T> fp,
     lds[k * CP + r] = (row < nrows && k < kz_cut) ? *reinterpret_cast<const C2 *>(a + row * nz_ + 2 * k) : C2{0, 0};
   }
   __syncthreads();
-  // Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) exp(-2 pi i k/NZ)
-  for (int f = tid; f < ROWS * HP; f += NT) {
-    const int p = f % HP, r = f / HP;
-    const int k = p, kk = M - p;
-    const C2 xk = lds[k * CP + r], xkk = lds[kk * CP + r];
-    {
-      const C2 cj{xkk.x, -xkk.y};
-      const C2 e = cadd(xk, cj), d = csub(xk, cj);
-      const double2 w = twr[k];
-      const C2 o = cmul(d, C2{(T)w.x, (T)-w.y});
-      lds[k * CP + r] = C2{e.x - o.y, e.y + o.x};
-    }
-    if (p > 0 && kk != k) {
-      const C2 cj{xk.x, -xk.y};
-      const C2 e = cadd(xkk, cj), d = csub(xkk, cj);
-      const double2 w = twr[kk];
-      const C2 o = cmul(d, C2{(T)w.x, (T)-w.y});
-      lds[kk * CP + r] = C2{e.x - o.y, e.y + o.x};
-    }
-  }
-  __syncthreads();
-  fft_stages<T, M, R, ROWS, CP, -1, 1>(lds, tw, tid);
+  // Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) exp(-2 pi i k/NZ),
+  // formed by the first Stockham stage from the pair in LDS
+  auto ld = [&](int k, int r, int) -> C2 {
+    const C2 xk = lds[k * CP + r], xkk = lds[(M - k) * CP + r];
+    const C2 cj{xkk.x, -xkk.y};
+    const C2 e = cadd(xk, cj), d = csub(xk, cj);
+    const double2 w = twr[k];
+    const C2 o = cmul(d, C2{(T)w.x, (T)-w.y});
+    return C2{e.x - o.y, e.y + o.x};
+  };
+  auto none = [](int, int, int, C2) {};
+  fft_stages_io<T, M, R, ROWS, CP, -1, 1, true, false>(lds, tw, tid, ld, none);
   for (int f = tid; f < ROWS * H; f += NT) {
     const int m = f % H, r = f / H;
     const long row = row0 + r;
