@@ -1394,11 +1394,19 @@ void launch_f_hydro(spm_ctx *c, int w, const T *u0, const T *u1, const T *u2, do
 
 // Calc_f_hydro_correct_precision(p, fld, jikan) on any real-space field
 // (the electrolyte step passes the f field, sp_3d_ns.cxx:97-106).
+static int f_hydro_launch(spm_ctx *c, double *const fld[3]);
 int p_calc_f_hydro_field(spm_ctx *c, double *const fld[3]) {
   if (c->np == 0) return 0;
   // anchors (x_int, residue) for the current positions
   int rc = p_build_tiles(c);
   if (rc) return rc;
+  return f_hydro_launch(c, fld);
+}
+int p_calc_f_hydro_listed(spm_ctx *c) {
+  if (c->np == 0) return 0;
+  return f_hydro_launch(c, c->u);
+}
+static int f_hydro_launch(spm_ctx *c, double *const fld[3]) {
   const double dmy = (-c->pm.dx3 * c->pm.rho) / c->prm.dt;
   tbeg(c);
   // waves per particle: about 8 waves per compute unit in all, 1 to 16 each
@@ -1486,6 +1494,10 @@ int p_velocity(spm_ctx *c, int euler) {
   if (c->np == 0) return 0;
   int rc = p_force(c);
   if (rc) return rc;
+  return p_velocity_forced(c, euler);
+}
+int p_velocity_forced(spm_ctx *c, int euler) {
+  if (c->np == 0) return 0;
   tbeg(c);
   hipLaunchKernelGGL(velocity_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pbuf, c->spec,
                      c->prm.dt, 0.5 * c->prm.dt, euler);

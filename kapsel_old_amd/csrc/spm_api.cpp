@@ -225,11 +225,42 @@ static int make_f_z(spm_ctx *c, double *const f3[3]) {
 // particles as the literal schedule does; u (a spectrum at the end of a
 // step), phi and up are rebuilt by materialize() when a spm_step call
 // returns; f is scratch.
+// The particle work of a free-motion step that reads only the previous
+// step's velocities (MD_solver_position, the tile lists at the new
+// positions, the LJ cell lists and forces), started on the context's second
+// stream after everything queued on its stream so far.  The fluid passes
+// that follow on the stream touch no particle data; fused_step joins the
+// two before Calc_f_hydro.
+static int particle_head_side(spm_ctx *c) {
+  SPM_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+  SPM_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  hipStream_t main = c->stream;
+  c->stream = c->side;  // the kernels and their timing events go to the side stream
+  int rc = p_position(c, c->ts == 0);
+  if (!rc) rc = p_build_tiles(c);
+  if (!rc) rc = p_force(c);
+  c->stream = main;
+  if (rc) return rc;
+  SPM_HIP(c, hipEventRecord(c->ev_join, c->side));
+  return 0;
+}
+
+// the rest of particle_motion once particle_head_side's work has joined
+static int particle_tail(spm_ctx *c) {
+  SPM_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  RC(p_calc_f_hydro_listed(c));
+  if (c->closure) RC(closure_apply(c));
+  RC(p_velocity_forced(c, c->ts == 0));
+  return thermostat(c);
+}
+
 int fused_step(spm_ctx *c) {
   double *const u3[3] = {c->u[0], c->u[1], c->u[2]};
   double *const p5[5] = {c->u[0], c->u[1], c->u[2], c->work[0], c->work[1]};
   double *const f3[3] = {c->f[0], c->f[1], c->f[2]};
   const double nu = c->prm.eta * (1. / c->prm.rho);
+  const bool overlap = c->motion == SPM_MOTION_FREE && c->np > 0;
+  if (overlap) RC(particle_head_side(c));
   // NS_solver_slavedEuler
   RC(fx_zeta2u(c, 0));
   RC(fft_pass_y(c, u3, 3, -1));
@@ -241,7 +272,7 @@ int fused_step(spm_ctx *c) {
   RC(fft_pass_y(c, u3, 3, -1));
   RC(fft_pass_z_c2r(c, u3, 3));
   // particles (p_calc_f_hydro also builds the tile lists at the new positions)
-  RC(particle_motion(c));
+  RC(overlap ? particle_tail(c) : particle_motion(c));
   // Make_phi_u_particle + Make_f_particle_dt_sole + Add_f_particle + U2zeta_k
   // (with no particles the tile lists stay empty and f = 0)
   RC(make_f_z(c, f3));
@@ -323,6 +354,9 @@ void free_ctx(spm_ctx *c) {
     if (c->noise_host[s]) (void)hipHostFree(c->noise_host[s]);
     if (c->noise_done[s]) (void)hipEventDestroy(c->noise_done[s]);
   }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -434,6 +468,9 @@ int setup(spm_ctx *c) {
 
   SPM_HIP(c, hipSetDevice(c->device));
   SPM_HIP(c, hipStreamCreate(&c->stream));
+  SPM_HIP(c, hipStreamCreate(&c->side));
+  SPM_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  SPM_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
 
   for (int d = 0; d < 2; ++d) RC(dalloc(c, &c->zeta[d], c->nfield));
   for (int d = 0; d < 3; ++d) {

@@ -123,6 +123,11 @@ struct spm_ctx {
   int device = 0;
   int n_cu = 256;  // compute units of the device (persistent launches)
   hipStream_t stream = nullptr;
+  // second stream of the fused step: the particle work that needs only the
+  // previous step's velocities (positions, tile lists, LJ cell lists and
+  // forces) runs on it beside the Navier-Stokes passes; fork and join events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
   size_t nfield = 0;  // elements per field
   int ts = 0;
@@ -343,6 +348,10 @@ int p_position(spm_ctx *c, int euler);
 int p_calc_f_hydro(spm_ctx *c);
 int p_force(spm_ctx *c);
 int p_velocity(spm_ctx *c, int euler);
+// the same without their first parts (tile lists; LJ forces), for a caller
+// that has run those already
+int p_calc_f_hydro_listed(spm_ctx *c);
+int p_velocity_forced(spm_ctx *c, int euler);
 int p_thermostat_apply(spm_ctx *c, const double *noise_dev);
 // noise_dev = slot *pool_counter of pool_dev, then the counter advances
 int p_noise_from_pool(spm_ctx *c);
