@@ -562,4 +562,21 @@ __device__ __forceinline__ void dma_tile(const FieldBuf &b, cplx<double> *buf, i
   for (int q = 0; q < R; ++q) dma16(b, &buf[q * NT + wave * 64], off[q]);
 }
 
+// Direct-to-LDS loads of a tile of FP32 complex values, 4 bytes per lane
+// (gfx950 has no 8-byte LDS load): dword k NT + tid of the tile, NT = C N / R
+// threads and k < 2R, is float f % (2C) of row f / (2C), f = k NT + tid, at
+// LDS float offset f.  The threads of a point are then not those that load
+// it, so a block waits for its loads (s_waitcnt vmcnt(0)) and passes a
+// barrier before the first stage reads the tile.
+template <int N, int R, int C>
+__device__ __forceinline__ void dma_tile4(const FieldBuf &b, cplx<float> *buf, int tid, const unsigned *off) {
+  constexpr int NT = C * N / R;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  float *fb = reinterpret_cast<float *>(buf);
+#pragma unroll
+  for (int k = 0; k < 2 * R; ++k)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(b.r, (lds_void *)&fb[k * NT + wave * 64], 4, off[k], 0, 0, 0);
+}
+
+
 }  // namespace spm

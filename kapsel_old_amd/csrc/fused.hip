@@ -718,17 +718,61 @@ __device__ __forceinline__ void advect_coef(const Grid &g, const ModeK &m, int f
   b = m.kx != 0 ? cf[2] : (m.ky != 0 ? cf[0] : cf[1]);
 }
 
-// x_update for FP64 fields with direct-to-LDS loads: one block per tile, two
-// LDS tile buffers, the next product spectrum in flight into one buffer while
-// the current one is transformed in the other; no VGPRs hold loads in
-// flight, so two blocks fit a CU.  Same arithmetic as x_update_kernel.
-template <int N, int R, int C>
-__global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const double *p0, const double *p1,
-                                                                   const double *p2, const double *p3,
-                                                                   const double *p4, double *z0, double *z1,
+// Byte offsets of a thread's direct-to-LDS tile loads (the same in every
+// field): FP64, its R first-stage points; FP32, its 2R dwords (dma_tile4).
+// Columns outside the field or the 2/3 box read as zero.
+template <typename T, int N, int R, int C>
+__device__ __forceinline__ void tile_offsets(const Grid &g, int t, int tid, long es, unsigned *off) {
+  const long H = g.hp;
+  auto col_ok = [&](int col) {
+    const int j = col / (int)H, kz = col - j * (int)H;
+    return col < es && col_kept(g, j, kz);
+  };
+  if constexpr (sizeof(T) == 8) {
+    const int col = t * C + tid % C, j0 = tid / C;
+    const bool ok = col_ok(col);
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+      off[q] = opaque(ok ? (unsigned)(2 * (col + (long)(j0 + q * (N / R)) * es) * (long)sizeof(T)) : 0xfffffff0u);
+  } else {
+    constexpr int NT = C * N / R;
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) {
+      const int f = k * NT + tid, row = f / (2 * C), cc = (f % (2 * C)) / 2, part = f & 1;
+      const int col = t * C + cc;
+      off[k] = opaque(col_ok(col) ? (unsigned)((2 * (col + (long)row * es) + part) * (long)sizeof(T)) : 0xfffffff0u);
+    }
+  }
+}
+
+template <typename T, int N, int R, int C>
+__device__ __forceinline__ void dma_any(const FieldBuf &b, cplx<T> *buf, int tid, const unsigned *off) {
+  if constexpr (sizeof(T) == 8)
+    dma_tile<N, R, C>(b, buf, tid, off);
+  else
+    dma_tile4<N, R, C>(b, buf, tid, off);
+}
+
+// Before a transform reads a tile an FP32 block loaded: every wave's loads
+// have landed (its loads in flight are this tile's only) and every wave has
+// passed the barrier.  FP64 tiles are read by the lanes that loaded them.
+template <typename T>
+__device__ __forceinline__ void tile_landed() {
+  if constexpr (sizeof(T) == 4) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    lds_barrier();
+  }
+}
+
+// x_update with direct-to-LDS loads: one block per tile, two LDS tile
+// buffers, the next product spectrum in flight into one buffer while the
+// current one is transformed in the other; no VGPRs hold loads in flight,
+// so two blocks (FP64) fit a CU.  Same arithmetic as x_update_kernel.
+template <typename T, int N, int R, int C>
+__global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const T *p0, const T *p1, const T *p2,
+                                                                   const T *p3, const T *p4, T *z0, T *z1,
                                                                    double nu, double dt,
                                                                    const double2 *__restrict__ tw_g) {
-  using T = double;
   using C2 = cplx<T>;
   __shared__ C2 bufA[N * C];
   __shared__ C2 bufB[N * C];
@@ -746,12 +790,9 @@ __global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const
   const size_t nel = 2 * (size_t)g.nx * es;
   const FieldBuf bp[5] = {field_buf(p0, nel), field_buf(p1, nel), field_buf(p2, nel), field_buf(p3, nel),
                           field_buf(p4, nel)};
-  // byte offsets of the thread's first-stage points (the same in every field)
-  unsigned off[R];
-#pragma unroll
-  for (int q = 0; q < R; ++q)
-    off[q] = opaque(col_ok ? (unsigned)(2 * (col + (long)(j0 + q * (N / R)) * es) * (long)sizeof(T)) : 0xfffffff0u);
-  dma_tile<N, R, C>(bp[0], bufA, tid, off);
+  unsigned off[sizeof(T) == 8 ? R : 2 * R];
+  tile_offsets<T, N, R, C>(g, t, tid, es, off);
+  dma_any<T, N, R, C>(bp[0], bufA, tid, off);
   const FieldBuf bz0 = field_buf<T>(z0, nel), bz1 = field_buf<T>(z1, nel);
   C2 acc0[R], acc1[R], za[R], zb[R];
 #pragma unroll
@@ -762,7 +803,7 @@ __global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const
     C2 *oth = f % 2 == 0 ? bufB : bufA;
     auto hook = [&]() {
       if (f < 4) {
-        dma_tile<N, R, C>(bp[f + 1], oth, tid, off);
+        dma_any<T, N, R, C>(bp[f + 1], oth, tid, off);
       } else {
         // zeta at the modes this thread updates, loaded while the last
         // spectrum is transformed
@@ -785,6 +826,7 @@ __global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const
       acc1[q].y -= b * P.y;
       __builtin_amdgcn_sched_barrier(0);  // points in turn: fewer live registers
     };
+    tile_landed<T>();
     fft_stages_lds<T, N, R, C, +1, 1>(buf, tw, tid, hook, st);
     pin_acc<R>(acc0, acc1);
   }
@@ -942,12 +984,10 @@ __device__ __forceinline__ void project_coef(const Grid &g, const ModeK &m, int 
 // x_project for FP64 fields with direct-to-LDS loads (see
 // x_update_dma_kernel): the three components of f through two LDS buffers,
 // one block per tile.  Same arithmetic as x_project_kernel.
-template <int N, int R, int C>
-__global__ __launch_bounds__(C *N / R, 4) void x_project_dma_kernel(Grid g, const double *f0, const double *f1,
-                                                                    const double *f2, double *z0, double *z1,
-                                                                    double *uk_dc,
+template <typename T, int N, int R, int C>
+__global__ __launch_bounds__(C *N / R, 4) void x_project_dma_kernel(Grid g, const T *f0, const T *f1, const T *f2,
+                                                                    T *z0, T *z1, double *uk_dc,
                                                                     const double2 *__restrict__ tw_g) {
-  using T = double;
   using C2 = cplx<T>;
   __shared__ C2 bufA[N * C];
   __shared__ C2 bufB[N * C];
@@ -965,11 +1005,9 @@ __global__ __launch_bounds__(C *N / R, 4) void x_project_dma_kernel(Grid g, cons
   const bool col_ok = col < es && col_kept(g, j, kz);
   const size_t nel = 2 * (size_t)g.nx * es;
   const FieldBuf bf[3] = {field_buf(f0, nel), field_buf(f1, nel), field_buf(f2, nel)};
-  unsigned off[R];
-#pragma unroll
-  for (int q = 0; q < R; ++q)
-    off[q] = opaque(col_ok ? (unsigned)(2 * (col + (long)(j0 + q * (N / R)) * es) * (long)sizeof(T)) : 0xfffffff0u);
-  dma_tile<N, R, C>(bf[0], bufA, tid, off);
+  unsigned off[sizeof(T) == 8 ? R : 2 * R];
+  tile_offsets<T, N, R, C>(g, t, tid, es, off);
+  dma_any<T, N, R, C>(bf[0], bufA, tid, off);
   const FieldBuf bz0 = field_buf<T>(z0, nel), bz1 = field_buf<T>(z1, nel);
   C2 acc0[R], acc1[R], za[R], zb[R];
   double f00[3] = {0.0, 0.0, 0.0};
@@ -981,7 +1019,7 @@ __global__ __launch_bounds__(C *N / R, 4) void x_project_dma_kernel(Grid g, cons
     C2 *oth = f % 2 == 0 ? bufB : bufA;
     auto hook = [&]() {
       if (f < 2) {
-        dma_tile<N, R, C>(bf[f + 1], oth, tid, off);
+        dma_any<T, N, R, C>(bf[f + 1], oth, tid, off);
       } else {
 #pragma unroll
         for (int q = 0; q < R; ++q) {
@@ -1003,6 +1041,7 @@ __global__ __launch_bounds__(C *N / R, 4) void x_project_dma_kernel(Grid g, cons
       acc1[q].x += b * F.y;
       acc1[q].y -= b * F.x;
     };
+    tile_landed<T>();
     fft_stages_lds<T, N, R, C, +1, 1>(buf, tw, tid, hook, st);
     pin_acc<R>(acc0, acc1);
   }
@@ -1079,10 +1118,13 @@ inline T *F(double *p) {
 #ifndef SPM_ZADV_IO
 #define SPM_ZADV_IO 1
 #endif
-// FP64 x_update / x_project through direct-to-LDS loads (0: the register-
-// prefetch kernels)
+// x_update / x_project through direct-to-LDS loads (0: the register-prefetch
+// kernels)
 #ifndef SPM_X_DMA_F64
 #define SPM_X_DMA_F64 1
+#endif
+#ifndef SPM_X_DMA_F32
+#define SPM_X_DMA_F32 1
 #endif
 inline int x_grid(const spm_ctx *c, int ntiles, int per_cu) {
   const int n = per_cu > 0 ? per_cu * c->n_cu : ntiles;
@@ -1127,10 +1169,10 @@ int fx_update_t(spm_ctx *c, double nu, double dt) {
     constexpr int R = kFusedR(S), C = kFusedC(S);
     const int ntiles = (g.ny * g.hp + C - 1) / C;
     constexpr int per_cu = sizeof(T) == 8 ? SPM_X_UPDATE_BLOCKS_F64 : SPM_X_UPDATE_BLOCKS_F32;
-    if constexpr (sizeof(T) == 8 && SPM_X_DMA_F64) {
-      hipLaunchKernelGGL((x_update_dma_kernel<S, R, C>), dim3(ntiles), dim3(C * S / R), 0, c->stream, g,
-                         c->u[0], c->u[1], c->u[2], c->work[0], c->work[1], c->zeta[0], c->zeta[1], nu, dt,
-                         c->tw_x);
+    if constexpr (sizeof(T) == 8 ? SPM_X_DMA_F64 : SPM_X_DMA_F32) {
+      hipLaunchKernelGGL((x_update_dma_kernel<T, S, R, C>), dim3(ntiles), dim3(C * S / R), 0, c->stream, g,
+                         F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
+                         F<T>(c->zeta[0]), F<T>(c->zeta[1]), nu, dt, c->tw_x);
       return 0;
     }
     hipLaunchKernelGGL((x_update_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->u[0]),
@@ -1147,9 +1189,10 @@ int fx_project_t(spm_ctx *c) {
     constexpr int R = kFusedR(S), C = kFusedC(S);
     const int ntiles = (g.ny * g.hp + C - 1) / C;
     constexpr int per_cu = sizeof(T) == 8 ? SPM_X_PROJECT_BLOCKS_F64 : SPM_X_PROJECT_BLOCKS_F32;
-    if constexpr (sizeof(T) == 8 && SPM_X_DMA_F64) {
-      hipLaunchKernelGGL((x_project_dma_kernel<S, R, C>), dim3(ntiles), dim3(C * S / R), 0, c->stream, g,
-                         c->f[0], c->f[1], c->f[2], c->zeta[0], c->zeta[1], c->uk_dc, c->tw_x);
+    if constexpr (sizeof(T) == 8 ? SPM_X_DMA_F64 : SPM_X_DMA_F32) {
+      hipLaunchKernelGGL((x_project_dma_kernel<T, S, R, C>), dim3(ntiles), dim3(C * S / R), 0, c->stream, g,
+                         F<T>(c->f[0]), F<T>(c->f[1]), F<T>(c->f[2]), F<T>(c->zeta[0]), F<T>(c->zeta[1]), c->uk_dc,
+                         c->tw_x);
       return 0;
     }
     hipLaunchKernelGGL((x_project_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->f[0]),
