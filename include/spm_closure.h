@@ -5,7 +5,7 @@
  * A small MLP corrects the hydrodynamic force and torque that
  * Calc_f_hydro_correct_precision (md_force.cxx:263-389) computes on a coarse
  * grid, so that fewer grid points per radius are needed; the plain method on
- * a fine grid is the fallback and the judge.  Per particle it reads 114
+ * a fine grid is the fallback and the judge.  Per particle it reads 126
  * inputs and returns 6 outputs:
  *
  *   inputs   [0, 3)     relative velocity: the phi-weighted mean of
@@ -17,13 +17,18 @@
  *                       trilinear in the grid
  *            [84, 111)  phi(x_p + a o): the particles' profiles summed at
  *                       the same points, clamped at 1
- *            [111, 114) gap vector to the nearest other particle: the centre
- *                       separation r times (1 - 2a / |r|), minimum image
- *                       (zero without another particle)
+ *            [111, 114) gap vector to the nearest other particle j: the
+ *                       centre separation r times (1 - 2a / |r|), minimum
+ *                       image
+ *            [114, 117) its relative velocity v_j - v_p
+ *            [117, 120) its relative angular velocity w_j - w_p
+ *            [120, 123) gap vector to the second nearest particle k
+ *            [123, 126) its relative velocity v_k - v_p
+ *                       (zeros where there is no such particle)
  *   outputs  [0, 3) force correction, [3, 6) torque correction
  *
  * with x_p the particle's centre and a its radius.  The network is
- * 114 (padded to 128) -> 128 -> 128 -> 128 -> 6 with exact GELU between the
+ * 126 (padded to 128) -> 128 -> 128 -> 128 -> 6 with exact GELU between the
  * layers, BF16 weights and activations into the matrix cores and FP32
  * accumulation; inputs are normalised as (x - shift) * scale and outputs
  * multiplied by out_scale.  FP32 master weights, trained with Adam on the
@@ -31,7 +36,7 @@
  *
  * Parameter layout of spm_closure_get/put_params (SPM_CLOSURE_NPARAM
  * floats): W1 [128 in][128 out], b1[128], W2, b2, W3, b3 (the same),
- * W4 [128 in][16 out], b4[16]; input rows 114..127 of W1 and output
+ * W4 [128 in][16 out], b4[16]; input rows 126..127 of W1 and output
  * columns 6..15 of W4 / b4 are padding and stay zero.
  */
 #ifndef SPM_CLOSURE_H
@@ -45,7 +50,7 @@
 extern "C" {
 #endif
 
-#define SPM_CLOSURE_IN 114
+#define SPM_CLOSURE_IN 126
 #define SPM_CLOSURE_OUT 6
 #define SPM_CLOSURE_NPARAM 51600
 
@@ -60,17 +65,17 @@ int spm_closure_destroy(spm_closure *cl);
 int spm_closure_init(spm_closure *cl, uint64_t seed);
 int spm_closure_get_params(spm_closure *cl, float *host);
 int spm_closure_put_params(spm_closure *cl, const float *host);
-/* Input normalisation and output scale (host arrays of 114, 114, 6). */
+/* Input normalisation and output scale (host arrays of 126, 126, 6). */
 int spm_closure_set_scaling(spm_closure *cl, const float *in_shift, const float *in_scale, const float *out_scale);
 int spm_closure_get_scaling(spm_closure *cl, float *in_shift, float *in_scale, float *out_scale);
-/* y[n][6] = net(x[n][114]) for host arrays, n <= max_batch. */
+/* y[n][6] = net(x[n][126]) for host arrays, n <= max_batch. */
 int spm_closure_forward(spm_closure *cl, const float *x, float *y, int n);
 /* Gradient of the mean squared error (in normalised outputs, averaged over
  * the n samples and 6 outputs) with respect to every parameter, for host
  * arrays (test hook for the backward kernels); grad holds
  * SPM_CLOSURE_NPARAM floats, loss one. */
 int spm_closure_gradient(spm_closure *cl, const float *x, const float *target, int n, float *grad, float *loss);
-/* Training set held on the device: x[n][114], target[n][6] (physical
+/* Training set held on the device: x[n][126], target[n][6] (physical
  * units; divided by out_scale inside). */
 int spm_closure_set_data(spm_closure *cl, const float *x, const float *target, int n);
 /* `steps` Adam steps (lr, beta1 0.9, beta2 0.999, eps 1e-8) over
@@ -84,9 +89,9 @@ int spm_closure_train(spm_closure *cl, int steps, int batch, float lr, float *lo
  * keeps the pointer; the network must outlive the attachment.  n_particles
  * must not exceed the network's max_batch. */
 int spm_set_closure(spm_ctx *ctx, spm_closure *cl);
-/* The 114 inputs of every particle for the context's current real-space u,
+/* The 126 inputs of every particle for the context's current real-space u,
  * particles and f_hydro (as the step leaves them after
- * spm_calc_f_hydro / spm_zeta_k2u; host array [n_particles][114]). */
+ * spm_calc_f_hydro / spm_zeta_k2u; host array [n_particles][126]). */
 int spm_closure_features(spm_ctx *ctx, float *x);
 
 /* Particle motion of the step (no counterpart in the reference):

@@ -62,7 +62,7 @@ def resistance_run(n_grid, a, pos, v, w, dt, steps, precision=abi.FP64, closure=
 
 
 def pair(n_grid, a, pos, v, w, dt, steps, refine=4):
-    """One coarse/fine pair: coarse inputs x [n][114] and the target
+    """One coarse/fine pair: coarse inputs x [n][126] and the target
     correction t [n][6] = (F_fine - F_coarse, T_fine / refine - T_coarse)."""
     fc, tc, x = resistance_run(n_grid, a, pos, v, w, dt, steps, with_features=True)
     r = refine
@@ -147,8 +147,12 @@ def transform(x, t, S):
     xn[:, 0:3] = x[:, 0:3] @ S.T
     xn[:, 3:84] = (x[:, 3:84].reshape(n, 27, 3)[:, idx, :] @ S.T).reshape(n, 81)
     xn[:, 84:111] = x[:, 84:111][:, idx]
-    xn[:, 111:114] = x[:, 111:114] @ S.T
     det = np.rint(np.linalg.det(S))
+    xn[:, 111:114] = x[:, 111:114] @ S.T  # nearest neighbour: gap, velocity, angular velocity
+    xn[:, 114:117] = x[:, 114:117] @ S.T
+    xn[:, 117:120] = det * (x[:, 117:120] @ S.T)
+    xn[:, 120:123] = x[:, 120:123] @ S.T  # second nearest: gap, velocity
+    xn[:, 123:126] = x[:, 123:126] @ S.T
     tn = np.empty_like(t)
     tn[:, 0:3] = t[:, 0:3] @ S.T
     tn[:, 3:6] = det * (t[:, 3:6] @ S.T)
@@ -249,10 +253,10 @@ def mlp_workload(batch=65536, train_steps=40, minibatch=8192, forwards=10, seed=
     samples."""
     from .spm import Closure
     rng = np.random.default_rng(seed)
-    x = rng.standard_normal((batch, 114)).astype(np.float32)
+    x = rng.standard_normal((batch, 126)).astype(np.float32)
     t = (0.1 * rng.standard_normal((batch, 6))).astype(np.float32)
     net = Closure(batch, seed=3)
-    net.set_scaling(np.zeros(114), np.ones(114), np.ones(6))
+    net.set_scaling(np.zeros(126), np.ones(126), np.ones(6))
     net.set_data(x, t)
     loss = net.train(train_steps, minibatch, 1e-3)
     for _ in range(forwards):

@@ -512,9 +512,15 @@ __global__ void closure_samples_kernel(Grid g, PModel pm, const double *__restri
     for (int d = 0; d < 3; ++d) xr[d] = (float)(slip_scale * phi::P(pb, pm.np, P_F_HYDRO + d, n));
 }
 
-// gap vector to the nearest other particle (minimum image): one wavefront
-// per particle, lanes striding over the others, then a shuffle reduction
-// keeping the smallest distance and, among equal ones, the lowest index
+// The two nearest other particles (minimum image): one wavefront per
+// particle, lanes striding over the others keeping their two nearest, then a
+// shuffle reduction merging the pairs (smaller distance first, the lower
+// index among equal ones).  Writes, for the nearest, the gap vector (the
+// centre separation r times 1 - 2a/|r|), the relative velocity v_j - v_n and
+// angular velocity w_j - w_n; for the second nearest, the gap vector and
+// relative velocity; zeros where there is no such particle.
+__device__ __forceinline__ bool nearer(double d, int i, double e, int j) { return d < e || (d == e && i < j); }
+
 __global__ __launch_bounds__(256) void closure_gap_kernel(Grid g, PModel pm, const double *__restrict__ pb,
                                                           float *x) {
   const int lane = threadIdx.x & 63;
@@ -523,8 +529,8 @@ __global__ __launch_bounds__(256) void closure_gap_kernel(Grid g, PModel pm, con
   const double L[3] = {g.lx, g.ly, g.lz};
   const double iL[3] = {1.0 / g.lx, 1.0 / g.ly, 1.0 / g.lz};
   const double xn[3] = {phi::P(pb, pm.np, P_X, n), phi::P(pb, pm.np, P_X + 1, n), phi::P(pb, pm.np, P_X + 2, n)};
-  double best = 1e300;
-  int bi = pm.np;
+  double b1 = 1e300, b2 = 1e300;
+  int i1 = pm.np, i2 = pm.np;
   for (int m = lane; m < pm.np; m += 64) {
     if (m == n) continue;
     double r2 = 0.0;
@@ -533,32 +539,50 @@ __global__ __launch_bounds__(256) void closure_gap_kernel(Grid g, PModel pm, con
       r -= L[d] * rint(r * iL[d]);
       r2 += r * r;
     }
-    if (r2 < best) {
-      best = r2;
-      bi = m;
+    if (nearer(r2, m, b1, i1)) {
+      b2 = b1;
+      i2 = i1;
+      b1 = r2;
+      i1 = m;
+    } else if (nearer(r2, m, b2, i2)) {
+      b2 = r2;
+      i2 = m;
     }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    const double ob = __shfl_xor(best, off, 64);
-    const int oi = __shfl_xor(bi, off, 64);
-    if (ob < best || (ob == best && oi < bi)) {
-      best = ob;
-      bi = oi;
+    const double o1 = __shfl_xor(b1, off, 64), o2 = __shfl_xor(b2, off, 64);
+    const int j1 = __shfl_xor(i1, off, 64), j2 = __shfl_xor(i2, off, 64);
+    // the two nearest of {(b1, i1), (b2, i2), (o1, j1), (o2, j2)}, each pair sorted
+    if (nearer(o1, j1, b1, i1)) {
+      const bool keep_b1 = nearer(b1, i1, o2, j2);
+      b2 = keep_b1 ? b1 : o2;
+      i2 = keep_b1 ? i1 : j2;
+      b1 = o1;
+      i1 = j1;
+    } else if (nearer(o1, j1, b2, i2)) {
+      b2 = o1;
+      i2 = j1;
     }
   }
   if (lane != 0) return;
   float *xr = x + (long)n * mlp::IN;
-  if (bi >= pm.np) {
-    for (int d = 0; d < 3; ++d) xr[111 + d] = 0.f;
-    return;
-  }
-  const double f = 1.0 - 2.0 * pm.radius / sqrt(best);
-  for (int d = 0; d < 3; ++d) {
-    double r = phi::P(pb, pm.np, P_X + d, bi) - xn[d];
-    r -= L[d] * rint(r * iL[d]);
-    xr[111 + d] = (float)(r * f);
-  }
+  auto put = [&](int j, double dist2, int at, bool omega) {
+    if (j >= pm.np) {
+      for (int d = 0; d < (omega ? 9 : 6); ++d) xr[at + d] = 0.f;
+      return;
+    }
+    const double f = 1.0 - 2.0 * pm.radius / sqrt(dist2);
+    for (int d = 0; d < 3; ++d) {
+      double r = phi::P(pb, pm.np, P_X + d, j) - xn[d];
+      r -= L[d] * rint(r * iL[d]);
+      xr[at + d] = (float)(r * f);
+      xr[at + 3 + d] = (float)(phi::P(pb, pm.np, P_V + d, j) - phi::P(pb, pm.np, P_V + d, n));
+      if (omega) xr[at + 6 + d] = (float)(phi::P(pb, pm.np, P_OMEGA + d, j) - phi::P(pb, pm.np, P_OMEGA + d, n));
+    }
+  };
+  put(i1, b1, 111, true);
+  put(i2, b2, 120, false);
 }
 
 // f_hydro += y[0..3), torque_hydro += y[3..6) (torque only with rotation,
@@ -763,7 +787,7 @@ int closure_apply(spm_ctx *c) {
                        c->prm.a * c->prm.dx, slip_scale(c), cl->x);
   hipLaunchKernelGGL(closure_gap_kernel, dim3((c->np + 3) / 4), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
                      cl->x);
-  tend(c, SPM_K_PARTICLE, 4.0 * 114 * c->np);
+  tend(c, SPM_K_PARTICLE, 4.0 * mlp::IN * c->np);
   SPM_LAUNCH_CHECK(c);
   // the network runs on the context's stream
   hipStream_t keep = cl->stream;
@@ -877,11 +901,11 @@ int spm_closure_get_params(spm_closure *cl, float *host) {
 int spm_closure_put_params(spm_closure *cl, const float *host) {
   if (!cl || !host) return SPM_ERR_INVALID;
   using namespace spm::mlp;
-  // padding must stay zero: inputs beyond 114, outputs beyond 6
+  // padding must stay zero: inputs beyond SPM_CLOSURE_IN, outputs beyond 6
   for (int i = IN; i < W; ++i)
     for (int o = 0; o < W; ++o)
       if (host[OFF_W[0] + i * W + o] != 0.f) {
-        cl->err = "W1 rows beyond input 114 must be zero";
+        cl->err = "W1 rows beyond the inputs must be zero";
         return SPM_ERR_INVALID;
       }
   for (int i = 0; i < W; ++i)

@@ -235,8 +235,8 @@ class SPM:
         self._call("set_closure", closure._h if closure is not None else None)
 
     def closure_features(self):
-        """The closure's 114 inputs of every particle, float32 [n][114]."""
-        x = np.zeros((self.n_particles, 114), dtype=np.float32)
+        """The closure's inputs of every particle, float32 [n][Closure.IN]."""
+        x = np.zeros((self.n_particles, Closure.IN), dtype=np.float32)
         self._call("closure_features", x.ctypes.data_as(C.POINTER(C.c_float)))
         return x
 
@@ -395,11 +395,11 @@ def _fp(a):
 
 class Closure:
     """The learned force/torque closure's network on one GPU (an
-    spm_closure): 114 inputs -> 128 -> 128 -> 128 -> 6, BF16 weights on the
+    spm_closure): 126 inputs -> 128 -> 128 -> 128 -> 6, BF16 weights on the
     matrix cores with FP32 accumulation, trained with Adam on the GPU."""
 
     NPARAM = 51600
-    IN, OUT = 114, 6
+    IN, OUT = 126, 6
 
     def __init__(self, max_batch, device=0, seed=0):
         h = C.c_void_p()
@@ -440,7 +440,7 @@ class Closure:
         self._call("set_scaling", _fp(a), _fp(b), _fp(c))
 
     def get_scaling(self):
-        a, b, c = np.zeros(114, np.float32), np.zeros(114, np.float32), np.zeros(6, np.float32)
+        a, b, c = np.zeros(self.IN, np.float32), np.zeros(self.IN, np.float32), np.zeros(6, np.float32)
         self._call("get_scaling", _fp(a), _fp(b), _fp(c))
         return a, b, c
 

@@ -1,5 +1,5 @@
 """CPU restatement of the learned closure's network (include/spm_closure.h):
-114 (padded to 128) -> 128 -> 128 -> 128 -> 6 with exact GELU, its
+126 (padded to 128) -> 128 -> 128 -> 128 -> 6 with exact GELU, its
 mean-squared-error gradient and the Adam update.
 
 TEST INFRASTRUCTURE ONLY: used by tests/ to check the HIP kernels
@@ -16,7 +16,7 @@ import math
 
 import numpy as np
 
-IN, OUT, W, OUTP = 114, 6, 128, 16
+IN, OUT, W, OUTP = 126, 6, 128, 16
 NOUT = [W, W, W, OUTP]
 OFF_W = [0, W * W + W, 2 * (W * W + W), 3 * (W * W + W)]
 OFF_B = [W * W, 2 * W * W + W, 3 * W * W + 2 * W, 3 * (W * W + W) + W * OUTP]

@@ -199,8 +199,12 @@ def test_features_match_numpy(SPM):
         r -= n * np.rint(r / n)
         dist = np.linalg.norm(r, axis=1)
         dist[m] = np.inf
-        k = int(np.argmin(dist))
+        k, k2 = [int(i) for i in np.argsort(dist, kind="stable")[:2]]
         assert np.allclose(x[m, 111:114], r[k] * (1 - 2 * a / dist[k]), rtol=1e-6, atol=1e-6)
+        assert np.allclose(x[m, 114:117], q["v"][k] - q["v"][m], rtol=1e-6, atol=1e-9)
+        assert np.allclose(x[m, 117:120], q["omega"][k] - q["omega"][m], rtol=1e-6, atol=1e-9)
+        assert np.allclose(x[m, 120:123], r[k2] * (1 - 2 * a / dist[k2]), rtol=1e-6, atol=1e-6)
+        assert np.allclose(x[m, 123:126], q["v"][k2] - q["v"][m], rtol=1e-6, atol=1e-9)
 
 
 def test_hook_adds_the_network_output(SPM, Closure):
@@ -257,3 +261,55 @@ def test_closure_lowers_coarse_drag_error(SPM, Closure):
     print("32^3 vs 128^3: %d training samples, median drag error %.4f without, %.4f with the closure"
           % (len(x), e0, e1))
     assert e1 < 0.5 * e0
+
+
+def test_features_turn_with_the_lattice_symmetries(SPM):
+    """The inputs of a configuration mapped by a lattice symmetry S (positions
+    S x mod L, velocities S v, angular velocities det(S) S w, the flow
+    u'(S x) = S u(x)) equal closure.transform of the original inputs: the
+    permutation of the 27 samples, vectors turned with S and the relative
+    angular velocity (a pseudovector) with det(S) S.  This pins the
+    augmentation that multiplies the training set by 48."""
+    from kapsel_old_amd import abi
+    from kapsel_old_amd import closure as cl
+    from tests.helpers import particles, smooth_field
+    n, a, npart = 32, 3.0, 10
+    prm = abi.make_params(n, a=a, n_particles=npart, rotation=1)
+    p = particles(npart, (n, n, n), a, seed=8, vel=0.02, omega=0.005, min_factor=2.2)
+    g = SPM(prm)
+    shape = (n, n, g.get_field("u0").shape[2])
+    us = [smooth_field(shape, n, seed=60 + d, kmax=3, scale=0.05) for d in range(3)]
+    S = np.array([[0, 1, 0], [0, 0, -1], [-1, 0, 0]], dtype=float)
+    det = np.rint(np.linalg.det(S))
+
+    def features(parts, fields):
+        h = SPM(prm)
+        h.put_particles(parts)
+        for d in range(3):
+            h.put_field("u%d" % d, fields[d])
+        h.Calc_f_hydro_correct_precision()
+        x = h.closure_features()
+        h.close()
+        return x
+
+    x0 = features(p, us)
+    # the mapped configuration
+    q = p.copy()
+    q["x"] = (p["x"] @ S.T) % n
+    q["x_nopbc"] = q["x"]
+    q["v"] = p["v"] @ S.T
+    q["omega"] = det * (p["omega"] @ S.T)
+    coords = np.indices((n, n, n)).reshape(3, -1)
+    moved = ((S @ coords) % n).astype(int)
+    U = np.stack([u[:, :, :n].reshape(-1) for u in us])
+    U2 = S @ U
+    vs = []
+    for d in range(3):
+        v = np.zeros(shape)
+        v[moved[0], moved[1], moved[2]] = U2[d]
+        vs.append(v)
+    x1 = features(q, vs)
+    xt, _ = cl.transform(x0, np.zeros((npart, 6), np.float32), S)
+    err = np.abs(x1 - xt).max() / np.abs(xt).max()
+    print("inputs of the mapped configuration vs transformed inputs: rel err %.2e" % err)
+    assert err < 1e-5
