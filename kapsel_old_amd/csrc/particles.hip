@@ -679,15 +679,36 @@ __device__ __forceinline__ double lj_f(const PModel &pm, double x) {
 // cells around the particle's own, each visited once even when fewer than
 // three span an axis) and sums its pairs in list order; the 64 lane sums are
 // combined by a fixed shuffle tree, so the result is run-to-run identical.
+// With all_pairs (few particles) the lanes stride over every other particle
+// instead of the neighbour cells: the same pairs within the cutoff (the
+// nearest image, each once), without the cell lists' five launches.
 __global__ __launch_bounds__(256) void force_kernel(Grid g, PModel pm, double *pb, const int *__restrict__ spec,
                                                     const int *__restrict__ pcell, const int *__restrict__ cell_start,
                                                     const int *__restrict__ cell_count,
-                                                    const int *__restrict__ cell_list, int cx, int cy, int cz) {
+                                                    const int *__restrict__ cell_list, int cx, int cy, int cz,
+                                                    int all_pairs) {
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (i >= pm.np) return;  // whole wavefronts leave together
   double F[3] = {0.0, 0.0, 0.0};
-  if (pm.lj_on) {
+  if (pm.lj_on && all_pairs) {
+    const double xi[3] = {P(pb, pm.np, P_X, i), P(pb, pm.np, P_X + 1, i), P(pb, pm.np, P_X + 2, i)};
+    for (int j = lane; j < pm.np; j += 64) {
+      if (j == i) continue;
+      const double xj[3] = {P(pb, pm.np, P_X, j), P(pb, pm.np, P_X + 1, j), P(pb, pm.np, P_X + 2, j)};
+      double rv[3];
+      const double r = distance0(g, xi, xj, rv);
+      if (r < pm.lj_cutoff) {
+        const double ljf = lj_f(pm, r);
+        const double cap = 1.79769313486231570815e+308 / r;
+        const double dmy = cap < ljf ? cap : ljf;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) F[d] += dmy * -rv[d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[d] = wave_sum(F[d]);
+  } else if (pm.lj_on) {
     const int ci = pcell[i];
     const int c[3] = {ci / (cy * cz), (ci / cz) % cy, ci % cz};
     const int nc[3] = {cx, cy, cz};
@@ -1469,10 +1490,13 @@ int p_surface_normal(spm_ctx *c) {  // FP64 normals whatever the field type
   return 0;
 }
 
+// particles up to which the LJ forces take all pairs rather than cell lists
+constexpr int kPairsMax = 256;
 int p_force(spm_ctx *c) {
   if (c->np == 0) return 0;
   tbeg(c);
-  if (c->pm.lj_on) {
+  const int all_pairs = c->np <= kPairsMax;
+  if (c->pm.lj_on && !all_pairs) {
     hipLaunchKernelGGL(izero_kernel, dim3(blocks(c->ncells, 256)), dim3(256), 0, c->stream, c->cell_count, c->ncells);
     hipLaunchKernelGGL(cell_count_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->grid, c->pm, c->pbuf,
                        c->cells[0], c->cells[1], c->cells[2], c->cell_count, c->pcell);
@@ -1484,7 +1508,8 @@ int p_force(spm_ctx *c) {
                        c->cell_count, c->cell_list, c->ncells);
   }
   hipLaunchKernelGGL(force_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
-                     c->pcell, c->cell_start, c->cell_count, c->cell_list, c->cells[0], c->cells[1], c->cells[2]);
+                     c->pcell, c->cell_start, c->cell_count, c->cell_list, c->cells[0], c->cells[1], c->cells[2],
+                     all_pairs);
   tend(c, SPM_K_PARTICLE, 8.0 * 6 * c->np);
   SPM_LAUNCH_CHECK(c);
   return 0;
