@@ -768,11 +768,12 @@ __device__ __forceinline__ void tile_landed() {
 // buffers, the next product spectrum in flight into one buffer while the
 // current one is transformed in the other; no VGPRs hold loads in flight,
 // so two blocks (FP64) fit a CU.  Same arithmetic as x_update_kernel.
-template <typename T, int N, int R, int C>
+template <typename T, int N, int R, int C, bool ZU>
 __global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const T *p0, const T *p1, const T *p2,
                                                                    const T *p3, const T *p4, T *z0, T *z1,
                                                                    double nu, double dt,
-                                                                   const double2 *__restrict__ tw_g) {
+                                                                   const double2 *__restrict__ tw_g, T *u0, T *u1,
+                                                                   T *u2, double *uk_dc, int fix_mask) {
   using C2 = cplx<T>;
   __shared__ C2 bufA[N * C];
   __shared__ C2 bufB[N * C];
@@ -781,6 +782,12 @@ __global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const
   const int t = fused_x_tile<T>();
   if (!tile_kept<C>(g, t * C)) return;  // zeta stays zero outside the 2/3 box
   for (int i = tid; i < N; i += blockDim.x) tw[i] = tw_g[i];
+  // uk_dc as it was on entry (only the block holding mode 0 uses it, and
+  // it clears the FIX_CELL components after every thread has read them)
+  double dc_in[3] = {0.0, 0.0, 0.0};
+  if constexpr (ZU)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dc_in[d] = uk_dc[d];
   const int c = tid % C, j0 = tid / C;
   const long H = g.hp;
   const long es = (long)g.ny * H;
@@ -830,7 +837,11 @@ __global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const
     fft_stages_lds<T, N, R, C, +1, 1>(buf, tw, tid, hook, st);
     pin_acc<R>(acc0, acc1);
   }
-  if (!col_ok) return;
+  if constexpr (ZU) {
+    lds_barrier();  // the last transform has read bufA
+  } else {
+    if (!col_ok) return;
+  }
   const T dmy0 = (T)(-nu * dt), dmy1 = (T)(1. / nu);
 #pragma unroll
   for (int q = 0; q < R; ++q) {
@@ -838,6 +849,10 @@ __global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const
     __builtin_amdgcn_sched_barrier(0);
     const int i = opaque_after_memory(last_stage_point<N, R>(j0, q));
     const ModeK m = mode_k(g, i, j, kz);
+    if (ZU && !(col_ok && m.keep)) {  // zero, as Zeta_k2u_k reads it there
+      bufA[i * C + c] = bufB[i * C + c] = C2{0, 0};
+      continue;
+    }
     if (!m.keep) continue;  // zeta stays zero outside the 2/3 box
     const long e = 2 * (col + (long)i * es);
     const T kk2 = (T)(sq(g.wave_x * (double)m.kx) + sq(g.wave_y * (double)m.ky) + sq(g.wave_z * (double)m.kz));
@@ -850,6 +865,49 @@ __global__ __launch_bounds__(C *N / R, 4) void x_update_dma_kernel(Grid g, const
     b.y += dmy * (b.y - dmy1 * ik2 * acc1[q].y);
     *reinterpret_cast<C2 *>(z0 + e) = a;
     *reinterpret_cast<C2 *>(z1 + e) = b;
+    if constexpr (ZU) {
+      bufA[i * C + c] = a;
+      bufB[i * C + c] = b;
+    }
+  }
+  if constexpr (ZU) {
+    // FIX_CELL, truncation and Zeta_k2u_k on the new zeta, then the inverse
+    // x transforms of the three velocity spectra, as x_zeta2u_kernel does
+    // from zeta in memory: the new zeta moves from the forward transform's
+    // last-stage points to the inverse one's first-stage points through LDS
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = j0 + q * (N / R);
+      za[q] = bufA[i * C + c];
+      zb[q] = bufB[i * C + c];
+    }
+    lds_barrier();
+    const bool has0 = t * C == 0;  // the tile holding column (j, kz) = (0, 0)
+    double dc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dc[d] = (fix_mask & (1 << d)) ? 0.0 : dc_in[d];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      pin_acc<R>(za, zb);
+      T *out = d == 0 ? u0 : (d == 1 ? u1 : u2);
+      auto ld = [&](int i, int, int q) -> C2 {
+        const ModeK m = mode_k(g, i, j, kz);
+        C2 v = zeta2u_mode<T>(g, m, za[q], zb[q], d);
+        if (has0 && i == 0 && c == 0) v.x = (T)dc[d];
+        return v;
+      };
+      auto st = [&](int i, int, int, C2 v) {
+        if (col_ok) *reinterpret_cast<C2 *>(out + 2 * (col + i * es)) = v;
+      };
+      fft_stages_io<T, N, R, C, C, -1, 1, true, true>(bufA, tw, tid, ld, st);
+    }
+    if (has0) {  // the same in every thread of the block
+      __syncthreads();  // every thread has read uk_dc
+      if (tid == 0)
+        for (int d = 0; d < 3; ++d)
+          if (fix_mask & (1 << d)) uk_dc[d] = 0.0;
+    }
   }
 }
 
@@ -1126,6 +1184,10 @@ inline T *F(double *p) {
 #ifndef SPM_X_DMA_F32
 #define SPM_X_DMA_F32 1
 #endif
+// the fused step runs x_update and the following x_zeta2u as one launch
+#ifndef SPM_X_UPDATE_ZU
+#define SPM_X_UPDATE_ZU 1
+#endif
 inline int x_grid(const spm_ctx *c, int ntiles, int per_cu) {
   const int n = per_cu > 0 ? per_cu * c->n_cu : ntiles;
   return n < ntiles ? n : ntiles;
@@ -1170,14 +1232,31 @@ int fx_update_t(spm_ctx *c, double nu, double dt) {
     const int ntiles = (g.ny * g.hp + C - 1) / C;
     constexpr int per_cu = sizeof(T) == 8 ? SPM_X_UPDATE_BLOCKS_F64 : SPM_X_UPDATE_BLOCKS_F32;
     if constexpr (sizeof(T) == 8 ? SPM_X_DMA_F64 : SPM_X_DMA_F32) {
-      hipLaunchKernelGGL((x_update_dma_kernel<T, S, R, C>), dim3(ntiles), dim3(C * S / R), 0, c->stream, g,
+      hipLaunchKernelGGL((x_update_dma_kernel<T, S, R, C, false>), dim3(ntiles), dim3(C * S / R), 0, c->stream, g,
                          F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
-                         F<T>(c->zeta[0]), F<T>(c->zeta[1]), nu, dt, c->tw_x);
+                         F<T>(c->zeta[0]), F<T>(c->zeta[1]), nu, dt, c->tw_x, nullptr, nullptr, nullptr,
+                         nullptr, 0);
       return 0;
     }
     hipLaunchKernelGGL((x_update_kernel<T, S, R, C, (per_cu > 0)>), dim3(x_grid(c, ntiles, per_cu)), dim3(C * S / R), 0, c->stream, g, F<T>(c->u[0]),
                        F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]), F<T>(c->zeta[0]),
                        F<T>(c->zeta[1]), nu, dt, c->tw_x, ntiles);
+  });
+  return 0;
+}
+
+// x_update followed by x_zeta2u(fix_mask) in one launch: the new zeta
+// stays on chip (the direct-to-LDS path only)
+template <typename T>
+int fx_update_zeta2u_t(spm_ctx *c, double nu, double dt, int fix_mask) {
+  const Grid &g = c->grid;
+  SPM_SIZE_SWITCH(g.nx, {
+    constexpr int R = kFusedR(S), C = kFusedC(S);
+    const int ntiles = (g.ny * g.hp + C - 1) / C;
+    hipLaunchKernelGGL((x_update_dma_kernel<T, S, R, C, true>), dim3(ntiles), dim3(C * S / R), 0, c->stream, g,
+                       F<T>(c->u[0]), F<T>(c->u[1]), F<T>(c->u[2]), F<T>(c->work[0]), F<T>(c->work[1]),
+                       F<T>(c->zeta[0]), F<T>(c->zeta[1]), nu, dt, c->tw_x, F<T>(c->u[0]), F<T>(c->u[1]),
+                       F<T>(c->u[2]), c->uk_dc, fix_mask);
   });
   return 0;
 }
@@ -1243,6 +1322,21 @@ int fx_update(spm_ctx *c, double nu, double dt) {
   const int rc = SPM_PREC_CALL(c, fx_update_t, nu, dt);
   const Band b = band(c);
   tend(c, SPM_K_FFT_X, (5.0 * b.col + 4.0 * b.col * b.kx) * field_bytes(c));
+  if (rc) return rc;
+  SPM_LAUNCH_CHECK(c);
+  return 0;
+}
+
+int fx_update_zeta2u(spm_ctx *c, double nu, double dt, int fix_mask) {
+  const bool fp32 = c->prm.precision == SPM_FP32;
+  if (!SPM_X_UPDATE_ZU || !(fp32 ? SPM_X_DMA_F32 : SPM_X_DMA_F64)) {
+    const int rc = fx_update(c, nu, dt);
+    return rc ? rc : fx_zeta2u(c, fix_mask);
+  }
+  tbeg(c);
+  const int rc = SPM_PREC_CALL(c, fx_update_zeta2u_t, nu, dt, fix_mask);
+  const Band b = band(c);
+  tend(c, SPM_K_FFT_X, (8.0 * b.col + 4.0 * b.col * b.kx) * field_bytes(c));
   if (rc) return rc;
   SPM_LAUNCH_CHECK(c);
   return 0;

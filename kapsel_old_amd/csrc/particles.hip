@@ -1,7 +1,7 @@
 // particles.hip -- particle <-> grid coupling and particle dynamics.
 //
 //   make_phi   Make_phi_u_primitive (make_phi.cxx:221-298) as an
-//              owner-computes gather: the grid is cut into 8x8x8 tiles,
+//              owner-computes gather: the grid is cut into cubic tiles,
 //              each tile holds the sorted list of particles whose stencil
 //              cube reaches it, and each grid point sums its particles in
 //              ascending index -- the order in which the reference's
@@ -205,36 +205,23 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int *__restrict__ coun
 
 inline unsigned scan_blocks(int n) { return (unsigned)((n + 4095) / 4096 > 0 ? (n + 4095) / 4096 : 1); }
 
-// Sort each list segment ascending, one 64-thread block per segment.  A
-// segment of up to kSortLds entries is copied to LDS and every entry goes
-// straight to its rank (the number of smaller entries, equal ones ranked by
-// position, so repeated entries keep their count); a longer one falls back
-// to an insertion sort by one lane.
+// Sort each list segment ascending, 64 segments per 64-thread block.  A
+// lane sorts its own segment of up to kSortLane entries in place
+// (insertion sort); the wavefront then takes the block's longer segments
+// one at a time: up to kSortLds entries are copied to LDS and every entry
+// goes straight to its rank (the number of smaller entries, equal ones
+// ranked by position, so repeated entries keep their count), a longer one
+// is insertion-sorted by one lane.
+constexpr int kSortLane = 16;
 constexpr int kSortLds = 1024;
 __global__ __launch_bounds__(64) void sort_segments_kernel(const int *__restrict__ start, const int *__restrict__ count,
                                                            int *list, int n) {
-  const int s = blockIdx.x;
-  if (s >= n) return;
-  int *a = list + start[s];
-  const int m = count[s];
-  if (m <= 1) return;
   const int lane = threadIdx.x;
-  if (m <= kSortLds) {
-    __shared__ int buf[kSortLds];
-    for (int i = lane; i < m; i += 64) buf[i] = a[i];
-    __syncthreads();
-    for (int i = lane; i < m; i += 64) {
-      const int v = buf[i];
-      int r = 0;
-      for (int j = 0; j < m; ++j) {
-        const int w = buf[j];
-        r += (w < v) || (w == v && j < i);
-      }
-      a[r] = v;
-    }
-    return;
-  }
-  if (lane == 0)
+  const int s = blockIdx.x * 64 + lane;
+  const int m = s < n ? count[s] : 0;
+  const int s0 = s < n ? start[s] : 0;
+  if (m > 1 && m <= kSortLane) {
+    int *a = list + s0;
     for (int i = 1; i < m; ++i) {
       const int v = a[i];
       int j = i - 1;
@@ -244,6 +231,39 @@ __global__ __launch_bounds__(64) void sort_segments_kernel(const int *__restrict
       }
       a[j + 1] = v;
     }
+  }
+  __shared__ int buf[kSortLds];
+  unsigned long long big = __ballot(m > kSortLane);
+  while (big) {
+    const int l = __ffsll((long long)big) - 1;
+    big &= big - 1;
+    const int mm = __shfl(m, l);
+    int *a = list + __shfl(s0, l);
+    if (mm <= kSortLds) {
+      for (int i = lane; i < mm; i += 64) buf[i] = a[i];
+      __syncthreads();
+      for (int i = lane; i < mm; i += 64) {
+        const int v = buf[i];
+        int r = 0;
+        for (int j = 0; j < mm; ++j) {
+          const int w = buf[j];
+          r += (w < v) || (w == v && j < i);
+        }
+        a[r] = v;
+      }
+      __syncthreads();  // buf is refilled for the next segment
+    } else if (lane == 0) {
+      for (int i = 1; i < mm; ++i) {
+        const int v = a[i];
+        int j = i - 1;
+        while (j >= 0 && a[j] > v) {
+          a[j + 1] = a[j];
+          --j;
+        }
+        a[j + 1] = v;
+      }
+    }
+  }
 }
 
 // The whole tile-list build of a small box in one block: counts in LDS,
@@ -347,7 +367,7 @@ __global__ void izero_kernel(int *a, int n) {
 }
 
 // ---------------------------------------------------------------------
-// make_phi: owner-computes over 8x8x8 tiles (one point per thread)
+// make_phi: owner-computes over cubic tiles (one point per thread)
 // ---------------------------------------------------------------------
 // The stencil point of particle n that lands on grid index gi along an axis
 // of N points has offset cell = gi - x_int taken into [-N/2, N/2); it
@@ -361,7 +381,7 @@ __global__ void izero_kernel(int *a, int n) {
 //   mode 2: f = up - phi u (Make_f_particle_dt_nonsole on the fly; phi and
 //           up are not stored)
 template <typename T>
-__global__ __launch_bounds__(512) void make_phi_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+__global__ __launch_bounds__(kPhiTilePoints) void make_phi_kernel(Grid g, PModel pm, const double *__restrict__ pb,
                                                        const int *__restrict__ pxint, const double *__restrict__ pres,
                                                        const int *__restrict__ tile_start,
                                                        const int *__restrict__ tile_count,
@@ -431,7 +451,7 @@ __device__ __forceinline__ bool stencil_offset(const int gi[3], const int *xi, c
 // Make_phi_qq_particle (operate_electrolyte.cxx:179-227) before its
 // normalisation: phi (no clamp) and sum_n q_n DPhi(d_n), particles in
 // ascending order at every point as the reference's particle loop adds.
-__global__ __launch_bounds__(512) void make_phi_qq_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+__global__ __launch_bounds__(kPhiTilePoints) void make_phi_qq_kernel(Grid g, PModel pm, const double *__restrict__ pb,
                                                           const int *__restrict__ spec, const int *__restrict__ pxint,
                                                           const int *__restrict__ tile_start,
                                                           const int *__restrict__ tile_count,
@@ -467,7 +487,7 @@ __global__ __launch_bounds__(512) void make_phi_qq_kernel(Grid g, PModel pm, con
 // r/|r| of the particles whose shell |r| - a| < xi/2 holds the point, with
 // r = offset * dx - residue (Relative_coord).  Any such point lies in the
 // particle's stencil, so the shell test alone selects the reference's points.
-__global__ __launch_bounds__(512) void surface_normal_kernel(Grid g, PModel pm, const int *__restrict__ pxint,
+__global__ __launch_bounds__(kPhiTilePoints) void surface_normal_kernel(Grid g, PModel pm, const int *__restrict__ pxint,
                                                              const double *__restrict__ pres,
                                                              const int *__restrict__ tile_start,
                                                              const int *__restrict__ tile_count,
@@ -976,7 +996,7 @@ __global__ void obl_tiles_kernel(Grid g, PModel pm, const double *__restrict__ p
     tile_list[atomicAdd(&tile_cursor[ti], 1)] = n;
 }
 
-// One block per 8x8x8 tile, one point per thread.  For every particle of
+// One block per tile, one point per thread.  For every particle of
 // the tile's sorted list (each once) the stencil cells that can map to this
 // point are the y offsets c1 = j - xi1 (+NY across the upper face, -NY
 // across the lower one) within the cube, and for each the x offsets next
@@ -984,7 +1004,7 @@ __global__ void obl_tiles_kernel(Grid g, PModel pm, const double *__restrict__ p
 // own map (rel_coord_obl) lands on this point and its distance is below
 // a + xi/2, so the sums are the scatter's, added in ascending particle
 // order as the reference's particle loop adds them.
-__global__ __launch_bounds__(512) void make_phi_obl_kernel(Grid g, PModel pm, const double *__restrict__ pb,
+__global__ __launch_bounds__(kPhiTilePoints) void make_phi_obl_kernel(Grid g, PModel pm, const double *__restrict__ pb,
                                                            const int *__restrict__ pxint,
                                                            const double *__restrict__ pres,
                                                            const int *__restrict__ tile_start,
@@ -1329,7 +1349,7 @@ int p_build_tiles(spm_ctx *c) {
   // tiles a stencil cube of 2 nh + 1 points can reach per axis (the same
   // bound sizes tile_list)
   int k[3];
-  for (int d = 0; d < 3; ++d) k[d] = std::min(c->tiles[d], (2 * c->pm.nh + 1 + 7) / 8 + 1);
+  for (int d = 0; d < 3; ++d) k[d] = std::min(c->tiles[d], (2 * c->pm.nh + kTile) / kTile + 1);
   const long slots = (long)c->np * k[0] * k[1] * k[2];
   if (c->ntiles <= kSmallTiles && slots <= kSmallSlots) {
     hipLaunchKernelGGL(tiles_small_kernel, dim3(1), dim3(1024), 0, c->stream, c->grid, c->pm, c->pbuf, c->pxint,
@@ -1346,7 +1366,7 @@ int p_build_tiles(spm_ctx *c) {
                      c->ntiles);
   hipLaunchKernelGGL(tile_fill_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->grid, c->pm, c->pxint,
                      c->pres, c->tiles[0], c->tiles[1], c->tiles[2], k[0], k[1], k[2], c->tile_cursor, c->tile_list);
-  hipLaunchKernelGGL(sort_segments_kernel, dim3(c->ntiles), dim3(64), 0, c->stream, c->tile_start,
+  hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ntiles, 64)), dim3(64), 0, c->stream, c->tile_start,
                      c->tile_count, c->tile_list, c->ntiles);
   tend(c, SPM_K_PARTICLE, 16.0 * c->ntiles);
   SPM_LAUNCH_CHECK(c);
@@ -1357,13 +1377,13 @@ static int launch_make_phi(spm_ctx *c, int mode, double *o0, double *o1, double 
   tbeg(c);
   if (c->prm.precision == SPM_FP32) {
     using T = float;
-    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(512), 0,
+    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(kPhiTilePoints), 0,
                        c->stream, c->grid, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count,
                        c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2], (T *)o0, (T *)o1, (T *)o2, (T *)o3,
                        (const T *)c->u[0], (const T *)c->u[1], (const T *)c->u[2], mode);
   } else {
     using T = double;
-    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(512), 0,
+    hipLaunchKernelGGL(make_phi_kernel<T>, dim3(c->ntiles), dim3(kPhiTilePoints), 0,
                        c->stream, c->grid, c->pm, c->pbuf, c->pxint, c->pres, c->tile_start, c->tile_count,
                        c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2], (T *)o0, (T *)o1, (T *)o2, (T *)o3,
                        (const T *)c->u[0], (const T *)c->u[1], (const T *)c->u[2], mode);
@@ -1467,7 +1487,7 @@ int p_make_phi_qq(spm_ctx *c, double *phi, double *surface) {
   }
   RCP(p_build_tiles(c));
   tbeg(c);
-  hipLaunchKernelGGL(make_phi_qq_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
+  hipLaunchKernelGGL(make_phi_qq_kernel, dim3(c->ntiles), dim3(kPhiTilePoints), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
                      c->pxint, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2],
                      c->qtab, phi, surface);
   tend(c, SPM_K_MAKE_PHI, 16.0 * ref_count(c->grid));
@@ -1482,7 +1502,7 @@ int p_surface_normal(spm_ctx *c) {  // FP64 normals whatever the field type
     return 0;
   }
   tbeg(c);
-  hipLaunchKernelGGL(surface_normal_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pxint,
+  hipLaunchKernelGGL(surface_normal_kernel, dim3(c->ntiles), dim3(kPhiTilePoints), 0, c->stream, c->grid, c->pm, c->pxint,
                      c->pres, c->tile_start, c->tile_count, c->tile_list, c->tiles[0], c->tiles[1], c->tiles[2],
                      c->surface_normal[0], c->surface_normal[1], c->surface_normal[2]);
   tend(c, SPM_K_MAKE_PHI, 24.0 * ref_count(c->grid));
@@ -1504,7 +1524,7 @@ int p_force(spm_ctx *c) {
                        c->ncells);
     hipLaunchKernelGGL(cell_fill_kernel, dim3(blocks(c->np, 128)), dim3(128), 0, c->stream, c->pm, c->pcell,
                        c->cell_cursor, c->cell_list);
-    hipLaunchKernelGGL(sort_segments_kernel, dim3(c->ncells), dim3(64), 0, c->stream, c->cell_start,
+    hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ncells, 64)), dim3(64), 0, c->stream, c->cell_start,
                        c->cell_count, c->cell_list, c->ncells);
   }
   hipLaunchKernelGGL(force_kernel, dim3(blocks(c->np, 4)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf, c->spec,
@@ -1572,7 +1592,7 @@ int p_velocity_obl(spm_ctx *c, int euler) {
 
 int obl_tile_span(const spm_ctx *c, int axis) {
   const int width = 2 * c->pm.nh + 1 + (axis == 0 ? 2 : 0);
-  return std::min(c->tiles[axis], (width + 7) / 8 + 1);
+  return std::min(c->tiles[axis], (width + kTile - 1) / kTile + 1);
 }
 
 // phi and up at the anchors p_calc_f_hydro_obl built (positions unchanged
@@ -1598,12 +1618,12 @@ int p_make_phi_obl(spm_ctx *c, const double *sre) {
                      c->tile_start_obl, c->tile_cursor_obl, c->ntiles);
   hipLaunchKernelGGL(obl_tiles_kernel, dim3(blocks(slots, 256)), dim3(256), 0, c->stream, c->grid, c->pm, c->pbuf,
                      c->pxint, c->le_deg, k[0], k[1], k[2], c->tile_count_obl, c->tile_cursor_obl, c->tile_list_obl, 1);
-  hipLaunchKernelGGL(sort_segments_kernel, dim3(c->ntiles), dim3(64), 0, c->stream, c->tile_start_obl,
+  hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks(c->ntiles, 64)), dim3(64), 0, c->stream, c->tile_start_obl,
                      c->tile_count_obl, c->tile_list_obl, c->ntiles);
   tend(c, SPM_K_PARTICLE, 16.0 * c->ntiles);
   SPM_LAUNCH_CHECK(c);
   tbeg(c);
-  hipLaunchKernelGGL(make_phi_obl_kernel, dim3(c->ntiles), dim3(512), 0, c->stream, c->grid, c->pm, c->pbuf,
+  hipLaunchKernelGGL(make_phi_obl_kernel, dim3(c->ntiles), dim3(kPhiTilePoints), 0, c->stream, c->grid, c->pm, c->pbuf,
                      c->pxint, c->pres, c->tile_start_obl, c->tile_count_obl, c->tile_list_obl, c->le_deg, sre, c->phi,
                      c->up[0], c->up[1], c->up[2]);
   tend(c, SPM_K_MAKE_PHI, 4.0 * 8.0 * (double)ref_count(c->grid));

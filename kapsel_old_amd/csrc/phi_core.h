@@ -9,7 +9,7 @@
 namespace spm {
 namespace phi {
 
-constexpr int kTile = 8;  // tiles of 8x8x8 grid points carry the particle lists
+constexpr int kTile = kPhiTile;  // cubic tiles of grid points carry the particle lists
 
 __device__ __forceinline__ int nint(double a) { return a >= 0.0 ? (int)(a + .5) : (int)(a - .5); }
 

@@ -266,9 +266,9 @@ int fused_step(spm_ctx *c) {
   RC(fft_pass_y(c, u3, 3, -1));
   RC(fz_advect(c));
   RC(fft_pass_y(c, p5, 5, +1));
-  RC(fx_update(c, nu, c->prm.dt));
-  // FIX_CELL, truncation, Zeta_k2u
-  RC(fx_zeta2u(c, fix_mask(c)));
+  // FIX_CELL, truncation, Zeta_k2u (the solver's last x pass and the first
+  // of Zeta_k2u share a launch)
+  RC(fx_update_zeta2u(c, nu, c->prm.dt, fix_mask(c)));
   RC(fft_pass_y(c, u3, 3, -1));
   RC(fft_pass_z_c2r(c, u3, 3));
   // particles (p_calc_f_hydro also builds the tile lists at the new positions)
@@ -522,16 +522,16 @@ int setup(spm_ctx *c) {
   }
 
   // tiles for make_phi
-  c->tiles[0] = g.nx / 8;
-  c->tiles[1] = g.ny / 8;
-  c->tiles[2] = g.nz / 8;
+  c->tiles[0] = g.nx / kPhiTile;
+  c->tiles[1] = g.ny / kPhiTile;
+  c->tiles[2] = g.nz / kPhiTile;
   c->ntiles = c->tiles[0] * c->tiles[1] * c->tiles[2];
   RC(dalloc(c, &c->tile_count, c->ntiles));
   RC(dalloc(c, &c->tile_start, c->ntiles));
   RC(dalloc(c, &c->tile_cursor, c->ntiles));
   {
     long span = 1;
-    for (int d = 0; d < 3; ++d) span *= std::min(c->tiles[d], (2 * m.nh + 1 + 7) / 8 + 1);
+    for (int d = 0; d < 3; ++d) span *= std::min(c->tiles[d], (2 * m.nh + kPhiTile) / kPhiTile + 1);
     c->tile_list_cap = (int)std::max<long>(1, span * np);
     RC(dalloc(c, &c->tile_list, c->tile_list_cap));
   }

@@ -42,6 +42,13 @@ using cplx = typename cplx_t<T>::type;
 // XCD has its own L2.  Give XCD k the contiguous run of tiles
 // [k*n/8, (k+1)*n/8) so that tiles sharing a cache line at their edges meet
 // in one L2.  The tail beyond a multiple of 8 keeps its own numbering.
+// edge of the tiles that carry make_phi's particle lists (grid points)
+#ifndef SPM_PHI_TILE
+#define SPM_PHI_TILE 8
+#endif
+constexpr int kPhiTile = SPM_PHI_TILE;
+constexpr int kPhiTilePoints = kPhiTile * kPhiTile * kPhiTile;
+
 #ifndef SPM_XCD_REMAP
 #define SPM_XCD_REMAP 1
 #endif
@@ -321,6 +328,7 @@ int fft_pass_z_r2c(spm_ctx *c, double *const *a, int n);
 int fx_zeta2u(spm_ctx *c, int fix_mask);
 int fz_advect(spm_ctx *c);
 int fx_update(spm_ctx *c, double nu, double dt);
+int fx_update_zeta2u(spm_ctx *c, double nu, double dt, int fix_mask);
 int fx_project(spm_ctx *c);
 
 // spectral.hip
