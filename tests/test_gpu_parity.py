@@ -119,6 +119,17 @@ def test_make_phi_u_particle(SPM, grid, n, a):
     fields_close(o, g, ["phi", "up0", "up1", "up2"], grid[2], spectrum=True)
 
 
+def test_make_phi_dense_lists(SPM):
+    """300 spheres of a = 1.5 in 32^3: each 8^3 tile's list holds about 34
+    particles (a tile is reached from within a + xi/2 + sqrt(3) dx), more
+    than one lane sorts, so the wavefront's rank sort orders them; 300 x 27
+    particle-tile slots take the multi-kernel list build."""
+    grid = (32, 32, 32)
+    o, g, _ = _particle_case(SPM, grid, 300, 1.5, 11)
+    both((o, g), "Make_phi_u_particle")
+    fields_close(o, g, ["phi", "up0", "up1", "up2"], 32, spectrum=True)
+
+
 def test_make_phi_overlap_and_wrap(SPM):
     """Overlapping particles (phi clamped to 1) and particles across the
     periodic boundary."""
